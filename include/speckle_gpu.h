@@ -1,0 +1,159 @@
+/*
+ * speckle_gpu.h -- C-ABI of the MI355X SIE solve loop (libspeckle_gpu.so).
+ *
+ * Drop-in boundary for the reference module API of SpeckleSim's
+ * triangle MLFMM-GMRES path.  Every entry point takes plain pointers and
+ * sizes; complex vectors are interleaved (re, im) doubles, i.e. Fortran
+ * complex(8) / C double _Complex layout, in the reference's unknown order
+ * x = [xe(1:N); xh(1:N)] (src/lib_sie_tri_mlfmm_interface.f90:300-311).
+ * Indices are 0-based unless stated; mesh triangles are 1-based as the
+ * reference reads them.  All functions return SPK_OK (0) or a negative
+ * error code; spk_last_error() gives the message of the calling thread.
+ * A handle is used by one host thread at a time (the reference calls the
+ * matvec from its single GMRES reverse-communication loop).
+ */
+#ifndef SPECKLE_GPU_H
+#define SPECKLE_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPK_OK 0
+#define SPK_ERR_ARG -1
+#define SPK_ERR_HIP -2
+#define SPK_ERR_STATE -3
+#define SPK_ERR_MEM -4
+#define SPK_ERR_MESH -5
+
+/* Formulations, src/lib_sie_tri_calculation_mod.f90:463-644. */
+#define SPK_PMCHWT 1
+#define SPK_MCTF 2
+#define SPK_ICTF 3
+#define SPK_MCTF2 4
+#define SPK_PMCHWT_S 5
+#define SPK_JMCFIE 6
+#define SPK_CNF 7
+#define SPK_MNMF 8
+
+/* Arithmetic of the MLFMM far field / inner solve. */
+#define SPK_FP64 0
+#define SPK_MIXED 1 /* complex64 far field + inner GMRES, FP64 refinement */
+
+/* Orthogonalisation, lib_math_solver_GMRES.f90 (icntl(5)). */
+#define SPK_ORTH_MGS 0
+#define SPK_ORTH_IMGS 1
+#define SPK_ORTH_CGS 2
+#define SPK_ORTH_ICGS 3
+
+typedef struct spk_handle spk_handle;
+
+/* Solver parameters: the reference's module globals set by main.f90
+ * (set_calculation_parameters, initalize_gmres) and
+ * set_local_material_parameters_tri (src/lib_sie_tri_data_container.f90:212-224). */
+typedef struct spk_params {
+  double lambda;                 /* vacuum wavelength [m] */
+  double eps_r1_re, eps_r1_im;   /* exterior relative permittivity */
+  double eps_r2_re, eps_r2_im;   /* interior relative permittivity */
+  int formulation;               /* SPK_PMCHWT ... */
+  int normal_convention;         /* calc_p(7): 1 or 2 */
+  int quad_points;               /* ng: 1,3,4,6,7,12 (reference: 3) */
+  int tree_s_opt;                /* tree_s_opt (reference: 250) */
+  int truncation_number;         /* K at every level (reference: 19) */
+  int precision;                 /* SPK_FP64 or SPK_MIXED */
+  int device;                    /* HIP device ordinal */
+} spk_params;
+
+/* GMRES knobs: lib_sie_solver_gmres_get_parameter
+ * (src/lib_sie_solver_gmres_interface.f90:102-121). */
+typedef struct spk_gmres_params {
+  int max_iterations;            /* icntl(7) */
+  int restart;                   /* m (<=0: = max_iterations) */
+  double tolerance;              /* cntl(1), Arnoldi backward error */
+  int orthogonalization;         /* SPK_ORTH_* */
+  int use_initial_guess;         /* icntl(6) */
+  int refinement_steps;          /* SPK_MIXED: outer FP64 refinement sweeps */
+  double refinement_tolerance;   /* SPK_MIXED: target FP64 residual */
+} spk_gmres_params;
+
+typedef struct spk_gmres_info {
+  int info;                      /* info(1): 0 converged, -4 not converged */
+  int iterations;                /* info(2) (sum over refinement sweeps) */
+  double arnoldi_backward_error; /* last bea */
+  double backward_error;         /* rinfo(1): true ||b-Ax||/||b|| */
+  int matvecs;
+  double seconds;
+} spk_gmres_info;
+
+typedef struct spk_stats {
+  int64_t n_edges;               /* m_pairs */
+  int64_t near_blocks;           /* nnz of the near-field CSR */
+  int l_min, l_max;
+  int samples;                   /* S = 2K^2+2 */
+  int64_t boxes_per_level[32];
+  int64_t i4_pairs_per_level[32];
+  int64_t singular_pairs;        /* triangle pairs on the singular branch */
+  double setup_ms, near_fill_ms, patterns_ms, tables_ms, tree_ms;
+  double bytes_near, bytes_patterns;
+} spk_stats;
+
+/* Create a context on the chosen device (no mesh yet). */
+int spk_create(const spk_params *params, spk_handle **out);
+void spk_destroy(spk_handle *h);
+const char *spk_last_error(void);
+
+/* lib_sie_ml_fmm_precalculation_tri (src/lib_sie_tri_mlfmm_interface.f90:59-92):
+ * RWG topology, octree, radiation patterns, TL tables and the near-field
+ * impedance fill on the GPU.  pp: 3*np node coordinates [m];
+ * tt: 3*nt node indices, 1-based (read_pp_tt_file_tri). */
+int spk_setup_tri(spk_handle *h, int np, const double *pp, int nt, const int *tt);
+
+int spk_get_stats(const spk_handle *h, spk_stats *out);
+int spk_num_edges(const spk_handle *h, int64_t *n_edges);
+
+/* lib_sie_ml_fmm_calculate_vector_b_tri (:285-319): y = Z x on host
+ * buffers of 2N complex. */
+int spk_matvec(spk_handle *h, const double *x, double *y);
+/* Same on device buffers (HBM-resident), stream 0 of the handle. */
+int spk_matvec_device(spk_handle *h, const void *d_x, void *d_y);
+/* Far field only / near field only (parity probes, timing splits). */
+int spk_matvec_parts(spk_handle *h, const double *x, double *y_near, double *y_far);
+
+/* get_vector_b_tri with plane-wave illumination
+ * (src/lib_sie_tri_calculation_mod.f90:391-460, 939-1002). */
+int spk_rhs_plane_wave(spk_handle *h, const double k_in[3], const double e_in[3], double *b);
+
+/* lib_math_solver_gmres_run (src/lib/libmath/src/math/lib_math_solver_GMRES.f90:159-379)
+ * with the GPU matvec; x holds the initial guess on entry. */
+int spk_gmres(spk_handle *h, const double *b, double *x, const spk_gmres_params *p,
+              spk_gmres_info *info);
+int spk_gmres_device(spk_handle *h, const void *d_b, void *d_x, const spk_gmres_params *p,
+                     spk_gmres_info *info);
+/* Arnoldi backward error per iteration of the last solve (host copy). */
+int spk_gmres_history(const spk_handle *h, double *hist, int cap, int *count);
+
+/* Parity probes. */
+/* Pair%corner / Pair%element of every RWG edge (1-based, N x 2 each). */
+int spk_get_rwg(const spk_handle *h, int *corner, int *element);
+/* Leaf Morton index of every edge, tree scaling D and x_min. */
+int spk_get_tree(const spk_handle *h, int64_t *leaf_uindex, int *l_min, int *l_max,
+                 double *D, double *xmin3);
+/* I4 list of box n at level l: returns count in *count (cap entries written). */
+int spk_get_i4(const spk_handle *h, int level, int64_t n, int64_t *out, int cap, int *count);
+/* Near-field CSR in the reference's layout (ia, ja 0-based, N+1 / nnz)
+ * and values D11,D12,D21,D22 per entry (nnz*8 doubles); any pointer may be
+ * NULL to skip it. */
+int spk_get_near_csr(const spk_handle *h, int64_t *ia, int32_t *ja, double *abcd);
+/* Blocks Z(m,n) for arbitrary pairs (lib_sie_tri_get_impedance_edges):
+ * out = npairs*8 doubles. */
+int spk_near_blocks(spk_handle *h, int npairs, const int *m, const int *n, double *out);
+/* Radiation pattern of edges (PreCalculate_RT): out = count * 4 * S complex,
+ * [edge][k1 theta, k1 phi, k2 theta, k2 phi][sample]. */
+int spk_get_patterns(const spk_handle *h, int first, int count, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,278 @@
+"""ctypes binding of the CPU restatement in ``spk_oracle.c``.
+
+Test infrastructure only: the checker the GPU product is compared against.
+Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` may import this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libspk_oracle.so")
+
+FORMULATIONS = {"PMCHWT": 1, "MCTF": 2, "ICTF": 3, "MCTF2": 4, "PMCHWT-S": 5,
+                "JMCFIE": 6, "CNF": 7, "MNMF": 8}
+
+
+class _Params(C.Structure):
+    _fields_ = [("lambda_", C.c_double), ("eps_r1_re", C.c_double), ("eps_r1_im", C.c_double),
+                ("eps_r2_re", C.c_double), ("eps_r2_im", C.c_double), ("formulation", C.c_int),
+                ("normal_convention", C.c_int), ("ngp", C.c_int), ("s_opt", C.c_int),
+                ("K", C.c_int)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle library missing: {LIB_PATH} (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        dp = C.POINTER(C.c_double)
+        ip = C.POINTER(C.c_int)
+        L.spo_create.restype = C.c_void_p
+        L.spo_create.argtypes = [C.POINTER(_Params), C.c_int, dp, C.c_int, ip]
+        L.spo_destroy.argtypes = [C.c_void_p]
+        L.spo_n_edges.argtypes = [C.c_void_p]
+        L.spo_get_pairs.argtypes = [C.c_void_p, ip, ip]
+        L.spo_get_fn.argtypes = [C.c_void_p, C.c_int, C.c_int, dp]
+        L.spo_near_block.argtypes = [C.c_void_p, C.c_int, C.c_int, dp]
+        L.spo_tree_build.argtypes = [C.c_void_p]
+        L.spo_tree_info.argtypes = [C.c_void_p, ip, ip, dp, dp]
+        L.spo_edge_leaf.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
+        L.spo_e2_of_edge.argtypes = [C.c_void_p, C.c_int, ip, C.c_int]
+        L.spo_i4.argtypes = [C.c_void_p, C.c_int, C.c_int64, C.POINTER(C.c_int64), C.c_int]
+        L.spo_near_csr_build.restype = C.c_int64
+        L.spo_near_csr_build.argtypes = [C.c_void_p]
+        L.spo_near_csr_get.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+        L.spo_near_fill.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
+        L.spo_near_values.argtypes = [C.c_void_p, dp]
+        L.spo_mlfmm_setup.argtypes = [C.c_void_p, C.c_int]
+        L.spo_matvec.argtypes = [C.c_void_p, dp, dp, C.c_int]
+        L.spo_rhs_plane.argtypes = [C.c_void_p, dp, dp, dp]
+        L.spo_gmres.argtypes = [C.c_void_p, dp, dp, C.c_int, C.c_int, C.c_double, C.c_int,
+                                C.c_int, dp, ip, dp]
+        L.spo_gmres_dense.argtypes = [C.c_int, dp, dp, dp, C.c_int, C.c_int, C.c_double, C.c_int,
+                                      dp, ip, dp]
+        L.spo_quadrature_tri.argtypes = [C.c_int, dp, dp, dp]
+        L.spo_iq_l.restype = C.c_double
+        L.spo_iq_l.argtypes = [C.c_int, dp, dp, dp]
+        L.spo_iq_s_all.argtypes = [dp, dp, dp, dp, C.c_int, dp]
+        for f in (L.spo_k2q, L.spo_k3q, L.spo_k4q):
+            f.argtypes = [C.c_int, dp, dp, C.c_int, dp]
+        L.spo_gauss_legendre.argtypes = [C.c_int, dp, dp]
+        L.spo_hankel2_sph.argtypes = [C.c_int, C.c_double, C.c_double, dp]
+        L.spo_legendre_p.argtypes = [C.c_int, C.c_double, dp]
+        L.spo_tl.argtypes = [C.c_int, C.c_double, C.c_double, dp, dp]
+        L.spo_dense_fill.argtypes = [C.c_void_p, dp, C.c_int]
+        L.spo_get_pattern.argtypes = [C.c_void_p, C.c_int, dp]
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _i(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+class Oracle:
+    """One mesh + material + formulation, as the reference's module state."""
+
+    def __init__(self, pp, tt_1based, lam, eps_r2, eps_r1=1.0 + 0j, formulation="PMCHWT",
+                 normal_convention=1, ngp=3, s_opt=250, K=19):
+        L = lib()
+        self.pp = np.ascontiguousarray(pp, dtype=np.float64).reshape(-1, 3)
+        self.tt = np.ascontiguousarray(tt_1based, dtype=np.int32).reshape(-1, 3)
+        f = FORMULATIONS[formulation] if isinstance(formulation, str) else int(formulation)
+        self.params = _Params(lam, complex(eps_r1).real, complex(eps_r1).imag,
+                              complex(eps_r2).real, complex(eps_r2).imag, f,
+                              normal_convention, ngp, s_opt, K)
+        self.h = L.spo_create(C.byref(self.params), self.pp.shape[0], _d(self.pp),
+                              self.tt.shape[0], _i(self.tt))
+        self.N = L.spo_n_edges(self.h)
+        self._csr = None
+
+    def close(self):
+        if self.h:
+            lib().spo_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def pairs(self):
+        c = np.zeros((self.N, 2), np.int32)
+        e = np.zeros((self.N, 2), np.int32)
+        lib().spo_get_pairs(self.h, _i(c), _i(e))
+        return c, e
+
+    def fn(self, m, ot):
+        o = np.zeros(19)
+        lib().spo_get_fn(self.h, m, ot, _d(o))
+        return o
+
+    def near_block(self, m, n):
+        o = np.zeros(8)
+        lib().spo_near_block(self.h, m, n, _d(o))
+        return (o[0::2] + 1j * o[1::2]).reshape(2, 2)
+
+    def tree(self):
+        L = lib()
+        lmax = L.spo_tree_build(self.h)
+        a = C.c_int()
+        b = C.c_int()
+        D = np.zeros(1)
+        xm = np.zeros(3)
+        L.spo_tree_info(self.h, C.byref(a), C.byref(b), _d(D), _d(xm))
+        leaf = np.zeros(self.N, np.int64)
+        L.spo_edge_leaf(self.h, leaf.ctypes.data_as(C.POINTER(C.c_int64)))
+        return dict(l_min=a.value, l_max=b.value, D=D[0], xmin=xm, leaf=leaf)
+
+    def e2(self, m):
+        L = lib()
+        k = L.spo_e2_of_edge(self.h, m, None, 0)
+        o = np.zeros(k, np.int32)
+        L.spo_e2_of_edge(self.h, m, _i(o), k)
+        return o
+
+    def i4(self, level, n):
+        o = np.zeros(216, np.int64)
+        k = lib().spo_i4(self.h, level, n, o.ctypes.data_as(C.POINTER(C.c_int64)), 216)
+        return o[:k]
+
+    def near_csr(self, fill=True, nthreads=0, rows=None):
+        L = lib()
+        nnz = L.spo_near_csr_build(self.h)
+        ia = np.zeros(self.N + 1, np.int32)
+        ja = np.zeros(nnz, np.int32)
+        L.spo_near_csr_get(self.h, ia.ctypes.data_as(C.POINTER(C.c_int32)),
+                           ja.ctypes.data_as(C.POINTER(C.c_int32)))
+        vals = None
+        if fill:
+            lo, hi = (0, self.N) if rows is None else rows
+            L.spo_near_fill(self.h, lo, hi, nthreads)
+            v = np.zeros(nnz * 8)
+            L.spo_near_values(self.h, _d(v))
+            vals = (v[0::2] + 1j * v[1::2]).reshape(nnz, 4)
+        self._csr = (ia, ja, vals)
+        return ia, ja, vals
+
+    def near_fill_rows(self, lo, hi, nthreads=0):
+        lib().spo_near_fill(self.h, lo, hi, nthreads)
+
+    def dense(self, nthreads=0):
+        Z = np.zeros((2 * self.N, 2 * self.N), np.complex128)
+        lib().spo_dense_fill(self.h, _d(Z.view(np.float64)), nthreads)
+        return Z
+
+    def mlfmm_setup(self, nthreads=0):
+        lib().spo_mlfmm_setup(self.h, nthreads)
+
+    def pattern(self, m):
+        S = 2 * self.params.K * self.params.K + 2
+        o = np.zeros(8 * S)
+        lib().spo_get_pattern(self.h, m, _d(o))
+        return (o[0::2] + 1j * o[1::2]).reshape(4, S)
+
+    def matvec(self, x, nthreads=0):
+        x = np.ascontiguousarray(x, dtype=np.complex128)
+        y = np.zeros(2 * self.N, np.complex128)
+        lib().spo_matvec(self.h, _d(x.view(np.float64)), _d(y.view(np.float64)), nthreads)
+        return y
+
+    def rhs_plane(self, k_in, e_in):
+        k = np.ascontiguousarray(k_in, dtype=np.float64)
+        e = np.ascontiguousarray(e_in, dtype=np.float64)
+        b = np.zeros(2 * self.N, np.complex128)
+        lib().spo_rhs_plane(self.h, _d(k), _d(e), _d(b.view(np.float64)))
+        return b
+
+    def gmres(self, b, maxit=650, restart=0, tol=1e-3, orth=1, nthreads=0, x0=None):
+        b = np.ascontiguousarray(b, dtype=np.complex128)
+        x = np.zeros(2 * self.N, np.complex128) if x0 is None else np.array(x0, np.complex128)
+        hist = np.zeros(maxit)
+        it = C.c_int()
+        be = np.zeros(1)
+        info = lib().spo_gmres(self.h, _d(b.view(np.float64)), _d(x.view(np.float64)), maxit,
+                               restart, tol, orth, nthreads, _d(hist), C.byref(it), _d(be))
+        return x, dict(info=info, iters=it.value, be=be[0], hist=hist[:it.value])
+
+
+def gmres_dense(A, b, maxit, restart=0, tol=1e-6, orth=1):
+    A = np.asfortranarray(A, dtype=np.complex128)
+    b = np.ascontiguousarray(b, dtype=np.complex128)
+    n = b.shape[0]
+    x = np.zeros(n, np.complex128)
+    hist = np.zeros(maxit)
+    it = C.c_int()
+    be = np.zeros(1)
+    Af = np.ascontiguousarray(A.T.reshape(-1))  # column-major flattening
+    info = lib().spo_gmres_dense(n, _d(Af.view(np.float64)), _d(b.view(np.float64)),
+                                 _d(x.view(np.float64)), maxit, restart, tol, orth, _d(hist),
+                                 C.byref(it), _d(be))
+    return x, dict(info=info, iters=it.value, be=be[0], hist=hist[:it.value])
+
+
+def quadrature_tri(ngp):
+    a = np.zeros(20)
+    b = np.zeros(20)
+    w = np.zeros(20)
+    lib().spo_quadrature_tri(ngp, _d(a), _d(b), _d(w))
+    return a[:ngp], b[:ngp], w[:ngp]
+
+
+def iq_l(q, ra, p1, p2):
+    return lib().spo_iq_l(q, _d(np.asarray(ra, float)), _d(np.asarray(p1, float)),
+                          _d(np.asarray(p2, float)))
+
+
+def iq_s_all(ra, corners, nq, mid, iqs_l=False):
+    o = np.zeros(3)
+    lib().spo_iq_s_all(_d(np.asarray(ra, float)), _d(np.ascontiguousarray(corners, float)),
+                       _d(np.asarray(nq, float)), _d(np.asarray(mid, float)), int(iqs_l), _d(o))
+    return o
+
+
+def kq(which, q, ra, fn19, iqs_l=False):
+    o = np.zeros(3)
+    f = {2: lib().spo_k2q, 3: lib().spo_k3q, 4: lib().spo_k4q}[which]
+    f(q, _d(np.asarray(ra, float)), _d(np.ascontiguousarray(fn19, float)), int(iqs_l), _d(o))
+    return o
+
+
+def gauss_legendre(n):
+    x = np.zeros(n)
+    w = np.zeros(n)
+    lib().spo_gauss_legendre(n, _d(x), _d(w))
+    return x, w
+
+
+def hankel2_sph(n, z):
+    o = np.zeros(2 * n)
+    lib().spo_hankel2_sph(n, complex(z).real, complex(z).imag, _d(o))
+    return o[0::2] + 1j * o[1::2]
+
+
+def legendre_p(n, x):
+    o = np.zeros(n)
+    lib().spo_legendre_p(n, x, _d(o))
+    return o
+
+
+def tl(K, k, r):
+    S = 2 * K * K + 2
+    o = np.zeros(2 * S)
+    lib().spo_tl(K, complex(k).real, complex(k).imag, _d(np.asarray(r, float)), _d(o))
+    return o[0::2] + 1j * o[1::2]

@@ -1,0 +1,124 @@
+/*
+ * spk_oracle.h -- CPU restatement of the SpeckleSim SIE solve loop.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker the GPU product is
+ * compared against.  Only tests/, __graft_entry__.smoke() and the cpu_baseline
+ * leg of bench.py may load it; the product (specklesim_amd) never links it.
+ *
+ * It follows the reference Fortran line by line where rounding matters
+ * (single-precision literals, CMPLX default-kind rounding, evaluation order),
+ * citing each routine it restates.  Indices in this API are 0-based; the
+ * reference is 1-based.
+ */
+#ifndef SPK_ORACLE_H
+#define SPK_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Formulation codes, lib_sie_tri_calculation_mod.f90:463-644. */
+enum {
+  SPO_PMCHWT = 1, SPO_MCTF = 2, SPO_ICTF = 3, SPO_MCTF2 = 4,
+  SPO_PMCHWT_S = 5, SPO_JMCFIE = 6, SPO_CNF = 7, SPO_MNMF = 8
+};
+
+typedef struct spo_params {
+  double lambda;            /* vacuum wavelength [m]                       */
+  double eps_r1_re, eps_r1_im;
+  double eps_r2_re, eps_r2_im;
+  int formulation;          /* SPO_* */
+  int normal_convention;    /* calc_p(7): 1 or 2 */
+  int ngp;                  /* triangle rule points (reference ng = 3) */
+  int s_opt;                /* tree_s_opt (250) */
+  int K;                    /* truncation number, all levels (19) */
+} spo_params;
+
+typedef struct spo_ctx spo_ctx;
+
+spo_ctx *spo_create(const spo_params *prm, int np, const double *pp,
+                    int nt, const int *tt_1based);
+void spo_destroy(spo_ctx *c);
+
+int spo_n_edges(const spo_ctx *c);
+/* corner[2N], element[2N]: reference Pair%corner / Pair%element, 1-based. */
+void spo_get_pairs(const spo_ctx *c, int *corner, int *element);
+/* RWG descriptor of edge m (0-based), side ot (1,2):
+ * out[0..8] corners p1,p2,pc; [9] area; [10] length; [11] df; [12..14] mid;
+ * [15..17] nq; [18] sign. */
+void spo_get_fn(const spo_ctx *c, int m, int ot, double *out);
+
+/* 2x2 complex impedance block Z(m,n) with r_ref of row m
+ * (lib_sie_tri_get_impedance_edges).  out = D11,D12,D21,D22 as re/im. */
+void spo_near_block(const spo_ctx *c, int m, int n, double *out);
+
+/* Octree (lib_tree.f90): returns l_max. */
+int spo_tree_build(spo_ctx *c);
+void spo_tree_info(const spo_ctx *c, int *l_min, int *l_max, double *D,
+                   double *xmin3);
+void spo_edge_leaf(const spo_ctx *c, int64_t *uindex);
+/* E2 element list (edge numbers, 0-based) of edge m's leaf box, in the
+ * reference's box order; returns count (or required size if cap too small). */
+int spo_e2_of_edge(const spo_ctx *c, int m, int *out, int cap);
+/* I4 box list of box n at level l (lib_tree_get_domain_i4). */
+int spo_i4(const spo_ctx *c, int level, int64_t n, int64_t *out, int cap);
+
+/* Near-field CSR (precalculation_near_field_csr_tri). */
+int64_t spo_near_csr_build(spo_ctx *c);   /* returns nnz */
+void spo_near_csr_get(const spo_ctx *c, int32_t *ia, int32_t *ja);
+/* Fill rows [row_lo,row_hi) with nthreads OpenMP threads. */
+void spo_near_fill(spo_ctx *c, int row_lo, int row_hi, int nthreads);
+void spo_near_values(const spo_ctx *c, double *abcd /* nnz*8 */);
+
+/* MLFMM: patterns + TL tables, then y = Z x. x,y: 2N complex (re,im). */
+void spo_mlfmm_setup(spo_ctx *c, int nthreads);
+void spo_matvec(spo_ctx *c, const double *x, double *y, int nthreads);
+/* Dense far-field operator check: y = near + exact (non-MLFMM) far field is
+ * not part of the reference; only the MLFMM matvec is. */
+
+/* Plane-wave right-hand side (get_vector_b_tri, Plane illumination). */
+void spo_rhs_plane(const spo_ctx *c, const double *k_in, const double *e_in,
+                   double *b);
+
+/* CERFACS GMRES (zPackgmres.f90 / lib_math_solver_GMRES.f90) with the
+ * oracle's matvec.  orth: 0 MGS, 1 IMGS, 2 CGS, 3 ICGS.  hist[maxit] gets the
+ * Arnoldi backward error per iteration.  Returns info(1). */
+int spo_gmres(spo_ctx *c, const double *b, double *x, int maxit, int restart,
+              double tol, int orth, int nthreads, double *hist, int *iters,
+              double *be_out);
+
+/* Generic CERFACS GMRES on a dense complex matrix A (column-major n x n),
+ * used to pin the solver against textbook cases. */
+int spo_gmres_dense(int n, const double *A, const double *b, double *x,
+                    int maxit, int restart, double tol, int orth,
+                    double *hist, int *iters, double *be_out);
+
+/* Dense 2N x 2N matrix (row-major complex) of all pair blocks. */
+void spo_dense_fill(const spo_ctx *c, double *Z, int nthreads);
+
+/* Radiation pattern of edge m after spo_mlfmm_setup: 4*S complex
+ * [k1 theta, k1 phi, k2 theta, k2 phi][sample]. */
+void spo_get_pattern(const spo_ctx *c, int m, double *out);
+
+/* Building blocks exposed for pinning tests. */
+void spo_quadrature_tri(int ngp, double *alpha, double *beta, double *w);
+double spo_iq_l(int q, const double *ra, const double *p1, const double *p2);
+/* Is_n3, Is_n1, Is_p1 of Iq_s for a triangle fn (corners, nq, mid). */
+void spo_iq_s_all(const double *ra, const double *corners9, const double *nq,
+                  const double *mid, int iqs_l, double *out3);
+void spo_k2q(int q, const double *ra, const double *fn19, int iqs_l, double *o);
+void spo_k3q(int q, const double *ra, const double *fn19, int iqs_l, double *o);
+void spo_k4q(int q, const double *ra, const double *fn19, int iqs_l, double *o);
+void spo_gauss_legendre(int n, double *x, double *w);
+void spo_hankel2_sph(int n, double zre, double zim, double *out /* 2n */);
+void spo_legendre_p(int n, double x, double *out);
+/* Transfer function table entry TL_k(r) for K terms at the K sampling dirs:
+ * out[2*(2K^2+2)] complex for medium kd (re,im). */
+void spo_tl(int K, double kre, double kim, const double *r, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,5 @@
+"""MI355X-native SIE solve loop (near-field fill + MLFMM matvec + GMRES)
+for SpeckleSim's triangle discretisation.  The compute path is
+libspeckle_gpu.so (HIP, gfx950); see include/speckle_gpu.h."""
+from .sie import SieTriMlfmm, plane_wave, FORMULATIONS  # noqa: F401
+from . import meshes  # noqa: F401

@@ -1,0 +1,97 @@
+"""ctypes binding of libspeckle_gpu.so (the C-ABI in include/speckle_gpu.h).
+
+The library is the product: there is no fallback.  If it is missing or the
+GPU is unavailable, calls raise ``SpeckleError``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libspeckle_gpu.so")
+
+# Exported entry points, in include/speckle_gpu.h order.
+EXPORTS = [
+    "spk_create", "spk_destroy", "spk_last_error", "spk_setup_tri", "spk_get_stats",
+    "spk_num_edges", "spk_matvec", "spk_matvec_device", "spk_matvec_parts", "spk_rhs_plane_wave",
+    "spk_gmres", "spk_gmres_device", "spk_gmres_history", "spk_get_rwg", "spk_get_tree",
+    "spk_get_i4", "spk_get_near_csr", "spk_near_blocks", "spk_get_patterns",
+]
+
+
+class SpeckleError(RuntimeError):
+    pass
+
+
+class Params(C.Structure):
+    _fields_ = [("lambda_", C.c_double), ("eps_r1_re", C.c_double), ("eps_r1_im", C.c_double),
+                ("eps_r2_re", C.c_double), ("eps_r2_im", C.c_double), ("formulation", C.c_int),
+                ("normal_convention", C.c_int), ("quad_points", C.c_int),
+                ("tree_s_opt", C.c_int), ("truncation_number", C.c_int),
+                ("precision", C.c_int), ("device", C.c_int)]
+
+
+class GmresParams(C.Structure):
+    _fields_ = [("max_iterations", C.c_int), ("restart", C.c_int), ("tolerance", C.c_double),
+                ("orthogonalization", C.c_int), ("use_initial_guess", C.c_int),
+                ("refinement_steps", C.c_int), ("refinement_tolerance", C.c_double)]
+
+
+class GmresInfo(C.Structure):
+    _fields_ = [("info", C.c_int), ("iterations", C.c_int),
+                ("arnoldi_backward_error", C.c_double), ("backward_error", C.c_double),
+                ("matvecs", C.c_int), ("seconds", C.c_double)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("n_edges", C.c_int64), ("near_blocks", C.c_int64), ("l_min", C.c_int),
+                ("l_max", C.c_int), ("samples", C.c_int),
+                ("boxes_per_level", C.c_int64 * 32), ("i4_pairs_per_level", C.c_int64 * 32),
+                ("singular_pairs", C.c_int64), ("setup_ms", C.c_double),
+                ("near_fill_ms", C.c_double), ("patterns_ms", C.c_double),
+                ("tables_ms", C.c_double), ("tree_ms", C.c_double), ("bytes_near", C.c_double),
+                ("bytes_patterns", C.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libspeckle_gpu.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise SpeckleError(f"{LIB_PATH} is missing: build it with "
+                           "`python -m specklesim_amd.build` (no CPU fallback exists)")
+    L = C.CDLL(LIB_PATH)
+    vp, dp, ip, i64p = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_int64)
+    L.spk_last_error.restype = C.c_char_p
+    L.spk_create.argtypes = [C.POINTER(Params), C.POINTER(vp)]
+    L.spk_destroy.argtypes = [vp]
+    L.spk_destroy.restype = None
+    L.spk_setup_tri.argtypes = [vp, C.c_int, dp, C.c_int, ip]
+    L.spk_get_stats.argtypes = [vp, C.POINTER(Stats)]
+    L.spk_num_edges.argtypes = [vp, i64p]
+    L.spk_matvec.argtypes = [vp, dp, dp]
+    L.spk_matvec_device.argtypes = [vp, vp, vp]
+    L.spk_matvec_parts.argtypes = [vp, dp, dp, dp]
+    L.spk_rhs_plane_wave.argtypes = [vp, dp, dp, dp]
+    L.spk_gmres.argtypes = [vp, dp, dp, C.POINTER(GmresParams), C.POINTER(GmresInfo)]
+    L.spk_gmres_device.argtypes = [vp, vp, vp, C.POINTER(GmresParams), C.POINTER(GmresInfo)]
+    L.spk_gmres_history.argtypes = [vp, dp, C.c_int, ip]
+    L.spk_get_rwg.argtypes = [vp, ip, ip]
+    L.spk_get_tree.argtypes = [vp, i64p, ip, ip, dp, dp]
+    L.spk_get_i4.argtypes = [vp, C.c_int, C.c_int64, i64p, C.c_int, ip]
+    L.spk_get_near_csr.argtypes = [vp, i64p, C.POINTER(C.c_int32), dp]
+    L.spk_near_blocks.argtypes = [vp, C.c_int, ip, ip, dp]
+    L.spk_get_patterns.argtypes = [vp, C.c_int, C.c_int, dp]
+    _lib = L
+    return L
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        msg = lib().spk_last_error()
+        raise SpeckleError(f"libspeckle_gpu error {rc}: {msg.decode() if msg else ''}")

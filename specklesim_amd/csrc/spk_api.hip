@@ -1,0 +1,851 @@
+// spk_api.hip -- C-ABI of the SIE solve loop: handle, precalculation
+// (lib_sie_ml_fmm_precalculation_tri), matvec
+// (lib_sie_ml_fmm_calculate_vector_b_tri), right-hand side
+// (get_vector_b_tri) and the CERFACS GMRES driver
+// (lib_math_solver_gmres_run) on HBM-resident vectors.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "spk_device.h"
+
+using namespace spk;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                    \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return fail(SPK_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));     \
+  } while (0)
+
+using zc = std::complex<double>;
+double now_ms() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+template <class T>
+struct DBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  ~DBuf() { reset(); }
+  void reset() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  hipError_t alloc(size_t count) {
+    reset();
+    n = count;
+    if (count == 0) return hipSuccess;
+    return hipMalloc(&p, count * sizeof(T));
+  }
+  hipError_t upload(const std::vector<T> &v) {
+    hipError_t e = alloc(v.size());
+    if (e != hipSuccess || v.empty()) return e;
+    return hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+  }
+};
+
+struct LevelDev {
+  DBuf<zd> B, D, tlt, up, down;
+  DBuf<int> child, parent, octant, i4_off, i4_box, i4_slot;
+  int nb = 0;
+};
+
+}  // namespace
+
+struct spk_handle {
+  spk_params p{};
+  Material mat{};
+  Tables tb;
+  int ngp = 3;
+  double qa[20], qb[20], qw[20], qa12[20], qb12[20], qw12[20];
+  hipStream_t st = nullptr;
+  bool ready = false;
+  // host mesh/topology
+  int N = 0;
+  std::vector<int> corner, element;
+  std::vector<FnRec> fn;
+  std::vector<ElemRec> er;
+  std::vector<double> rref;
+  Tree tree;
+  // device
+  DBuf<FnRec> d_fn;
+  DBuf<ElemRec> d_er;
+  DBuf<double> d_rref;
+  DBuf<int> d_perm, d_box_first, d_box_count, d_e2_off, d_e2_list, d_leaf_box, d_leafbox_of_pos;
+  DBuf<int64_t> d_near_off;
+  DBuf<zd> d_blocks;
+  DBuf<int> d_row_box, d_row_r0;
+  int spmv_wgs = 0;
+  DBuf<zd> d_rt, d_W;
+  DBuf<double> d_khat, d_tth, d_tph, d_q, d_leaf_centre;
+  std::vector<LevelDev> lev;
+  DBuf<zd> d_x, d_y, d_yfar, d_tmp;
+  zd sc1{1, 0}, sc2{1, 0};
+  // gmres workspace
+  DBuf<zd> g_V, g_w, g_r0, g_xc, g_b, g_h, g_partial, g_scal, g_y;
+  int g_m = 0;
+  std::vector<double> hist;
+  void *scan_tmp = nullptr;
+  size_t scan_bytes = 0;
+  spk_stats stats{};
+  ~spk_handle() {
+    if (scan_tmp) (void)hipFree(scan_tmp);
+    if (st) (void)hipStreamDestroy(st);
+  }
+};
+
+namespace {
+
+NearArgs near_args(const spk_handle *h) {
+  NearArgs a{};
+  a.fn = h->d_fn.p;
+  a.er = h->d_er.p;
+  a.rref = h->d_rref.p;
+  a.perm = h->d_perm.p;
+  a.box_first = h->d_box_first.p;
+  a.box_count = h->d_box_count.p;
+  a.e2_off = h->d_e2_off.p;
+  a.e2_list = h->d_e2_list.p;
+  a.near_off = h->d_near_off.p;
+  a.nbl = (int)h->tree.box_first.size();
+  a.form = h->p.formulation;
+  a.ngp = h->ngp;
+  a.edge_swap = (a.form == SPK_PMCHWT || a.form == SPK_MCTF || a.form == SPK_PMCHWT_S ||
+                 a.form == SPK_JMCFIE);
+  a.mat = h->mat;
+  for (int i = 0; i < 12; ++i) {
+    a.qa[i] = h->qa[i];
+    a.qb[i] = h->qb[i];
+    a.qw[i] = h->qw[i];
+    a.qa12[i] = h->qa12[i];
+    a.qb12[i] = h->qb12[i];
+    a.qw12[i] = h->qw12[i];
+  }
+  return a;
+}
+
+int near_fill(spk_handle *h) {
+  const int64_t nnz = h->tree.nnz;
+  HIPCHK(h->d_blocks.alloc(4 * (size_t)nnz));
+  if (nnz == 0) return SPK_OK;
+  NearArgs a = near_args(h);
+  DBuf<uint8_t> cls;
+  DBuf<int> ca, cb, ba, bb;
+  HIPCHK(cls.alloc(nnz));
+  HIPCHK(ca.alloc(nnz + 1));
+  HIPCHK(cb.alloc(nnz + 1));
+  HIPCHK(ba.alloc(nnz + 1));
+  HIPCHK(bb.alloc(nnz + 1));
+  HIPCHK(hipMemsetAsync(ca.p + nnz, 0, sizeof(int), h->st));
+  HIPCHK(hipMemsetAsync(cb.p + nnz, 0, sizeof(int), h->st));
+  launch_near_classify(a, nnz, cls.p, ca.p, cb.p, h->st);
+  exclusive_scan_int(ca.p, ba.p, nnz + 1, &h->scan_tmp, &h->scan_bytes, h->st);
+  exclusive_scan_int(cb.p, bb.p, nnz + 1, &h->scan_tmp, &h->scan_bytes, h->st);
+  int na = 0, nb = 0;
+  HIPCHK(hipMemcpyAsync(&na, ba.p + nnz, sizeof(int), hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipMemcpyAsync(&nb, bb.p + nnz, sizeof(int), hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  HIPCHK(hipGetLastError());
+  DBuf<NearTask> ta, tb;
+  DBuf<zd> kt, zz;
+  HIPCHK(ta.alloc(std::max(na, 1)));
+  HIPCHK(tb.alloc(std::max(nb, 1)));
+  HIPCHK(kt.alloc(2 * (size_t)std::max(nb, 1)));
+  HIPCHK(zz.alloc(4 * (size_t)std::max(na, 1)));
+  launch_near_emit(a, nnz, cls.p, ba.p, bb.p, ta.p, tb.p, h->st);
+  launch_near_edge(a, nb, tb.p, kt.p, h->st);
+  launch_near_singular(a, na, ta.p, cls.p, bb.p, kt.p, zz.p, h->st);
+  launch_near_regular(a, nnz, cls.p, ba.p, zz.p, h->d_blocks.p, h->st);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(h->st));
+  h->stats.singular_pairs = na;
+  return SPK_OK;
+}
+
+int setup_mlfmm(spk_handle *h) {
+  const Tree &t = h->tree;
+  const int S = h->tb.S, N = h->N;
+  double t0 = now_ms();
+  // leaf centres
+  const Level &LL = t.lev[t.l_max];
+  std::vector<double> lc(3 * LL.n.size());
+  for (size_t b = 0; b < LL.n.size(); ++b) box_centre(t, LL.n[b], t.l_max, &lc[3 * b]);
+  HIPCHK(h->d_leaf_centre.upload(lc));
+  HIPCHK(h->d_khat.upload(h->tb.khat));
+  HIPCHK(h->d_tth.upload(h->tb.ttheta));
+  HIPCHK(h->d_tph.upload(h->tb.tphi));
+  std::vector<double> q(60);
+  for (int i = 0; i < 20; ++i) {
+    q[i] = h->qa[i];
+    q[20 + i] = h->qb[i];
+    q[40 + i] = h->qw[i];
+  }
+  HIPCHK(h->d_q.upload(q));
+  HIPCHK(h->d_rt.alloc((size_t)N * 4 * S));
+  launch_patterns(N, S, h->ngp, h->d_fn.p, h->d_perm.p, h->d_leaf_box.p, h->d_leaf_centre.p,
+                  h->d_khat.p, h->d_tth.p, h->d_tph.p, h->d_q.p, h->d_q.p + 20, h->d_q.p + 40,
+                  zcx(h->mat.k1), zcx(h->mat.k2), h->d_rt.p, h->st);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(h->st));
+  h->stats.patterns_ms = now_ms() - t0;
+  t0 = now_ms();
+  // levels
+  h->lev.clear();
+  h->lev.resize(t.l_max + 1);
+  const int nthreads = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  for (int l = t.l_min; l <= t.l_max; ++l) {
+    const Level &L = t.lev[l];
+    LevelDev &D = h->lev[l];
+    D.nb = (int)L.n.size();
+    HIPCHK(D.B.alloc((size_t)D.nb * 8 * S));
+    HIPCHK(D.D.alloc((size_t)D.nb * 8 * S));
+    if (l < t.l_max) HIPCHK(D.child.upload(L.child));
+    HIPCHK(D.parent.upload(L.parent));
+    HIPCHK(D.octant.upload(L.octant));
+    HIPCHK(D.i4_off.upload(L.i4_off));
+    HIPCHK(D.i4_box.upload(L.i4_box));
+    HIPCHK(D.i4_slot.upload(L.i4_slot));
+    std::vector<cplx> tl;
+    make_tl_level(h->tb, h->mat, t, l, &tl, nthreads);
+    std::vector<zd> tlz(tl.size());
+    for (size_t i = 0; i < tl.size(); ++i) tlz[i] = zd{tl[i].re, tl[i].im};
+    HIPCHK(D.tlt.upload(tlz));
+    std::vector<cplx> up, down;
+    make_phase_level(h->tb, h->mat, t, l, &up, &down);
+    std::vector<zd> uz(up.size()), dz(down.size());
+    for (size_t i = 0; i < up.size(); ++i) {
+      uz[i] = zd{up[i].re, up[i].im};
+      dz[i] = zd{down[i].re, down[i].im};
+    }
+    HIPCHK(D.up.upload(uz));
+    HIPCHK(D.down.upload(dz));
+    h->stats.boxes_per_level[l] = D.nb;
+    h->stats.i4_pairs_per_level[l] = (int64_t)L.i4_box.size();
+  }
+  HIPCHK(h->d_W.alloc((size_t)LL.n.size() * 8 * S));
+  h->stats.tables_ms = now_ms() - t0;
+  // formulation scaling of the far field (MLFMM_tri_summation_New :478-495)
+  const zc e1(h->mat.eta1.re, h->mat.eta1.im), e2(h->mat.eta2.re, h->mat.eta2.im);
+  const zc ea(h->mat.eta_a.re, h->mat.eta_a.im);
+  zc s1 = 1.0, s2 = 1.0;
+  switch (h->p.formulation) {
+    case SPK_PMCHWT_S: s2 = e1; break;
+    case SPK_MCTF: s2 = e1 * e2; break;
+    case SPK_ICTF: s1 = 1.0 / ea; s2 = ea; break;
+    case SPK_MCTF2:
+      s1 = zc(h->mat.a_lc[0].re, h->mat.a_lc[0].im) / e1;
+      s2 = zc(h->mat.d_lc[0].re, h->mat.d_lc[0].im) * e1;
+      break;
+    default: break;
+  }
+  h->sc1 = zd{s1.real(), s1.imag()};
+  h->sc2 = zd{s2.real(), s2.imag()};
+  return SPK_OK;
+}
+
+// y = Z x on device vectors (2N complex each), far part into yfar first.
+int matvec_dev(spk_handle *h, const zd *x, zd *y, bool near_on = true, bool far_on = true) {
+  const Tree &t = h->tree;
+  const int S = h->tb.S, N = h->N, lmax = t.l_max, lmin = t.l_min;
+  hipStream_t st = h->st;
+  zd *yfar = h->d_yfar.p;
+  if (far_on) {
+    LevelDev &LL = h->lev[lmax];
+    launch_agg(LL.nb, S, N, h->d_box_first.p, h->d_box_count.p, h->d_perm.p, h->d_rt.p, x, LL.B.p,
+               st);
+    for (int l = lmax - 1; l >= lmin; --l)
+      launch_ss(h->lev[l].nb, S, h->lev[l].child.p, h->lev[l + 1].B.p, h->lev[l + 1].up.p,
+                h->lev[l].B.p, st);
+    for (int l = lmin; l <= lmax; ++l)
+      launch_m2l(h->lev[l].nb, S, h->lev[l].i4_off.p, h->lev[l].i4_box.p, h->lev[l].i4_slot.p,
+                 h->lev[l].B.p, h->lev[l].tlt.p, h->lev[l].D.p, st);
+    for (int l = lmin + 1; l <= lmax; ++l)
+      launch_rr(h->lev[l].nb, S, h->lev[l].parent.p, h->lev[l].octant.p, h->lev[l - 1].D.p,
+                h->lev[l].down.p, h->lev[l].D.p, st);
+    const zd I = {0, 1};
+    const zd kd1 = I * zcx(h->mat.k1), kd2 = I * zcx(h->mat.k2);
+    const zd my1 = zd{h->mat.my1, 0} * I * h->mat.omega, my2 = zd{h->mat.my2, 0} * I * h->mat.omega;
+    const zd ep1 = zcx(h->mat.eps1) * I * h->mat.omega, ep2 = zcx(h->mat.eps2) * I * h->mat.omega;
+    launch_fold(LL.nb, S, LL.D.p, kd1, kd2, my1, my2, ep1, ep2, h->d_W.p, st);
+    launch_disagg(N, S, h->d_perm.p, h->d_leafbox_of_pos.p, h->d_rt.p, h->d_W.p, h->sc1, h->sc2,
+                  near_on ? yfar : y, st);
+  }
+  if (near_on)
+    launch_near_spmv_rows(h->spmv_wgs, N, h->d_row_box.p, h->d_row_r0.p, h->d_box_first.p,
+                          h->d_box_count.p, h->d_e2_off.p, h->d_e2_list.p, h->d_near_off.p,
+                          h->d_perm.p, h->d_blocks.p, x, far_on ? yfar : nullptr, y, st);
+  HIPCHK(hipGetLastError());
+  return SPK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *spk_last_error(void) { return g_err.c_str(); }
+
+int spk_create(const spk_params *params, spk_handle **out) {
+  if (!params || !out) return fail(SPK_ERR_ARG, "null argument");
+  if (params->lambda <= 0) return fail(SPK_ERR_ARG, "lambda must be positive");
+  if (params->formulation < 1 || params->formulation > 8)
+    return fail(SPK_ERR_ARG, "unknown formulation");
+  const int ng = params->quad_points ? params->quad_points : 3;
+  if (ng != 1 && ng != 3 && ng != 4 && ng != 6 && ng != 7 && ng != 12)
+    return fail(SPK_ERR_ARG, "quad_points must be 1, 3, 4, 6, 7 or 12");
+  if (params->precision != SPK_FP64 && params->precision != SPK_MIXED)
+    return fail(SPK_ERR_ARG, "unknown precision");
+  int ndev = 0;
+  HIPCHK(hipGetDeviceCount(&ndev));
+  if (params->device < 0 || params->device >= ndev)
+    return fail(SPK_ERR_ARG, "device ordinal out of range");
+  HIPCHK(hipSetDevice(params->device));
+  std::unique_ptr<spk_handle> h(new spk_handle());
+  h->p = *params;
+  if (h->p.tree_s_opt <= 0) h->p.tree_s_opt = 250;
+  if (h->p.truncation_number <= 0) h->p.truncation_number = 19;
+  if (h->p.normal_convention != 2) h->p.normal_convention = 1;
+  h->ngp = ng;
+  make_material(h->p, &h->mat);
+  quadrature_tri(ng, h->qa, h->qb, h->qw);
+  quadrature_tri(12, h->qa12, h->qb12, h->qw12);
+  HIPCHK(hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking));
+  *out = h.release();
+  return SPK_OK;
+}
+
+void spk_destroy(spk_handle *h) { delete h; }
+
+int spk_setup_tri(spk_handle *h, int np, const double *pp, int nt, const int *tt) {
+  if (!h || !pp || !tt || np < 3 || nt < 1) return fail(SPK_ERR_ARG, "bad mesh arguments");
+  HIPCHK(hipSetDevice(h->p.device));
+  h->ready = false;
+  h->stats = spk_stats{};
+  const double t0 = now_ms();
+  std::string err;
+  int rc = build_rwg(np, pp, nt, tt, h->p.normal_convention, &h->corner, &h->element, &h->fn,
+                     &h->er, &h->rref, &err);
+  if (rc != SPK_OK) return fail(rc, err);
+  h->N = (int)h->rref.size();
+  if (h->N < 1) return fail(SPK_ERR_MESH, "mesh has no interior edges (no RWG pairs)");
+  build_tree(h->fn, h->N, h->p.tree_s_opt, &h->tree);
+  make_tables(h->p.truncation_number, &h->tb);
+  const Tree &t = h->tree;
+  h->stats.tree_ms = now_ms() - t0;
+  // upload topology
+  HIPCHK(h->d_fn.upload(h->fn));
+  HIPCHK(h->d_er.upload(h->er));
+  HIPCHK(h->d_rref.upload(h->rref));
+  HIPCHK(h->d_perm.upload(t.perm));
+  HIPCHK(h->d_box_first.upload(t.box_first));
+  HIPCHK(h->d_box_count.upload(t.box_count));
+  HIPCHK(h->d_e2_off.upload(t.e2_off));
+  HIPCHK(h->d_e2_list.upload(t.e2_list));
+  HIPCHK(h->d_near_off.upload(t.near_off));
+  HIPCHK(h->d_leaf_box.upload(t.leaf_box));
+  std::vector<int> lbp(h->N);
+  for (int p = 0; p < h->N; ++p) lbp[p] = t.leaf_box[t.perm[p]];
+  HIPCHK(h->d_leafbox_of_pos.upload(lbp));
+  std::vector<int> rb, r0;
+  for (size_t b = 0; b < t.box_count.size(); ++b)
+    for (int r = 0; r < t.box_count[b]; r += kSpmvRowsPerWG) {
+      rb.push_back((int)b);
+      r0.push_back(r);
+    }
+  h->spmv_wgs = (int)rb.size();
+  HIPCHK(h->d_row_box.upload(rb));
+  HIPCHK(h->d_row_r0.upload(r0));
+  HIPCHK(h->d_x.alloc(2 * (size_t)h->N));
+  HIPCHK(h->d_y.alloc(2 * (size_t)h->N));
+  HIPCHK(h->d_yfar.alloc(2 * (size_t)h->N));
+  HIPCHK(h->d_tmp.alloc(2 * (size_t)h->N));
+  double t1 = now_ms();
+  rc = near_fill(h);
+  if (rc != SPK_OK) return rc;
+  h->stats.near_fill_ms = now_ms() - t1;
+  rc = setup_mlfmm(h);
+  if (rc != SPK_OK) return rc;
+  h->stats.n_edges = h->N;
+  h->stats.near_blocks = t.nnz;
+  h->stats.l_min = t.l_min;
+  h->stats.l_max = t.l_max;
+  h->stats.samples = h->tb.S;
+  h->stats.bytes_near = (double)t.nnz * 64.0;
+  h->stats.bytes_patterns = (double)h->N * 4 * h->tb.S * 16.0;
+  h->stats.setup_ms = now_ms() - t0;
+  h->ready = true;
+  return SPK_OK;
+}
+
+int spk_get_stats(const spk_handle *h, spk_stats *out) {
+  if (!h || !out) return fail(SPK_ERR_ARG, "null argument");
+  *out = h->stats;
+  return SPK_OK;
+}
+
+int spk_num_edges(const spk_handle *h, int64_t *n) {
+  if (!h || !n) return fail(SPK_ERR_ARG, "null argument");
+  *n = h->N;
+  return SPK_OK;
+}
+
+int spk_matvec_device(spk_handle *h, const void *d_x, void *d_y) {
+  if (!h || !d_x || !d_y) return fail(SPK_ERR_ARG, "null argument");
+  if (!h->ready) return fail(SPK_ERR_STATE, "spk_setup_tri has not completed");
+  return matvec_dev(h, (const zd *)d_x, (zd *)d_y);
+}
+
+int spk_matvec(spk_handle *h, const double *x, double *y) {
+  if (!h || !x || !y) return fail(SPK_ERR_ARG, "null argument");
+  if (!h->ready) return fail(SPK_ERR_STATE, "spk_setup_tri has not completed");
+  const size_t bytes = 2 * (size_t)h->N * sizeof(zd);
+  HIPCHK(hipMemcpyAsync(h->d_x.p, x, bytes, hipMemcpyHostToDevice, h->st));
+  int rc = matvec_dev(h, h->d_x.p, h->d_y.p);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(y, h->d_y.p, bytes, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  return SPK_OK;
+}
+
+int spk_matvec_parts(spk_handle *h, const double *x, double *y_near, double *y_far) {
+  if (!h || !x) return fail(SPK_ERR_ARG, "null argument");
+  if (!h->ready) return fail(SPK_ERR_STATE, "spk_setup_tri has not completed");
+  const size_t bytes = 2 * (size_t)h->N * sizeof(zd);
+  HIPCHK(hipMemcpyAsync(h->d_x.p, x, bytes, hipMemcpyHostToDevice, h->st));
+  if (y_near) {
+    int rc = matvec_dev(h, h->d_x.p, h->d_y.p, true, false);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(y_near, h->d_y.p, bytes, hipMemcpyDeviceToHost, h->st));
+  }
+  if (y_far) {
+    int rc = matvec_dev(h, h->d_x.p, h->d_y.p, false, true);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(y_far, h->d_y.p, bytes, hipMemcpyDeviceToHost, h->st));
+  }
+  HIPCHK(hipStreamSynchronize(h->st));
+  return SPK_OK;
+}
+
+// get_vector_b_tri / incident_field_tri_general, plane wave
+int spk_rhs_plane_wave(spk_handle *h, const double k_in[3], const double e_in[3], double *bout) {
+  if (!h || !k_in || !e_in || !bout) return fail(SPK_ERR_ARG, "null argument");
+  if (h->N <= 0) return fail(SPK_ERR_STATE, "no mesh");
+  const int N = h->N, ng = h->ngp;
+  const zc k1(h->mat.k1.re, h->mat.k1.im);
+  const zc e1(h->mat.eta1.re, h->mat.eta1.im), e2(h->mat.eta2.re, h->mat.eta2.im);
+  const zc ea(h->mat.eta_a.re, h->mat.eta_a.im);
+  const zc a0(h->mat.a_lc[0].re, h->mat.a_lc[0].im), d0(h->mat.d_lc[0].re, h->mat.d_lc[0].im);
+  const zc ep1(h->mat.eps1.re, h->mat.eps1.im), ep2(h->mat.eps2.re, h->mat.eps2.im);
+  zc *b = (zc *)bout;
+  for (int m = 0; m < N; ++m) {
+    zc S4[4] = {0, 0, 0, 0};
+    for (int ot = 0; ot < 2; ++ot) {
+      const FnRec &f = h->fn[2 * m + ot];
+      for (int i = 0; i < ng; ++i) {
+        double Ra[3], rfm[3];
+        const double w0 = 1 - h->qa[i] - h->qb[i];
+        for (int d = 0; d < 3; ++d) {
+          Ra[d] = w0 * f.c[d] + h->qa[i] * f.c[3 + d] + h->qb[i] * f.c[6 + d];
+          rfm[d] = (Ra[d] - f.c[6 + d]) * f.df * 0.5;
+        }
+        const zc ph = (k1 * k_in[0]) * Ra[0] + (k1 * k_in[1]) * Ra[1] + (k1 * k_in[2]) * Ra[2];
+        const zc e = std::exp(zc(0, -1) * ph);
+        const zc E[3] = {e_in[0] * e, e_in[1] * e, e_in[2] * e};
+        const double c0my = h->mat.my1 * kC0;
+        const zc H[3] = {(k_in[1] * E[2] - k_in[2] * E[1]) / c0my,
+                         (k_in[2] * E[0] - k_in[0] * E[2]) / c0my,
+                         (k_in[0] * E[1] - k_in[1] * E[0]) / c0my};
+        const double *nf = f.nqf;
+        const zc nE[3] = {nf[1] * E[2] - nf[2] * E[1], nf[2] * E[0] - nf[0] * E[2],
+                          nf[0] * E[1] - nf[1] * E[0]};
+        const zc nH[3] = {nf[1] * H[2] - nf[2] * H[1], nf[2] * H[0] - nf[0] * H[2],
+                          nf[0] * H[1] - nf[1] * H[0]};
+        const double wa = h->qw[i];
+        S4[0] += (rfm[0] * E[0] + rfm[1] * E[1] + rfm[2] * E[2]) * wa * f.area;
+        S4[1] += (rfm[0] * H[0] + rfm[1] * H[1] + rfm[2] * H[2]) * wa * f.area;
+        S4[2] += (rfm[0] * nE[0] + rfm[1] * nE[1] + rfm[2] * nE[2]) * wa * f.area;
+        S4[3] += (rfm[0] * nH[0] + rfm[1] * nH[1] + rfm[2] * nH[2]) * wa * f.area;
+      }
+    }
+    switch (h->p.formulation) {
+      case SPK_PMCHWT: b[m] = S4[0]; b[m + N] = S4[1]; break;
+      case SPK_PMCHWT_S: b[m] = S4[0]; b[m + N] = S4[1] * e1; break;
+      case SPK_MCTF: b[m] = S4[0]; b[m + N] = S4[1] * e1 * e2; break;
+      case SPK_JMCFIE: b[m] = 1.0 / e1 * S4[0] + S4[3]; b[m + N] = -S4[2] + e1 * S4[1]; break;
+      case SPK_MCTF2: b[m] = a0 / e1 * S4[0]; b[m + N] = d0 * e1 * S4[1]; break;
+      case SPK_ICTF: b[m] = S4[0] / ea; b[m + N] = S4[1] * ea; break;
+      case SPK_CNF: b[m] = S4[3]; b[m + N] = -S4[2]; break;
+      default:
+        b[m] = S4[3] * h->mat.my1 / (h->mat.my1 + h->mat.my2);
+        b[m + N] = -S4[2] * ep1 / (ep1 + ep2);
+        break;
+    }
+  }
+  return SPK_OK;
+}
+
+// ---------------------------------------------------------------------------
+// CERFACS GMRES (zPackgmres.f90 :537-1565) on device vectors.
+// ---------------------------------------------------------------------------
+static void zrotg_(zc *ca, zc cb, double *cc, zc *s) {
+  if (std::abs(*ca) == 0.0) {
+    *cc = 0.0;
+    *s = 1.0;
+    *ca = cb;
+    return;
+  }
+  const double scale = std::abs(*ca) + std::abs(cb);
+  const double a1 = std::abs(*ca / zc(scale, 0.0)), b1 = std::abs(cb / zc(scale, 0.0));
+  const double norm = scale * std::sqrt(a1 * a1 + b1 * b1);
+  const zc alpha = *ca / std::abs(*ca);
+  *cc = std::abs(*ca) / norm;
+  *s = alpha * std::conj(cb) / norm;
+  *ca = alpha * norm;
+}
+static void zrot1(zc *x, zc *y, double cc, zc s) {
+  const zc t = cc * *x + s * *y;
+  *y = cc * *y - std::conj(s) * *x;
+  *x = t;
+}
+
+static int dev_dot(spk_handle *h, const zd *x, const zd *y, int64_t n, zc *out) {
+  launch_zdotc(n, x, y, h->g_partial.p, h->g_scal.p, h->st);
+  zd r;
+  HIPCHK(hipMemcpyAsync(&r, h->g_scal.p, sizeof(zd), hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  *out = zc(r.x, r.y);
+  return SPK_OK;
+}
+
+int spk_gmres_device(spk_handle *h, const void *d_b, void *d_x, const spk_gmres_params *gp,
+                     spk_gmres_info *info) {
+  if (!h || !d_b || !d_x || !gp) return fail(SPK_ERR_ARG, "null argument");
+  if (!h->ready) return fail(SPK_ERR_STATE, "spk_setup_tri has not completed");
+  const double t_start = now_ms();
+  const int64_t n = 2 * (int64_t)h->N;
+  const int iterMax = std::max(1, gp->max_iterations);
+  int m = gp->restart > 0 ? std::min(gp->restart, iterMax) : iterMax;
+  const double tol = gp->tolerance;
+  const int orth = gp->orthogonalization;
+  const zd *b = (const zd *)d_b;
+  zd *x = (zd *)d_x;
+  if (h->g_m != m || h->g_V.n != (size_t)n * (m + 1)) {
+    HIPCHK(h->g_V.alloc((size_t)n * (m + 1)));
+    h->g_m = m;
+  }
+  if (h->g_w.n != (size_t)n) {
+    HIPCHK(h->g_w.alloc(n));
+    HIPCHK(h->g_r0.alloc(n));
+    HIPCHK(h->g_xc.alloc(n));
+  }
+  HIPCHK(h->g_h.alloc(m + 1));
+  HIPCHK(h->g_y.alloc(m + 1));
+  HIPCHK(h->g_partial.alloc((size_t)reduce_blocks(n) * (m + 1)));
+  HIPCHK(h->g_scal.alloc(4));
+  zd *V = h->g_V.p, *w = h->g_w.p, *r0 = h->g_r0.p, *xc = h->g_xc.p;
+  std::vector<zc> H((size_t)(m + 1) * (m + 1), 0.0), yc(m + 1), rs(m + 1), hh(m + 1);
+  std::vector<double> rc(m + 1);
+  auto HH = [&](int i, int j) -> zc & { return H[(size_t)j * (m + 1) + i]; };
+  h->hist.clear();
+  int matvecs = 0, info1 = 0, it_total = 0, compRsd = 0;
+  double be = 0.0, bea = 0.0;
+  zc dtmp;
+  int rc_ = SPK_OK;
+  if (!gp->use_initial_guess) HIPCHK(hipMemsetAsync(x, 0, n * sizeof(zd), h->st));
+  if ((rc_ = dev_dot(h, b, b, n, &dtmp))) return rc_;
+  const double bn = std::sqrt(dtmp.real());
+  if (bn == 0.0) {
+    HIPCHK(hipMemsetAsync(x, 0, n * sizeof(zd), h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    if (info) *info = spk_gmres_info{0, 0, 0.0, 0.0, 0, (now_ms() - t_start) / 1e3};
+    return SPK_OK;
+  }
+  const double sPb = bn;
+  if ((rc_ = matvec_dev(h, x, r0))) return rc_;
+  ++matvecs;
+  launch_zsub(n, b, r0, w, h->st);
+  if ((rc_ = dev_dot(h, w, w, n, &dtmp))) return rc_;
+  double beta = std::sqrt(dtmp.real());
+  if (beta == 0.0) {
+    if (info) *info = spk_gmres_info{0, 0, 0.0, 0.0, matvecs, (now_ms() - t_start) / 1e3};
+    return SPK_OK;
+  }
+  int iterOut = 0;
+  bool done = false;
+  while (!done) {
+    launch_zscal_copy(n, zd{1.0 / beta, 0.0}, w, V, h->st);
+    HH(0, m) = beta;
+    for (int j = 1; j <= m; ++j) HH(j, m) = 0.0;
+    int jH = 1;
+    for (;;) {
+      if ((rc_ = matvec_dev(h, V + (size_t)(jH - 1) * n, w))) return rc_;
+      ++matvecs;
+      for (int j = 0; j < jH; ++j) HH(j, jH - 1) = 0.0;
+      int nOrtho = 0;
+      double dnormw = 0.0;
+      for (;;) {
+        ++nOrtho;
+        double dloo = 0.0;
+        if (orth == SPK_ORTH_MGS || orth == SPK_ORTH_IMGS) {
+          for (int j = 0; j < jH; ++j)
+            launch_mgs_step(n, j ? V + (size_t)(j - 1) * n : nullptr, j ? h->g_h.p + (j - 1) : nullptr,
+                            V + (size_t)j * n, w, h->g_partial.p, h->g_h.p + j, h->st);
+          launch_zaxpy_dev(n, h->g_h.p + (jH - 1), -1.0, V + (size_t)(jH - 1) * n, w, h->st);
+        } else {
+          launch_gemv_c(n, jH, V, n, w, h->g_partial.p, h->g_h.p, h->st);
+          launch_gemv_n_sub(n, jH, V, n, h->g_h.p, w, h->st);
+        }
+        launch_zdotc(n, w, w, h->g_partial.p + (size_t)reduce_blocks(n) * m, h->g_scal.p, h->st);
+        std::vector<zd> hv(jH + 1);
+        HIPCHK(hipMemcpyAsync(hv.data(), h->g_h.p, jH * sizeof(zd), hipMemcpyDeviceToHost, h->st));
+        HIPCHK(hipMemcpyAsync(&hv[jH], h->g_scal.p, sizeof(zd), hipMemcpyDeviceToHost, h->st));
+        HIPCHK(hipStreamSynchronize(h->st));
+        for (int j = 0; j < jH; ++j) {
+          const zc dVi(hv[j].x, hv[j].y);
+          HH(j, jH - 1) += dVi;
+          dloo += std::abs(dVi) * std::abs(dVi);
+        }
+        dnormw = std::sqrt(hv[jH].x);
+        if (orth == SPK_ORTH_IMGS || orth == SPK_ORTH_ICGS) {
+          dloo = std::sqrt(dloo);
+          if (dnormw <= dloo && nOrtho < 3) continue;
+        }
+        break;
+      }
+      HH(jH, jH - 1) = dnormw;
+      launch_zscal_copy(n, zd{1.0 / dnormw, 0.0}, w, V + (size_t)jH * n, h->st);
+      for (int j = 0; j < jH - 1; ++j) zrot1(&HH(j, jH - 1), &HH(j + 1, jH - 1), rc[j], rs[j]);
+      zc ahj = HH(jH - 1, jH - 1);
+      const zc ahp = HH(jH, jH - 1);
+      double cc;
+      zc ss;
+      zrotg_(&ahj, ahp, &cc, &ss);
+      rc[jH - 1] = cc;
+      rs[jH - 1] = ss;
+      zrot1(&HH(jH - 1, m), &HH(jH, m), cc, ss);
+      zrot1(&HH(jH - 1, jH - 1), &HH(jH, jH - 1), cc, ss);
+      HH(jH, jH - 1) = 0.0;
+      const double dnormres = std::abs(HH(jH, m));
+      bea = dnormres / (0.0 * 1.0 + sPb);
+      it_total = iterOut * m + jH;
+      h->hist.push_back(bea);
+      if (bea <= tol || it_total >= iterMax) {
+        compRsd = 1;
+        for (int j = 0; j < jH; ++j) yc[j] = HH(j, m);
+        for (int j = jH - 1; j >= 0; --j)
+          if (yc[j] != 0.0) {
+            yc[j] = yc[j] / HH(j, j);
+            const zc tq = yc[j];
+            for (int i = j - 1; i >= 0; --i) yc[i] = yc[i] - tq * HH(i, j);
+          }
+        std::vector<zd> yd(jH);
+        for (int j = 0; j < jH; ++j) yd[j] = zd{yc[j].real(), yc[j].imag()};
+        HIPCHK(hipMemcpyAsync(h->g_y.p, yd.data(), jH * sizeof(zd), hipMemcpyHostToDevice, h->st));
+        launch_gemv_n(n, jH, V, n, h->g_y.p, r0, h->st);
+        HIPCHK(hipMemcpyAsync(xc, x, n * sizeof(zd), hipMemcpyDeviceToDevice, h->st));
+        launch_zadd(n, r0, xc, h->st);
+        if ((rc_ = matvec_dev(h, xc, w))) return rc_;
+        ++matvecs;
+        launch_zsub(n, b, w, r0, h->st);
+        if ((rc_ = dev_dot(h, r0, r0, n, &dtmp))) return rc_;
+        be = std::sqrt(dtmp.real()) / sPb;
+        if (be <= tol || it_total >= iterMax) {
+          HIPCHK(hipMemcpyAsync(x, xc, n * sizeof(zd), hipMemcpyDeviceToDevice, h->st));
+          info1 = (be <= tol) ? 0 : -4;
+          done = true;
+          break;
+        }
+      }
+      ++jH;
+      if (jH <= m) continue;
+      break;
+    }
+    if (done) break;
+    ++iterOut;
+    --jH;
+    if (bea > tol) {
+      for (int j = 0; j < jH; ++j) yc[j] = HH(j, m);
+      for (int j = jH - 1; j >= 0; --j)
+        if (yc[j] != 0.0) {
+          yc[j] = yc[j] / HH(j, j);
+          const zc tq = yc[j];
+          for (int i = j - 1; i >= 0; --i) yc[i] = yc[i] - tq * HH(i, j);
+        }
+      std::vector<zd> yd(jH);
+      for (int j = 0; j < jH; ++j) yd[j] = zd{yc[j].real(), yc[j].imag()};
+      HIPCHK(hipMemcpyAsync(h->g_y.p, yd.data(), jH * sizeof(zd), hipMemcpyHostToDevice, h->st));
+      launch_gemv_n(n, jH, V, n, h->g_y.p, r0, h->st);
+      HIPCHK(hipMemcpyAsync(xc, x, n * sizeof(zd), hipMemcpyDeviceToDevice, h->st));
+      launch_zadd(n, r0, xc, h->st);
+    }
+    HIPCHK(hipMemcpyAsync(x, xc, n * sizeof(zd), hipMemcpyDeviceToDevice, h->st));
+    if (compRsd) {
+      if ((rc_ = matvec_dev(h, x, r0))) return rc_;
+      ++matvecs;
+      launch_zsub(n, b, r0, w, h->st);
+      if ((rc_ = dev_dot(h, w, w, n, &dtmp))) return rc_;
+      beta = std::sqrt(dtmp.real());
+    } else {
+      beta = std::abs(HH(m, m));
+      for (int j = m - 1; j >= 0; --j) {
+        HH(j, m) = 0.0;
+        zrot1(&HH(j, m), &HH(j + 1, m), rc[j], -rs[j]);
+      }
+      std::vector<zd> hd(m + 1);
+      for (int j = 0; j <= m; ++j) hd[j] = zd{HH(j, m).real(), HH(j, m).imag()};
+      HIPCHK(hipMemcpyAsync(h->g_y.p, hd.data(), (m + 1) * sizeof(zd), hipMemcpyHostToDevice,
+                            h->st));
+      launch_gemv_n(n, m + 1, V, n, h->g_y.p, w, h->st);
+    }
+  }
+  HIPCHK(hipStreamSynchronize(h->st));
+  HIPCHK(hipGetLastError());
+  if (info) {
+    info->info = info1;
+    info->iterations = it_total;
+    info->arnoldi_backward_error = bea;
+    info->backward_error = be;
+    info->matvecs = matvecs;
+    info->seconds = (now_ms() - t_start) / 1e3;
+  }
+  return SPK_OK;
+}
+
+int spk_gmres(spk_handle *h, const double *b, double *x, const spk_gmres_params *p,
+              spk_gmres_info *info) {
+  if (!h || !b || !x || !p) return fail(SPK_ERR_ARG, "null argument");
+  if (!h->ready) return fail(SPK_ERR_STATE, "spk_setup_tri has not completed");
+  const size_t bytes = 2 * (size_t)h->N * sizeof(zd);
+  DBuf<zd> db, dx;
+  HIPCHK(db.alloc(2 * (size_t)h->N));
+  HIPCHK(dx.alloc(2 * (size_t)h->N));
+  HIPCHK(hipMemcpyAsync(db.p, b, bytes, hipMemcpyHostToDevice, h->st));
+  HIPCHK(hipMemcpyAsync(dx.p, x, bytes, hipMemcpyHostToDevice, h->st));
+  int rc = spk_gmres_device(h, db.p, dx.p, p, info);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(x, dx.p, bytes, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  return SPK_OK;
+}
+
+int spk_gmres_history(const spk_handle *h, double *hist, int cap, int *count) {
+  if (!h || !count) return fail(SPK_ERR_ARG, "null argument");
+  *count = (int)h->hist.size();
+  if (hist)
+    for (int i = 0; i < std::min(cap, *count); ++i) hist[i] = h->hist[i];
+  return SPK_OK;
+}
+
+int spk_get_rwg(const spk_handle *h, int *corner, int *element) {
+  if (!h) return fail(SPK_ERR_ARG, "null argument");
+  if (corner) std::memcpy(corner, h->corner.data(), h->corner.size() * sizeof(int));
+  if (element) std::memcpy(element, h->element.data(), h->element.size() * sizeof(int));
+  return SPK_OK;
+}
+
+int spk_get_tree(const spk_handle *h, int64_t *leaf, int *l_min, int *l_max, double *D,
+                 double *xmin3) {
+  if (!h) return fail(SPK_ERR_ARG, "null argument");
+  const Tree &t = h->tree;
+  if (leaf) std::memcpy(leaf, t.leaf.data(), t.leaf.size() * sizeof(int64_t));
+  if (l_min) *l_min = t.l_min;
+  if (l_max) *l_max = t.l_max;
+  if (D) *D = t.D;
+  if (xmin3) std::memcpy(xmin3, t.xmin, 3 * sizeof(double));
+  return SPK_OK;
+}
+
+int spk_get_i4(const spk_handle *h, int level, int64_t n, int64_t *out, int cap, int *count) {
+  if (!h || !count) return fail(SPK_ERR_ARG, "null argument");
+  const Tree &t = h->tree;
+  if (level < t.l_min || level > t.l_max) return fail(SPK_ERR_ARG, "level outside [l_min, l_max]");
+  const Level &L = t.lev[level];
+  auto it = std::lower_bound(L.n.begin(), L.n.end(), n);
+  if (it == L.n.end() || *it != n) return fail(SPK_ERR_ARG, "box is empty");
+  const int a = (int)(it - L.n.begin());
+  std::vector<int64_t> v;
+  for (int q = L.i4_off[a]; q < L.i4_off[a + 1]; ++q) v.push_back(L.n[L.i4_box[q]]);
+  std::sort(v.begin(), v.end());
+  *count = (int)v.size();
+  if (out)
+    for (int i = 0; i < std::min(cap, *count); ++i) out[i] = v[i];
+  return SPK_OK;
+}
+
+int spk_get_near_csr(const spk_handle *h, int64_t *ia, int32_t *ja, double *abcd) {
+  if (!h) return fail(SPK_ERR_ARG, "null argument");
+  if (!h->ready) return fail(SPK_ERR_STATE, "spk_setup_tri has not completed");
+  const Tree &t = h->tree;
+  std::vector<zd> blocks;
+  if (abcd) {
+    blocks.resize(4 * (size_t)t.nnz);
+    HIPCHK(hipMemcpy(blocks.data(), h->d_blocks.p, blocks.size() * sizeof(zd), hipMemcpyDeviceToHost));
+  }
+  int64_t off = 0;
+  for (int m = 0; m < h->N; ++m) {
+    const int b = t.leaf_box[m];
+    const int cols = t.e2_off[b + 1] - t.e2_off[b];
+    const int r = t.pos[m] - t.box_first[b];
+    if (ia) ia[m] = off;
+    if (ja) std::memcpy(ja + off, &t.e2_list[t.e2_off[b]], cols * sizeof(int));
+    if (abcd) {
+      const zd *src = &blocks[4 * (size_t)(t.near_off[b] + (int64_t)r * cols)];
+      std::memcpy(abcd + 8 * off, src, 4 * (size_t)cols * sizeof(zd));
+    }
+    off += cols;
+  }
+  if (ia) ia[h->N] = off;
+  return SPK_OK;
+}
+
+int spk_near_blocks(spk_handle *h, int npairs, const int *m, const int *n, double *out) {
+  if (!h || !m || !n || !out || npairs < 0) return fail(SPK_ERR_ARG, "null argument");
+  if (h->N <= 0 || !h->d_fn.p) return fail(SPK_ERR_STATE, "no mesh");
+  for (int i = 0; i < npairs; ++i)
+    if (m[i] < 0 || m[i] >= h->N || n[i] < 0 || n[i] >= h->N)
+      return fail(SPK_ERR_ARG, "edge index out of range");
+  DBuf<int> dm, dn;
+  DBuf<zd> dout;
+  HIPCHK(dm.upload(std::vector<int>(m, m + npairs)));
+  HIPCHK(dn.upload(std::vector<int>(n, n + npairs)));
+  HIPCHK(dout.alloc(4 * (size_t)std::max(npairs, 1)));
+  NearArgs a = near_args(h);
+  launch_near_pairs(a, npairs, dm.p, dn.p, dout.p, h->st);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(h->st));
+  if (npairs) HIPCHK(hipMemcpy(out, dout.p, 4 * (size_t)npairs * sizeof(zd), hipMemcpyDeviceToHost));
+  return SPK_OK;
+}
+
+int spk_get_patterns(const spk_handle *h, int first, int count, double *out) {
+  if (!h || !out) return fail(SPK_ERR_ARG, "null argument");
+  if (!h->ready) return fail(SPK_ERR_STATE, "spk_setup_tri has not completed");
+  if (first < 0 || count < 0 || first + count > h->N) return fail(SPK_ERR_ARG, "edge range");
+  const int S = h->tb.S;
+  for (int e = 0; e < count; ++e) {
+    const int p = h->tree.pos[first + e];
+    HIPCHK(hipMemcpy(out + (size_t)e * 8 * S, h->d_rt.p + (size_t)p * 4 * S, 4 * (size_t)S * sizeof(zd),
+                     hipMemcpyDeviceToHost));
+  }
+  return SPK_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,129 @@
+// spk_device.h -- device-side complex arithmetic and shared kernel argument
+// structs for the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "spk_internal.h"
+
+namespace spk {
+
+struct zd {
+  double x, y;
+};
+struct zf {
+  float x, y;
+};
+
+__host__ __device__ inline zd mk(double a, double b) { return zd{a, b}; }
+__host__ __device__ inline zd operator+(zd a, zd b) { return zd{a.x + b.x, a.y + b.y}; }
+__host__ __device__ inline zd operator-(zd a, zd b) { return zd{a.x - b.x, a.y - b.y}; }
+__host__ __device__ inline zd operator-(zd a) { return zd{-a.x, -a.y}; }
+__host__ __device__ inline zd operator*(zd a, zd b) {
+  return zd{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+__host__ __device__ inline zd operator*(zd a, double s) { return zd{a.x * s, a.y * s}; }
+__host__ __device__ inline zd operator*(double s, zd a) { return zd{a.x * s, a.y * s}; }
+__host__ __device__ inline zd conj(zd a) { return zd{a.x, -a.y}; }
+__host__ __device__ inline zd operator/(zd a, zd b) {
+  double d = b.x * b.x + b.y * b.y;
+  return zd{(a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d};
+}
+__host__ __device__ inline zd operator/(zd a, double s) { return zd{a.x / s, a.y / s}; }
+__host__ __device__ inline zd operator/(double s, zd b) { return zd{s, 0.0} / b; }
+__host__ __device__ inline zd zcx(const cplx &c) { return zd{c.re, c.im}; }
+// i*a
+__host__ __device__ inline zd mul_i(zd a) { return zd{-a.y, a.x}; }
+
+__host__ __device__ inline zf operator+(zf a, zf b) { return zf{a.x + b.x, a.y + b.y}; }
+__host__ __device__ inline zf operator*(zf a, zf b) {
+  return zf{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+
+// Near-field fill arguments (one struct, passed by value).
+struct NearArgs {
+  const FnRec *fn;
+  const ElemRec *er;
+  const double *rref;
+  const int *perm;
+  const int *box_first;
+  const int *box_count;
+  const int *e2_off;
+  const int *e2_list;
+  const int64_t *near_off;
+  int nbl;
+  int form;
+  int ngp;
+  int edge_swap;  // PMCHWT family + JMCFIE use the 12x12 edge-adjacent Kt
+  Material mat;
+  double qa[12], qb[12], qw[12];
+  double qa12[12], qb12[12], qw12[12];
+};
+
+struct NearTask {
+  int64_t g;  // block index
+  int p;      // triangle pair (t-1)*2+(s-1)
+  int pad;
+};
+
+// launch wrappers (spk_kernels.hip)
+void launch_near_classify(const NearArgs &a, int64_t nnz, uint8_t *cls, int *cnt_a, int *cnt_b,
+                          hipStream_t st);
+void launch_near_emit(const NearArgs &a, int64_t nnz, const uint8_t *cls, const int *base_a,
+                      const int *base_b, NearTask *ta, NearTask *tb, hipStream_t st);
+void launch_near_edge(const NearArgs &a, int ntask, const NearTask *tb, zd *kt, hipStream_t st);
+void launch_near_singular(const NearArgs &a, int ntask, const NearTask *ta, const uint8_t *cls,
+                          const int *base_b, const zd *kt, zd *zz, hipStream_t st);
+void launch_near_regular(const NearArgs &a, int64_t nnz, const uint8_t *cls, const int *base_a,
+                         const zd *zz, zd *blocks, hipStream_t st);
+void exclusive_scan_int(const int *in, int *out, int64_t n, void **tmp, size_t *tmp_bytes,
+                        hipStream_t st);
+
+// MLFMM (FP64)
+struct MlfmmArgs {
+  int N, S, nbl;
+  const int *perm;        // position -> edge
+  const int *box_first, *box_count;
+};
+void launch_patterns(int N, int S, int ngp, const FnRec *fn, const int *perm, const int *leaf_box,
+                     const double *leaf_centre, const double *khat, const double *tth,
+                     const double *tph, const double *qa, const double *qb, const double *qw,
+                     zd k1, zd k2, zd *rt, hipStream_t st);
+void launch_agg(int nbl, int S, int N, const int *box_first, const int *box_count,
+                const int *perm, const zd *rt, const zd *x, zd *C, hipStream_t st);
+void launch_ss(int nb, int S, const int *child, const zd *Cc, const zd *up, zd *P, hipStream_t st);
+void launch_m2l(int nb, int S, const int *i4_off, const int *i4_box, const int *i4_slot,
+                const zd *B, const zd *tlt, zd *Dt, hipStream_t st);
+void launch_rr(int nb, int S, const int *parent, const int *octant, const zd *Dp, const zd *down,
+               zd *Dc, hipStream_t st);
+void launch_fold(int nbl, int S, const zd *D, zd kd1, zd kd2, zd my1, zd my2, zd ep1, zd ep2,
+                 zd *W, hipStream_t st);
+void launch_disagg(int N, int S, const int *perm, const int *leaf_box_of_pos, const zd *rt,
+                   const zd *W, zd sc1, zd sc2, zd *y, hipStream_t st);
+void launch_near_spmv_rows(int nwg, int N, const int *row_box, const int *row_r0,
+                           const int *box_first, const int *box_count, const int *e2_off,
+                           const int *e2_list, const int64_t *near_off, const int *perm,
+                           const zd *blocks, const zd *x, const zd *yadd, zd *y, hipStream_t st);
+constexpr int kSpmvRowsPerWG = 16;
+// arbitrary (m, n) pairs, one thread per pair (parity probe)
+void launch_near_pairs(const NearArgs &a, int npairs, const int *m, const int *n, zd *out,
+                       hipStream_t st);
+
+// BLAS-1 (deterministic reductions)
+int reduce_blocks(int64_t n);
+void launch_zdotc(int64_t n, const zd *x, const zd *y, zd *partial, zd *out, hipStream_t st);
+void launch_zaxpy(int64_t n, zd a, const zd *x, zd *y, hipStream_t st);
+void launch_zaxpy_dev(int64_t n, const zd *a, double sgn, const zd *x, zd *y, hipStream_t st);
+void launch_zscal_copy(int64_t n, zd a, const zd *x, zd *y, hipStream_t st);
+void launch_zsub(int64_t n, const zd *b, const zd *x, zd *y, hipStream_t st);  // y = b - x
+void launch_mgs_step(int64_t n, const zd *vprev, const zd *hprev, const zd *vj, zd *w,
+                     zd *partial, zd *hout, hipStream_t st);
+void launch_gemv_c(int64_t n, int k, const zd *V, int64_t ldv, const zd *w, zd *partial,
+                   zd *out, hipStream_t st);
+void launch_gemv_n_sub(int64_t n, int k, const zd *V, int64_t ldv, const zd *h, zd *w,
+                       hipStream_t st);
+void launch_gemv_n(int64_t n, int k, const zd *V, int64_t ldv, const zd *h, zd *out,
+                   hipStream_t st);
+void launch_zadd(int64_t n, const zd *a, zd *y, hipStream_t st);  // y += a
+
+}  // namespace spk

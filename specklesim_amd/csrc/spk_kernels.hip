@@ -1,0 +1,1204 @@
+// spk_kernels.hip -- gfx950 kernels of the SIE solve loop.
+//
+// Near-field fill: one thread per 2x2 impedance block for the regular
+// (3x3-rule) triangle pairs, separate task kernels for the singular
+// (singularity-subtracted, Hanninen 2006) pairs and for the 12x12
+// edge-adjacent K term, so that each launch runs one branch type.
+// MLFMM: aggregation, SS, M2L, RR, fold + disaggregation, with the sample
+// index s mapped to consecutive lanes everywhere (coalesced [..][S] rows).
+// Near SpMV: one workgroup per (leaf box, row chunk) over the box's dense
+// block matrix, x gathered once into LDS.
+#include <hipcub/hipcub.hpp>
+
+#include "spk_device.h"
+
+namespace spk {
+
+constexpr double PI = kPi;
+
+// ---------------------------------------------------------------------------
+// geometry helpers (src/lib_sie_math.f90:550-597)
+// ---------------------------------------------------------------------------
+__device__ inline void simplex(double a, double b, const double *c, double *o) {
+  const double w0 = 1 - a - b;
+  o[0] = w0 * c[0] + a * c[3] + b * c[6];
+  o[1] = w0 * c[1] + a * c[4] + b * c[7];
+  o[2] = w0 * c[2] + a * c[5] + b * c[8];
+}
+__device__ inline void cross3(const double *a, const double *b, double *o) {
+  o[0] = a[1] * b[2] - a[2] * b[1];
+  o[1] = a[2] * b[0] - a[0] * b[2];
+  o[2] = a[0] * b[1] - a[1] * b[0];
+}
+__device__ inline double dot3(const double *a, const double *b) {
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+__device__ inline double len3(const double *a) { return sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]); }
+// cross_rc with the real operand already rounded to float (nqf)
+__device__ inline void cross_rc(const double *a, const zd *b, zd *o) {
+  o[0] = a[1] * b[2] - a[2] * b[1];
+  o[1] = a[2] * b[0] - a[0] * b[2];
+  o[2] = a[0] * b[1] - a[1] * b[0];
+}
+__device__ inline zd dot_rc(const double *a, const zd *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+// exp(-i k R)
+__device__ inline zd expmikr(zd k, double R) {
+  double s, c;
+  sincos(k.x * R, &s, &c);
+  const double e = (k.y != 0.0) ? exp(k.y * R) : 1.0;
+  return zd{e * c, -e * s};
+}
+
+// Diver_Green (src/lib_sie_tri_calculation_mod.f90:1421-1432)
+__device__ inline void green(zd k, double R, zd &G, zd &DG) {
+  const zd E = expmikr(k, R);
+  const zd ikr = zd{-k.y * R, k.x * R};
+  const double inv = 1.0 / (4 * PI * R);
+  G = E * inv;
+  DG = -((zd{1.0 + ikr.x, ikr.y} * E) * (inv / R));
+}
+// Diver_Green_s (:1435-1451)
+__device__ inline void green_s(zd k, double R, zd &Gs, zd &DGs) {
+  const zd E = expmikr(k, R);
+  const zd k2 = k * k;
+  if (R == 0.0) {
+    Gs = zd{k.y, -k.x} / (4 * PI);
+    DGs = k2 / (8 * PI);
+  } else {
+    const zd ikr = zd{-k.y * R, k.x * R};
+    Gs = (E - zd{1.0, 0.0}) / (4 * PI * R) + k2 * R / (8 * PI);
+    DGs = (zd{1.0, 0.0} - zd{1.0 + ikr.x, ikr.y} * E) / (4 * PI * (R * R)) + k2 / (8 * PI);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Hanninen analytic integrals (PIER 63, 2006): Iq_L (:1074-1119),
+// Iq_s (:1256-1331), K2q (:1333-1358), K3q (:1395-1419), K4q (:1360-1393).
+// ---------------------------------------------------------------------------
+__device__ void iq_l_all(const double *Ra, const double *p1, const double *p2, double *o) {
+  double d[3] = {p2[0] - p1[0], p2[1] - p1[1], p2[2] - p1[2]};
+  const double l = len3(d);
+  const double s[3] = {d[0] / l, d[1] / l, d[2] / l};
+  const double t2[3] = {p2[0] - Ra[0], p2[1] - Ra[1], p2[2] - Ra[2]};
+  const double t1[3] = {p1[0] - Ra[0], p1[1] - Ra[1], p1[2] - Ra[2]};
+  const double Sp = dot3(t2, s), Sn = dot3(t1, s);
+  const double a1[3] = {Ra[0] - p1[0], Ra[1] - p1[1], Ra[2] - p1[2]};
+  const double a2[3] = {Ra[0] - p2[0], Ra[1] - p2[1], Ra[2] - p2[2]};
+  const double Rn = len3(a1), Rp = len3(a2);
+  const double Sum_p = Rp + Sp, Sum_n = Rn + Sn, Dif_p = Rp - Sp, Dif_n = Rn - Sn;
+  const double R0 = sqrt(Rp * Rp - Sp * Sp);
+  double n1;
+  if (R0 == 0.0)
+    n1 = log(Sp / Sn);
+  else if (fabs(Dif_p) > fabs(Sum_n))
+    n1 = log(Dif_n / Dif_p);
+  else
+    n1 = log(Sum_p / Sum_n);
+  const double p1v = 0.5 * (R0 * R0) * n1 + 0.5 * (Sp * Rp - Sn * Rn);
+  const double p3v = 0.75 * (R0 * R0) * p1v + 0.25 * (Sp * (Rp * Rp * Rp) - Sn * (Rn * Rn * Rn));
+  o[0] = n1;
+  o[1] = p1v;
+  o[2] = p3v;
+}
+
+struct Anal {
+  double Is_n3, Is_n1, Is_p1, h;
+  double m[3][3];
+  double ILst[3][3];
+};
+
+__device__ inline void edge_ps_pt(const double *c, int i, const double *&ps, const double *&pt) {
+  if (i == 0) {
+    ps = c + 6;
+    pt = c + 3;
+  } else if (i == 1) {
+    ps = c;
+    pt = c + 6;
+  } else {
+    ps = c + 3;
+    pt = c;
+  }
+}
+
+__device__ void analytic_eval(const double *Ra, const FnRec &f, bool iqs, Anal &A) {
+  double arr[3][3];
+  for (int i = 0; i < 3; ++i) {
+    double d[3] = {f.c[3 * i] - Ra[0], f.c[3 * i + 1] - Ra[1], f.c[3 * i + 2] - Ra[2]};
+    const double l = len3(d);
+    arr[i][0] = d[0] / l;
+    arr[i][1] = d[1] / l;
+    arr[i][2] = d[2] / l;
+  }
+  const double x = 1 + dot3(arr[0], arr[1]) + dot3(arr[0], arr[2]) + dot3(arr[1], arr[2]);
+  double r[3];
+  cross3(arr[1], arr[2], r);
+  const double y = fabs(dot3(arr[0], r));
+  const double Omeg = 2 * atan2(y, x);
+  const double dm[3] = {Ra[0] - f.mid[0], Ra[1] - f.mid[1], Ra[2] - f.mid[2]};
+  const double h = dot3(f.nq, dm);
+  double Is_n3 = 0.0;
+  if (h != 0.0 && !iqs && Omeg <= PI && Omeg > -PI) Is_n3 = (-1 / h) * Omeg;
+  // m_vector (:1040-1050)
+  {
+    const double *c = f.c;
+    double d[3], xx[3];
+    d[0] = c[6] - c[3]; d[1] = c[7] - c[4]; d[2] = c[8] - c[5];
+    cross3(f.nq, d, xx);
+    double l = len3(xx);
+    A.m[0][0] = xx[0] / l; A.m[0][1] = xx[1] / l; A.m[0][2] = xx[2] / l;
+    d[0] = c[0] - c[6]; d[1] = c[1] - c[7]; d[2] = c[2] - c[8];
+    cross3(f.nq, d, xx);
+    l = len3(xx);
+    A.m[1][0] = xx[0] / l; A.m[1][1] = xx[1] / l; A.m[1][2] = xx[2] / l;
+    d[0] = c[3] - c[0]; d[1] = c[4] - c[1]; d[2] = c[5] - c[2];
+    cross3(f.nq, d, xx);
+    l = len3(xx);
+    A.m[2][0] = xx[0] / l; A.m[2][1] = xx[1] / l; A.m[2][2] = xx[2] / l;
+  }
+  double tv[3], ILts[3][3], sum_il = 0.0;
+  for (int i = 0; i < 3; ++i) {
+    const double *ps, *pt;
+    edge_ps_pt(f.c, i, ps, pt);
+    const double mp[3] = {Ra[0] - (pt[0] + ps[0]) / 2, Ra[1] - (pt[1] + ps[1]) / 2,
+                          Ra[2] - (pt[2] + ps[2]) / 2};
+    tv[i] = dot3(A.m[i], mp);
+    iq_l_all(Ra, pt, ps, ILts[i]);
+    iq_l_all(Ra, ps, pt, A.ILst[i]);
+    sum_il = sum_il + tv[i] * ILts[i][0];
+  }
+  const double Is_n1 = 1.0 * ((-1 * (h * h)) * Is_n3 - sum_il);
+  sum_il = 0.0;
+  for (int i = 0; i < 3; ++i) sum_il = sum_il + tv[i] * ILts[i][1];
+  const double Is_p1 = (double)(1.0f / 3.0f) * ((h * h) * Is_n1 - sum_il);
+  A.Is_n3 = Is_n3;
+  A.Is_n1 = Is_n1;
+  A.Is_p1 = Is_p1;
+  A.h = h;
+}
+
+__device__ inline double iq_s_val(const Anal &A, int q, bool iqs) {
+  if (iqs && q > -1) return 0.0;
+  if (q == -3) return A.Is_n3;
+  if (q == -1) return A.Is_n1;
+  return A.Is_p1;
+}
+
+__device__ void k2q_eval(int q, const double *Ra, const FnRec &f, bool iqs, const Anal &A, double *o) {
+  const int col = (q == -1) ? 1 : 2;
+  double sum[3] = {0, 0, 0};
+  for (int i = 0; i < 3; ++i)
+    for (int k = 0; k < 3; ++k) sum[k] = sum[k] + A.ILst[i][col] * A.m[i][k];
+  const double Iqs = iq_s_val(A, q, iqs);
+  const double coef = (q == -1) ? 1.0 : (double)(1.0f / 3.0f);
+  for (int k = 0; k < 3; ++k) {
+    const double rho = Ra[k] - A.h * f.nq[k];
+    o[k] = (coef * sum[k] + (rho - f.c[6 + k]) * Iqs) * f.df * 0.5;
+  }
+}
+__device__ void k3q_eval(int q, const FnRec &f, bool iqs, const Anal &A, double *o) {
+  const int col = (q == -1) ? 0 : 1;
+  double sum[3] = {0, 0, 0};
+  for (int i = 0; i < 3; ++i)
+    for (int k = 0; k < 3; ++k) sum[k] = sum[k] + A.m[i][k] * A.ILst[i][col];
+  const double Is = iq_s_val(A, q - 2, iqs);
+  for (int k = 0; k < 3; ++k) o[k] = sum[k] - ((A.h * (double)q) * f.nq[k]) * Is;
+}
+__device__ void k4q_eval(int q, const double *Ra, const FnRec &f, bool iqs, const Anal &A, double *o) {
+  if (iqs) {
+    o[0] = o[1] = o[2] = 0.0;
+    return;
+  }
+  double k3[3], x[3];
+  k3q_eval(q, f, iqs, A, k3);
+  const double d[3] = {Ra[0] - f.c[6], Ra[1] - f.c[7], Ra[2] - f.c[8]};
+  cross3(d, k3, x);
+  for (int k = 0; k < 3; ++k) o[k] = (-x[k] * f.df) * 0.5;
+}
+
+// result_element_JMCFIE per medium
+struct Rel {
+  zd Kn, Kt, Tn, Tt;
+};
+
+// tri_outer_integration (:2255-2302), both media at once.
+template <bool NEED_N>
+__device__ void normal_pair(const FnRec &fm, const FnRec &fn, const NearArgs &A, Rel *rv) {
+  const zd kk[2] = {zcx(A.mat.k1), zcx(A.mat.k2)};
+  const zd ik[2] = {1.0 / kk[0], 1.0 / kk[1]};
+  for (int md = 0; md < 2; ++md) rv[md] = Rel{{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+  double x_nt = 0.0, x_nn = 0.0;
+  const int ngp = A.ngp;
+  for (int j = 0; j < ngp; ++j) {
+    double ra[3], rfm[3], rfn[3];
+    simplex(A.qa[j], A.qb[j], fm.c, ra);
+    for (int k = 0; k < 3; ++k) {
+      rfm[k] = 0.5 * (ra[k] - fm.c[6 + k]) * fm.df;
+      rfn[k] = 0.5 * (ra[k] - fn.c[6 + k]) * fn.df;
+    }
+    const double TRF = A.qw[j] * fm.area;
+    zd IK[2][3], ITa[2][3], ITbn[2][3], ITbt[2];
+    for (int md = 0; md < 2; ++md) {
+      ITbt[md] = zd{0, 0};
+      for (int k = 0; k < 3; ++k) IK[md][k] = ITa[md][k] = ITbn[md][k] = zd{0, 0};
+    }
+    for (int i = 0; i < ngp; ++i) {
+      double rq[3], rfq[3], R[3], Rn[3], x[3];
+      simplex(A.qa[i], A.qb[i], fn.c, rq);
+      for (int k = 0; k < 3; ++k) {
+        rfq[k] = 0.5 * (rq[k] - fn.c[6 + k]) * fn.df;
+        R[k] = ra[k] - rq[k];
+      }
+      const double Rl = len3(R);
+      for (int k = 0; k < 3; ++k) Rn[k] = R[k] / Rl;
+      cross3(rfq, Rn, x);
+      const double TRFi = A.qw[i] * fn.area;
+#pragma unroll
+      for (int md = 0; md < 2; ++md) {
+        zd G, DG;
+        green(kk[md], Rl, G, DG);
+        for (int k = 0; k < 3; ++k) {
+          IK[md][k] = IK[md][k] + x[k] * DG * TRFi;
+          ITa[md][k] = ITa[md][k] + rfq[k] * G * TRFi;
+          if (NEED_N) ITbn[md][k] = ITbn[md][k] + (fn.df * Rn[k]) * DG * TRFi;
+        }
+        ITbt[md] = ITbt[md] + fn.df * G * TRFi;
+      }
+    }
+#pragma unroll
+    for (int md = 0; md < 2; ++md) {
+      rv[md].Kt = rv[md].Kt + dot_rc(rfm, IK[md]) * TRF;
+      const zd tmp_s = (ik[md] * fm.df) * ITbt[md];
+      rv[md].Tt = rv[md].Tt + zd{0.0, TRF} * (kk[md] * dot_rc(rfm, ITa[md]) - tmp_s);
+      if (NEED_N) {
+        zd nk[3], av[3], tv[3];
+        cross_rc(fm.nqf, IK[md], nk);
+        rv[md].Kn = rv[md].Kn + dot_rc(rfm, nk) * TRF;
+        for (int k = 0; k < 3; ++k) av[k] = kk[md] * ITa[md][k] + ik[md] * ITbn[md][k];
+        cross_rc(fm.nqf, av, tv);
+        rv[md].Tn = rv[md].Tn + zd{0.0, TRF} * dot_rc(rfm, tv);
+      }
+    }
+    double nx[3];
+    cross3(fm.nq, rfm, nx);
+    x_nt = x_nt - dot3(nx, rfn) * TRF;
+    x_nn = x_nn - dot3(rfm, rfn) * TRF;
+  }
+  const double fnt = (double)(float)x_nt, fnn = (double)(float)x_nn;
+  rv[0].Kt = rv[0].Kt + zd{-0.5 * fnt, 0};
+  rv[1].Kt = rv[1].Kt + zd{0.5 * fnt, 0};
+  rv[0].Kn = rv[0].Kn + zd{-0.5 * fnn, 0};
+  rv[1].Kn = rv[1].Kn + zd{0.5 * fnn, 0};
+}
+
+// tri_outer_integration_singular (:2199-2253) with the analytic part
+// (:2115-2145) shared by both media and the smoothed kernel (:2080-2113).
+__device__ void singular_pair(const FnRec &fm, const FnRec &fn, bool iqs, const NearArgs &A,
+                              Rel *rv) {
+  const zd kk[2] = {zcx(A.mat.k1), zcx(A.mat.k2)};
+  const zd ik[2] = {1.0 / kk[0], 1.0 / kk[1]};
+  for (int md = 0; md < 2; ++md) rv[md] = Rel{{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+  double x_nt = 0.0, x_nn = 0.0;
+  const int ngp = A.ngp;
+  for (int j = 0; j < ngp; ++j) {
+    double ra[3], rfm[3], rfn[3];
+    simplex(A.qa[j], A.qb[j], fm.c, ra);
+    for (int k = 0; k < 3; ++k) {
+      rfm[k] = 0.5 * (ra[k] - fm.c[6 + k]) * fm.df;
+      rfn[k] = 0.5 * (ra[k] - fn.c[6 + k]) * fn.df;
+    }
+    const double TRF = A.qw[j] * fm.area;
+    Anal an;
+    analytic_eval(ra, fn, iqs, an);
+    double k2n[3], k2p[3], k3n[3], k3p[3], k4n[3], k4p[3];
+    k2q_eval(-1, ra, fn, iqs, an, k2n);
+    k3q_eval(-1, fn, iqs, an, k3n);
+    k4q_eval(-1, ra, fn, iqs, an, k4n);
+    k2q_eval(1, ra, fn, iqs, an, k2p);
+    k3q_eval(1, fn, iqs, an, k3p);
+    k4q_eval(1, ra, fn, iqs, an, k4p);
+    const double Iqs_n1 = iq_s_val(an, -1, iqs), Iqs_p1 = iq_s_val(an, 1, iqs);
+    for (int md = 0; md < 2; ++md) {
+      const zd kd = kk[md], kd2 = kd * kd;
+      zd IK[3], ITa[3], ITbn[3], ITbt;
+      // analytic part
+      for (int k = 0; k < 3; ++k) {
+        IK[k] = iqs ? zd{0, 0} : zd{(1 / (4 * PI)) * k4n[k], 0} - kd2 / (8 * PI) * k4p[k];
+        ITa[k] = zd{(1 / (4 * PI)) * k2n[k], 0} - kd2 / (8 * PI) * k2p[k];
+        ITbn[k] = -(zd{(fn.df / (4 * PI)) * k3n[k], 0} - kd2 * fn.df / (8 * PI)) * k3p[k];
+      }
+      ITbt = zd{(fn.df / (4 * PI)) * Iqs_n1, 0} - kd2 * fn.df / (8 * PI) * Iqs_p1;
+      // smooth part
+      zd sIK[3] = {{0, 0}, {0, 0}, {0, 0}}, sITa[3] = {{0, 0}, {0, 0}, {0, 0}},
+         sITbn[3] = {{0, 0}, {0, 0}, {0, 0}}, sITbt = {0, 0};
+      for (int i = 0; i < ngp; ++i) {
+        double rq[3], rfq[3], R[3];
+        simplex(A.qa[i], A.qb[i], fn.c, rq);
+        for (int k = 0; k < 3; ++k) {
+          rfq[k] = 0.5 * (rq[k] - fn.c[6 + k]) * fn.df;
+          R[k] = ra[k] - rq[k];
+        }
+        const double Rl = len3(R);
+        zd Gs, DGs;
+        green_s(kd, Rl, Gs, DGs);
+        const double TRFi = A.qw[i] * fn.area;
+        if (!iqs) {
+          double Rn[3] = {R[0] / Rl, R[1] / Rl, R[2] / Rl}, x[3];
+          cross3(rfq, Rn, x);
+          for (int k = 0; k < 3; ++k) {
+            sIK[k] = sIK[k] + x[k] * DGs * TRFi;
+            sITbn[k] = sITbn[k] + (fn.df * Rn[k]) * DGs * TRFi;
+          }
+        }
+        for (int k = 0; k < 3; ++k) sITa[k] = sITa[k] + rfq[k] * Gs * TRFi;
+        sITbt = sITbt + fn.df * Gs * TRFi;
+      }
+      for (int k = 0; k < 3; ++k) {
+        ITbn[k] = ITbn[k] + sITbn[k];
+        IK[k] = IK[k] + sIK[k];
+        ITa[k] = ITa[k] + sITa[k];
+      }
+      ITbt = ITbt + sITbt;
+      zd nk[3], av[3], t1[3], t2[3], tv[3];
+      cross_rc(fm.nqf, IK, nk);
+      rv[md].Kn = rv[md].Kn + dot_rc(rfm, nk) * TRF;
+      rv[md].Kt = rv[md].Kt + dot_rc(rfm, IK) * TRF;
+      for (int k = 0; k < 3; ++k) av[k] = kd * ITa[k];
+      cross_rc(fm.nqf, av, t1);
+      for (int k = 0; k < 3; ++k) av[k] = ik[md] * ITbn[k];
+      cross_rc(fm.nqf, av, t2);
+      for (int k = 0; k < 3; ++k) tv[k] = t1[k] + t2[k];
+      rv[md].Tn = rv[md].Tn + zd{0.0, TRF} * dot_rc(rfm, tv);
+      const zd tmp_s = (ik[md] * fm.df) * ITbt;
+      rv[md].Tt = rv[md].Tt + zd{0.0, TRF} * (kd * dot_rc(rfm, ITa) - tmp_s);
+    }
+    double nx[3];
+    cross3(fm.nq, rfm, nx);
+    x_nt = x_nt - dot3(nx, rfn) * TRF;
+    x_nn = x_nn - dot3(rfm, rfn) * TRF;
+  }
+  const double fnt = (double)(float)x_nt, fnn = (double)(float)x_nn;
+  rv[0].Kt = rv[0].Kt + zd{-0.5 * fnt, 0};
+  rv[1].Kt = rv[1].Kt + zd{0.5 * fnt, 0};
+  rv[0].Kn = rv[0].Kn + zd{-0.5 * fnn, 0};
+  rv[1].Kn = rv[1].Kn + zd{0.5 * fnn, 0};
+}
+
+// tri_outer_integration_edge_adjacent (:2165-2197), 12x12 rule, both media.
+__device__ void edge_adjacent_kt(const FnRec &fm, const FnRec &fn, const NearArgs &A, zd *kt) {
+  const zd kk[2] = {zcx(A.mat.k1), zcx(A.mat.k2)};
+  zd K[2] = {{0, 0}, {0, 0}};
+  double x_nt = 0.0;
+  for (int j = 0; j < 12; ++j) {
+    double ra[3], rfm[3], rfn[3];
+    simplex(A.qa12[j], A.qb12[j], fm.c, ra);
+    for (int k = 0; k < 3; ++k) {
+      rfm[k] = 0.5 * (ra[k] - fm.c[6 + k]) * fm.df;
+      rfn[k] = 0.5 * (ra[k] - fn.c[6 + k]) * fn.df;
+    }
+    const double TRF = A.qw12[j] * fm.area;
+    zd Ik[2][3] = {{{0, 0}, {0, 0}, {0, 0}}, {{0, 0}, {0, 0}, {0, 0}}};
+    for (int i = 0; i < 12; ++i) {
+      double rq[3], rfq[3], R[3], Rn[3], x[3];
+      simplex(A.qa12[i], A.qb12[i], fn.c, rq);
+      for (int k = 0; k < 3; ++k) {
+        rfq[k] = 0.5 * (rq[k] - fn.c[6 + k]) * fn.df;
+        R[k] = ra[k] - rq[k];
+      }
+      const double Rl = len3(R);
+      for (int k = 0; k < 3; ++k) Rn[k] = R[k] / Rl;
+      cross3(rfq, Rn, x);
+      const double TRFi = A.qw12[i] * fn.area;
+#pragma unroll
+      for (int md = 0; md < 2; ++md) {
+        zd G, DG;
+        green(kk[md], Rl, G, DG);
+        for (int k = 0; k < 3; ++k) Ik[md][k] = Ik[md][k] + x[k] * DG * TRFi;
+      }
+    }
+    for (int md = 0; md < 2; ++md) K[md] = K[md] + dot_rc(rfm, Ik[md]) * TRF;
+    double nx[3];
+    cross3(fm.nq, rfm, nx);
+    x_nt = x_nt - dot3(rfn, nx) * TRF;
+  }
+  const double fnt = (double)(float)x_nt;
+  kt[0] = K[0] + zd{-0.5 * fnt, 0};
+  kt[1] = K[1] + zd{0.5 * fnt, 0};
+}
+
+// tri_impedance_building_* (:1713-2009): per-pair zz
+__device__ void assemble_zz(const NearArgs &A, const Rel *r, zd *zz) {
+  const zd e1 = zcx(A.mat.eta1), e2 = zcx(A.mat.eta2);
+  const zd one = {1.0, 0.0};
+  switch (A.form) {
+    case SPK_PMCHWT:
+    case SPK_MCTF:
+    case SPK_PMCHWT_S:
+      zz[0] = e1 * r[0].Tt + e2 * r[1].Tt;
+      zz[1] = -(r[0].Kt + r[1].Kt);
+      zz[2] = (one / e1) * r[0].Tt + (one / e2) * r[1].Tt;
+      zz[3] = zd{0, 0};
+      break;
+    case SPK_JMCFIE:
+      zz[0] = r[0].Kn - r[1].Kn + r[0].Tt + r[1].Tt;
+      zz[1] = -((one / e1) * (r[0].Kt - r[0].Tn) + (one / e2) * (r[1].Kt + r[1].Tn));
+      zz[2] = -(e1 * (r[0].Tn - r[0].Kt)) + e2 * (r[1].Kt + r[1].Tn);
+      zz[3] = zd{0, 0};
+      break;
+    case SPK_MCTF2: {
+      const zd a0 = zcx(A.mat.a_lc[0]), a1 = zcx(A.mat.a_lc[1]);
+      const zd d0 = zcx(A.mat.d_lc[0]), d1 = zcx(A.mat.d_lc[1]);
+      zz[0] = r[0].Tt * a0 + r[1].Tt * a1;
+      zz[1] = -(r[0].Kt * a0 / e1 + r[1].Kt * a1 / e2);
+      zz[2] = e1 * d0 * r[0].Kt + r[1].Kt * e2 * d1;
+      zz[3] = d0 * r[0].Tt + d1 * r[1].Tt;
+    } break;
+    case SPK_ICTF: {
+      const zd ea = zcx(A.mat.eta_a);
+      zz[0] = (e1 * r[0].Tt + e2 * r[1].Tt) / ea;
+      zz[1] = -(r[0].Kt + r[1].Kt) / ea;
+      zz[2] = (r[0].Kt + r[1].Kt) * ea;
+      zz[3] = (r[0].Tt / e1 + r[1].Tt / e2) * ea;
+    } break;
+    case SPK_CNF:
+      zz[0] = r[0].Kn - r[1].Kn;
+      zz[1] = r[0].Tn / e1 - r[1].Tn / e2;
+      zz[2] = e2 * r[1].Tn - e1 * r[0].Tn;
+      zz[3] = zd{0, 0};
+      break;
+    default: {  // MNMF
+      const zd ep1 = zcx(A.mat.eps1), ep2 = zcx(A.mat.eps2);
+      const zd my1 = {A.mat.my1, 0}, my2 = {A.mat.my2, 0};
+      const zd b1 = ep1 / (ep1 + ep2), b2 = ep2 / (ep1 + ep2);
+      const zd c1 = my1 / (my1 + my2), c2 = my2 / (my1 + my2);
+      zz[0] = b1 * r[0].Kn - b2 * r[1].Kn;
+      zz[1] = b1 / e1 * r[0].Tn - b2 / e2 * r[1].Tn;
+      zz[2] = e2 * c2 * r[1].Tn - e1 * c1 * r[0].Tn;
+      zz[3] = c1 * r[0].Kn - c2 * r[1].Kn;
+    } break;
+  }
+}
+
+// lib_sie_tri_get_impedance_edges block assembly (:463-644)
+__device__ inline void accumulate_D(const NearArgs &A, zd *D, const zd *zz) {
+  switch (A.form) {
+    case SPK_PMCHWT:
+      D[0] = D[0] + zz[0];
+      D[1] = D[1] + zz[1];
+      D[2] = -D[1];
+      D[3] = D[3] + zz[2];
+      break;
+    case SPK_MCTF: {
+      const zd e12 = zcx(A.mat.eta1) * zcx(A.mat.eta2);
+      D[0] = D[0] + zz[0];
+      D[1] = D[1] + zz[1];
+      D[2] = -D[1] * e12;
+      D[3] = D[3] + zz[2] * e12;
+    } break;
+    case SPK_PMCHWT_S:
+      D[0] = D[0] + zz[0];
+      D[1] = D[1] + zz[1];
+      D[2] = -D[2] + zz[1];
+      D[3] = D[3] + zz[2];
+      break;
+    case SPK_JMCFIE:
+      D[0] = D[0] + zz[0];
+      D[1] = D[1] + zz[1];
+      D[2] = D[2] + zz[2];
+      D[3] = D[0];
+      break;
+    default:
+      D[0] = D[0] + zz[0];
+      D[1] = D[1] + zz[1];
+      D[2] = D[2] + zz[2];
+      D[3] = D[3] + zz[3];
+      break;
+  }
+}
+
+__device__ inline void finish_D(const NearArgs &A, zd *D) {
+  if (A.form == SPK_PMCHWT_S) {
+    const zd e1 = zcx(A.mat.eta1);
+    D[1] = D[1] * e1;
+    D[2] = D[2] * e1;
+    D[3] = D[3] * (e1 * e1);
+  }
+}
+
+// block index -> (row edge m, column edge n)
+__device__ inline void decode_block(const NearArgs &A, int64_t g, int &m, int &n) {
+  int lo = 0, hi = A.nbl - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (A.near_off[mid] <= g)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const int b = lo;
+  const int cols = A.e2_off[b + 1] - A.e2_off[b];
+  const int64_t rel = g - A.near_off[b];
+  const int r = (int)(rel / cols), c = (int)(rel - (int64_t)r * cols);
+  m = A.perm[A.box_first[b] + r];
+  n = A.e2_list[A.e2_off[b] + c];
+}
+
+__device__ inline bool edge_adj(const ElemRec &a, const ElemRec &b) {
+  int cnt = 0;
+  for (int j = 0; j < 3; ++j)
+    for (int i = 0; i < 3; ++i) cnt += (a.node[j] == b.node[i]);
+  return cnt == 2;
+}
+
+__global__ void k_near_classify(NearArgs A, int64_t nnz, uint8_t *cls, int *cnt_a, int *cnt_b) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nnz) return;
+  int m, n;
+  decode_block(A, g, m, n);
+  const double rr = 1.5 * A.rref[m];
+  unsigned bits = 0;
+  for (int p = 0; p < 4; ++p) {
+    const int t = p >> 1, s = p & 1;
+    const FnRec &fm = A.fn[2 * m + t];
+    const FnRec &fq = A.fn[2 * n + s];
+    const double d[3] = {fm.mid[0] - fq.mid[0], fm.mid[1] - fq.mid[1], fm.mid[2] - fq.mid[2]};
+    const double r_ob = len3(d);
+    if (!(r_ob > rr)) {
+      bits |= 1u << p;
+      const ElemRec &ea = A.er[2 * m + t], &eb = A.er[2 * n + s];
+      if (A.edge_swap && ea.elem != eb.elem && edge_adj(ea, eb)) bits |= 16u << p;
+    }
+  }
+  cls[g] = (uint8_t)bits;
+  cnt_a[g] = __popc(bits & 15u);
+  cnt_b[g] = __popc(bits >> 4);
+}
+
+__global__ void k_near_emit(NearArgs A, int64_t nnz, const uint8_t *cls, const int *base_a,
+                            const int *base_b, NearTask *ta, NearTask *tb) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nnz) return;
+  const unsigned bits = cls[g];
+  if (!bits) return;
+  int ra = base_a[g], rb = base_b[g];
+  for (int p = 0; p < 4; ++p) {
+    if (bits & (1u << p)) ta[ra++] = NearTask{g, p, 0};
+    if (bits & (16u << p)) tb[rb++] = NearTask{g, p, 0};
+  }
+}
+
+__global__ void __launch_bounds__(128) k_near_edge(NearArgs A, int ntask, const NearTask *tb, zd *kt) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntask) return;
+  int m, n;
+  decode_block(A, tb[t].g, m, n);
+  const int p = tb[t].p;
+  const FnRec fm = A.fn[2 * m + (p >> 1)];
+  const FnRec fq = A.fn[2 * n + (p & 1)];
+  zd k2[2];
+  edge_adjacent_kt(fm, fq, A, k2);
+  kt[2 * t] = k2[0];
+  kt[2 * t + 1] = k2[1];
+}
+
+__global__ void __launch_bounds__(128) k_near_singular(NearArgs A, int ntask, const NearTask *ta,
+                                                       const uint8_t *cls, const int *base_b,
+                                                       const zd *kt, zd *zz) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntask) return;
+  const int64_t g = ta[t].g;
+  const int p = ta[t].p;
+  int m, n;
+  decode_block(A, g, m, n);
+  const FnRec fm = A.fn[2 * m + (p >> 1)];
+  const FnRec fq = A.fn[2 * n + (p & 1)];
+  const bool iqs = A.er[2 * m + (p >> 1)].elem == A.er[2 * n + (p & 1)].elem;
+  Rel rv[2];
+  singular_pair(fm, fq, iqs, A, rv);
+  const unsigned bits = cls[g];
+  if (bits & (16u << p)) {
+    const int rank = __popc((bits >> 4) & ((1u << p) - 1u));
+    const int idx = base_b[g] + rank;
+    rv[0].Kt = kt[2 * idx];
+    rv[1].Kt = kt[2 * idx + 1];
+  }
+  zd z[4];
+  assemble_zz(A, rv, z);
+  for (int k = 0; k < 4; ++k) zz[4 * (int64_t)t + k] = z[k];
+}
+
+template <bool NEED_N>
+__global__ void __launch_bounds__(256) k_near_regular(NearArgs A, int64_t nnz, const uint8_t *cls,
+                                                      const int *base_a, const zd *zz, zd *blocks) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nnz) return;
+  int m, n;
+  decode_block(A, g, m, n);
+  const unsigned bits = cls[g];
+  zd D[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+  for (int p = 0; p < 4; ++p) {
+    zd z[4];
+    if (bits & (1u << p)) {
+      const int rank = __popc(bits & ((1u << p) - 1u));
+      const int64_t idx = base_a[g] + rank;
+      for (int k = 0; k < 4; ++k) z[k] = zz[4 * idx + k];
+    } else {
+      const FnRec &fm = A.fn[2 * m + (p >> 1)];
+      const FnRec &fq = A.fn[2 * n + (p & 1)];
+      Rel rv[2];
+      normal_pair<NEED_N>(fm, fq, A, rv);
+      assemble_zz(A, rv, z);
+    }
+    accumulate_D(A, D, z);
+  }
+  finish_D(A, D);
+  for (int k = 0; k < 4; ++k) blocks[4 * g + k] = D[k];
+}
+
+static inline unsigned nblk(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
+static inline unsigned gcap(int64_t n) { unsigned b = nblk(n, 256); return b < 2048u ? (b ? b : 1u) : 2048u; }
+
+// One thread per requested (m, n): all four triangle pairs with the
+// reference's branch rules inline (parity probe for arbitrary pairs).
+__global__ void __launch_bounds__(64) k_near_pairs(NearArgs A, int np_, const int *pm,
+                                                   const int *pn, zd *out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= np_) return;
+  const int m = pm[t], n = pn[t];
+  const double rr = 1.5 * A.rref[m];
+  zd D[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+  for (int p = 0; p < 4; ++p) {
+    const FnRec fm = A.fn[2 * m + (p >> 1)];
+    const FnRec fq = A.fn[2 * n + (p & 1)];
+    const ElemRec ea = A.er[2 * m + (p >> 1)], eb = A.er[2 * n + (p & 1)];
+    const double d[3] = {fm.mid[0] - fq.mid[0], fm.mid[1] - fq.mid[1], fm.mid[2] - fq.mid[2]};
+    Rel rv[2];
+    if (!(len3(d) > rr)) {
+      const bool iqs = ea.elem == eb.elem;
+      singular_pair(fm, fq, iqs, A, rv);
+      if (A.edge_swap && !iqs && edge_adj(ea, eb)) {
+        zd kt[2];
+        edge_adjacent_kt(fm, fq, A, kt);
+        rv[0].Kt = kt[0];
+        rv[1].Kt = kt[1];
+      }
+    } else {
+      normal_pair<true>(fm, fq, A, rv);
+    }
+    zd z[4];
+    assemble_zz(A, rv, z);
+    accumulate_D(A, D, z);
+  }
+  finish_D(A, D);
+  for (int k = 0; k < 4; ++k) out[4 * t + k] = D[k];
+}
+
+void launch_near_pairs(const NearArgs &a, int npairs, const int *m, const int *n, zd *out,
+                       hipStream_t st) {
+  if (npairs <= 0) return;
+  hipLaunchKernelGGL(k_near_pairs, dim3(nblk(npairs, 64)), dim3(64), 0, st, a, npairs, m, n, out);
+}
+
+void launch_near_classify(const NearArgs &a, int64_t nnz, uint8_t *cls, int *cnt_a, int *cnt_b,
+                          hipStream_t st) {
+  if (nnz <= 0) return;
+  hipLaunchKernelGGL(k_near_classify, dim3(nblk(nnz, 256)), dim3(256), 0, st, a, nnz, cls, cnt_a,
+                     cnt_b);
+}
+void launch_near_emit(const NearArgs &a, int64_t nnz, const uint8_t *cls, const int *base_a,
+                      const int *base_b, NearTask *ta, NearTask *tb, hipStream_t st) {
+  if (nnz <= 0) return;
+  hipLaunchKernelGGL(k_near_emit, dim3(nblk(nnz, 256)), dim3(256), 0, st, a, nnz, cls, base_a,
+                     base_b, ta, tb);
+}
+void launch_near_edge(const NearArgs &a, int ntask, const NearTask *tb, zd *kt, hipStream_t st) {
+  if (ntask <= 0) return;
+  hipLaunchKernelGGL(k_near_edge, dim3(nblk(ntask, 128)), dim3(128), 0, st, a, ntask, tb, kt);
+}
+void launch_near_singular(const NearArgs &a, int ntask, const NearTask *ta, const uint8_t *cls,
+                          const int *base_b, const zd *kt, zd *zz, hipStream_t st) {
+  if (ntask <= 0) return;
+  hipLaunchKernelGGL(k_near_singular, dim3(nblk(ntask, 128)), dim3(128), 0, st, a, ntask, ta, cls,
+                     base_b, kt, zz);
+}
+void launch_near_regular(const NearArgs &a, int64_t nnz, const uint8_t *cls, const int *base_a,
+                         const zd *zz, zd *blocks, hipStream_t st) {
+  if (nnz <= 0) return;
+  const bool need_n = (a.form == SPK_JMCFIE || a.form == SPK_CNF || a.form == SPK_MNMF);
+  if (need_n)
+    hipLaunchKernelGGL(k_near_regular<true>, dim3(nblk(nnz, 256)), dim3(256), 0, st, a, nnz, cls,
+                       base_a, zz, blocks);
+  else
+    hipLaunchKernelGGL(k_near_regular<false>, dim3(nblk(nnz, 256)), dim3(256), 0, st, a, nnz, cls,
+                       base_a, zz, blocks);
+}
+
+void exclusive_scan_int(const int *in, int *out, int64_t n, void **tmp, size_t *tmp_bytes,
+                        hipStream_t st) {
+  size_t need = 0;
+  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, need, in, out, (int)n, st);
+  if (need > *tmp_bytes) {
+    if (*tmp) (void)hipFree(*tmp);
+    (void)hipMalloc(tmp, need);
+    *tmp_bytes = need;
+  }
+  (void)hipcub::DeviceScan::ExclusiveSum(*tmp, need, in, out, (int)n, st);
+}
+
+// ---------------------------------------------------------------------------
+// MLFMM kernels
+// ---------------------------------------------------------------------------
+// PreCalculate_RT / RT_function_SCSArr_New (src/lib_sie_tri_mlfmm.f90:503-583):
+// one thread per (position, sample).
+__global__ void k_patterns(int N, int S, int ngp, const FnRec *fn, const int *perm,
+                           const int *leaf_box, const double *leaf_centre, const double *khat,
+                           const double *tth, const double *tph, const double *qa,
+                           const double *qb, const double *qw, zd k1, zd k2, zd *rt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p = blockIdx.y;
+  if (i >= S || p >= N) return;
+  const int m = perm[p];
+  const double *Rc = &leaf_centre[3 * leaf_box[m]];
+  const double kh[3] = {khat[3 * i], khat[3 * i + 1], khat[3 * i + 2]};
+  zd o[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+  for (int ll = 0; ll < 2; ++ll) {
+    const FnRec &f = fn[2 * m + ll];
+    const double dfm = f.sign * f.length;
+    zd sa[3] = {{0, 0}, {0, 0}, {0, 0}}, sb[3] = {{0, 0}, {0, 0}, {0, 0}};
+    for (int j = 0; j < ngp; ++j) {
+      double Ro[3], fm[3], Roc[3];
+      simplex(qa[j], qb[j], f.c, Ro);
+      for (int k = 0; k < 3; ++k) {
+        fm[k] = 0.5 * (Ro[k] - f.c[6 + k]) * dfm;
+        Roc[k] = Ro[k] - Rc[k];
+      }
+      const double tmp = dot3(kh, Roc);
+      const zd ff1 = expmikr(k1, tmp) * qw[j];
+      const zd ff2 = expmikr(k2, tmp) * qw[j];
+      for (int k = 0; k < 3; ++k) {
+        sa[k] = sa[k] + fm[k] * ff1;
+        sb[k] = sb[k] + fm[k] * ff2;
+      }
+    }
+    const double tt3[3] = {tth[3 * i], tth[3 * i + 1], tth[3 * i + 2]};
+    const double tp3[3] = {tph[3 * i], tph[3 * i + 1], tph[3 * i + 2]};
+    o[0] = o[0] + dot_rc(tt3, sa);
+    o[1] = o[1] + dot_rc(tp3, sa);
+    o[2] = o[2] + dot_rc(tt3, sb);
+    o[3] = o[3] + dot_rc(tp3, sb);
+  }
+  zd *out = rt + (size_t)p * 4 * S;
+  for (int q = 0; q < 4; ++q) out[(size_t)q * S + i] = o[q];
+}
+
+void launch_patterns(int N, int S, int ngp, const FnRec *fn, const int *perm, const int *leaf_box,
+                     const double *leaf_centre, const double *khat, const double *tth,
+                     const double *tph, const double *qa, const double *qb, const double *qw,
+                     zd k1, zd k2, zd *rt, hipStream_t st) {
+  if (N <= 0) return;
+  hipLaunchKernelGGL(k_patterns, dim3(nblk(S, 256), N), dim3(256), 0, st, N, S, ngp, fn, perm,
+                     leaf_box, leaf_centre, khat, tth, tph, qa, qb, qw, k1, k2, rt);
+}
+
+// lib_sie_ml_fmm_get_c (src/lib_sie_tri_mlfmm_interface.f90:395-448):
+// C[b][comp*4+cur*2+k][s] = sum_{edges of b} conj(RT[k][comp][s]) * x_cur
+__global__ void k_agg(int S, int N, const int *box_first, const int *box_count, const int *perm,
+                      const zd *rt, const zd *x, zd *C) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y, q = blockIdx.z;  // q = k*2 + comp
+  if (s >= S) return;
+  const int k = q >> 1, comp = q & 1;
+  zd ae = {0, 0}, ah = {0, 0};
+  const int p0 = box_first[b], cnt = box_count[b];
+  for (int e = 0; e < cnt; ++e) {
+    const int p = p0 + e;
+    const int m = perm[p];
+    const zd f = conj(rt[((size_t)p * 4 + q) * S + s]);
+    const zd xe = x[m], xh = x[m + N];
+    ae = ae + f * xe;
+    ah = ah + f * xh;
+  }
+  zd *Cb = C + (size_t)b * 8 * S;
+  Cb[(size_t)(comp * 4 + 0 * 2 + k) * S + s] = ae;
+  Cb[(size_t)(comp * 4 + 1 * 2 + k) * S + s] = ah;
+}
+
+void launch_agg(int nbl, int S, int N, const int *box_first, const int *box_count,
+                const int *perm, const zd *rt, const zd *x, zd *C, hipStream_t st) {
+  if (nbl <= 0) return;
+  hipLaunchKernelGGL(k_agg, dim3(nblk(S, 256), nbl, 4), dim3(256), 0, st, S, N, box_first,
+                     box_count, perm, rt, x, C);
+}
+
+// translation_SS (:466-528): P[it][s] = sum_o C_child[it][s] * up[o][k][s]
+__global__ void k_ss(int S, const int *child, const zd *Cc, const zd *up, zd *P) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  if (s >= S) return;
+  zd acc[8];
+  for (int it = 0; it < 8; ++it) acc[it] = zd{0, 0};
+  for (int o = 0; o < 8; ++o) {
+    const int c = child[8 * b + o];
+    if (c < 0) continue;
+    const zd u0 = up[((size_t)o * 2 + 0) * S + s], u1 = up[((size_t)o * 2 + 1) * S + s];
+    const zd *src = Cc + (size_t)c * 8 * S;
+#pragma unroll
+    for (int it = 0; it < 8; ++it) acc[it] = acc[it] + src[(size_t)it * S + s] * ((it & 1) ? u1 : u0);
+  }
+  zd *dst = P + (size_t)b * 8 * S;
+  for (int it = 0; it < 8; ++it) dst[(size_t)it * S + s] = acc[it];
+}
+
+void launch_ss(int nb, int S, const int *child, const zd *Cc, const zd *up, zd *P, hipStream_t st) {
+  if (nb <= 0) return;
+  hipLaunchKernelGGL(k_ss, dim3(nblk(S, 256), nb), dim3(256), 0, st, S, child, Cc, up, P);
+}
+
+// translation_SR (:547-586): Dt[a][it][s] = sum_{b in I4(a)} B[b][it][s]*TLT[slot][k][s]
+__global__ void k_m2l(int S, const int *i4_off, const int *i4_box, const int *i4_slot,
+                      const zd *B, const zd *tlt, zd *Dt) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int a = blockIdx.y;
+  if (s >= S) return;
+  zd acc[8];
+  for (int it = 0; it < 8; ++it) acc[it] = zd{0, 0};
+  const int q0 = i4_off[a], q1 = i4_off[a + 1];
+  for (int q = q0; q < q1; ++q) {
+    const int b = i4_box[q], slot = i4_slot[q];
+    const zd t0 = tlt[((size_t)slot * 2 + 0) * S + s], t1 = tlt[((size_t)slot * 2 + 1) * S + s];
+    const zd *src = B + (size_t)b * 8 * S;
+#pragma unroll
+    for (int it = 0; it < 8; ++it) acc[it] = acc[it] + src[(size_t)it * S + s] * ((it & 1) ? t1 : t0);
+  }
+  zd *dst = Dt + (size_t)a * 8 * S;
+  for (int it = 0; it < 8; ++it) dst[(size_t)it * S + s] = acc[it];
+}
+
+void launch_m2l(int nb, int S, const int *i4_off, const int *i4_box, const int *i4_slot,
+                const zd *B, const zd *tlt, zd *Dt, hipStream_t st) {
+  if (nb <= 0) return;
+  hipLaunchKernelGGL(k_m2l, dim3(nblk(S, 128), nb), dim3(128), 0, st, S, i4_off, i4_box, i4_slot,
+                     B, tlt, Dt);
+}
+
+// translation_RR (:600-655): Dc[it][s] += Dp[it][s]*down[o][k][s]
+__global__ void k_rr(int S, const int *parent, const int *octant, const zd *Dp, const zd *down,
+                     zd *Dc) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  if (s >= S) return;
+  const int p = parent[b], o = octant[b];
+  const zd d0 = down[((size_t)o * 2 + 0) * S + s], d1 = down[((size_t)o * 2 + 1) * S + s];
+  const zd *src = Dp + (size_t)p * 8 * S;
+  zd *dst = Dc + (size_t)b * 8 * S;
+#pragma unroll
+  for (int it = 0; it < 8; ++it)
+    dst[(size_t)it * S + s] = dst[(size_t)it * S + s] + src[(size_t)it * S + s] * ((it & 1) ? d1 : d0);
+}
+
+void launch_rr(int nb, int S, const int *parent, const int *octant, const zd *Dp, const zd *down,
+               zd *Dc, hipStream_t st) {
+  if (nb <= 0) return;
+  hipLaunchKernelGGL(k_rr, dim3(nblk(S, 256), nb), dim3(256), 0, st, S, parent, octant, Dp, down,
+                     Dc);
+}
+
+// MLFMM_tri_summation_New (src/lib_sie_tri_mlfmm.f90:416-497) folded per box:
+//   y_e = sum_{k,s} fa*W1a + fb*W1b,  y_h = sum fa*W2a + fb*W2b
+// W1a = Dbh*kd+Dae*my, W1b = -Dah*kd+Dbe*my, W2a = -Dbe*kd+Dah*eps,
+// W2b = Dae*kd+Dbh*eps.  W layout [box][k*4 + {1a,1b,2a,2b}][S].
+__global__ void k_fold(int S, const zd *D, zd kd1, zd kd2, zd my1, zd my2, zd ep1, zd ep2, zd *W) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  if (s >= S) return;
+  const zd *Db = D + (size_t)b * 8 * S;
+  zd *Wb = W + (size_t)b * 8 * S;
+  for (int k = 0; k < 2; ++k) {
+    const zd kd = k ? kd2 : kd1, my = k ? my2 : my1, ep = k ? ep2 : ep1;
+    const zd Dae = Db[(size_t)(0 + k) * S + s], Dah = Db[(size_t)(2 + k) * S + s];
+    const zd Dbe = Db[(size_t)(4 + k) * S + s], Dbh = Db[(size_t)(6 + k) * S + s];
+    Wb[(size_t)(k * 4 + 0) * S + s] = Dbh * kd + Dae * my;
+    Wb[(size_t)(k * 4 + 1) * S + s] = -(Dah * kd) + Dbe * my;
+    Wb[(size_t)(k * 4 + 2) * S + s] = -(Dbe * kd) + Dah * ep;
+    Wb[(size_t)(k * 4 + 3) * S + s] = Dae * kd + Dbh * ep;
+  }
+}
+
+void launch_fold(int nbl, int S, const zd *D, zd kd1, zd kd2, zd my1, zd my2, zd ep1, zd ep2,
+                 zd *W, hipStream_t st) {
+  if (nbl <= 0) return;
+  hipLaunchKernelGGL(k_fold, dim3(nblk(S, 256), nbl), dim3(256), 0, st, S, D, kd1, kd2, my1, my2,
+                     ep1, ep2, W);
+}
+
+__device__ inline zd wave_sum(zd v) {
+  for (int off = 32; off > 0; off >>= 1) {
+    v.x += __shfl_down(v.x, off, 64);
+    v.y += __shfl_down(v.y, off, 64);
+  }
+  return v;
+}
+
+// one wavefront per position: y[m] = sc1*y_e, y[m+N] = sc2*y_h (far part)
+__global__ void __launch_bounds__(256) k_disagg(int N, int S, const int *perm,
+                                                const int *leaf_box_of_pos, const zd *rt,
+                                                const zd *W, zd sc1, zd sc2, zd *y) {
+  const int lane = threadIdx.x & 63;
+  const int p = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (p >= N) return;
+  const int b = leaf_box_of_pos[p];
+  const zd *f = rt + (size_t)p * 4 * S;
+  const zd *Wb = W + (size_t)b * 8 * S;
+  zd s1 = {0, 0}, s2 = {0, 0};
+  for (int k = 0; k < 2; ++k) {
+    const zd *fa = f + (size_t)(k * 2 + 0) * S, *fb = f + (size_t)(k * 2 + 1) * S;
+    const zd *w1a = Wb + (size_t)(k * 4 + 0) * S, *w1b = Wb + (size_t)(k * 4 + 1) * S;
+    const zd *w2a = Wb + (size_t)(k * 4 + 2) * S, *w2b = Wb + (size_t)(k * 4 + 3) * S;
+    for (int s = lane; s < S; s += 64) {
+      const zd a = fa[s], bb = fb[s];
+      s1 = s1 + a * w1a[s] + bb * w1b[s];
+      s2 = s2 + a * w2a[s] + bb * w2b[s];
+    }
+  }
+  s1 = wave_sum(s1);
+  s2 = wave_sum(s2);
+  if (lane == 0) {
+    const int m = perm[p];
+    y[m] = s1 * sc1;
+    y[m + N] = s2 * sc2;
+  }
+}
+
+void launch_disagg(int N, int S, const int *perm, const int *leaf_box_of_pos, const zd *rt,
+                   const zd *W, zd sc1, zd sc2, zd *y, hipStream_t st) {
+  if (N <= 0) return;
+  hipLaunchKernelGGL(k_disagg, dim3(nblk(N, 4)), dim3(256), 0, st, N, S, perm, leaf_box_of_pos,
+                     rt, W, sc1, sc2, y);
+}
+
+// Near SpMV over the box-dense block matrices.  Workgroup = (leaf box,
+// chunk of kRowsPerWG rows); x of the box's E2 columns staged in LDS in
+// chunks of kColChunk; one wavefront per row, lanes over columns.
+constexpr int kRowsPerWG = kSpmvRowsPerWG;
+constexpr int kColChunk = 1024;
+
+__global__ void __launch_bounds__(256) k_near_spmv(int N, const int *row_box, const int *row_r0,
+                                                   const int *box_first, const int *box_count,
+                                                   const int *e2_off, const int *e2_list,
+                                                   const int64_t *near_off, const int *perm,
+                                                   const zd *blocks, const zd *x, const zd *yadd,
+                                                   zd *y) {
+  __shared__ zd xs[2 * kColChunk];
+  const int b = row_box[blockIdx.x], r0 = row_r0[blockIdx.x];
+  const int rows = min(kRowsPerWG, box_count[b] - r0);
+  const int c0 = e2_off[b], cols = e2_off[b + 1] - c0;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  zd ae[kRowsPerWG / 4], ah[kRowsPerWG / 4];
+  for (int q = 0; q < kRowsPerWG / 4; ++q) ae[q] = ah[q] = zd{0, 0};
+  for (int cc = 0; cc < cols; cc += kColChunk) {
+    const int nc = min(kColChunk, cols - cc);
+    __syncthreads();
+    for (int c = threadIdx.x; c < nc; c += blockDim.x) {
+      const int n = e2_list[c0 + cc + c];
+      xs[2 * c] = x[n];
+      xs[2 * c + 1] = x[n + N];
+    }
+    __syncthreads();
+    for (int q = 0; q < kRowsPerWG / 4; ++q) {
+      const int r = r0 + wv + 4 * q;
+      if (r >= r0 + rows) break;
+      const zd *Z = blocks + 4 * (near_off[b] + (int64_t)(r - 0) * cols + cc);
+      for (int c = lane; c < nc; c += 64) {
+        const zd z0 = Z[4 * c], z1 = Z[4 * c + 1], z2 = Z[4 * c + 2], z3 = Z[4 * c + 3];
+        const zd xe = xs[2 * c], xh = xs[2 * c + 1];
+        ae[q] = ae[q] + z0 * xe + z1 * xh;
+        ah[q] = ah[q] + z2 * xe + z3 * xh;
+      }
+    }
+  }
+  for (int q = 0; q < kRowsPerWG / 4; ++q) {
+    const int r = r0 + wv + 4 * q;
+    const zd se = wave_sum(ae[q]), sh = wave_sum(ah[q]);
+    if (r < r0 + rows && lane == 0) {
+      const int m = perm[box_first[b] + r];
+      zd oe = se, oh = sh;
+      if (yadd) {
+        oe = oe + yadd[m];
+        oh = oh + yadd[m + N];
+      }
+      y[m] = oe;
+      y[m + N] = oh;
+    }
+  }
+}
+
+void launch_near_spmv_rows(int nwg, int N, const int *row_box, const int *row_r0,
+                           const int *box_first, const int *box_count, const int *e2_off,
+                           const int *e2_list, const int64_t *near_off, const int *perm,
+                           const zd *blocks, const zd *x, const zd *yadd, zd *y, hipStream_t st) {
+  if (nwg <= 0) return;
+  hipLaunchKernelGGL(k_near_spmv, dim3(nwg), dim3(256), 0, st, N, row_box, row_r0, box_first,
+                     box_count, e2_off, e2_list, near_off, perm, blocks, x, yadd, y);
+}
+
+// ---------------------------------------------------------------------------
+// BLAS-1 with deterministic two-stage reductions (fixed grid of kRedBlocks)
+// ---------------------------------------------------------------------------
+constexpr int kRedBlocks = 256;
+int reduce_blocks(int64_t) { return kRedBlocks; }
+
+__device__ inline zd block_sum(zd v) {
+  __shared__ zd red[4];
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) red[wv] = v;
+  __syncthreads();
+  zd t = {0, 0};
+  if (threadIdx.x == 0) t = red[0] + red[1] + red[2] + red[3];
+  __syncthreads();
+  return t;
+}
+
+__global__ void k_dot_partial(int64_t n, const zd *x, const zd *y, zd *partial) {
+  zd acc = {0, 0};
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    acc = acc + conj(x[i]) * y[i];
+  acc = block_sum(acc);
+  if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+}
+__global__ void k_reduce_final(int nparts, const zd *partial, zd *out) {
+  zd acc = {0, 0};
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x) acc = acc + partial[i];
+  acc = block_sum(acc);
+  if (threadIdx.x == 0) *out = acc;
+}
+void launch_zdotc(int64_t n, const zd *x, const zd *y, zd *partial, zd *out, hipStream_t st) {
+  hipLaunchKernelGGL(k_dot_partial, dim3(kRedBlocks), dim3(256), 0, st, n, x, y, partial);
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, st, kRedBlocks, partial, out);
+}
+
+__global__ void k_zaxpy(int64_t n, zd a, const zd *x, zd *y) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = y[i] + a * x[i];
+}
+void launch_zaxpy(int64_t n, zd a, const zd *x, zd *y, hipStream_t st) {
+  hipLaunchKernelGGL(k_zaxpy, dim3(gcap(n)), dim3(256), 0, st, n, a, x, y);
+}
+__global__ void k_zaxpy_dev(int64_t n, const zd *a, double sgn, const zd *x, zd *y) {
+  const zd c = (*a) * sgn;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = y[i] + c * x[i];
+}
+void launch_zaxpy_dev(int64_t n, const zd *a, double sgn, const zd *x, zd *y, hipStream_t st) {
+  hipLaunchKernelGGL(k_zaxpy_dev, dim3(gcap(n)), dim3(256), 0, st, n, a,
+                     sgn, x, y);
+}
+__global__ void k_zscal_copy(int64_t n, zd a, const zd *x, zd *y) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = a * x[i];
+}
+void launch_zscal_copy(int64_t n, zd a, const zd *x, zd *y, hipStream_t st) {
+  hipLaunchKernelGGL(k_zscal_copy, dim3(gcap(n)), dim3(256), 0, st, n, a,
+                     x, y);
+}
+__global__ void k_zsub(int64_t n, const zd *b, const zd *x, zd *y) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = b[i] - x[i];
+}
+void launch_zsub(int64_t n, const zd *b, const zd *x, zd *y, hipStream_t st) {
+  hipLaunchKernelGGL(k_zsub, dim3(gcap(n)), dim3(256), 0, st, n, b, x, y);
+}
+__global__ void k_zadd(int64_t n, const zd *a, zd *y) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = y[i] + a[i];
+}
+void launch_zadd(int64_t n, const zd *a, zd *y, hipStream_t st) {
+  hipLaunchKernelGGL(k_zadd, dim3(gcap(n)), dim3(256), 0, st, n, a, y);
+}
+
+// MGS step j: w -= h_{j-1} v_{j-1} (if vprev), then partial <v_j, w>.
+__global__ void k_mgs_partial(int64_t n, const zd *vprev, const zd *hprev, const zd *vj, zd *w,
+                              zd *partial) {
+  const zd c = vprev ? -(*hprev) : zd{0, 0};
+  zd acc = {0, 0};
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    zd wi = w[i];
+    if (vprev) {
+      wi = wi + c * vprev[i];
+      w[i] = wi;
+    }
+    acc = acc + conj(vj[i]) * wi;
+  }
+  acc = block_sum(acc);
+  if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+}
+void launch_mgs_step(int64_t n, const zd *vprev, const zd *hprev, const zd *vj, zd *w,
+                     zd *partial, zd *hout, hipStream_t st) {
+  hipLaunchKernelGGL(k_mgs_partial, dim3(kRedBlocks), dim3(256), 0, st, n, vprev, hprev, vj, w,
+                     partial);
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, st, kRedBlocks, partial, hout);
+}
+
+// h[j] = <V_j, w> for j < k: partial[blk*k + j], then reduce per j.
+__global__ void k_gemv_c_partial(int64_t n, int k, const zd *V, int64_t ldv, const zd *w,
+                                 zd *partial) {
+  const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+  const int64_t i0 = blockIdx.x * chunk, i1 = min<int64_t>(n, i0 + chunk);
+  for (int j = 0; j < k; ++j) {
+    zd acc = {0, 0};
+    const zd *vj = V + (size_t)j * ldv;
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) acc = acc + conj(vj[i]) * w[i];
+    acc = block_sum(acc);
+    if (threadIdx.x == 0) partial[(size_t)blockIdx.x * k + j] = acc;
+  }
+}
+__global__ void k_gemv_c_final(int nparts, int k, const zd *partial, zd *out) {
+  const int j = blockIdx.x;
+  zd acc = {0, 0};
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x) acc = acc + partial[(size_t)i * k + j];
+  acc = block_sum(acc);
+  if (threadIdx.x == 0) out[j] = acc;
+}
+void launch_gemv_c(int64_t n, int k, const zd *V, int64_t ldv, const zd *w, zd *partial, zd *out,
+                   hipStream_t st) {
+  if (k <= 0) return;
+  hipLaunchKernelGGL(k_gemv_c_partial, dim3(kRedBlocks), dim3(256), 0, st, n, k, V, ldv, w,
+                     partial);
+  hipLaunchKernelGGL(k_gemv_c_final, dim3(k), dim3(256), 0, st, kRedBlocks, k, partial, out);
+}
+__global__ void k_gemv_n_sub(int64_t n, int k, const zd *V, int64_t ldv, const zd *h, zd *w) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    zd acc = {0, 0};
+    for (int j = 0; j < k; ++j) acc = acc + V[(size_t)j * ldv + i] * h[j];
+    w[i] = w[i] - acc;
+  }
+}
+void launch_gemv_n_sub(int64_t n, int k, const zd *V, int64_t ldv, const zd *h, zd *w,
+                       hipStream_t st) {
+  if (k <= 0) return;
+  hipLaunchKernelGGL(k_gemv_n_sub, dim3(gcap(n)), dim3(256), 0, st, n, k,
+                     V, ldv, h, w);
+}
+__global__ void k_gemv_n(int64_t n, int k, const zd *V, int64_t ldv, const zd *h, zd *out) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    zd acc = {0, 0};
+    for (int j = 0; j < k; ++j) acc = acc + V[(size_t)j * ldv + i] * h[j];
+    out[i] = acc;
+  }
+}
+void launch_gemv_n(int64_t n, int k, const zd *V, int64_t ldv, const zd *h, zd *out,
+                   hipStream_t st) {
+  hipLaunchKernelGGL(k_gemv_n, dim3(gcap(n)), dim3(256), 0, st, n, k, V,
+                     ldv, h, out);
+}
+
+}  // namespace spk

@@ -1,0 +1,212 @@
+"""Host-side mirror of SpeckleSim's triangle MLFMM-GMRES module API.
+
+``SieTriMlfmm`` follows ``lib_sie_tri_mlfmm_interface`` and
+``lib_sie_solver_gmres_interface`` of the reference
+(src/lib_sie_tri_mlfmm_interface.f90:18-26, src/lib_sie_solver_gmres_interface.f90:27-29):
+
+=====================================  ==========================================
+reference routine                      method here (GPU through libspeckle_gpu.so)
+=====================================  ==========================================
+lib_sie_ml_fmm_precalculation_tri      precalculation_tri(pp, tt)
+lib_sie_ml_fmm_calculate_vector_b_tri  calculate_vector_b_tri(vector_x)
+get_vector_b_tri (plane wave)          get_vector_b_tri(k_in, e_in)
+lib_sie_ml_fmm_get_init_vector_x_tri   get_init_vector_x_tri(x_initial)
+lib_math_solver_gmres_run              gmres_run(vector_b, ...)
+lib_sie_tri_get_impedance_edges        impedance_edges(m, n)   (0-based)
+=====================================  ==========================================
+
+Vectors are complex128 numpy arrays of length 2N in the reference's order
+[xe(1:N); xh(1:N)].  Mesh triangle indices are 1-based (``tt_*`` files).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _native as nat
+
+FORMULATIONS = {"PMCHWT": 1, "MCTF": 2, "ICTF": 3, "MCTF2": 4, "PMCHWT-S": 5,
+                "JMCFIE": 6, "CNF": 7, "MNMF": 8}
+ORTHOGONALIZATION = {"MGS": 0, "IMGS": 1, "CGS": 2, "ICGS": 3}
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def plane_wave(theta_in=0.0, phi_in=0.0, k_direction=1, polarization="TM"):
+    """Incident plane wave of set_illumination_parameter (main.f90:245-297):
+    k_in = (sin t cos p, sin t sin p, sign cos t) with sign = +1 for
+    calc_p(7)=1 and -1 for calc_p(7)=2, E_in from field_rotation_matrix
+    (main.f90:301-324) with TM -> (1,0), TE -> (0,1)."""
+    sign = 1.0 if k_direction == 1 else -1.0
+    k = np.array([np.sin(theta_in) * np.cos(phi_in), np.sin(theta_in) * np.sin(phi_in),
+                  sign * np.cos(theta_in)])
+    ein = (1.0, 0.0) if polarization.upper() == "TM" else (0.0, 1.0)
+    a, b, c, d = np.sin(theta_in), np.cos(theta_in), np.sin(phi_in), np.cos(phi_in)
+    rm = np.array([[d * d * b + c * c, c * d * (b - 1)], [c * d * (b - 1), c * c * b + d * d],
+                   [a * d, a * c]])
+    return k, rm @ np.asarray(ein)
+
+
+class SieTriMlfmm:
+    """One mesh on one GPU: near-field blocks, MLFMM and GMRES in HBM."""
+
+    def __init__(self, wavelength, eps_r2, eps_r1=1.0 + 0.0j, formulation="PMCHWT",
+                 normal_convention=1, ng=3, tree_s_opt=250, truncation_number=19,
+                 precision="fp64", device=0):
+        L = nat.lib()
+        f = FORMULATIONS[formulation] if isinstance(formulation, str) else int(formulation)
+        prec = {"fp64": 0, "mixed": 1}[precision]
+        eps_r1, eps_r2 = complex(eps_r1), complex(eps_r2)
+        self.params = nat.Params(wavelength, eps_r1.real, eps_r1.imag, eps_r2.real, eps_r2.imag,
+                                 f, normal_convention, ng, tree_s_opt, truncation_number, prec,
+                                 device)
+        h = C.c_void_p()
+        nat.check(L.spk_create(C.byref(self.params), C.byref(h)))
+        self._h = h
+        self.N = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            nat.lib().spk_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- lib_sie_ml_fmm_precalculation_tri ---------------------------------
+    def precalculation_tri(self, pp, tt):
+        pp = np.ascontiguousarray(pp, dtype=np.float64).reshape(-1, 3)
+        tt = np.ascontiguousarray(tt, dtype=np.int32).reshape(-1, 3)
+        nat.check(nat.lib().spk_setup_tri(self._h, pp.shape[0], _dp(pp), tt.shape[0],
+                                          tt.ctypes.data_as(C.POINTER(C.c_int))))
+        n = C.c_int64()
+        nat.check(nat.lib().spk_num_edges(self._h, C.byref(n)))
+        self.N = n.value
+        return self
+
+    # -- lib_sie_ml_fmm_calculate_vector_b_tri ------------------------------
+    def calculate_vector_b_tri(self, vector_x):
+        x = np.ascontiguousarray(vector_x, dtype=np.complex128)
+        if x.shape != (2 * self.N,):
+            raise ValueError(f"vector_x must have length 2N = {2 * self.N}")
+        y = np.empty_like(x)
+        nat.check(nat.lib().spk_matvec(self._h, _dp(x.view(np.float64)), _dp(y.view(np.float64))))
+        return y
+
+    def matvec_parts(self, vector_x):
+        x = np.ascontiguousarray(vector_x, dtype=np.complex128)
+        yn = np.empty_like(x)
+        yf = np.empty_like(x)
+        nat.check(nat.lib().spk_matvec_parts(self._h, _dp(x.view(np.float64)),
+                                             _dp(yn.view(np.float64)), _dp(yf.view(np.float64))))
+        return yn, yf
+
+    def calculate_vector_b_tri_device(self, d_x, d_y):
+        """Device-pointer matvec (e.g. torch complex128 tensors' data_ptr())."""
+        nat.check(nat.lib().spk_matvec_device(self._h, C.c_void_p(int(d_x)), C.c_void_p(int(d_y))))
+
+    # -- get_vector_b_tri / get_init_vector_x_tri --------------------------
+    def get_vector_b_tri(self, k_in, e_in):
+        k = np.ascontiguousarray(k_in, dtype=np.float64)
+        e = np.ascontiguousarray(e_in, dtype=np.float64)
+        b = np.empty(2 * self.N, np.complex128)
+        nat.check(nat.lib().spk_rhs_plane_wave(self._h, _dp(k), _dp(e), _dp(b.view(np.float64))))
+        return b
+
+    def get_init_vector_x_tri(self, x_initial=0.0):
+        return np.full(2 * self.N, complex(x_initial), dtype=np.complex128)
+
+    # -- lib_math_solver_gmres_run ------------------------------------------
+    def gmres_run(self, vector_b, x0=None, max_iterations=650, restart=0, tolerance=1e-3,
+                  orthogonalization="IMGS", use_initial_guess=True):
+        b = np.ascontiguousarray(vector_b, dtype=np.complex128)
+        x = self.get_init_vector_x_tri() if x0 is None else np.array(x0, dtype=np.complex128)
+        gp = nat.GmresParams(max_iterations, restart, tolerance,
+                             ORTHOGONALIZATION[orthogonalization], int(use_initial_guess), 0, 0.0)
+        info = nat.GmresInfo()
+        nat.check(nat.lib().spk_gmres(self._h, _dp(b.view(np.float64)), _dp(x.view(np.float64)),
+                                      C.byref(gp), C.byref(info)))
+        return x, dict(info=info.info, iters=info.iterations,
+                       bea=info.arnoldi_backward_error, be=info.backward_error,
+                       matvecs=info.matvecs, seconds=info.seconds, hist=self.gmres_history())
+
+    def gmres_run_device(self, d_b, d_x, max_iterations=650, restart=0, tolerance=1e-3,
+                         orthogonalization="IMGS", use_initial_guess=True):
+        gp = nat.GmresParams(max_iterations, restart, tolerance,
+                             ORTHOGONALIZATION[orthogonalization], int(use_initial_guess), 0, 0.0)
+        info = nat.GmresInfo()
+        nat.check(nat.lib().spk_gmres_device(self._h, C.c_void_p(int(d_b)), C.c_void_p(int(d_x)),
+                                             C.byref(gp), C.byref(info)))
+        return dict(info=info.info, iters=info.iterations, bea=info.arnoldi_backward_error,
+                    be=info.backward_error, matvecs=info.matvecs, seconds=info.seconds)
+
+    def gmres_history(self):
+        n = C.c_int()
+        nat.check(nat.lib().spk_gmres_history(self._h, None, 0, C.byref(n)))
+        h = np.zeros(max(n.value, 1))
+        nat.check(nat.lib().spk_gmres_history(self._h, _dp(h), n.value, C.byref(n)))
+        return h[:n.value]
+
+    # -- probes ---------------------------------------------------------------
+    def stats(self):
+        s = nat.Stats()
+        nat.check(nat.lib().spk_get_stats(self._h, C.byref(s)))
+        d = {k: getattr(s, k) for k, _ in nat.Stats._fields_}
+        d["boxes_per_level"] = list(s.boxes_per_level)[: s.l_max + 1]
+        d["i4_pairs_per_level"] = list(s.i4_pairs_per_level)[: s.l_max + 1]
+        return d
+
+    def rwg(self):
+        c = np.zeros((self.N, 2), np.int32)
+        e = np.zeros((self.N, 2), np.int32)
+        nat.check(nat.lib().spk_get_rwg(self._h, c.ctypes.data_as(C.POINTER(C.c_int)),
+                                        e.ctypes.data_as(C.POINTER(C.c_int))))
+        return c, e
+
+    def tree(self):
+        leaf = np.zeros(self.N, np.int64)
+        a, b = C.c_int(), C.c_int()
+        D = np.zeros(1)
+        xm = np.zeros(3)
+        nat.check(nat.lib().spk_get_tree(self._h, leaf.ctypes.data_as(C.POINTER(C.c_int64)),
+                                         C.byref(a), C.byref(b), _dp(D), _dp(xm)))
+        return dict(l_min=a.value, l_max=b.value, D=D[0], xmin=xm, leaf=leaf)
+
+    def i4(self, level, n):
+        cnt = C.c_int()
+        out = np.zeros(216, np.int64)
+        nat.check(nat.lib().spk_get_i4(self._h, level, n, out.ctypes.data_as(C.POINTER(C.c_int64)),
+                                       216, C.byref(cnt)))
+        return out[:cnt.value]
+
+    def near_csr(self, values=True):
+        st = self.stats()
+        nnz = st["near_blocks"]
+        ia = np.zeros(self.N + 1, np.int64)
+        ja = np.zeros(nnz, np.int32)
+        v = np.zeros(nnz * 8) if values else None
+        nat.check(nat.lib().spk_get_near_csr(self._h, ia.ctypes.data_as(C.POINTER(C.c_int64)),
+                                             ja.ctypes.data_as(C.POINTER(C.c_int32)),
+                                             _dp(v) if values else None))
+        vals = (v[0::2] + 1j * v[1::2]).reshape(nnz, 4) if values else None
+        return ia, ja, vals
+
+    def impedance_edges(self, m, n):
+        m = np.ascontiguousarray(np.atleast_1d(m), dtype=np.int32)
+        n = np.ascontiguousarray(np.atleast_1d(n), dtype=np.int32)
+        out = np.zeros(8 * m.size)
+        nat.check(nat.lib().spk_near_blocks(self._h, m.size, m.ctypes.data_as(C.POINTER(C.c_int)),
+                                            n.ctypes.data_as(C.POINTER(C.c_int)), _dp(out)))
+        return (out[0::2] + 1j * out[1::2]).reshape(-1, 2, 2)
+
+    def patterns(self, first, count):
+        S = self.stats()["samples"]
+        out = np.zeros(count * 8 * S)
+        nat.check(nat.lib().spk_get_patterns(self._h, first, count, _dp(out)))
+        return (out[0::2] + 1j * out[1::2]).reshape(count, 4, S)
