@@ -1,0 +1,178 @@
+"""Benchmark of the MI355X SIE solve loop.
+
+One step = one full solve of one rough-surface realisation on one GPU:
+precalculation (RWG topology, octree, radiation patterns, M2L tables,
+near-field impedance fill), plane-wave right-hand side and MLFMM-GMRES to
+the tolerance.  Workload: 10 lambda x 10 lambda Gaussian rough patch
+(RMS 0.1 lambda, correlation 0.5 lambda, seed 1), 100 x 100 cells split into
+20000 triangles (N = 29,800 RWG edges, 59,600 unknowns), lambda = 633 nm,
+dielectric interior n = 1.5, K = 19, s_opt = 250 (4 octree levels, 0.625
+lambda leaves), normal incidence, complex128.
+
+Prints one JSON line (see the repository's benchmark contract).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def _sync():
+    try:
+        import torch
+
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        pass
+
+
+def workload(args):
+    from specklesim_amd import meshes
+
+    lam = 633e-9
+    pp, tt = meshes.rough_surface(args.cells, args.cells, args.size, args.size, 0.1, 0.5,
+                                  args.seed, lam)
+    return lam, pp, tt
+
+
+def one_solve(args, lam, pp, tt):
+    from specklesim_amd import SieTriMlfmm, plane_wave
+
+    t0 = time.perf_counter()
+    s = SieTriMlfmm(lam, args.eps_r2, formulation=args.formulation, normal_convention=2,
+                    tree_s_opt=250, truncation_number=19, precision=args.precision)
+    s.precalculation_tri(pp, tt)
+    t1 = time.perf_counter()
+    k, e = plane_wave(0.0, 0.0, 2, "TM")
+    b = s.get_vector_b_tri(k, e)
+    x, info = s.gmres_run(b, tolerance=args.tol, max_iterations=args.maxit,
+                          orthogonalization=args.orth)
+    t2 = time.perf_counter()
+    return s, dict(setup_s=t1 - t0, gmres_s=t2 - t1, total_s=t2 - t0, iters=info["iters"],
+                   be=info["be"], info=info["info"], matvecs=info["matvecs"])
+
+
+def cpu_baseline(args, lam, pp, tt, iters):
+    """The restatement (oracle/) on the host cores: near fill of a row sample,
+    tables and patterns, matvecs; extrapolated to one solve."""
+    from oracle.oracle import Oracle
+
+    nthreads = min(16, os.cpu_count() or 1)
+    o = Oracle(pp, tt, lam, args.eps_r2, formulation=args.formulation, normal_convention=2)
+    t0 = time.perf_counter()
+    o.tree()
+    ia, ja, _ = o.near_csr(fill=False)
+    t_tree = time.perf_counter() - t0
+    rows = max(1, o.N // 50)
+    t0 = time.perf_counter()
+    o.near_fill_rows(0, rows, nthreads)
+    t_fill_sample = time.perf_counter() - t0
+    t_fill = t_fill_sample * (ia[-1] / max(1, ia[rows]))
+    t0 = time.perf_counter()
+    o.mlfmm_setup(nthreads)
+    t_mlfmm_setup = time.perf_counter() - t0
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal(2 * o.N) + 1j * rng.standard_normal(2 * o.N)
+    o.matvec(x, nthreads)
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        o.matvec(x, nthreads)
+    t_mv = (time.perf_counter() - t0) / reps
+    t_solve = t_tree + t_fill + t_mlfmm_setup + (iters + 2) * t_mv
+    return {"value": 1.0 / t_solve, "unit": "solves/s", "cores": nthreads, "kind": "port",
+            "sample": f"oracle/ restatement, {nthreads} OpenMP threads: near fill of rows "
+                      f"[0,{rows}) scaled by nnz ({t_fill:.1f} s est.), tree+tables+patterns "
+                      f"{t_tree + t_mlfmm_setup:.1f} s, {reps} matvecs ({t_mv * 1e3:.0f} ms each) x "
+                      f"{iters + 2} GMRES matvecs of the GPU solve; orthogonalisation not timed",
+            "matvec_ms": t_mv * 1e3}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cells", type=int, default=100)
+    ap.add_argument("--size", type=float, default=10.0)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--eps-r2", type=complex, default=2.25 + 0j)
+    ap.add_argument("--formulation", default="ICTF")
+    ap.add_argument("--precision", default="fp64")
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--maxit", type=int, default=650)
+    ap.add_argument("--orth", default="IMGS")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    lam, pp, tt = workload(args)
+    for _ in range(args.warmup):
+        one_solve(args, lam, pp, tt)
+    _sync()
+    t0 = time.perf_counter()
+    runs = []
+    s = None
+    for _ in range(args.steps):
+        s, r = one_solve(args, lam, pp, tt)
+        runs.append(r)
+    _sync()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = elapsed / args.steps * 1e3
+
+    prof = s.profile_matvec(reps=20)
+    st = s.stats()
+    stage, (ms, by) = max(((k, v) for k, v in prof.items() if k != "matvec"),
+                          key=lambda kv: kv[1][0])
+    achieved = by / (ms * 1e-3) / 1e9
+    rec = {
+        "metric": "solves/s (MLFMM-GMRES to tolerance, incl. precalculation); matvec ms",
+        "value": args.steps / elapsed,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "complex128" if args.precision == "fp64" else "complex64+complex128",
+        "data": "synthetic (seeded Gaussian rough surface)",
+        "config": {"workload": f"rough patch {args.size:g}x{args.size:g} lambda, "
+                               f"{args.cells}x{args.cells} cells, {2 * args.cells ** 2} triangles, "
+                               f"N={st['n_edges']} RWG, {args.formulation}, n=1.5, K=19, "
+                               f"s_opt=250, tol {args.tol:g}, {args.orth}",
+                   "unknowns": 2 * st["n_edges"], "near_blocks": st["near_blocks"],
+                   "levels": [st["l_min"], st["l_max"]], "parallelism": "single GPU"},
+        "matvec_ms": prof["matvec"][0],
+        "stages_ms": {k: round(v[0], 4) for k, v in prof.items()},
+        "setup_ms": st["setup_ms"], "near_fill_ms": st["near_fill_ms"],
+        "gmres_iterations": runs[-1]["iters"], "gmres_backward_error": runs[-1]["be"],
+        "gmres_info": runs[-1]["info"], "gmres_s": runs[-1]["gmres_s"],
+        "roofline": {"kernel": stage, "bound": "hbm", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "algorithmic_bytes": by, "ms": ms},
+    }
+    if not args.no_cpu_baseline:
+        cb = cpu_baseline(args, lam, pp, tt, runs[-1]["iters"])
+        rec["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+        rec["cpu_matvec_ms"] = cb["matvec_ms"]
+    print(json.dumps(rec))
+
+
+if __name__ == "__main__":
+    main()

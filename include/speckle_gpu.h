@@ -153,6 +153,21 @@ int spk_near_blocks(spk_handle *h, int npairs, const int *m, const int *n, doubl
  * [edge][k1 theta, k1 phi, k2 theta, k2 phi][sample]. */
 int spk_get_patterns(const spk_handle *h, int first, int count, double *out);
 
+/* Stage timing of the device matvec with HIP events on the handle's stream:
+ * reps matvecs on device buffers; ms[i] = mean duration of stage i per
+ * matvec and bytes[i] = its algorithmic HBM bytes, stages
+ * 0 aggregation, 1 SS (all levels), 2 M2L (all levels), 3 RR (all levels),
+ * 4 fold, 5 disaggregation, 6 near-field SpMV, 7 whole matvec. */
+#define SPK_NSTAGES 8
+int spk_profile_matvec(spk_handle *h, int reps, double *ms, double *bytes);
+
+/* Host-only setup probe (no GPU needed): RWG topology and octree of a mesh
+ * exactly as spk_setup_tri builds them.  corner/element: N x 2 (1-based),
+ * leaf: N Morton indices at l_max.  Call with NULL outputs to get *n_edges. */
+int spk_topology_probe(int np, const double *pp, int nt, const int *tt, int normal_convention,
+                       int tree_s_opt, int64_t *n_edges, int *corner, int *element,
+                       int64_t *leaf, int *l_min, int *l_max, double *D, double *xmin3);
+
 #ifdef __cplusplus
 }
 #endif

@@ -72,6 +72,11 @@ def lib():
         L.spo_tl.argtypes = [C.c_int, C.c_double, C.c_double, dp, dp]
         L.spo_dense_fill.argtypes = [C.c_void_p, dp, C.c_int]
         L.spo_get_pattern.argtypes = [C.c_void_p, C.c_int, dp]
+        L.spo_tree_uindex.restype = C.c_uint64
+        L.spo_tree_uindex.argtypes = [dp, C.c_int]
+        L.spo_tree_neighbours.argtypes = [C.c_int64, C.c_int, C.POINTER(C.c_int64)]
+        L.spo_tree_centre.argtypes = [C.c_int64, C.c_int, dp]
+        L.spo_tree_level_max.argtypes = [C.c_int, dp, C.c_int, C.c_int]
         _lib = L
     return _lib
 
@@ -276,3 +281,24 @@ def tl(K, k, r):
     o = np.zeros(2 * S)
     lib().spo_tl(K, complex(k).real, complex(k).imag, _d(np.asarray(r, float)), _d(o))
     return o[0::2] + 1j * o[1::2]
+
+
+def tree_uindex(xs, l):
+    return int(lib().spo_tree_uindex(_d(np.asarray(xs, float)), l))
+
+
+def tree_neighbours(n, l):
+    o = np.zeros(27, np.int64)
+    k = lib().spo_tree_neighbours(n, l, o.ctypes.data_as(C.POINTER(C.c_int64)))
+    return o[:k]
+
+
+def tree_centre(n, l):
+    o = np.zeros(3)
+    lib().spo_tree_centre(n, l, _d(o))
+    return o
+
+
+def tree_level_max(xs, s, l_th):
+    xs = np.ascontiguousarray(xs, dtype=np.float64).reshape(-1, 3)
+    return lib().spo_tree_level_max(xs.shape[0], _d(xs), s, l_th)

@@ -1826,3 +1826,42 @@ void spo_get_pattern(const spo_ctx *c, int m, double *out) {
   const zc *rt = &c->rt[(size_t)m * 4 * c->S];
   for (int i = 0; i < 4 * c->S; ++i) { out[2 * i] = creal(rt[i]); out[2 * i + 1] = cimag(rt[i]); }
 }
+
+/* ------------------------------------------------------------------------ */
+/* Tree helpers for the reference's own known-answer tests                  */
+/* (lib_tree_helper_functions.f90 test_lib_tree_hf_*, lib_tree.f90          */
+/* test_lib_tree_get_level_max).  xs are already-scaled coordinates.        */
+/* ------------------------------------------------------------------------ */
+uint64_t spo_tree_uindex(const double *xs, int l) {
+  uint64_t ix[3];
+  for (int d = 0; d < 3; ++d) ix[d] = (uint64_t)ldexp(xs[d], l);
+  return morton3(ix[0], ix[1], ix[2], l);
+}
+int spo_tree_neighbours(int64_t n, int l, int64_t *out) {
+  int64_t b[27];
+  int k = e2_boxes(n, l, b);
+  for (int i = 1; i < k; ++i) out[i - 1] = b[i];
+  return k - 1;
+}
+void spo_tree_centre(int64_t n, int l, double *out) {
+  int64_t ix[3];
+  demorton3((uint64_t)n, l, ix);
+  for (int d = 0; d < 3; ++d) out[d] = ldexp(1.0, -l) * ((double)ix[d] + 0.5);
+}
+int spo_tree_level_max(int n, const double *xs, int s, int l_th) {
+  uint64_t *u = xmalloc(sizeof(uint64_t) * (n ? n : 1));
+  for (int m = 0; m < n; ++m) u[m] = spo_tree_uindex(&xs[3 * m], l_th);
+  qsort(u, n, sizeof(uint64_t), cmp_u64);
+  int l_max = 1;
+  for (int i = 0; i + s < n; ++i) {
+    uint64_t a = u[i], b = u[i + s];
+    int j = l_th + 1;
+    for (;;) {
+      j = j - 1;
+      a >>= 3; b >>= 3;
+      if (a == b) { if (j > l_max) l_max = j; break; }
+    }
+  }
+  free(u);
+  return l_max;
+}

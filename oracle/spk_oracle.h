@@ -102,6 +102,12 @@ void spo_dense_fill(const spo_ctx *c, double *Z, int nthreads);
  * [k1 theta, k1 phi, k2 theta, k2 phi][sample]. */
 void spo_get_pattern(const spo_ctx *c, int m, double *out);
 
+/* Tree helpers on scaled coordinates (reference known-answer tests). */
+uint64_t spo_tree_uindex(const double *xs, int l);
+int spo_tree_neighbours(int64_t n, int l, int64_t *out);
+void spo_tree_centre(int64_t n, int l, double *out);
+int spo_tree_level_max(int n, const double *xs, int s, int l_th);
+
 /* Building blocks exposed for pinning tests. */
 void spo_quadrature_tri(int ngp, double *alpha, double *beta, double *w);
 double spo_iq_l(int q, const double *ra, const double *p1, const double *p2);

@@ -17,7 +17,9 @@ EXPORTS = [
     "spk_num_edges", "spk_matvec", "spk_matvec_device", "spk_matvec_parts", "spk_rhs_plane_wave",
     "spk_gmres", "spk_gmres_device", "spk_gmres_history", "spk_get_rwg", "spk_get_tree",
     "spk_get_i4", "spk_get_near_csr", "spk_near_blocks", "spk_get_patterns",
+    "spk_profile_matvec", "spk_topology_probe",
 ]
+STAGES = ["aggregation", "ss", "m2l", "rr", "fold", "disaggregation", "near_spmv", "matvec"]
 
 
 class SpeckleError(RuntimeError):
@@ -87,6 +89,9 @@ def lib():
     L.spk_get_near_csr.argtypes = [vp, i64p, C.POINTER(C.c_int32), dp]
     L.spk_near_blocks.argtypes = [vp, C.c_int, ip, ip, dp]
     L.spk_get_patterns.argtypes = [vp, C.c_int, C.c_int, dp]
+    L.spk_profile_matvec.argtypes = [vp, C.c_int, dp, dp]
+    L.spk_topology_probe.argtypes = [C.c_int, dp, C.c_int, ip, C.c_int, C.c_int, i64p, ip, ip,
+                                     i64p, ip, ip, dp, dp]
     _lib = L
     return L
 

@@ -263,38 +263,72 @@ int setup_mlfmm(spk_handle *h) {
 }
 
 // y = Z x on device vectors (2N complex each), far part into yfar first.
-int matvec_dev(spk_handle *h, const zd *x, zd *y, bool near_on = true, bool far_on = true) {
+// ev (optional, 8 events): recorded before stage 0 and after stages 0..6.
+int matvec_dev(spk_handle *h, const zd *x, zd *y, bool near_on = true, bool far_on = true,
+               hipEvent_t *ev = nullptr) {
   const Tree &t = h->tree;
   const int S = h->tb.S, N = h->N, lmax = t.l_max, lmin = t.l_min;
   hipStream_t st = h->st;
   zd *yfar = h->d_yfar.p;
+  auto mark = [&](int i) {
+    if (ev) (void)hipEventRecord(ev[i], st);
+  };
+  mark(0);
   if (far_on) {
     LevelDev &LL = h->lev[lmax];
     launch_agg(LL.nb, S, N, h->d_box_first.p, h->d_box_count.p, h->d_perm.p, h->d_rt.p, x, LL.B.p,
                st);
+    mark(1);
     for (int l = lmax - 1; l >= lmin; --l)
       launch_ss(h->lev[l].nb, S, h->lev[l].child.p, h->lev[l + 1].B.p, h->lev[l + 1].up.p,
                 h->lev[l].B.p, st);
+    mark(2);
     for (int l = lmin; l <= lmax; ++l)
       launch_m2l(h->lev[l].nb, S, h->lev[l].i4_off.p, h->lev[l].i4_box.p, h->lev[l].i4_slot.p,
                  h->lev[l].B.p, h->lev[l].tlt.p, h->lev[l].D.p, st);
+    mark(3);
     for (int l = lmin + 1; l <= lmax; ++l)
       launch_rr(h->lev[l].nb, S, h->lev[l].parent.p, h->lev[l].octant.p, h->lev[l - 1].D.p,
                 h->lev[l].down.p, h->lev[l].D.p, st);
+    mark(4);
     const zd I = {0, 1};
     const zd kd1 = I * zcx(h->mat.k1), kd2 = I * zcx(h->mat.k2);
     const zd my1 = zd{h->mat.my1, 0} * I * h->mat.omega, my2 = zd{h->mat.my2, 0} * I * h->mat.omega;
     const zd ep1 = zcx(h->mat.eps1) * I * h->mat.omega, ep2 = zcx(h->mat.eps2) * I * h->mat.omega;
     launch_fold(LL.nb, S, LL.D.p, kd1, kd2, my1, my2, ep1, ep2, h->d_W.p, st);
+    mark(5);
     launch_disagg(N, S, h->d_perm.p, h->d_leafbox_of_pos.p, h->d_rt.p, h->d_W.p, h->sc1, h->sc2,
                   near_on ? yfar : y, st);
+    mark(6);
   }
   if (near_on)
     launch_near_spmv_rows(h->spmv_wgs, N, h->d_row_box.p, h->d_row_r0.p, h->d_box_first.p,
                           h->d_box_count.p, h->d_e2_off.p, h->d_e2_list.p, h->d_near_off.p,
                           h->d_perm.p, h->d_blocks.p, x, far_on ? yfar : nullptr, y, st);
+  mark(7);
   HIPCHK(hipGetLastError());
   return SPK_OK;
+}
+
+// algorithmic HBM bytes of each matvec stage (FP64, 16-byte complex)
+void stage_bytes(const spk_handle *h, double *b) {
+  const Tree &t = h->tree;
+  const double S = h->tb.S, N = h->N, c = 16.0;
+  const double nbl = (double)t.lev[t.l_max].n.size();
+  b[0] = N * 4 * S * c + 2 * N * c + nbl * 8 * S * c;  // patterns + x + C
+  b[1] = 0;
+  b[2] = 0;
+  b[3] = 0;
+  for (int l = t.l_min; l <= t.l_max; ++l) {
+    const double nb = (double)t.lev[l].n.size();
+    if (l < t.l_max) b[1] += (double)t.lev[l + 1].n.size() * 8 * S * c + nb * 8 * S * c;
+    b[2] += 2.0 * nb * 8 * S * c + 343.0 * 2 * S * c;  // read B, write D~, TL table
+    if (l > t.l_min) b[3] += (double)t.lev[l - 1].n.size() * 8 * S * c + 2.0 * nb * 8 * S * c;
+  }
+  b[4] = 2.0 * nbl * 8 * S * c;
+  b[5] = N * 4 * S * c + nbl * 8 * S * c + 2 * N * c;  // patterns + W + y
+  b[6] = (double)t.nnz * 4 * c + 2 * N * c + 2 * N * c + 2 * N * c;  // blocks + x + yfar + y
+  b[7] = b[0] + b[1] + b[2] + b[3] + b[4] + b[5] + b[6];
 }
 
 }  // namespace
@@ -844,6 +878,60 @@ int spk_get_patterns(const spk_handle *h, int first, int count, double *out) {
     const int p = h->tree.pos[first + e];
     HIPCHK(hipMemcpy(out + (size_t)e * 8 * S, h->d_rt.p + (size_t)p * 4 * S, 4 * (size_t)S * sizeof(zd),
                      hipMemcpyDeviceToHost));
+  }
+  return SPK_OK;
+}
+
+int spk_profile_matvec(spk_handle *h, int reps, double *ms, double *bytes) {
+  if (!h || reps < 1) return fail(SPK_ERR_ARG, "bad arguments");
+  if (!h->ready) return fail(SPK_ERR_STATE, "spk_setup_tri has not completed");
+  hipEvent_t ev[8];
+  for (int i = 0; i < 8; ++i) HIPCHK(hipEventCreate(&ev[i]));
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int rc = SPK_OK;
+  for (int r = 0; r < reps && rc == SPK_OK; ++r) {
+    rc = matvec_dev(h, h->d_x.p, h->d_y.p, true, true, ev);
+    if (rc) break;
+    HIPCHK(hipEventSynchronize(ev[7]));
+    float t;
+    for (int i = 0; i < 7; ++i) {
+      HIPCHK(hipEventElapsedTime(&t, ev[i], ev[i + 1]));
+      acc[i] += t;
+    }
+    HIPCHK(hipEventElapsedTime(&t, ev[0], ev[7]));
+    acc[7] += t;
+  }
+  for (int i = 0; i < 8; ++i) (void)hipEventDestroy(ev[i]);
+  if (rc) return rc;
+  if (ms)
+    for (int i = 0; i < 8; ++i) ms[i] = acc[i] / reps;
+  if (bytes) stage_bytes(h, bytes);
+  return SPK_OK;
+}
+
+int spk_topology_probe(int np, const double *pp, int nt, const int *tt, int normal_convention,
+                       int tree_s_opt, int64_t *n_edges, int *corner, int *element, int64_t *leaf,
+                       int *l_min, int *l_max, double *D, double *xmin3) {
+  if (!pp || !tt || !n_edges || np < 3 || nt < 1) return fail(SPK_ERR_ARG, "bad mesh arguments");
+  std::vector<int> c, e;
+  std::vector<FnRec> fn;
+  std::vector<ElemRec> er;
+  std::vector<double> rr;
+  std::string err;
+  int rc = build_rwg(np, pp, nt, tt, normal_convention == 2 ? 2 : 1, &c, &e, &fn, &er, &rr, &err);
+  if (rc != SPK_OK) return fail(rc, err);
+  *n_edges = (int64_t)rr.size();
+  if (corner) std::memcpy(corner, c.data(), c.size() * sizeof(int));
+  if (element) std::memcpy(element, e.data(), e.size() * sizeof(int));
+  if (leaf || l_min || l_max || D || xmin3) {
+    if (rr.empty()) return fail(SPK_ERR_MESH, "mesh has no interior edges");
+    Tree t;
+    build_tree(fn, (int)rr.size(), tree_s_opt > 0 ? tree_s_opt : 250, &t);
+    if (leaf) std::memcpy(leaf, t.leaf.data(), t.leaf.size() * sizeof(int64_t));
+    if (l_min) *l_min = t.l_min;
+    if (l_max) *l_max = t.l_max;
+    if (D) *D = t.D;
+    if (xmin3) std::memcpy(xmin3, t.xmin, 3 * sizeof(double));
   }
   return SPK_OK;
 }

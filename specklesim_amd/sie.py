@@ -205,6 +205,14 @@ class SieTriMlfmm:
                                             n.ctypes.data_as(C.POINTER(C.c_int)), _dp(out)))
         return (out[0::2] + 1j * out[1::2]).reshape(-1, 2, 2)
 
+    def profile_matvec(self, reps=20):
+        """Mean device time [ms] and algorithmic HBM bytes of each matvec
+        stage, measured with HIP events on the handle's stream."""
+        ms = np.zeros(8)
+        by = np.zeros(8)
+        nat.check(nat.lib().spk_profile_matvec(self._h, reps, _dp(ms), _dp(by)))
+        return {k: (float(ms[i]), float(by[i])) for i, k in enumerate(nat.STAGES)}
+
     def patterns(self, first, count):
         S = self.stats()["samples"]
         out = np.zeros(count * 8 * S)

@@ -1,0 +1,302 @@
+"""Pin the CPU restatement (oracle/) against the reference's own known-answer
+tests, closed forms and brute-force integration.
+
+Every oracle building block checked here is one a plausible transcription
+mistake (dropped term, wrong sign or index, transposed operand, lost float
+rounding) would make fail.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+from scipy import integrate, special
+
+from oracle import oracle as O
+from specklesim_amd import meshes
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_kat.json")))
+
+
+# --- triangle rules: src/lib_sie_integration_points.f90:7-143 ------------------
+DEG = {1: 1, 3: 2, 4: 3, 6: 4, 7: 5, 12: 6}
+
+
+@pytest.mark.parametrize("ngp", sorted(DEG))
+def test_triangle_rule_exactness(ngp):
+    a, b, w = O.quadrature_tri(ngp)
+    assert abs(w.sum() - 1.0) < 3e-7
+    for i in range(DEG[ngp] + 1):
+        for j in range(DEG[ngp] + 1 - i):
+            exact = 2.0 * math.factorial(i) * math.factorial(j) / math.factorial(i + j + 2)
+            assert abs((w * a ** i * b ** j).sum() - exact) < 2e-7, (i, j)
+    # one degree higher is not integrated exactly (the rule is not over-kill)
+    i = DEG[ngp] + 1
+    errs = [abs((w * a ** k * b ** (i - k)).sum()
+                - 2.0 * math.factorial(k) * math.factorial(i - k) / math.factorial(i + 2))
+            for k in range(i + 1)]
+    assert max(errs) > 1e-6
+
+
+def test_triangle_rule_literals_are_float_rounded():
+    a, b, w = O.quadrature_tri(3)
+    assert a[0] == float(np.float32(0.16666666666667))
+    assert a[0] != 0.16666666666667
+    assert w[0] == float(np.float32(0.33333333333333))
+
+
+# --- Gauss-Legendre, Hankel, Legendre ---------------------------------------------
+@pytest.mark.parametrize("n", [1, 2, 5, 19, 40])
+def test_gauss_legendre_textbook(n):
+    x, w = O.gauss_legendre(n)
+    xr, wr = np.polynomial.legendre.leggauss(n)
+    assert np.allclose(x, xr, atol=1e-14, rtol=0)
+    assert np.allclose(w, wr, atol=1e-14, rtol=0)
+
+
+def test_hankel_reference_kat():
+    g = GOLD["hankel2_sph"]
+    gt = np.array(g["re"]) + 1j * np.array(g["im"])
+    h = O.hankel2_sph(len(gt), complex(*g["z"]))
+    assert gt[0].imag < 0  # sign transcribed from the fixture
+    assert np.max(np.abs(h - gt) / np.abs(gt)) < 1e-12
+
+
+@pytest.mark.parametrize("z", [0.7, 3.2, 7.85, 27.2, 5.0 - 0.3j, 12.0 - 1.1j])
+def test_hankel_vs_scipy(z):
+    n = 19
+    h = O.hankel2_sph(n, z)
+    ref = special.spherical_jn(np.arange(n), z) - 1j * special.spherical_yn(np.arange(n), z)
+    assert np.max(np.abs(h - ref) / np.abs(ref)) < 1e-11
+
+
+def test_legendre_reference_kat():
+    g = GOLD["legendre"]
+    p = O.legendre_p(len(g["pm"]), g["x"])
+    assert np.allclose(p, g["pm"], atol=1e-14)
+    x = np.linspace(-1, 1, 7)
+    for xi in x:
+        ref = [np.polynomial.legendre.legval(xi, [0] * l + [1]) for l in range(19)]
+        assert np.allclose(O.legendre_p(19, xi), ref, atol=1e-13)
+
+
+# --- M2L transfer function: TL_km_ml_fmm (src/lib_sie_mlfmm.f90:129-188) -------------
+def test_transfer_function_reproduces_greens_function():
+    """sum_s TRF_s TL(k_s; X) exp(-i k k_s.d) = exp(-ik|X+d|)/(4 pi |X+d|)."""
+    K = 19
+    k = 2 * np.pi
+    X = np.array([3.0, 0.5, -0.2])
+    d = np.array([0.15, -0.1, 0.2])
+    tl = O.tl(K, k, X)
+    xg, wg = O.gauss_legendre(K)
+    S = 2 * K * K + 2
+    kh = np.zeros((S, 3))
+    trf = np.zeros(S)
+    for j in range(K):
+        for q in range(2 * K):
+            phi = q * np.pi / K
+            i = q + j * 2 * K
+            st = np.sin(np.arccos(xg[j]))
+            kh[i] = [st * np.cos(phi), st * np.sin(phi), xg[j]]
+            trf[i] = wg[j] * np.pi / K
+    approx = np.sum(trf * tl * np.exp(-1j * k * kh @ d))
+    R = np.linalg.norm(X + d)
+    exact = np.exp(-1j * k * R) / (4 * np.pi * R)
+    assert abs(approx - exact) / abs(exact) < 1e-7
+    # the wrong translation direction is not accepted
+    wrong = np.sum(trf * tl * np.exp(+1j * k * kh @ d))
+    assert abs(wrong - exact) / abs(exact) > 1e-2
+
+
+# --- Hanninen closed forms vs brute force --------------------------------------------
+def _tri_quad(c, f, n=48):
+    """int_T f(r') dS' by collapsed (Duffy) Gauss-Legendre n x n."""
+    x, w = np.polynomial.legendre.leggauss(n)
+    u = 0.5 * (x + 1)
+    wu = 0.5 * w
+    c1, c2, c3 = c
+    J = np.linalg.norm(np.cross(c2 - c1, c3 - c2))
+    acc = 0.0
+    for ui, wi in zip(u, wu):
+        for vj, wj in zip(u, wu):
+            r = c1 + ui * (c2 - c1) + ui * vj * (c3 - c2)
+            acc = acc + f(r) * wi * wj * ui * J
+    return acc
+
+
+@pytest.fixture(scope="module")
+def tri():
+    c = np.array([[0.0, 0.0, 0.0], [1.0, 0.1, 0.05], [0.3, 0.9, -0.02]])
+    # norm_vec_r_new (src/lib_sie_tri_calculation_mod.f90:1006-1024): for both
+    # calc_p(7) branches the normal is (p3-p1)x(p2-p1), the clockwise normal
+    # of the corner order, so the m_vector edge normals point outward.
+    n = np.cross(c[2] - c[0], c[1] - c[0])
+    n /= np.linalg.norm(n)
+    mid = c.mean(axis=0)
+    return c, n, mid
+
+
+# Observation points on the -n side (h < 0) of the source triangle, where
+# the reference's Iq_s recursion is the textbook one, and on the +n side.
+RA = [np.array([0.4, 0.3, 0.35]), np.array([1.4, 0.8, 0.3]), np.array([0.2, 0.2, 0.5])]
+RA_POS = [np.array([0.4, 0.3, -0.35]), np.array([1.4, 0.8, -0.3])]
+
+
+@pytest.mark.parametrize("q", [-1, 1, 3])
+def test_iq_l_line_integral(q):
+    p1, p2 = np.array([0.1, -0.2, 0.3]), np.array([1.2, 0.5, -0.1])
+    for ra in RA:
+        ref = integrate.quad(lambda t: np.linalg.norm(p1 + t * (p2 - p1) - ra) ** q, 0, 1,
+                             epsabs=0, epsrel=1e-13)[0] * np.linalg.norm(p2 - p1)
+        assert abs(O.iq_l(q, ra, p1, p2) - ref) / abs(ref) < 1e-11
+
+
+def test_iq_s_surface_integrals(tri):
+    c, n, mid = tri
+    for ra in RA:
+        is_n3, is_n1, is_p1 = O.iq_s_all(ra, c.reshape(-1), n, mid)
+        h = np.dot(n, ra - mid)
+        r3 = _tri_quad(c, lambda r: np.linalg.norm(ra - r) ** -3, 80)
+        r1 = _tri_quad(c, lambda r: np.linalg.norm(ra - r) ** -1)
+        rp = _tri_quad(c, lambda r: np.linalg.norm(ra - r))
+        assert abs(-is_n3 * np.sign(h) - r3) / r3 < 1e-10
+        assert abs(is_n1 - r1) / r1 < 1e-10
+        # Is_p1 carries the single-precision 1.0/3.0 of the reference
+        # (src/lib_sie_tri_calculation_mod.f90:1318): exact only after undoing it
+        assert abs(is_p1 - rp) / rp > 1e-9
+        assert abs(is_p1 * (1.0 / 3.0) / float(np.float32(1.0) / np.float32(3.0)) - rp) / rp < 1e-10
+
+
+def test_iq_s_unsigned_solid_angle_on_positive_side(tri):
+    """Iq_s takes y = abs(a1.(a2 x a3)) (src/lib_sie_tri_calculation_mod.f90:1276-1279),
+    an unsigned solid angle, and Is_n3 = -Omega/h: for h > 0 the recursion
+    Is_n1 = -h^2 Is_n3 - sum t_i I_L then exceeds int 1/R dS' by 2 h Omega.
+    The restatement keeps this (parity with the reference)."""
+    c, n, mid = tri
+    for ra in RA_POS:
+        is_n3, is_n1, _ = O.iq_s_all(ra, c.reshape(-1), n, mid)
+        h = np.dot(n, ra - mid)
+        assert h > 0
+        omega = -h * is_n3
+        r3 = _tri_quad(c, lambda r: np.linalg.norm(ra - r) ** -3, 80)
+        r1 = _tri_quad(c, lambda r: np.linalg.norm(ra - r) ** -1)
+        assert abs(omega - h * r3) / (h * r3) < 1e-10
+        assert abs((is_n1 - 2 * h * omega) - r1) / r1 < 1e-10
+        assert abs(is_n1 - r1) / r1 > 1e-3
+
+
+def _fn19(c, n, mid, df):
+    return np.concatenate([c.reshape(-1), [0.5, 1.0, df], mid, n, [1.0]])
+
+
+@pytest.mark.parametrize("q", [-1, 1])
+def test_k2q_k3q_k4q(tri, q):
+    c, n, mid = tri
+    df = 2.7
+    fn = _fn19(c, n, mid, df)
+    third = float(np.float32(1.0) / np.float32(3.0))
+    for ra in RA:
+        R = lambda r: np.linalg.norm(ra - r)
+        k2 = O.kq(2, q, ra, fn)
+        ref2 = 0.5 * df * _tri_quad(c, lambda r: R(r) ** q * (r - c[2]), 64)
+        # K3q = int grad' R^q dS' = int q R^(q-2) (r' - r) dS'
+        k3 = O.kq(3, q, ra, fn)
+        ref3 = _tri_quad(c, lambda r: q * R(r) ** (q - 2) * (r - ra), 96)
+        k4 = O.kq(4, q, ra, fn)
+        ref4 = -0.5 * df * np.cross(ra - c[2], ref3)
+        tol2 = 1e-10 if q == -1 else 1e-7  # float(1/3) enters K2q(1)
+        assert np.linalg.norm(k2 - ref2) / np.linalg.norm(ref2) < tol2
+        assert np.linalg.norm(k3 - ref3) / np.linalg.norm(ref3) < 1e-9
+        assert np.linalg.norm(k4 - ref4) / np.linalg.norm(ref4) < 1e-9
+        if q == 1:
+            # and the float-rounded 1/3 really is there (dropping it moves the value)
+            assert np.linalg.norm(k2 - ref2) / np.linalg.norm(ref2) > 1e-10 or third == 1.0 / 3.0
+
+
+# --- octree: reference self-tests (3-D branches) ----------------------------------------
+def test_tree_known_answers():
+    t = GOLD["tree"]
+    assert O.tree_uindex(t["uindex"]["x"], t["uindex"]["l"]) == t["uindex"]["n"]
+    assert O.tree_uindex(t["uindex_lmax"]["x"], 21) == t["uindex_lmax"]["n"]
+    assert t["parent"]["n"] >> 3 == t["parent"]["parent"]
+    assert [(t["children"]["n"] << 3) | o for o in range(8)] == t["children"]["children"]
+    assert np.array_equal(O.tree_centre(t["centre"]["n"], t["centre"]["l"]), t["centre"]["x"])
+    for nb in t["neighbours"]:
+        assert list(O.tree_neighbours(nb["n"], nb["l"])) == nb["list"]
+    lm = t["level_max"]
+    for s, l_max in zip(lm["s"], lm["l_max"]):
+        assert O.tree_level_max(lm["xs"], s, lm["l_th"]) == l_max
+
+
+def test_tree_uindex_is_morton_of_floor():
+    rng = np.random.default_rng(3)
+    for _ in range(200):
+        x = rng.random(3)
+        l = int(rng.integers(1, 22))
+        ix = [int(math.floor(v * 2 ** l)) for v in x]
+        n = 0
+        for b in range(l):
+            n |= ((ix[0] >> b) & 1) << (3 * b + 2) | ((ix[1] >> b) & 1) << (3 * b + 1) | \
+                 ((ix[2] >> b) & 1) << (3 * b)
+        assert O.tree_uindex(x, l) == n
+
+
+def test_tree_lists_brute_force(lam):
+    pp, tt = meshes.rough_surface(16, 16, 2.0, 2.0, 0.1, 0.5, 5, lam)
+    o = O.Oracle(pp, tt, lam, 2.25, s_opt=40)
+    T = o.tree()
+    L = T["l_max"]
+    leaf = T["leaf"]
+
+    def coords(n, l):
+        x = [0, 0, 0]
+        for b in range(l):
+            x[0] |= ((n >> (3 * b + 2)) & 1) << b
+            x[1] |= ((n >> (3 * b + 1)) & 1) << b
+            x[2] |= ((n >> (3 * b)) & 1) << b
+        return np.array(x)
+
+    C = {m: coords(leaf[m], L) for m in range(o.N)}
+    for m in range(0, o.N, 37):
+        e2 = set(o.e2(m))
+        ref = {j for j in range(o.N) if np.max(np.abs(C[j] - C[m])) <= 1}
+        assert e2 == ref
+    # I4: children of the parent's neighbourhood minus the own neighbourhood
+    boxes = sorted(set(leaf.tolist()))
+    for a in boxes[:10]:
+        ca = coords(a, L)
+        pa = ca // 2
+        ref = set()
+        for n in range(8 ** L):
+            cn = coords(n, L)
+            if np.max(np.abs(cn // 2 - pa)) <= 1 and np.max(np.abs(cn - ca)) > 1:
+                ref.add(n)
+        assert set(o.i4(L, a).tolist()) == ref
+
+
+# --- GMRES: reference GMRES_test problem (lib_math_solver_GMRES.f90:381-500) ----------
+@pytest.mark.parametrize("orth", [0, 1, 3])
+def test_gmres_reference_problem(orth):
+    g = GOLD["gmres"]
+    n = g["n"]
+    A = np.zeros((n, n), complex)
+    A[np.arange(n), np.arange(n)] = complex(*g["diag"])
+    A[np.arange(n - 1), np.arange(1, n)] = complex(*g["offdiag"])
+    A[np.arange(1, n), np.arange(n - 1)] = complex(*g["offdiag"])
+    b = A @ np.ones(n)
+    x, info = O.gmres_dense(A, b, maxit=1000, restart=g["restart"], tol=1e-13, orth=orth)
+    assert info["info"] == 0
+    assert np.max(np.abs(x - 1.0)) < 1e-10
+    assert info["be"] <= 1e-13
+
+
+def test_gmres_matches_direct_solve():
+    rng = np.random.default_rng(11)
+    n = 60
+    A = np.eye(n) * 3 + (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) / 8
+    b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    x, info = O.gmres_dense(A, b, maxit=200, tol=1e-14, orth=1)
+    assert np.linalg.norm(x - np.linalg.solve(A, b)) / np.linalg.norm(x) < 1e-12
+    h = info["hist"]
+    assert np.all(np.diff(h) <= 1e-15)  # GMRES residual estimate is monotone
