@@ -58,7 +58,32 @@ def build(verbose: bool = False) -> str:
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp,
           *[j[0] for j in jobs], "-lpthread"])
     os.replace(tmp, LIB)
+    build_fortran()
     return LIB
+
+
+FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
+FORTRAN_DIR = os.path.join(HERE, "fortran")
+FORTRAN_EXE = os.path.join(HERE, "spk_fortran_check")
+
+
+def build_fortran() -> str | None:
+    """The Fortran ISO_C_BINDING module and its driver program (a Fortran
+    host calling the library the way the reference's solver does).  Skipped
+    when no Fortran compiler is installed."""
+    if not os.path.exists(FLANG):
+        return None
+    mod = os.path.join(BUILD_DIR, "fortran")
+    os.makedirs(mod, exist_ok=True)
+    obj = os.path.join(mod, "spk_gpu_binding.o")
+    _run([FLANG, "-O2", "-module-dir", mod, "-c",
+          os.path.join(FORTRAN_DIR, "spk_gpu_binding.f90"), "-o", obj])
+    tmp = FORTRAN_EXE + ".tmp"
+    _run([FLANG, "-O2", "-module-dir", mod, "-I", mod,
+          os.path.join(FORTRAN_DIR, "spk_fortran_check.f90"), obj, "-L", HERE, "-lspeckle_gpu",
+          "-Wl,-rpath,$ORIGIN", "-o", tmp])
+    os.replace(tmp, FORTRAN_EXE)
+    return FORTRAN_EXE
 
 
 if __name__ == "__main__":
