@@ -102,6 +102,7 @@ struct spk_handle {
   zd sc1{1, 0}, sc2{1, 0};
   // gmres workspace
   DBuf<zd> g_V, g_w, g_r0, g_xc, g_b, g_h, g_partial, g_scal, g_y;
+  DBuf<unsigned> g_ticket;  // last-block reduction counter, zero between launches
   int g_m = 0;
   std::vector<double> hist;
   void *scan_tmp = nullptr;
@@ -561,7 +562,7 @@ static void zrot1(zc *x, zc *y, double cc, zc s) {
 }
 
 static int dev_dot(spk_handle *h, const zd *x, const zd *y, int64_t n, zc *out) {
-  launch_zdotc(n, x, y, h->g_partial.p, h->g_scal.p, h->st);
+  launch_zdotc(n, x, y, h->g_partial.p, h->g_scal.p, h->g_ticket.p, h->st);
   zd r;
   HIPCHK(hipMemcpyAsync(&r, h->g_scal.p, sizeof(zd), hipMemcpyDeviceToHost, h->st));
   HIPCHK(hipStreamSynchronize(h->st));
@@ -594,6 +595,10 @@ int spk_gmres_device(spk_handle *h, const void *d_b, void *d_x, const spk_gmres_
   HIPCHK(h->g_y.alloc(m + 1));
   HIPCHK(h->g_partial.alloc((size_t)reduce_blocks(n) * (m + 1)));
   HIPCHK(h->g_scal.alloc(4));
+  if (!h->g_ticket.p) {
+    HIPCHK(h->g_ticket.alloc(1));
+    HIPCHK(hipMemsetAsync(h->g_ticket.p, 0, sizeof(unsigned), h->st));
+  }
   zd *V = h->g_V.p, *w = h->g_w.p, *r0 = h->g_r0.p, *xc = h->g_xc.p;
   std::vector<zc> H((size_t)(m + 1) * (m + 1), 0.0), yc(m + 1), rs(m + 1), hh(m + 1);
   std::vector<double> rc(m + 1);
@@ -641,13 +646,14 @@ int spk_gmres_device(spk_handle *h, const void *d_b, void *d_x, const spk_gmres_
         if (orth == SPK_ORTH_MGS || orth == SPK_ORTH_IMGS) {
           for (int j = 0; j < jH; ++j)
             launch_mgs_step(n, j ? V + (size_t)(j - 1) * n : nullptr, j ? h->g_h.p + (j - 1) : nullptr,
-                            V + (size_t)j * n, w, h->g_partial.p, h->g_h.p + j, h->st);
+                            V + (size_t)j * n, w, h->g_partial.p, h->g_h.p + j, h->g_ticket.p, h->st);
           launch_zaxpy_dev(n, h->g_h.p + (jH - 1), -1.0, V + (size_t)(jH - 1) * n, w, h->st);
         } else {
           launch_gemv_c(n, jH, V, n, w, h->g_partial.p, h->g_h.p, h->st);
           launch_gemv_n_sub(n, jH, V, n, h->g_h.p, w, h->st);
         }
-        launch_zdotc(n, w, w, h->g_partial.p + (size_t)reduce_blocks(n) * m, h->g_scal.p, h->st);
+        launch_zdotc(n, w, w, h->g_partial.p + (size_t)reduce_blocks(n) * m, h->g_scal.p,
+                     h->g_ticket.p, h->st);
         std::vector<zd> hv(jH + 1);
         HIPCHK(hipMemcpyAsync(hv.data(), h->g_h.p, jH * sizeof(zd), hipMemcpyDeviceToHost, h->st));
         HIPCHK(hipMemcpyAsync(&hv[jH], h->g_scal.p, sizeof(zd), hipMemcpyDeviceToHost, h->st));

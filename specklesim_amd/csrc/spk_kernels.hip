@@ -1059,13 +1059,42 @@ __device__ inline zd block_sum(zd v) {
   return t;
 }
 
-__global__ void k_dot_partial(int64_t n, const zd *x, const zd *y, zd *partial) {
+// Last-block reduction: every block publishes its partial, then takes a
+// ticket; the block that draws the last ticket sums the partials in a fixed
+// order (thread t takes partials t, t+256, ...; then block_sum), so the
+// result is the same as a separate one-block reduce kernel, bit for bit.
+// The ticket counter is zero on entry and left at zero on exit.
+__device__ inline bool arrive_last(unsigned *ticket) {
+  __shared__ bool last;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const unsigned t = atomicAdd(ticket, 1u);
+    last = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (last) __threadfence();
+  return last;
+}
+__device__ inline void reduce_partials_last(const zd *partial, zd *out, unsigned *ticket) {
+  zd acc = {0, 0};
+  for (int i = threadIdx.x; i < (int)gridDim.x; i += blockDim.x) acc = acc + partial[i];
+  acc = block_sum(acc);
+  if (threadIdx.x == 0) {
+    *out = acc;
+    atomicExch(ticket, 0u);
+  }
+}
+
+__global__ void k_dot(int64_t n, const zd *x, const zd *y, zd *partial, zd *out,
+                      unsigned *ticket) {
   zd acc = {0, 0};
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
     acc = acc + conj(x[i]) * y[i];
   acc = block_sum(acc);
   if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+  if (arrive_last(ticket)) reduce_partials_last(partial, out, ticket);
 }
 __global__ void k_reduce_final(int nparts, const zd *partial, zd *out) {
   zd acc = {0, 0};
@@ -1073,9 +1102,9 @@ __global__ void k_reduce_final(int nparts, const zd *partial, zd *out) {
   acc = block_sum(acc);
   if (threadIdx.x == 0) *out = acc;
 }
-void launch_zdotc(int64_t n, const zd *x, const zd *y, zd *partial, zd *out, hipStream_t st) {
-  hipLaunchKernelGGL(k_dot_partial, dim3(kRedBlocks), dim3(256), 0, st, n, x, y, partial);
-  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, st, kRedBlocks, partial, out);
+void launch_zdotc(int64_t n, const zd *x, const zd *y, zd *partial, zd *out, unsigned *ticket,
+                  hipStream_t st) {
+  hipLaunchKernelGGL(k_dot, dim3(kRedBlocks), dim3(256), 0, st, n, x, y, partial, out, ticket);
 }
 
 __global__ void k_zaxpy(int64_t n, zd a, const zd *x, zd *y) {
@@ -1122,9 +1151,9 @@ void launch_zadd(int64_t n, const zd *a, zd *y, hipStream_t st) {
   hipLaunchKernelGGL(k_zadd, dim3(gcap(n)), dim3(256), 0, st, n, a, y);
 }
 
-// MGS step j: w -= h_{j-1} v_{j-1} (if vprev), then partial <v_j, w>.
-__global__ void k_mgs_partial(int64_t n, const zd *vprev, const zd *hprev, const zd *vj, zd *w,
-                              zd *partial) {
+// MGS step j: w -= h_{j-1} v_{j-1} (if vprev), then h_j = <v_j, w>, one launch.
+__global__ void k_mgs_step(int64_t n, const zd *vprev, const zd *hprev, const zd *vj, zd *w,
+                           zd *partial, zd *hout, unsigned *ticket) {
   const zd c = vprev ? -(*hprev) : zd{0, 0};
   zd acc = {0, 0};
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -1138,12 +1167,12 @@ __global__ void k_mgs_partial(int64_t n, const zd *vprev, const zd *hprev, const
   }
   acc = block_sum(acc);
   if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+  if (arrive_last(ticket)) reduce_partials_last(partial, hout, ticket);
 }
 void launch_mgs_step(int64_t n, const zd *vprev, const zd *hprev, const zd *vj, zd *w,
-                     zd *partial, zd *hout, hipStream_t st) {
-  hipLaunchKernelGGL(k_mgs_partial, dim3(kRedBlocks), dim3(256), 0, st, n, vprev, hprev, vj, w,
-                     partial);
-  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, st, kRedBlocks, partial, hout);
+                     zd *partial, zd *hout, unsigned *ticket, hipStream_t st) {
+  hipLaunchKernelGGL(k_mgs_step, dim3(kRedBlocks), dim3(256), 0, st, n, vprev, hprev, vj, w,
+                     partial, hout, ticket);
 }
 
 // h[j] = <V_j, w> for j < k: partial[blk*k + j], then reduce per j.

@@ -1,0 +1,67 @@
+"""Matvec profiling driver: set up one rough-surface mesh and repeat the
+MLFMM + near-field matvec, printing per-stage device times (HIP events on
+the handle's stream) and algorithmic HBM bytes as one JSON line.  Meant to
+run under ``rocprofv3`` (kernel trace or counters) without the GMRES solve
+around it.
+
+    python -m specklesim_amd.profile_matvec [--cells 100] [--size 10] [--reps 20]
+    python -m specklesim_amd.profile_matvec --copy-mib 1024
+
+``--copy-mib`` instead runs five plain device-to-device copies of that size
+(torch's copy kernel), the calibration for FETCH_SIZE / WRITE_SIZE counters.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+
+
+def copy_calibration(mib: int) -> dict:
+    import torch
+
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    return {"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 5}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--cells", type=int, default=100)
+    ap.add_argument("--size", type=float, default=10.0)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--eps-r2", type=complex, default=2.25 + 0j)
+    ap.add_argument("--formulation", default="ICTF")
+    ap.add_argument("--precision", default="fp64")
+    ap.add_argument("--reps", type=int, default=20)
+    ap.add_argument("--copy-mib", type=int, default=0)
+    args = ap.parse_args(argv)
+    if args.copy_mib:
+        print(json.dumps(copy_calibration(args.copy_mib)), flush=True)
+        return
+
+    from . import SieTriMlfmm, meshes
+
+    lam = 633e-9
+    pp, tt = meshes.rough_surface(args.cells, args.cells, args.size, args.size, 0.1, 0.5,
+                                  args.seed, lam)
+    s = SieTriMlfmm(lam, args.eps_r2, formulation=args.formulation, normal_convention=2,
+                    tree_s_opt=250, truncation_number=19, precision=args.precision)
+    s.precalculation_tri(pp, tt)
+    prof = s.profile_matvec(reps=args.reps)
+    st = s.stats()
+    out = {"workload": f"rough patch {args.size:g}x{args.size:g} lambda, {args.cells}x{args.cells} "
+                       f"cells, N={st['n_edges']}, {args.formulation}",
+           "reps": args.reps, "near_blocks": st["near_blocks"],
+           "stages": {k: {"ms": v[0], "bytes": v[1],
+                          "GBps": (v[1] / (v[0] * 1e-3) / 1e9) if v[0] > 0 else None}
+                      for k, v in prof.items()}}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
