@@ -8,6 +8,7 @@
 #include <cmath>
 #include <complex>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -102,7 +103,9 @@ struct spk_handle {
   zd sc1{1, 0}, sc2{1, 0};
   // gmres workspace
   DBuf<zd> g_V, g_w, g_r0, g_xc, g_b, g_h, g_partial, g_scal, g_y;
-  DBuf<unsigned> g_ticket;  // last-block reduction counter, zero between launches
+  DBuf<unsigned long long> g_ctl;    // MGS sweep abort flag
+  DBuf<unsigned char> g_slots;       // MGS sweep exchange slots
+  unsigned long long sweep_seq = 0;  // sequence number of the last sweep launch
   int g_m = 0;
   std::vector<double> hist;
   void *scan_tmp = nullptr;
@@ -562,7 +565,7 @@ static void zrot1(zc *x, zc *y, double cc, zc s) {
 }
 
 static int dev_dot(spk_handle *h, const zd *x, const zd *y, int64_t n, zc *out) {
-  launch_zdotc(n, x, y, h->g_partial.p, h->g_scal.p, h->g_ticket.p, h->st);
+  launch_zdotc(n, x, y, h->g_partial.p, h->g_scal.p, h->st);
   zd r;
   HIPCHK(hipMemcpyAsync(&r, h->g_scal.p, sizeof(zd), hipMemcpyDeviceToHost, h->st));
   HIPCHK(hipStreamSynchronize(h->st));
@@ -595,9 +598,22 @@ int spk_gmres_device(spk_handle *h, const void *d_b, void *d_x, const spk_gmres_
   HIPCHK(h->g_y.alloc(m + 1));
   HIPCHK(h->g_partial.alloc((size_t)reduce_blocks(n) * (m + 1)));
   HIPCHK(h->g_scal.alloc(4));
-  if (!h->g_ticket.p) {
-    HIPCHK(h->g_ticket.alloc(1));
-    HIPCHK(hipMemsetAsync(h->g_ticket.p, 0, sizeof(unsigned), h->st));
+  // SPK_MGS_SWEEP=0 selects the per-step launches instead of the cooperative sweep
+  const char *sw = std::getenv("SPK_MGS_SWEEP");
+  const bool sweep_on = !(sw && sw[0] == '0');
+  const int sweepG =
+      (sweep_on && (orth == SPK_ORTH_MGS || orth == SPK_ORTH_IMGS)) ? mgs_sweep_blocks(n) : 0;
+  if (sweepG) {
+    const size_t need = (size_t)(m + 1) * 256 * mgs_sweep_slot_bytes();
+    if (h->g_slots.n < need) {
+      HIPCHK(h->g_slots.alloc(need));
+      HIPCHK(hipMemsetAsync(h->g_slots.p, 0, need, h->st));
+      h->sweep_seq = 0;
+    }
+    if (!h->g_ctl.p) {
+      HIPCHK(h->g_ctl.alloc(1));
+      HIPCHK(hipMemsetAsync(h->g_ctl.p, 0, sizeof(unsigned long long), h->st));
+    }
   }
   zd *V = h->g_V.p, *w = h->g_w.p, *r0 = h->g_r0.p, *xc = h->g_xc.p;
   std::vector<zc> H((size_t)(m + 1) * (m + 1), 0.0), yc(m + 1), rs(m + 1), hh(m + 1);
@@ -643,21 +659,31 @@ int spk_gmres_device(spk_handle *h, const void *d_b, void *d_x, const spk_gmres_
       for (;;) {
         ++nOrtho;
         double dloo = 0.0;
-        if (orth == SPK_ORTH_MGS || orth == SPK_ORTH_IMGS) {
+        if (sweepG) {
+          // the whole MGS pass and |w|^2 in one cooperative launch
+          HIPCHK(launch_mgs_sweep(sweepG, n, jH, V, n, w, h->g_slots.p, h->g_h.p, h->g_scal.p,
+                                  h->g_ctl.p, ++h->sweep_seq, h->st));
+        } else if (orth == SPK_ORTH_MGS || orth == SPK_ORTH_IMGS) {
           for (int j = 0; j < jH; ++j)
             launch_mgs_step(n, j ? V + (size_t)(j - 1) * n : nullptr, j ? h->g_h.p + (j - 1) : nullptr,
-                            V + (size_t)j * n, w, h->g_partial.p, h->g_h.p + j, h->g_ticket.p, h->st);
+                            V + (size_t)j * n, w, h->g_partial.p, h->g_h.p + j, h->st);
           launch_zaxpy_dev(n, h->g_h.p + (jH - 1), -1.0, V + (size_t)(jH - 1) * n, w, h->st);
         } else {
           launch_gemv_c(n, jH, V, n, w, h->g_partial.p, h->g_h.p, h->st);
           launch_gemv_n_sub(n, jH, V, n, h->g_h.p, w, h->st);
         }
-        launch_zdotc(n, w, w, h->g_partial.p + (size_t)reduce_blocks(n) * m, h->g_scal.p,
-                     h->g_ticket.p, h->st);
+        if (!sweepG)
+          launch_zdotc(n, w, w, h->g_partial.p + (size_t)reduce_blocks(n) * m, h->g_scal.p, h->st);
         std::vector<zd> hv(jH + 1);
         HIPCHK(hipMemcpyAsync(hv.data(), h->g_h.p, jH * sizeof(zd), hipMemcpyDeviceToHost, h->st));
         HIPCHK(hipMemcpyAsync(&hv[jH], h->g_scal.p, sizeof(zd), hipMemcpyDeviceToHost, h->st));
         HIPCHK(hipStreamSynchronize(h->st));
+        if (sweepG && hv[jH].y != 0.0) {
+          // an exchange wait ran out of polls: the pass is void
+          HIPCHK(hipMemsetAsync(h->g_ctl.p, 0, sizeof(unsigned long long), h->st));
+          HIPCHK(hipStreamSynchronize(h->st));
+          return fail(SPK_ERR_HIP, "Gram-Schmidt sweep: partial-sum exchange timed out");
+        }
         for (int j = 0; j < jH; ++j) {
           const zc dVi(hv[j].x, hv[j].y);
           HH(j, jH - 1) += dVi;

@@ -111,15 +111,22 @@ void launch_near_pairs(const NearArgs &a, int npairs, const int *m, const int *n
 
 // BLAS-1 (deterministic reductions)
 int reduce_blocks(int64_t n);
-// single launch; ticket: a device counter that is zero between launches
-void launch_zdotc(int64_t n, const zd *x, const zd *y, zd *partial, zd *out, unsigned *ticket,
-                  hipStream_t st);
+void launch_zdotc(int64_t n, const zd *x, const zd *y, zd *partial, zd *out, hipStream_t st);
 void launch_zaxpy(int64_t n, zd a, const zd *x, zd *y, hipStream_t st);
 void launch_zaxpy_dev(int64_t n, const zd *a, double sgn, const zd *x, zd *y, hipStream_t st);
 void launch_zscal_copy(int64_t n, zd a, const zd *x, zd *y, hipStream_t st);
 void launch_zsub(int64_t n, const zd *b, const zd *x, zd *y, hipStream_t st);  // y = b - x
 void launch_mgs_step(int64_t n, const zd *vprev, const zd *hprev, const zd *vj, zd *w,
-                     zd *partial, zd *hout, unsigned *ticket, hipStream_t st);
+                     zd *partial, zd *hout, hipStream_t st);
+// one MGS pass over V[:, 0:k] in one cooperative launch (see the kernel).
+// slots: (k+1)*G records of mgs_sweep_slot_bytes(), zeroed once at
+// allocation; seq: a number above every earlier launch's on these slots;
+// ctl[0]: abort flag (zero); nrm = (|w|^2, abort flag)
+int mgs_sweep_blocks(int64_t n);  // 0: n too large for the register slices
+size_t mgs_sweep_slot_bytes();
+hipError_t launch_mgs_sweep(int G, int64_t n, int k, const zd *V, int64_t ldv, zd *w, void *slots,
+                            zd *hout, zd *nrm, unsigned long long *ctl, unsigned long long seq,
+                            hipStream_t st);
 void launch_gemv_c(int64_t n, int k, const zd *V, int64_t ldv, const zd *w, zd *partial,
                    zd *out, hipStream_t st);
 void launch_gemv_n_sub(int64_t n, int k, const zd *V, int64_t ldv, const zd *h, zd *w,

@@ -1059,42 +1059,13 @@ __device__ inline zd block_sum(zd v) {
   return t;
 }
 
-// Last-block reduction: every block publishes its partial, then takes a
-// ticket; the block that draws the last ticket sums the partials in a fixed
-// order (thread t takes partials t, t+256, ...; then block_sum), so the
-// result is the same as a separate one-block reduce kernel, bit for bit.
-// The ticket counter is zero on entry and left at zero on exit.
-__device__ inline bool arrive_last(unsigned *ticket) {
-  __shared__ bool last;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    const unsigned t = atomicAdd(ticket, 1u);
-    last = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (last) __threadfence();
-  return last;
-}
-__device__ inline void reduce_partials_last(const zd *partial, zd *out, unsigned *ticket) {
-  zd acc = {0, 0};
-  for (int i = threadIdx.x; i < (int)gridDim.x; i += blockDim.x) acc = acc + partial[i];
-  acc = block_sum(acc);
-  if (threadIdx.x == 0) {
-    *out = acc;
-    atomicExch(ticket, 0u);
-  }
-}
-
-__global__ void k_dot(int64_t n, const zd *x, const zd *y, zd *partial, zd *out,
-                      unsigned *ticket) {
+__global__ void k_dot_partial(int64_t n, const zd *x, const zd *y, zd *partial) {
   zd acc = {0, 0};
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
     acc = acc + conj(x[i]) * y[i];
   acc = block_sum(acc);
   if (threadIdx.x == 0) partial[blockIdx.x] = acc;
-  if (arrive_last(ticket)) reduce_partials_last(partial, out, ticket);
 }
 __global__ void k_reduce_final(int nparts, const zd *partial, zd *out) {
   zd acc = {0, 0};
@@ -1102,9 +1073,9 @@ __global__ void k_reduce_final(int nparts, const zd *partial, zd *out) {
   acc = block_sum(acc);
   if (threadIdx.x == 0) *out = acc;
 }
-void launch_zdotc(int64_t n, const zd *x, const zd *y, zd *partial, zd *out, unsigned *ticket,
-                  hipStream_t st) {
-  hipLaunchKernelGGL(k_dot, dim3(kRedBlocks), dim3(256), 0, st, n, x, y, partial, out, ticket);
+void launch_zdotc(int64_t n, const zd *x, const zd *y, zd *partial, zd *out, hipStream_t st) {
+  hipLaunchKernelGGL(k_dot_partial, dim3(kRedBlocks), dim3(256), 0, st, n, x, y, partial);
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, st, kRedBlocks, partial, out);
 }
 
 __global__ void k_zaxpy(int64_t n, zd a, const zd *x, zd *y) {
@@ -1151,9 +1122,9 @@ void launch_zadd(int64_t n, const zd *a, zd *y, hipStream_t st) {
   hipLaunchKernelGGL(k_zadd, dim3(gcap(n)), dim3(256), 0, st, n, a, y);
 }
 
-// MGS step j: w -= h_{j-1} v_{j-1} (if vprev), then h_j = <v_j, w>, one launch.
-__global__ void k_mgs_step(int64_t n, const zd *vprev, const zd *hprev, const zd *vj, zd *w,
-                           zd *partial, zd *hout, unsigned *ticket) {
+// MGS step j: w -= h_{j-1} v_{j-1} (if vprev), then partial <v_j, w>.
+__global__ void k_mgs_partial(int64_t n, const zd *vprev, const zd *hprev, const zd *vj, zd *w,
+                              zd *partial) {
   const zd c = vprev ? -(*hprev) : zd{0, 0};
   zd acc = {0, 0};
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -1167,12 +1138,166 @@ __global__ void k_mgs_step(int64_t n, const zd *vprev, const zd *hprev, const zd
   }
   acc = block_sum(acc);
   if (threadIdx.x == 0) partial[blockIdx.x] = acc;
-  if (arrive_last(ticket)) reduce_partials_last(partial, hout, ticket);
 }
 void launch_mgs_step(int64_t n, const zd *vprev, const zd *hprev, const zd *vj, zd *w,
-                     zd *partial, zd *hout, unsigned *ticket, hipStream_t st) {
-  hipLaunchKernelGGL(k_mgs_step, dim3(kRedBlocks), dim3(256), 0, st, n, vprev, hprev, vj, w,
-                     partial, hout, ticket);
+                     zd *partial, zd *hout, hipStream_t st) {
+  hipLaunchKernelGGL(k_mgs_partial, dim3(kRedBlocks), dim3(256), 0, st, n, vprev, hprev, vj, w,
+                     partial);
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, st, kRedBlocks, partial, hout);
+}
+
+// ---------------------------------------------------------------------------
+// One modified Gram-Schmidt pass in a single cooperative launch.
+//
+// For j = 0..k-1: h_j = <v_j, w>; w -= h_j v_j (sequential, as MGS), then
+// |w|^2.  Each block owns a contiguous slice of w, held in registers for the
+// whole pass; the next basis vector's slice is loaded before each exchange.
+//
+// Exchange without fences: block b publishes its step-j partial in its own
+// slot (j, b) with agent-scope atomic stores of (re, im), waits for them to
+// complete (s_waitcnt vmcnt(0)) and then stores the launch's sequence number
+// into the slot.  Lane q of every block's first wave polls slots q, q+64, ...
+// with agent-scope atomic loads until they carry this launch's sequence
+// number, then reads the values; every block sums them in the same order, so
+// all blocks hold the same h_j.  Slots from earlier launches carry smaller
+// sequence numbers.  A poll gives up after kPollLimit tries and raises
+// ctl[0]; every later poll then stops at once, all waves exit, nrm->y reports
+// the flag and the host discards the pass.
+// ---------------------------------------------------------------------------
+constexpr long kPollLimit = 1L << 22;
+
+struct alignas(32) SweepSlot {
+  double re, im;
+  unsigned long long seq, pad;
+};
+
+__device__ inline void slot_publish(SweepSlot *s, zd v, unsigned long long seq) {
+  __hip_atomic_store(&s->re, v.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(&s->im, v.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  __hip_atomic_store(&s->seq, seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// lane-parallel sum of the G slots of one step (first wave only)
+__device__ inline zd slots_sum(SweepSlot *slots, int G, unsigned long long seq,
+                               unsigned long long *abort_flag) {
+  const int lane = threadIdx.x & 63;
+  zd sp = {0, 0};
+  for (int q = lane; q < G; q += 64) {
+    SweepSlot *s = slots + q;
+    long polls = 0;
+    while (__hip_atomic_load(&s->seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != seq) {
+      if ((++polls & 255) == 0) {
+        if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+        if (polls > kPollLimit) {
+          atomicExch(abort_flag, 1ull);
+          break;
+        }
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    asm volatile("" ::: "memory");
+    sp.x += __hip_atomic_load(&s->re, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    sp.y += __hip_atomic_load(&s->im, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  return wave_sum(sp);
+}
+
+template <int E>
+__global__ void __launch_bounds__(256) k_mgs_sweep(int64_t n, int k, const zd *V, int64_t ldv,
+                                                   zd *w, SweepSlot *slots, zd *hout, zd *nrm,
+                                                   unsigned long long *ctl,
+                                                   unsigned long long seq) {
+  const int G = gridDim.x, tid = threadIdx.x, lane = tid & 63;
+  const int64_t chunk = (n + G - 1) / G;
+  const int64_t i0 = (int64_t)blockIdx.x * chunk;
+  const int64_t i1 = min(n, i0 + chunk);
+  __shared__ zd hsh;
+  zd wr[E], vr[E], vn[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int64_t i = i0 + tid + e * 256;
+    wr[e] = i < i1 ? w[i] : zd{0, 0};
+    vr[e] = (k > 0 && i < i1) ? V[i] : zd{0, 0};
+    vn[e] = zd{0, 0};
+  }
+  for (int j = 0; j < k; ++j) {
+    zd acc = {0, 0};
+#pragma unroll
+    for (int e = 0; e < E; ++e) acc = acc + conj(vr[e]) * wr[e];
+    acc = block_sum(acc);
+    if (tid == 0) slot_publish(slots + (size_t)j * G + blockIdx.x, acc, seq);
+    if (j + 1 < k) {
+      const zd *vnext = V + (size_t)(j + 1) * ldv;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int64_t i = i0 + tid + e * 256;
+        vn[e] = i < i1 ? vnext[i] : zd{0, 0};
+      }
+    }
+    if (tid < 64) {
+      const zd sp = slots_sum(slots + (size_t)j * G, G, seq, ctl);
+      if (lane == 0) hsh = sp;
+    }
+    __syncthreads();
+    const zd h = hsh;
+    if (blockIdx.x == 0 && tid == 0) hout[j] = h;
+    const zd c = -h;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      wr[e] = wr[e] + c * vr[e];
+      vr[e] = vn[e];
+    }
+    __syncthreads();
+  }
+  zd acc = {0, 0};
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    acc = acc + conj(wr[e]) * wr[e];
+    const int64_t i = i0 + tid + e * 256;
+    if (i < i1) w[i] = wr[e];
+  }
+  acc = block_sum(acc);
+  if (tid == 0) slot_publish(slots + (size_t)k * G + blockIdx.x, acc, seq);
+  if (blockIdx.x == 0 && tid < 64) {
+    const zd sp = slots_sum(slots + (size_t)k * G, G, seq, ctl);
+    if (lane == 0) {
+      const unsigned long long ab =
+          __hip_atomic_load(ctl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      *nrm = zd{sp.x, ab ? 1.0 : 0.0};
+    }
+  }
+}
+
+size_t mgs_sweep_slot_bytes() { return sizeof(SweepSlot); }
+
+int mgs_sweep_blocks(int64_t n) {
+  // 64 blocks while a slice fits 16 elements per thread, then more blocks
+  int64_t g = 64;
+  while (g < 256 && (n + g - 1) / g > 256 * 16) g *= 2;
+  if ((n + g - 1) / g > 256 * 16) return 0;  // too large for register slices
+  return (int)g;
+}
+
+template <int E>
+static hipError_t launch_sweep_e(int G, int64_t n, int k, const zd *V, int64_t ldv, zd *w,
+                                 void *slots, zd *hout, zd *nrm, unsigned long long *ctl,
+                                 unsigned long long seq, hipStream_t st) {
+  SweepSlot *sl = (SweepSlot *)slots;
+  void *args[] = {&n, &k, (void *)&V, &ldv, &w, &sl, &hout, &nrm, &ctl, &seq};
+  return hipLaunchCooperativeKernel((const void *)k_mgs_sweep<E>, dim3(G), dim3(256), args, 0, st);
+}
+
+hipError_t launch_mgs_sweep(int G, int64_t n, int k, const zd *V, int64_t ldv, zd *w, void *slots,
+                            zd *hout, zd *nrm, unsigned long long *ctl, unsigned long long seq,
+                            hipStream_t st) {
+  const int64_t chunk = (n + G - 1) / G;
+  const int64_t e = (chunk + 255) / 256;
+  if (e <= 1) return launch_sweep_e<1>(G, n, k, V, ldv, w, slots, hout, nrm, ctl, seq, st);
+  if (e <= 2) return launch_sweep_e<2>(G, n, k, V, ldv, w, slots, hout, nrm, ctl, seq, st);
+  if (e <= 4) return launch_sweep_e<4>(G, n, k, V, ldv, w, slots, hout, nrm, ctl, seq, st);
+  if (e <= 8) return launch_sweep_e<8>(G, n, k, V, ldv, w, slots, hout, nrm, ctl, seq, st);
+  if (e <= 16) return launch_sweep_e<16>(G, n, k, V, ldv, w, slots, hout, nrm, ctl, seq, st);
+  return hipErrorInvalidValue;
 }
 
 // h[j] = <V_j, w> for j < k: partial[blk*k + j], then reduce per j.
