@@ -26,6 +26,14 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on the
+# default workload (100 x 100 cells, 10 lambda, ICTF, complex128), each counter
+# scaled by what it reads for a plain 1 GiB device copy (FETCH x 2.0,
+# WRITE x 1.0); profiles/matvec_traffic_rough10x10_ictf_fp64.txt
+MEASURED_TRAFFIC = {("near_spmv", 100, 10.0, "fp64"): 1.8556e9,
+                    ("aggregation", 100, 10.0, "fp64"): 1.4491e9,
+                    ("disaggregation", 100, 10.0, "fp64"): 1.6288e9}
+
 
 def _sync():
     try:
@@ -47,6 +55,10 @@ def workload(args):
 
 
 def one_solve(args, lam, pp, tt):
+    """Precalculation from the host mesh, then the right-hand side and GMRES
+    on device buffers (b and x stay in HBM)."""
+    import torch
+
     from specklesim_amd import SieTriMlfmm, plane_wave
 
     t0 = time.perf_counter()
@@ -55,9 +67,12 @@ def one_solve(args, lam, pp, tt):
     s.precalculation_tri(pp, tt)
     t1 = time.perf_counter()
     k, e = plane_wave(0.0, 0.0, 2, "TM")
-    b = s.get_vector_b_tri(k, e)
-    x, info = s.gmres_run(b, tolerance=args.tol, max_iterations=args.maxit,
-                          orthogonalization=args.orth)
+    d_b = torch.empty(2 * s.N, dtype=torch.complex128, device="cuda")
+    d_x = torch.empty(2 * s.N, dtype=torch.complex128, device="cuda")
+    s.get_vector_b_tri_device(k, e, d_b.data_ptr())
+    info = s.gmres_run_device(d_b.data_ptr(), d_x.data_ptr(), tolerance=args.tol,
+                              max_iterations=args.maxit, orthogonalization=args.orth,
+                              use_initial_guess=False)
     t2 = time.perf_counter()
     return s, dict(setup_s=t1 - t0, gmres_s=t2 - t1, total_s=t2 - t0, iters=info["iters"],
                    be=info["be"], info=info["info"], matvecs=info["matvecs"])
@@ -136,9 +151,11 @@ def main():
 
     prof = s.profile_matvec(reps=20)
     st = s.stats()
+    # the roofline kernel: the longest HBM-bound stage of the matvec
     stage, (ms, by) = max(((k, v) for k, v in prof.items() if k != "matvec"),
                           key=lambda kv: kv[1][0])
     achieved = by / (ms * 1e-3) / 1e9
+    traffic = MEASURED_TRAFFIC.get((stage, args.cells, args.size, args.precision))
     rec = {
         "metric": "solves/s (MLFMM-GMRES to tolerance, incl. precalculation); matvec ms",
         "value": args.steps / elapsed,
@@ -160,12 +177,14 @@ def main():
                    "levels": [st["l_min"], st["l_max"]], "parallelism": "single GPU"},
         "matvec_ms": prof["matvec"][0],
         "stages_ms": {k: round(v[0], 4) for k, v in prof.items()},
+        "stages_GBps": {k: round(v[1] / (v[0] * 1e-3) / 1e9, 1) for k, v in prof.items()
+                        if v[0] > 0},
         "setup_ms": st["setup_ms"], "near_fill_ms": st["near_fill_ms"],
         "gmres_iterations": runs[-1]["iters"], "gmres_backward_error": runs[-1]["be"],
         "gmres_info": runs[-1]["info"], "gmres_s": runs[-1]["gmres_s"],
         "roofline": {"kernel": stage, "bound": "hbm", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "algorithmic_bytes": by, "ms": ms},
+                     "traffic": traffic, "algorithmic_bytes": by, "ms": ms},
     }
     if not args.no_cpu_baseline:
         cb = cpu_baseline(args, lam, pp, tt, runs[-1]["iters"])

@@ -124,6 +124,9 @@ int spk_matvec_parts(spk_handle *h, const double *x, double *y_near, double *y_f
 /* get_vector_b_tri with plane-wave illumination
  * (src/lib_sie_tri_calculation_mod.f90:391-460, 939-1002). */
 int spk_rhs_plane_wave(spk_handle *h, const double k_in[3], const double e_in[3], double *b);
+/* Same, written to a device buffer of 2N complex (HBM-resident). */
+int spk_rhs_plane_wave_device(spk_handle *h, const double k_in[3], const double e_in[3],
+                              void *d_b);
 
 /* lib_math_solver_gmres_run (src/lib/libmath/src/math/lib_math_solver_GMRES.f90:159-379)
  * with the GPU matvec; x holds the initial guess on entry. */

@@ -483,60 +483,69 @@ int spk_matvec_parts(spk_handle *h, const double *x, double *y_near, double *y_f
 }
 
 // get_vector_b_tri / incident_field_tri_general, plane wave
-int spk_rhs_plane_wave(spk_handle *h, const double k_in[3], const double e_in[3], double *bout) {
-  if (!h || !k_in || !e_in || !bout) return fail(SPK_ERR_ARG, "null argument");
-  if (h->N <= 0) return fail(SPK_ERR_STATE, "no mesh");
-  const int N = h->N, ng = h->ngp;
-  const zc k1(h->mat.k1.re, h->mat.k1.im);
+static int rhs_plane_dev(spk_handle *h, const double k_in[3], const double e_in[3], zd *d_b) {
+  RhsArgs a{};
+  a.N = h->N;
+  a.ng = h->ngp;
+  for (int i = 0; i < 20; ++i) {
+    a.qa[i] = h->qa[i];
+    a.qb[i] = h->qb[i];
+    a.qw[i] = h->qw[i];
+  }
+  a.k1 = zd{h->mat.k1.re, h->mat.k1.im};
+  for (int d = 0; d < 3; ++d) {
+    a.kin[d] = k_in[d];
+    a.ein[d] = e_in[d];
+  }
+  a.c0my = h->mat.my1 * kC0;
   const zc e1(h->mat.eta1.re, h->mat.eta1.im), e2(h->mat.eta2.re, h->mat.eta2.im);
   const zc ea(h->mat.eta_a.re, h->mat.eta_a.im);
   const zc a0(h->mat.a_lc[0].re, h->mat.a_lc[0].im), d0(h->mat.d_lc[0].re, h->mat.d_lc[0].im);
   const zc ep1(h->mat.eps1.re, h->mat.eps1.im), ep2(h->mat.eps2.re, h->mat.eps2.im);
-  zc *b = (zc *)bout;
-  for (int m = 0; m < N; ++m) {
-    zc S4[4] = {0, 0, 0, 0};
-    for (int ot = 0; ot < 2; ++ot) {
-      const FnRec &f = h->fn[2 * m + ot];
-      for (int i = 0; i < ng; ++i) {
-        double Ra[3], rfm[3];
-        const double w0 = 1 - h->qa[i] - h->qb[i];
-        for (int d = 0; d < 3; ++d) {
-          Ra[d] = w0 * f.c[d] + h->qa[i] * f.c[3 + d] + h->qb[i] * f.c[6 + d];
-          rfm[d] = (Ra[d] - f.c[6 + d]) * f.df * 0.5;
-        }
-        const zc ph = (k1 * k_in[0]) * Ra[0] + (k1 * k_in[1]) * Ra[1] + (k1 * k_in[2]) * Ra[2];
-        const zc e = std::exp(zc(0, -1) * ph);
-        const zc E[3] = {e_in[0] * e, e_in[1] * e, e_in[2] * e};
-        const double c0my = h->mat.my1 * kC0;
-        const zc H[3] = {(k_in[1] * E[2] - k_in[2] * E[1]) / c0my,
-                         (k_in[2] * E[0] - k_in[0] * E[2]) / c0my,
-                         (k_in[0] * E[1] - k_in[1] * E[0]) / c0my};
-        const double *nf = f.nqf;
-        const zc nE[3] = {nf[1] * E[2] - nf[2] * E[1], nf[2] * E[0] - nf[0] * E[2],
-                          nf[0] * E[1] - nf[1] * E[0]};
-        const zc nH[3] = {nf[1] * H[2] - nf[2] * H[1], nf[2] * H[0] - nf[0] * H[2],
-                          nf[0] * H[1] - nf[1] * H[0]};
-        const double wa = h->qw[i];
-        S4[0] += (rfm[0] * E[0] + rfm[1] * E[1] + rfm[2] * E[2]) * wa * f.area;
-        S4[1] += (rfm[0] * H[0] + rfm[1] * H[1] + rfm[2] * H[2]) * wa * f.area;
-        S4[2] += (rfm[0] * nE[0] + rfm[1] * nE[1] + rfm[2] * nE[2]) * wa * f.area;
-        S4[3] += (rfm[0] * nH[0] + rfm[1] * nH[1] + rfm[2] * nH[2]) * wa * f.area;
-      }
-    }
-    switch (h->p.formulation) {
-      case SPK_PMCHWT: b[m] = S4[0]; b[m + N] = S4[1]; break;
-      case SPK_PMCHWT_S: b[m] = S4[0]; b[m + N] = S4[1] * e1; break;
-      case SPK_MCTF: b[m] = S4[0]; b[m + N] = S4[1] * e1 * e2; break;
-      case SPK_JMCFIE: b[m] = 1.0 / e1 * S4[0] + S4[3]; b[m + N] = -S4[2] + e1 * S4[1]; break;
-      case SPK_MCTF2: b[m] = a0 / e1 * S4[0]; b[m + N] = d0 * e1 * S4[1]; break;
-      case SPK_ICTF: b[m] = S4[0] / ea; b[m + N] = S4[1] * ea; break;
-      case SPK_CNF: b[m] = S4[3]; b[m + N] = -S4[2]; break;
-      default:
-        b[m] = S4[3] * h->mat.my1 / (h->mat.my1 + h->mat.my2);
-        b[m + N] = -S4[2] * ep1 / (ep1 + ep2);
-        break;
-    }
+  // the formulation's combination of (<f,E>, <f,H>, <f,nxE>, <f,nxH>)
+  // (the select case of get_vector_b_tri, src/lib_sie_tri_calculation_mod.f90:391-460)
+  zc ca[4] = {0, 0, 0, 0}, cb[4] = {0, 0, 0, 0};
+  switch (h->p.formulation) {
+    case SPK_PMCHWT: ca[0] = 1.0; cb[1] = 1.0; break;
+    case SPK_PMCHWT_S: ca[0] = 1.0; cb[1] = e1; break;
+    case SPK_MCTF: ca[0] = 1.0; cb[1] = e1 * e2; break;
+    case SPK_JMCFIE: ca[0] = 1.0 / e1; ca[3] = 1.0; cb[2] = -1.0; cb[1] = e1; break;
+    case SPK_MCTF2: ca[0] = a0 / e1; cb[1] = d0 * e1; break;
+    case SPK_ICTF: ca[0] = 1.0 / ea; cb[1] = ea; break;
+    case SPK_CNF: ca[3] = 1.0; cb[2] = -1.0; break;
+    default:
+      ca[3] = h->mat.my1 / (h->mat.my1 + h->mat.my2);
+      cb[2] = -ep1 / (ep1 + ep2);
+      break;
   }
+  for (int q = 0; q < 4; ++q) {
+    a.ca[q] = zd{ca[q].real(), ca[q].imag()};
+    a.cb[q] = zd{cb[q].real(), cb[q].imag()};
+  }
+  launch_rhs_plane(a, h->d_fn.p, d_b, h->st);
+  HIPCHK(hipGetLastError());
+  return SPK_OK;
+}
+
+int spk_rhs_plane_wave_device(spk_handle *h, const double k_in[3], const double e_in[3],
+                              void *d_b) {
+  if (!h || !k_in || !e_in || !d_b) return fail(SPK_ERR_ARG, "null argument");
+  if (h->N <= 0 || !h->d_fn.p) return fail(SPK_ERR_STATE, "no mesh");
+  int rc = rhs_plane_dev(h, k_in, e_in, (zd *)d_b);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(h->st));
+  return SPK_OK;
+}
+
+int spk_rhs_plane_wave(spk_handle *h, const double k_in[3], const double e_in[3], double *bout) {
+  if (!h || !k_in || !e_in || !bout) return fail(SPK_ERR_ARG, "null argument");
+  if (h->N <= 0 || !h->d_fn.p) return fail(SPK_ERR_STATE, "no mesh");
+  DBuf<zd> db;
+  HIPCHK(db.alloc(2 * (size_t)h->N));
+  int rc = rhs_plane_dev(h, k_in, e_in, db.p);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(bout, db.p, 2 * (size_t)h->N * sizeof(zd), hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
   return SPK_OK;
 }
 

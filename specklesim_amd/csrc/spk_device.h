@@ -109,6 +109,19 @@ constexpr int kSpmvRowsPerWG = 16;
 void launch_near_pairs(const NearArgs &a, int npairs, const int *m, const int *n, zd *out,
                        hipStream_t st);
 
+// plane-wave right-hand side, one thread per RWG edge (reference order):
+// b[m] = sum_q ca[q] S_q(m), b[m+N] = sum_q cb[q] S_q(m) with the four
+// tested incident-field integrals S = (<f,E>, <f,H>, <f,n x E>, <f,n x H>)
+struct RhsArgs {
+  int N, ng;
+  double qa[20], qb[20], qw[20];
+  zd k1;
+  double kin[3], ein[3];
+  double c0my;  // my1 * c0
+  zd ca[4], cb[4];
+};
+void launch_rhs_plane(const RhsArgs &a, const FnRec *fn, zd *b, hipStream_t st);
+
 // BLAS-1 (deterministic reductions)
 int reduce_blocks(int64_t n);
 void launch_zdotc(int64_t n, const zd *x, const zd *y, zd *partial, zd *out, hipStream_t st);

@@ -1042,6 +1042,61 @@ void launch_near_spmv_rows(int nwg, int N, const int *row_box, const int *row_r0
 }
 
 // ---------------------------------------------------------------------------
+// Plane-wave right-hand side (get_vector_b_tri / incident_field_tri_general,
+// src/lib_sie_tri_calculation_mod.f90:391-460, 939-1002): at the ng rule
+// points of both triangles of edge m, E = e_in exp(-i k1 k_in.r),
+// H = k_in x E / (my1 c0), tested with f_m(r) = df/2 (r - p_c).
+// ---------------------------------------------------------------------------
+__global__ void k_rhs_plane(RhsArgs a, const FnRec *fn, zd *b) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= a.N) return;
+  zd S4[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+  for (int ot = 0; ot < 2; ++ot) {
+    const FnRec &f = fn[2 * m + ot];
+    for (int i = 0; i < a.ng; ++i) {
+      double Ra[3], rfm[3];
+      const double w0 = 1 - a.qa[i] - a.qb[i];
+      for (int d = 0; d < 3; ++d) {
+        Ra[d] = w0 * f.c[d] + a.qa[i] * f.c[3 + d] + a.qb[i] * f.c[6 + d];
+        rfm[d] = (Ra[d] - f.c[6 + d]) * f.df * 0.5;
+      }
+      const zd ph = (a.k1 * a.kin[0]) * Ra[0] + (a.k1 * a.kin[1]) * Ra[1] +
+                    (a.k1 * a.kin[2]) * Ra[2];
+      // exp(-i ph) = exp(Im ph) (cos Re ph - i sin Re ph)
+      double sn, cs;
+      sincos(ph.x, &sn, &cs);
+      const double mag = exp(ph.y);
+      const zd e = {mag * cs, -mag * sn};
+      const zd E[3] = {e * a.ein[0], e * a.ein[1], e * a.ein[2]};
+      const zd H[3] = {(E[2] * a.kin[1] - E[1] * a.kin[2]) * (1.0 / a.c0my),
+                       (E[0] * a.kin[2] - E[2] * a.kin[0]) * (1.0 / a.c0my),
+                       (E[1] * a.kin[0] - E[0] * a.kin[1]) * (1.0 / a.c0my)};
+      const double *nf = f.nqf;
+      const zd nE[3] = {E[2] * nf[1] - E[1] * nf[2], E[0] * nf[2] - E[2] * nf[0],
+                        E[1] * nf[0] - E[0] * nf[1]};
+      const zd nH[3] = {H[2] * nf[1] - H[1] * nf[2], H[0] * nf[2] - H[2] * nf[0],
+                        H[1] * nf[0] - H[0] * nf[1]};
+      const double wa = a.qw[i] * f.area;
+      S4[0] = S4[0] + (E[0] * rfm[0] + E[1] * rfm[1] + E[2] * rfm[2]) * wa;
+      S4[1] = S4[1] + (H[0] * rfm[0] + H[1] * rfm[1] + H[2] * rfm[2]) * wa;
+      S4[2] = S4[2] + (nE[0] * rfm[0] + nE[1] * rfm[1] + nE[2] * rfm[2]) * wa;
+      S4[3] = S4[3] + (nH[0] * rfm[0] + nH[1] * rfm[1] + nH[2] * rfm[2]) * wa;
+    }
+  }
+  zd be = {0, 0}, bh = {0, 0};
+  for (int q = 0; q < 4; ++q) {
+    be = be + a.ca[q] * S4[q];
+    bh = bh + a.cb[q] * S4[q];
+  }
+  b[m] = be;
+  b[m + a.N] = bh;
+}
+void launch_rhs_plane(const RhsArgs &a, const FnRec *fn, zd *b, hipStream_t st) {
+  if (a.N <= 0) return;
+  hipLaunchKernelGGL(k_rhs_plane, dim3(nblk(a.N, 128)), dim3(128), 0, st, a, fn, b);
+}
+
+// ---------------------------------------------------------------------------
 // BLAS-1 with deterministic two-stage reductions (fixed grid of kRedBlocks)
 // ---------------------------------------------------------------------------
 constexpr int kRedBlocks = 256;

@@ -119,6 +119,12 @@ class SieTriMlfmm:
         nat.check(nat.lib().spk_rhs_plane_wave(self._h, _dp(k), _dp(e), _dp(b.view(np.float64))))
         return b
 
+    def get_vector_b_tri_device(self, k_in, e_in, d_b):
+        """Plane-wave right-hand side into a device buffer of 2N complex128."""
+        k = np.ascontiguousarray(k_in, dtype=np.float64)
+        e = np.ascontiguousarray(e_in, dtype=np.float64)
+        nat.check(nat.lib().spk_rhs_plane_wave_device(self._h, _dp(k), _dp(e), C.c_void_p(int(d_b))))
+
     def get_init_vector_x_tri(self, x_initial=0.0):
         return np.full(2 * self.N, complex(x_initial), dtype=np.complex128)
 

@@ -85,9 +85,15 @@ def test_patterns(patch5):
 
 
 def test_rhs(patch5):
+    import torch
+
     pp, tt, o, g, _ = patch5
     k, e = plane_wave(0.3, 0.2, 2, "TM")
-    assert rel(g.get_vector_b_tri(k, e), o.rhs_plane(k, e)) < 1e-13
+    b = g.get_vector_b_tri(k, e)
+    assert rel(b, o.rhs_plane(k, e)) < 1e-13
+    d_b = torch.empty(2 * g.N, dtype=torch.complex128, device="cuda")
+    g.get_vector_b_tri_device(k, e, d_b.data_ptr())
+    assert np.array_equal(d_b.cpu().numpy(), b)
 
 
 def test_matvec_linear_and_deterministic(patch5):
@@ -121,6 +127,9 @@ def test_all_formulations_near_blocks(form):
     for i in range(2):
         for j in range(2):
             assert rel(zg[:, i, j], zo[:, i, j]) < 1e-10, (form, i, j)
+    # the formulation's right-hand side (oblique TE incidence)
+    k, e = plane_wave(0.4, 0.9, 2, "TE")
+    assert rel(g.get_vector_b_tri(k, e), o.rhs_plane(k, e)) < 1e-13, form
     # the fill kernels (classified task lists) equal the pair probe
     ia, ja, vg = g.near_csr()
     rows = np.repeat(np.arange(o.N), np.diff(ia))
