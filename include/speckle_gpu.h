@@ -128,6 +128,19 @@ int spk_rhs_plane_wave(spk_handle *h, const double k_in[3], const double e_in[3]
 int spk_rhs_plane_wave_device(spk_handle *h, const double k_in[3], const double e_in[3],
                               void *d_b);
 
+/* Integration_scattered_field_tri (src/lib_sie_tri_calculation_mod.f90:2782-2825):
+ * scattered E and H in the exterior medium at npts points r[3*npts] (m) from
+ * the surface currents x = [J; M] (2N complex, reference order), with the
+ * reference's 3-point rule; E, H: 3*npts complex each.  As in the reference,
+ * the charge term of E is omitted (:2811 multiplies it by 0).  The BRDF and
+ * RCS evaluations of src/lib_sie_tri_solver_normal.f90:325-358 are built on
+ * this call. */
+int spk_scattered_field(spk_handle *h, int npts, const double *r, const double *x, double *E,
+                        double *H);
+/* Same with x in device memory. */
+int spk_scattered_field_device(spk_handle *h, int npts, const double *r, const void *d_x,
+                               double *E, double *H);
+
 /* lib_math_solver_gmres_run (src/lib/libmath/src/math/lib_math_solver_GMRES.f90:159-379)
  * with the GPU matvec; x holds the initial guess on entry. */
 int spk_gmres(spk_handle *h, const double *b, double *x, const spk_gmres_params *p,

@@ -56,6 +56,7 @@ def lib():
         L.spo_mlfmm_setup.argtypes = [C.c_void_p, C.c_int]
         L.spo_matvec.argtypes = [C.c_void_p, dp, dp, C.c_int]
         L.spo_rhs_plane.argtypes = [C.c_void_p, dp, dp, dp]
+        L.spo_scattered_field.argtypes = [C.c_void_p, C.c_int, dp, dp, dp, dp]
         L.spo_gmres.argtypes = [C.c_void_p, dp, dp, C.c_int, C.c_int, C.c_double, C.c_int,
                                 C.c_int, dp, ip, dp]
         L.spo_gmres_dense.argtypes = [C.c_int, dp, dp, dp, C.c_int, C.c_int, C.c_double, C.c_int,
@@ -203,6 +204,15 @@ class Oracle:
         b = np.zeros(2 * self.N, np.complex128)
         lib().spo_rhs_plane(self.h, _d(k), _d(e), _d(b.view(np.float64)))
         return b
+
+    def scattered_field(self, points, x):
+        pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+        x = np.ascontiguousarray(x, dtype=np.complex128)
+        E = np.zeros((pts.shape[0], 3), np.complex128)
+        H = np.zeros((pts.shape[0], 3), np.complex128)
+        lib().spo_scattered_field(self.h, pts.shape[0], _d(pts), _d(x.view(np.float64)),
+                                  _d(E.view(np.float64)), _d(H.view(np.float64)))
+        return E, H
 
     def gmres(self, b, maxit=650, restart=0, tol=1e-3, orth=1, nthreads=0, x0=None):
         b = np.ascontiguousarray(b, dtype=np.complex128)

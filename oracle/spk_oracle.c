@@ -1580,6 +1580,63 @@ void spo_rhs_plane(const spo_ctx *c, const double *kin, const double *ein, doubl
 }
 
 /* ------------------------------------------------------------------------ */
+/* Scattered field at observation points: Integration_scattered_field_tri    */
+/* (src/lib_sie_tri_calculation_mod.f90:2782-2825).  The rule is the 3-point */
+/* one (ngp = 3 at every call site, lib_sie_tri_solver_normal.f90:331-350);  */
+/* the charge term of E is multiplied by 0 in the reference (:2811) and is   */
+/* left out here.                                                            */
+/* ------------------------------------------------------------------------ */
+void spo_scattered_field(const spo_ctx *c, int npts, const double *r, const double *xin,
+                         double *Eout, double *Hout) {
+  const int N = c->N, ng = 3;
+  double qa[20], qb[20], qw[20];
+  spo_quadrature_tri(ng, qa, qb, qw);
+  const zc *x = (const zc *)xin;
+  zc *E = (zc *)Eout, *H = (zc *)Hout;
+  const zc iwmy = I * c->omega * c->my1, iweps = I * c->omega * c->eps1;
+  const zc inv_iwmy = 1.0 / (I * c->omega * c->my1);
+  for (int p = 0; p < npts; ++p) {
+    const double *ra = r + 3 * p;
+    zc SE[3] = {0, 0, 0}, SH[3] = {0, 0, 0};
+    for (int m = 0; m < N; ++m) {
+      const zc J = x[m], M = x[m + N];
+      for (int t = 0; t < 2; ++t) {
+        const fnrwg *f = &c->fn[2 * m + t];
+        const double dfm = f->sign * f->length;
+        for (int i = 0; i < ng; ++i) {
+          double rq[3], fm[3], d[3];
+          simplex(qa[i], qb[i], f->c, rq);
+          for (int k = 0; k < 3; ++k) fm[k] = 0.5 * dfm * (rq[k] - f->c[2][k]);
+          for (int k = 0; k < 3; ++k) d[k] = ra[k] - rq[k];
+          const double R = vlen(d);
+          zc G, DG;
+          diver_green(c->k1, R, &G, &DG);
+          zc gG[3], Mf[3], cE[3], fxg[3];
+          for (int k = 0; k < 3; ++k) gG[k] = d[k] / R * DG;
+          for (int k = 0; k < 3; ++k) Mf[k] = M * fm[k];
+          cE[0] = Mf[1] * gG[2] - Mf[2] * gG[1];
+          cE[1] = Mf[2] * gG[0] - Mf[0] * gG[2];
+          cE[2] = Mf[0] * gG[1] - Mf[1] * gG[0];
+          fxg[0] = fm[1] * gG[2] - fm[2] * gG[1];
+          fxg[1] = fm[2] * gG[0] - fm[0] * gG[2];
+          fxg[2] = fm[0] * gG[1] - fm[1] * gG[0];
+          for (int k = 0; k < 3; ++k) {
+            const zc ke = iwmy * J * fm[k] * G - cE[k];
+            SE[k] = SE[k] - ke * qw[i];
+            const zc kh = iweps * M * fm[k] * G - inv_iwmy * M * dfm * gG[k] + J * fxg[k];
+            SH[k] = SH[k] - kh * qw[i];
+          }
+        }
+      }
+    }
+    for (int k = 0; k < 3; ++k) {
+      E[3 * p + k] = SE[k];
+      H[3 * p + k] = SH[k];
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------ */
 /* CERFACS GMRES (src/lib/libmath_external/src/algorithm_842/zPackgmres.f90 */
 /* :537-1565), straight-line form of the reverse-communication loop, with   */
 /* the netlib zrotg/zrot of algorithm_842/blas.f:266-380.                   */

@@ -81,6 +81,10 @@ void spo_matvec(spo_ctx *c, const double *x, double *y, int nthreads);
 /* Plane-wave right-hand side (get_vector_b_tri, Plane illumination). */
 void spo_rhs_plane(const spo_ctx *c, const double *k_in, const double *e_in,
                    double *b);
+/* scattered E and H at npts points r[3*npts] from x = [J; M] (2N complex),
+ * Integration_scattered_field_tri; E, H: 3*npts complex each */
+void spo_scattered_field(const spo_ctx *c, int npts, const double *r, const double *x,
+                         double *E, double *H);
 
 /* CERFACS GMRES (zPackgmres.f90 / lib_math_solver_GMRES.f90) with the
  * oracle's matvec.  orth: 0 MGS, 1 IMGS, 2 CGS, 3 ICGS.  hist[maxit] gets the

@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "libspeckle_gpu.so")
 EXPORTS = [
     "spk_create", "spk_destroy", "spk_last_error", "spk_setup_tri", "spk_get_stats",
     "spk_num_edges", "spk_matvec", "spk_matvec_device", "spk_matvec_parts", "spk_rhs_plane_wave",
-    "spk_rhs_plane_wave_device",
+    "spk_rhs_plane_wave_device", "spk_scattered_field", "spk_scattered_field_device",
     "spk_gmres", "spk_gmres_device", "spk_gmres_history", "spk_get_rwg", "spk_get_tree",
     "spk_get_i4", "spk_get_near_csr", "spk_near_blocks", "spk_get_patterns",
     "spk_profile_matvec", "spk_topology_probe",
@@ -82,6 +82,8 @@ def lib():
     L.spk_matvec_parts.argtypes = [vp, dp, dp, dp]
     L.spk_rhs_plane_wave.argtypes = [vp, dp, dp, dp]
     L.spk_rhs_plane_wave_device.argtypes = [vp, dp, dp, vp]
+    L.spk_scattered_field.argtypes = [vp, C.c_int, dp, dp, dp, dp]
+    L.spk_scattered_field_device.argtypes = [vp, C.c_int, dp, vp, dp, dp]
     L.spk_gmres.argtypes = [vp, dp, dp, C.POINTER(GmresParams), C.POINTER(GmresInfo)]
     L.spk_gmres_device.argtypes = [vp, vp, vp, C.POINTER(GmresParams), C.POINTER(GmresInfo)]
     L.spk_gmres_history.argtypes = [vp, dp, C.c_int, ip]

@@ -549,6 +549,58 @@ int spk_rhs_plane_wave(spk_handle *h, const double k_in[3], const double e_in[3]
   return SPK_OK;
 }
 
+static int scattered_dev(spk_handle *h, int npts, const double *r, const zd *d_x, double *E,
+                         double *H) {
+  ScatArgs a{};
+  a.N = h->N;
+  a.npts = npts;
+  double qa[20], qb[20], qw[20];
+  quadrature_tri(3, qa, qb, qw);
+  for (int i = 0; i < 3; ++i) {
+    a.qa[i] = qa[i];
+    a.qb[i] = qb[i];
+    a.qw[i] = qw[i];
+  }
+  const zc iw(0.0, h->mat.omega);
+  const zc eps1(h->mat.eps1.re, h->mat.eps1.im);
+  const zc iwmy = iw * h->mat.my1, iweps = iw * eps1, inv_iwmy = 1.0 / (iw * h->mat.my1);
+  a.k1 = zd{h->mat.k1.re, h->mat.k1.im};
+  a.iwmy = zd{iwmy.real(), iwmy.imag()};
+  a.iweps = zd{iweps.real(), iweps.imag()};
+  a.inv_iwmy = zd{inv_iwmy.real(), inv_iwmy.imag()};
+  DBuf<double> dr;
+  DBuf<zd> dE, dH;
+  HIPCHK(dr.alloc(3 * (size_t)npts));
+  HIPCHK(dE.alloc(3 * (size_t)npts));
+  HIPCHK(dH.alloc(3 * (size_t)npts));
+  HIPCHK(hipMemcpyAsync(dr.p, r, 3 * (size_t)npts * sizeof(double), hipMemcpyHostToDevice, h->st));
+  launch_scattered(a, h->d_fn.p, d_x, dr.p, dE.p, dH.p, h->st);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(E, dE.p, 3 * (size_t)npts * sizeof(zd), hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipMemcpyAsync(H, dH.p, 3 * (size_t)npts * sizeof(zd), hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  return SPK_OK;
+}
+
+int spk_scattered_field_device(spk_handle *h, int npts, const double *r, const void *d_x,
+                               double *E, double *H) {
+  if (!h || !r || !d_x || !E || !H || npts < 0) return fail(SPK_ERR_ARG, "bad argument");
+  if (h->N <= 0 || !h->d_fn.p) return fail(SPK_ERR_STATE, "no mesh");
+  if (npts == 0) return SPK_OK;
+  return scattered_dev(h, npts, r, (const zd *)d_x, E, H);
+}
+
+int spk_scattered_field(spk_handle *h, int npts, const double *r, const double *x, double *E,
+                        double *H) {
+  if (!h || !r || !x || !E || !H || npts < 0) return fail(SPK_ERR_ARG, "bad argument");
+  if (h->N <= 0 || !h->d_fn.p) return fail(SPK_ERR_STATE, "no mesh");
+  if (npts == 0) return SPK_OK;
+  DBuf<zd> dx;
+  HIPCHK(dx.alloc(2 * (size_t)h->N));
+  HIPCHK(hipMemcpyAsync(dx.p, x, 2 * (size_t)h->N * sizeof(zd), hipMemcpyHostToDevice, h->st));
+  return scattered_dev(h, npts, r, dx.p, E, H);
+}
+
 // ---------------------------------------------------------------------------
 // CERFACS GMRES (zPackgmres.f90 :537-1565) on device vectors.
 // ---------------------------------------------------------------------------

@@ -122,6 +122,15 @@ struct RhsArgs {
 };
 void launch_rhs_plane(const RhsArgs &a, const FnRec *fn, zd *b, hipStream_t st);
 
+// scattered E and H at observation points (Integration_scattered_field_tri)
+struct ScatArgs {
+  int N, npts;
+  double qa[3], qb[3], qw[3];
+  zd k1, iwmy, iweps, inv_iwmy;
+};
+void launch_scattered(const ScatArgs &a, const FnRec *fn, const zd *x, const double *r, zd *E,
+                      zd *H, hipStream_t st);
+
 // BLAS-1 (deterministic reductions)
 int reduce_blocks(int64_t n);
 void launch_zdotc(int64_t n, const zd *x, const zd *y, zd *partial, zd *out, hipStream_t st);

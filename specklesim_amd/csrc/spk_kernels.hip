@@ -1097,6 +1097,60 @@ void launch_rhs_plane(const RhsArgs &a, const FnRec *fn, zd *b, hipStream_t st) 
 }
 
 // ---------------------------------------------------------------------------
+// Scattered field at observation points: Integration_scattered_field_tri
+// (src/lib_sie_tri_calculation_mod.f90:2782-2825) with its 3-point rule.
+// One workgroup per point, threads over RWG edges, fixed-order block
+// reduction of the six field components.  The reference multiplies the
+// charge term of E by 0 (:2811); it is left out.
+// ---------------------------------------------------------------------------
+__device__ inline zd block_sum(zd v);
+
+__global__ void __launch_bounds__(256) k_scattered(ScatArgs a, const FnRec *fn, const zd *x,
+                                                   const double *r, zd *Eo, zd *Ho) {
+  const int p = blockIdx.x;
+  const double ra[3] = {r[3 * p], r[3 * p + 1], r[3 * p + 2]};
+  zd SE[3] = {{0, 0}, {0, 0}, {0, 0}}, SH[3] = {{0, 0}, {0, 0}, {0, 0}};
+  for (int m = threadIdx.x; m < a.N; m += blockDim.x) {
+    const zd J = x[m], M = x[m + a.N];
+    for (int t = 0; t < 2; ++t) {
+      const FnRec &f = fn[2 * m + t];
+      const double dfm = f.sign * f.length;
+      for (int i = 0; i < 3; ++i) {
+        const double w0 = 1 - a.qa[i] - a.qb[i];
+        double fm[3], d[3];
+        for (int k = 0; k < 3; ++k) {
+          const double rq = w0 * f.c[k] + a.qa[i] * f.c[3 + k] + a.qb[i] * f.c[6 + k];
+          fm[k] = 0.5 * dfm * (rq - f.c[6 + k]);
+          d[k] = ra[k] - rq;
+        }
+        const double R = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        zd G, DG;
+        green(a.k1, R, G, DG);
+        const zd gG[3] = {DG * (d[0] / R), DG * (d[1] / R), DG * (d[2] / R)};
+        const zd Mf[3] = {M * fm[0], M * fm[1], M * fm[2]};
+        const zd cE[3] = {Mf[1] * gG[2] - Mf[2] * gG[1], Mf[2] * gG[0] - Mf[0] * gG[2],
+                          Mf[0] * gG[1] - Mf[1] * gG[0]};
+        const zd fxg[3] = {gG[2] * fm[1] - gG[1] * fm[2], gG[0] * fm[2] - gG[2] * fm[0],
+                           gG[1] * fm[0] - gG[0] * fm[1]};
+        const zd cJ = a.iwmy * J, cM = a.iweps * M, cQ = a.inv_iwmy * M * dfm;
+        for (int k = 0; k < 3; ++k) {
+          SE[k] = SE[k] - (cJ * fm[k] * G - cE[k]) * a.qw[i];
+          SH[k] = SH[k] - (cM * fm[k] * G - cQ * gG[k] + J * fxg[k]) * a.qw[i];
+        }
+      }
+    }
+  }
+  for (int k = 0; k < 3; ++k) {
+    const zd e = block_sum(SE[k]);
+    const zd h = block_sum(SH[k]);
+    if (threadIdx.x == 0) {
+      Eo[3 * p + k] = e;
+      Ho[3 * p + k] = h;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // BLAS-1 with deterministic two-stage reductions (fixed grid of kRedBlocks)
 // ---------------------------------------------------------------------------
 constexpr int kRedBlocks = 256;
@@ -1408,6 +1462,12 @@ void launch_gemv_n(int64_t n, int k, const zd *V, int64_t ldv, const zd *h, zd *
                    hipStream_t st) {
   hipLaunchKernelGGL(k_gemv_n, dim3(gcap(n)), dim3(256), 0, st, n, k, V,
                      ldv, h, out);
+}
+
+void launch_scattered(const ScatArgs &a, const FnRec *fn, const zd *x, const double *r, zd *E,
+                      zd *H, hipStream_t st) {
+  if (a.npts <= 0) return;
+  hipLaunchKernelGGL(k_scattered, dim3(a.npts), dim3(256), 0, st, a, fn, x, r, E, H);
 }
 
 }  // namespace spk

@@ -128,6 +128,31 @@ class SieTriMlfmm:
     def get_init_vector_x_tri(self, x_initial=0.0):
         return np.full(2 * self.N, complex(x_initial), dtype=np.complex128)
 
+    # -- Integration_scattered_field_tri -------------------------------------
+    def scattered_field(self, points, vector_x):
+        """Scattered (E, H) at points (m) in the exterior medium from the
+        surface currents x = [J; M]; arrays of shape (npts, 3) complex."""
+        pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+        x = np.ascontiguousarray(vector_x, dtype=np.complex128)
+        if x.shape != (2 * self.N,):
+            raise ValueError(f"vector_x must have length 2N = {2 * self.N}")
+        E = np.zeros((pts.shape[0], 3), np.complex128)
+        H = np.zeros((pts.shape[0], 3), np.complex128)
+        nat.check(nat.lib().spk_scattered_field(self._h, pts.shape[0], _dp(pts),
+                                                _dp(x.view(np.float64)),
+                                                _dp(E.view(np.float64)), _dp(H.view(np.float64))))
+        return E, H
+
+    def scattered_field_device(self, points, d_x):
+        pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+        E = np.zeros((pts.shape[0], 3), np.complex128)
+        H = np.zeros((pts.shape[0], 3), np.complex128)
+        nat.check(nat.lib().spk_scattered_field_device(self._h, pts.shape[0], _dp(pts),
+                                                       C.c_void_p(int(d_x)),
+                                                       _dp(E.view(np.float64)),
+                                                       _dp(H.view(np.float64))))
+        return E, H
+
     # -- lib_math_solver_gmres_run ------------------------------------------
     def gmres_run(self, vector_b, x0=None, max_iterations=650, restart=0, tolerance=1e-3,
                   orthogonalization="IMGS", use_initial_guess=True):

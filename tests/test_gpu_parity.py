@@ -109,6 +109,32 @@ def test_matvec_linear_and_deterministic(patch5):
     assert rel(yn + yf, g.calculate_vector_b_tri(x1)) < 1e-14
 
 
+def test_scattered_field_and_brdf(patch5):
+    """Integration_scattered_field_tri on the GPU against the restatement:
+    near-zone and far-zone points, host and device x; the BRDF curve."""
+    import torch
+
+    from specklesim_amd import farfield
+
+    pp, tt, o, g, _ = patch5
+    rng = np.random.default_rng(11)
+    x = rng.standard_normal(2 * o.N) + 1j * rng.standard_normal(2 * o.N)
+    pts = np.concatenate([rng.standard_normal((6, 3)) * 3 * LAM,
+                          farfield.sphere_points((-1.2, 1.2), 9, 0.3, 1e3 * LAM)[1]])
+    Eg, Hg = g.scattered_field(pts, x)
+    Eo, Ho = o.scattered_field(pts, x)
+    for i in range(pts.shape[0]):
+        assert np.linalg.norm(Eg[i] - Eo[i]) < 1e-12 * np.linalg.norm(Eo[i]), i
+        assert np.linalg.norm(Hg[i] - Ho[i]) < 1e-12 * np.linalg.norm(Ho[i]), i
+    d_x = torch.from_numpy(x).to("cuda")
+    Ed, Hd = g.scattered_field_device(pts, d_x.data_ptr())
+    assert np.array_equal(Ed, Eg) and np.array_equal(Hd, Hg)
+    th, bg = farfield.brdf(g, x, 0.0, (-1.2, 1.2), 9, 0.3, 1e3 * LAM)
+    _, po = farfield.sphere_points((-1.2, 1.2), 9, 0.3, 1e3 * LAM)
+    bo = farfield.intensity(o.scattered_field(po, x)[0], po) / np.cos(th)
+    assert rel(bg, bo) < 1e-12
+
+
 @pytest.mark.parametrize("form", FORMS)
 def test_all_formulations_near_blocks(form):
     pp, tt = meshes.rough_surface(12, 12, 1.2, 1.2, 0.1, 0.5, 9, LAM)
