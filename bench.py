@@ -36,13 +36,9 @@ MEASURED_TRAFFIC = {("near_spmv", 100, 10.0, "fp64"): 1.8556e9,
 
 
 def _sync():
-    try:
-        import torch
+    from specklesim_amd import synchronize
 
-        if torch.cuda.is_available():
-            torch.cuda.synchronize()
-    except Exception:
-        pass
+    synchronize(0)
 
 
 def workload(args):
@@ -57,9 +53,7 @@ def workload(args):
 def one_solve(args, lam, pp, tt):
     """Precalculation from the host mesh, then the right-hand side and GMRES
     on device buffers (b and x stay in HBM)."""
-    import torch
-
-    from specklesim_amd import SieTriMlfmm, plane_wave
+    from specklesim_amd import DeviceBuffer, SieTriMlfmm, plane_wave
 
     t0 = time.perf_counter()
     s = SieTriMlfmm(lam, args.eps_r2, formulation=args.formulation, normal_convention=2,
@@ -67,10 +61,10 @@ def one_solve(args, lam, pp, tt):
     s.precalculation_tri(pp, tt)
     t1 = time.perf_counter()
     k, e = plane_wave(0.0, 0.0, 2, "TM")
-    d_b = torch.empty(2 * s.N, dtype=torch.complex128, device="cuda")
-    d_x = torch.empty(2 * s.N, dtype=torch.complex128, device="cuda")
-    s.get_vector_b_tri_device(k, e, d_b.data_ptr())
-    info = s.gmres_run_device(d_b.data_ptr(), d_x.data_ptr(), tolerance=args.tol,
+    d_b = DeviceBuffer(2 * s.N * 16)
+    d_x = DeviceBuffer(2 * s.N * 16)
+    s.get_vector_b_tri_device(k, e, d_b.ptr)
+    info = s.gmres_run_device(d_b.ptr, d_x.ptr, tolerance=args.tol,
                               max_iterations=args.maxit, orthogonalization=args.orth,
                               use_initial_guess=False)
     t2 = time.perf_counter()

@@ -15,6 +15,7 @@
 #ifndef SPECKLE_GPU_H
 #define SPECKLE_GPU_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -109,6 +110,17 @@ const char *spk_last_error(void);
  * impedance fill on the GPU.  pp: 3*np node coordinates [m];
  * tt: 3*nt node indices, 1-based (read_pp_tt_file_tri). */
 int spk_setup_tri(spk_handle *h, int np, const double *pp, int nt, const int *tt);
+
+/* Device memory for the *_device entry points (no counterpart in the
+ * reference).  A caller that has its own HIP allocations may pass those
+ * instead; they must come from the same HIP runtime as this library. */
+#define SPK_COPY_H2D 1
+#define SPK_COPY_D2H 2
+#define SPK_COPY_D2D 3
+int spk_device_alloc(int device, size_t bytes, void **d_ptr);
+int spk_device_free(void *d_ptr);
+int spk_device_copy(void *dst, const void *src, size_t bytes, int direction);
+int spk_device_synchronize(int device);
 
 int spk_get_stats(const spk_handle *h, spk_stats *out);
 int spk_num_edges(const spk_handle *h, int64_t *n_edges);

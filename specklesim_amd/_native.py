@@ -8,6 +8,8 @@ from __future__ import annotations
 import ctypes as C
 import os
 
+import numpy as np
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libspeckle_gpu.so")
 
@@ -16,6 +18,7 @@ EXPORTS = [
     "spk_create", "spk_destroy", "spk_last_error", "spk_setup_tri", "spk_get_stats",
     "spk_num_edges", "spk_matvec", "spk_matvec_device", "spk_matvec_parts", "spk_rhs_plane_wave",
     "spk_rhs_plane_wave_device", "spk_scattered_field", "spk_scattered_field_device",
+    "spk_device_alloc", "spk_device_free", "spk_device_copy", "spk_device_synchronize",
     "spk_gmres", "spk_gmres_device", "spk_gmres_history", "spk_get_rwg", "spk_get_tree",
     "spk_get_i4", "spk_get_near_csr", "spk_near_blocks", "spk_get_patterns",
     "spk_profile_matvec", "spk_topology_probe",
@@ -83,6 +86,10 @@ def lib():
     L.spk_rhs_plane_wave.argtypes = [vp, dp, dp, dp]
     L.spk_rhs_plane_wave_device.argtypes = [vp, dp, dp, vp]
     L.spk_scattered_field.argtypes = [vp, C.c_int, dp, dp, dp, dp]
+    L.spk_device_alloc.argtypes = [C.c_int, C.c_size_t, C.POINTER(vp)]
+    L.spk_device_free.argtypes = [vp]
+    L.spk_device_copy.argtypes = [vp, vp, C.c_size_t, C.c_int]
+    L.spk_device_synchronize.argtypes = [C.c_int]
     L.spk_scattered_field_device.argtypes = [vp, C.c_int, dp, vp, dp, dp]
     L.spk_gmres.argtypes = [vp, dp, dp, C.POINTER(GmresParams), C.POINTER(GmresInfo)]
     L.spk_gmres_device.argtypes = [vp, vp, vp, C.POINTER(GmresParams), C.POINTER(GmresInfo)]
@@ -104,3 +111,43 @@ def check(rc: int) -> None:
     if rc != 0:
         msg = lib().spk_last_error()
         raise SpeckleError(f"libspeckle_gpu error {rc}: {msg.decode() if msg else ''}")
+
+
+class DeviceBuffer:
+    """HBM buffer owned by libspeckle_gpu's HIP runtime (for the *_device
+    entry points); ``ptr`` is the device address."""
+
+    def __init__(self, nbytes: int, device: int = 0):
+        p = C.c_void_p()
+        check(lib().spk_device_alloc(device, nbytes, C.byref(p)))
+        self.ptr = p.value or 0
+        self.nbytes = nbytes
+
+    def upload(self, a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        if a.nbytes != self.nbytes:
+            raise ValueError("size mismatch")
+        check(lib().spk_device_copy(C.c_void_p(self.ptr), a.ctypes.data_as(C.c_void_p),
+                                    self.nbytes, 1))
+        return self
+
+    def download(self, dtype=np.complex128) -> np.ndarray:
+        out = np.empty(self.nbytes // np.dtype(dtype).itemsize, dtype)
+        check(lib().spk_device_copy(out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr),
+                                    self.nbytes, 2))
+        return out
+
+    def free(self) -> None:
+        if self.ptr:
+            lib().spk_device_free(C.c_void_p(self.ptr))
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def synchronize(device: int = 0) -> None:
+    check(lib().spk_device_synchronize(device))

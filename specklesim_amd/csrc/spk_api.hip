@@ -433,6 +433,39 @@ int spk_setup_tri(spk_handle *h, int np, const double *pp, int nt, const int *tt
   return SPK_OK;
 }
 
+int spk_device_alloc(int device, size_t bytes, void **d_ptr) {
+  if (!d_ptr) return fail(SPK_ERR_ARG, "null argument");
+  *d_ptr = nullptr;
+  HIPCHK(hipSetDevice(device));
+  if (bytes == 0) return SPK_OK;
+  HIPCHK(hipMalloc(d_ptr, bytes));
+  return SPK_OK;
+}
+
+int spk_device_free(void *d_ptr) {
+  if (d_ptr) HIPCHK(hipFree(d_ptr));
+  return SPK_OK;
+}
+
+int spk_device_copy(void *dst, const void *src, size_t bytes, int direction) {
+  if ((!dst || !src) && bytes) return fail(SPK_ERR_ARG, "null argument");
+  if (bytes == 0) return SPK_OK;
+  hipMemcpyKind k = direction == SPK_COPY_H2D   ? hipMemcpyHostToDevice
+                    : direction == SPK_COPY_D2H ? hipMemcpyDeviceToHost
+                    : direction == SPK_COPY_D2D ? hipMemcpyDeviceToDevice
+                                                : hipMemcpyDefault;
+  if (direction < SPK_COPY_H2D || direction > SPK_COPY_D2D)
+    return fail(SPK_ERR_ARG, "unknown copy direction");
+  HIPCHK(hipMemcpy(dst, src, bytes, k));
+  return SPK_OK;
+}
+
+int spk_device_synchronize(int device) {
+  HIPCHK(hipSetDevice(device));
+  HIPCHK(hipDeviceSynchronize());
+  return SPK_OK;
+}
+
 int spk_get_stats(const spk_handle *h, spk_stats *out) {
   if (!h || !out) return fail(SPK_ERR_ARG, "null argument");
   *out = h->stats;

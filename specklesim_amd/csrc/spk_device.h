@@ -36,9 +36,20 @@ __host__ __device__ inline zd zcx(const cplx &c) { return zd{c.re, c.im}; }
 __host__ __device__ inline zd mul_i(zd a) { return zd{-a.y, a.x}; }
 
 __host__ __device__ inline zf operator+(zf a, zf b) { return zf{a.x + b.x, a.y + b.y}; }
+__host__ __device__ inline zf operator-(zf a, zf b) { return zf{a.x - b.x, a.y - b.y}; }
+__host__ __device__ inline zf operator-(zf a) { return zf{-a.x, -a.y}; }
 __host__ __device__ inline zf operator*(zf a, zf b) {
   return zf{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
 }
+__host__ __device__ inline zf operator*(zf a, float s) { return zf{a.x * s, a.y * s}; }
+__host__ __device__ inline zf conj(zf a) { return zf{a.x, -a.y}; }
+__host__ __device__ inline zd widen(zf a) { return zd{(double)a.x, (double)a.y}; }
+__host__ __device__ inline zd widen(zd a) { return a; }
+
+// value conversion to the kernel's complex type
+template <class T> __host__ __device__ inline T cvt(zd a);
+template <> __host__ __device__ inline zd cvt<zd>(zd a) { return a; }
+template <> __host__ __device__ inline zf cvt<zf>(zd a) { return zf{(float)a.x, (float)a.y}; }
 
 // Near-field fill arguments (one struct, passed by value).
 struct NearArgs {
@@ -89,21 +100,29 @@ void launch_patterns(int N, int S, int ngp, const FnRec *fn, const int *perm, co
                      const double *leaf_centre, const double *khat, const double *tth,
                      const double *tph, const double *qa, const double *qb, const double *qw,
                      zd k1, zd k2, zd *rt, hipStream_t st);
+template <class T>
 void launch_agg(int nbl, int S, int N, const int *box_first, const int *box_count,
-                const int *perm, const zd *rt, const zd *x, zd *C, hipStream_t st);
-void launch_ss(int nb, int S, const int *child, const zd *Cc, const zd *up, zd *P, hipStream_t st);
+                const int *perm, const T *rt, const T *x, T *C, hipStream_t st);
+template <class T>
+void launch_ss(int nb, int S, const int *child, const T *Cc, const T *up, T *P, hipStream_t st);
+template <class T>
 void launch_m2l(int nb, int S, const int *i4_off, const int *i4_box, const int *i4_slot,
-                const zd *B, const zd *tlt, zd *Dt, hipStream_t st);
-void launch_rr(int nb, int S, const int *parent, const int *octant, const zd *Dp, const zd *down,
-               zd *Dc, hipStream_t st);
-void launch_fold(int nbl, int S, const zd *D, zd kd1, zd kd2, zd my1, zd my2, zd ep1, zd ep2,
-                 zd *W, hipStream_t st);
-void launch_disagg(int N, int S, const int *perm, const int *leaf_box_of_pos, const zd *rt,
-                   const zd *W, zd sc1, zd sc2, zd *y, hipStream_t st);
+                const T *B, const T *tlt, T *Dt, hipStream_t st);
+template <class T>
+void launch_rr(int nb, int S, const int *parent, const int *octant, const T *Dp, const T *down,
+               T *Dc, hipStream_t st);
+template <class T>
+void launch_fold(int nbl, int S, const T *D, zd kd1, zd kd2, zd my1, zd my2, zd ep1, zd ep2,
+                 T *W, hipStream_t st);
+template <class T>
+void launch_disagg(int N, int S, const int *perm, const int *leaf_box_of_pos, const T *rt,
+                   const T *W, zd sc1, zd sc2, T *y, hipStream_t st);
+template <class T>
 void launch_near_spmv_rows(int nwg, int N, const int *row_box, const int *row_r0,
                            const int *box_first, const int *box_count, const int *e2_off,
                            const int *e2_list, const int64_t *near_off, const int *perm,
-                           const zd *blocks, const zd *x, const zd *yadd, zd *y, hipStream_t st);
+                           const T *blocks, const T *x, const T *yadd, T *y, hipStream_t st);
+void launch_to_zf(int64_t n, const zd *a, zf *b, hipStream_t st);
 constexpr int kSpmvRowsPerWG = 16;
 // arbitrary (m, n) pairs, one thread per pair (parity probe)
 void launch_near_pairs(const NearArgs &a, int npairs, const int *m, const int *n, zd *out,
@@ -133,7 +152,11 @@ void launch_scattered(const ScatArgs &a, const FnRec *fn, const zd *x, const dou
 
 // BLAS-1 (deterministic reductions)
 int reduce_blocks(int64_t n);
-void launch_zdotc(int64_t n, const zd *x, const zd *y, zd *partial, zd *out, hipStream_t st);
+template <class T>  // sum conj(x) y accumulated in complex128
+void launch_zdotc(int64_t n, const T *x, const T *y, zd *partial, zd *out, hipStream_t st);
+void launch_axpy_up(int64_t n, zd a, const zf *d, zd *x, hipStream_t st);        // x += a d
+void launch_scale_down(int64_t n, double s, const zd *x, zf *y, hipStream_t st);  // y = s x
+void launch_scal_copy_f(int64_t n, double s, const zf *x, zf *y, hipStream_t st);
 void launch_zaxpy(int64_t n, zd a, const zd *x, zd *y, hipStream_t st);
 void launch_zaxpy_dev(int64_t n, const zd *a, double sgn, const zd *x, zd *y, hipStream_t st);
 void launch_zscal_copy(int64_t n, zd a, const zd *x, zd *y, hipStream_t st);
@@ -149,11 +172,14 @@ size_t mgs_sweep_slot_bytes();
 hipError_t launch_mgs_sweep(int G, int64_t n, int k, const zd *V, int64_t ldv, zd *w, void *slots,
                             zd *hout, zd *nrm, unsigned long long *ctl, unsigned long long seq,
                             hipStream_t st);
-void launch_gemv_c(int64_t n, int k, const zd *V, int64_t ldv, const zd *w, zd *partial,
+template <class T>
+void launch_gemv_c(int64_t n, int k, const T *V, int64_t ldv, const T *w, zd *partial,
                    zd *out, hipStream_t st);
-void launch_gemv_n_sub(int64_t n, int k, const zd *V, int64_t ldv, const zd *h, zd *w,
+template <class T>
+void launch_gemv_n_sub(int64_t n, int k, const T *V, int64_t ldv, const zd *h, T *w,
                        hipStream_t st);
-void launch_gemv_n(int64_t n, int k, const zd *V, int64_t ldv, const zd *h, zd *out,
+template <class T>
+void launch_gemv_n(int64_t n, int k, const T *V, int64_t ldv, const zd *h, zd *out,
                    hipStream_t st);
 void launch_zadd(int64_t n, const zd *a, zd *y, hipStream_t st);  // y += a
 

@@ -800,123 +800,136 @@ void launch_patterns(int N, int S, int ngp, const FnRec *fn, const int *perm, co
                      leaf_box, leaf_centre, khat, tth, tph, qa, qb, qw, k1, k2, rt);
 }
 
+// The matvec kernels below are templates on the complex type T: zd
+// (complex128) for the FP64 operator, zf (complex64) for the mixed-precision
+// inner operator.  Sums accumulate in T.
+
 // lib_sie_ml_fmm_get_c (src/lib_sie_tri_mlfmm_interface.f90:395-448):
 // C[b][comp*4+cur*2+k][s] = sum_{edges of b} conj(RT[k][comp][s]) * x_cur
+template <class T>
 __global__ void k_agg(int S, int N, const int *box_first, const int *box_count, const int *perm,
-                      const zd *rt, const zd *x, zd *C) {
+                      const T *rt, const T *x, T *C) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   const int b = blockIdx.y, q = blockIdx.z;  // q = k*2 + comp
   if (s >= S) return;
   const int k = q >> 1, comp = q & 1;
-  zd ae = {0, 0}, ah = {0, 0};
+  T ae = {0, 0}, ah = {0, 0};
   const int p0 = box_first[b], cnt = box_count[b];
   for (int e = 0; e < cnt; ++e) {
     const int p = p0 + e;
     const int m = perm[p];
-    const zd f = conj(rt[((size_t)p * 4 + q) * S + s]);
-    const zd xe = x[m], xh = x[m + N];
+    const T f = conj(rt[((size_t)p * 4 + q) * S + s]);
+    const T xe = x[m], xh = x[m + N];
     ae = ae + f * xe;
     ah = ah + f * xh;
   }
-  zd *Cb = C + (size_t)b * 8 * S;
+  T *Cb = C + (size_t)b * 8 * S;
   Cb[(size_t)(comp * 4 + 0 * 2 + k) * S + s] = ae;
   Cb[(size_t)(comp * 4 + 1 * 2 + k) * S + s] = ah;
 }
 
+template <class T>
 void launch_agg(int nbl, int S, int N, const int *box_first, const int *box_count,
-                const int *perm, const zd *rt, const zd *x, zd *C, hipStream_t st) {
+                const int *perm, const T *rt, const T *x, T *C, hipStream_t st) {
   if (nbl <= 0) return;
-  hipLaunchKernelGGL(k_agg, dim3(nblk(S, 256), nbl, 4), dim3(256), 0, st, S, N, box_first,
+  hipLaunchKernelGGL(k_agg<T>, dim3(nblk(S, 256), nbl, 4), dim3(256), 0, st, S, N, box_first,
                      box_count, perm, rt, x, C);
 }
 
 // translation_SS (:466-528): P[it][s] = sum_o C_child[it][s] * up[o][k][s]
-__global__ void k_ss(int S, const int *child, const zd *Cc, const zd *up, zd *P) {
+template <class T>
+__global__ void k_ss(int S, const int *child, const T *Cc, const T *up, T *P) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   const int b = blockIdx.y;
   if (s >= S) return;
-  zd acc[8];
-  for (int it = 0; it < 8; ++it) acc[it] = zd{0, 0};
+  T acc[8];
+  for (int it = 0; it < 8; ++it) acc[it] = T{0, 0};
   for (int o = 0; o < 8; ++o) {
     const int c = child[8 * b + o];
     if (c < 0) continue;
-    const zd u0 = up[((size_t)o * 2 + 0) * S + s], u1 = up[((size_t)o * 2 + 1) * S + s];
-    const zd *src = Cc + (size_t)c * 8 * S;
+    const T u0 = up[((size_t)o * 2 + 0) * S + s], u1 = up[((size_t)o * 2 + 1) * S + s];
+    const T *src = Cc + (size_t)c * 8 * S;
 #pragma unroll
     for (int it = 0; it < 8; ++it) acc[it] = acc[it] + src[(size_t)it * S + s] * ((it & 1) ? u1 : u0);
   }
-  zd *dst = P + (size_t)b * 8 * S;
+  T *dst = P + (size_t)b * 8 * S;
   for (int it = 0; it < 8; ++it) dst[(size_t)it * S + s] = acc[it];
 }
 
-void launch_ss(int nb, int S, const int *child, const zd *Cc, const zd *up, zd *P, hipStream_t st) {
+template <class T>
+void launch_ss(int nb, int S, const int *child, const T *Cc, const T *up, T *P, hipStream_t st) {
   if (nb <= 0) return;
-  hipLaunchKernelGGL(k_ss, dim3(nblk(S, 256), nb), dim3(256), 0, st, S, child, Cc, up, P);
+  hipLaunchKernelGGL(k_ss<T>, dim3(nblk(S, 256), nb), dim3(256), 0, st, S, child, Cc, up, P);
 }
 
 // translation_SR (:547-586): Dt[a][it][s] = sum_{b in I4(a)} B[b][it][s]*TLT[slot][k][s]
+template <class T>
 __global__ void k_m2l(int S, const int *i4_off, const int *i4_box, const int *i4_slot,
-                      const zd *B, const zd *tlt, zd *Dt) {
+                      const T *B, const T *tlt, T *Dt) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   const int a = blockIdx.y;
   if (s >= S) return;
-  zd acc[8];
-  for (int it = 0; it < 8; ++it) acc[it] = zd{0, 0};
+  T acc[8];
+  for (int it = 0; it < 8; ++it) acc[it] = T{0, 0};
   const int q0 = i4_off[a], q1 = i4_off[a + 1];
   for (int q = q0; q < q1; ++q) {
     const int b = i4_box[q], slot = i4_slot[q];
-    const zd t0 = tlt[((size_t)slot * 2 + 0) * S + s], t1 = tlt[((size_t)slot * 2 + 1) * S + s];
-    const zd *src = B + (size_t)b * 8 * S;
+    const T t0 = tlt[((size_t)slot * 2 + 0) * S + s], t1 = tlt[((size_t)slot * 2 + 1) * S + s];
+    const T *src = B + (size_t)b * 8 * S;
 #pragma unroll
     for (int it = 0; it < 8; ++it) acc[it] = acc[it] + src[(size_t)it * S + s] * ((it & 1) ? t1 : t0);
   }
-  zd *dst = Dt + (size_t)a * 8 * S;
+  T *dst = Dt + (size_t)a * 8 * S;
   for (int it = 0; it < 8; ++it) dst[(size_t)it * S + s] = acc[it];
 }
 
+template <class T>
 void launch_m2l(int nb, int S, const int *i4_off, const int *i4_box, const int *i4_slot,
-                const zd *B, const zd *tlt, zd *Dt, hipStream_t st) {
+                const T *B, const T *tlt, T *Dt, hipStream_t st) {
   if (nb <= 0) return;
-  hipLaunchKernelGGL(k_m2l, dim3(nblk(S, 128), nb), dim3(128), 0, st, S, i4_off, i4_box, i4_slot,
-                     B, tlt, Dt);
+  hipLaunchKernelGGL(k_m2l<T>, dim3(nblk(S, 128), nb), dim3(128), 0, st, S, i4_off, i4_box,
+                     i4_slot, B, tlt, Dt);
 }
 
 // translation_RR (:600-655): Dc[it][s] += Dp[it][s]*down[o][k][s]
-__global__ void k_rr(int S, const int *parent, const int *octant, const zd *Dp, const zd *down,
-                     zd *Dc) {
+template <class T>
+__global__ void k_rr(int S, const int *parent, const int *octant, const T *Dp, const T *down,
+                     T *Dc) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   const int b = blockIdx.y;
   if (s >= S) return;
   const int p = parent[b], o = octant[b];
-  const zd d0 = down[((size_t)o * 2 + 0) * S + s], d1 = down[((size_t)o * 2 + 1) * S + s];
-  const zd *src = Dp + (size_t)p * 8 * S;
-  zd *dst = Dc + (size_t)b * 8 * S;
+  const T d0 = down[((size_t)o * 2 + 0) * S + s], d1 = down[((size_t)o * 2 + 1) * S + s];
+  const T *src = Dp + (size_t)p * 8 * S;
+  T *dst = Dc + (size_t)b * 8 * S;
 #pragma unroll
   for (int it = 0; it < 8; ++it)
     dst[(size_t)it * S + s] = dst[(size_t)it * S + s] + src[(size_t)it * S + s] * ((it & 1) ? d1 : d0);
 }
 
-void launch_rr(int nb, int S, const int *parent, const int *octant, const zd *Dp, const zd *down,
-               zd *Dc, hipStream_t st) {
+template <class T>
+void launch_rr(int nb, int S, const int *parent, const int *octant, const T *Dp, const T *down,
+               T *Dc, hipStream_t st) {
   if (nb <= 0) return;
-  hipLaunchKernelGGL(k_rr, dim3(nblk(S, 256), nb), dim3(256), 0, st, S, parent, octant, Dp, down,
-                     Dc);
+  hipLaunchKernelGGL(k_rr<T>, dim3(nblk(S, 256), nb), dim3(256), 0, st, S, parent, octant, Dp,
+                     down, Dc);
 }
 
 // MLFMM_tri_summation_New (src/lib_sie_tri_mlfmm.f90:416-497) folded per box:
 //   y_e = sum_{k,s} fa*W1a + fb*W1b,  y_h = sum fa*W2a + fb*W2b
 // W1a = Dbh*kd+Dae*my, W1b = -Dah*kd+Dbe*my, W2a = -Dbe*kd+Dah*eps,
 // W2b = Dae*kd+Dbh*eps.  W layout [box][k*4 + {1a,1b,2a,2b}][S].
-__global__ void k_fold(int S, const zd *D, zd kd1, zd kd2, zd my1, zd my2, zd ep1, zd ep2, zd *W) {
+template <class T>
+__global__ void k_fold(int S, const T *D, T kd1, T kd2, T my1, T my2, T ep1, T ep2, T *W) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   const int b = blockIdx.y;
   if (s >= S) return;
-  const zd *Db = D + (size_t)b * 8 * S;
-  zd *Wb = W + (size_t)b * 8 * S;
+  const T *Db = D + (size_t)b * 8 * S;
+  T *Wb = W + (size_t)b * 8 * S;
   for (int k = 0; k < 2; ++k) {
-    const zd kd = k ? kd2 : kd1, my = k ? my2 : my1, ep = k ? ep2 : ep1;
-    const zd Dae = Db[(size_t)(0 + k) * S + s], Dah = Db[(size_t)(2 + k) * S + s];
-    const zd Dbe = Db[(size_t)(4 + k) * S + s], Dbh = Db[(size_t)(6 + k) * S + s];
+    const T kd = k ? kd2 : kd1, my = k ? my2 : my1, ep = k ? ep2 : ep1;
+    const T Dae = Db[(size_t)(0 + k) * S + s], Dah = Db[(size_t)(2 + k) * S + s];
+    const T Dbe = Db[(size_t)(4 + k) * S + s], Dbh = Db[(size_t)(6 + k) * S + s];
     Wb[(size_t)(k * 4 + 0) * S + s] = Dbh * kd + Dae * my;
     Wb[(size_t)(k * 4 + 1) * S + s] = -(Dah * kd) + Dbe * my;
     Wb[(size_t)(k * 4 + 2) * S + s] = -(Dbe * kd) + Dah * ep;
@@ -924,11 +937,12 @@ __global__ void k_fold(int S, const zd *D, zd kd1, zd kd2, zd my1, zd my2, zd ep
   }
 }
 
-void launch_fold(int nbl, int S, const zd *D, zd kd1, zd kd2, zd my1, zd my2, zd ep1, zd ep2,
-                 zd *W, hipStream_t st) {
+template <class T>
+void launch_fold(int nbl, int S, const T *D, zd kd1, zd kd2, zd my1, zd my2, zd ep1, zd ep2,
+                 T *W, hipStream_t st) {
   if (nbl <= 0) return;
-  hipLaunchKernelGGL(k_fold, dim3(nblk(S, 256), nbl), dim3(256), 0, st, S, D, kd1, kd2, my1, my2,
-                     ep1, ep2, W);
+  hipLaunchKernelGGL(k_fold<T>, dim3(nblk(S, 256), nbl), dim3(256), 0, st, S, D, cvt<T>(kd1),
+                     cvt<T>(kd2), cvt<T>(my1), cvt<T>(my2), cvt<T>(ep1), cvt<T>(ep2), W);
 }
 
 __device__ inline zd wave_sum(zd v) {
@@ -938,24 +952,32 @@ __device__ inline zd wave_sum(zd v) {
   }
   return v;
 }
+__device__ inline zf wave_sum(zf v) {
+  for (int off = 32; off > 0; off >>= 1) {
+    v.x += __shfl_down(v.x, off, 64);
+    v.y += __shfl_down(v.y, off, 64);
+  }
+  return v;
+}
 
 // one wavefront per position: y[m] = sc1*y_e, y[m+N] = sc2*y_h (far part)
+template <class T>
 __global__ void __launch_bounds__(256) k_disagg(int N, int S, const int *perm,
-                                                const int *leaf_box_of_pos, const zd *rt,
-                                                const zd *W, zd sc1, zd sc2, zd *y) {
+                                                const int *leaf_box_of_pos, const T *rt,
+                                                const T *W, T sc1, T sc2, T *y) {
   const int lane = threadIdx.x & 63;
   const int p = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (p >= N) return;
   const int b = leaf_box_of_pos[p];
-  const zd *f = rt + (size_t)p * 4 * S;
-  const zd *Wb = W + (size_t)b * 8 * S;
-  zd s1 = {0, 0}, s2 = {0, 0};
+  const T *f = rt + (size_t)p * 4 * S;
+  const T *Wb = W + (size_t)b * 8 * S;
+  T s1 = {0, 0}, s2 = {0, 0};
   for (int k = 0; k < 2; ++k) {
-    const zd *fa = f + (size_t)(k * 2 + 0) * S, *fb = f + (size_t)(k * 2 + 1) * S;
-    const zd *w1a = Wb + (size_t)(k * 4 + 0) * S, *w1b = Wb + (size_t)(k * 4 + 1) * S;
-    const zd *w2a = Wb + (size_t)(k * 4 + 2) * S, *w2b = Wb + (size_t)(k * 4 + 3) * S;
+    const T *fa = f + (size_t)(k * 2 + 0) * S, *fb = f + (size_t)(k * 2 + 1) * S;
+    const T *w1a = Wb + (size_t)(k * 4 + 0) * S, *w1b = Wb + (size_t)(k * 4 + 1) * S;
+    const T *w2a = Wb + (size_t)(k * 4 + 2) * S, *w2b = Wb + (size_t)(k * 4 + 3) * S;
     for (int s = lane; s < S; s += 64) {
-      const zd a = fa[s], bb = fb[s];
+      const T a = fa[s], bb = fb[s];
       s1 = s1 + a * w1a[s] + bb * w1b[s];
       s2 = s2 + a * w2a[s] + bb * w2b[s];
     }
@@ -969,11 +991,12 @@ __global__ void __launch_bounds__(256) k_disagg(int N, int S, const int *perm,
   }
 }
 
-void launch_disagg(int N, int S, const int *perm, const int *leaf_box_of_pos, const zd *rt,
-                   const zd *W, zd sc1, zd sc2, zd *y, hipStream_t st) {
+template <class T>
+void launch_disagg(int N, int S, const int *perm, const int *leaf_box_of_pos, const T *rt,
+                   const T *W, zd sc1, zd sc2, T *y, hipStream_t st) {
   if (N <= 0) return;
-  hipLaunchKernelGGL(k_disagg, dim3(nblk(N, 4)), dim3(256), 0, st, N, S, perm, leaf_box_of_pos,
-                     rt, W, sc1, sc2, y);
+  hipLaunchKernelGGL(k_disagg<T>, dim3(nblk(N, 4)), dim3(256), 0, st, N, S, perm,
+                     leaf_box_of_pos, rt, W, cvt<T>(sc1), cvt<T>(sc2), y);
 }
 
 // Near SpMV over the box-dense block matrices.  Workgroup = (leaf box,
@@ -982,19 +1005,20 @@ void launch_disagg(int N, int S, const int *perm, const int *leaf_box_of_pos, co
 constexpr int kRowsPerWG = kSpmvRowsPerWG;
 constexpr int kColChunk = 1024;
 
+template <class T>
 __global__ void __launch_bounds__(256) k_near_spmv(int N, const int *row_box, const int *row_r0,
                                                    const int *box_first, const int *box_count,
                                                    const int *e2_off, const int *e2_list,
                                                    const int64_t *near_off, const int *perm,
-                                                   const zd *blocks, const zd *x, const zd *yadd,
-                                                   zd *y) {
-  __shared__ zd xs[2 * kColChunk];
+                                                   const T *blocks, const T *x, const T *yadd,
+                                                   T *y) {
+  __shared__ T xs[2 * kColChunk];
   const int b = row_box[blockIdx.x], r0 = row_r0[blockIdx.x];
   const int rows = min(kRowsPerWG, box_count[b] - r0);
   const int c0 = e2_off[b], cols = e2_off[b + 1] - c0;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  zd ae[kRowsPerWG / 4], ah[kRowsPerWG / 4];
-  for (int q = 0; q < kRowsPerWG / 4; ++q) ae[q] = ah[q] = zd{0, 0};
+  T ae[kRowsPerWG / 4], ah[kRowsPerWG / 4];
+  for (int q = 0; q < kRowsPerWG / 4; ++q) ae[q] = ah[q] = T{0, 0};
   for (int cc = 0; cc < cols; cc += kColChunk) {
     const int nc = min(kColChunk, cols - cc);
     __syncthreads();
@@ -1007,10 +1031,10 @@ __global__ void __launch_bounds__(256) k_near_spmv(int N, const int *row_box, co
     for (int q = 0; q < kRowsPerWG / 4; ++q) {
       const int r = r0 + wv + 4 * q;
       if (r >= r0 + rows) break;
-      const zd *Z = blocks + 4 * (near_off[b] + (int64_t)(r - 0) * cols + cc);
+      const T *Z = blocks + 4 * (near_off[b] + (int64_t)(r - 0) * cols + cc);
       for (int c = lane; c < nc; c += 64) {
-        const zd z0 = Z[4 * c], z1 = Z[4 * c + 1], z2 = Z[4 * c + 2], z3 = Z[4 * c + 3];
-        const zd xe = xs[2 * c], xh = xs[2 * c + 1];
+        const T z0 = Z[4 * c], z1 = Z[4 * c + 1], z2 = Z[4 * c + 2], z3 = Z[4 * c + 3];
+        const T xe = xs[2 * c], xh = xs[2 * c + 1];
         ae[q] = ae[q] + z0 * xe + z1 * xh;
         ah[q] = ah[q] + z2 * xe + z3 * xh;
       }
@@ -1018,10 +1042,10 @@ __global__ void __launch_bounds__(256) k_near_spmv(int N, const int *row_box, co
   }
   for (int q = 0; q < kRowsPerWG / 4; ++q) {
     const int r = r0 + wv + 4 * q;
-    const zd se = wave_sum(ae[q]), sh = wave_sum(ah[q]);
+    const T se = wave_sum(ae[q]), sh = wave_sum(ah[q]);
     if (r < r0 + rows && lane == 0) {
       const int m = perm[box_first[b] + r];
-      zd oe = se, oh = sh;
+      T oe = se, oh = sh;
       if (yadd) {
         oe = oe + yadd[m];
         oh = oh + yadd[m + N];
@@ -1032,13 +1056,43 @@ __global__ void __launch_bounds__(256) k_near_spmv(int N, const int *row_box, co
   }
 }
 
+template <class T>
 void launch_near_spmv_rows(int nwg, int N, const int *row_box, const int *row_r0,
                            const int *box_first, const int *box_count, const int *e2_off,
                            const int *e2_list, const int64_t *near_off, const int *perm,
-                           const zd *blocks, const zd *x, const zd *yadd, zd *y, hipStream_t st) {
+                           const T *blocks, const T *x, const T *yadd, T *y, hipStream_t st) {
   if (nwg <= 0) return;
-  hipLaunchKernelGGL(k_near_spmv, dim3(nwg), dim3(256), 0, st, N, row_box, row_r0, box_first,
+  hipLaunchKernelGGL(k_near_spmv<T>, dim3(nwg), dim3(256), 0, st, N, row_box, row_r0, box_first,
                      box_count, e2_off, e2_list, near_off, perm, blocks, x, yadd, y);
+}
+
+#define SPK_MATVEC_INSTANTIATE(T)                                                               \
+  template void launch_agg<T>(int, int, int, const int *, const int *, const int *, const T *,   \
+                              const T *, T *, hipStream_t);                                      \
+  template void launch_ss<T>(int, int, const int *, const T *, const T *, T *, hipStream_t);     \
+  template void launch_m2l<T>(int, int, const int *, const int *, const int *, const T *,        \
+                              const T *, T *, hipStream_t);                                      \
+  template void launch_rr<T>(int, int, const int *, const int *, const T *, const T *, T *,      \
+                             hipStream_t);                                                       \
+  template void launch_fold<T>(int, int, const T *, zd, zd, zd, zd, zd, zd, T *, hipStream_t);   \
+  template void launch_disagg<T>(int, int, const int *, const int *, const T *, const T *, zd,   \
+                                 zd, T *, hipStream_t);                                          \
+  template void launch_near_spmv_rows<T>(int, int, const int *, const int *, const int *,        \
+                                         const int *, const int *, const int *, const int64_t *, \
+                                         const int *, const T *, const T *, const T *, T *,      \
+                                         hipStream_t);
+SPK_MATVEC_INSTANTIATE(zd)
+SPK_MATVEC_INSTANTIATE(zf)
+
+// complex128 -> complex64 copy (the mixed-precision operator's tables)
+__global__ void k_to_zf(int64_t n, const zd *a, zf *b) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = zf{(float)a[i].x, (float)a[i].y};
+}
+void launch_to_zf(int64_t n, const zd *a, zf *b, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_to_zf, dim3(gcap(n)), dim3(256), 0, st, n, a, b);
 }
 
 // ---------------------------------------------------------------------------
@@ -1168,11 +1222,12 @@ __device__ inline zd block_sum(zd v) {
   return t;
 }
 
-__global__ void k_dot_partial(int64_t n, const zd *x, const zd *y, zd *partial) {
+template <class T>
+__global__ void k_dot_partial(int64_t n, const T *x, const T *y, zd *partial) {
   zd acc = {0, 0};
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
-    acc = acc + conj(x[i]) * y[i];
+    acc = acc + conj(widen(x[i])) * widen(y[i]);
   acc = block_sum(acc);
   if (threadIdx.x == 0) partial[blockIdx.x] = acc;
 }
@@ -1182,9 +1237,40 @@ __global__ void k_reduce_final(int nparts, const zd *partial, zd *out) {
   acc = block_sum(acc);
   if (threadIdx.x == 0) *out = acc;
 }
-void launch_zdotc(int64_t n, const zd *x, const zd *y, zd *partial, zd *out, hipStream_t st) {
-  hipLaunchKernelGGL(k_dot_partial, dim3(kRedBlocks), dim3(256), 0, st, n, x, y, partial);
+template <class T>
+void launch_zdotc(int64_t n, const T *x, const T *y, zd *partial, zd *out, hipStream_t st) {
+  hipLaunchKernelGGL(k_dot_partial<T>, dim3(kRedBlocks), dim3(256), 0, st, n, x, y, partial);
   hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, st, kRedBlocks, partial, out);
+}
+template void launch_zdotc<zd>(int64_t, const zd *, const zd *, zd *, zd *, hipStream_t);
+template void launch_zdotc<zf>(int64_t, const zf *, const zf *, zd *, zd *, hipStream_t);
+
+// x += a * d (d complex64, x complex128)
+__global__ void k_axpy_up(int64_t n, zd a, const zf *d, zd *x) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    x[i] = x[i] + a * widen(d[i]);
+}
+void launch_axpy_up(int64_t n, zd a, const zf *d, zd *x, hipStream_t st) {
+  hipLaunchKernelGGL(k_axpy_up, dim3(gcap(n)), dim3(256), 0, st, n, a, d, x);
+}
+// y = s * x rounded to complex64
+__global__ void k_scale_down(int64_t n, double s, const zd *x, zf *y) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = zf{(float)(x[i].x * s), (float)(x[i].y * s)};
+}
+void launch_scale_down(int64_t n, double s, const zd *x, zf *y, hipStream_t st) {
+  hipLaunchKernelGGL(k_scale_down, dim3(gcap(n)), dim3(256), 0, st, n, s, x, y);
+}
+// y = s * x in complex64 (s computed in double)
+__global__ void k_scal_copy_f(int64_t n, double s, const zf *x, zf *y) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = zf{(float)(x[i].x * s), (float)(x[i].y * s)};
+}
+void launch_scal_copy_f(int64_t n, double s, const zf *x, zf *y, hipStream_t st) {
+  hipLaunchKernelGGL(k_scal_copy_f, dim3(gcap(n)), dim3(256), 0, st, n, s, x, y);
 }
 
 __global__ void k_zaxpy(int64_t n, zd a, const zd *x, zd *y) {
@@ -1409,60 +1495,97 @@ hipError_t launch_mgs_sweep(int G, int64_t n, int k, const zd *V, int64_t ldv, z
   return hipErrorInvalidValue;
 }
 
-// h[j] = <V_j, w> for j < k: partial[blk*k + j], then reduce per j.
-__global__ void k_gemv_c_partial(int64_t n, int k, const zd *V, int64_t ldv, const zd *w,
-                                 zd *partial) {
-  const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+// h[j] = <V_j, w> for j < k.  Each block owns a contiguous slice of rows and
+// accumulates a tile of 8 columns at a time (one read of its w slice per
+// tile, one 8-wide block reduction per tile): partial[j * G + block].  The
+// final kernel sums the G partials of each j in a fixed order.  Products
+// accumulate in complex128 whatever the storage type T.
+constexpr int kGemvTile = 8;
+template <class T>
+__global__ void __launch_bounds__(256) k_gemv_c_partial(int64_t n, int k, const T *V, int64_t ldv,
+                                                        const T *w, zd *partial) {
+  __shared__ zd red[4][kGemvTile];
+  const int G = gridDim.x, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int64_t chunk = (n + G - 1) / G;
   const int64_t i0 = blockIdx.x * chunk, i1 = min<int64_t>(n, i0 + chunk);
-  for (int j = 0; j < k; ++j) {
-    zd acc = {0, 0};
-    const zd *vj = V + (size_t)j * ldv;
-    for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) acc = acc + conj(vj[i]) * w[i];
-    acc = block_sum(acc);
-    if (threadIdx.x == 0) partial[(size_t)blockIdx.x * k + j] = acc;
+  for (int j0 = 0; j0 < k; j0 += kGemvTile) {
+    const int nt = min(kGemvTile, k - j0);
+    zd acc[kGemvTile];
+#pragma unroll
+    for (int t = 0; t < kGemvTile; ++t) acc[t] = zd{0, 0};
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+      const zd wi = widen(w[i]);
+#pragma unroll
+      for (int t = 0; t < kGemvTile; ++t)
+        if (t < nt) acc[t] = acc[t] + conj(widen(V[(size_t)(j0 + t) * ldv + i])) * wi;
+    }
+#pragma unroll
+    for (int t = 0; t < kGemvTile; ++t) {
+      const zd v = wave_sum(acc[t]);
+      if (lane == 0) red[wv][t] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < nt) {
+      const int t = threadIdx.x;
+      partial[(size_t)(j0 + t) * G + blockIdx.x] = red[0][t] + red[1][t] + red[2][t] + red[3][t];
+    }
+    __syncthreads();
   }
 }
 __global__ void k_gemv_c_final(int nparts, int k, const zd *partial, zd *out) {
   const int j = blockIdx.x;
   zd acc = {0, 0};
-  for (int i = threadIdx.x; i < nparts; i += blockDim.x) acc = acc + partial[(size_t)i * k + j];
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x) acc = acc + partial[(size_t)j * nparts + i];
   acc = block_sum(acc);
   if (threadIdx.x == 0) out[j] = acc;
 }
-void launch_gemv_c(int64_t n, int k, const zd *V, int64_t ldv, const zd *w, zd *partial, zd *out,
+template <class T>
+void launch_gemv_c(int64_t n, int k, const T *V, int64_t ldv, const T *w, zd *partial, zd *out,
                    hipStream_t st) {
   if (k <= 0) return;
-  hipLaunchKernelGGL(k_gemv_c_partial, dim3(kRedBlocks), dim3(256), 0, st, n, k, V, ldv, w,
+  hipLaunchKernelGGL(k_gemv_c_partial<T>, dim3(kRedBlocks), dim3(256), 0, st, n, k, V, ldv, w,
                      partial);
   hipLaunchKernelGGL(k_gemv_c_final, dim3(k), dim3(256), 0, st, kRedBlocks, k, partial, out);
 }
-__global__ void k_gemv_n_sub(int64_t n, int k, const zd *V, int64_t ldv, const zd *h, zd *w) {
+// w -= V h (products in complex128, stored as T)
+template <class T>
+__global__ void k_gemv_n_sub(int64_t n, int k, const T *V, int64_t ldv, const zd *h, T *w) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     zd acc = {0, 0};
-    for (int j = 0; j < k; ++j) acc = acc + V[(size_t)j * ldv + i] * h[j];
-    w[i] = w[i] - acc;
+    for (int j = 0; j < k; ++j) acc = acc + widen(V[(size_t)j * ldv + i]) * h[j];
+    w[i] = cvt<T>(widen(w[i]) - acc);
   }
 }
-void launch_gemv_n_sub(int64_t n, int k, const zd *V, int64_t ldv, const zd *h, zd *w,
+template <class T>
+void launch_gemv_n_sub(int64_t n, int k, const T *V, int64_t ldv, const zd *h, T *w,
                        hipStream_t st) {
   if (k <= 0) return;
-  hipLaunchKernelGGL(k_gemv_n_sub, dim3(gcap(n)), dim3(256), 0, st, n, k,
-                     V, ldv, h, w);
+  hipLaunchKernelGGL(k_gemv_n_sub<T>, dim3(gcap(n)), dim3(256), 0, st, n, k, V, ldv, h, w);
 }
-__global__ void k_gemv_n(int64_t n, int k, const zd *V, int64_t ldv, const zd *h, zd *out) {
+// out = V h (complex128 result)
+template <class T>
+__global__ void k_gemv_n(int64_t n, int k, const T *V, int64_t ldv, const zd *h, zd *out) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     zd acc = {0, 0};
-    for (int j = 0; j < k; ++j) acc = acc + V[(size_t)j * ldv + i] * h[j];
+    for (int j = 0; j < k; ++j) acc = acc + widen(V[(size_t)j * ldv + i]) * h[j];
     out[i] = acc;
   }
 }
-void launch_gemv_n(int64_t n, int k, const zd *V, int64_t ldv, const zd *h, zd *out,
+template <class T>
+void launch_gemv_n(int64_t n, int k, const T *V, int64_t ldv, const zd *h, zd *out,
                    hipStream_t st) {
-  hipLaunchKernelGGL(k_gemv_n, dim3(gcap(n)), dim3(256), 0, st, n, k, V,
-                     ldv, h, out);
+  hipLaunchKernelGGL(k_gemv_n<T>, dim3(gcap(n)), dim3(256), 0, st, n, k, V, ldv, h, out);
 }
+#define SPK_GEMV_INSTANTIATE(T)                                                                  \
+  template void launch_gemv_c<T>(int64_t, int, const T *, int64_t, const T *, zd *, zd *,         \
+                                 hipStream_t);                                                    \
+  template void launch_gemv_n_sub<T>(int64_t, int, const T *, int64_t, const zd *, T *,           \
+                                     hipStream_t);                                                \
+  template void launch_gemv_n<T>(int64_t, int, const T *, int64_t, const zd *, zd *, hipStream_t);
+SPK_GEMV_INSTANTIATE(zd)
+SPK_GEMV_INSTANTIATE(zf)
 
 void launch_scattered(const ScatArgs &a, const FnRec *fn, const zd *x, const double *r, zd *E,
                       zd *H, hipStream_t st) {
