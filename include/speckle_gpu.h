@@ -41,7 +41,7 @@ extern "C" {
 
 /* Arithmetic of the MLFMM far field / inner solve. */
 #define SPK_FP64 0
-#define SPK_MIXED 1 /* complex64 far field + inner GMRES, FP64 refinement */
+#define SPK_MIXED 1 /* complex64 operator + inner GMRES, FP64 refinement (no counterpart in the reference) */
 
 /* Orthogonalisation, lib_math_solver_GMRES.f90 (icntl(5)). */
 #define SPK_ORTH_MGS 0
@@ -75,8 +75,9 @@ typedef struct spk_gmres_params {
   double tolerance;              /* cntl(1), Arnoldi backward error */
   int orthogonalization;         /* SPK_ORTH_* */
   int use_initial_guess;         /* icntl(6) */
-  int refinement_steps;          /* SPK_MIXED: outer FP64 refinement sweeps */
-  double refinement_tolerance;   /* SPK_MIXED: target FP64 residual */
+  int refinement_steps;          /* SPK_MIXED: maximum outer refinement steps (0: 30) */
+  double refinement_tolerance;   /* SPK_MIXED: relative residual of each inner complex64
+                                    solve (0: 1e-4); tolerance is the FP64 target */
 } spk_gmres_params;
 
 typedef struct spk_gmres_info {
@@ -130,6 +131,12 @@ int spk_num_edges(const spk_handle *h, int64_t *n_edges);
 int spk_matvec(spk_handle *h, const double *x, double *y);
 /* Same on device buffers (HBM-resident), stream 0 of the handle. */
 int spk_matvec_device(spk_handle *h, const void *d_x, void *d_y);
+/* The complex64 operator of an SPK_MIXED handle (the inner operator of the
+ * mixed solve): y = Z x with near blocks, patterns and tables rounded to
+ * complex64; vectors of 2N interleaved float pairs.  No counterpart in the
+ * reference (which is FP64 only). */
+int spk_matvec_c64(spk_handle *h, const float *x, float *y);
+int spk_matvec_c64_device(spk_handle *h, const void *d_x, void *d_y);
 /* Far field only / near field only (parity probes, timing splits). */
 int spk_matvec_parts(spk_handle *h, const double *x, double *y_near, double *y_far);
 
@@ -188,6 +195,9 @@ int spk_get_patterns(const spk_handle *h, int first, int count, double *out);
  * 4 fold, 5 disaggregation, 6 near-field SpMV, 7 whole matvec. */
 #define SPK_NSTAGES 8
 int spk_profile_matvec(spk_handle *h, int reps, double *ms, double *bytes);
+/* Same for the complex64 operator of an SPK_MIXED handle (bytes at 8 B per
+ * complex). */
+int spk_profile_matvec_c64(spk_handle *h, int reps, double *ms, double *bytes);
 
 /* Host-only setup probe (no GPU needed): RWG topology and octree of a mesh
  * exactly as spk_setup_tri builds them.  corner/element: N x 2 (1-based),

@@ -16,12 +16,13 @@ LIB_PATH = os.path.join(HERE, "libspeckle_gpu.so")
 # Exported entry points, in include/speckle_gpu.h order.
 EXPORTS = [
     "spk_create", "spk_destroy", "spk_last_error", "spk_setup_tri", "spk_get_stats",
-    "spk_num_edges", "spk_matvec", "spk_matvec_device", "spk_matvec_parts", "spk_rhs_plane_wave",
+    "spk_num_edges", "spk_matvec", "spk_matvec_device", "spk_matvec_c64",
+    "spk_matvec_c64_device", "spk_matvec_parts", "spk_rhs_plane_wave",
     "spk_rhs_plane_wave_device", "spk_scattered_field", "spk_scattered_field_device",
     "spk_device_alloc", "spk_device_free", "spk_device_copy", "spk_device_synchronize",
     "spk_gmres", "spk_gmres_device", "spk_gmres_history", "spk_get_rwg", "spk_get_tree",
     "spk_get_i4", "spk_get_near_csr", "spk_near_blocks", "spk_get_patterns",
-    "spk_profile_matvec", "spk_topology_probe",
+    "spk_profile_matvec", "spk_profile_matvec_c64", "spk_topology_probe",
 ]
 STAGES = ["aggregation", "ss", "m2l", "rr", "fold", "disaggregation", "near_spmv", "matvec"]
 
@@ -82,6 +83,8 @@ def lib():
     L.spk_num_edges.argtypes = [vp, i64p]
     L.spk_matvec.argtypes = [vp, dp, dp]
     L.spk_matvec_device.argtypes = [vp, vp, vp]
+    L.spk_matvec_c64.argtypes = [vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    L.spk_matvec_c64_device.argtypes = [vp, vp, vp]
     L.spk_matvec_parts.argtypes = [vp, dp, dp, dp]
     L.spk_rhs_plane_wave.argtypes = [vp, dp, dp, dp]
     L.spk_rhs_plane_wave_device.argtypes = [vp, dp, dp, vp]
@@ -101,6 +104,7 @@ def lib():
     L.spk_near_blocks.argtypes = [vp, C.c_int, ip, ip, dp]
     L.spk_get_patterns.argtypes = [vp, C.c_int, C.c_int, dp]
     L.spk_profile_matvec.argtypes = [vp, C.c_int, dp, dp]
+    L.spk_profile_matvec_c64.argtypes = [vp, C.c_int, dp, dp]
     L.spk_topology_probe.argtypes = [C.c_int, dp, C.c_int, ip, C.c_int, C.c_int, i64p, ip, ip,
                                      i64p, ip, ip, dp, dp]
     _lib = L

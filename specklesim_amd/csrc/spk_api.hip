@@ -13,6 +13,7 @@
 #include <memory>
 #include <string>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 #include "spk_device.h"
@@ -66,6 +67,7 @@ struct DBuf {
 
 struct LevelDev {
   DBuf<zd> B, D, tlt, up, down;
+  DBuf<zf> fB, fD, ftlt, fup, fdown;  // complex64 mirror (mixed precision)
   DBuf<int> child, parent, octant, i4_off, i4_box, i4_slot;
   int nb = 0;
 };
@@ -101,6 +103,14 @@ struct spk_handle {
   std::vector<LevelDev> lev;
   DBuf<zd> d_x, d_y, d_yfar, d_tmp;
   zd sc1{1, 0}, sc2{1, 0};
+  // complex64 mirror of the operator (precision == SPK_MIXED): near blocks,
+  // patterns, folded W and the far-part scratch
+  DBuf<zf> f_blocks, f_rt, f_W, f_yfar, f_x, f_y;
+  bool mixed_ready = false;
+  // mixed-precision inner solver workspace
+  DBuf<zf> i_V, i_w;
+  int i_m = 0;
+  int refinements = 0;  // outer steps of the last mixed solve
   // gmres workspace
   DBuf<zd> g_V, g_w, g_r0, g_xc, g_b, g_h, g_partial, g_scal, g_y;
   DBuf<unsigned long long> g_ctl;    // MGS sweep abort flag
@@ -266,58 +276,118 @@ int setup_mlfmm(spk_handle *h) {
   return SPK_OK;
 }
 
+// Buffers of the operator in complex128 (zd) or its complex64 mirror (zf).
+static zd *pB(LevelDev &L, zd *) { return L.B.p; }
+static zf *pB(LevelDev &L, zf *) { return L.fB.p; }
+static zd *pD(LevelDev &L, zd *) { return L.D.p; }
+static zf *pD(LevelDev &L, zf *) { return L.fD.p; }
+static const zd *pT(LevelDev &L, zd *) { return L.tlt.p; }
+static const zf *pT(LevelDev &L, zf *) { return L.ftlt.p; }
+static const zd *pUp(LevelDev &L, zd *) { return L.up.p; }
+static const zf *pUp(LevelDev &L, zf *) { return L.fup.p; }
+static const zd *pDn(LevelDev &L, zd *) { return L.down.p; }
+static const zf *pDn(LevelDev &L, zf *) { return L.fdown.p; }
+static const zd *pRt(spk_handle *h, zd *) { return h->d_rt.p; }
+static const zf *pRt(spk_handle *h, zf *) { return h->f_rt.p; }
+static zd *pW(spk_handle *h, zd *) { return h->d_W.p; }
+static zf *pW(spk_handle *h, zf *) { return h->f_W.p; }
+static const zd *pBlk(spk_handle *h, zd *) { return h->d_blocks.p; }
+static const zf *pBlk(spk_handle *h, zf *) { return h->f_blocks.p; }
+static zd *pYf(spk_handle *h, zd *) { return h->d_yfar.p; }
+static zf *pYf(spk_handle *h, zf *) { return h->f_yfar.p; }
+
 // y = Z x on device vectors (2N complex each), far part into yfar first.
+// T = zd: the FP64 operator; T = zf: its complex64 mirror.
 // ev (optional, 8 events): recorded before stage 0 and after stages 0..6.
-int matvec_dev(spk_handle *h, const zd *x, zd *y, bool near_on = true, bool far_on = true,
-               hipEvent_t *ev = nullptr) {
+template <class T>
+int matvec_t(spk_handle *h, const T *x, T *y, bool near_on = true, bool far_on = true,
+             hipEvent_t *ev = nullptr) {
+  T *tag = nullptr;
   const Tree &t = h->tree;
   const int S = h->tb.S, N = h->N, lmax = t.l_max, lmin = t.l_min;
   hipStream_t st = h->st;
-  zd *yfar = h->d_yfar.p;
+  T *yfar = pYf(h, tag);
   auto mark = [&](int i) {
     if (ev) (void)hipEventRecord(ev[i], st);
   };
   mark(0);
   if (far_on) {
     LevelDev &LL = h->lev[lmax];
-    launch_agg(LL.nb, S, N, h->d_box_first.p, h->d_box_count.p, h->d_perm.p, h->d_rt.p, x, LL.B.p,
-               st);
+    launch_agg(LL.nb, S, N, h->d_box_first.p, h->d_box_count.p, h->d_perm.p, pRt(h, tag), x,
+               pB(LL, tag), st);
     mark(1);
     for (int l = lmax - 1; l >= lmin; --l)
-      launch_ss(h->lev[l].nb, S, h->lev[l].child.p, h->lev[l + 1].B.p, h->lev[l + 1].up.p,
-                h->lev[l].B.p, st);
+      launch_ss(h->lev[l].nb, S, h->lev[l].child.p, (const T *)pB(h->lev[l + 1], tag),
+                pUp(h->lev[l + 1], tag), pB(h->lev[l], tag), st);
     mark(2);
     for (int l = lmin; l <= lmax; ++l)
       launch_m2l(h->lev[l].nb, S, h->lev[l].i4_off.p, h->lev[l].i4_box.p, h->lev[l].i4_slot.p,
-                 h->lev[l].B.p, h->lev[l].tlt.p, h->lev[l].D.p, st);
+                 (const T *)pB(h->lev[l], tag), pT(h->lev[l], tag), pD(h->lev[l], tag), st);
     mark(3);
     for (int l = lmin + 1; l <= lmax; ++l)
-      launch_rr(h->lev[l].nb, S, h->lev[l].parent.p, h->lev[l].octant.p, h->lev[l - 1].D.p,
-                h->lev[l].down.p, h->lev[l].D.p, st);
+      launch_rr(h->lev[l].nb, S, h->lev[l].parent.p, h->lev[l].octant.p,
+                (const T *)pD(h->lev[l - 1], tag), pDn(h->lev[l], tag), pD(h->lev[l], tag), st);
     mark(4);
     const zd I = {0, 1};
     const zd kd1 = I * zcx(h->mat.k1), kd2 = I * zcx(h->mat.k2);
     const zd my1 = zd{h->mat.my1, 0} * I * h->mat.omega, my2 = zd{h->mat.my2, 0} * I * h->mat.omega;
     const zd ep1 = zcx(h->mat.eps1) * I * h->mat.omega, ep2 = zcx(h->mat.eps2) * I * h->mat.omega;
-    launch_fold(LL.nb, S, LL.D.p, kd1, kd2, my1, my2, ep1, ep2, h->d_W.p, st);
+    launch_fold(LL.nb, S, (const T *)pD(LL, tag), kd1, kd2, my1, my2, ep1, ep2, pW(h, tag), st);
     mark(5);
-    launch_disagg(N, S, h->d_perm.p, h->d_leafbox_of_pos.p, h->d_rt.p, h->d_W.p, h->sc1, h->sc2,
-                  near_on ? yfar : y, st);
+    launch_disagg(N, S, h->d_perm.p, h->d_leafbox_of_pos.p, pRt(h, tag), (const T *)pW(h, tag),
+                  h->sc1, h->sc2, near_on ? yfar : y, st);
     mark(6);
   }
   if (near_on)
     launch_near_spmv_rows(h->spmv_wgs, N, h->d_row_box.p, h->d_row_r0.p, h->d_box_first.p,
                           h->d_box_count.p, h->d_e2_off.p, h->d_e2_list.p, h->d_near_off.p,
-                          h->d_perm.p, h->d_blocks.p, x, far_on ? yfar : nullptr, y, st);
+                          h->d_perm.p, pBlk(h, tag), x, far_on ? (const T *)yfar : nullptr, y,
+                          st);
   mark(7);
   HIPCHK(hipGetLastError());
   return SPK_OK;
 }
 
-// algorithmic HBM bytes of each matvec stage (FP64, 16-byte complex)
-void stage_bytes(const spk_handle *h, double *b) {
+int matvec_dev(spk_handle *h, const zd *x, zd *y, bool near_on = true, bool far_on = true,
+               hipEvent_t *ev = nullptr) {
+  return matvec_t<zd>(h, x, y, near_on, far_on, ev);
+}
+
+// complex64 mirror of the operator: the near blocks, patterns, translation
+// and phase tables rounded once from their complex128 values.
+int build_mixed_mirror(spk_handle *h) {
+  const size_t S = h->tb.S;
   const Tree &t = h->tree;
-  const double S = h->tb.S, N = h->N, c = 16.0;
+  HIPCHK(h->f_blocks.alloc(h->d_blocks.n));
+  launch_to_zf((int64_t)h->d_blocks.n, h->d_blocks.p, h->f_blocks.p, h->st);
+  HIPCHK(h->f_rt.alloc(h->d_rt.n));
+  launch_to_zf((int64_t)h->d_rt.n, h->d_rt.p, h->f_rt.p, h->st);
+  HIPCHK(h->f_W.alloc(h->d_W.n));
+  HIPCHK(h->f_yfar.alloc(2 * (size_t)h->N));
+  HIPCHK(h->f_x.alloc(2 * (size_t)h->N));
+  HIPCHK(h->f_y.alloc(2 * (size_t)h->N));
+  for (int l = t.l_min; l <= t.l_max; ++l) {
+    LevelDev &D = h->lev[l];
+    HIPCHK(D.fB.alloc(D.B.n));
+    HIPCHK(D.fD.alloc(D.D.n));
+    HIPCHK(D.ftlt.alloc(D.tlt.n));
+    launch_to_zf((int64_t)D.tlt.n, D.tlt.p, D.ftlt.p, h->st);
+    HIPCHK(D.fup.alloc(D.up.n));
+    launch_to_zf((int64_t)D.up.n, D.up.p, D.fup.p, h->st);
+    HIPCHK(D.fdown.alloc(D.down.n));
+    launch_to_zf((int64_t)D.down.n, D.down.p, D.fdown.p, h->st);
+  }
+  (void)S;
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(h->st));
+  h->mixed_ready = true;
+  return SPK_OK;
+}
+
+// algorithmic HBM bytes of each matvec stage (FP64, 16-byte complex)
+void stage_bytes(const spk_handle *h, double *b, double c = 16.0) {
+  const Tree &t = h->tree;
+  const double S = h->tb.S, N = h->N;
   const double nbl = (double)t.lev[t.l_max].n.size();
   b[0] = N * 4 * S * c + 2 * N * c + nbl * 8 * S * c;  // patterns + x + C
   b[1] = 0;
@@ -376,6 +446,7 @@ int spk_setup_tri(spk_handle *h, int np, const double *pp, int nt, const int *tt
   if (!h || !pp || !tt || np < 3 || nt < 1) return fail(SPK_ERR_ARG, "bad mesh arguments");
   HIPCHK(hipSetDevice(h->p.device));
   h->ready = false;
+  h->mixed_ready = false;
   h->stats = spk_stats{};
   const double t0 = now_ms();
   std::string err;
@@ -428,6 +499,10 @@ int spk_setup_tri(spk_handle *h, int np, const double *pp, int nt, const int *tt
   h->stats.samples = h->tb.S;
   h->stats.bytes_near = (double)t.nnz * 64.0;
   h->stats.bytes_patterns = (double)h->N * 4 * h->tb.S * 16.0;
+  if (h->p.precision == SPK_MIXED) {
+    rc = build_mixed_mirror(h);
+    if (rc != SPK_OK) return rc;
+  }
   h->stats.setup_ms = now_ms() - t0;
   h->ready = true;
   return SPK_OK;
@@ -482,6 +557,29 @@ int spk_matvec_device(spk_handle *h, const void *d_x, void *d_y) {
   if (!h || !d_x || !d_y) return fail(SPK_ERR_ARG, "null argument");
   if (!h->ready) return fail(SPK_ERR_STATE, "spk_setup_tri has not completed");
   return matvec_dev(h, (const zd *)d_x, (zd *)d_y);
+}
+
+int spk_matvec_c64_device(spk_handle *h, const void *d_x, void *d_y) {
+  if (!h || !d_x || !d_y) return fail(SPK_ERR_ARG, "null argument");
+  if (!h->ready) return fail(SPK_ERR_STATE, "spk_setup_tri has not completed");
+  if (!h->mixed_ready) return fail(SPK_ERR_STATE, "handle was not created with SPK_MIXED");
+  int rc = matvec_t<zf>(h, (const zf *)d_x, (zf *)d_y);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(h->st));
+  return SPK_OK;
+}
+
+int spk_matvec_c64(spk_handle *h, const float *x, float *y) {
+  if (!h || !x || !y) return fail(SPK_ERR_ARG, "null argument");
+  if (!h->ready) return fail(SPK_ERR_STATE, "spk_setup_tri has not completed");
+  if (!h->mixed_ready) return fail(SPK_ERR_STATE, "handle was not created with SPK_MIXED");
+  const size_t bytes = 2 * (size_t)h->N * sizeof(zf);
+  HIPCHK(hipMemcpyAsync(h->f_x.p, x, bytes, hipMemcpyHostToDevice, h->st));
+  int rc = matvec_t<zf>(h, h->f_x.p, h->f_y.p);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(y, h->f_y.p, bytes, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  return SPK_OK;
 }
 
 int spk_matvec(spk_handle *h, const double *x, double *y) {
@@ -667,10 +765,176 @@ static int dev_dot(spk_handle *h, const zd *x, const zd *y, int64_t n, zc *out) 
   return SPK_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Mixed precision: iterative refinement around a complex64 inner GMRES.
+//
+// Outer loop (complex128): r = b - Z x with the FP64 operator; stop when
+// |r| / |b| <= tolerance.  Inner solve: GMRES on the complex64 operator for
+// Z d = r / |r|, Krylov basis stored in complex64, Gram-Schmidt by CGS with
+// the CERFACS reorthogonalisation test (ICGS), dot products accumulated in
+// complex128 and the Hessenberg matrix and Givens rotations kept in
+// complex128 on the host; it stops at the relative residual
+// refinement_tolerance (default 1e-4).  x += |r| d.  No counterpart in the
+// reference, which computes in FP64 only.
+// ---------------------------------------------------------------------------
+static int gmres_mixed(spk_handle *h, const zd *b, zd *x, const spk_gmres_params *gp,
+                       spk_gmres_info *info) {
+  if (!h->mixed_ready) return fail(SPK_ERR_STATE, "handle was not created with SPK_MIXED");
+  const double t_start = now_ms();
+  const int64_t n = 2 * (int64_t)h->N;
+  const int max_it = std::max(1, gp->max_iterations);
+  const int m = gp->restart > 0 ? std::min(gp->restart, max_it) : max_it;
+  const double tol = gp->tolerance;
+  const double inner_tol = gp->refinement_tolerance > 0 ? gp->refinement_tolerance : 1e-4;
+  const int max_outer = gp->refinement_steps > 0 ? gp->refinement_steps : 30;
+  if (h->i_m != m || h->i_V.n != (size_t)n * (m + 1)) {
+    HIPCHK(h->i_V.alloc((size_t)n * (m + 1)));
+    h->i_m = m;
+  }
+  if (h->i_w.n != (size_t)n) HIPCHK(h->i_w.alloc(n));
+  if (h->g_w.n != (size_t)n) {
+    HIPCHK(h->g_w.alloc(n));
+    HIPCHK(h->g_r0.alloc(n));
+    HIPCHK(h->g_xc.alloc(n));
+  }
+  HIPCHK(h->g_h.alloc(m + 1));
+  HIPCHK(h->g_y.alloc(m + 1));
+  HIPCHK(h->g_partial.alloc((size_t)reduce_blocks(n) * (m + 1)));
+  HIPCHK(h->g_scal.alloc(4));
+  zf *V = h->i_V.p, *w = h->i_w.p;
+  zd *r = h->g_r0.p, *tmp = h->g_w.p, *d = h->g_xc.p;
+  zd *partial_nrm = h->g_partial.p + (size_t)reduce_blocks(n) * m;
+  h->hist.clear();
+  int rc = SPK_OK;
+  auto dot_host = [&](const auto *u, const auto *v, double *out) -> int {
+    launch_zdotc(n, u, v, h->g_partial.p, h->g_scal.p, h->st);
+    zd t;
+    HIPCHK(hipMemcpyAsync(&t, h->g_scal.p, sizeof(zd), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    *out = t.x;
+    return SPK_OK;
+  };
+  if (!gp->use_initial_guess) HIPCHK(hipMemsetAsync(x, 0, n * sizeof(zd), h->st));
+  double bb;
+  if ((rc = dot_host(b, b, &bb))) return rc;
+  const double bn = std::sqrt(bb);
+  int matvecs = 0, total = 0, outer = 0;
+  double be = 0.0, bea = 0.0;
+  if (bn == 0.0) {
+    HIPCHK(hipMemsetAsync(x, 0, n * sizeof(zd), h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    if (info) *info = spk_gmres_info{0, 0, 0.0, 0.0, 0, (now_ms() - t_start) / 1e3};
+    return SPK_OK;
+  }
+  auto residual = [&](double *rn) -> int {
+    int e = matvec_dev(h, x, tmp);
+    if (e) return e;
+    ++matvecs;
+    launch_zsub(n, b, tmp, r, h->st);
+    double rr;
+    if ((e = dot_host(r, r, &rr))) return e;
+    *rn = std::sqrt(rr);
+    return SPK_OK;
+  };
+  double rn;
+  if ((rc = residual(&rn))) return rc;
+  be = rn / bn;
+  bea = be;
+  std::vector<zc> H((size_t)(m + 1) * (m + 1)), g(m + 1), rs(m + 1), yc(m + 1);
+  std::vector<double> rcs(m + 1);
+  auto HH = [&](int i, int j) -> zc & { return H[(size_t)j * (m + 1) + i]; };
+  while (be > tol && total < max_it && outer < max_outer) {
+    // V_0 = (r / |r|) in complex64, renormalised after rounding
+    launch_scale_down(n, 1.0 / rn, r, V, h->st);
+    double vv;
+    if ((rc = dot_host(V, V, &vv))) return rc;
+    const double beta = std::sqrt(vv);
+    launch_scal_copy_f(n, 1.0 / beta, V, V, h->st);
+    std::fill(H.begin(), H.end(), zc(0.0));
+    std::fill(g.begin(), g.end(), zc(0.0));
+    g[0] = beta;
+    const int jmax = std::min(m, max_it - total);
+    int k = 0;
+    double est = 1.0;
+    for (int j = 0; j < jmax; ++j) {
+      if ((rc = matvec_t<zf>(h, V + (size_t)j * n, w))) return rc;
+      ++matvecs;
+      int nOrtho = 0;
+      double dnormw = 0.0;
+      for (;;) {
+        ++nOrtho;
+        launch_gemv_c(n, j + 1, (const zf *)V, n, (const zf *)w, h->g_partial.p, h->g_h.p, h->st);
+        launch_gemv_n_sub(n, j + 1, (const zf *)V, n, h->g_h.p, w, h->st);
+        launch_zdotc(n, (const zf *)w, (const zf *)w, partial_nrm, h->g_scal.p, h->st);
+        std::vector<zd> hv(j + 2);
+        HIPCHK(hipMemcpyAsync(hv.data(), h->g_h.p, (j + 1) * sizeof(zd), hipMemcpyDeviceToHost,
+                              h->st));
+        HIPCHK(hipMemcpyAsync(&hv[j + 1], h->g_scal.p, sizeof(zd), hipMemcpyDeviceToHost, h->st));
+        HIPCHK(hipStreamSynchronize(h->st));
+        double dloo = 0.0;
+        for (int i = 0; i <= j; ++i) {
+          const zc hi(hv[i].x, hv[i].y);
+          HH(i, j) += hi;
+          dloo += std::norm(hi);
+        }
+        dloo = std::sqrt(dloo);
+        dnormw = std::sqrt(std::max(hv[j + 1].x, 0.0));
+        if (dnormw <= dloo && nOrtho < 3) continue;
+        break;
+      }
+      HH(j + 1, j) = dnormw;
+      if (dnormw > 0.0) launch_scal_copy_f(n, 1.0 / dnormw, w, V + (size_t)(j + 1) * n, h->st);
+      for (int i = 0; i < j; ++i) zrot1(&HH(i, j), &HH(i + 1, j), rcs[i], rs[i]);
+      zc a = HH(j, j);
+      double cc;
+      zc ss;
+      zrotg_(&a, HH(j + 1, j), &cc, &ss);
+      rcs[j] = cc;
+      rs[j] = ss;
+      zrot1(&g[j], &g[j + 1], cc, ss);
+      zrot1(&HH(j, j), &HH(j + 1, j), cc, ss);
+      HH(j + 1, j) = 0.0;
+      est = std::abs(g[j + 1]) / beta;
+      ++total;
+      k = j + 1;
+      h->hist.push_back(est * be);
+      if (est <= inner_tol || dnormw == 0.0) break;
+    }
+    bea = est * be;
+    // y = H^-1 g, d = V y, x += |r| d
+    for (int j = k - 1; j >= 0; --j) {
+      zc sum = g[j];
+      for (int i = j + 1; i < k; ++i) sum -= HH(j, i) * yc[i];
+      yc[j] = sum / HH(j, j);
+    }
+    std::vector<zd> yd(k);
+    for (int j = 0; j < k; ++j) yd[j] = zd{yc[j].real(), yc[j].imag()};
+    HIPCHK(hipMemcpyAsync(h->g_y.p, yd.data(), k * sizeof(zd), hipMemcpyHostToDevice, h->st));
+    launch_gemv_n(n, k, (const zf *)V, n, h->g_y.p, d, h->st);
+    launch_zaxpy(n, zd{rn, 0.0}, d, x, h->st);
+    if ((rc = residual(&rn))) return rc;
+    be = rn / bn;
+    ++outer;
+  }
+  HIPCHK(hipStreamSynchronize(h->st));
+  HIPCHK(hipGetLastError());
+  if (info) {
+    info->info = be <= tol ? 0 : -4;
+    info->iterations = total;
+    info->arnoldi_backward_error = bea;
+    info->backward_error = be;
+    info->matvecs = matvecs;
+    info->seconds = (now_ms() - t_start) / 1e3;
+  }
+  h->refinements = outer;
+  return SPK_OK;
+}
+
 int spk_gmres_device(spk_handle *h, const void *d_b, void *d_x, const spk_gmres_params *gp,
                      spk_gmres_info *info) {
   if (!h || !d_b || !d_x || !gp) return fail(SPK_ERR_ARG, "null argument");
   if (!h->ready) return fail(SPK_ERR_STATE, "spk_setup_tri has not completed");
+  if (h->p.precision == SPK_MIXED) return gmres_mixed(h, (const zd *)d_b, (zd *)d_x, gp, info);
   const double t_start = now_ms();
   const int64_t n = 2 * (int64_t)h->N;
   const int iterMax = std::max(1, gp->max_iterations);
@@ -1008,15 +1272,23 @@ int spk_get_patterns(const spk_handle *h, int first, int count, double *out) {
   return SPK_OK;
 }
 
-int spk_profile_matvec(spk_handle *h, int reps, double *ms, double *bytes) {
-  if (!h || reps < 1) return fail(SPK_ERR_ARG, "bad arguments");
-  if (!h->ready) return fail(SPK_ERR_STATE, "spk_setup_tri has not completed");
+extern "C++" {
+template <class T>
+static int profile_matvec_t(spk_handle *h, int reps, double *ms, double *bytes) {
   hipEvent_t ev[8];
   for (int i = 0; i < 8; ++i) HIPCHK(hipEventCreate(&ev[i]));
   double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   int rc = SPK_OK;
+  T *x = nullptr, *y = nullptr;
+  if constexpr (std::is_same<T, zd>::value) {
+    x = h->d_x.p;
+    y = h->d_y.p;
+  } else {
+    x = h->f_x.p;
+    y = h->f_y.p;
+  }
   for (int r = 0; r < reps && rc == SPK_OK; ++r) {
-    rc = matvec_dev(h, h->d_x.p, h->d_y.p, true, true, ev);
+    rc = matvec_t<T>(h, x, y, true, true, ev);
     if (rc) break;
     HIPCHK(hipEventSynchronize(ev[7]));
     float t;
@@ -1031,8 +1303,22 @@ int spk_profile_matvec(spk_handle *h, int reps, double *ms, double *bytes) {
   if (rc) return rc;
   if (ms)
     for (int i = 0; i < 8; ++i) ms[i] = acc[i] / reps;
-  if (bytes) stage_bytes(h, bytes);
+  if (bytes) stage_bytes(h, bytes, (double)sizeof(T));
   return SPK_OK;
+}
+}  // extern "C++"
+
+int spk_profile_matvec(spk_handle *h, int reps, double *ms, double *bytes) {
+  if (!h || reps < 1) return fail(SPK_ERR_ARG, "bad arguments");
+  if (!h->ready) return fail(SPK_ERR_STATE, "spk_setup_tri has not completed");
+  return profile_matvec_t<zd>(h, reps, ms, bytes);
+}
+
+int spk_profile_matvec_c64(spk_handle *h, int reps, double *ms, double *bytes) {
+  if (!h || reps < 1) return fail(SPK_ERR_ARG, "bad arguments");
+  if (!h->ready) return fail(SPK_ERR_STATE, "spk_setup_tri has not completed");
+  if (!h->mixed_ready) return fail(SPK_ERR_STATE, "handle was not created with SPK_MIXED");
+  return profile_matvec_t<zf>(h, reps, ms, bytes);
 }
 
 int spk_topology_probe(int np, const double *pp, int nt, const int *tt, int normal_convention,

@@ -99,6 +99,17 @@ class SieTriMlfmm:
         nat.check(nat.lib().spk_matvec(self._h, _dp(x.view(np.float64)), _dp(y.view(np.float64))))
         return y
 
+    def calculate_vector_b_tri_c64(self, vector_x):
+        """The complex64 operator of a precision="mixed" handle."""
+        x = np.ascontiguousarray(vector_x, dtype=np.complex64)
+        if x.shape != (2 * self.N,):
+            raise ValueError(f"vector_x must have length 2N = {2 * self.N}")
+        y = np.empty_like(x)
+        fp = C.POINTER(C.c_float)
+        nat.check(nat.lib().spk_matvec_c64(self._h, x.view(np.float32).ctypes.data_as(fp),
+                                           y.view(np.float32).ctypes.data_as(fp)))
+        return y
+
     def matvec_parts(self, vector_x):
         x = np.ascontiguousarray(vector_x, dtype=np.complex128)
         yn = np.empty_like(x)
@@ -155,11 +166,16 @@ class SieTriMlfmm:
 
     # -- lib_math_solver_gmres_run ------------------------------------------
     def gmres_run(self, vector_b, x0=None, max_iterations=650, restart=0, tolerance=1e-3,
-                  orthogonalization="IMGS", use_initial_guess=True):
+                  orthogonalization="IMGS", use_initial_guess=True, refinement_steps=0,
+                  refinement_tolerance=0.0):
+        """CERFACS GMRES on the FP64 operator; on a precision="mixed" handle,
+        FP64 iterative refinement around complex64 inner GMRES solves
+        (refinement_tolerance: inner relative residual, default 1e-4)."""
         b = np.ascontiguousarray(vector_b, dtype=np.complex128)
         x = self.get_init_vector_x_tri() if x0 is None else np.array(x0, dtype=np.complex128)
         gp = nat.GmresParams(max_iterations, restart, tolerance,
-                             ORTHOGONALIZATION[orthogonalization], int(use_initial_guess), 0, 0.0)
+                             ORTHOGONALIZATION[orthogonalization], int(use_initial_guess),
+                             refinement_steps, refinement_tolerance)
         info = nat.GmresInfo()
         nat.check(nat.lib().spk_gmres(self._h, _dp(b.view(np.float64)), _dp(x.view(np.float64)),
                                       C.byref(gp), C.byref(info)))
@@ -168,9 +184,11 @@ class SieTriMlfmm:
                        matvecs=info.matvecs, seconds=info.seconds, hist=self.gmres_history())
 
     def gmres_run_device(self, d_b, d_x, max_iterations=650, restart=0, tolerance=1e-3,
-                         orthogonalization="IMGS", use_initial_guess=True):
+                         orthogonalization="IMGS", use_initial_guess=True, refinement_steps=0,
+                         refinement_tolerance=0.0):
         gp = nat.GmresParams(max_iterations, restart, tolerance,
-                             ORTHOGONALIZATION[orthogonalization], int(use_initial_guess), 0, 0.0)
+                             ORTHOGONALIZATION[orthogonalization], int(use_initial_guess),
+                             refinement_steps, refinement_tolerance)
         info = nat.GmresInfo()
         nat.check(nat.lib().spk_gmres_device(self._h, C.c_void_p(int(d_b)), C.c_void_p(int(d_x)),
                                              C.byref(gp), C.byref(info)))
@@ -236,12 +254,14 @@ class SieTriMlfmm:
                                             n.ctypes.data_as(C.POINTER(C.c_int)), _dp(out)))
         return (out[0::2] + 1j * out[1::2]).reshape(-1, 2, 2)
 
-    def profile_matvec(self, reps=20):
+    def profile_matvec(self, reps=20, c64=False):
         """Mean device time [ms] and algorithmic HBM bytes of each matvec
-        stage, measured with HIP events on the handle's stream."""
+        stage, measured with HIP events on the handle's stream (c64: the
+        complex64 operator of a mixed handle)."""
         ms = np.zeros(8)
         by = np.zeros(8)
-        nat.check(nat.lib().spk_profile_matvec(self._h, reps, _dp(ms), _dp(by)))
+        f = nat.lib().spk_profile_matvec_c64 if c64 else nat.lib().spk_profile_matvec
+        nat.check(f(self._h, reps, _dp(ms), _dp(by)))
         return {k: (float(ms[i]), float(by[i])) for i, k in enumerate(nat.STAGES)}
 
     def patterns(self, first, count):

@@ -1,0 +1,67 @@
+"""Mixed precision (SPK_MIXED): the complex64 operator against the FP64
+operator, and FP64 iterative refinement around complex64 inner GMRES against
+the plain FP64 solve.  No reference counterpart: the bar is the FP64 path
+of this library (itself at parity with the restatement of the reference).
+Tolerances: complex64 operator 1e-5 relative L2 (single-precision rounding
+of tables and sums); refined solution to the FP64 tolerance; far-field
+(BRDF) intensity within 1e-4 of the FP64 solution's.
+"""
+import numpy as np
+import pytest
+
+from specklesim_amd import SieTriMlfmm, farfield, meshes, plane_wave
+
+pytestmark = pytest.mark.gpu
+LAM = 633e-9
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.fixture(scope="module")
+def pair():
+    pp, tt = meshes.rough_surface(40, 40, 5.0, 5.0, 0.1, 0.5, 6, LAM)
+    g64 = SieTriMlfmm(LAM, 2.25, formulation="ICTF", normal_convention=2).precalculation_tri(pp, tt)
+    g32 = SieTriMlfmm(LAM, 2.25, formulation="ICTF", normal_convention=2,
+                      precision="mixed").precalculation_tri(pp, tt)
+    return g64, g32
+
+
+def test_c64_operator(pair):
+    g64, g32 = pair
+    rng = np.random.default_rng(4)
+    x = (rng.standard_normal(2 * g64.N) + 1j * rng.standard_normal(2 * g64.N)).astype(np.complex64)
+    y64 = g64.calculate_vector_b_tri(x.astype(np.complex128))
+    y32 = g32.calculate_vector_b_tri_c64(x)
+    assert y32.dtype == np.complex64
+    assert rel(y32.astype(np.complex128), y64) < 1e-5
+    # the FP64 operator of a mixed handle is the FP64 operator
+    assert np.array_equal(g32.calculate_vector_b_tri(x.astype(np.complex128)), y64)
+
+
+def test_c64_operator_needs_mixed_handle(pair):
+    g64, _ = pair
+    from specklesim_amd._native import SpeckleError
+
+    with pytest.raises(SpeckleError):
+        g64.calculate_vector_b_tri_c64(np.zeros(2 * g64.N, np.complex64))
+
+
+def test_refined_solution_matches_fp64(pair):
+    g64, g32 = pair
+    k, e = plane_wave(0.0, 0.0, 2, "TM")
+    b = g64.get_vector_b_tri(k, e)
+    x64, i64 = g64.gmres_run(b, tolerance=1e-6, max_iterations=650)
+    x32, i32 = g32.gmres_run(b, tolerance=1e-6, max_iterations=1300)
+    assert i64["info"] == 0 and i32["info"] == 0
+    # backward error is the FP64 true residual
+    r = b - g64.calculate_vector_b_tri(x32)
+    assert abs(np.linalg.norm(r) / np.linalg.norm(b) - i32["be"]) < 1e-10
+    assert i32["be"] <= 1e-6
+    # both solve the same FP64 system to 1e-6: forward error bounded by the
+    # condition of the system times the two residuals
+    assert rel(x32, x64) < 1e-4
+    th, b64 = farfield.brdf(g64, x64, 0.0, (-1.2, 1.2), 49, 0.0, 1e3 * LAM)
+    _, b32 = farfield.brdf(g64, x32, 0.0, (-1.2, 1.2), 49, 0.0, 1e3 * LAM)
+    assert rel(b32, b64) < 1e-4
