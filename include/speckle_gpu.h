@@ -78,15 +78,22 @@ typedef struct spk_gmres_params {
   int refinement_steps;          /* SPK_MIXED: maximum outer refinement steps (0: 30) */
   double refinement_tolerance;   /* SPK_MIXED: relative residual of each inner complex64
                                     solve (0: 1e-4); tolerance is the FP64 target */
+  int preconditioning;           /* icntl(4): SPK_PRECOND_NONE or SPK_PRECOND_LEFT_JACOBI
+                                    (precondition_left, src/lib_sie_solver_gmres_interface.f90:115;
+                                    diagonal of the self blocks, lib_sie_tri_mlfmm_interface.f90:94-109);
+                                    FP64 only */
 } spk_gmres_params;
+#define SPK_PRECOND_NONE 0
+#define SPK_PRECOND_LEFT_JACOBI 1
 
 typedef struct spk_gmres_info {
   int info;                      /* info(1): 0 converged, -4 not converged */
   int iterations;                /* info(2) (sum over refinement sweeps) */
   double arnoldi_backward_error; /* last bea */
-  double backward_error;         /* rinfo(1): true ||b-Ax||/||b|| */
+  double backward_error;         /* rinfo(1): ||M^-1 (b-Ax)|| / ||M^-1 b|| (M = I unpreconditioned) */
   int matvecs;
   double seconds;
+  double true_backward_error;    /* rinfo(2): ||b-Ax|| / ||b|| */
 } spk_gmres_info;
 
 typedef struct spk_stats {

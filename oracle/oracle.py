@@ -59,8 +59,12 @@ def lib():
         L.spo_scattered_field.argtypes = [C.c_void_p, C.c_int, dp, dp, dp, dp]
         L.spo_gmres.argtypes = [C.c_void_p, dp, dp, C.c_int, C.c_int, C.c_double, C.c_int,
                                 C.c_int, dp, ip, dp]
+        L.spo_gmres_prec.argtypes = [C.c_void_p, dp, dp, C.c_int, C.c_int, C.c_double, C.c_int,
+                                     C.c_int, C.c_int, dp, ip, dp, dp]
         L.spo_gmres_dense.argtypes = [C.c_int, dp, dp, dp, C.c_int, C.c_int, C.c_double, C.c_int,
                                       dp, ip, dp]
+        L.spo_gmres_dense_prec.argtypes = [C.c_int, dp, dp, dp, dp, C.c_int, C.c_int, C.c_double,
+                                           C.c_int, dp, ip, dp, dp]
         L.spo_quadrature_tri.argtypes = [C.c_int, dp, dp, dp]
         L.spo_iq_l.restype = C.c_double
         L.spo_iq_l.argtypes = [C.c_int, dp, dp, dp]
@@ -214,6 +218,18 @@ class Oracle:
                                   _d(E.view(np.float64)), _d(H.view(np.float64)))
         return E, H
 
+    def gmres_left_jacobi(self, b, maxit=650, restart=0, tol=1e-3, orth=1, nthreads=0):
+        b = np.ascontiguousarray(b, dtype=np.complex128)
+        x = np.zeros(2 * self.N, np.complex128)
+        hist = np.zeros(maxit)
+        it = C.c_int()
+        be = np.zeros(1)
+        tbe = np.zeros(1)
+        info = lib().spo_gmres_prec(self.h, _d(b.view(np.float64)), _d(x.view(np.float64)), maxit,
+                                    restart, tol, orth, 1, nthreads, _d(hist), C.byref(it), _d(be),
+                                    _d(tbe))
+        return x, dict(info=info, iters=it.value, be=be[0], tbe=tbe[0], hist=hist[:it.value])
+
     def gmres(self, b, maxit=650, restart=0, tol=1e-3, orth=1, nthreads=0, x0=None):
         b = np.ascontiguousarray(b, dtype=np.complex128)
         x = np.zeros(2 * self.N, np.complex128) if x0 is None else np.array(x0, np.complex128)
@@ -238,6 +254,23 @@ def gmres_dense(A, b, maxit, restart=0, tol=1e-6, orth=1):
                                  _d(x.view(np.float64)), maxit, restart, tol, orth, _d(hist),
                                  C.byref(it), _d(be))
     return x, dict(info=info, iters=it.value, be=be[0], hist=hist[:it.value])
+
+
+def gmres_dense_left_jacobi(A, diag, b, maxit, restart=0, tol=1e-6, orth=1):
+    A = np.asfortranarray(A, dtype=np.complex128)
+    b = np.ascontiguousarray(b, dtype=np.complex128)
+    dg = np.ascontiguousarray(diag, dtype=np.complex128)
+    n = b.shape[0]
+    x = np.zeros(n, np.complex128)
+    hist = np.zeros(maxit)
+    it = C.c_int()
+    be = np.zeros(1)
+    tbe = np.zeros(1)
+    Af = np.ascontiguousarray(A.T.reshape(-1))
+    info = lib().spo_gmres_dense_prec(n, _d(Af.view(np.float64)), _d(dg.view(np.float64)),
+                                      _d(b.view(np.float64)), _d(x.view(np.float64)), maxit,
+                                      restart, tol, orth, _d(hist), C.byref(it), _d(be), _d(tbe))
+    return x, dict(info=info, iters=it.value, be=be[0], tbe=tbe[0], hist=hist[:it.value])
 
 
 def quadrature_tri(ngp):

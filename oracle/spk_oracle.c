@@ -1664,8 +1664,20 @@ static void zrot1(zc *x, zc *y, double cc, zc s) {
   *x = t;
 }
 
-static int gmres_core(int n, spo_mv_fn mv, void *ctx, const zc *b, zc *x, int iterMax, int m,
-                      double tol, int orth, double *hist, int *iters, double *be_out) {
+/* Left preconditioning by a diagonal (CERFACS icntl(4) = 1 with the
+ * reference's lib_sie_ml_fmm_preconditioner_left, src/lib_sie_tri_mlfmm_interface.f90:321-339):
+ * y = x / diag.  diag == NULL: no preconditioning. */
+static void precond_left(int n, const zc *diag, const zc *x, zc *y) {
+  if (!diag) {
+    memcpy(y, x, sizeof(zc) * n);
+    return;
+  }
+  for (int i = 0; i < n; ++i) y[i] = x[i] / diag[i];
+}
+
+static int gmres_core(int n, spo_mv_fn mv, void *ctx, const zc *diag, const zc *b, zc *x,
+                      int iterMax, int m, double tol, int orth, double *hist, int *iters,
+                      double *be_out, double *tbe_out) {
   zc *V = xmalloc(sizeof(zc) * (size_t)n * (m + 1));
   zc *H = xmalloc(sizeof(zc) * (size_t)(m + 1) * (m + 1)); /* H(i,j) = H[(j)*(m+1)+i] */
   zc *w = xmalloc(sizeof(zc) * n), *r0 = xmalloc(sizeof(zc) * n);
@@ -1676,15 +1688,21 @@ static int gmres_core(int n, spo_mv_fn mv, void *ctx, const zc *b, zc *x, int it
   int info = 0, it_total = 0, compRsd = 0;
   double be = 0.0;
   double bn = sqrt(creal(zdotc(n, b, b)));
+  if (tbe_out) *tbe_out = 0.0;
   if (bn == 0.0) {
     for (int j = 0; j < n; ++j) x[j] = 0.0;
     *iters = 0; *be_out = 0.0;
     goto done;
   }
+  /* sPb: norm of the preconditioned right-hand side (zPackgmres.f90:890-923) */
   double sPb = bn;
+  if (diag) {
+    precond_left(n, diag, b, r0);
+    sPb = sqrt(creal(zdotc(n, r0, r0)));
+  }
   mv(ctx, x, r0);
   for (int j = 0; j < n; ++j) r0[j] = b[j] - r0[j];
-  memcpy(w, r0, sizeof(zc) * n);
+  precond_left(n, diag, r0, w);
   double beta = sqrt(creal(zdotc(n, w, w)));
   if (beta == 0.0) { *iters = 0; *be_out = 0.0; goto done; }
   int iterOut = 0;
@@ -1698,7 +1716,7 @@ static int gmres_core(int n, spo_mv_fn mv, void *ctx, const zc *b, zc *x, int it
     for (;;) {
       memcpy(w, &V[(size_t)(jH - 1) * n], sizeof(zc) * n);
       mv(ctx, w, r0);
-      memcpy(w, r0, sizeof(zc) * n);
+      precond_left(n, diag, r0, w);
       for (int j = 0; j < jH; ++j) HH(j, jH - 1) = 0.0;
       int nOrtho = 0;
       double dnormw;
@@ -1770,7 +1788,8 @@ static int gmres_core(int n, spo_mv_fn mv, void *ctx, const zc *b, zc *x, int it
         memcpy(r0, xc, sizeof(zc) * n);
         mv(ctx, r0, w);
         for (int i = 0; i < n; ++i) w[i] = b[i] - w[i];
-        memcpy(r0, w, sizeof(zc) * n);
+        const double true_res = sqrt(creal(zdotc(n, w, w)));
+        precond_left(n, diag, w, r0);
         double dres = sqrt(creal(zdotc(n, r0, r0)));
         be = dres / sPb;
         if (be <= tol || it_total >= iterMax) {
@@ -1778,6 +1797,8 @@ static int gmres_core(int n, spo_mv_fn mv, void *ctx, const zc *b, zc *x, int it
           info = (be <= tol) ? 0 : -4;
           *iters = it_total;
           *be_out = be;
+          /* rinfo(2): backward error of the unpreconditioned system (:1390) */
+          if (tbe_out) *tbe_out = true_res / bn;
           goto done;
         }
       }
@@ -1806,7 +1827,8 @@ static int gmres_core(int n, spo_mv_fn mv, void *ctx, const zc *b, zc *x, int it
     memcpy(x, xc, sizeof(zc) * n);
     if (compRsd) {
       mv(ctx, x, r0);
-      for (int i = 0; i < n; ++i) w[i] = b[i] - r0[i];
+      for (int i = 0; i < n; ++i) r0[i] = b[i] - r0[i];
+      precond_left(n, diag, r0, w);
       beta = sqrt(creal(zdotc(n, w, w)));
     } else {
       beta = cabs(HH(m, m));
@@ -1833,12 +1855,43 @@ static void mv_oracle(void *p, const zc *x, zc *y) {
   spo_matvec(q->c, (const double *)x, (double *)y, q->nt);
 }
 
+/* precalculation_diagonal_element (src/lib_sie_tri_mlfmm_interface.f90:94-109):
+ * diag = [Z_mm(1,1); Z_mm(2,2)] from the near-field CSR. */
+static void jacobi_diag(const spo_ctx *c, zc *diag) {
+  const int N = c->N;
+  for (int m = 0; m < N; ++m) {
+    zc z11 = 0.0, z22 = 0.0;
+    for (int64_t j = c->ia[m]; j < c->ia[m + 1]; ++j)
+      if (c->ja[j] == m) {
+        z11 = c->za[4 * j + 0];
+        z22 = c->za[4 * j + 3];
+        break;
+      }
+    diag[m] = z11;
+    diag[m + N] = z22;
+  }
+}
+
 int spo_gmres(spo_ctx *c, const double *b, double *x, int maxit, int restart, double tol,
               int orth, int nthreads, double *hist, int *iters, double *be_out) {
+  return spo_gmres_prec(c, b, x, maxit, restart, tol, orth, 0, nthreads, hist, iters, be_out,
+                        NULL);
+}
+
+int spo_gmres_prec(spo_ctx *c, const double *b, double *x, int maxit, int restart, double tol,
+                   int orth, int left_jacobi, int nthreads, double *hist, int *iters,
+                   double *be_out, double *tbe_out) {
   mvctx q = {c, nthreads};
   if (restart <= 0 || restart > maxit) restart = maxit;
-  return gmres_core(2 * c->N, mv_oracle, &q, (const zc *)b, (zc *)x, maxit, restart, tol, orth,
-                    hist, iters, be_out);
+  zc *diag = NULL;
+  if (left_jacobi) {
+    diag = xmalloc(sizeof(zc) * 2 * (size_t)c->N);
+    jacobi_diag(c, diag);
+  }
+  int info = gmres_core(2 * c->N, mv_oracle, &q, diag, (const zc *)b, (zc *)x, maxit, restart,
+                        tol, orth, hist, iters, be_out, tbe_out);
+  free(diag);
+  return info;
 }
 
 typedef struct { int n; const zc *A; } densectx;
@@ -1853,8 +1906,17 @@ int spo_gmres_dense(int n, const double *A, const double *b, double *x, int maxi
                     double tol, int orth, double *hist, int *iters, double *be_out) {
   densectx d = {n, (const zc *)A};
   if (restart <= 0 || restart > maxit) restart = maxit;
-  return gmres_core(n, mv_dense, &d, (const zc *)b, (zc *)x, maxit, restart, tol, orth, hist,
-                    iters, be_out);
+  return gmres_core(n, mv_dense, &d, NULL, (const zc *)b, (zc *)x, maxit, restart, tol, orth,
+                    hist, iters, be_out, NULL);
+}
+
+int spo_gmres_dense_prec(int n, const double *A, const double *diag, const double *b, double *x,
+                         int maxit, int restart, double tol, int orth, double *hist, int *iters,
+                         double *be_out, double *tbe_out) {
+  densectx d = {n, (const zc *)A};
+  if (restart <= 0 || restart > maxit) restart = maxit;
+  return gmres_core(n, mv_dense, &d, (const zc *)diag, (const zc *)b, (zc *)x, maxit, restart, tol,
+                    orth, hist, iters, be_out, tbe_out);
 }
 
 /* Dense impedance matrix over all pairs, the fill loop of

@@ -92,12 +92,21 @@ void spo_scattered_field(const spo_ctx *c, int npts, const double *r, const doub
 int spo_gmres(spo_ctx *c, const double *b, double *x, int maxit, int restart,
               double tol, int orth, int nthreads, double *hist, int *iters,
               double *be_out);
+/* the same with left Jacobi preconditioning when left_jacobi != 0
+ * (diagonal of the near-field self blocks; needs the near fill); tbe_out:
+ * backward error of the unpreconditioned system (rinfo(2)) */
+int spo_gmres_prec(spo_ctx *c, const double *b, double *x, int maxit, int restart, double tol,
+                   int orth, int left_jacobi, int nthreads, double *hist, int *iters,
+                   double *be_out, double *tbe_out);
 
 /* Generic CERFACS GMRES on a dense complex matrix A (column-major n x n),
  * used to pin the solver against textbook cases. */
 int spo_gmres_dense(int n, const double *A, const double *b, double *x,
                     int maxit, int restart, double tol, int orth,
                     double *hist, int *iters, double *be_out);
+int spo_gmres_dense_prec(int n, const double *A, const double *diag, const double *b, double *x,
+                         int maxit, int restart, double tol, int orth, double *hist, int *iters,
+                         double *be_out, double *tbe_out);
 
 /* Dense 2N x 2N matrix (row-major complex) of all pair blocks. */
 void spo_dense_fill(const spo_ctx *c, double *Z, int nthreads);

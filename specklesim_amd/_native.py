@@ -42,13 +42,15 @@ class Params(C.Structure):
 class GmresParams(C.Structure):
     _fields_ = [("max_iterations", C.c_int), ("restart", C.c_int), ("tolerance", C.c_double),
                 ("orthogonalization", C.c_int), ("use_initial_guess", C.c_int),
-                ("refinement_steps", C.c_int), ("refinement_tolerance", C.c_double)]
+                ("refinement_steps", C.c_int), ("refinement_tolerance", C.c_double),
+                ("preconditioning", C.c_int)]
 
 
 class GmresInfo(C.Structure):
     _fields_ = [("info", C.c_int), ("iterations", C.c_int),
                 ("arnoldi_backward_error", C.c_double), ("backward_error", C.c_double),
-                ("matvecs", C.c_int), ("seconds", C.c_double)]
+                ("matvecs", C.c_int), ("seconds", C.c_double),
+                ("true_backward_error", C.c_double)]
 
 
 class Stats(C.Structure):

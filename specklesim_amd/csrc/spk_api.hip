@@ -111,6 +111,8 @@ struct spk_handle {
   DBuf<zf> i_V, i_w;
   int i_m = 0;
   int refinements = 0;  // outer steps of the last mixed solve
+  DBuf<zd> d_diag;      // Jacobi diagonal, built on first use
+  bool diag_ready = false;
   // gmres workspace
   DBuf<zd> g_V, g_w, g_r0, g_xc, g_b, g_h, g_partial, g_scal, g_y;
   DBuf<unsigned long long> g_ctl;    // MGS sweep abort flag
@@ -447,6 +449,7 @@ int spk_setup_tri(spk_handle *h, int np, const double *pp, int nt, const int *tt
   HIPCHK(hipSetDevice(h->p.device));
   h->ready = false;
   h->mixed_ready = false;
+  h->diag_ready = false;
   h->stats = spk_stats{};
   const double t0 = now_ms();
   std::string err;
@@ -780,6 +783,8 @@ static int dev_dot(spk_handle *h, const zd *x, const zd *y, int64_t n, zc *out) 
 static int gmres_mixed(spk_handle *h, const zd *b, zd *x, const spk_gmres_params *gp,
                        spk_gmres_info *info) {
   if (!h->mixed_ready) return fail(SPK_ERR_STATE, "handle was not created with SPK_MIXED");
+  if (gp->preconditioning != SPK_PRECOND_NONE)
+    return fail(SPK_ERR_ARG, "preconditioning is not available with SPK_MIXED");
   const double t_start = now_ms();
   const int64_t n = 2 * (int64_t)h->N;
   const int max_it = std::max(1, gp->max_iterations);
@@ -823,7 +828,7 @@ static int gmres_mixed(spk_handle *h, const zd *b, zd *x, const spk_gmres_params
   if (bn == 0.0) {
     HIPCHK(hipMemsetAsync(x, 0, n * sizeof(zd), h->st));
     HIPCHK(hipStreamSynchronize(h->st));
-    if (info) *info = spk_gmres_info{0, 0, 0.0, 0.0, 0, (now_ms() - t_start) / 1e3};
+    if (info) *info = spk_gmres_info{0, 0, 0.0, 0.0, 0, (now_ms() - t_start) / 1e3, 0.0};
     return SPK_OK;
   }
   auto residual = [&](double *rn) -> int {
@@ -925,6 +930,7 @@ static int gmres_mixed(spk_handle *h, const zd *b, zd *x, const spk_gmres_params
     info->backward_error = be;
     info->matvecs = matvecs;
     info->seconds = (now_ms() - t_start) / 1e3;
+    info->true_backward_error = be;
   }
   h->refinements = outer;
   return SPK_OK;
@@ -982,23 +988,47 @@ int spk_gmres_device(spk_handle *h, const void *d_b, void *d_x, const spk_gmres_
   double be = 0.0, bea = 0.0;
   zc dtmp;
   int rc_ = SPK_OK;
+  // left Jacobi preconditioning: M^-1 applied where zPackgmres requests
+  // precondLeft (:890-923, :955, :1054, :1310-1340, :1510)
+  const bool prec = gp->preconditioning == SPK_PRECOND_LEFT_JACOBI;
+  if (gp->preconditioning != SPK_PRECOND_NONE && !prec)
+    return fail(SPK_ERR_ARG, "unknown preconditioning");
+  if (prec && !h->diag_ready) {
+    HIPCHK(h->d_diag.alloc(n));
+    launch_jacobi_diag(h->N, h->d_perm.p, h->d_leafbox_of_pos.p, h->d_box_first.p, h->d_e2_off.p,
+                       h->d_e2_list.p, h->d_near_off.p, h->d_blocks.p, h->d_diag.p, h->st);
+    HIPCHK(hipGetLastError());
+    h->diag_ready = true;
+  }
+  auto precond = [&](const zd *in, zd *out) {
+    if (prec) launch_div_diag(n, h->d_diag.p, in, out, h->st);
+    else if (in != out)
+      (void)hipMemcpyAsync(out, in, n * sizeof(zd), hipMemcpyDeviceToDevice, h->st);
+  };
+  double tbe = 0.0;
   if (!gp->use_initial_guess) HIPCHK(hipMemsetAsync(x, 0, n * sizeof(zd), h->st));
   if ((rc_ = dev_dot(h, b, b, n, &dtmp))) return rc_;
   const double bn = std::sqrt(dtmp.real());
   if (bn == 0.0) {
     HIPCHK(hipMemsetAsync(x, 0, n * sizeof(zd), h->st));
     HIPCHK(hipStreamSynchronize(h->st));
-    if (info) *info = spk_gmres_info{0, 0, 0.0, 0.0, 0, (now_ms() - t_start) / 1e3};
+    if (info) *info = spk_gmres_info{0, 0, 0.0, 0.0, 0, (now_ms() - t_start) / 1e3, 0.0};
     return SPK_OK;
   }
-  const double sPb = bn;
+  double sPb = bn;
+  if (prec) {
+    precond(b, r0);
+    if ((rc_ = dev_dot(h, r0, r0, n, &dtmp))) return rc_;
+    sPb = std::sqrt(dtmp.real());
+  }
   if ((rc_ = matvec_dev(h, x, r0))) return rc_;
   ++matvecs;
-  launch_zsub(n, b, r0, w, h->st);
+  launch_zsub(n, b, r0, r0, h->st);
+  precond(r0, w);
   if ((rc_ = dev_dot(h, w, w, n, &dtmp))) return rc_;
   double beta = std::sqrt(dtmp.real());
   if (beta == 0.0) {
-    if (info) *info = spk_gmres_info{0, 0, 0.0, 0.0, matvecs, (now_ms() - t_start) / 1e3};
+    if (info) *info = spk_gmres_info{0, 0, 0.0, 0.0, matvecs, (now_ms() - t_start) / 1e3, 0.0};
     return SPK_OK;
   }
   int iterOut = 0;
@@ -1009,7 +1039,12 @@ int spk_gmres_device(spk_handle *h, const void *d_b, void *d_x, const spk_gmres_
     for (int j = 1; j <= m; ++j) HH(j, m) = 0.0;
     int jH = 1;
     for (;;) {
-      if ((rc_ = matvec_dev(h, V + (size_t)(jH - 1) * n, w))) return rc_;
+      if (prec) {
+        if ((rc_ = matvec_dev(h, V + (size_t)(jH - 1) * n, r0))) return rc_;
+        precond(r0, w);
+      } else if ((rc_ = matvec_dev(h, V + (size_t)(jH - 1) * n, w))) {
+        return rc_;
+      }
       ++matvecs;
       for (int j = 0; j < jH; ++j) HH(j, jH - 1) = 0.0;
       int nOrtho = 0;
@@ -1090,7 +1125,15 @@ int spk_gmres_device(spk_handle *h, const void *d_b, void *d_x, const spk_gmres_
         ++matvecs;
         launch_zsub(n, b, w, r0, h->st);
         if ((rc_ = dev_dot(h, r0, r0, n, &dtmp))) return rc_;
-        be = std::sqrt(dtmp.real()) / sPb;
+        const double true_res = std::sqrt(dtmp.real());
+        if (prec) {
+          precond(r0, w);
+          if ((rc_ = dev_dot(h, w, w, n, &dtmp))) return rc_;
+          be = std::sqrt(dtmp.real()) / sPb;
+        } else {
+          be = true_res / sPb;
+        }
+        tbe = true_res / bn;
         if (be <= tol || it_total >= iterMax) {
           HIPCHK(hipMemcpyAsync(x, xc, n * sizeof(zd), hipMemcpyDeviceToDevice, h->st));
           info1 = (be <= tol) ? 0 : -4;
@@ -1124,7 +1167,8 @@ int spk_gmres_device(spk_handle *h, const void *d_b, void *d_x, const spk_gmres_
     if (compRsd) {
       if ((rc_ = matvec_dev(h, x, r0))) return rc_;
       ++matvecs;
-      launch_zsub(n, b, r0, w, h->st);
+      launch_zsub(n, b, r0, r0, h->st);
+      precond(r0, w);
       if ((rc_ = dev_dot(h, w, w, n, &dtmp))) return rc_;
       beta = std::sqrt(dtmp.real());
     } else {
@@ -1149,6 +1193,7 @@ int spk_gmres_device(spk_handle *h, const void *d_b, void *d_x, const spk_gmres_
     info->backward_error = be;
     info->matvecs = matvecs;
     info->seconds = (now_ms() - t_start) / 1e3;
+    info->true_backward_error = tbe;
   }
   return SPK_OK;
 }

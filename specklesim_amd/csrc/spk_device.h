@@ -150,6 +150,12 @@ struct ScatArgs {
 void launch_scattered(const ScatArgs &a, const FnRec *fn, const zd *x, const double *r, zd *E,
                       zd *H, hipStream_t st);
 
+// Jacobi preconditioner: diag = [Z_mm(1,1); Z_mm(2,2)] (reference order), y = x / diag
+void launch_jacobi_diag(int N, const int *perm, const int *leaf_box_of_pos, const int *box_first,
+                        const int *e2_off, const int *e2_list, const int64_t *near_off,
+                        const zd *blocks, zd *diag, hipStream_t st);
+void launch_div_diag(int64_t n, const zd *diag, const zd *x, zd *y, hipStream_t st);
+
 // BLAS-1 (deterministic reductions)
 int reduce_blocks(int64_t n);
 template <class T>  // sum conj(x) y accumulated in complex128

@@ -1096,6 +1096,55 @@ void launch_to_zf(int64_t n, const zd *a, zf *b, hipStream_t st) {
 }
 
 // ---------------------------------------------------------------------------
+// Jacobi preconditioner (precalculation_diagonal_element and
+// lib_sie_ml_fmm_preconditioner_left, src/lib_sie_tri_mlfmm_interface.f90:94-109,
+// 321-339): diag = [Z_mm(1,1); Z_mm(2,2)], y = x / diag.  One thread per
+// box-major position finds the self column in its box's sorted E2 list.
+// ---------------------------------------------------------------------------
+__global__ void k_jacobi_diag(int N, const int *perm, const int *leaf_box_of_pos,
+                              const int *box_first, const int *e2_off, const int *e2_list,
+                              const int64_t *near_off, const zd *blocks, zd *diag) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  const int m = perm[p], b = leaf_box_of_pos[p];
+  const int r = p - box_first[b];
+  const int c0 = e2_off[b], cols = e2_off[b + 1] - c0;
+  int lo = 0, hi = cols - 1, c = -1;
+  while (lo <= hi) {
+    const int md = (lo + hi) >> 1, v = e2_list[c0 + md];
+    if (v == m) {
+      c = md;
+      break;
+    }
+    if (v < m) lo = md + 1;
+    else hi = md - 1;
+  }
+  zd d11 = {0, 0}, d22 = {0, 0};
+  if (c >= 0) {
+    const zd *Z = blocks + 4 * (near_off[b] + (int64_t)r * cols + c);
+    d11 = Z[0];
+    d22 = Z[3];
+  }
+  diag[m] = d11;
+  diag[m + N] = d22;
+}
+void launch_jacobi_diag(int N, const int *perm, const int *leaf_box_of_pos, const int *box_first,
+                        const int *e2_off, const int *e2_list, const int64_t *near_off,
+                        const zd *blocks, zd *diag, hipStream_t st) {
+  if (N <= 0) return;
+  hipLaunchKernelGGL(k_jacobi_diag, dim3(nblk(N, 256)), dim3(256), 0, st, N, perm,
+                     leaf_box_of_pos, box_first, e2_off, e2_list, near_off, blocks, diag);
+}
+__global__ void k_div_diag(int64_t n, const zd *diag, const zd *x, zd *y) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = x[i] / diag[i];
+}
+void launch_div_diag(int64_t n, const zd *diag, const zd *x, zd *y, hipStream_t st) {
+  hipLaunchKernelGGL(k_div_diag, dim3(gcap(n)), dim3(256), 0, st, n, diag, x, y);
+}
+
+// ---------------------------------------------------------------------------
 // Plane-wave right-hand side (get_vector_b_tri / incident_field_tri_general,
 // src/lib_sie_tri_calculation_mod.f90:391-460, 939-1002): at the ng rule
 // points of both triangles of edge m, E = e_in exp(-i k1 k_in.r),

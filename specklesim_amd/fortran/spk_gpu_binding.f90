@@ -27,6 +27,7 @@ module spk_gpu_binding
       real(c_double) :: tolerance
       integer(c_int) :: orthogonalization, use_initial_guess, refinement_steps
       real(c_double) :: refinement_tolerance
+      integer(c_int) :: preconditioning
    end type
 
    type, bind(c) :: spk_gmres_info
@@ -34,6 +35,7 @@ module spk_gpu_binding
       real(c_double) :: arnoldi_backward_error, backward_error
       integer(c_int) :: matvecs
       real(c_double) :: seconds
+      real(c_double) :: true_backward_error
    end type
 
    interface
@@ -207,6 +209,7 @@ contains
       p%use_initial_guess = 1
       p%refinement_steps = 0
       p%refinement_tolerance = 0d0
+      p%preconditioning = 0
       call spk_check(spk_gmres(handle, vector_b, vector_x, p, info), 'spk_gmres')
       iterations = info%iterations
       backward_error = info%backward_error

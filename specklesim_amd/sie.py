@@ -167,33 +167,37 @@ class SieTriMlfmm:
     # -- lib_math_solver_gmres_run ------------------------------------------
     def gmres_run(self, vector_b, x0=None, max_iterations=650, restart=0, tolerance=1e-3,
                   orthogonalization="IMGS", use_initial_guess=True, refinement_steps=0,
-                  refinement_tolerance=0.0):
-        """CERFACS GMRES on the FP64 operator; on a precision="mixed" handle,
-        FP64 iterative refinement around complex64 inner GMRES solves
-        (refinement_tolerance: inner relative residual, default 1e-4)."""
+                  refinement_tolerance=0.0, precondition_left=False):
+        """CERFACS GMRES on the FP64 operator (precondition_left: the
+        reference's Jacobi left preconditioner); on a precision="mixed"
+        handle, FP64 iterative refinement around complex64 inner GMRES
+        solves (refinement_tolerance: inner relative residual, default
+        1e-4)."""
         b = np.ascontiguousarray(vector_b, dtype=np.complex128)
         x = self.get_init_vector_x_tri() if x0 is None else np.array(x0, dtype=np.complex128)
         gp = nat.GmresParams(max_iterations, restart, tolerance,
                              ORTHOGONALIZATION[orthogonalization], int(use_initial_guess),
-                             refinement_steps, refinement_tolerance)
+                             refinement_steps, refinement_tolerance, int(precondition_left))
         info = nat.GmresInfo()
         nat.check(nat.lib().spk_gmres(self._h, _dp(b.view(np.float64)), _dp(x.view(np.float64)),
                                       C.byref(gp), C.byref(info)))
         return x, dict(info=info.info, iters=info.iterations,
                        bea=info.arnoldi_backward_error, be=info.backward_error,
-                       matvecs=info.matvecs, seconds=info.seconds, hist=self.gmres_history())
+                       tbe=info.true_backward_error, matvecs=info.matvecs,
+                       seconds=info.seconds, hist=self.gmres_history())
 
     def gmres_run_device(self, d_b, d_x, max_iterations=650, restart=0, tolerance=1e-3,
                          orthogonalization="IMGS", use_initial_guess=True, refinement_steps=0,
-                         refinement_tolerance=0.0):
+                         refinement_tolerance=0.0, precondition_left=False):
         gp = nat.GmresParams(max_iterations, restart, tolerance,
                              ORTHOGONALIZATION[orthogonalization], int(use_initial_guess),
-                             refinement_steps, refinement_tolerance)
+                             refinement_steps, refinement_tolerance, int(precondition_left))
         info = nat.GmresInfo()
         nat.check(nat.lib().spk_gmres_device(self._h, C.c_void_p(int(d_b)), C.c_void_p(int(d_x)),
                                              C.byref(gp), C.byref(info)))
         return dict(info=info.info, iters=info.iterations, bea=info.arnoldi_backward_error,
-                    be=info.backward_error, matvecs=info.matvecs, seconds=info.seconds)
+                    be=info.backward_error, tbe=info.true_backward_error, matvecs=info.matvecs,
+                    seconds=info.seconds)
 
     def gmres_history(self):
         n = C.c_int()

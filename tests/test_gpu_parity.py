@@ -191,6 +191,24 @@ def test_gmres_converged_solution():
     assert np.linalg.norm(ro) / np.linalg.norm(b) < 1.01e-8
 
 
+def test_gmres_left_jacobi():
+    """The reference's Jacobi left preconditioner (precondition_left) on
+    PMCHWT, which stagnates near 1.5e-3 without it on this open patch:
+    GPU and restatement converge together; the second backward error is the
+    unpreconditioned residual."""
+    o, g, b, bo = _patch24("PMCHWT", 250)
+    xg, ig = g.gmres_run(b, tolerance=1e-8, max_iterations=400, precondition_left=True)
+    xo, io = o.gmres_left_jacobi(bo, tol=1e-8, maxit=400, nthreads=16)
+    assert ig["info"] == io["info"] == 0
+    assert abs(ig["iters"] - io["iters"]) <= 2
+    n = min(len(ig["hist"]), len(io["hist"]))
+    assert np.max(np.abs(np.log(ig["hist"][:n] / io["hist"][:n]))) < 1e-3
+    assert rel(xg, xo) < 1e-5
+    r = b - g.calculate_vector_b_tri(xg)
+    assert abs(np.linalg.norm(r) / np.linalg.norm(b) - ig["tbe"]) < 1e-10
+    assert abs(ig["tbe"] / io["tbe"] - 1) < 1e-2
+
+
 def test_gmres_two_level_history():
     """PMCHWT on the same patch with s_opt = 60 (two far-field levels, 0.29
     lambda leaves; PMCHWT on an open patch stagnates near 1.5e-3): 100

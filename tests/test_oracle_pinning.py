@@ -300,3 +300,28 @@ def test_gmres_matches_direct_solve():
     assert np.linalg.norm(x - np.linalg.solve(A, b)) / np.linalg.norm(x) < 1e-12
     h = info["hist"]
     assert np.all(np.diff(h) <= 1e-15)  # GMRES residual estimate is monotone
+
+
+@pytest.mark.parametrize("orth", [0, 1, 3])
+def test_gmres_left_jacobi_equals_plain_gmres_on_scaled_system(orth):
+    """Left preconditioning with a diagonal D (zPackgmres.f90 icntl(4) = 1 with
+    lib_sie_ml_fmm_preconditioner_left) is GMRES on (D^-1 A, D^-1 b): same
+    Arnoldi backward errors, same solution; its second backward error is
+    the unpreconditioned |b - A x| / |b|."""
+    rng = np.random.default_rng(21 + orth)
+    n = 60
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    A += np.diag(np.linspace(5, 40, n) * np.exp(1j * np.linspace(0, 3, n)))
+    b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    d = np.diag(A).copy()
+    xp, ip = O.gmres_dense_left_jacobi(A, d, b, 80, tol=1e-10, orth=orth)
+    xs, is_ = O.gmres_dense(A / d[:, None], b / d, 80, tol=1e-10, orth=orth)
+    assert ip["info"] == is_["info"] == 0
+    assert ip["iters"] == is_["iters"]
+    assert np.max(np.abs(np.log(ip["hist"] / is_["hist"]))) < 1e-8
+    assert np.linalg.norm(xp - xs) < 1e-9 * np.linalg.norm(xs)
+    assert abs(ip["be"] - is_["be"]) < 1e-12
+    assert abs(ip["tbe"] - np.linalg.norm(b - A @ xp) / np.linalg.norm(b)) < 1e-12
+    # and the preconditioner changes the iteration (a no-op D would not)
+    xu, iu = O.gmres_dense(A, b, 80, tol=1e-10, orth=orth)
+    assert iu["iters"] != ip["iters"] or np.max(np.abs(iu["hist"] - ip["hist"])) > 1e-6
