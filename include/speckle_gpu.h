@@ -119,6 +119,41 @@ const char *spk_last_error(void);
  * tt: 3*nt node indices, 1-based (read_pp_tt_file_tri). */
 int spk_setup_tri(spk_handle *h, int np, const double *pp, int nt, const int *tt);
 
+/* ------------------------------------------------------------------------ */
+/* 8-node quadrilaterals with 10 Huber edge functions: the near/dense fill   */
+/* ------------------------------------------------------------------------ */
+typedef struct spk_quad spk_quad;
+/* Handle for the quadrilateral fill: lambda, eps_r1/eps_r2 and formulation
+ * (PMCHWT, MCTF or ICTF, the three the reference assembles) from params;
+ * set_local_material_parameters_quad (src/lib_sie_quad_data_container.f90:118-129). */
+int spk_quad_create(const spk_params *params, spk_quad **out);
+void spk_quad_destroy(spk_quad *q);
+/* The height-field surface of sorting_quad (src/lib_sie_quad_data_container.f90:325-404):
+ * Lx x Ly (m), nx x ny 8-node elements, z[(2nx+1)*(2ny+1)] node heights in the
+ * height file's order (x index fastest); then the edge/corner neighbours
+ * (Neighbours_EdgeCorner_Full, src/lib_sie_quad_calculation_mod.f90:952-995) and
+ * the numbering of the edge functions (Sorting_edge, :624-698). */
+int spk_quad_setup(spk_quad *q, double Lx, double Ly, int nx, int ny, const double *z);
+int spk_quad_sizes(const spk_quad *q, int *n_edges, int *n_elements, int *n_nodes);
+/* nodes pp[3*n_nodes] and element vertices vert8[8*n_elements] (1-based) */
+int spk_quad_get_mesh(const spk_quad *q, double *pp, int *vert8);
+/* Edge_I(s)%edge_p(1:5) for all edge functions: element, local index,
+ * neighbour element, its local index, orientation (1-based, 0 = none) */
+int spk_quad_get_edges(const spk_quad *q, int *edge_p5);
+/* impedance_edge_versus_element(s, t, Sum_aa(3, 10))
+ * (src/lib_sie_quad_singularity.f90:22-98) for n pairs, 0-based edge
+ * function s[i] and element t[i]; out[i*60 + (k*10 + n)*2 + {0,1}] */
+int spk_quad_edge_element(spk_quad *q, int n, const int *s, const int *t, double *out);
+/* The assembled 2M x 2M system of the formulation
+ * (src/lib_sie_quad_solver_normal.f90:165-210), row-major complex: host or
+ * device destination. */
+int spk_quad_fill(spk_quad *q, double *MM);
+int spk_quad_fill_device(spk_quad *q, void *d_MM);
+/* Host-only (no GPU): mesh, sizes and Edge_I of spk_quad_setup; arrays may
+ * be NULL to query the sizes first. */
+int spk_quad_topology_probe(double Lx, double Ly, int nx, int ny, const double *z, int *n_edges,
+                            int *n_elements, int *n_nodes, double *pp, int *vert8, int *edge_p5);
+
 /* Device memory for the *_device entry points (no counterpart in the
  * reference).  A caller that has its own HIP allocations may pass those
  * instead; they must come from the same HIP runtime as this library. */

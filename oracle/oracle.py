@@ -59,6 +59,18 @@ def lib():
         L.spo_scattered_field.argtypes = [C.c_void_p, C.c_int, dp, dp, dp, dp]
         L.spo_gmres.argtypes = [C.c_void_p, dp, dp, C.c_int, C.c_int, C.c_double, C.c_int,
                                 C.c_int, dp, ip, dp]
+        L.spq_create.restype = C.c_void_p
+        L.spq_create.argtypes = [C.c_double] * 5 + [C.c_double, C.c_double, C.c_int, C.c_int, dp]
+        L.spq_destroy.argtypes = [C.c_void_p]
+        for f in ("spq_n_edges", "spq_n_elements", "spq_n_nodes"):
+            getattr(L, f).argtypes = [C.c_void_p]
+        L.spq_nodes.argtypes = [C.c_void_p, dp]
+        L.spq_elements.argtypes = [C.c_void_p, ip]
+        L.spq_edges.argtypes = [C.c_void_p, ip]
+        L.spq_edge_element.argtypes = [C.c_void_p, C.c_int, C.c_int, dp]
+        L.spq_assemble.argtypes = [C.c_void_p, C.c_int, dp, C.c_int]
+        L.spq_geometry_probe.argtypes = [C.c_double, C.c_double, dp, dp]
+        L.spq_regular_pair.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, dp]
         L.spo_gmres_prec.argtypes = [C.c_void_p, dp, dp, C.c_int, C.c_int, C.c_double, C.c_int,
                                      C.c_int, C.c_int, dp, ip, dp, dp]
         L.spo_gmres_dense.argtypes = [C.c_int, dp, dp, dp, C.c_int, C.c_int, C.c_double, C.c_int,
@@ -271,6 +283,72 @@ def gmres_dense_left_jacobi(A, diag, b, maxit, restart=0, tol=1e-6, orth=1):
                                       _d(b.view(np.float64)), _d(x.view(np.float64)), maxit,
                                       restart, tol, orth, _d(hist), C.byref(it), _d(be), _d(tbe))
     return x, dict(info=info, iters=it.value, be=be[0], tbe=tbe[0], hist=hist[:it.value])
+
+
+class QuadOracle:
+    """Restatement of the reference's 8-node quadrilateral fill (spk_quad.c)
+    on the height-field mesh of sorting_quad."""
+
+    def __init__(self, lam, eps_r2, Lx, Ly, nx, ny, z, eps_r1=1.0):
+        L = lib()
+        e1, e2 = complex(eps_r1), complex(eps_r2)
+        z = np.ascontiguousarray(z, dtype=np.float64).reshape(-1)
+        assert z.size == (2 * nx + 1) * (2 * ny + 1)
+        self._z = z
+        self.h = L.spq_create(lam, e1.real, e1.imag, e2.real, e2.imag, Lx, Ly, nx, ny, _d(z))
+        if not self.h:
+            raise ValueError("bad quad mesh")
+        self.M = L.spq_n_edges(self.h)
+        self.n_el = L.spq_n_elements(self.h)
+        self.n_nodes = L.spq_n_nodes(self.h)
+
+    def __del__(self):
+        try:
+            lib().spq_destroy(self.h)
+        except Exception:
+            pass
+
+    def nodes(self):
+        pp = np.zeros((self.n_nodes, 3))
+        lib().spq_nodes(self.h, _d(pp))
+        return pp
+
+    def elements(self):
+        v = np.zeros((self.n_el, 8), np.int32)
+        lib().spq_elements(self.h, v.ctypes.data_as(C.POINTER(C.c_int)))
+        return v
+
+    def edges(self):
+        e = np.zeros((self.M, 5), np.int32)
+        lib().spq_edges(self.h, e.ctypes.data_as(C.POINTER(C.c_int)))
+        return e
+
+    def edge_element(self, s, t):
+        """Sum_aa(3, 10) of impedance_edge_versus_element, 1-based s, t."""
+        o = np.zeros(60)
+        lib().spq_edge_element(self.h, int(s), int(t), _d(o))
+        return (o[0::2] + 1j * o[1::2]).reshape(3, 10)
+
+    def regular_pair(self, s, t, ngp):
+        o = np.zeros(60)
+        lib().spq_regular_pair(self.h, int(s), int(t), int(ngp), _d(o))
+        return (o[0::2] + 1j * o[1::2]).reshape(3, 10)
+
+    def assemble(self, formulation="PMCHWT", nthreads=0):
+        f = {"PMCHWT": 1, "MCTF": 2, "ICTF": 3}[formulation]
+        MM = np.zeros((2 * self.M, 2 * self.M), np.complex128)
+        lib().spq_assemble(self.h, f, _d(MM.view(np.float64)), nthreads)
+        return MM
+
+
+def quad_geometry(k, e, El):
+    """Element geometry probe: dict of shape functions, derivatives, JJ, nq,
+    the 10 edge functions and their n.curl at (k, e)."""
+    El = np.ascontiguousarray(El, dtype=np.float64).reshape(24)
+    o = np.zeros(68)
+    lib().spq_geometry_probe(float(k), float(e), _d(El), _d(o))
+    return dict(N=o[0:8], dk=o[8:16], de=o[16:24], JJ=o[24], nq=o[25:28],
+                fn=o[28:58].reshape(10, 3), nrot=o[58:68])
 
 
 def quadrature_tri(ngp):

@@ -137,6 +137,32 @@ void spo_legendre_p(int n, double x, double *out);
  * out[2*(2K^2+2)] complex for medium kd (re,im). */
 void spo_tl(int K, double kre, double kim, const double *r, double *out);
 
+/* ------------------------------------------------------------------------ */
+/* 8-node quadrilaterals with 10 Huber edge functions (spk_quad.c)           */
+/* ------------------------------------------------------------------------ */
+typedef struct spq_ctx spq_ctx;
+/* the height-field mesh of sorting_quad: Lx x Ly, nx x ny elements,
+ * z[(2nx+1)*(2ny+1)] in file order (x index fastest) */
+spq_ctx *spq_create(double lambda, double er1_re, double er1_im, double er2_re, double er2_im,
+                    double Lx, double Ly, int nx, int ny, const double *z);
+void spq_destroy(spq_ctx *c);
+int spq_n_edges(const spq_ctx *c);
+int spq_n_elements(const spq_ctx *c);
+int spq_n_nodes(const spq_ctx *c);
+void spq_nodes(const spq_ctx *c, double *pp);
+void spq_elements(const spq_ctx *c, int *v8);
+void spq_edges(const spq_ctx *c, int *edge_p5);
+/* impedance_edge_versus_element(s, t) for 1-based s (edge function) and t
+ * (element): out = Sum_aa(3, 10) as [k][n] complex */
+void spq_edge_element(const spq_ctx *c, int s, int t, double *out);
+/* all (s, t) and the assembled 2M x 2M system (row-major complex),
+ * formulation 1 PMCHWT, 2 MCTF, 3 ICTF */
+void spq_assemble(const spq_ctx *c, int formulation, double *MM, int nthreads);
+/* probes: element geometry at (k, e) (68 doubles, see spk_quad.c) and the
+ * regular rule of any order */
+void spq_geometry_probe(double k, double e, const double *El24, double *out);
+void spq_regular_pair(const spq_ctx *c, int s, int t, int ngp, double *out);
+
 #ifdef __cplusplus
 }
 #endif

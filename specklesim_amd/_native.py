@@ -19,6 +19,9 @@ EXPORTS = [
     "spk_num_edges", "spk_matvec", "spk_matvec_device", "spk_matvec_c64",
     "spk_matvec_c64_device", "spk_matvec_parts", "spk_rhs_plane_wave",
     "spk_rhs_plane_wave_device", "spk_scattered_field", "spk_scattered_field_device",
+    "spk_quad_create", "spk_quad_destroy", "spk_quad_setup", "spk_quad_sizes",
+    "spk_quad_get_mesh", "spk_quad_get_edges", "spk_quad_edge_element", "spk_quad_fill",
+    "spk_quad_fill_device", "spk_quad_topology_probe",
     "spk_device_alloc", "spk_device_free", "spk_device_copy", "spk_device_synchronize",
     "spk_gmres", "spk_gmres_device", "spk_gmres_history", "spk_get_rwg", "spk_get_tree",
     "spk_get_i4", "spk_get_near_csr", "spk_near_blocks", "spk_get_patterns",
@@ -91,6 +94,18 @@ def lib():
     L.spk_rhs_plane_wave.argtypes = [vp, dp, dp, dp]
     L.spk_rhs_plane_wave_device.argtypes = [vp, dp, dp, vp]
     L.spk_scattered_field.argtypes = [vp, C.c_int, dp, dp, dp, dp]
+    L.spk_quad_create.argtypes = [C.POINTER(Params), C.POINTER(vp)]
+    L.spk_quad_destroy.argtypes = [vp]
+    L.spk_quad_destroy.restype = None
+    L.spk_quad_setup.argtypes = [vp, C.c_double, C.c_double, C.c_int, C.c_int, dp]
+    L.spk_quad_sizes.argtypes = [vp, ip, ip, ip]
+    L.spk_quad_get_mesh.argtypes = [vp, dp, ip]
+    L.spk_quad_get_edges.argtypes = [vp, ip]
+    L.spk_quad_edge_element.argtypes = [vp, C.c_int, ip, ip, dp]
+    L.spk_quad_fill.argtypes = [vp, dp]
+    L.spk_quad_fill_device.argtypes = [vp, vp]
+    L.spk_quad_topology_probe.argtypes = [C.c_double, C.c_double, C.c_int, C.c_int, dp, ip, ip, ip,
+                                          dp, ip, ip]
     L.spk_device_alloc.argtypes = [C.c_int, C.c_size_t, C.POINTER(vp)]
     L.spk_device_free.argtypes = [vp]
     L.spk_device_copy.argtypes = [vp, vp, C.c_size_t, C.c_int]

@@ -16,8 +16,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libspeckle_gpu.so")
 BUILD_DIR = os.path.join(HERE, "_build")
 
-HIP_SOURCES = ["spk_kernels.hip", "spk_api.hip"]
-CXX_SOURCES = ["spk_host.cpp"]
+HIP_SOURCES = ["spk_kernels.hip", "spk_api.hip", "spk_quad.hip"]
+CXX_SOURCES = ["spk_host.cpp", "spk_quad_host.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SPK_OFFLOAD_ARCH", "gfx950")

@@ -20,14 +20,17 @@
 
 using namespace spk;
 
-namespace {
-
+namespace spk {
 thread_local std::string g_err;
-
-int fail(int code, const std::string &msg) {
+int set_error(int code, const std::string &msg) {
   g_err = msg;
   return code;
 }
+}  // namespace spk
+
+namespace {
+
+int fail(int code, const std::string &msg) { return spk::set_error(code, msg); }
 
 #define HIPCHK(expr)                                                                    \
   do {                                                                                  \
@@ -411,7 +414,7 @@ void stage_bytes(const spk_handle *h, double *b, double c = 16.0) {
 
 extern "C" {
 
-const char *spk_last_error(void) { return g_err.c_str(); }
+const char *spk_last_error(void) { return spk::g_err.c_str(); }
 
 int spk_create(const spk_params *params, spk_handle **out) {
   if (!params || !out) return fail(SPK_ERR_ARG, "null argument");

@@ -111,6 +111,8 @@ void make_tl_level(const Tables &tb, const Material &mat, const Tree &t, int l,
 void make_phase_level(const Tables &tb, const Material &mat, const Tree &t, int l,
                       std::vector<cplx> *up, std::vector<cplx> *down);
 void quadrature_tri(int ngp, double *a, double *b, double *w);
+// the library's error message (spk_last_error); returns code
+int set_error(int code, const std::string &msg);
 void gauss_legendre(int n, double *x, double *w);
 void hankel2_sph(int n, double zre, double zim, cplx *out);
 

@@ -172,3 +172,42 @@ def read_pp_tt(pp_path, tt_path):
     pp = np.loadtxt(pp_path, dtype=np.float64).reshape(-1, 3)
     tt = np.loadtxt(tt_path, dtype=np.int64).reshape(-1, 3).astype(np.int32)
     return pp, tt
+
+
+# ---------------------------------------------------------------------------
+# 8-node quadrilateral height fields (the reference's quad surface input)
+# ---------------------------------------------------------------------------
+def rough_height_field(nx: int = 10, ny: int = 10, lx: float = 2.0, ly: float = 2.0,
+                       rms: float = 0.1, corr: float = 0.5, seed: int = 1,
+                       wavelength: float = 633e-9):
+    """Gaussian rough height field for nx x ny 8-node quadrilaterals: node
+    heights on the (2nx+1) x (2ny+1) grid, same statistics and generator as
+    rough_surface.  Returns (Lx, Ly, nx, ny, z) with lengths in metres and z
+    in the height file's order (x index fastest)."""
+    gx, gy = 2 * nx + 1, 2 * ny + 1
+    w = gaussian_normals(seed, gx * gy).reshape(gy, gx)
+    kx = 2 * np.pi * np.fft.fftfreq(gx, d=lx / (gx - 1))
+    ky = 2 * np.pi * np.fft.fftfreq(gy, d=ly / (gy - 1))
+    kk = kx[None, :] ** 2 + ky[:, None] ** 2
+    h = np.real(np.fft.ifft2(np.fft.fft2(w) * np.exp(-kk * corr * corr / 8.0)))
+    h = h - h.mean()
+    h = h * (rms / np.sqrt(np.mean(h * h)))
+    return lx * wavelength, ly * wavelength, nx, ny, (h * wavelength).ravel()
+
+
+def write_quad_height_file(path, Lx, Ly, nx, ny, z):
+    """The reference's quad surface file (src/lib_sie_quad_data_container.f90:325-345):
+    line 1 `Lx Ly`, line 2 `Nx Ny`, then (2Nx+1)(2Ny+1) heights, x index fastest."""
+    with open(path, "w") as f:
+        f.write(f"{Lx:.17e} {Ly:.17e}\n{nx} {ny}\n")
+        for v in np.asarray(z, float).ravel():
+            f.write(f"{v:.17e}\n")
+
+
+def read_quad_height_file(path):
+    with open(path) as f:
+        toks = f.read().split()
+    Lx, Ly = float(toks[0]), float(toks[1])
+    nx, ny = int(float(toks[2])), int(float(toks[3]))
+    z = np.array([float(t) for t in toks[4:4 + (2 * nx + 1) * (2 * ny + 1)]])
+    return Lx, Ly, nx, ny, z

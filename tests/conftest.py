@@ -14,8 +14,8 @@ def pytest_configure(config):
 
 def _ensure_oracle():
     lib = os.path.join(ROOT, "oracle", "libspk_oracle.so")
-    src = os.path.join(ROOT, "oracle", "spk_oracle.c")
-    if not os.path.exists(lib) or os.path.getmtime(lib) < os.path.getmtime(src):
+    srcs = [os.path.join(ROOT, "oracle", f) for f in ("spk_oracle.c", "spk_quad.c", "spk_oracle.h")]
+    if not os.path.exists(lib) or os.path.getmtime(lib) < max(os.path.getmtime(s) for s in srcs):
         import __graft_entry__
 
         __graft_entry__._build_oracle()
