@@ -128,7 +128,7 @@ typedef struct spk_quad spk_quad;
  * set_local_material_parameters_quad (src/lib_sie_quad_data_container.f90:118-129). */
 int spk_quad_create(const spk_params *params, spk_quad **out);
 void spk_quad_destroy(spk_quad *q);
-/* The height-field surface of sorting_quad (src/lib_sie_quad_data_container.f90:325-404):
+/* The height-field surface of sorting_quad (src/lib_sie_quad_data_container.f90:321-403):
  * Lx x Ly (m), nx x ny 8-node elements, z[(2nx+1)*(2ny+1)] node heights in the
  * height file's order (x index fastest); then the edge/corner neighbours
  * (Neighbours_EdgeCorner_Full, src/lib_sie_quad_calculation_mod.f90:952-995) and

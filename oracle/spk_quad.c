@@ -6,7 +6,7 @@
  * against.  Nothing in the product links or calls it.
  *
  * Follows (file:line in the reference):
- *   sorting_quad                        src/lib_sie_quad_data_container.f90:325-404
+ *   sorting_quad                        src/lib_sie_quad_data_container.f90:321-403
  *   set_local_material_parameters_quad  src/lib_sie_quad_data_container.f90:118-129
  *   Neighbours_EdgeCorner_Full          src/lib_sie_quad_calculation_mod.f90:952-995
  *   Common_EdgeCorner                   :997-1024

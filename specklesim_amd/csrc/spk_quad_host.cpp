@@ -1,6 +1,6 @@
 // spk_quad_host.cpp -- host-side mesh and topology of the 8-node
 // quadrilateral discretisation: the height-field grid (sorting_quad,
-// src/lib_sie_quad_data_container.f90:325-404), edge and corner neighbours
+// src/lib_sie_quad_data_container.f90:321-403), edge and corner neighbours
 // (Neighbours_EdgeCorner_Full / Common_EdgeCorner,
 // src/lib_sie_quad_calculation_mod.f90:952-1024), the global numbering of
 // the 10 edge functions per element (Sorting_edge / find_edges_in_element,

@@ -76,6 +76,42 @@ module spk_gpu_binding
          real(c_double), intent(in) :: k_in(3), e_in(3)
          complex(c_double_complex), intent(out) :: b(*)
       end function
+      ! 8-node quadrilaterals: the fill of lib_sie_quad_solver_normal
+      integer(c_int) function spk_quad_create(params, handle) bind(c, name="spk_quad_create")
+         import :: c_int, c_ptr, spk_params
+         type(spk_params), intent(in) :: params
+         type(c_ptr), intent(out) :: handle
+      end function
+      subroutine spk_quad_destroy(handle) bind(c, name="spk_quad_destroy")
+         import :: c_ptr
+         type(c_ptr), value :: handle
+      end subroutine
+      integer(c_int) function spk_quad_setup(handle, lx, ly, nx, ny, z) bind(c, name="spk_quad_setup")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: handle
+         real(c_double), value :: lx, ly
+         integer(c_int), value :: nx, ny
+         real(c_double), intent(in) :: z(*)
+      end function
+      integer(c_int) function spk_quad_sizes(handle, n_edges, n_elements, n_nodes) &
+            bind(c, name="spk_quad_sizes")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: handle
+         integer(c_int), intent(out) :: n_edges, n_elements, n_nodes
+      end function
+      integer(c_int) function spk_quad_edge_element(handle, n, s, t, out) &
+            bind(c, name="spk_quad_edge_element")
+         import :: c_int, c_ptr, c_double_complex
+         type(c_ptr), value :: handle
+         integer(c_int), value :: n
+         integer(c_int), intent(in) :: s(*), t(*)
+         complex(c_double_complex), intent(out) :: out(*)
+      end function
+      integer(c_int) function spk_quad_fill(handle, mm) bind(c, name="spk_quad_fill")
+         import :: c_int, c_ptr, c_double_complex
+         type(c_ptr), value :: handle
+         complex(c_double_complex), intent(out) :: mm(*)
+      end function
       integer(c_int) function spk_gmres(handle, b, x, p, info) bind(c, name="spk_gmres")
          import :: c_int, c_ptr, c_double_complex, spk_gmres_params, spk_gmres_info
          type(c_ptr), value :: handle

@@ -196,7 +196,7 @@ def rough_height_field(nx: int = 10, ny: int = 10, lx: float = 2.0, ly: float = 
 
 
 def write_quad_height_file(path, Lx, Ly, nx, ny, z):
-    """The reference's quad surface file (src/lib_sie_quad_data_container.f90:325-345):
+    """The reference's quad surface file (src/lib_sie_quad_data_container.f90:330-345):
     line 1 `Lx Ly`, line 2 `Nx Ny`, then (2Nx+1)(2Ny+1) heights, x index fastest."""
     with open(path, "w") as f:
         f.write(f"{Lx:.17e} {Ly:.17e}\n{nx} {ny}\n")
