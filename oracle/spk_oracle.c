@@ -1546,7 +1546,10 @@ void spo_rhs_plane(const spo_ctx *c, const double *kin, const double *ein, doubl
         double Ra[3], rfm[3];
         simplex(qa[i], qb[i], f->c, Ra);
         for (int k = 0; k < 3; ++k) rfm[k] = (Ra[k] - f->c[2][k]) * f->df * 0.5;
-        zc kk[3] = {c->k1 * kin[0], c->k1 * kin[1], c->k1 * kin[2]};
+        /* dot_product(k1*k_in, R_a) (:977): Fortran conjugates the complex
+         * first argument, so the phase is conj(k1) k_in.R */
+        const zc k1c = conj(c->k1);
+        zc kk[3] = {k1c * kin[0], k1c * kin[1], k1c * kin[2]};
         zc ph = kk[0] * Ra[0] + kk[1] * Ra[1] + kk[2] * Ra[2];
         zc e = cexp(-I * ph);
         zc E[3] = {ein[0] * e, ein[1] * e, ein[2] * e};

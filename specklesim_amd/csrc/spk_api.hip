@@ -102,7 +102,7 @@ struct spk_handle {
   DBuf<int> d_row_box, d_row_r0;
   int spmv_wgs = 0;
   DBuf<zd> d_rt, d_W;
-  DBuf<double> d_khat, d_tth, d_tph, d_q, d_leaf_centre;
+  DBuf<double> d_khat, d_tth, d_tph, d_q, d_leaf_centre, d_trf;
   std::vector<LevelDev> lev;
   DBuf<zd> d_x, d_y, d_yfar, d_tmp;
   zd sc1{1, 0}, sc2{1, 0};
@@ -118,6 +118,9 @@ struct spk_handle {
   bool diag_ready = false;
   // gmres workspace
   DBuf<zd> g_V, g_w, g_r0, g_xc, g_b, g_h, g_partial, g_scal, g_y;
+  // one-reduce MGS: T = (I + L)^-1, [c; g], T^H g, partial sums of V h and of T
+  DBuf<zd> g_T, g_c, g_tg, g_pn, g_pT;
+  int g_T_m = -1;
   DBuf<unsigned long long> g_ctl;    // MGS sweep abort flag
   DBuf<unsigned char> g_slots;       // MGS sweep exchange slots
   unsigned long long sweep_seq = 0;  // sequence number of the last sweep launch
@@ -210,6 +213,7 @@ int setup_mlfmm(spk_handle *h) {
   for (size_t b = 0; b < LL.n.size(); ++b) box_centre(t, LL.n[b], t.l_max, &lc[3 * b]);
   HIPCHK(h->d_leaf_centre.upload(lc));
   HIPCHK(h->d_khat.upload(h->tb.khat));
+  HIPCHK(h->d_trf.upload(h->tb.trf));
   HIPCHK(h->d_tth.upload(h->tb.ttheta));
   HIPCHK(h->d_tph.upload(h->tb.tphi));
   std::vector<double> q(60);
@@ -243,11 +247,26 @@ int setup_mlfmm(spk_handle *h) {
     HIPCHK(D.i4_off.upload(L.i4_off));
     HIPCHK(D.i4_box.upload(L.i4_box));
     HIPCHK(D.i4_slot.upload(L.i4_slot));
-    std::vector<cplx> tl;
-    make_tl_level(h->tb, h->mat, t, l, &tl, nthreads);
-    std::vector<zd> tlz(tl.size());
-    for (size_t i = 0; i < tl.size(); ++i) tlz[i] = zd{tl[i].re, tl[i].im};
-    HIPCHK(D.tlt.upload(tlz));
+    {
+      // TL on the device from the host's per-slot Hankel factors
+      std::vector<double> rhat;
+      std::vector<cplx> coef;
+      std::vector<int> live;
+      tl_coefficients(h->tb, h->mat, t, l, &rhat, &coef, &live);
+      std::vector<zd> cz(coef.size());
+      for (size_t i = 0; i < coef.size(); ++i) cz[i] = zd{coef[i].re, coef[i].im};
+      DBuf<double> d_rhat;
+      DBuf<zd> d_coef;
+      DBuf<int> d_live;
+      HIPCHK(d_rhat.upload(rhat));
+      HIPCHK(d_coef.upload(cz));
+      HIPCHK(d_live.upload(live));
+      HIPCHK(D.tlt.alloc(343 * 2 * (size_t)S));
+      launch_tl_level(S, h->tb.K, h->d_khat.p, h->d_trf.p, d_rhat.p, d_coef.p, d_live.p, D.tlt.p,
+                      h->st);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipStreamSynchronize(h->st));
+    }
     std::vector<cplx> up, down;
     make_phase_level(h->tb, h->mat, t, l, &up, &down);
     std::vector<zd> uz(up.size()), dz(down.size());
@@ -629,7 +648,9 @@ static int rhs_plane_dev(spk_handle *h, const double k_in[3], const double e_in[
     a.qb[i] = h->qb[i];
     a.qw[i] = h->qw[i];
   }
-  a.k1 = zd{h->mat.k1.re, h->mat.k1.im};
+  // the phase is dot_product(k1*k_in, R_a) (:977), and Fortran's
+  // dot_product conjugates a complex first argument: exp(-i conj(k1) k_in.R)
+  a.k1 = zd{h->mat.k1.re, -h->mat.k1.im};
   for (int d = 0; d < 3; ++d) {
     a.kin[d] = k_in[d];
     a.ein[d] = e_in[d];
@@ -807,11 +828,13 @@ static int gmres_mixed(spk_handle *h, const zd *b, zd *x, const spk_gmres_params
   }
   HIPCHK(h->g_h.alloc(m + 1));
   HIPCHK(h->g_y.alloc(m + 1));
-  HIPCHK(h->g_partial.alloc((size_t)reduce_blocks(n) * (m + 1)));
+  const size_t gc_elems = gemv_c_partial_elems(n, m + 1, 1);
+  HIPCHK(h->g_partial.alloc(gc_elems + (size_t)reduce_blocks(n)));
+  HIPCHK(h->g_pn.alloc(gemv_n_partial_elems(n, m + 1)));
   HIPCHK(h->g_scal.alloc(4));
   zf *V = h->i_V.p, *w = h->i_w.p;
   zd *r = h->g_r0.p, *tmp = h->g_w.p, *d = h->g_xc.p;
-  zd *partial_nrm = h->g_partial.p + (size_t)reduce_blocks(n) * m;
+  zd *partial_nrm = h->g_partial.p + gc_elems;
   h->hist.clear();
   int rc = SPK_OK;
   auto dot_host = [&](const auto *u, const auto *v, double *out) -> int {
@@ -871,8 +894,9 @@ static int gmres_mixed(spk_handle *h, const zd *b, zd *x, const spk_gmres_params
       double dnormw = 0.0;
       for (;;) {
         ++nOrtho;
-        launch_gemv_c(n, j + 1, (const zf *)V, n, (const zf *)w, h->g_partial.p, h->g_h.p, h->st);
-        launch_gemv_n_sub(n, j + 1, (const zf *)V, n, h->g_h.p, w, h->st);
+        launch_gemv_c(n, j + 1, (const zf *)V, n, (const zf *)w, (const zf *)nullptr,
+                      h->g_partial.p, h->g_h.p, h->st);
+        launch_gemv_n_sub(n, j + 1, (const zf *)V, n, h->g_h.p, w, h->g_pn.p, h->st);
         launch_zdotc(n, (const zf *)w, (const zf *)w, partial_nrm, h->g_scal.p, h->st);
         std::vector<zd> hv(j + 2);
         HIPCHK(hipMemcpyAsync(hv.data(), h->g_h.p, (j + 1) * sizeof(zd), hipMemcpyDeviceToHost,
@@ -918,7 +942,7 @@ static int gmres_mixed(spk_handle *h, const zd *b, zd *x, const spk_gmres_params
     std::vector<zd> yd(k);
     for (int j = 0; j < k; ++j) yd[j] = zd{yc[j].real(), yc[j].imag()};
     HIPCHK(hipMemcpyAsync(h->g_y.p, yd.data(), k * sizeof(zd), hipMemcpyHostToDevice, h->st));
-    launch_gemv_n(n, k, (const zf *)V, n, h->g_y.p, d, h->st);
+    launch_gemv_n(n, k, (const zf *)V, n, h->g_y.p, d, h->g_pn.p, h->st);
     launch_zaxpy(n, zd{rn, 0.0}, d, x, h->st);
     if ((rc = residual(&rn))) return rc;
     be = rn / bn;
@@ -963,13 +987,30 @@ int spk_gmres_device(spk_handle *h, const void *d_b, void *d_x, const spk_gmres_
   }
   HIPCHK(h->g_h.alloc(m + 1));
   HIPCHK(h->g_y.alloc(m + 1));
-  HIPCHK(h->g_partial.alloc((size_t)reduce_blocks(n) * (m + 1)));
+  const size_t gc_elems = gemv_c_partial_elems(n, m + 1, 2);
+  HIPCHK(h->g_partial.alloc(gc_elems + (size_t)reduce_blocks(n)));
+  zd *partial_nrm = h->g_partial.p + gc_elems;
+  HIPCHK(h->g_pn.alloc(gemv_n_partial_elems(n, m + 1)));
   HIPCHK(h->g_scal.alloc(4));
-  // SPK_MGS_SWEEP=0 selects the per-step launches instead of the cooperative sweep
-  const char *sw = std::getenv("SPK_MGS_SWEEP");
-  const bool sweep_on = !(sw && sw[0] == '0');
-  const int sweepG =
-      (sweep_on && (orth == SPK_ORTH_MGS || orth == SPK_ORTH_IMGS)) ? mgs_sweep_blocks(n) : 0;
+  // MGS and IMGS passes: by default the one-reduce form (h = (I + L)^-1 V^H w,
+  // two reads of V per pass); SPK_MGS=sweep selects the cooperative
+  // column-by-column sweep, SPK_MGS=steps one launch pair per column.
+  const char *mgs_env = std::getenv("SPK_MGS");
+  const std::string mgs_kind = mgs_env ? mgs_env : "";
+  const bool is_mgs = orth == SPK_ORTH_MGS || orth == SPK_ORTH_IMGS;
+  const int sweepG = (is_mgs && mgs_kind == "sweep") ? mgs_sweep_blocks(n) : 0;
+  const bool mgs_steps = is_mgs && mgs_kind == "steps";
+  const bool one_reduce = is_mgs && !sweepG && !mgs_steps;
+  const int ldt = m + 1;
+  if (one_reduce && (h->g_T_m != m || !h->g_T.p)) {
+    HIPCHK(h->g_T.alloc((size_t)ldt * ldt));
+    HIPCHK(hipMemsetAsync(h->g_T.p, 0, (size_t)ldt * ldt * sizeof(zd), h->st));
+    HIPCHK(h->g_c.alloc(2 * (size_t)ldt));
+    HIPCHK(h->g_tg.alloc(ldt));
+    // T products: k <= ldt columns, at most ceil(k / 32) column chunks
+    HIPCHK(h->g_pT.alloc(std::max(gemv_c_partial_elems(ldt, ldt, 1), (size_t)((ldt + 31) / 32) * ldt)));
+    h->g_T_m = m;
+  }
   if (sweepG) {
     const size_t need = (size_t)(m + 1) * 256 * mgs_sweep_slot_bytes();
     if (h->g_slots.n < need) {
@@ -1059,17 +1100,33 @@ int spk_gmres_device(spk_handle *h, const void *d_b, void *d_x, const spk_gmres_
           // the whole MGS pass and |w|^2 in one cooperative launch
           HIPCHK(launch_mgs_sweep(sweepG, n, jH, V, n, w, h->g_slots.p, h->g_h.p, h->g_scal.p,
                                   h->g_ctl.p, ++h->sweep_seq, h->st));
-        } else if (orth == SPK_ORTH_MGS || orth == SPK_ORTH_IMGS) {
+        } else if (one_reduce) {
+          zd *Tm = h->g_T.p, *c = h->g_c.p;
+          if (nOrtho == 1 && jH > 1) {
+            // c = V^H w and g = V^H v_{jH-1} in one read of V; T grows by row jH-1
+            launch_gemv_c(n, jH, (const zd *)V, n, (const zd *)w,
+                          (const zd *)(V + (size_t)(jH - 1) * n), h->g_partial.p, c, h->st);
+            launch_gemv_c((int64_t)(jH - 1), jH - 1, (const zd *)Tm, (int64_t)ldt,
+                          (const zd *)(c + jH), (const zd *)nullptr, h->g_pT.p, h->g_tg.p, h->st);
+            launch_icwy_row(jH - 1, Tm, ldt, h->g_tg.p, h->st);
+          } else {
+            launch_gemv_c(n, jH, (const zd *)V, n, (const zd *)w, (const zd *)nullptr,
+                          h->g_partial.p, c, h->st);
+            if (nOrtho == 1) launch_icwy_row(0, Tm, ldt, h->g_tg.p, h->st);
+          }
+          launch_gemv_n((int64_t)jH, jH, (const zd *)Tm, (int64_t)ldt, c, h->g_h.p, h->g_pT.p, h->st);
+          launch_gemv_n_sub(n, jH, (const zd *)V, n, h->g_h.p, w, h->g_pn.p, h->st);
+        } else if (mgs_steps) {
           for (int j = 0; j < jH; ++j)
             launch_mgs_step(n, j ? V + (size_t)(j - 1) * n : nullptr, j ? h->g_h.p + (j - 1) : nullptr,
                             V + (size_t)j * n, w, h->g_partial.p, h->g_h.p + j, h->st);
           launch_zaxpy_dev(n, h->g_h.p + (jH - 1), -1.0, V + (size_t)(jH - 1) * n, w, h->st);
         } else {
-          launch_gemv_c(n, jH, V, n, w, h->g_partial.p, h->g_h.p, h->st);
-          launch_gemv_n_sub(n, jH, V, n, h->g_h.p, w, h->st);
+          launch_gemv_c(n, jH, (const zd *)V, n, (const zd *)w, (const zd *)nullptr,
+                        h->g_partial.p, h->g_h.p, h->st);
+          launch_gemv_n_sub(n, jH, (const zd *)V, n, h->g_h.p, w, h->g_pn.p, h->st);
         }
-        if (!sweepG)
-          launch_zdotc(n, w, w, h->g_partial.p + (size_t)reduce_blocks(n) * m, h->g_scal.p, h->st);
+        if (!sweepG) launch_zdotc(n, w, w, partial_nrm, h->g_scal.p, h->st);
         std::vector<zd> hv(jH + 1);
         HIPCHK(hipMemcpyAsync(hv.data(), h->g_h.p, jH * sizeof(zd), hipMemcpyDeviceToHost, h->st));
         HIPCHK(hipMemcpyAsync(&hv[jH], h->g_scal.p, sizeof(zd), hipMemcpyDeviceToHost, h->st));
@@ -1121,7 +1178,7 @@ int spk_gmres_device(spk_handle *h, const void *d_b, void *d_x, const spk_gmres_
         std::vector<zd> yd(jH);
         for (int j = 0; j < jH; ++j) yd[j] = zd{yc[j].real(), yc[j].imag()};
         HIPCHK(hipMemcpyAsync(h->g_y.p, yd.data(), jH * sizeof(zd), hipMemcpyHostToDevice, h->st));
-        launch_gemv_n(n, jH, V, n, h->g_y.p, r0, h->st);
+        launch_gemv_n(n, jH, (const zd *)V, n, h->g_y.p, r0, h->g_pn.p, h->st);
         HIPCHK(hipMemcpyAsync(xc, x, n * sizeof(zd), hipMemcpyDeviceToDevice, h->st));
         launch_zadd(n, r0, xc, h->st);
         if ((rc_ = matvec_dev(h, xc, w))) return rc_;
@@ -1162,7 +1219,7 @@ int spk_gmres_device(spk_handle *h, const void *d_b, void *d_x, const spk_gmres_
       std::vector<zd> yd(jH);
       for (int j = 0; j < jH; ++j) yd[j] = zd{yc[j].real(), yc[j].imag()};
       HIPCHK(hipMemcpyAsync(h->g_y.p, yd.data(), jH * sizeof(zd), hipMemcpyHostToDevice, h->st));
-      launch_gemv_n(n, jH, V, n, h->g_y.p, r0, h->st);
+      launch_gemv_n(n, jH, (const zd *)V, n, h->g_y.p, r0, h->g_pn.p, h->st);
       HIPCHK(hipMemcpyAsync(xc, x, n * sizeof(zd), hipMemcpyDeviceToDevice, h->st));
       launch_zadd(n, r0, xc, h->st);
     }
@@ -1184,7 +1241,7 @@ int spk_gmres_device(spk_handle *h, const void *d_b, void *d_x, const spk_gmres_
       for (int j = 0; j <= m; ++j) hd[j] = zd{HH(j, m).real(), HH(j, m).imag()};
       HIPCHK(hipMemcpyAsync(h->g_y.p, hd.data(), (m + 1) * sizeof(zd), hipMemcpyHostToDevice,
                             h->st));
-      launch_gemv_n(n, m + 1, V, n, h->g_y.p, w, h->st);
+      launch_gemv_n(n, m + 1, (const zd *)V, n, h->g_y.p, w, h->g_pn.p, h->st);
     }
   }
   HIPCHK(hipStreamSynchronize(h->st));

@@ -123,6 +123,9 @@ void launch_near_spmv_rows(int nwg, int N, const int *row_box, const int *row_r0
                            const int *e2_list, const int64_t *near_off, const int *perm,
                            const T *blocks, const T *x, const T *yadd, T *y, hipStream_t st);
 void launch_to_zf(int64_t n, const zd *a, zf *b, hipStream_t st);
+// TL table of one level from the host's per-slot series factors
+void launch_tl_level(int S, int K, const double *khat, const double *trf, const double *rhat,
+                     const zd *coef, const int *live, zd *tlt, hipStream_t st);
 constexpr int kSpmvRowsPerWG = 16;
 // arbitrary (m, n) pairs, one thread per pair (parity probe)
 void launch_near_pairs(const NearArgs &a, int npairs, const int *m, const int *n, zd *out,
@@ -178,15 +181,22 @@ size_t mgs_sweep_slot_bytes();
 hipError_t launch_mgs_sweep(int G, int64_t n, int k, const zd *V, int64_t ldv, zd *w, void *slots,
                             zd *hout, zd *nrm, unsigned long long *ctl, unsigned long long seq,
                             hipStream_t st);
+// out[v*k + j] = <V_j, w_v> for j < k, v = 0 (w0) and, when w1 is not null,
+// v = 1 (w1); partial: gemv_c_partial_elems(n, k, 1 or 2) complex128
 template <class T>
-void launch_gemv_c(int64_t n, int k, const T *V, int64_t ldv, const T *w, zd *partial,
-                   zd *out, hipStream_t st);
+void launch_gemv_c(int64_t n, int k, const T *V, int64_t ldv, const T *w0, const T *w1,
+                   zd *partial, zd *out, hipStream_t st);
+size_t gemv_c_partial_elems(int64_t n, int kmax, int nv);
+// w -= V h and out = V h over k columns; partial: gemv_n_partial_elems(n, k)
 template <class T>
-void launch_gemv_n_sub(int64_t n, int k, const T *V, int64_t ldv, const zd *h, T *w,
+void launch_gemv_n_sub(int64_t n, int k, const T *V, int64_t ldv, const zd *h, T *w, zd *partial,
                        hipStream_t st);
 template <class T>
-void launch_gemv_n(int64_t n, int k, const T *V, int64_t ldv, const zd *h, zd *out,
+void launch_gemv_n(int64_t n, int k, const T *V, int64_t ldv, const zd *h, zd *out, zd *partial,
                    hipStream_t st);
+size_t gemv_n_partial_elems(int64_t n, int kmax);
+// one-reduce MGS: row k1 of T = (I + L)^-1 from tg = T^H g
+void launch_icwy_row(int k1, zd *Tm, int ldt, const zd *tg, hipStream_t st);
 void launch_zadd(int64_t n, const zd *a, zd *y, hipStream_t st);  // y += a
 
 }  // namespace spk

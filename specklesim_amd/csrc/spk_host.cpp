@@ -603,6 +603,44 @@ void make_tl_level(const Tables &tb, const Material &mat, const Tree &t, int l,
   for (auto &x : th) x.join();
 }
 
+// The per-slot factors of the TL series for the device table kernel: for
+// slot ts of the 7^3 offset grid and medium med, rhat (3 doubles) and
+// c_q = h_q^(2)(k r) (-i)^(q+1) (2q+1) k / (4 pi)^2, q = 0..K-1, so that
+// TL(khat) = sum_q c_q P_q(rhat.khat); live = 0 for the zeroed slots (the
+// 3^3 near cube, and the lossy interior medium above the leaf level).
+void tl_coefficients(const Tables &tb, const Material &mat, const Tree &t, int l,
+                     std::vector<double> *rhat, std::vector<cplx> *coef,
+                     std::vector<int> *live) {
+  const int K = tb.K;
+  rhat->assign(343 * 3, 0.0);
+  coef->assign(343 * 2 * (size_t)K, cplx{0, 0});
+  live->assign(343 * 2, 0);
+  const double d_c = (1.0 / (double)(1LL << l)) * t.D;
+  const double x_min = -(3 * d_c), x_max = -x_min, dx = (x_max - x_min) / 6;
+  const bool lossy_skip = (std::fabs(mat.k2.im) > 0.0) && (l < t.l_max);
+  std::vector<cplx> hl(K);
+  for (int ts = 0; ts < 343; ++ts) {
+    const int i = ts / 49, j = (ts / 7) % 7, k = ts % 7;
+    if (i >= 2 && i <= 4 && j >= 2 && j <= 4 && k >= 2 && k <= 4) continue;
+    const double r[3] = {x_min + i * dx, x_min + j * dx, x_min + k * dx};
+    const double rl = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+    for (int d = 0; d < 3; ++d) (*rhat)[3 * ts + d] = r[d] / rl;
+    for (int med = 0; med < 2; ++med) {
+      if (med == 1 && lossy_skip) continue;
+      const zc kd = med == 0 ? Z(mat.k1) : Z(mat.k2);
+      const zc zr = kd * rl;
+      hankel2_sph(K, zr.real(), zr.imag(), hl.data());
+      zc mi = 1.0;
+      for (int q = 1; q <= K; ++q) {
+        mi = mi * zc(0.0, -1.0);
+        (*coef)[((size_t)ts * 2 + med) * K + (q - 1)] =
+            C(Z(hl[q - 1]) * mi * (double)(2 * (q - 1) + 1) * kd / ((4 * kPi) * (4 * kPi)));
+      }
+      (*live)[ts * 2 + med] = 1;
+    }
+  }
+}
+
 // SS/RR phases of translation_SS/RR (src/lib_sie_tri_mlfmm_interface.f90:466-655):
 // child level l, octant o: delta = centre(child) - centre(parent)
 // = +-2^-(l+1) D per axis; up = exp(+i k khat.delta), down = exp(-i k khat.delta).

@@ -108,6 +108,8 @@ void make_tables(int K, Tables *tb);
 void make_tl_level(const Tables &tb, const Material &mat, const Tree &t, int l,
                    std::vector<cplx> *out, int nthreads);
 // SS / RR phase tables for child level l: [8][2][S] complex each.
+void tl_coefficients(const Tables &tb, const Material &mat, const Tree &t, int l,
+                     std::vector<double> *rhat, std::vector<cplx> *coef, std::vector<int> *live);
 void make_phase_level(const Tables &tb, const Material &mat, const Tree &t, int l,
                       std::vector<cplx> *up, std::vector<cplx> *down);
 void quadrature_tri(int ngp, double *a, double *b, double *w);

@@ -1145,6 +1145,43 @@ void launch_div_diag(int64_t n, const zd *diag, const zd *x, zd *y, hipStream_t 
 }
 
 // ---------------------------------------------------------------------------
+// TL tables of one level (TL_km_ml_fmm, src/lib_sie_mlfmm.f90:129-188, on the
+// 7^3 offset grid of precalculation_LT3_TL, :27-82), pre-multiplied by
+// TRF: tlt[slot][med][s] = TRF_s sum_q c_q P_q(rhat.khat_s) with the slot
+// factors c_q from the host.  One thread per (slot, medium, sample).
+// ---------------------------------------------------------------------------
+__global__ void k_tl_level(int S, int K, const double *khat, const double *trf,
+                           const double *rhat, const zd *coef, const int *live, zd *tlt) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int sm = blockIdx.y;  // slot * 2 + medium
+  if (s >= S) return;
+  zd *o = tlt + (size_t)sm * S + s;
+  if (!live[sm]) {
+    *o = zd{0, 0};
+    return;
+  }
+  const int ts = sm >> 1;
+  const double x = rhat[3 * ts] * khat[3 * s] + rhat[3 * ts + 1] * khat[3 * s + 1] +
+                   rhat[3 * ts + 2] * khat[3 * s + 2];
+  const zd *c = coef + (size_t)sm * K;
+  double p0 = 1.0, p1 = x;
+  zd acc = c[0] * p0;
+  if (K > 1) acc = acc + c[1] * p1;
+  for (int q = 2; q < K; ++q) {
+    const double p2 = (2.0 * q - 1.0) / q * x * p1 - (q - 1.0) / q * p0;
+    acc = acc + c[q] * p2;
+    p0 = p1;
+    p1 = p2;
+  }
+  *o = acc * trf[s];
+}
+void launch_tl_level(int S, int K, const double *khat, const double *trf, const double *rhat,
+                     const zd *coef, const int *live, zd *tlt, hipStream_t st) {
+  hipLaunchKernelGGL(k_tl_level, dim3(nblk(S, 256), 343 * 2), dim3(256), 0, st, S, K, khat, trf,
+                     rhat, coef, live, tlt);
+}
+
+// ---------------------------------------------------------------------------
 // Plane-wave right-hand side (get_vector_b_tri / incident_field_tri_general,
 // src/lib_sie_tri_calculation_mod.f90:391-460, 939-1002): at the ng rule
 // points of both triangles of edge m, E = e_in exp(-i k1 k_in.r),
@@ -1544,97 +1581,189 @@ hipError_t launch_mgs_sweep(int G, int64_t n, int k, const zd *V, int64_t ldv, z
   return hipErrorInvalidValue;
 }
 
-// h[j] = <V_j, w> for j < k.  Each block owns a contiguous slice of rows and
-// accumulates a tile of 8 columns at a time (one read of its w slice per
-// tile, one 8-wide block reduction per tile): partial[j * G + block].  The
-// final kernel sums the G partials of each j in a fixed order.  Products
-// accumulate in complex128 whatever the storage type T.
-constexpr int kGemvTile = 8;
-template <class T>
-__global__ void __launch_bounds__(256) k_gemv_c_partial(int64_t n, int k, const T *V, int64_t ldv,
-                                                        const T *w, zd *partial) {
-  __shared__ zd red[4][kGemvTile];
-  const int G = gridDim.x, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int64_t chunk = (n + G - 1) / G;
-  const int64_t i0 = blockIdx.x * chunk, i1 = min<int64_t>(n, i0 + chunk);
-  for (int j0 = 0; j0 < k; j0 += kGemvTile) {
-    const int nt = min(kGemvTile, k - j0);
-    zd acc[kGemvTile];
+// Tall-skinny products with the Krylov basis V (n rows, column j at
+// V + j*ldv), dot products accumulated in complex128 whatever the storage
+// type T.  Both kernels split the work over a 2-D grid so that a launch has
+// about 2048 workgroups whatever k is, and write partial sums that a second
+// kernel adds in a fixed order (results do not depend on scheduling).
+//
+// out[v*k + j] = <V_j, w_v> (v < NV): grid (row chunks R) x (column tiles of
+// CT); every thread keeps NV*CT accumulators over its rows of the chunk;
+// partial[(v*k + j) * R + chunk].
+constexpr int kGemvWorkgroups = 2048;
+constexpr int kGemvMaxRowChunks = 256;
+static inline int gemv_c_rows(int64_t n, int ctiles) {
+  const int64_t by_rows = (n + 255) / 256;
+  const int64_t by_fill = (kGemvWorkgroups + ctiles - 1) / ctiles;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(std::min(by_rows, by_fill), kGemvMaxRowChunks));
+}
+size_t gemv_c_partial_elems(int64_t n, int kmax, int nv) {
+  (void)n;
+  return (size_t)nv * (size_t)kmax * kGemvMaxRowChunks;
+}
+template <class T, int NV, int CT>
+__global__ void __launch_bounds__(256) k_gemv_c_tile(int64_t n, int k, const T *V, int64_t ldv,
+                                                     const T *w0, const T *w1, int64_t rows,
+                                                     zd *partial) {
+  __shared__ zd red[4][NV * CT];
+  const int R = gridDim.x, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int j0 = blockIdx.y * CT, nt = min(CT, k - j0);
+  const int64_t i0 = (int64_t)blockIdx.x * rows, i1 = min<int64_t>(n, i0 + rows);
+  zd acc[NV][CT];
 #pragma unroll
-    for (int t = 0; t < kGemvTile; ++t) acc[t] = zd{0, 0};
-    for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-      const zd wi = widen(w[i]);
+  for (int v = 0; v < NV; ++v)
 #pragma unroll
-      for (int t = 0; t < kGemvTile; ++t)
-        if (t < nt) acc[t] = acc[t] + conj(widen(V[(size_t)(j0 + t) * ldv + i])) * wi;
-    }
+    for (int t = 0; t < CT; ++t) acc[v][t] = zd{0, 0};
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) {
+    zd x[NV];
+    x[0] = widen(w0[i]);
+    if (NV > 1) x[NV - 1] = widen(w1[i]);
 #pragma unroll
-    for (int t = 0; t < kGemvTile; ++t) {
-      const zd v = wave_sum(acc[t]);
-      if (lane == 0) red[wv][t] = v;
+    for (int t = 0; t < CT; ++t)
+      if (t < nt) {
+        const zd c = conj(widen(V[(size_t)(j0 + t) * ldv + i]));
+#pragma unroll
+        for (int v = 0; v < NV; ++v) acc[v][t] = acc[v][t] + c * x[v];
+      }
+  }
+#pragma unroll
+  for (int v = 0; v < NV; ++v)
+#pragma unroll
+    for (int t = 0; t < CT; ++t) {
+      const zd r = wave_sum(acc[v][t]);
+      if (lane == 0) red[wv][v * CT + t] = r;
     }
-    __syncthreads();
-    if (threadIdx.x < nt) {
-      const int t = threadIdx.x;
-      partial[(size_t)(j0 + t) * G + blockIdx.x] = red[0][t] + red[1][t] + red[2][t] + red[3][t];
-    }
-    __syncthreads();
+  __syncthreads();
+  if (threadIdx.x < NV * CT) {
+    const int v = threadIdx.x / CT, t = threadIdx.x % CT;
+    if (t < nt)
+      partial[((size_t)v * k + j0 + t) * R + blockIdx.x] =
+          red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
   }
 }
-__global__ void k_gemv_c_final(int nparts, int k, const zd *partial, zd *out) {
-  const int j = blockIdx.x;
+// out[o] = sum_r partial[o * R + r], one wave per output, fixed order
+__global__ void k_gemv_c_final(int nout, int R, const zd *partial, zd *out) {
+  const int o = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (o >= nout) return;
   zd acc = {0, 0};
-  for (int i = threadIdx.x; i < nparts; i += blockDim.x) acc = acc + partial[(size_t)j * nparts + i];
-  acc = block_sum(acc);
-  if (threadIdx.x == 0) out[j] = acc;
+  for (int r = lane; r < R; r += 64) acc = acc + partial[(size_t)o * R + r];
+  acc = wave_sum(acc);
+  if (lane == 0) out[o] = acc;
 }
 template <class T>
-void launch_gemv_c(int64_t n, int k, const T *V, int64_t ldv, const T *w, zd *partial, zd *out,
-                   hipStream_t st) {
+void launch_gemv_c(int64_t n, int k, const T *V, int64_t ldv, const T *w0, const T *w1,
+                   zd *partial, zd *out, hipStream_t st) {
   if (k <= 0) return;
-  hipLaunchKernelGGL(k_gemv_c_partial<T>, dim3(kRedBlocks), dim3(256), 0, st, n, k, V, ldv, w,
-                     partial);
-  hipLaunchKernelGGL(k_gemv_c_final, dim3(k), dim3(256), 0, st, kRedBlocks, k, partial, out);
+  const int nv = w1 ? 2 : 1;
+  const int CT = w1 ? 8 : 16;
+  const int ctiles = (k + CT - 1) / CT;
+  const int R = gemv_c_rows(n, ctiles);
+  const int64_t rows = (n + R - 1) / R;
+  if (w1)
+    hipLaunchKernelGGL((k_gemv_c_tile<T, 2, 8>), dim3(R, ctiles), dim3(256), 0, st, n, k, V, ldv,
+                       w0, w1, rows, partial);
+  else
+    hipLaunchKernelGGL((k_gemv_c_tile<T, 1, 16>), dim3(R, ctiles), dim3(256), 0, st, n, k, V, ldv,
+                       w0, w0, rows, partial);
+  hipLaunchKernelGGL(k_gemv_c_final, dim3((nv * k + 3) / 4), dim3(256), 0, st, nv * k, R, partial,
+                     out);
 }
-// w -= V h (products in complex128, stored as T)
-template <class T>
-__global__ void k_gemv_n_sub(int64_t n, int k, const T *V, int64_t ldv, const zd *h, T *w) {
+
+// V h: grid (row tiles of 256) x (column chunks C); each thread sums its
+// row over the chunk's columns into partial[chunk * n + i]; the final
+// kernel adds the C partials in order and writes out = V h or w -= V h.
+// A single chunk skips the partials.
+static inline int gemv_n_chunks(int64_t n, int k) {
+  const int64_t tiles = (n + 255) / 256;
+  const int64_t by_fill = std::max<int64_t>(1, kGemvWorkgroups / tiles);
+  const int64_t by_cols = std::max(1, (k + 31) / 32);
+  return (int)std::min(by_fill, by_cols);
+}
+size_t gemv_n_partial_elems(int64_t n, int kmax) {
+  return (size_t)gemv_n_chunks(n, kmax) * (size_t)n;
+}
+template <class T, int MODE>  // MODE 0: partial, 1: out = V h, 2: w -= V h
+__global__ void __launch_bounds__(256) k_gemv_n_tile(int64_t n, int k, int cols, const T *V,
+                                                     int64_t ldv, const zd *h, zd *partial,
+                                                     zd *out, T *w) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int j0 = blockIdx.y * cols, j1 = min(k, j0 + cols);
+  zd acc = {0, 0};
+  int j = j0;
+  for (; j + 8 <= j1; j += 8) {
+    zd v[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) v[t] = widen(V[(size_t)(j + t) * ldv + i]);
+#pragma unroll
+    for (int t = 0; t < 8; ++t) acc = acc + v[t] * h[j + t];
+  }
+  for (; j < j1; ++j) acc = acc + widen(V[(size_t)j * ldv + i]) * h[j];
+  if (MODE == 0) partial[(size_t)blockIdx.y * n + i] = acc;
+  else if (MODE == 1) out[i] = acc;
+  else w[i] = cvt<T>(widen(w[i]) - acc);
+}
+template <class T, int MODE>
+__global__ void k_gemv_n_final(int64_t n, int C, const zd *partial, zd *out, T *w) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
-    zd acc = {0, 0};
-    for (int j = 0; j < k; ++j) acc = acc + widen(V[(size_t)j * ldv + i]) * h[j];
-    w[i] = cvt<T>(widen(w[i]) - acc);
+    zd acc = partial[i];
+    for (int c = 1; c < C; ++c) acc = acc + partial[(size_t)c * n + i];
+    if (MODE == 1) out[i] = acc;
+    else w[i] = cvt<T>(widen(w[i]) - acc);
   }
 }
+template <class T, int MODE>
+static void launch_gemv_n_mode(int64_t n, int k, const T *V, int64_t ldv, const zd *h, zd *out,
+                               T *w, zd *partial, hipStream_t st) {
+  const int C = gemv_n_chunks(n, k);
+  const int cols = (k + C - 1) / C;
+  const unsigned tiles = (unsigned)((n + 255) / 256);
+  if (C == 1) {
+    hipLaunchKernelGGL((k_gemv_n_tile<T, MODE>), dim3(tiles, 1), dim3(256), 0, st, n, k, cols, V,
+                       ldv, h, partial, out, w);
+    return;
+  }
+  hipLaunchKernelGGL((k_gemv_n_tile<T, 0>), dim3(tiles, C), dim3(256), 0, st, n, k, cols, V, ldv,
+                     h, partial, out, w);
+  hipLaunchKernelGGL((k_gemv_n_final<T, MODE>), dim3(gcap(n)), dim3(256), 0, st, n, C, partial,
+                     out, w);
+}
 template <class T>
-void launch_gemv_n_sub(int64_t n, int k, const T *V, int64_t ldv, const zd *h, T *w,
+void launch_gemv_n_sub(int64_t n, int k, const T *V, int64_t ldv, const zd *h, T *w, zd *partial,
                        hipStream_t st) {
   if (k <= 0) return;
-  hipLaunchKernelGGL(k_gemv_n_sub<T>, dim3(gcap(n)), dim3(256), 0, st, n, k, V, ldv, h, w);
-}
-// out = V h (complex128 result)
-template <class T>
-__global__ void k_gemv_n(int64_t n, int k, const T *V, int64_t ldv, const zd *h, zd *out) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    zd acc = {0, 0};
-    for (int j = 0; j < k; ++j) acc = acc + widen(V[(size_t)j * ldv + i]) * h[j];
-    out[i] = acc;
-  }
+  launch_gemv_n_mode<T, 2>(n, k, V, ldv, h, nullptr, w, partial, st);
 }
 template <class T>
-void launch_gemv_n(int64_t n, int k, const T *V, int64_t ldv, const zd *h, zd *out,
+void launch_gemv_n(int64_t n, int k, const T *V, int64_t ldv, const zd *h, zd *out, zd *partial,
                    hipStream_t st) {
-  hipLaunchKernelGGL(k_gemv_n<T>, dim3(gcap(n)), dim3(256), 0, st, n, k, V, ldv, h, out);
+  // k = 0 writes zeros (the tile kernel sums an empty range)
+  launch_gemv_n_mode<T, 1>(n, k, V, ldv, h, out, (T *)nullptr, partial, st);
 }
 #define SPK_GEMV_INSTANTIATE(T)                                                                  \
-  template void launch_gemv_c<T>(int64_t, int, const T *, int64_t, const T *, zd *, zd *,         \
-                                 hipStream_t);                                                    \
-  template void launch_gemv_n_sub<T>(int64_t, int, const T *, int64_t, const zd *, T *,           \
+  template void launch_gemv_c<T>(int64_t, int, const T *, int64_t, const T *, const T *, zd *,    \
+                                 zd *, hipStream_t);                                              \
+  template void launch_gemv_n_sub<T>(int64_t, int, const T *, int64_t, const zd *, T *, zd *,     \
                                      hipStream_t);                                                \
-  template void launch_gemv_n<T>(int64_t, int, const T *, int64_t, const zd *, zd *, hipStream_t);
+  template void launch_gemv_n<T>(int64_t, int, const T *, int64_t, const zd *, zd *, zd *,        \
+                                 hipStream_t);
 SPK_GEMV_INSTANTIATE(zd)
 SPK_GEMV_INSTANTIATE(zf)
+
+// One-reduce MGS (the inverse compact WY form of modified Gram-Schmidt):
+// with L the strictly lower part of V^H V, the MGS coefficients of one pass
+// over k basis vectors are h = (I + L)^-1 c, c = V^H w.  Tm holds
+// T = (I + L)^-1 column-major (column j at Tm + j*ldt, rows j..), grown by
+// one row per Arnoldi step: row k1 = -conj(T^H g) with g_i = <v_i, v_k1>.
+__global__ void k_icwy_row(int k1, zd *Tm, int ldt, const zd *tg) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < k1) Tm[(size_t)j * ldt + k1] = -conj(tg[j]);
+  if (j == k1) Tm[(size_t)k1 * ldt + k1] = zd{1.0, 0.0};
+}
+void launch_icwy_row(int k1, zd *Tm, int ldt, const zd *tg, hipStream_t st) {
+  hipLaunchKernelGGL(k_icwy_row, dim3(nblk(k1 + 1, 256)), dim3(256), 0, st, k1, Tm, ldt, tg);
+}
 
 void launch_scattered(const ScatArgs &a, const FnRec *fn, const zd *x, const double *r, zd *E,
                       zd *H, hipStream_t st) {

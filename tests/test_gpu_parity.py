@@ -237,6 +237,23 @@ def test_gmres_orthogonalization_schemes(patch5, orth):
     assert rel(x, xo) < 1e-6
 
 
+@pytest.mark.parametrize("kind", ["sweep", "steps"])
+def test_mgs_pass_forms_agree(patch5, kind, monkeypatch):
+    """The three forms of one MGS pass -- one-reduce (default: h = (I + L)^-1
+    V^H w), the cooperative column sweep and one launch pair per column --
+    give the same IMGS history and solution to rounding."""
+    pp, tt, o, g, _ = patch5
+    k, e = plane_wave(0.0, 0.0, 2, "TM")
+    b = g.get_vector_b_tri(k, e)
+    monkeypatch.delenv("SPK_MGS", raising=False)
+    x0, i0 = g.gmres_run(b, tolerance=1e-12, max_iterations=40, orthogonalization="IMGS")
+    monkeypatch.setenv("SPK_MGS", kind)
+    x1, i1 = g.gmres_run(b, tolerance=1e-12, max_iterations=40, orthogonalization="IMGS")
+    assert i0["iters"] == i1["iters"] == 40
+    assert np.max(np.abs(np.log(i0["hist"] / i1["hist"]))) < 1e-8
+    assert rel(x1, x0) < 1e-8
+
+
 def test_small_sphere_far_field_conditioning():
     """0.25 lambda leaf boxes (icosphere, r = lambda/2): the reference's K = 19
     sum is ill-conditioned there; GPU and restatement agree to the rounding
