@@ -149,6 +149,12 @@ int spk_quad_edge_element(spk_quad *q, int n, const int *s, const int *t, double
  * device destination. */
 int spk_quad_fill(spk_quad *q, double *MM);
 int spk_quad_fill_device(spk_quad *q, void *d_MM);
+/* get_vector_b_quad with plane-wave illumination (incident_int,
+ * src/lib_sie_quad_calculation_mod.f90:91-138, 1207-1240): b[2M] complex,
+ * [E part; H part] scaled by the formulation; host or device destination. */
+int spk_quad_rhs_plane_wave(spk_quad *q, const double k_in[3], const double e_in[3], double *b);
+int spk_quad_rhs_plane_wave_device(spk_quad *q, const double k_in[3], const double e_in[3],
+                                   void *d_b);
 /* Host-only (no GPU): mesh, sizes and Edge_I of spk_quad_setup; arrays may
  * be NULL to query the sizes first. */
 int spk_quad_topology_probe(double Lx, double Ly, int nx, int ny, const double *z, int *n_edges,

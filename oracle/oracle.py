@@ -71,6 +71,7 @@ def lib():
         L.spq_assemble.argtypes = [C.c_void_p, C.c_int, dp, C.c_int]
         L.spq_geometry_probe.argtypes = [C.c_double, C.c_double, dp, dp]
         L.spq_regular_pair.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, dp]
+        L.spq_rhs_plane.argtypes = [C.c_void_p, C.c_int, dp, dp, dp]
         L.spo_gmres_prec.argtypes = [C.c_void_p, dp, dp, C.c_int, C.c_int, C.c_double, C.c_int,
                                      C.c_int, C.c_int, dp, ip, dp, dp]
         L.spo_gmres_dense.argtypes = [C.c_int, dp, dp, dp, C.c_int, C.c_int, C.c_double, C.c_int,
@@ -339,6 +340,15 @@ class QuadOracle:
         MM = np.zeros((2 * self.M, 2 * self.M), np.complex128)
         lib().spq_assemble(self.h, f, _d(MM.view(np.float64)), nthreads)
         return MM
+
+    def rhs_plane(self, k_in, e_in, formulation="PMCHWT"):
+        """get_vector_b_quad with a plane wave: b[2M] complex."""
+        f = {"PMCHWT": 1, "MCTF": 2, "ICTF": 3}[formulation]
+        k = np.ascontiguousarray(k_in, dtype=np.float64)
+        e = np.ascontiguousarray(e_in, dtype=np.float64)
+        b = np.zeros(2 * self.M, np.complex128)
+        lib().spq_rhs_plane(self.h, f, _d(k), _d(e), _d(b.view(np.float64)))
+        return b
 
 
 def quad_geometry(k, e, El):

@@ -162,6 +162,9 @@ void spq_assemble(const spq_ctx *c, int formulation, double *MM, int nthreads);
  * regular rule of any order */
 void spq_geometry_probe(double k, double e, const double *El24, double *out);
 void spq_regular_pair(const spq_ctx *c, int s, int t, int ngp, double *out);
+/* plane-wave right-hand side b[2M] (complex) for PMCHWT (1), MCTF (2), ICTF (3) */
+void spq_rhs_plane(const spq_ctx *c, int formulation, const double *kin, const double *ein,
+                   double *bout);
 
 #ifdef __cplusplus
 }

@@ -24,6 +24,7 @@
  *   ff_ABC_II_Edge, ff_phi_AB_Edge      :581-637
  *   Lm_*_EA, GL_Int_*_EA_8Node          :656-1045
  *   matrix assembly                     src/lib_sie_quad_solver_normal.f90:165-210
+ *   get_vector_b_quad, incident_int     src/lib_sie_quad_calculation_mod.f90:91-138, 1207-1240
  *
  * The 1-D Gauss-Legendre rules are computed (spo_gauss_legendre); the
  * reference tabulates the same nodes as 16-digit double literals
@@ -866,6 +867,57 @@ void spq_geometry_probe(double k, double e, const double *El24, double *out) {
   memcpy(out + 25, d.nq, 3 * sizeof(double));
   memcpy(out + 28, d.N, 30 * sizeof(double));
   memcpy(out + 58, d.nrot, 10 * sizeof(double));
+}
+
+/* Plane-wave right-hand side (get_vector_b_quad with incident_int,
+ * src/lib_sie_quad_calculation_mod.f90:91-138, 1207-1240): for edge function
+ * s, the 6 x 6 Gauss rule over its own element ii only (edge_p(1)), of
+ * N_pI . E_inc and N_pI . H_inc with E_inc = E0 exp(-i conj(k1) k.R) -- the
+ * reference's dot_product(k1*k_in, R_a) conjugates its complex first
+ * argument -- and H_inc = (k x E_inc)/(my_1 c0), my_1 = my_0; then the
+ * formulation's scaling: PMCHWT (E, H), MCTF (E, H eta1 eta2), ICTF
+ * (E / eta_a, H eta_a), eta_a = (eta1 + eta2)/2.  The edge vectors are those
+ * of paras_st_b: (Ev x n)/JJ and (n x Eu)/JJ; Edge_vector_new divides by
+ * Eu.(Ev x n), which equals JJ = |Eu x Ev|. */
+void spq_rhs_plane(const spq_ctx *c, int formulation, const double *kin, const double *ein,
+                   double *bout) {
+  const int M = c->medge, ngp = 6;
+  double x[64], w[64];
+  spo_gauss_legendre(ngp, x, w);
+  zc *b = (zc *)bout;
+  const zc k1c = conj(c->k1);
+  const double c0my = QMY0 * QC0;
+  const zc ea = (c->eta1 + c->eta2) / 2, e12 = c->eta1 * c->eta2;
+  for (int s = 0; s < M; ++s) {
+    const int ii = c->edge_p[5 * s + 0], pI = c->edge_p[5 * s + 1];
+    double El[8][3];
+    element_nodes(c, ii, El);
+    zc SE = 0, SH = 0;
+    for (int j = 0; j < ngp; ++j)
+      for (int i = 0; i < ngp; ++i) {
+        delem d;
+        paras_b(x[i], x[j], El, &d);
+        double Ra[3];
+        r_local(x[i], x[j], El, Ra);
+        const double TRF = w[i] * w[j] * d.JJ;
+        const zc e = cexp(-I * k1c * (kin[0] * Ra[0] + kin[1] * Ra[1] + kin[2] * Ra[2]));
+        const zc E[3] = {ein[0] * e, ein[1] * e, ein[2] * e};
+        const zc H[3] = {(kin[1] * E[2] - kin[2] * E[1]) / c0my,
+                         (kin[2] * E[0] - kin[0] * E[2]) / c0my,
+                         (kin[0] * E[1] - kin[1] * E[0]) / c0my};
+        const double *Nv = d.N[pI - 1];
+        SE += (Nv[0] * E[0] + Nv[1] * E[1] + Nv[2] * E[2]) * TRF;
+        SH += (Nv[0] * H[0] + Nv[1] * H[1] + Nv[2] * H[2]) * TRF;
+      }
+    zc be = SE, bh = SH;
+    if (formulation == 2) bh = SH * e12;
+    else if (formulation == 3) {
+      be = SE / ea;
+      bh = SH * ea;
+    }
+    b[s] = be;
+    b[M + s] = bh;
+  }
 }
 
 /* the regular rule of impedance_edge_versus_element at any order ngp for

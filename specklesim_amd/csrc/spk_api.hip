@@ -99,7 +99,8 @@ struct spk_handle {
   DBuf<int> d_perm, d_box_first, d_box_count, d_e2_off, d_e2_list, d_leaf_box, d_leafbox_of_pos;
   DBuf<int64_t> d_near_off;
   DBuf<zd> d_blocks;
-  DBuf<int> d_row_box, d_row_r0;
+  DBuf<int> d_row_box, d_row_r0, d_disagg_task;
+  int disagg_tasks = 0;
   int spmv_wgs = 0;
   DBuf<zd> d_rt, d_W;
   DBuf<double> d_khat, d_tth, d_tph, d_q, d_leaf_centre, d_trf;
@@ -358,8 +359,8 @@ int matvec_t(spk_handle *h, const T *x, T *y, bool near_on = true, bool far_on =
     const zd ep1 = zcx(h->mat.eps1) * I * h->mat.omega, ep2 = zcx(h->mat.eps2) * I * h->mat.omega;
     launch_fold(LL.nb, S, (const T *)pD(LL, tag), kd1, kd2, my1, my2, ep1, ep2, pW(h, tag), st);
     mark(5);
-    launch_disagg(N, S, h->d_perm.p, h->d_leafbox_of_pos.p, pRt(h, tag), (const T *)pW(h, tag),
-                  h->sc1, h->sc2, near_on ? yfar : y, st);
+    launch_disagg(h->disagg_tasks, N, S, h->d_disagg_task.p, h->d_perm.p, h->d_leafbox_of_pos.p,
+                  pRt(h, tag), (const T *)pW(h, tag), h->sc1, h->sc2, near_on ? yfar : y, st);
     mark(6);
   }
   if (near_on)
@@ -507,6 +508,14 @@ int spk_setup_tri(spk_handle *h, int np, const double *pp, int nt, const int *tt
   h->spmv_wgs = (int)rb.size();
   HIPCHK(h->d_row_box.upload(rb));
   HIPCHK(h->d_row_r0.upload(r0));
+  // disaggregation tasks: up to kDisaggEdges consecutive positions of one box
+  if (h->N >= (1 << 28)) return fail(SPK_ERR_ARG, "more than 2^28 RWG edges");
+  std::vector<int> dt;
+  for (size_t b = 0; b < t.box_count.size(); ++b)
+    for (int r = 0; r < t.box_count[b]; r += kDisaggEdges)
+      dt.push_back((t.box_first[b] + r) * 8 + std::min(kDisaggEdges, t.box_count[b] - r));
+  h->disagg_tasks = (int)dt.size();
+  HIPCHK(h->d_disagg_task.upload(dt));
   HIPCHK(h->d_x.alloc(2 * (size_t)h->N));
   HIPCHK(h->d_y.alloc(2 * (size_t)h->N));
   HIPCHK(h->d_yfar.alloc(2 * (size_t)h->N));

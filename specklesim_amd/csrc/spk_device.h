@@ -115,8 +115,9 @@ template <class T>
 void launch_fold(int nbl, int S, const T *D, zd kd1, zd kd2, zd my1, zd my2, zd ep1, zd ep2,
                  T *W, hipStream_t st);
 template <class T>
-void launch_disagg(int N, int S, const int *perm, const int *leaf_box_of_pos, const T *rt,
-                   const T *W, zd sc1, zd sc2, T *y, hipStream_t st);
+void launch_disagg(int ntask, int N, int S, const int *task, const int *perm,
+                   const int *leaf_box_of_pos, const T *rt, const T *W, zd sc1, zd sc2, T *y,
+                   hipStream_t st);
 template <class T>
 void launch_near_spmv_rows(int nwg, int N, const int *row_box, const int *row_r0,
                            const int *box_first, const int *box_count, const int *e2_off,
@@ -127,6 +128,8 @@ void launch_to_zf(int64_t n, const zd *a, zf *b, hipStream_t st);
 void launch_tl_level(int S, int K, const double *khat, const double *trf, const double *rhat,
                      const zd *coef, const int *live, zd *tlt, hipStream_t st);
 constexpr int kSpmvRowsPerWG = 16;
+// edges per disaggregation task (one wavefront, one leaf box)
+constexpr int kDisaggEdges = 4;
 // arbitrary (m, n) pairs, one thread per pair (parity probe)
 void launch_near_pairs(const NearArgs &a, int npairs, const int *m, const int *n, zd *out,
                        hipStream_t st);

@@ -815,6 +815,7 @@ __global__ void k_agg(int S, int N, const int *box_first, const int *box_count, 
   const int k = q >> 1, comp = q & 1;
   T ae = {0, 0}, ah = {0, 0};
   const int p0 = box_first[b], cnt = box_count[b];
+#pragma unroll 4
   for (int e = 0; e < cnt; ++e) {
     const int p = p0 + e;
     const int m = perm[p];
@@ -961,41 +962,57 @@ __device__ inline zf wave_sum(zf v) {
 }
 
 // one wavefront per position: y[m] = sc1*y_e, y[m+N] = sc2*y_h (far part)
+// One wavefront per task of up to kDisaggEdges consecutive edges of one leaf
+// box: each W sample is loaded once for the group (W is the box's folded
+// coefficient array), the patterns of the group's edges stream from HBM.
+// task[t] = p0 * 8 + count.  Per edge the sums run in the same order as one
+// wavefront per edge would (media, then lane-strided samples).
 template <class T>
-__global__ void __launch_bounds__(256) k_disagg(int N, int S, const int *perm,
-                                                const int *leaf_box_of_pos, const T *rt,
-                                                const T *W, T sc1, T sc2, T *y) {
+__global__ void __launch_bounds__(256) k_disagg(int ntask, int N, int S, const int *task,
+                                                const int *perm, const int *leaf_box_of_pos,
+                                                const T *rt, const T *W, T sc1, T sc2, T *y) {
+  constexpr int E = kDisaggEdges;
   const int lane = threadIdx.x & 63;
-  const int p = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (p >= N) return;
-  const int b = leaf_box_of_pos[p];
-  const T *f = rt + (size_t)p * 4 * S;
+  const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (t >= ntask) return;
+  const int code = task[t], p0 = code >> 3, cnt = code & 7;
+  const int b = leaf_box_of_pos[p0];
   const T *Wb = W + (size_t)b * 8 * S;
-  T s1 = {0, 0}, s2 = {0, 0};
+  T s1[E], s2[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) s1[e] = s2[e] = T{0, 0};
   for (int k = 0; k < 2; ++k) {
-    const T *fa = f + (size_t)(k * 2 + 0) * S, *fb = f + (size_t)(k * 2 + 1) * S;
     const T *w1a = Wb + (size_t)(k * 4 + 0) * S, *w1b = Wb + (size_t)(k * 4 + 1) * S;
     const T *w2a = Wb + (size_t)(k * 4 + 2) * S, *w2b = Wb + (size_t)(k * 4 + 3) * S;
     for (int s = lane; s < S; s += 64) {
-      const T a = fa[s], bb = fb[s];
-      s1 = s1 + a * w1a[s] + bb * w1b[s];
-      s2 = s2 + a * w2a[s] + bb * w2b[s];
+      const T wa = w1a[s], wb = w1b[s], wc = w2a[s], wd = w2b[s];
+#pragma unroll
+      for (int e = 0; e < E; ++e)
+        if (e < cnt) {
+          const T *f = rt + ((size_t)(p0 + e) * 4 + k * 2) * S;
+          const T a = f[s], bb = f[S + s];
+          s1[e] = s1[e] + a * wa + bb * wb;
+          s2[e] = s2[e] + a * wc + bb * wd;
+        }
     }
   }
-  s1 = wave_sum(s1);
-  s2 = wave_sum(s2);
-  if (lane == 0) {
-    const int m = perm[p];
-    y[m] = s1 * sc1;
-    y[m + N] = s2 * sc2;
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const T r1 = wave_sum(s1[e]), r2 = wave_sum(s2[e]);
+    if (lane == 0 && e < cnt) {
+      const int m = perm[p0 + e];
+      y[m] = r1 * sc1;
+      y[m + N] = r2 * sc2;
+    }
   }
 }
 
 template <class T>
-void launch_disagg(int N, int S, const int *perm, const int *leaf_box_of_pos, const T *rt,
-                   const T *W, zd sc1, zd sc2, T *y, hipStream_t st) {
-  if (N <= 0) return;
-  hipLaunchKernelGGL(k_disagg<T>, dim3(nblk(N, 4)), dim3(256), 0, st, N, S, perm,
+void launch_disagg(int ntask, int N, int S, const int *task, const int *perm,
+                   const int *leaf_box_of_pos, const T *rt, const T *W, zd sc1, zd sc2, T *y,
+                   hipStream_t st) {
+  if (ntask <= 0) return;
+  hipLaunchKernelGGL(k_disagg<T>, dim3(nblk(ntask, 4)), dim3(256), 0, st, ntask, N, S, task, perm,
                      leaf_box_of_pos, rt, W, cvt<T>(sc1), cvt<T>(sc2), y);
 }
 
@@ -1075,8 +1092,8 @@ void launch_near_spmv_rows(int nwg, int N, const int *row_box, const int *row_r0
   template void launch_rr<T>(int, int, const int *, const int *, const T *, const T *, T *,      \
                              hipStream_t);                                                       \
   template void launch_fold<T>(int, int, const T *, zd, zd, zd, zd, zd, zd, T *, hipStream_t);   \
-  template void launch_disagg<T>(int, int, const int *, const int *, const T *, const T *, zd,   \
-                                 zd, T *, hipStream_t);                                          \
+  template void launch_disagg<T>(int, int, int, const int *, const int *, const int *, const T *, \
+                                 const T *, zd, zd, T *, hipStream_t);                           \
   template void launch_near_spmv_rows<T>(int, int, const int *, const int *, const int *,        \
                                          const int *, const int *, const int *, const int64_t *, \
                                          const int *, const T *, const T *, const T *, T *,      \

@@ -60,6 +60,16 @@ void launch_quad_edge_adj(const QuadDev &d, const QuadTopo &T, int n, const int 
 // sums[r[i]][t[i]][:] = src[i][:] (30 complex each)
 void launch_quad_scatter(int n, int nel, const int *r, const int *t, const zd *src, zd *sums,
                          hipStream_t st);
+// plane-wave right-hand side b[2M]: phase factor k1c = conj(k1), H = (k x E)/c0my,
+// formulation scaling b = [sE * <N, E>; sH * <N, H>]
+struct QuadRhsArgs {
+  double kin[3], ein[3];
+  zd k1c;
+  double c0my;
+  zd sE, sH;
+};
+void launch_quad_rhs_plane(const QuadDev &d, const QuadTopo &T, const QuadRhsArgs &a, zd *b,
+                           hipStream_t st);
 // MM (row-major 2M x 2M) rows [s0, s0+rows) from sums[r][t][30]
 void launch_quad_assemble(const QuadTopo &T, int formulation, zd eta_a, zd eta12, int s0,
                           int rows, const zd *sums, zd *MM, hipStream_t st);

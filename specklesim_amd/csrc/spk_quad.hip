@@ -588,6 +588,45 @@ __global__ void k_quad_assemble(QuadTopo T, int form, zd eta_a, zd eta12, int s0
   MM[(M + s) * n2 + M + tc] = d;
 }
 
+// Plane-wave right-hand side (get_vector_b_quad with incident_int,
+// src/lib_sie_quad_calculation_mod.f90:91-138, 1207-1240): edge function s
+// is tested on its own element edge_p(1) only, 6 x 6 Gauss points; the
+// phase is exp(-i conj(k1) k.R) (Fortran's dot_product conjugates the
+// complex first argument of dot_product(k1*k_in, R_a)); H = (k x E)/(my0 c0).
+// One thread per edge function.
+__global__ void k_quad_rhs_plane(QuadDev D, QuadTopo T, QuadRhsArgs a, zd *b) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= T.medge) return;
+  const int ii = T.edge_p[5 * s] - 1, pI = T.edge_p[5 * s + 1];
+  double El[24];
+  for (int q = 0; q < 24; ++q) El[q] = T.el[(size_t)ii * 24 + q];
+  zd SE = {0, 0}, SH = {0, 0};
+  for (int j = 0; j < 6; ++j)
+    for (int i = 0; i < 6; ++i) {
+      const double kk = D.x6[i], ee = D.x6[j];
+      Geo g;
+      geo(kk, ee, El, g);
+      double R[3], Na[3];
+      rloc(kk, ee, El, R);
+      obs_fn(g, pI, kk, ee, Na);
+      const double TRF = D.w6[i] * D.w6[j] * g.JJ;
+      const double kr = a.kin[0] * R[0] + a.kin[1] * R[1] + a.kin[2] * R[2];
+      // exp(-i k1c kr) = exp(Im(k1c) kr) (cos(Re k1c kr) - i sin(Re k1c kr))
+      double sn, cs;
+      sincos(a.k1c.x * kr, &sn, &cs);
+      const double mag = exp(a.k1c.y * kr);
+      const zd e = {mag * cs, -mag * sn};
+      const zd E[3] = {e * a.ein[0], e * a.ein[1], e * a.ein[2]};
+      const zd H[3] = {(E[2] * a.kin[1] - E[1] * a.kin[2]) * (1.0 / a.c0my),
+                       (E[0] * a.kin[2] - E[2] * a.kin[0]) * (1.0 / a.c0my),
+                       (E[1] * a.kin[0] - E[0] * a.kin[1]) * (1.0 / a.c0my)};
+      SE = SE + (E[0] * Na[0] + E[1] * Na[1] + E[2] * Na[2]) * TRF;
+      SH = SH + (H[0] * Na[0] + H[1] * Na[1] + H[2] * Na[2]) * TRF;
+    }
+  b[s] = SE * a.sE;
+  b[T.medge + s] = SH * a.sH;
+}
+
 inline unsigned nb(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
 // sums[r[i]][t[i]][0:30] = src[i][0:30]
@@ -629,6 +668,11 @@ void launch_quad_edge_adj(const QuadDev &d, const QuadTopo &T, int n, const int 
                           const int *side, zd *out, hipStream_t st) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_quad_edge_adj, dim3(n), dim3(128), 0, st, d, T, s, t, side, out);
+}
+void launch_quad_rhs_plane(const QuadDev &d, const QuadTopo &T, const QuadRhsArgs &a, zd *b,
+                           hipStream_t st) {
+  if (T.medge <= 0) return;
+  hipLaunchKernelGGL(k_quad_rhs_plane, dim3(nb(T.medge, 128)), dim3(128), 0, st, d, T, a, b);
 }
 void launch_quad_assemble(const QuadTopo &T, int formulation, zd eta_a, zd eta12, int s0,
                           int rows, const zd *sums, zd *MM, hipStream_t st) {
@@ -933,6 +977,44 @@ int spk_quad_fill_device(spk_quad *q, void *d_MM) {
     QCHK(hipGetLastError());
     QCHK(hipStreamSynchronize(q->st));
   }
+  return SPK_OK;
+}
+
+int spk_quad_rhs_plane_wave_device(spk_quad *q, const double k_in[3], const double e_in[3],
+                                   void *d_b) {
+  if (!q || !k_in || !e_in || !d_b) return set_error(SPK_ERR_ARG, "null argument");
+  if (!q->ready) return set_error(SPK_ERR_STATE, "spk_quad_setup has not completed");
+  QCHK(hipSetDevice(q->p.device));
+  QuadRhsArgs a{};
+  for (int d = 0; d < 3; ++d) {
+    a.kin[d] = k_in[d];
+    a.ein[d] = e_in[d];
+  }
+  a.k1c = zd{q->dev.k1.x, -q->dev.k1.y};
+  a.c0my = (double)kMy0 * (double)kC0;
+  a.sE = zd{1.0, 0.0};
+  a.sH = zd{1.0, 0.0};
+  if (q->p.formulation == SPK_MCTF) {
+    a.sH = q->eta12;
+  } else if (q->p.formulation == SPK_ICTF) {
+    a.sE = zd{1.0, 0.0} / q->eta_a;
+    a.sH = q->eta_a;
+  }
+  launch_quad_rhs_plane(q->dev, q->topo(), a, (zd *)d_b, q->st);
+  QCHK(hipGetLastError());
+  QCHK(hipStreamSynchronize(q->st));
+  return SPK_OK;
+}
+
+int spk_quad_rhs_plane_wave(spk_quad *q, const double k_in[3], const double e_in[3], double *b) {
+  if (!q || !b) return set_error(SPK_ERR_ARG, "null argument");
+  if (!q->ready) return set_error(SPK_ERR_STATE, "spk_quad_setup has not completed");
+  QCHK(hipSetDevice(q->p.device));
+  QBuf<zd> d;
+  QCHK(d.alloc(2 * (size_t)q->mesh.medge));
+  const int rc = spk_quad_rhs_plane_wave_device(q, k_in, e_in, d.p);
+  if (rc) return rc;
+  QCHK(hipMemcpy(b, d.p, 2 * (size_t)q->mesh.medge * sizeof(zd), hipMemcpyDeviceToHost));
   return SPK_OK;
 }
 

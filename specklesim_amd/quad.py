@@ -84,6 +84,22 @@ class QuadFill:
     def fill_device(self, d_MM):
         nat.check(nat.lib().spk_quad_fill_device(self._h, C.c_void_p(int(d_MM))))
 
+    def get_vector_b_quad(self, k_in, e_in):
+        """Plane-wave right-hand side (get_vector_b_quad,
+        src/lib_sie_quad_calculation_mod.f90:91-138): b[2M] complex."""
+        k = np.ascontiguousarray(k_in, dtype=np.float64)
+        e = np.ascontiguousarray(e_in, dtype=np.float64)
+        b = np.zeros(2 * self.M, np.complex128)
+        nat.check(nat.lib().spk_quad_rhs_plane_wave(self._h, _dp(k), _dp(e),
+                                                    _dp(b.view(np.float64))))
+        return b
+
+    def get_vector_b_quad_device(self, k_in, e_in, d_b):
+        k = np.ascontiguousarray(k_in, dtype=np.float64)
+        e = np.ascontiguousarray(e_in, dtype=np.float64)
+        nat.check(nat.lib().spk_quad_rhs_plane_wave_device(self._h, _dp(k), _dp(e),
+                                                           C.c_void_p(int(d_b))))
+
 
 def topology_probe(Lx, Ly, nx, ny, z):
     """Host-only mesh and edge numbering of spk_quad_setup (no GPU needed):

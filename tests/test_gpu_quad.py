@@ -68,3 +68,21 @@ def test_assembled_system(form):
     scale = np.abs(ref).max(axis=1)
     assert np.max(np.abs(MM - ref).max(axis=1) / scale) < 1e-9
     assert np.array_equal(g.fill(), MM)
+
+
+@pytest.mark.parametrize("form", ["PMCHWT", "MCTF", "ICTF"])
+def test_plane_wave_rhs(form):
+    """get_vector_b_quad (src/lib_sie_quad_calculation_mod.f90:91-138) on the
+    GPU against the restatement, lossy exterior and interior, oblique
+    incidence: 1e-12 relative."""
+    Lx, Ly, nx, ny, z = meshes.rough_height_field(4, 3, 1.5, 1.1, 0.05, 0.5, 9, LAM)
+    eps1, eps2 = 1.3 - 0.05j, 18.392 - 0.4164j
+    q = O.QuadOracle(LAM, eps2, Lx, Ly, nx, ny, z, eps_r1=eps1)
+    g = QuadFill(LAM, eps2, formulation=form, eps_r1=eps1).setup(Lx, Ly, nx, ny, z)
+    th = np.deg2rad(25.0)
+    k = np.array([np.sin(th), 0.0, -np.cos(th)])
+    for e in (np.array([0.0, 1.0, 0.0]), np.array([np.cos(th), 0.0, np.sin(th)])):
+        b = g.get_vector_b_quad(k, e)
+        ref = q.rhs_plane(k, e, form)
+        M = q.M
+        assert rel(b[:M], ref[:M]) < 1e-12 and rel(b[M:], ref[M:]) < 1e-12

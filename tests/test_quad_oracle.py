@@ -169,3 +169,61 @@ def test_height_file_round_trip(tmp_path):
     meshes.write_quad_height_file(str(f), Lx, Ly, nx, ny, z)
     Lx2, Ly2, nx2, ny2, z2 = meshes.read_quad_height_file(str(f))
     assert (Lx2, Ly2, nx2, ny2) == (Lx, Ly, nx, ny) and np.array_equal(z2, z)
+
+
+def _brute_rhs(q, k, E0, ng=24):
+    """int over element edge_p(1) of N_pI . E_inc, N_pI . H_inc by a ng x ng
+    Gauss product, from the pinned geometry probe (lossless exterior)."""
+    x, w = np.polynomial.legendre.leggauss(ng)
+    pp, v, e = q.nodes(), q.elements(), q.edges()
+    k1 = 2 * np.pi / LAM
+    mu0c0 = float(np.float32(1.2566370614e-6)) * float(np.float32(2.99792458e8))
+    out = np.zeros((q.M, 2), complex)
+    for s in range(q.M):
+        El = pp[v[e[s, 0] - 1] - 1]
+        pI = e[s, 1] - 1
+        for j in range(ng):
+            for i in range(ng):
+                g = O.quad_geometry(x[i], x[j], El)
+                R = g["N"] @ El
+                E = E0 * np.exp(-1j * k1 * (k @ R))
+                H = np.cross(k, E) / mu0c0
+                t = w[i] * w[j] * g["JJ"]
+                out[s, 0] += (g["fn"][pI] @ E) * t
+                out[s, 1] += (g["fn"][pI] @ H) * t
+    return out
+
+
+def test_plane_wave_rhs_against_brute_force():
+    """get_vector_b_quad (src/lib_sie_quad_calculation_mod.f90:91-138): each
+    edge function tested on its own element only, with the 6-point rule; a
+    24-point product of the same integrand agrees to the 6-point rule's error."""
+    Lx, Ly, nx, ny, z = meshes.rough_height_field(2, 2, 0.8, 0.8, 0.05, 0.5, 4, LAM)
+    q = O.QuadOracle(LAM, 2.25, Lx, Ly, nx, ny, z)
+    th = np.deg2rad(30.0)
+    k = np.array([np.sin(th), 0.0, -np.cos(th)])
+    E0 = np.array([np.cos(th), 0.0, np.sin(th)])
+    b = q.rhs_plane(k, E0, "PMCHWT")
+    ref = _brute_rhs(q, k, E0)
+    M = q.M
+    assert np.linalg.norm(b[:M] - ref[:, 0]) < 1e-5 * np.linalg.norm(ref[:, 0])
+    assert np.linalg.norm(b[M:] - ref[:, 1]) < 1e-5 * np.linalg.norm(ref[:, 1])
+
+
+def test_plane_wave_rhs_height_shift_phase():
+    """Raising the surface by h multiplies the RHS by exp(-i conj(k1) k_z h):
+    the phase of dot_product(k1*k_in, R_a) (:1231), whose complex first
+    argument Fortran conjugates (visible with a lossy exterior medium)."""
+    Lx, Ly, nx, ny, z = meshes.rough_height_field(2, 3, 0.8, 1.1, 0.05, 0.5, 8, LAM)
+    eps1 = 1.7 - 0.3j
+    h = 0.41 * LAM
+    k = np.array([0.6, 0.0, -0.8])
+    E0 = np.array([0.8, 0.0, 0.6])
+    q0 = O.QuadOracle(LAM, 2.25, Lx, Ly, nx, ny, z, eps_r1=eps1)
+    q1 = O.QuadOracle(LAM, 2.25, Lx, Ly, nx, ny, z + h, eps_r1=eps1)
+    k1 = 2 * np.pi / LAM * np.sqrt(eps1)
+    f = np.exp(-1j * np.conj(k1) * k[2] * h)
+    assert abs(f - np.exp(-1j * k1 * k[2] * h)) > 0.1 * abs(f)  # the test tells them apart
+    for form in ("PMCHWT", "ICTF"):
+        b0, b1 = q0.rhs_plane(k, E0, form), q1.rhs_plane(k, E0, form)
+        assert np.linalg.norm(b1 - b0 * f) < 1e-12 * np.linalg.norm(b1)
