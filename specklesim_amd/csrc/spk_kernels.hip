@@ -806,6 +806,17 @@ void launch_patterns(int N, int S, int ngp, const FnRec *fn, const int *perm, co
 
 // lib_sie_ml_fmm_get_c (src/lib_sie_tri_mlfmm_interface.f90:395-448):
 // C[b][comp*4+cur*2+k][s] = sum_{edges of b} conj(RT[k][comp][s]) * x_cur
+// complex64 values are moved in pairs (one 16-byte load covers two values)
+template <class T>
+constexpr int kPack = sizeof(T) == 8 ? 2 : 1;
+struct zf2 {
+  zf a, b;
+};
+__device__ inline zf2 ld2(const zf *p) {
+  const float4 v = *reinterpret_cast<const float4 *>(p);
+  return zf2{zf{v.x, v.y}, zf{v.z, v.w}};
+}
+
 template <class T>
 __global__ void k_agg(int S, int N, const int *box_first, const int *box_count, const int *perm,
                       const T *rt, const T *x, T *C) {
@@ -984,16 +995,31 @@ __global__ void __launch_bounds__(256) k_disagg(int ntask, int N, int S, const i
   for (int k = 0; k < 2; ++k) {
     const T *w1a = Wb + (size_t)(k * 4 + 0) * S, *w1b = Wb + (size_t)(k * 4 + 1) * S;
     const T *w2a = Wb + (size_t)(k * 4 + 2) * S, *w2b = Wb + (size_t)(k * 4 + 3) * S;
-    for (int s = lane; s < S; s += 64) {
-      const T wa = w1a[s], wb = w1b[s], wc = w2a[s], wd = w2b[s];
+    if constexpr (kPack<T> == 2) {
+      // complex64: each lane takes two consecutive samples per 16-byte load
+      for (int s = 2 * lane; s < S; s += 128) {
+        const zf2 wa = ld2(w1a + s), wb = ld2(w1b + s), wc = ld2(w2a + s), wd = ld2(w2b + s);
 #pragma unroll
-      for (int e = 0; e < E; ++e)
-        if (e < cnt) {
-          const T *f = rt + ((size_t)(p0 + e) * 4 + k * 2) * S;
-          const T a = f[s], bb = f[S + s];
-          s1[e] = s1[e] + a * wa + bb * wb;
-          s2[e] = s2[e] + a * wc + bb * wd;
-        }
+        for (int e = 0; e < E; ++e)
+          if (e < cnt) {
+            const T *f = rt + ((size_t)(p0 + e) * 4 + k * 2) * S;
+            const zf2 a = ld2(f + s), bb = ld2(f + S + s);
+            s1[e] = s1[e] + a.a * wa.a + bb.a * wb.a + a.b * wa.b + bb.b * wb.b;
+            s2[e] = s2[e] + a.a * wc.a + bb.a * wd.a + a.b * wc.b + bb.b * wd.b;
+          }
+      }
+    } else {
+      for (int s = lane; s < S; s += 64) {
+        const T wa = w1a[s], wb = w1b[s], wc = w2a[s], wd = w2b[s];
+#pragma unroll
+        for (int e = 0; e < E; ++e)
+          if (e < cnt) {
+            const T *f = rt + ((size_t)(p0 + e) * 4 + k * 2) * S;
+            const T a = f[s], bb = f[S + s];
+            s1[e] = s1[e] + a * wa + bb * wb;
+            s2[e] = s2[e] + a * wc + bb * wd;
+          }
+      }
     }
   }
 #pragma unroll
@@ -1050,7 +1076,19 @@ __global__ void __launch_bounds__(256) k_near_spmv(int N, const int *row_box, co
       if (r >= r0 + rows) break;
       const T *Z = blocks + 4 * (near_off[b] + (int64_t)(r - 0) * cols + cc);
       for (int c = lane; c < nc; c += 64) {
-        const T z0 = Z[4 * c], z1 = Z[4 * c + 1], z2 = Z[4 * c + 2], z3 = Z[4 * c + 3];
+        T z0, z1, z2, z3;
+        if constexpr (kPack<T> == 2) {
+          const zf2 u = ld2(Z + 4 * c), v = ld2(Z + 4 * c + 2);
+          z0 = u.a;
+          z1 = u.b;
+          z2 = v.a;
+          z3 = v.b;
+        } else {
+          z0 = Z[4 * c];
+          z1 = Z[4 * c + 1];
+          z2 = Z[4 * c + 2];
+          z3 = Z[4 * c + 3];
+        }
         const T xe = xs[2 * c], xh = xs[2 * c + 1];
         ae[q] = ae[q] + z0 * xe + z1 * xh;
         ah[q] = ah[q] + z2 * xe + z3 * xh;

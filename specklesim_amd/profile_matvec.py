@@ -52,10 +52,12 @@ def main(argv=None):
     s = SieTriMlfmm(lam, args.eps_r2, formulation=args.formulation, normal_convention=2,
                     tree_s_opt=250, truncation_number=19, precision=args.precision)
     s.precalculation_tri(pp, tt)
-    prof = s.profile_matvec(reps=args.reps)
+    c64 = args.precision == "mixed"
+    prof = s.profile_matvec(reps=args.reps, c64=c64)
     st = s.stats()
     out = {"workload": f"rough patch {args.size:g}x{args.size:g} lambda, {args.cells}x{args.cells} "
                        f"cells, N={st['n_edges']}, {args.formulation}",
+           "operator": "complex64" if c64 else "complex128",
            "reps": args.reps, "near_blocks": st["near_blocks"],
            "stages": {k: {"ms": v[0], "bytes": v[1],
                           "GBps": (v[1] / (v[0] * 1e-3) / 1e9) if v[0] > 0 else None}
