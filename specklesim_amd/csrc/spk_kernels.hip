@@ -995,29 +995,42 @@ __global__ void __launch_bounds__(256) k_disagg(int ntask, int N, int S, const i
   for (int k = 0; k < 2; ++k) {
     const T *w1a = Wb + (size_t)(k * 4 + 0) * S, *w1b = Wb + (size_t)(k * 4 + 1) * S;
     const T *w2a = Wb + (size_t)(k * 4 + 2) * S, *w2b = Wb + (size_t)(k * 4 + 3) * S;
+    // the group's pattern rows (a short group repeats its last edge, whose
+    // products are then dropped), so that all loads of a step issue together
+    const T *f[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) f[e] = rt + ((size_t)(p0 + min(e, cnt - 1)) * 4 + k * 2) * S;
     if constexpr (kPack<T> == 2) {
       // complex64: each lane takes two consecutive samples per 16-byte load
       for (int s = 2 * lane; s < S; s += 128) {
         const zf2 wa = ld2(w1a + s), wb = ld2(w1b + s), wc = ld2(w2a + s), wd = ld2(w2b + s);
+        zf2 a[E], bb[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          a[e] = ld2(f[e] + s);
+          bb[e] = ld2(f[e] + S + s);
+        }
 #pragma unroll
         for (int e = 0; e < E; ++e)
           if (e < cnt) {
-            const T *f = rt + ((size_t)(p0 + e) * 4 + k * 2) * S;
-            const zf2 a = ld2(f + s), bb = ld2(f + S + s);
-            s1[e] = s1[e] + a.a * wa.a + bb.a * wb.a + a.b * wa.b + bb.b * wb.b;
-            s2[e] = s2[e] + a.a * wc.a + bb.a * wd.a + a.b * wc.b + bb.b * wd.b;
+            s1[e] = s1[e] + a[e].a * wa.a + bb[e].a * wb.a + a[e].b * wa.b + bb[e].b * wb.b;
+            s2[e] = s2[e] + a[e].a * wc.a + bb[e].a * wd.a + a[e].b * wc.b + bb[e].b * wd.b;
           }
       }
     } else {
       for (int s = lane; s < S; s += 64) {
         const T wa = w1a[s], wb = w1b[s], wc = w2a[s], wd = w2b[s];
+        T a[E], bb[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          a[e] = f[e][s];
+          bb[e] = f[e][S + s];
+        }
 #pragma unroll
         for (int e = 0; e < E; ++e)
           if (e < cnt) {
-            const T *f = rt + ((size_t)(p0 + e) * 4 + k * 2) * S;
-            const T a = f[s], bb = f[S + s];
-            s1[e] = s1[e] + a * wa + bb * wb;
-            s2[e] = s2[e] + a * wc + bb * wd;
+            s1[e] = s1[e] + a[e] * wa + bb[e] * wb;
+            s2[e] = s2[e] + a[e] * wc + bb[e] * wd;
           }
       }
     }
