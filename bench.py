@@ -36,10 +36,13 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on the
 # default workload (100 x 100 cells, 10 lambda, ICTF, complex128), each counter
 # scaled by what it reads for a plain 1 GiB device copy (FETCH x 2.0,
-# WRITE x 1.0); profiles/matvec_traffic_rough10x10_ictf_fp64.txt
-MEASURED_TRAFFIC = {("near_spmv", 100, 10.0, "fp64"): 1.8556e9,
-                    ("aggregation", 100, 10.0, "fp64"): 1.4491e9,
-                    ("disaggregation", 100, 10.0, "fp64"): 1.6288e9}
+# WRITE x 1.0); profiles/matvec_traffic_rough10x10_ictf.txt
+MEASURED_TRAFFIC = {("near_spmv", 100, 10.0, "fp64"): 1.8555e9,
+                    ("aggregation", 100, 10.0, "fp64"): 1.4732e9,
+                    ("disaggregation", 100, 10.0, "fp64"): 1.6627e9,
+                    ("near_spmv", 100, 10.0, "c64"): 0.9289e9,
+                    ("aggregation", 100, 10.0, "c64"): 0.7573e9,
+                    ("disaggregation", 100, 10.0, "c64"): 0.8181e9}
 
 
 def _sync():
@@ -186,11 +189,17 @@ def main():
 
     prof = s.profile_matvec(reps=20)
     st = s.stats()
-    # the roofline kernel: the longest HBM-bound stage of the matvec
-    stage, (ms, by) = max(((k, v) for k, v in prof.items() if k != "matvec"),
+    # the roofline kernel: the longest HBM-bound stage of the headline path's
+    # matvec (the complex64 operator when the mixed path leads)
+    pc = paths["mixed"]["solver"].profile_matvec(reps=20, c64=True) if "mixed" in paths else None
+    rprof, rprec = (pc, "c64") if head == "mixed" else (prof, "fp64")
+    stage, (ms, by) = max(((k, v) for k, v in rprof.items() if k != "matvec"),
                           key=lambda kv: kv[1][0])
     achieved = by / (ms * 1e-3) / 1e9
-    traffic = MEASURED_TRAFFIC.get((stage, args.cells, args.size, "fp64"))
+    traffic = MEASURED_TRAFFIC.get((stage, args.cells, args.size, rprec))
+    from specklesim_amd._native import read_bandwidth
+
+    ceiling = read_bandwidth(0, 2048, 5)
     rec = {
         "metric": "solves/s (MLFMM-GMRES to tolerance, incl. precalculation); matvec ms",
         "value": args.steps / elapsed,
@@ -215,6 +224,8 @@ def main():
         "stages_GBps": {k: round(v[1] / (v[0] * 1e-3) / 1e9, 1) for k, v in prof.items()
                         if v[0] > 0},
         "setup_ms": st["setup_ms"], "near_fill_ms": st["near_fill_ms"],
+        "setup_split_ms": {k: round(st[k], 3) for k in ("tree_ms", "near_fill_ms", "patterns_ms",
+                                                         "tables_ms")},
         "gmres_iterations": runs[-1]["iters"], "gmres_backward_error": runs[-1]["be"],
         "gmres_info": runs[-1]["info"], "gmres_s": runs[-1]["gmres_s"],
         "paths": {k: {"solves_per_s": v["solves_per_s"], "ms_per_step": v["ms_per_step"],
@@ -227,12 +238,13 @@ def main():
         "headline_path": head,
         "mixed_vs_fp64": (paths["mixed"]["solves_per_s"] / paths["fp64"]["solves_per_s"]
                           if "mixed" in paths and "fp64" in paths else None),
-        "roofline": {"kernel": stage, "bound": "hbm", "achieved": achieved,
+        "roofline": {"kernel": stage, "operator": "complex64" if rprec == "c64" else "complex128",
+                     "bound": "hbm", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic, "algorithmic_bytes": by, "ms": ms},
+                     "traffic": traffic, "algorithmic_bytes": by, "ms": ms,
+                     "read_ceiling": ceiling, "frac_of_read_ceiling": achieved / ceiling},
     }
-    if "mixed" in paths:
-        pc = paths["mixed"]["solver"].profile_matvec(reps=20, c64=True)
+    if pc is not None:
         rec["matvec_c64_ms"] = pc["matvec"][0]
         rec["stages_c64_ms"] = {k: round(v[0], 4) for k, v in pc.items()}
     if not args.no_cpu_baseline:

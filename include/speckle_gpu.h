@@ -243,6 +243,10 @@ int spk_get_patterns(const spk_handle *h, int first, int count, double *out);
  * 4 fold, 5 disaggregation, 6 near-field SpMV, 7 whole matvec. */
 #define SPK_NSTAGES 8
 int spk_profile_matvec(spk_handle *h, int reps, double *ms, double *bytes);
+/* No counterpart in the reference: the streaming-read bandwidth of a
+ * device (best of reps reads of a bytes-long buffer, GB/s), the practical
+ * ceiling for the HBM-bound kernels. */
+int spk_probe_read_bandwidth(int device, size_t bytes, int reps, double *gbps);
 /* Same for the complex64 operator of an SPK_MIXED handle (bytes at 8 B per
  * complex). */
 int spk_profile_matvec_c64(spk_handle *h, int reps, double *ms, double *bytes);

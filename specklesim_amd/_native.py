@@ -22,7 +22,7 @@ EXPORTS = [
     "spk_quad_create", "spk_quad_destroy", "spk_quad_setup", "spk_quad_sizes",
     "spk_quad_get_mesh", "spk_quad_get_edges", "spk_quad_edge_element", "spk_quad_fill",
     "spk_quad_fill_device", "spk_quad_topology_probe", "spk_quad_rhs_plane_wave",
-    "spk_quad_rhs_plane_wave_device",
+    "spk_quad_rhs_plane_wave_device", "spk_probe_read_bandwidth",
     "spk_device_alloc", "spk_device_free", "spk_device_copy", "spk_device_synchronize",
     "spk_gmres", "spk_gmres_device", "spk_gmres_history", "spk_get_rwg", "spk_get_tree",
     "spk_get_i4", "spk_get_near_csr", "spk_near_blocks", "spk_get_patterns",
@@ -107,6 +107,7 @@ def lib():
     L.spk_quad_fill_device.argtypes = [vp, vp]
     L.spk_quad_rhs_plane_wave.argtypes = [vp, dp, dp, dp]
     L.spk_quad_rhs_plane_wave_device.argtypes = [vp, dp, dp, vp]
+    L.spk_probe_read_bandwidth.argtypes = [C.c_int, C.c_size_t, C.c_int, dp]
     L.spk_quad_topology_probe.argtypes = [C.c_double, C.c_double, C.c_int, C.c_int, dp, ip, ip, ip,
                                           dp, ip, ip]
     L.spk_device_alloc.argtypes = [C.c_int, C.c_size_t, C.POINTER(vp)]
@@ -175,3 +176,13 @@ class DeviceBuffer:
 
 def synchronize(device: int = 0) -> None:
     check(lib().spk_device_synchronize(device))
+
+
+def read_bandwidth(device=0, mib=2048, reps=5):
+    """Streaming-read bandwidth of the device in GB/s (spk_probe_read_bandwidth)."""
+    import numpy as np
+
+    out = np.zeros(1)
+    check(lib().spk_probe_read_bandwidth(device, mib << 20, reps,
+                                         out.ctypes.data_as(C.POINTER(C.c_double))))
+    return float(out[0])

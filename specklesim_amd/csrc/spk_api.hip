@@ -1422,6 +1422,34 @@ static int profile_matvec_t(spk_handle *h, int reps, double *ms, double *bytes) 
 }
 }  // extern "C++"
 
+int spk_probe_read_bandwidth(int device, size_t bytes, int reps, double *gbps) {
+  if (!gbps || bytes < (1u << 20) || reps < 1) return fail(SPK_ERR_ARG, "bad probe arguments");
+  HIPCHK(hipSetDevice(device));
+  DBuf<double2> buf;
+  DBuf<double> out;
+  HIPCHK(buf.alloc(bytes / 16));
+  HIPCHK(out.alloc(1));
+  HIPCHK(hipMemset(buf.p, 0, (bytes / 16) * 16));
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  float best = 0.f;
+  for (int r = 0; r <= reps; ++r) {
+    (void)hipEventRecord(e0, nullptr);
+    launch_stream_read(buf.p, (bytes / 16) * 16, out.p, nullptr);
+    (void)hipEventRecord(e1, nullptr);
+    (void)hipEventSynchronize(e1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    if (r > 0 && (best == 0.f || ms < best)) best = ms;  // the first launch warms up
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  HIPCHK(hipGetLastError());
+  *gbps = (double)((bytes / 16) * 16) / (best * 1e-3) / 1e9;
+  return SPK_OK;
+}
+
 int spk_profile_matvec(spk_handle *h, int reps, double *ms, double *bytes) {
   if (!h || reps < 1) return fail(SPK_ERR_ARG, "bad arguments");
   if (!h->ready) return fail(SPK_ERR_STATE, "spk_setup_tri has not completed");

@@ -198,6 +198,8 @@ template <class T>
 void launch_gemv_n(int64_t n, int k, const T *V, int64_t ldv, const zd *h, zd *out, zd *partial,
                    hipStream_t st);
 size_t gemv_n_partial_elems(int64_t n, int kmax);
+// streaming read of a buffer (bandwidth probe)
+void launch_stream_read(const void *a, size_t bytes, double *out, hipStream_t st);
 // one-reduce MGS: row k1 of T = (I + L)^-1 from tg = T^H g
 void launch_icwy_row(int k1, zd *Tm, int ldt, const zd *tg, hipStream_t st);
 void launch_zadd(int64_t n, const zd *a, zd *y, hipStream_t st);  // y += a

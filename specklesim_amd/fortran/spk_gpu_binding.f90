@@ -112,6 +112,14 @@ module spk_gpu_binding
          type(c_ptr), value :: handle
          complex(c_double_complex), intent(out) :: mm(*)
       end function
+      ! get_vector_b_quad with a plane wave (src/lib_sie_quad_calculation_mod.f90:91-138)
+      integer(c_int) function spk_quad_rhs_plane_wave(handle, k_in, e_in, b) &
+            bind(c, name="spk_quad_rhs_plane_wave")
+         import :: c_int, c_ptr, c_double, c_double_complex
+         type(c_ptr), value :: handle
+         real(c_double), intent(in) :: k_in(3), e_in(3)
+         complex(c_double_complex), intent(out) :: b(*)
+      end function
       integer(c_int) function spk_gmres(handle, b, x, p, info) bind(c, name="spk_gmres")
          import :: c_int, c_ptr, c_double_complex, spk_gmres_params, spk_gmres_info
          type(c_ptr), value :: handle
