@@ -122,6 +122,23 @@ struct spk_handle {
   // one-reduce MGS: T = (I + L)^-1, [c; g], T^H g, partial sums of V h and of T
   DBuf<zd> g_T, g_c, g_tg, g_pn, g_pT;
   int g_T_m = -1;
+  // pinned host copy of one pass's projection coefficients and |w|^2
+  struct Pinned {
+    zd *p = nullptr;
+    size_t n = 0;
+    ~Pinned() {
+      if (p) (void)hipHostFree(p);
+    }
+    hipError_t ensure(size_t count) {
+      if (count <= n) return hipSuccess;
+      if (p) (void)hipHostFree(p);
+      p = nullptr;
+      n = 0;
+      hipError_t e = hipHostMalloc((void **)&p, count * sizeof(zd), hipHostMallocDefault);
+      if (e == hipSuccess) n = count;
+      return e;
+    }
+  } pin;
   DBuf<unsigned long long> g_ctl;    // MGS sweep abort flag
   DBuf<unsigned char> g_slots;       // MGS sweep exchange slots
   unsigned long long sweep_seq = 0;  // sequence number of the last sweep launch
@@ -837,6 +854,7 @@ static int gmres_mixed(spk_handle *h, const zd *b, zd *x, const spk_gmres_params
   }
   HIPCHK(h->g_h.alloc(m + 1));
   HIPCHK(h->g_y.alloc(m + 1));
+  HIPCHK(h->pin.ensure(m + 2));
   const size_t gc_elems = gemv_c_partial_elems(n, m + 1, 1);
   HIPCHK(h->g_partial.alloc(gc_elems + (size_t)reduce_blocks(n)));
   HIPCHK(h->g_pn.alloc(gemv_n_partial_elems(n, m + 1)));
@@ -906,11 +924,11 @@ static int gmres_mixed(spk_handle *h, const zd *b, zd *x, const spk_gmres_params
         launch_gemv_c(n, j + 1, (const zf *)V, n, (const zf *)w, (const zf *)nullptr,
                       h->g_partial.p, h->g_h.p, h->st);
         launch_gemv_n_sub(n, j + 1, (const zf *)V, n, h->g_h.p, w, h->g_pn.p, h->st);
-        launch_zdotc(n, (const zf *)w, (const zf *)w, partial_nrm, h->g_scal.p, h->st);
-        std::vector<zd> hv(j + 2);
-        HIPCHK(hipMemcpyAsync(hv.data(), h->g_h.p, (j + 1) * sizeof(zd), hipMemcpyDeviceToHost,
+        // |w|^2 lands after the coefficients: one copy to pinned memory per pass
+        launch_zdotc(n, (const zf *)w, (const zf *)w, partial_nrm, h->g_h.p + (j + 1), h->st);
+        const zd *hv = h->pin.p;
+        HIPCHK(hipMemcpyAsync(h->pin.p, h->g_h.p, (j + 2) * sizeof(zd), hipMemcpyDeviceToHost,
                               h->st));
-        HIPCHK(hipMemcpyAsync(&hv[j + 1], h->g_scal.p, sizeof(zd), hipMemcpyDeviceToHost, h->st));
         HIPCHK(hipStreamSynchronize(h->st));
         double dloo = 0.0;
         for (int i = 0; i <= j; ++i) {
@@ -995,6 +1013,7 @@ int spk_gmres_device(spk_handle *h, const void *d_b, void *d_x, const spk_gmres_
     HIPCHK(h->g_xc.alloc(n));
   }
   HIPCHK(h->g_h.alloc(m + 1));
+  HIPCHK(h->pin.ensure(m + 2));
   HIPCHK(h->g_y.alloc(m + 1));
   const size_t gc_elems = gemv_c_partial_elems(n, m + 1, 2);
   HIPCHK(h->g_partial.alloc(gc_elems + (size_t)reduce_blocks(n)));
@@ -1107,7 +1126,7 @@ int spk_gmres_device(spk_handle *h, const void *d_b, void *d_x, const spk_gmres_
         double dloo = 0.0;
         if (sweepG) {
           // the whole MGS pass and |w|^2 in one cooperative launch
-          HIPCHK(launch_mgs_sweep(sweepG, n, jH, V, n, w, h->g_slots.p, h->g_h.p, h->g_scal.p,
+          HIPCHK(launch_mgs_sweep(sweepG, n, jH, V, n, w, h->g_slots.p, h->g_h.p, h->g_h.p + jH,
                                   h->g_ctl.p, ++h->sweep_seq, h->st));
         } else if (one_reduce) {
           zd *Tm = h->g_T.p, *c = h->g_c.p;
@@ -1135,10 +1154,11 @@ int spk_gmres_device(spk_handle *h, const void *d_b, void *d_x, const spk_gmres_
                         h->g_partial.p, h->g_h.p, h->st);
           launch_gemv_n_sub(n, jH, (const zd *)V, n, h->g_h.p, w, h->g_pn.p, h->st);
         }
-        if (!sweepG) launch_zdotc(n, w, w, partial_nrm, h->g_scal.p, h->st);
-        std::vector<zd> hv(jH + 1);
-        HIPCHK(hipMemcpyAsync(hv.data(), h->g_h.p, jH * sizeof(zd), hipMemcpyDeviceToHost, h->st));
-        HIPCHK(hipMemcpyAsync(&hv[jH], h->g_scal.p, sizeof(zd), hipMemcpyDeviceToHost, h->st));
+        // |w|^2 lands after the coefficients: one copy to pinned memory per pass
+        if (!sweepG) launch_zdotc(n, w, w, partial_nrm, h->g_h.p + jH, h->st);
+        const zd *hv = h->pin.p;
+        HIPCHK(hipMemcpyAsync(h->pin.p, h->g_h.p, (jH + 1) * sizeof(zd), hipMemcpyDeviceToHost,
+                              h->st));
         HIPCHK(hipStreamSynchronize(h->st));
         if (sweepG && hv[jH].y != 0.0) {
           // an exchange wait ran out of polls: the pass is void

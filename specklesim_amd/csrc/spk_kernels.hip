@@ -1661,7 +1661,8 @@ hipError_t launch_mgs_sweep(int G, int64_t n, int k, const zd *V, int64_t ldv, z
 constexpr int kGemvWorkgroups = 2048;
 constexpr int kGemvMaxRowChunks = 256;
 static inline int gemv_c_rows(int64_t n, int ctiles) {
-  const int64_t by_rows = (n + 255) / 256;
+  // at least 4 rows per thread, so that the per-tile reduction is amortised
+  const int64_t by_rows = (n + 1023) / 1024;
   const int64_t by_fill = (kGemvWorkgroups + ctiles - 1) / ctiles;
   return (int)std::max<int64_t>(1, std::min<int64_t>(std::min(by_rows, by_fill), kGemvMaxRowChunks));
 }
