@@ -77,7 +77,8 @@ typedef struct spk_gmres_params {
   int use_initial_guess;         /* icntl(6) */
   int refinement_steps;          /* SPK_MIXED: maximum outer refinement steps (0: 30) */
   double refinement_tolerance;   /* SPK_MIXED: relative residual of each inner complex64
-                                    solve (0: 1e-4); tolerance is the FP64 target */
+                                    solve (0: 3e-4), relaxed to tolerance / (2 |r|/|b|) for
+                                    the later solves; tolerance is the FP64 target */
   int preconditioning;           /* icntl(4): SPK_PRECOND_NONE or SPK_PRECOND_LEFT_JACOBI
                                     (precondition_left, src/lib_sie_solver_gmres_interface.f90:115;
                                     diagonal of the self blocks, lib_sie_tri_mlfmm_interface.f90:94-109);

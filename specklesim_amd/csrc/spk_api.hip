@@ -827,8 +827,9 @@ static int dev_dot(spk_handle *h, const zd *x, const zd *y, int64_t n, zc *out) 
 // the CERFACS reorthogonalisation test (ICGS), dot products accumulated in
 // complex128 and the Hessenberg matrix and Givens rotations kept in
 // complex128 on the host; it stops at the relative residual
-// refinement_tolerance (default 1e-4).  x += |r| d.  No counterpart in the
-// reference, which computes in FP64 only.
+// max(refinement_tolerance (default 3e-4), tolerance / (2 |r|/|b|)): the
+// last solves need only the reduction still missing.  x += |r| d.  No
+// counterpart in the reference, which computes in FP64 only.
 // ---------------------------------------------------------------------------
 static int gmres_mixed(spk_handle *h, const zd *b, zd *x, const spk_gmres_params *gp,
                        spk_gmres_info *info) {
@@ -840,7 +841,7 @@ static int gmres_mixed(spk_handle *h, const zd *b, zd *x, const spk_gmres_params
   const int max_it = std::max(1, gp->max_iterations);
   const int m = gp->restart > 0 ? std::min(gp->restart, max_it) : max_it;
   const double tol = gp->tolerance;
-  const double inner_tol = gp->refinement_tolerance > 0 ? gp->refinement_tolerance : 1e-4;
+  const double inner_tol = gp->refinement_tolerance > 0 ? gp->refinement_tolerance : 3e-4;
   const int max_outer = gp->refinement_steps > 0 ? gp->refinement_steps : 30;
   if (h->i_m != m || h->i_V.n != (size_t)n * (m + 1)) {
     HIPCHK(h->i_V.alloc((size_t)n * (m + 1)));
@@ -914,6 +915,9 @@ static int gmres_mixed(spk_handle *h, const zd *b, zd *x, const spk_gmres_params
     const int jmax = std::min(m, max_it - total);
     int k = 0;
     double est = 1.0;
+    // this inner solve only has to supply the reduction still missing, with
+    // a factor 2 margin: stop at max(refinement tolerance, tol / (2 be))
+    const double stop_at = std::max(inner_tol, std::min(0.5, 0.5 * tol / be));
     for (int j = 0; j < jmax; ++j) {
       if ((rc = matvec_t<zf>(h, V + (size_t)j * n, w))) return rc;
       ++matvecs;
@@ -957,7 +961,7 @@ static int gmres_mixed(spk_handle *h, const zd *b, zd *x, const spk_gmres_params
       ++total;
       k = j + 1;
       h->hist.push_back(est * be);
-      if (est <= inner_tol || dnormw == 0.0) break;
+      if (est <= stop_at || dnormw == 0.0) break;
     }
     bea = est * be;
     // y = H^-1 g, d = V y, x += |r| d

@@ -171,8 +171,8 @@ class SieTriMlfmm:
         """CERFACS GMRES on the FP64 operator (precondition_left: the
         reference's Jacobi left preconditioner); on a precision="mixed"
         handle, FP64 iterative refinement around complex64 inner GMRES
-        solves (refinement_tolerance: inner relative residual, default
-        1e-4)."""
+        solves (refinement_tolerance: inner relative residual of the first
+        solve, default 3e-4; later solves stop at tolerance / (2 |r|/|b|))."""
         b = np.ascontiguousarray(vector_b, dtype=np.complex128)
         x = self.get_init_vector_x_tri() if x0 is None else np.array(x0, dtype=np.complex128)
         gp = nat.GmresParams(max_iterations, restart, tolerance,
