@@ -161,6 +161,15 @@ int spk_quad_rhs_plane_wave_device(spk_quad *q, const double k_in[3], const doub
 int spk_quad_topology_probe(double Lx, double Ly, int nx, int ny, const double *z, int *n_edges,
                             int *n_elements, int *n_nodes, double *pp, int *vert8, int *edge_p5);
 
+/* Mixed-precision solves of R <= 16 right-hand sides of one mesh (incidence
+ * angles) on an SPK_MIXED handle: each runs the refinement of spk_gmres
+ * (FP64 residuals, complex64 inner GMRES with its own basis); the inner
+ * steps of all right-hand sides share one batched complex64 matvec.  d_b,
+ * d_x: host arrays of R device pointers (2N complex128 each); info[R].  No
+ * counterpart in the reference, which solves one incidence at a time. */
+int spk_gmres_batch_device(spk_handle *h, int R, const void *const *d_b, void *const *d_x,
+                           const spk_gmres_params *params, spk_gmres_info *info);
+
 /* Device memory for the *_device entry points (no counterpart in the
  * reference).  A caller that has its own HIP allocations may pass those
  * instead; they must come from the same HIP runtime as this library. */
@@ -186,6 +195,14 @@ int spk_matvec_device(spk_handle *h, const void *d_x, void *d_y);
  * reference (which is FP64 only). */
 int spk_matvec_c64(spk_handle *h, const float *x, float *y);
 int spk_matvec_c64_device(spk_handle *h, const void *d_x, void *d_y);
+/* The matvec for R <= 16 right-hand sides at once (incidence angles of one
+ * mesh; no counterpart in the reference, which applies
+ * lib_sie_ml_fmm_calculate_vector_b_tri to one vector at a time): d_x, d_y
+ * are host arrays of R device pointers to 2N-complex vectors.  The operator
+ * is read once per call for all R vectors. */
+int spk_matvec_batch_device(spk_handle *h, int R, const void *const *d_x, void *const *d_y);
+/* Same for the complex64 operator of an SPK_MIXED handle (2N complex64 vectors). */
+int spk_matvec_batch_c64_device(spk_handle *h, int R, const void *const *d_x, void *const *d_y);
 /* Far field only / near field only (parity probes, timing splits). */
 int spk_matvec_parts(spk_handle *h, const double *x, double *y_near, double *y_far);
 

@@ -22,7 +22,8 @@ EXPORTS = [
     "spk_quad_create", "spk_quad_destroy", "spk_quad_setup", "spk_quad_sizes",
     "spk_quad_get_mesh", "spk_quad_get_edges", "spk_quad_edge_element", "spk_quad_fill",
     "spk_quad_fill_device", "spk_quad_topology_probe", "spk_quad_rhs_plane_wave",
-    "spk_quad_rhs_plane_wave_device", "spk_probe_read_bandwidth",
+    "spk_quad_rhs_plane_wave_device", "spk_probe_read_bandwidth", "spk_matvec_batch_device",
+    "spk_matvec_batch_c64_device", "spk_gmres_batch_device",
     "spk_device_alloc", "spk_device_free", "spk_device_copy", "spk_device_synchronize",
     "spk_gmres", "spk_gmres_device", "spk_gmres_history", "spk_get_rwg", "spk_get_tree",
     "spk_get_i4", "spk_get_near_csr", "spk_near_blocks", "spk_get_patterns",
@@ -108,6 +109,11 @@ def lib():
     L.spk_quad_rhs_plane_wave.argtypes = [vp, dp, dp, dp]
     L.spk_quad_rhs_plane_wave_device.argtypes = [vp, dp, dp, vp]
     L.spk_probe_read_bandwidth.argtypes = [C.c_int, C.c_size_t, C.c_int, dp]
+    vpp = C.POINTER(C.c_void_p)
+    L.spk_matvec_batch_device.argtypes = [vp, C.c_int, vpp, vpp]
+    L.spk_matvec_batch_c64_device.argtypes = [vp, C.c_int, vpp, vpp]
+    L.spk_gmres_batch_device.argtypes = [vp, C.c_int, vpp, vpp, C.POINTER(GmresParams),
+                                         C.POINTER(GmresInfo)]
     L.spk_quad_topology_probe.argtypes = [C.c_double, C.c_double, C.c_int, C.c_int, dp, ip, ip, ip,
                                           dp, ip, ip]
     L.spk_device_alloc.argtypes = [C.c_int, C.c_size_t, C.POINTER(vp)]

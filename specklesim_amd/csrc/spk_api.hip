@@ -68,6 +68,16 @@ struct DBuf {
   }
 };
 
+// buffers of the batched matvec (R right-hand sides), one set per precision
+template <class T>
+struct BatchBufs {
+  int R = 0;
+  std::vector<DBuf<T>> B, D;  // per level, [box][it][s][r]
+  DBuf<T> W, YF;              // folded leaf coefficients; far part [r][2N]
+  DBuf<const T *> xs;
+  DBuf<T *> ys;
+};
+
 struct LevelDev {
   DBuf<zd> B, D, tlt, up, down;
   DBuf<zf> fB, fD, ftlt, fup, fdown;  // complex64 mirror (mixed precision)
@@ -101,6 +111,11 @@ struct spk_handle {
   DBuf<zd> d_blocks;
   DBuf<int> d_row_box, d_row_r0, d_disagg_task;
   int disagg_tasks = 0;
+  // batched matvec: disaggregation tiles (box, first position), buffers
+  DBuf<int> d_tile_box, d_tile_p0;
+  int ntiles = 0;
+  BatchBufs<zd> bd;
+  BatchBufs<zf> bf;
   int spmv_wgs = 0;
   DBuf<zd> d_rt, d_W;
   DBuf<double> d_khat, d_tth, d_tph, d_q, d_leaf_centre, d_trf;
@@ -390,6 +405,71 @@ int matvec_t(spk_handle *h, const T *x, T *y, bool near_on = true, bool far_on =
   return SPK_OK;
 }
 
+static BatchBufs<zd> &pBB(spk_handle *h, zd *) { return h->bd; }
+static BatchBufs<zf> &pBB(spk_handle *h, zf *) { return h->bf; }
+
+// batch buffers for R right-hand sides (kept for later calls with R or fewer)
+template <class T>
+static int batch_ensure(spk_handle *h, int R) {
+  T *tag = nullptr;
+  BatchBufs<T> &BB = pBB(h, tag);
+  if (BB.R >= R) return SPK_OK;
+  const Tree &t = h->tree;
+  const size_t S = h->tb.S;
+  BB.B.clear();
+  BB.D.clear();
+  BB.B.resize(t.l_max + 1);
+  BB.D.resize(t.l_max + 1);
+  for (int l = t.l_min; l <= t.l_max; ++l) {
+    HIPCHK(BB.B[l].alloc((size_t)h->lev[l].nb * 8 * S * R));
+    HIPCHK(BB.D[l].alloc((size_t)h->lev[l].nb * 8 * S * R));
+  }
+  HIPCHK(BB.W.alloc((size_t)h->lev[t.l_max].nb * 8 * S * R));
+  HIPCHK(BB.YF.alloc((size_t)R * 2 * h->N));
+  HIPCHK(BB.xs.alloc(kBatchMax));
+  HIPCHK(BB.ys.alloc(kBatchMax));
+  BB.R = R;
+  return SPK_OK;
+}
+
+// ys[r] = Z xs[r] for r < R <= kBatchMax; xs, ys: device arrays of R pointers.
+template <class T>
+int matvec_batch_t(spk_handle *h, int R, const T *const *xs, T *const *ys) {
+  T *tag = nullptr;
+  if (R < 1 || R > kBatchMax) return fail(SPK_ERR_ARG, "batch size must be 1..16");
+  int rc = batch_ensure<T>(h, R);
+  if (rc) return rc;
+  BatchBufs<T> &BB = pBB(h, tag);
+  const Tree &t = h->tree;
+  const int S = h->tb.S, N = h->N, lmax = t.l_max, lmin = t.l_min;
+  hipStream_t st = h->st;
+  LevelDev &LL = h->lev[lmax];
+  launch_batch_agg(LL.nb, S, N, R, h->d_box_first.p, h->d_box_count.p, h->d_perm.p, pRt(h, tag),
+                   xs, BB.B[lmax].p, st);
+  for (int l = lmax - 1; l >= lmin; --l)
+    launch_batch_ss(h->lev[l].nb, S, R, h->lev[l].child.p, (const T *)BB.B[l + 1].p,
+                    pUp(h->lev[l + 1], tag), BB.B[l].p, st);
+  for (int l = lmin; l <= lmax; ++l)
+    launch_batch_m2l(h->lev[l].nb, S, R, h->lev[l].i4_off.p, h->lev[l].i4_box.p,
+                     h->lev[l].i4_slot.p, (const T *)BB.B[l].p, pT(h->lev[l], tag), BB.D[l].p, st);
+  for (int l = lmin + 1; l <= lmax; ++l)
+    launch_batch_rr(h->lev[l].nb, S, R, h->lev[l].parent.p, h->lev[l].octant.p,
+                    (const T *)BB.D[l - 1].p, pDn(h->lev[l], tag), BB.D[l].p, st);
+  const zd I = {0, 1};
+  const zd kd1 = I * zcx(h->mat.k1), kd2 = I * zcx(h->mat.k2);
+  const zd my1 = zd{h->mat.my1, 0} * I * h->mat.omega, my2 = zd{h->mat.my2, 0} * I * h->mat.omega;
+  const zd ep1 = zcx(h->mat.eps1) * I * h->mat.omega, ep2 = zcx(h->mat.eps2) * I * h->mat.omega;
+  launch_batch_fold(LL.nb, S, R, (const T *)BB.D[lmax].p, kd1, kd2, my1, my2, ep1, ep2, BB.W.p, st);
+  launch_batch_disagg(h->ntiles, N, S, R, h->d_tile_box.p, h->d_tile_p0.p, h->d_box_first.p,
+                      h->d_box_count.p, h->d_perm.p, pRt(h, tag), (const T *)BB.W.p, h->sc1,
+                      h->sc2, BB.YF.p, st);
+  launch_batch_near(h->spmv_wgs, N, R, h->d_row_box.p, h->d_row_r0.p, h->d_box_first.p,
+                    h->d_box_count.p, h->d_e2_off.p, h->d_e2_list.p, h->d_near_off.p, h->d_perm.p,
+                    pBlk(h, tag), xs, (const T *)BB.YF.p, ys, st);
+  HIPCHK(hipGetLastError());
+  return SPK_OK;
+}
+
 int matvec_dev(spk_handle *h, const zd *x, zd *y, bool near_on = true, bool far_on = true,
                hipEvent_t *ev = nullptr) {
   return matvec_t<zd>(h, x, y, near_on, far_on, ev);
@@ -533,6 +613,19 @@ int spk_setup_tri(spk_handle *h, int np, const double *pp, int nt, const int *tt
       dt.push_back((t.box_first[b] + r) * 8 + std::min(kDisaggEdges, t.box_count[b] - r));
   h->disagg_tasks = (int)dt.size();
   HIPCHK(h->d_disagg_task.upload(dt));
+  {
+    std::vector<int> tb, tp;
+    for (size_t b = 0; b < t.box_count.size(); ++b)
+      for (int r = 0; r < t.box_count[b]; r += 16) {
+        tb.push_back((int)b);
+        tp.push_back(t.box_first[b] + r);
+      }
+    h->ntiles = (int)tb.size();
+    HIPCHK(h->d_tile_box.upload(tb));
+    HIPCHK(h->d_tile_p0.upload(tp));
+    h->bd = BatchBufs<zd>();
+    h->bf = BatchBufs<zf>();
+  }
   HIPCHK(h->d_x.alloc(2 * (size_t)h->N));
   HIPCHK(h->d_y.alloc(2 * (size_t)h->N));
   HIPCHK(h->d_yfar.alloc(2 * (size_t)h->N));
@@ -618,6 +711,39 @@ int spk_matvec_c64_device(spk_handle *h, const void *d_x, void *d_y) {
   if (rc) return rc;
   HIPCHK(hipStreamSynchronize(h->st));
   return SPK_OK;
+}
+
+}  // extern "C"
+
+// the batched C-ABI entry points: host arrays of R device pointers
+template <class T>
+static int matvec_batch_api(spk_handle *h, int R, const void *const *d_x, void *const *d_y) {
+  if (!h || !d_x || !d_y) return fail(SPK_ERR_ARG, "null argument");
+  if (!h->ready) return fail(SPK_ERR_STATE, "spk_setup_tri has not completed");
+  if (R < 1 || R > kBatchMax) return fail(SPK_ERR_ARG, "batch size must be 1..16");
+  for (int r = 0; r < R; ++r)
+    if (!d_x[r] || !d_y[r]) return fail(SPK_ERR_ARG, "null vector in the batch");
+  int rc = batch_ensure<T>(h, R);
+  if (rc) return rc;
+  T *tag = nullptr;
+  BatchBufs<T> &BB = pBB(h, tag);
+  HIPCHK(hipMemcpy(BB.xs.p, d_x, R * sizeof(void *), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(BB.ys.p, d_y, R * sizeof(void *), hipMemcpyHostToDevice));
+  rc = matvec_batch_t<T>(h, R, BB.xs.p, BB.ys.p);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(h->st));
+  return SPK_OK;
+}
+
+extern "C" {
+
+int spk_matvec_batch_device(spk_handle *h, int R, const void *const *d_x, void *const *d_y) {
+  return matvec_batch_api<zd>(h, R, d_x, d_y);
+}
+
+int spk_matvec_batch_c64_device(spk_handle *h, int R, const void *const *d_x, void *const *d_y) {
+  if (h && !h->mixed_ready) return fail(SPK_ERR_STATE, "handle was not created with SPK_MIXED");
+  return matvec_batch_api<zf>(h, R, d_x, d_y);
 }
 
 int spk_matvec_c64(spk_handle *h, const float *x, float *y) {
@@ -992,6 +1118,274 @@ static int gmres_mixed(spk_handle *h, const zd *b, zd *x, const spk_gmres_params
   }
   h->refinements = outer;
   return SPK_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Mixed precision for R <= 16 right-hand sides of one mesh (incidence angles).
+// Each right-hand side runs the refinement of gmres_mixed with its own
+// complex64 Krylov basis; the inner Arnoldi steps of all right-hand sides
+// that are inside an inner solve advance together, with one batched
+// complex64 matvec per step, and the FP64 residuals of those that finish an
+// inner solve in the same step are one batched FP64 matvec.  No counterpart
+// in the reference.
+// ---------------------------------------------------------------------------
+}  // extern "C"
+
+namespace {
+struct RhsState {
+  // host copies of the inner solve
+  std::vector<zc> H, g, rs, yc;
+  std::vector<double> rcs;
+  double bn = 0, rn = 0, be = 1, bea = 1, beta = 0, est = 1, stop_at = 0;
+  int j = 0, total = 0, outer = 0, matvecs = 0, phase = 0;  // phase 0 inner, 1 residual, 2 done
+};
+}  // namespace
+
+static int gmres_mixed_batch(spk_handle *h, int R, const zd *const *b, zd *const *x,
+                             const spk_gmres_params *gp, spk_gmres_info *info) {
+  if (!h->mixed_ready) return fail(SPK_ERR_STATE, "handle was not created with SPK_MIXED");
+  if (R < 1 || R > kBatchMax) return fail(SPK_ERR_ARG, "batch size must be 1..16");
+  if (gp->preconditioning != SPK_PRECOND_NONE)
+    return fail(SPK_ERR_ARG, "preconditioning is not available with SPK_MIXED");
+  const double t_start = now_ms();
+  const int64_t n = 2 * (int64_t)h->N;
+  const int max_it = std::max(1, gp->max_iterations);
+  const int m = gp->restart > 0 ? std::min(gp->restart, max_it) : max_it;
+  const double tol = gp->tolerance;
+  const double inner_tol = gp->refinement_tolerance > 0 ? gp->refinement_tolerance : 3e-4;
+  const int max_outer = gp->refinement_steps > 0 ? gp->refinement_steps : 30;
+  const size_t ldh = (size_t)m + 2;
+  // device state: bases, work vectors, coefficients
+  DBuf<zf> V, W;
+  DBuf<zd> Rv, Dv, Tv, Hd, Y, dots;
+  DBuf<const zf *> xs32;
+  DBuf<zf *> ys32;
+  DBuf<const zd *> xs64;
+  DBuf<zd *> ys64;
+  HIPCHK(V.alloc((size_t)R * (m + 1) * n));
+  HIPCHK(W.alloc((size_t)R * n));
+  HIPCHK(Rv.alloc((size_t)R * n));
+  HIPCHK(Dv.alloc(n));
+  HIPCHK(Tv.alloc((size_t)R * n));
+  HIPCHK(Hd.alloc((size_t)R * ldh));
+  HIPCHK(Y.alloc(ldh));
+  HIPCHK(dots.alloc(R));
+  HIPCHK(xs32.alloc(kBatchMax));
+  HIPCHK(ys32.alloc(kBatchMax));
+  HIPCHK(xs64.alloc(kBatchMax));
+  HIPCHK(ys64.alloc(kBatchMax));
+  const size_t gc_elems = gemv_c_partial_elems(n, m + 1, 1);
+  HIPCHK(h->g_partial.alloc(gc_elems + (size_t)reduce_blocks(n)));
+  zd *partial_nrm = h->g_partial.p + gc_elems;
+  HIPCHK(h->g_pn.alloc(gemv_n_partial_elems(n, m + 1)));
+  HIPCHK(h->pin.ensure((size_t)R * ldh));
+  auto Vr = [&](int r) { return V.p + (size_t)r * (m + 1) * n; };
+  auto Wr = [&](int r) { return W.p + (size_t)r * n; };
+  auto Rr = [&](int r) { return Rv.p + (size_t)r * n; };
+  auto Tr = [&](int r) { return Tv.p + (size_t)r * n; };
+  std::vector<RhsState> S(R);
+  for (auto &st : S) {
+    st.H.assign((size_t)(m + 1) * (m + 1), 0.0);
+    st.g.assign(m + 1, 0.0);
+    st.rs.assign(m + 1, 0.0);
+    st.yc.assign(m + 1, 0.0);
+    st.rcs.assign(m + 1, 0.0);
+  }
+  // |u_r|^2 for a list of right-hand sides, one transfer
+  auto norms2 = [&](const std::vector<int> &rr, auto vec_of, std::vector<double> *out) -> int {
+    for (size_t i = 0; i < rr.size(); ++i) {
+      const auto *u = vec_of(rr[i]);
+      launch_zdotc(n, u, u, partial_nrm, dots.p + i, h->st);
+    }
+    HIPCHK(hipMemcpyAsync(h->pin.p, dots.p, rr.size() * sizeof(zd), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    out->resize(rr.size());
+    for (size_t i = 0; i < rr.size(); ++i) (*out)[i] = std::max(h->pin.p[i].x, 0.0);
+    return SPK_OK;
+  };
+  int rc = SPK_OK;
+  std::vector<int> all(R);
+  for (int r = 0; r < R; ++r) all[r] = r;
+  std::vector<double> nv;
+  if ((rc = norms2(all, [&](int r) { return b[r]; }, &nv))) return rc;
+  for (int r = 0; r < R; ++r) {
+    S[r].bn = std::sqrt(nv[r]);
+    if (!gp->use_initial_guess) HIPCHK(hipMemsetAsync(x[r], 0, n * sizeof(zd), h->st));
+  }
+  // residuals r = b - Z x (batched FP64 matvec) for a set of right-hand sides;
+  // then each either finishes or starts its next inner solve
+  auto residuals = [&](const std::vector<int> &set) -> int {
+    if (set.empty()) return SPK_OK;
+    std::vector<const zd *> xp;
+    std::vector<zd *> yp;
+    for (int r : set) {
+      xp.push_back(x[r]);
+      yp.push_back(Tr(r));
+    }
+    HIPCHK(hipMemcpy(xs64.p, xp.data(), set.size() * sizeof(void *), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ys64.p, yp.data(), set.size() * sizeof(void *), hipMemcpyHostToDevice));
+    int e = matvec_batch_t<zd>(h, (int)set.size(), xs64.p, ys64.p);
+    if (e) return e;
+    for (int r : set) {
+      launch_zsub(n, b[r], Tr(r), Rr(r), h->st);
+      ++S[r].matvecs;
+    }
+    std::vector<double> rr2;
+    if ((e = norms2(set, Rr, &rr2))) return e;
+    std::vector<int> start;
+    for (size_t i = 0; i < set.size(); ++i) {
+      RhsState &st = S[set[i]];
+      st.rn = std::sqrt(rr2[i]);
+      st.be = st.bn > 0 ? st.rn / st.bn : 0.0;
+      if (st.outer == 0) st.bea = st.be;
+      if (st.bn == 0.0 || st.be <= tol || st.total >= max_it || st.outer >= max_outer) {
+        st.phase = 2;
+      } else {
+        st.phase = 0;
+        start.push_back(set[i]);
+      }
+    }
+    // V_0 = r / |r| in complex64, renormalised after rounding
+    for (int r : start) launch_scale_down(n, 1.0 / S[r].rn, Rr(r), Vr(r), h->st);
+    std::vector<double> vv;
+    if ((e = norms2(start, [&](int r) { return (const zf *)Vr(r); }, &vv))) return e;
+    for (size_t i = 0; i < start.size(); ++i) {
+      RhsState &st = S[start[i]];
+      st.beta = std::sqrt(vv[i]);
+      launch_scal_copy_f(n, 1.0 / st.beta, Vr(start[i]), Vr(start[i]), h->st);
+      std::fill(st.H.begin(), st.H.end(), zc(0.0));
+      std::fill(st.g.begin(), st.g.end(), zc(0.0));
+      st.g[0] = st.beta;
+      st.j = 0;
+      st.est = 1.0;
+      st.stop_at = std::max(inner_tol, std::min(0.5, 0.5 * tol / st.be));
+    }
+    return SPK_OK;
+  };
+  if ((rc = residuals(all))) return rc;
+  std::vector<int> act;
+  for (;;) {
+    act.clear();
+    for (int r = 0; r < R; ++r)
+      if (S[r].phase == 0) act.push_back(r);
+    if (act.empty()) break;
+    // one batched complex64 matvec: w_r = Z32 v_{r, j_r}
+    std::vector<const zf *> xp;
+    std::vector<zf *> yp;
+    for (int r : act) {
+      xp.push_back(Vr(r) + (size_t)S[r].j * n);
+      yp.push_back(Wr(r));
+    }
+    HIPCHK(hipMemcpy(xs32.p, xp.data(), act.size() * sizeof(void *), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ys32.p, yp.data(), act.size() * sizeof(void *), hipMemcpyHostToDevice));
+    if ((rc = matvec_batch_t<zf>(h, (int)act.size(), xs32.p, ys32.p))) return rc;
+    // ICGS passes for all active right-hand sides; a second pass only for
+    // those whose |w| fell below |h| (the CERFACS test)
+    std::vector<int> pass = act;
+    std::vector<double> dnormw(R, 0.0);
+    for (int np_ = 1; !pass.empty() && np_ <= 3; ++np_) {
+      for (int r : pass) {
+        const int j = S[r].j;
+        zd *hr = Hd.p + (size_t)r * ldh;
+        launch_gemv_c(n, j + 1, (const zf *)Vr(r), n, (const zf *)Wr(r), (const zf *)nullptr,
+                      h->g_partial.p, hr, h->st);
+        launch_gemv_n_sub(n, j + 1, (const zf *)Vr(r), n, hr, Wr(r), h->g_pn.p, h->st);
+        launch_zdotc(n, (const zf *)Wr(r), (const zf *)Wr(r), partial_nrm, hr + (j + 1), h->st);
+      }
+      HIPCHK(hipMemcpyAsync(h->pin.p, Hd.p, (size_t)R * ldh * sizeof(zd), hipMemcpyDeviceToHost,
+                            h->st));
+      HIPCHK(hipStreamSynchronize(h->st));
+      std::vector<int> again;
+      for (int r : pass) {
+        RhsState &st = S[r];
+        const int j = st.j;
+        const zd *hv = h->pin.p + (size_t)r * ldh;
+        double dloo = 0.0;
+        for (int i = 0; i <= j; ++i) {
+          const zc hi(hv[i].x, hv[i].y);
+          st.H[(size_t)j * (m + 1) + i] += hi;
+          dloo += std::norm(hi);
+        }
+        dnormw[r] = std::sqrt(std::max(hv[j + 1].x, 0.0));
+        if (dnormw[r] <= std::sqrt(dloo) && np_ < 3) again.push_back(r);
+      }
+      pass.swap(again);
+    }
+    // Hessenberg update, Givens rotations and the stopping test per right-hand side
+    std::vector<int> fin;
+    for (int r : act) {
+      RhsState &st = S[r];
+      const int j = st.j;
+      auto HH = [&](int i, int c) -> zc & { return st.H[(size_t)c * (m + 1) + i]; };
+      HH(j + 1, j) = dnormw[r];
+      if (dnormw[r] > 0.0)
+        launch_scal_copy_f(n, 1.0 / dnormw[r], Wr(r), Vr(r) + (size_t)(j + 1) * n, h->st);
+      for (int i = 0; i < j; ++i) zrot1(&HH(i, j), &HH(i + 1, j), st.rcs[i], st.rs[i]);
+      zc a = HH(j, j);
+      double cc;
+      zc ss;
+      zrotg_(&a, HH(j + 1, j), &cc, &ss);
+      st.rcs[j] = cc;
+      st.rs[j] = ss;
+      zrot1(&st.g[j], &st.g[j + 1], cc, ss);
+      zrot1(&HH(j, j), &HH(j + 1, j), cc, ss);
+      HH(j + 1, j) = 0.0;
+      st.est = std::abs(st.g[j + 1]) / st.beta;
+      ++st.total;
+      ++st.matvecs;
+      st.bea = st.est * st.be;
+      st.j = j + 1;
+      if (st.est <= st.stop_at || dnormw[r] == 0.0 || st.j >= m || st.total >= max_it) fin.push_back(r);
+    }
+    // finished inner solves: x += |r| V y, then their FP64 residuals together
+    for (int r : fin) {
+      RhsState &st = S[r];
+      const int k = st.j;
+      auto HH = [&](int i, int c) -> zc & { return st.H[(size_t)c * (m + 1) + i]; };
+      for (int j = k - 1; j >= 0; --j) {
+        zc sum = st.g[j];
+        for (int i = j + 1; i < k; ++i) sum -= HH(j, i) * st.yc[i];
+        st.yc[j] = sum / HH(j, j);
+      }
+      std::vector<zd> yd(k);
+      for (int j = 0; j < k; ++j) yd[j] = zd{st.yc[j].real(), st.yc[j].imag()};
+      HIPCHK(hipMemcpyAsync(Y.p, yd.data(), k * sizeof(zd), hipMemcpyHostToDevice, h->st));
+      HIPCHK(hipStreamSynchronize(h->st));
+      launch_gemv_n(n, k, (const zf *)Vr(r), n, Y.p, Dv.p, h->g_pn.p, h->st);
+      launch_zaxpy(n, zd{st.rn, 0.0}, Dv.p, x[r], h->st);
+      ++st.outer;
+      st.phase = 1;
+    }
+    if ((rc = residuals(fin))) return rc;
+  }
+  HIPCHK(hipStreamSynchronize(h->st));
+  HIPCHK(hipGetLastError());
+  const double secs = (now_ms() - t_start) / 1e3;
+  for (int r = 0; r < R; ++r) {
+    const RhsState &st = S[r];
+    info[r].info = st.be <= tol ? 0 : -4;
+    info[r].iterations = st.total;
+    info[r].arnoldi_backward_error = st.bea;
+    info[r].backward_error = st.be;
+    info[r].matvecs = st.matvecs;
+    info[r].seconds = secs;
+    info[r].true_backward_error = st.be;
+  }
+  return SPK_OK;
+}
+
+extern "C" {
+
+int spk_gmres_batch_device(spk_handle *h, int R, const void *const *d_b, void *const *d_x,
+                           const spk_gmres_params *gp, spk_gmres_info *info) {
+  if (!h || !d_b || !d_x || !gp || !info) return fail(SPK_ERR_ARG, "null argument");
+  if (!h->ready) return fail(SPK_ERR_STATE, "spk_setup_tri has not completed");
+  if (h->p.precision != SPK_MIXED)
+    return fail(SPK_ERR_STATE, "the batched solve runs on an SPK_MIXED handle");
+  if (R < 1 || R > kBatchMax) return fail(SPK_ERR_ARG, "batch size must be 1..16");
+  for (int r = 0; r < R; ++r)
+    if (!d_b[r] || !d_x[r]) return fail(SPK_ERR_ARG, "null vector in the batch");
+  return gmres_mixed_batch(h, R, (const zd *const *)d_b, (zd *const *)d_x, gp, info);
 }
 
 int spk_gmres_device(spk_handle *h, const void *d_b, void *d_x, const spk_gmres_params *gp,

@@ -1,0 +1,355 @@
+// spk_batch.hip -- the MLFMM matvec applied to R <= 16 right-hand sides at
+// once (incidence angles of one mesh): every operator array (near blocks,
+// patterns, TL and phase tables) is read once per launch for all R vectors.
+//
+// Batched k-space arrays keep the right-hand side fastest: a box's
+// coefficients are [it][s][r], index (it * S + s) * R + r, so the
+// elementwise translation kernels run over u = s * R + r with the table
+// index s = u / R.  Vectors cross the launchers as device arrays of R
+// pointers (each a 2N complex vector in the reference order).
+// The arithmetic of each kernel is that of its single-vector counterpart in
+// spk_kernels.hip; only the order of the near-field and disaggregation sums
+// differs.
+#include "spk_device.h"
+
+namespace spk {
+
+namespace {
+
+inline unsigned bnblk(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
+
+template <class T>
+__device__ inline T bshfl_down(T v, int off) {
+  v.x = __shfl_down(v.x, off, 64);
+  v.y = __shfl_down(v.y, off, 64);
+  return v;
+}
+
+}  // namespace
+
+// Aggregation: C[b][it][s][r] = sum_{edges of b} conj(RT[p][q][s]) x_r(cur),
+// it = comp*4 + cur*2 + k.  Workgroup = (256 samples, leaf box, q); the
+// box's x values are staged in LDS 64 edges at a time and read as
+// broadcasts; each thread keeps 2R accumulators.
+template <class T>
+__global__ void __launch_bounds__(256) kb_agg(int S, int N, int R, const int *box_first,
+                                              const int *box_count, const int *perm, const T *rt,
+                                              const T *const *xs, T *C) {
+  __shared__ T xl[64 * 2 * kBatchMax];
+  const int s = blockIdx.x * 256 + threadIdx.x;
+  const int b = blockIdx.y, q = blockIdx.z, k = q >> 1, comp = q & 1;
+  const int p0 = box_first[b], cnt = box_count[b];
+  T ae[kBatchMax], ah[kBatchMax];
+#pragma unroll
+  for (int r = 0; r < kBatchMax; ++r) ae[r] = ah[r] = T{0, 0};
+  for (int e0 = 0; e0 < cnt; e0 += 64) {
+    const int ne = min(64, cnt - e0);
+    __syncthreads();
+    for (int i = threadIdx.x; i < ne * 2 * R; i += 256) {
+      const int e = i / (2 * R), rem = i - e * 2 * R, cur = rem / R, r = rem - cur * R;
+      const int m = perm[p0 + e0 + e];
+      xl[i] = xs[r][m + cur * N];
+    }
+    __syncthreads();
+    if (s < S) {
+      for (int e = 0; e < ne; ++e) {
+        const T f = conj(rt[((size_t)(p0 + e0 + e) * 4 + q) * S + s]);
+        const T *xe = xl + e * 2 * R;
+#pragma unroll
+        for (int r = 0; r < kBatchMax; ++r)
+          if (r < R) {
+            ae[r] = ae[r] + f * xe[r];
+            ah[r] = ah[r] + f * xe[R + r];
+          }
+      }
+    }
+  }
+  if (s < S) {
+    T *Cb = C + (size_t)b * 8 * S * R;
+#pragma unroll
+    for (int r = 0; r < kBatchMax; ++r)
+      if (r < R) {
+        Cb[((size_t)(comp * 4 + 0 + k) * S + s) * R + r] = ae[r];
+        Cb[((size_t)(comp * 4 + 2 + k) * S + s) * R + r] = ah[r];
+      }
+  }
+}
+
+// translation_SS over u = s*R + r
+template <class T>
+__global__ void kb_ss(int S, int R, const int *child, const T *Cc, const T *up, T *P) {
+  const int SR = S * R;
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  if (u >= SR) return;
+  const int s = u / R;
+  T acc[8];
+  for (int it = 0; it < 8; ++it) acc[it] = T{0, 0};
+  for (int o = 0; o < 8; ++o) {
+    const int c = child[8 * b + o];
+    if (c < 0) continue;
+    const T u0 = up[((size_t)o * 2 + 0) * S + s], u1 = up[((size_t)o * 2 + 1) * S + s];
+    const T *src = Cc + (size_t)c * 8 * SR;
+#pragma unroll
+    for (int it = 0; it < 8; ++it) acc[it] = acc[it] + src[(size_t)it * SR + u] * ((it & 1) ? u1 : u0);
+  }
+  T *dst = P + (size_t)b * 8 * SR;
+  for (int it = 0; it < 8; ++it) dst[(size_t)it * SR + u] = acc[it];
+}
+
+// translation_SR over u
+template <class T>
+__global__ void kb_m2l(int S, int R, const int *i4_off, const int *i4_box, const int *i4_slot,
+                       const T *B, const T *tlt, T *Dt) {
+  const int SR = S * R;
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  const int a = blockIdx.y;
+  if (u >= SR) return;
+  const int s = u / R;
+  T acc[8];
+  for (int it = 0; it < 8; ++it) acc[it] = T{0, 0};
+  const int q0 = i4_off[a], q1 = i4_off[a + 1];
+  for (int q = q0; q < q1; ++q) {
+    const int b = i4_box[q], slot = i4_slot[q];
+    const T t0 = tlt[((size_t)slot * 2 + 0) * S + s], t1 = tlt[((size_t)slot * 2 + 1) * S + s];
+    const T *src = B + (size_t)b * 8 * SR;
+#pragma unroll
+    for (int it = 0; it < 8; ++it) acc[it] = acc[it] + src[(size_t)it * SR + u] * ((it & 1) ? t1 : t0);
+  }
+  T *dst = Dt + (size_t)a * 8 * SR;
+  for (int it = 0; it < 8; ++it) dst[(size_t)it * SR + u] = acc[it];
+}
+
+// translation_RR over u (in place on the child level's D)
+template <class T>
+__global__ void kb_rr(int S, int R, const int *parent, const int *octant, const T *Dp,
+                      const T *down, T *Dc) {
+  const int SR = S * R;
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  if (u >= SR) return;
+  const int s = u / R;
+  const int p = parent[b], o = octant[b];
+  const T d0 = down[((size_t)o * 2 + 0) * S + s], d1 = down[((size_t)o * 2 + 1) * S + s];
+  const T *src = Dp + (size_t)p * 8 * SR;
+  T *dst = Dc + (size_t)b * 8 * SR;
+#pragma unroll
+  for (int it = 0; it < 8; ++it)
+    dst[(size_t)it * SR + u] = dst[(size_t)it * SR + u] + src[(size_t)it * SR + u] * ((it & 1) ? d1 : d0);
+}
+
+// the folded coefficients of the disaggregation over u
+template <class T>
+__global__ void kb_fold(int S, int R, const T *D, T kd1, T kd2, T my1, T my2, T ep1, T ep2, T *W) {
+  const int SR = S * R;
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  if (u >= SR) return;
+  const T *Db = D + (size_t)b * 8 * SR;
+  T *Wb = W + (size_t)b * 8 * SR;
+  for (int k = 0; k < 2; ++k) {
+    const T kd = k ? kd2 : kd1, my = k ? my2 : my1, ep = k ? ep2 : ep1;
+    const T Dae = Db[(size_t)(0 + k) * SR + u], Dah = Db[(size_t)(2 + k) * SR + u];
+    const T Dbe = Db[(size_t)(4 + k) * SR + u], Dbh = Db[(size_t)(6 + k) * SR + u];
+    Wb[(size_t)(k * 4 + 0) * SR + u] = Dbh * kd + Dae * my;
+    Wb[(size_t)(k * 4 + 1) * SR + u] = -(Dah * kd) + Dbe * my;
+    Wb[(size_t)(k * 4 + 2) * SR + u] = -(Dbe * kd) + Dah * ep;
+    Wb[(size_t)(k * 4 + 3) * SR + u] = Dae * kd + Dbh * ep;
+  }
+}
+
+// Disaggregation as a small GEMM per box: workgroup = (leaf box, tile of 16
+// consecutive edges); thread = (edge of the tile, right-hand side).  Samples
+// are staged in LDS in chunks of kBatchSC: the tile's pattern rows and the
+// box's folded W for all R.  yf[r][m] = sc1 * sum_{k,s} (fa W1a + fb W1b),
+// yf[r][m + N] = sc2 * sum (fa W2a + fb W2b).
+constexpr int kBatchSC = 32;
+template <class T>
+__global__ void __launch_bounds__(256) kb_disagg(int N, int S, int R, const int *tile_box,
+                                                 const int *tile_p0, const int *box_first,
+                                                 const int *box_count, const int *perm,
+                                                 const T *rt, const T *W, T sc1, T sc2, T *yf) {
+  __shared__ T wl[8 * kBatchSC * kBatchMax];
+  __shared__ T fl[16 * 4 * kBatchSC];
+  const int b = tile_box[blockIdx.x], p0 = tile_p0[blockIdx.x];
+  const int cnt = min(16, box_first[b] + box_count[b] - p0);
+  const int el = threadIdx.x >> 4, r = threadIdx.x & 15;
+  const int SR = S * R;
+  const T *Wb = W + (size_t)b * 8 * SR;
+  T se = {0, 0}, sh = {0, 0};
+  for (int s0 = 0; s0 < S; s0 += kBatchSC) {
+    const int ns = min(kBatchSC, S - s0);
+    __syncthreads();
+    // W[j][s0 + i][r] -> wl[(j * kBatchSC + i) * kBatchMax + r]
+    for (int t = threadIdx.x; t < 8 * ns * R; t += 256) {
+      const int j = t / (ns * R), rem = t - j * ns * R, i = rem / R, rr = rem - i * R;
+      wl[(j * kBatchSC + i) * kBatchMax + rr] = Wb[(size_t)j * SR + (size_t)(s0 + i) * R + rr];
+    }
+    // RT[p0 + e][row][s0 + i] -> fl[(e * 4 + row) * kBatchSC + i]
+    for (int t = threadIdx.x; t < cnt * 4 * ns; t += 256) {
+      const int e = t / (4 * ns), rem = t - e * 4 * ns, row = rem / ns, i = rem - row * ns;
+      fl[(e * 4 + row) * kBatchSC + i] = rt[((size_t)(p0 + e) * 4 + row) * S + s0 + i];
+    }
+    __syncthreads();
+    if (el < cnt && r < R) {
+      for (int k = 0; k < 2; ++k) {
+        const T *fa = fl + (el * 4 + k * 2 + 0) * kBatchSC, *fb = fl + (el * 4 + k * 2 + 1) * kBatchSC;
+        const T *w1a = wl + (size_t)((k * 4 + 0) * kBatchSC) * kBatchMax + r;
+        const T *w1b = wl + (size_t)((k * 4 + 1) * kBatchSC) * kBatchMax + r;
+        const T *w2a = wl + (size_t)((k * 4 + 2) * kBatchSC) * kBatchMax + r;
+        const T *w2b = wl + (size_t)((k * 4 + 3) * kBatchSC) * kBatchMax + r;
+        for (int i = 0; i < ns; ++i) {
+          const T a = fa[i], bb = fb[i];
+          se = se + a * w1a[i * kBatchMax] + bb * w1b[i * kBatchMax];
+          sh = sh + a * w2a[i * kBatchMax] + bb * w2b[i * kBatchMax];
+        }
+      }
+    }
+  }
+  if (el < cnt && r < R) {
+    const int m = perm[p0 + el];
+    yf[(size_t)r * 2 * N + m] = se * sc1;
+    yf[(size_t)r * 2 * N + m + N] = sh * sc2;
+  }
+}
+
+// Near field for R vectors: workgroup = (leaf box, 16 rows); one wave per
+// row, its lanes = 4 column groups x 16 right-hand sides; a 2x2 block is
+// loaded once for the 16 lanes of its group; x of the box's E2 columns for
+// all R staged in LDS in chunks of kBatchKC columns.  ys[r][m] = Z x_r (+ yf).
+constexpr int kBatchKC = 128;
+template <class T>
+__global__ void __launch_bounds__(256) kb_near(int N, int R, const int *row_box, const int *row_r0,
+                                               const int *box_first, const int *box_count,
+                                               const int *e2_off, const int *e2_list,
+                                               const int64_t *near_off, const int *perm,
+                                               const T *blocks, const T *const *xs, const T *yf,
+                                               T *const *ys) {
+  __shared__ T xsl[kBatchKC * 2 * kBatchMax];
+  const int b = row_box[blockIdx.x], r0 = row_r0[blockIdx.x];
+  const int rows = min(kSpmvRowsPerWG, box_count[b] - r0);
+  const int c0 = e2_off[b], cols = e2_off[b + 1] - c0;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int rl = lane & 15, cg = lane >> 4;
+  T ae[kSpmvRowsPerWG / 4], ah[kSpmvRowsPerWG / 4];
+  for (int q = 0; q < kSpmvRowsPerWG / 4; ++q) ae[q] = ah[q] = T{0, 0};
+  for (int cc = 0; cc < cols; cc += kBatchKC) {
+    const int nc = min(kBatchKC, cols - cc);
+    __syncthreads();
+    for (int t = threadIdx.x; t < nc * 2 * R; t += 256) {
+      const int c = t / (2 * R), rem = t - c * 2 * R, cur = rem / R, rr = rem - cur * R;
+      const int n = e2_list[c0 + cc + c];
+      xsl[(c * 2 + cur) * kBatchMax + rr] = xs[rr][n + cur * N];
+    }
+    __syncthreads();
+    if (rl < R) {
+      for (int q = 0; q < kSpmvRowsPerWG / 4; ++q) {
+        const int row = r0 + wv + 4 * q;
+        if (row >= r0 + rows) break;
+        const T *Z = blocks + 4 * (near_off[b] + (int64_t)row * cols + cc);
+        for (int c = cg; c < nc; c += 4) {
+          const T z0 = Z[4 * c], z1 = Z[4 * c + 1], z2 = Z[4 * c + 2], z3 = Z[4 * c + 3];
+          const T xe = xsl[(c * 2 + 0) * kBatchMax + rl], xh = xsl[(c * 2 + 1) * kBatchMax + rl];
+          ae[q] = ae[q] + z0 * xe + z1 * xh;
+          ah[q] = ah[q] + z2 * xe + z3 * xh;
+        }
+      }
+    }
+  }
+  for (int q = 0; q < kSpmvRowsPerWG / 4; ++q) {
+    // sum the four column groups: lanes rl, rl+16, rl+32, rl+48
+    T se = ae[q], sh = ah[q];
+    se = se + bshfl_down(se, 32);
+    sh = sh + bshfl_down(sh, 32);
+    se = se + bshfl_down(se, 16);
+    sh = sh + bshfl_down(sh, 16);
+    const int row = r0 + wv + 4 * q;
+    if (cg == 0 && rl < R && row < r0 + rows) {
+      const int m = perm[box_first[b] + row];
+      if (yf) {
+        se = se + yf[(size_t)rl * 2 * N + m];
+        sh = sh + yf[(size_t)rl * 2 * N + m + N];
+      }
+      ys[rl][m] = se;
+      ys[rl][m + N] = sh;
+    }
+  }
+}
+
+template <class T>
+void launch_batch_agg(int nbl, int S, int N, int R, const int *box_first, const int *box_count,
+                      const int *perm, const T *rt, const T *const *xs, T *C, hipStream_t st) {
+  if (nbl <= 0) return;
+  hipLaunchKernelGGL(kb_agg<T>, dim3(bnblk(S, 256), nbl, 4), dim3(256), 0, st, S, N, R, box_first,
+                     box_count, perm, rt, xs, C);
+}
+template <class T>
+void launch_batch_ss(int nb, int S, int R, const int *child, const T *Cc, const T *up, T *P,
+                     hipStream_t st) {
+  if (nb <= 0) return;
+  hipLaunchKernelGGL(kb_ss<T>, dim3(bnblk((int64_t)S * R, 256), nb), dim3(256), 0, st, S, R, child,
+                     Cc, up, P);
+}
+template <class T>
+void launch_batch_m2l(int nb, int S, int R, const int *i4_off, const int *i4_box,
+                      const int *i4_slot, const T *B, const T *tlt, T *Dt, hipStream_t st) {
+  if (nb <= 0) return;
+  hipLaunchKernelGGL(kb_m2l<T>, dim3(bnblk((int64_t)S * R, 128), nb), dim3(128), 0, st, S, R,
+                     i4_off, i4_box, i4_slot, B, tlt, Dt);
+}
+template <class T>
+void launch_batch_rr(int nb, int S, int R, const int *parent, const int *octant, const T *Dp,
+                     const T *down, T *Dc, hipStream_t st) {
+  if (nb <= 0) return;
+  hipLaunchKernelGGL(kb_rr<T>, dim3(bnblk((int64_t)S * R, 256), nb), dim3(256), 0, st, S, R,
+                     parent, octant, Dp, down, Dc);
+}
+template <class T>
+void launch_batch_fold(int nbl, int S, int R, const T *D, zd kd1, zd kd2, zd my1, zd my2, zd ep1,
+                       zd ep2, T *W, hipStream_t st) {
+  if (nbl <= 0) return;
+  hipLaunchKernelGGL(kb_fold<T>, dim3(bnblk((int64_t)S * R, 256), nbl), dim3(256), 0, st, S, R, D,
+                     cvt<T>(kd1), cvt<T>(kd2), cvt<T>(my1), cvt<T>(my2), cvt<T>(ep1), cvt<T>(ep2),
+                     W);
+}
+template <class T>
+void launch_batch_disagg(int ntiles, int N, int S, int R, const int *tile_box, const int *tile_p0,
+                         const int *box_first, const int *box_count, const int *perm, const T *rt,
+                         const T *W, zd sc1, zd sc2, T *yf, hipStream_t st) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL(kb_disagg<T>, dim3(ntiles), dim3(256), 0, st, N, S, R, tile_box, tile_p0,
+                     box_first, box_count, perm, rt, W, cvt<T>(sc1), cvt<T>(sc2), yf);
+}
+template <class T>
+void launch_batch_near(int nwg, int N, int R, const int *row_box, const int *row_r0,
+                       const int *box_first, const int *box_count, const int *e2_off,
+                       const int *e2_list, const int64_t *near_off, const int *perm,
+                       const T *blocks, const T *const *xs, const T *yf, T *const *ys,
+                       hipStream_t st) {
+  if (nwg <= 0) return;
+  hipLaunchKernelGGL(kb_near<T>, dim3(nwg), dim3(256), 0, st, N, R, row_box, row_r0, box_first,
+                     box_count, e2_off, e2_list, near_off, perm, blocks, xs, yf, ys);
+}
+
+#define SPK_BATCH_INSTANTIATE(T)                                                                  \
+  template void launch_batch_agg<T>(int, int, int, int, const int *, const int *, const int *,     \
+                                    const T *, const T *const *, T *, hipStream_t);                \
+  template void launch_batch_ss<T>(int, int, int, const int *, const T *, const T *, T *,          \
+                                   hipStream_t);                                                   \
+  template void launch_batch_m2l<T>(int, int, int, const int *, const int *, const int *,          \
+                                    const T *, const T *, T *, hipStream_t);                       \
+  template void launch_batch_rr<T>(int, int, int, const int *, const int *, const T *, const T *,  \
+                                   T *, hipStream_t);                                              \
+  template void launch_batch_fold<T>(int, int, int, const T *, zd, zd, zd, zd, zd, zd, T *,        \
+                                     hipStream_t);                                                 \
+  template void launch_batch_disagg<T>(int, int, int, int, const int *, const int *, const int *,  \
+                                       const int *, const int *, const T *, const T *, zd, zd,     \
+                                       T *, hipStream_t);                                          \
+  template void launch_batch_near<T>(int, int, int, const int *, const int *, const int *,         \
+                                     const int *, const int *, const int *, const int64_t *,       \
+                                     const int *, const T *, const T *const *, const T *,          \
+                                     T *const *, hipStream_t);
+SPK_BATCH_INSTANTIATE(zd)
+SPK_BATCH_INSTANTIATE(zf)
+
+}  // namespace spk

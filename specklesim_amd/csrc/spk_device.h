@@ -130,6 +130,8 @@ void launch_tl_level(int S, int K, const double *khat, const double *trf, const 
 constexpr int kSpmvRowsPerWG = 16;
 // edges per disaggregation task (one wavefront, one leaf box)
 constexpr int kDisaggEdges = 4;
+// right-hand sides per batched matvec launch
+constexpr int kBatchMax = 16;
 // arbitrary (m, n) pairs, one thread per pair (parity probe)
 void launch_near_pairs(const NearArgs &a, int npairs, const int *m, const int *n, zd *out,
                        hipStream_t st);
@@ -198,6 +200,35 @@ template <class T>
 void launch_gemv_n(int64_t n, int k, const T *V, int64_t ldv, const zd *h, zd *out, zd *partial,
                    hipStream_t st);
 size_t gemv_n_partial_elems(int64_t n, int kmax);
+// the matvec for R <= kBatchMax right-hand sides (spk_batch.hip); xs, ys:
+// device arrays of R vector pointers; k-space arrays [box][it][s][r]
+template <class T>
+void launch_batch_agg(int nbl, int S, int N, int R, const int *box_first, const int *box_count,
+                      const int *perm, const T *rt, const T *const *xs, T *C, hipStream_t st);
+template <class T>
+void launch_batch_ss(int nb, int S, int R, const int *child, const T *Cc, const T *up, T *P,
+                     hipStream_t st);
+template <class T>
+void launch_batch_m2l(int nb, int S, int R, const int *i4_off, const int *i4_box,
+                      const int *i4_slot, const T *B, const T *tlt, T *Dt, hipStream_t st);
+template <class T>
+void launch_batch_rr(int nb, int S, int R, const int *parent, const int *octant, const T *Dp,
+                     const T *down, T *Dc, hipStream_t st);
+template <class T>
+void launch_batch_fold(int nbl, int S, int R, const T *D, zd kd1, zd kd2, zd my1, zd my2, zd ep1,
+                       zd ep2, T *W, hipStream_t st);
+// tiles of up to 16 consecutive edges of one leaf box
+template <class T>
+void launch_batch_disagg(int ntiles, int N, int S, int R, const int *tile_box, const int *tile_p0,
+                         const int *box_first, const int *box_count, const int *perm, const T *rt,
+                         const T *W, zd sc1, zd sc2, T *yf, hipStream_t st);
+template <class T>
+void launch_batch_near(int nwg, int N, int R, const int *row_box, const int *row_r0,
+                       const int *box_first, const int *box_count, const int *e2_off,
+                       const int *e2_list, const int64_t *near_off, const int *perm,
+                       const T *blocks, const T *const *xs, const T *yf, T *const *ys,
+                       hipStream_t st);
+
 // streaming read of a buffer (bandwidth probe)
 void launch_stream_read(const void *a, size_t bytes, double *out, hipStream_t st);
 // one-reduce MGS: row k1 of T = (I + L)^-1 from tg = T^H g

@@ -122,6 +122,34 @@ class SieTriMlfmm:
         """Device-pointer matvec (e.g. torch complex128 tensors' data_ptr())."""
         nat.check(nat.lib().spk_matvec_device(self._h, C.c_void_p(int(d_x)), C.c_void_p(int(d_y))))
 
+    def calculate_vector_b_tri_batch_device(self, d_xs, d_ys, c64=False):
+        """Batched device matvec: d_xs, d_ys are sequences of R <= 16 device
+        pointers (2N complex128 vectors, or complex64 with c64=True on a
+        precision="mixed" handle)."""
+        R = len(d_xs)
+        if len(d_ys) != R:
+            raise ValueError("d_xs and d_ys must have the same length")
+        xs = (C.c_void_p * R)(*[int(p) for p in d_xs])
+        ys = (C.c_void_p * R)(*[int(p) for p in d_ys])
+        f = nat.lib().spk_matvec_batch_c64_device if c64 else nat.lib().spk_matvec_batch_device
+        nat.check(f(self._h, R, xs, ys))
+
+    def calculate_vector_b_tri_batch(self, X, c64=False):
+        """Host convenience: X is (R, 2N); returns Z X row by row through the
+        batched device matvec."""
+        from ._native import DeviceBuffer
+
+        dt = np.complex64 if c64 else np.complex128
+        X = np.ascontiguousarray(X, dtype=dt)
+        R = X.shape[0]
+        nb = 2 * self.N * X.itemsize
+        xb = [DeviceBuffer(nb) for _ in range(R)]
+        yb = [DeviceBuffer(nb) for _ in range(R)]
+        for r in range(R):
+            xb[r].upload(X[r])
+        self.calculate_vector_b_tri_batch_device([b.ptr for b in xb], [b.ptr for b in yb], c64)
+        return np.stack([yb[r].download(dt) for r in range(R)])
+
     # -- get_vector_b_tri / get_init_vector_x_tri --------------------------
     def get_vector_b_tri(self, k_in, e_in):
         k = np.ascontiguousarray(k_in, dtype=np.float64)
@@ -198,6 +226,26 @@ class SieTriMlfmm:
         return dict(info=info.info, iters=info.iterations, bea=info.arnoldi_backward_error,
                     be=info.backward_error, tbe=info.true_backward_error, matvecs=info.matvecs,
                     seconds=info.seconds)
+
+    def gmres_run_batch_device(self, d_bs, d_xs, max_iterations=650, tolerance=1e-3,
+                               use_initial_guess=True, refinement_steps=0,
+                               refinement_tolerance=0.0):
+        """Mixed-precision solves of R <= 16 right-hand sides (device
+        pointers to 2N complex128 vectors) on a precision="mixed" handle,
+        their inner complex64 steps sharing one batched matvec.  Returns one
+        info dict per right-hand side."""
+        R = len(d_bs)
+        if len(d_xs) != R:
+            raise ValueError("d_bs and d_xs must have the same length")
+        gp = nat.GmresParams(max_iterations, 0, tolerance, ORTHOGONALIZATION["ICGS"],
+                             int(use_initial_guess), refinement_steps, refinement_tolerance, 0)
+        infos = (nat.GmresInfo * R)()
+        bs = (C.c_void_p * R)(*[int(p) for p in d_bs])
+        xs = (C.c_void_p * R)(*[int(p) for p in d_xs])
+        nat.check(nat.lib().spk_gmres_batch_device(self._h, R, bs, xs, C.byref(gp), infos))
+        return [dict(info=i.info, iters=i.iterations, bea=i.arnoldi_backward_error,
+                     be=i.backward_error, tbe=i.true_backward_error, matvecs=i.matvecs,
+                     seconds=i.seconds) for i in infos]
 
     def gmres_history(self):
         n = C.c_int()
