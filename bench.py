@@ -98,6 +98,50 @@ def timed_path(args, lam, pp, tt, precision, orth=None):
                 solves_per_s=args.steps / elapsed, ms_per_step=elapsed / args.steps * 1e3)
 
 
+def batch_angles(R):
+    """R polar incidence angles uniform in 0-60 degrees (the ensemble's set),
+    phi = 0, TM polarisation, propagating towards -z."""
+    return [np.radians(60.0 * r / max(R - 1, 1)) for r in range(R)]
+
+
+def one_batch(args, lam, pp, tt, R):
+    """Precalculation, R right-hand sides on the device and one batched
+    mixed-precision solve of all of them."""
+    from specklesim_amd import DeviceBuffer, SieTriMlfmm, plane_wave
+
+    t0 = time.perf_counter()
+    s = SieTriMlfmm(lam, args.eps_r2, formulation=args.formulation, normal_convention=2,
+                    tree_s_opt=250, truncation_number=19, precision="mixed")
+    s.precalculation_tri(pp, tt)
+    t1 = time.perf_counter()
+    nb = 2 * s.N * 16
+    bs = [DeviceBuffer(nb) for _ in range(R)]
+    xs = [DeviceBuffer(nb) for _ in range(R)]
+    for b, th in zip(bs, batch_angles(R)):
+        k, e = plane_wave(th, 0.0, 2, "TM")
+        s.get_vector_b_tri_device(k, e, b.ptr)
+    infos = s.gmres_run_batch_device([b.ptr for b in bs], [x.ptr for x in xs],
+                                     max_iterations=2 * args.maxit, tolerance=args.tol,
+                                     use_initial_guess=False)
+    t2 = time.perf_counter()
+    return s, xs, dict(setup_s=t1 - t0, gmres_s=t2 - t1, infos=infos)
+
+
+def timed_batch(args, lam, pp, tt, R):
+    for _ in range(args.warmup):
+        one_batch(args, lam, pp, tt, R)
+    _sync()
+    t0 = time.perf_counter()
+    runs = []
+    for _ in range(args.steps):
+        s, xs, r = one_batch(args, lam, pp, tt, R)
+        runs.append(r)
+    _sync()
+    elapsed = time.perf_counter() - t0
+    return dict(solver=s, xs=xs, runs=runs, elapsed=elapsed,
+                solves_per_s=R * args.steps / elapsed, ms_per_step=elapsed / args.steps * 1e3)
+
+
 def brdf_curve(res, lam):
     """BRDF of a solution in the plane of incidence, 181 angles in
     (-80, 80) degrees on a sphere of 1000 wavelengths."""
@@ -159,6 +203,8 @@ def main():
     ap.add_argument("--maxit", type=int, default=650)
     ap.add_argument("--orth", default="IMGS")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch", type=int, default=16,
+                    help="incidence angles of the batched mixed path (0: skip it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1",
@@ -172,6 +218,9 @@ def main():
     if args.precision == "both" and args.orth != "ICGS":
         # the same FP64 solve with the reference's ICGS option (icntl(5) = 3)
         paths["fp64_icgs"] = timed_path(args, lam, pp, tt, "fp64", "ICGS")
+    batch = None
+    if args.batch and args.precision == "both":
+        batch = timed_batch(args, lam, pp, tt, args.batch)
     ff_err = None
     if "fp64" in paths and "mixed" in paths:
         b64, b32 = brdf_curve(paths["fp64"], lam), brdf_curve(paths["mixed"], lam)
@@ -247,6 +296,45 @@ def main():
     if pc is not None:
         rec["matvec_c64_ms"] = pc["matvec"][0]
         rec["stages_c64_ms"] = {k: round(v[0], 4) for k, v in pc.items()}
+    if batch is not None:
+        from specklesim_amd import farfield
+
+        # far field of the batch's normal-incidence solution against the FP64
+        # path's (the same right-hand side), and of its 60-degree solution
+        # against an FP64 solve of that right-hand side (not timed)
+        bres = dict(solver=batch["solver"], x=batch["xs"][0])
+        ff_b0 = float(np.linalg.norm(brdf_curve(bres, lam) - b64) / np.linalg.norm(b64))
+        from specklesim_amd import DeviceBuffer, SieTriMlfmm, plane_wave
+
+        th = batch_angles(args.batch)[-1]
+        s64 = paths["fp64"]["solver"]
+        k, e = plane_wave(th, 0.0, 2, "TM")
+        db = DeviceBuffer(2 * s64.N * 16)
+        dx = DeviceBuffer(2 * s64.N * 16)
+        s64.get_vector_b_tri_device(k, e, db.ptr)
+        s64.gmres_run_device(db.ptr, dx.ptr, tolerance=args.tol, max_iterations=args.maxit,
+                             use_initial_guess=False)
+        pts = (-np.radians(80), np.radians(80))
+        _, v64 = farfield.brdf(s64, dx.ptr, th, pts, 181, 0.0, 1e3 * lam, device=True)
+        _, vb = farfield.brdf(batch["solver"], batch["xs"][-1].ptr, th, pts, 181, 0.0, 1e3 * lam,
+                              device=True)
+        ff_b1 = float(np.linalg.norm(vb - v64) / np.linalg.norm(v64))
+        infos = batch["runs"][-1]["infos"]
+        rec["batched"] = {
+            "workload": f"the same mesh, {args.batch} incidence angles uniform in 0-60 deg "
+                        f"(TM, phi = 0), solved together: complex64 inner GMRES sharing one "
+                        f"batched matvec per step, FP64 refinement residuals",
+            "solves_per_s": batch["solves_per_s"], "ms_per_batch": batch["ms_per_step"],
+            "right_hand_sides": args.batch,
+            "setup_ms": batch["runs"][-1]["setup_s"] * 1e3,
+            "gmres_ms": batch["runs"][-1]["gmres_s"] * 1e3,
+            "iterations": [min(i["iters"] for i in infos), max(i["iters"] for i in infos)],
+            "backward_error_max": max(i["be"] for i in infos),
+            "all_converged": all(i["info"] == 0 for i in infos),
+            "farfield_rel_err_vs_fp64": {"0deg": ff_b0, f"{np.degrees(th):.0f}deg": ff_b1},
+            "vs_fp64_single": batch["solves_per_s"] / paths["fp64"]["solves_per_s"],
+            "vs_mixed_single": batch["solves_per_s"] / paths["mixed"]["solves_per_s"],
+        }
     if not args.no_cpu_baseline:
         it64 = (paths["fp64"] if "fp64" in paths else H)["runs"][-1]["iters"]
         cb = cpu_baseline(args, lam, pp, tt, it64)

@@ -73,7 +73,7 @@ template <class T>
 struct BatchBufs {
   int R = 0;
   std::vector<DBuf<T>> B, D;  // per level, [box][it][s][r]
-  DBuf<T> W, YF;              // folded leaf coefficients; far part [r][2N]
+  DBuf<T> W, YF, XI;          // folded leaf coefficients; far part [r][2N]; interleaved x
   DBuf<const T *> xs;
   DBuf<T *> ys;
 };
@@ -426,6 +426,7 @@ static int batch_ensure(spk_handle *h, int R) {
   }
   HIPCHK(BB.W.alloc((size_t)h->lev[t.l_max].nb * 8 * S * R));
   HIPCHK(BB.YF.alloc((size_t)R * 2 * h->N));
+  HIPCHK(BB.XI.alloc((size_t)kBatchMax * 2 * h->N));
   HIPCHK(BB.xs.alloc(kBatchMax));
   HIPCHK(BB.ys.alloc(kBatchMax));
   BB.R = R;
@@ -444,8 +445,9 @@ int matvec_batch_t(spk_handle *h, int R, const T *const *xs, T *const *ys) {
   const int S = h->tb.S, N = h->N, lmax = t.l_max, lmin = t.l_min;
   hipStream_t st = h->st;
   LevelDev &LL = h->lev[lmax];
+  launch_batch_interleave(N, R, xs, BB.XI.p, st);
   launch_batch_agg(LL.nb, S, N, R, h->d_box_first.p, h->d_box_count.p, h->d_perm.p, pRt(h, tag),
-                   xs, BB.B[lmax].p, st);
+                   (const T *)BB.XI.p, BB.B[lmax].p, st);
   for (int l = lmax - 1; l >= lmin; --l)
     launch_batch_ss(h->lev[l].nb, S, R, h->lev[l].child.p, (const T *)BB.B[l + 1].p,
                     pUp(h->lev[l + 1], tag), BB.B[l].p, st);
@@ -465,7 +467,7 @@ int matvec_batch_t(spk_handle *h, int R, const T *const *xs, T *const *ys) {
                       h->sc2, BB.YF.p, st);
   launch_batch_near(h->spmv_wgs, N, R, h->d_row_box.p, h->d_row_r0.p, h->d_box_first.p,
                     h->d_box_count.p, h->d_e2_off.p, h->d_e2_list.p, h->d_near_off.p, h->d_perm.p,
-                    pBlk(h, tag), xs, (const T *)BB.YF.p, ys, st);
+                    pBlk(h, tag), (const T *)BB.XI.p, (const T *)BB.YF.p, ys, st);
   HIPCHK(hipGetLastError());
   return SPK_OK;
 }
@@ -616,7 +618,7 @@ int spk_setup_tri(spk_handle *h, int np, const double *pp, int nt, const int *tt
   {
     std::vector<int> tb, tp;
     for (size_t b = 0; b < t.box_count.size(); ++b)
-      for (int r = 0; r < t.box_count[b]; r += 16) {
+      for (int r = 0; r < t.box_count[b]; r += kBatchTile) {
         tb.push_back((int)b);
         tp.push_back(t.box_first[b] + r);
       }

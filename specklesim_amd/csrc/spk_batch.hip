@@ -27,47 +27,58 @@ __device__ inline T bshfl_down(T v, int off) {
 
 }  // namespace
 
+// The R input vectors interleaved: XI[(m * 2 + cur) * RP + r] = x_r[m + cur N]
+// (zero for r >= R), so that a column's 2 RP values are one contiguous run.
+template <class T, int RP>
+__global__ void kb_interleave(int N, int R, const T *const *xs, T *XI) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)2 * N * RP) return;
+  const int r = (int)(t % RP);
+  const int64_t mc = t / RP;
+  const int m = (int)(mc >> 1), cur = (int)(mc & 1);
+  XI[t] = r < R ? xs[r][m + (int64_t)cur * N] : T{0, 0};
+}
+
 // Aggregation: C[b][it][s][r] = sum_{edges of b} conj(RT[p][q][s]) x_r(cur),
 // it = comp*4 + cur*2 + k.  Workgroup = (256 samples, leaf box, q); the
 // box's x values are staged in LDS 64 edges at a time and read as
-// broadcasts; each thread keeps 2R accumulators.
-template <class T>
+// broadcasts; each thread keeps 2R accumulators.  RP = R rounded up to a
+// power of two (lanes beyond R are zero and not written).
+template <class T, int RP>
 __global__ void __launch_bounds__(256) kb_agg(int S, int N, int R, const int *box_first,
                                               const int *box_count, const int *perm, const T *rt,
-                                              const T *const *xs, T *C) {
-  __shared__ T xl[64 * 2 * kBatchMax];
+                                              const T *XI, T *C) {
+  __shared__ T xl[64 * 2 * RP];
   const int s = blockIdx.x * 256 + threadIdx.x;
   const int b = blockIdx.y, q = blockIdx.z, k = q >> 1, comp = q & 1;
   const int p0 = box_first[b], cnt = box_count[b];
-  T ae[kBatchMax], ah[kBatchMax];
+  T ae[RP], ah[RP];
 #pragma unroll
-  for (int r = 0; r < kBatchMax; ++r) ae[r] = ah[r] = T{0, 0};
+  for (int r = 0; r < RP; ++r) ae[r] = ah[r] = T{0, 0};
   for (int e0 = 0; e0 < cnt; e0 += 64) {
     const int ne = min(64, cnt - e0);
     __syncthreads();
-    for (int i = threadIdx.x; i < ne * 2 * R; i += 256) {
-      const int e = i / (2 * R), rem = i - e * 2 * R, cur = rem / R, r = rem - cur * R;
-      const int m = perm[p0 + e0 + e];
-      xl[i] = xs[r][m + cur * N];
+    for (int i = threadIdx.x; i < ne * 2 * RP; i += 256) {
+      const int e = i / (2 * RP), w = i % (2 * RP);
+      xl[i] = XI[(size_t)perm[p0 + e0 + e] * 2 * RP + w];
     }
     __syncthreads();
     if (s < S) {
       for (int e = 0; e < ne; ++e) {
         const T f = conj(rt[((size_t)(p0 + e0 + e) * 4 + q) * S + s]);
-        const T *xe = xl + e * 2 * R;
+        const T *xe = xl + e * 2 * RP;
 #pragma unroll
-        for (int r = 0; r < kBatchMax; ++r)
-          if (r < R) {
-            ae[r] = ae[r] + f * xe[r];
-            ah[r] = ah[r] + f * xe[R + r];
-          }
+        for (int r = 0; r < RP; ++r) {
+          ae[r] = ae[r] + f * xe[r];
+          ah[r] = ah[r] + f * xe[RP + r];
+        }
       }
     }
   }
   if (s < S) {
     T *Cb = C + (size_t)b * 8 * S * R;
 #pragma unroll
-    for (int r = 0; r < kBatchMax; ++r)
+    for (int r = 0; r < RP; ++r)
       if (r < R) {
         Cb[((size_t)(comp * 4 + 0 + k) * S + s) * R + r] = ae[r];
         Cb[((size_t)(comp * 4 + 2 + k) * S + s) * R + r] = ah[r];
@@ -97,13 +108,19 @@ __global__ void kb_ss(int S, int R, const int *child, const T *Cc, const T *up, 
   for (int it = 0; it < 8; ++it) dst[(size_t)it * SR + u] = acc[it];
 }
 
-// translation_SR over u
+// translation_SR over u.  XCD-aware order: the 8 XCDs take workgroups
+// round robin, so workgroup L is given the logical tile (L % 8) * per + L / 8;
+// each XCD then sweeps a contiguous run of target boxes for one sample
+// chunk, whose I4 sources overlap and stay in that XCD's L2.
 template <class T>
-__global__ void kb_m2l(int S, int R, const int *i4_off, const int *i4_box, const int *i4_slot,
-                       const T *B, const T *tlt, T *Dt) {
+__global__ void kb_m2l(int S, int R, int nb, int gx, const int *i4_off, const int *i4_box,
+                       const int *i4_slot, const T *B, const T *tlt, T *Dt) {
   const int SR = S * R;
-  const int u = blockIdx.x * blockDim.x + threadIdx.x;
-  const int a = blockIdx.y;
+  const int per = gridDim.x / 8;
+  const int lin = (blockIdx.x % 8) * per + blockIdx.x / 8;
+  if (lin >= gx * nb) return;
+  const int u = (lin / nb) * blockDim.x + threadIdx.x;
+  const int a = lin % nb;
   if (u >= SR) return;
   const int s = u / R;
   T acc[8];
@@ -158,130 +175,180 @@ __global__ void kb_fold(int S, int R, const T *D, T kd1, T kd2, T my1, T my2, T 
   }
 }
 
-// Disaggregation as a small GEMM per box: workgroup = (leaf box, tile of 16
-// consecutive edges); thread = (edge of the tile, right-hand side).  Samples
-// are staged in LDS in chunks of kBatchSC: the tile's pattern rows and the
-// box's folded W for all R.  yf[r][m] = sc1 * sum_{k,s} (fa W1a + fb W1b),
-// yf[r][m + N] = sc2 * sum (fa W2a + fb W2b).
-constexpr int kBatchSC = 32;
-template <class T>
+// Disaggregation as a small GEMM per box: workgroup = (leaf box, tile of up
+// to 64 consecutive edges), 16 RP threads; thread = (group of 4 edges of the
+// tile, right-hand side).  Samples are staged in LDS in chunks of kBatchSC:
+// the tile's pattern rows and the box's folded W for all R; a W value read
+// from LDS serves the thread's 4 edges.
+// yf[r][m] = sc1 * sum_{k,s} (fa W1a + fb W1b), yf[r][m + N] = sc2 * sum (fa W2a + fb W2b).
+constexpr int kBatchSC = 16;
+template <class T, int RP>
 __global__ void __launch_bounds__(256) kb_disagg(int N, int S, int R, const int *tile_box,
                                                  const int *tile_p0, const int *box_first,
                                                  const int *box_count, const int *perm,
                                                  const T *rt, const T *W, T sc1, T sc2, T *yf) {
-  __shared__ T wl[8 * kBatchSC * kBatchMax];
-  __shared__ T fl[16 * 4 * kBatchSC];
+  constexpr int NT = 16 * RP, EPT = 4;
+  __shared__ T wl[8 * kBatchSC * RP];
+  __shared__ T fl[kBatchTile * 4 * kBatchSC];
   const int b = tile_box[blockIdx.x], p0 = tile_p0[blockIdx.x];
-  const int cnt = min(16, box_first[b] + box_count[b] - p0);
-  const int el = threadIdx.x >> 4, r = threadIdx.x & 15;
+  const int cnt = min(kBatchTile, box_first[b] + box_count[b] - p0);
+  const int eg = threadIdx.x / RP, r = threadIdx.x % RP;  // edges eg*4 .. eg*4+3
   const int SR = S * R;
   const T *Wb = W + (size_t)b * 8 * SR;
-  T se = {0, 0}, sh = {0, 0};
+  T se[EPT], sh[EPT];
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) se[e] = sh[e] = T{0, 0};
   for (int s0 = 0; s0 < S; s0 += kBatchSC) {
     const int ns = min(kBatchSC, S - s0);
     __syncthreads();
-    // W[j][s0 + i][r] -> wl[(j * kBatchSC + i) * kBatchMax + r]
-    for (int t = threadIdx.x; t < 8 * ns * R; t += 256) {
-      const int j = t / (ns * R), rem = t - j * ns * R, i = rem / R, rr = rem - i * R;
-      wl[(j * kBatchSC + i) * kBatchMax + rr] = Wb[(size_t)j * SR + (size_t)(s0 + i) * R + rr];
+    // W[j][s0 + i][r] -> wl[(j * kBatchSC + i) * RP + r]
+    for (int t = threadIdx.x; t < 8 * kBatchSC * RP; t += NT) {
+      const int j = t / (kBatchSC * RP), i = (t / RP) % kBatchSC, rr = t % RP;
+      wl[t] = (i < ns && rr < R) ? Wb[(size_t)j * SR + (size_t)(s0 + i) * R + rr] : T{0, 0};
     }
     // RT[p0 + e][row][s0 + i] -> fl[(e * 4 + row) * kBatchSC + i]
-    for (int t = threadIdx.x; t < cnt * 4 * ns; t += 256) {
-      const int e = t / (4 * ns), rem = t - e * 4 * ns, row = rem / ns, i = rem - row * ns;
-      fl[(e * 4 + row) * kBatchSC + i] = rt[((size_t)(p0 + e) * 4 + row) * S + s0 + i];
+    for (int t = threadIdx.x; t < kBatchTile * 4 * kBatchSC; t += NT) {
+      const int e = t / (4 * kBatchSC), row = (t / kBatchSC) & 3, i = t % kBatchSC;
+      fl[t] = (e < cnt && i < ns) ? rt[((size_t)(p0 + e) * 4 + row) * S + s0 + i] : T{0, 0};
     }
     __syncthreads();
-    if (el < cnt && r < R) {
+    if (eg * EPT < cnt) {
+#pragma unroll
       for (int k = 0; k < 2; ++k) {
-        const T *fa = fl + (el * 4 + k * 2 + 0) * kBatchSC, *fb = fl + (el * 4 + k * 2 + 1) * kBatchSC;
-        const T *w1a = wl + (size_t)((k * 4 + 0) * kBatchSC) * kBatchMax + r;
-        const T *w1b = wl + (size_t)((k * 4 + 1) * kBatchSC) * kBatchMax + r;
-        const T *w2a = wl + (size_t)((k * 4 + 2) * kBatchSC) * kBatchMax + r;
-        const T *w2b = wl + (size_t)((k * 4 + 3) * kBatchSC) * kBatchMax + r;
-        for (int i = 0; i < ns; ++i) {
-          const T a = fa[i], bb = fb[i];
-          se = se + a * w1a[i * kBatchMax] + bb * w1b[i * kBatchMax];
-          sh = sh + a * w2a[i * kBatchMax] + bb * w2b[i * kBatchMax];
+        const T *w1a = wl + (k * 4 + 0) * kBatchSC * RP + r, *w1b = wl + (k * 4 + 1) * kBatchSC * RP + r;
+        const T *w2a = wl + (k * 4 + 2) * kBatchSC * RP + r, *w2b = wl + (k * 4 + 3) * kBatchSC * RP + r;
+#pragma unroll 4
+        for (int i = 0; i < kBatchSC; ++i) {
+          const T wa = w1a[i * RP], wb = w1b[i * RP], wc = w2a[i * RP], wd = w2b[i * RP];
+#pragma unroll
+          for (int e = 0; e < EPT; ++e) {
+            const T *f = fl + ((eg * EPT + e) * 4 + k * 2) * kBatchSC;
+            const T a = f[i], bb = f[kBatchSC + i];
+            se[e] = se[e] + a * wa + bb * wb;
+            sh[e] = sh[e] + a * wc + bb * wd;
+          }
         }
       }
     }
   }
-  if (el < cnt && r < R) {
-    const int m = perm[p0 + el];
-    yf[(size_t)r * 2 * N + m] = se * sc1;
-    yf[(size_t)r * 2 * N + m + N] = sh * sc2;
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    const int el = eg * EPT + e;
+    if (el < cnt && r < R) {
+      const int m = perm[p0 + el];
+      yf[(size_t)r * 2 * N + m] = se[e] * sc1;
+      yf[(size_t)r * 2 * N + m + N] = sh[e] * sc2;
+    }
   }
 }
 
-// Near field for R vectors: workgroup = (leaf box, 16 rows); one wave per
-// row, its lanes = 4 column groups x 16 right-hand sides; a 2x2 block is
-// loaded once for the 16 lanes of its group; x of the box's E2 columns for
-// all R staged in LDS in chunks of kBatchKC columns.  ys[r][m] = Z x_r (+ yf).
-constexpr int kBatchKC = 128;
-template <class T>
+// Near field for R vectors: workgroup = (leaf box, 16 rows), 4 rows per
+// wave processed together.  A wave's lanes = CL column lanes x RG groups of
+// right-hand sides, each lane holding RPL = RP / RG of them: a lane loads the
+// 2x2 blocks of its columns (lanes of the same column share one address) and
+// applies them to its RPL vectors, read from the LDS copy of the box's E2
+// columns.  Rows are reduced over the column lanes at the end.
+template <class T, int RP>
 __global__ void __launch_bounds__(256) kb_near(int N, int R, const int *row_box, const int *row_r0,
                                                const int *box_first, const int *box_count,
                                                const int *e2_off, const int *e2_list,
                                                const int64_t *near_off, const int *perm,
-                                               const T *blocks, const T *const *xs, const T *yf,
+                                               const T *blocks, const T *XI, const T *yf,
                                                T *const *ys) {
-  __shared__ T xsl[kBatchKC * 2 * kBatchMax];
+  constexpr int RG = RP >= 8 ? RP / 4 : 1, RPL = RP / RG, CL = 64 / RG;
+  constexpr int KC = 32768 / (2 * RP * (int)sizeof(T));  // columns per LDS chunk (32 KB)
+  constexpr int NQ = kSpmvRowsPerWG / 4;
+  __shared__ T xsl[KC * 2 * RP];
   const int b = row_box[blockIdx.x], r0 = row_r0[blockIdx.x];
   const int rows = min(kSpmvRowsPerWG, box_count[b] - r0);
   const int c0 = e2_off[b], cols = e2_off[b + 1] - c0;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int rl = lane & 15, cg = lane >> 4;
-  T ae[kSpmvRowsPerWG / 4], ah[kSpmvRowsPerWG / 4];
-  for (int q = 0; q < kSpmvRowsPerWG / 4; ++q) ae[q] = ah[q] = T{0, 0};
-  for (int cc = 0; cc < cols; cc += kBatchKC) {
-    const int nc = min(kBatchKC, cols - cc);
+  const int rg = lane / CL, cl = lane % CL;
+  T ae[NQ][RPL], ah[NQ][RPL];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q)
+#pragma unroll
+    for (int v = 0; v < RPL; ++v) ae[q][v] = ah[q][v] = T{0, 0};
+  for (int cc = 0; cc < cols; cc += KC) {
+    const int nc = min(KC, cols - cc);
     __syncthreads();
-    for (int t = threadIdx.x; t < nc * 2 * R; t += 256) {
-      const int c = t / (2 * R), rem = t - c * 2 * R, cur = rem / R, rr = rem - cur * R;
-      const int n = e2_list[c0 + cc + c];
-      xsl[(c * 2 + cur) * kBatchMax + rr] = xs[rr][n + cur * N];
+    for (int t = threadIdx.x; t < nc * 2 * RP; t += 256) {
+      const int c = t / (2 * RP), w = t % (2 * RP);
+      xsl[t] = XI[(size_t)e2_list[c0 + cc + c] * 2 * RP + w];
     }
     __syncthreads();
-    if (rl < R) {
-      for (int q = 0; q < kSpmvRowsPerWG / 4; ++q) {
-        const int row = r0 + wv + 4 * q;
-        if (row >= r0 + rows) break;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int row = r0 + wv + 4 * q;
+      if (row < r0 + rows) {
         const T *Z = blocks + 4 * (near_off[b] + (int64_t)row * cols + cc);
-        for (int c = cg; c < nc; c += 4) {
+        for (int c = cl; c < nc; c += CL) {
           const T z0 = Z[4 * c], z1 = Z[4 * c + 1], z2 = Z[4 * c + 2], z3 = Z[4 * c + 3];
-          const T xe = xsl[(c * 2 + 0) * kBatchMax + rl], xh = xsl[(c * 2 + 1) * kBatchMax + rl];
-          ae[q] = ae[q] + z0 * xe + z1 * xh;
-          ah[q] = ah[q] + z2 * xe + z3 * xh;
+          const T *xe = xsl + (c * 2 + 0) * RP + rg * RPL, *xh = xsl + (c * 2 + 1) * RP + rg * RPL;
+#pragma unroll
+          for (int v = 0; v < RPL; ++v) {
+            ae[q][v] = ae[q][v] + z0 * xe[v] + z1 * xh[v];
+            ah[q][v] = ah[q][v] + z2 * xe[v] + z3 * xh[v];
+          }
         }
       }
     }
   }
-  for (int q = 0; q < kSpmvRowsPerWG / 4; ++q) {
-    // sum the four column groups: lanes rl, rl+16, rl+32, rl+48
-    T se = ae[q], sh = ah[q];
-    se = se + bshfl_down(se, 32);
-    sh = sh + bshfl_down(sh, 32);
-    se = se + bshfl_down(se, 16);
-    sh = sh + bshfl_down(sh, 16);
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
     const int row = r0 + wv + 4 * q;
-    if (cg == 0 && rl < R && row < r0 + rows) {
-      const int m = perm[box_first[b] + row];
-      if (yf) {
-        se = se + yf[(size_t)rl * 2 * N + m];
-        sh = sh + yf[(size_t)rl * 2 * N + m + N];
+#pragma unroll
+    for (int v = 0; v < RPL; ++v) {
+      T se = ae[q][v], sh = ah[q][v];
+#pragma unroll
+      for (int off = CL / 2; off > 0; off >>= 1) {
+        se = se + bshfl_down(se, off);
+        sh = sh + bshfl_down(sh, off);
       }
-      ys[rl][m] = se;
-      ys[rl][m + N] = sh;
+      const int r = rg * RPL + v;
+      if (cl == 0 && r < R && row < r0 + rows) {
+        const int m = perm[box_first[b] + row];
+        if (yf) {
+          se = se + yf[(size_t)r * 2 * N + m];
+          sh = sh + yf[(size_t)r * 2 * N + m + N];
+        }
+        ys[r][m] = se;
+        ys[r][m + N] = sh;
+      }
     }
   }
 }
 
+// padded batch width: R rounded up to a power of two
+inline int batch_pad(int R) { return R <= 1 ? 1 : R <= 2 ? 2 : R <= 4 ? 4 : R <= 8 ? 8 : 16; }
+
+template <class T>
+void launch_batch_interleave(int N, int R, const T *const *xs, T *XI, hipStream_t st) {
+  const int RP = batch_pad(R);
+  const unsigned g = bnblk((int64_t)2 * N * RP, 256);
+  switch (RP) {
+#define SPK_IL_CASE(P)                                                                          \
+  case P:                                                                                       \
+    hipLaunchKernelGGL((kb_interleave<T, P>), dim3(g), dim3(256), 0, st, N, R, xs, XI);          \
+    break;
+    SPK_IL_CASE(1) SPK_IL_CASE(2) SPK_IL_CASE(4) SPK_IL_CASE(8) SPK_IL_CASE(16)
+#undef SPK_IL_CASE
+  }
+}
 template <class T>
 void launch_batch_agg(int nbl, int S, int N, int R, const int *box_first, const int *box_count,
-                      const int *perm, const T *rt, const T *const *xs, T *C, hipStream_t st) {
+                      const int *perm, const T *rt, const T *XI, T *C, hipStream_t st) {
   if (nbl <= 0) return;
-  hipLaunchKernelGGL(kb_agg<T>, dim3(bnblk(S, 256), nbl, 4), dim3(256), 0, st, S, N, R, box_first,
-                     box_count, perm, rt, xs, C);
+  const dim3 g(bnblk(S, 256), nbl, 4);
+  switch (batch_pad(R)) {
+#define SPK_AGG_CASE(RP)                                                                       \
+  case RP:                                                                                     \
+    hipLaunchKernelGGL((kb_agg<T, RP>), g, dim3(256), 0, st, S, N, R, box_first, box_count,     \
+                       perm, rt, XI, C);                                                       \
+    break;
+    SPK_AGG_CASE(1) SPK_AGG_CASE(2) SPK_AGG_CASE(4) SPK_AGG_CASE(8) SPK_AGG_CASE(16)
+#undef SPK_AGG_CASE
+  }
 }
 template <class T>
 void launch_batch_ss(int nb, int S, int R, const int *child, const T *Cc, const T *up, T *P,
@@ -294,8 +361,10 @@ template <class T>
 void launch_batch_m2l(int nb, int S, int R, const int *i4_off, const int *i4_box,
                       const int *i4_slot, const T *B, const T *tlt, T *Dt, hipStream_t st) {
   if (nb <= 0) return;
-  hipLaunchKernelGGL(kb_m2l<T>, dim3(bnblk((int64_t)S * R, 128), nb), dim3(128), 0, st, S, R,
-                     i4_off, i4_box, i4_slot, B, tlt, Dt);
+  const int gx = (int)bnblk((int64_t)S * R, 128);
+  const int64_t g = ((int64_t)gx * nb + 7) / 8 * 8;  // a multiple of the 8 XCDs
+  hipLaunchKernelGGL(kb_m2l<T>, dim3((unsigned)g), dim3(128), 0, st, S, R, nb, gx, i4_off, i4_box,
+                     i4_slot, B, tlt, Dt);
 }
 template <class T>
 void launch_batch_rr(int nb, int S, int R, const int *parent, const int *octant, const T *Dp,
@@ -317,23 +386,40 @@ void launch_batch_disagg(int ntiles, int N, int S, int R, const int *tile_box, c
                          const int *box_first, const int *box_count, const int *perm, const T *rt,
                          const T *W, zd sc1, zd sc2, T *yf, hipStream_t st) {
   if (ntiles <= 0) return;
-  hipLaunchKernelGGL(kb_disagg<T>, dim3(ntiles), dim3(256), 0, st, N, S, R, tile_box, tile_p0,
-                     box_first, box_count, perm, rt, W, cvt<T>(sc1), cvt<T>(sc2), yf);
+  switch (batch_pad(R)) {
+#define SPK_DIS_CASE(RP)                                                                        \
+  case RP:                                                                                      \
+    hipLaunchKernelGGL((kb_disagg<T, RP>), dim3(ntiles), dim3(16 * RP), 0, st, N, S, R,         \
+                       tile_box, tile_p0, box_first, box_count, perm, rt, W, cvt<T>(sc1),        \
+                       cvt<T>(sc2), yf);                                                        \
+    break;
+    SPK_DIS_CASE(1) SPK_DIS_CASE(2) SPK_DIS_CASE(4) SPK_DIS_CASE(8) SPK_DIS_CASE(16)
+#undef SPK_DIS_CASE
+  }
 }
 template <class T>
 void launch_batch_near(int nwg, int N, int R, const int *row_box, const int *row_r0,
                        const int *box_first, const int *box_count, const int *e2_off,
                        const int *e2_list, const int64_t *near_off, const int *perm,
-                       const T *blocks, const T *const *xs, const T *yf, T *const *ys,
+                       const T *blocks, const T *XI, const T *yf, T *const *ys,
                        hipStream_t st) {
   if (nwg <= 0) return;
-  hipLaunchKernelGGL(kb_near<T>, dim3(nwg), dim3(256), 0, st, N, R, row_box, row_r0, box_first,
-                     box_count, e2_off, e2_list, near_off, perm, blocks, xs, yf, ys);
+  switch (batch_pad(R)) {
+#define SPK_NEAR_CASE(RP)                                                                       \
+  case RP:                                                                                      \
+    hipLaunchKernelGGL((kb_near<T, RP>), dim3(nwg), dim3(256), 0, st, N, R, row_box, row_r0,    \
+                       box_first, box_count, e2_off, e2_list, near_off, perm, blocks, XI, yf,    \
+                       ys);                                                                     \
+    break;
+    SPK_NEAR_CASE(1) SPK_NEAR_CASE(2) SPK_NEAR_CASE(4) SPK_NEAR_CASE(8) SPK_NEAR_CASE(16)
+#undef SPK_NEAR_CASE
+  }
 }
 
 #define SPK_BATCH_INSTANTIATE(T)                                                                  \
+  template void launch_batch_interleave<T>(int, int, const T *const *, T *, hipStream_t);         \
   template void launch_batch_agg<T>(int, int, int, int, const int *, const int *, const int *,     \
-                                    const T *, const T *const *, T *, hipStream_t);                \
+                                    const T *, const T *, T *, hipStream_t);                       \
   template void launch_batch_ss<T>(int, int, int, const int *, const T *, const T *, T *,          \
                                    hipStream_t);                                                   \
   template void launch_batch_m2l<T>(int, int, int, const int *, const int *, const int *,          \
@@ -347,8 +433,8 @@ void launch_batch_near(int nwg, int N, int R, const int *row_box, const int *row
                                        T *, hipStream_t);                                          \
   template void launch_batch_near<T>(int, int, int, const int *, const int *, const int *,         \
                                      const int *, const int *, const int *, const int64_t *,       \
-                                     const int *, const T *, const T *const *, const T *,          \
-                                     T *const *, hipStream_t);
+                                     const int *, const T *, const T *, const T *, T *const *,     \
+                                     hipStream_t);
 SPK_BATCH_INSTANTIATE(zd)
 SPK_BATCH_INSTANTIATE(zf)
 

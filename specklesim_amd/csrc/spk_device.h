@@ -130,8 +130,10 @@ void launch_tl_level(int S, int K, const double *khat, const double *trf, const 
 constexpr int kSpmvRowsPerWG = 16;
 // edges per disaggregation task (one wavefront, one leaf box)
 constexpr int kDisaggEdges = 4;
-// right-hand sides per batched matvec launch
+// right-hand sides per batched matvec launch; edges per batched
+// disaggregation tile
 constexpr int kBatchMax = 16;
+constexpr int kBatchTile = 64;
 // arbitrary (m, n) pairs, one thread per pair (parity probe)
 void launch_near_pairs(const NearArgs &a, int npairs, const int *m, const int *n, zd *out,
                        hipStream_t st);
@@ -202,9 +204,12 @@ void launch_gemv_n(int64_t n, int k, const T *V, int64_t ldv, const zd *h, zd *o
 size_t gemv_n_partial_elems(int64_t n, int kmax);
 // the matvec for R <= kBatchMax right-hand sides (spk_batch.hip); xs, ys:
 // device arrays of R vector pointers; k-space arrays [box][it][s][r]
+// XI[(m * 2 + cur) * RP + r] = xs[r][m + cur N], RP = R rounded up to a power of two
+template <class T>
+void launch_batch_interleave(int N, int R, const T *const *xs, T *XI, hipStream_t st);
 template <class T>
 void launch_batch_agg(int nbl, int S, int N, int R, const int *box_first, const int *box_count,
-                      const int *perm, const T *rt, const T *const *xs, T *C, hipStream_t st);
+                      const int *perm, const T *rt, const T *XI, T *C, hipStream_t st);
 template <class T>
 void launch_batch_ss(int nb, int S, int R, const int *child, const T *Cc, const T *up, T *P,
                      hipStream_t st);
@@ -217,7 +222,7 @@ void launch_batch_rr(int nb, int S, int R, const int *parent, const int *octant,
 template <class T>
 void launch_batch_fold(int nbl, int S, int R, const T *D, zd kd1, zd kd2, zd my1, zd my2, zd ep1,
                        zd ep2, T *W, hipStream_t st);
-// tiles of up to 16 consecutive edges of one leaf box
+// tiles of up to kBatchTile consecutive edges of one leaf box
 template <class T>
 void launch_batch_disagg(int ntiles, int N, int S, int R, const int *tile_box, const int *tile_p0,
                          const int *box_first, const int *box_count, const int *perm, const T *rt,
@@ -226,7 +231,7 @@ template <class T>
 void launch_batch_near(int nwg, int N, int R, const int *row_box, const int *row_r0,
                        const int *box_first, const int *box_count, const int *e2_off,
                        const int *e2_list, const int64_t *near_off, const int *perm,
-                       const T *blocks, const T *const *xs, const T *yf, T *const *ys,
+                       const T *blocks, const T *XI, const T *yf, T *const *ys,
                        hipStream_t st);
 
 // streaming read of a buffer (bandwidth probe)
