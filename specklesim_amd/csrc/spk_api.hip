@@ -73,7 +73,8 @@ template <class T>
 struct BatchBufs {
   int R = 0;
   std::vector<DBuf<T>> B, D;  // per level, [box][it][s][r]
-  DBuf<T> W, YF, XI;          // folded leaf coefficients; far part [r][2N]; interleaved x
+  DBuf<T> W, YF, YP, XI, XP;  // folded leaf coefficients; far part [r][2N] and its partial
+                              // slices; interleaved x
   DBuf<const T *> xs;
   DBuf<T *> ys;
 };
@@ -426,7 +427,9 @@ static int batch_ensure(spk_handle *h, int R) {
   }
   HIPCHK(BB.W.alloc((size_t)h->lev[t.l_max].nb * 8 * S * R));
   HIPCHK(BB.YF.alloc((size_t)R * 2 * h->N));
+  HIPCHK(BB.YP.alloc((size_t)kBatchSplit * R * 2 * h->N));
   HIPCHK(BB.XI.alloc((size_t)kBatchMax * 2 * h->N));
+  HIPCHK(BB.XP.alloc((size_t)kBatchMax * 2 * h->N));
   HIPCHK(BB.xs.alloc(kBatchMax));
   HIPCHK(BB.ys.alloc(kBatchMax));
   BB.R = R;
@@ -445,9 +448,9 @@ int matvec_batch_t(spk_handle *h, int R, const T *const *xs, T *const *ys) {
   const int S = h->tb.S, N = h->N, lmax = t.l_max, lmin = t.l_min;
   hipStream_t st = h->st;
   LevelDev &LL = h->lev[lmax];
-  launch_batch_interleave(N, R, xs, BB.XI.p, st);
+  launch_batch_interleave(N, R, h->d_perm.p, xs, BB.XI.p, BB.XP.p, st);
   launch_batch_agg(LL.nb, S, N, R, h->d_box_first.p, h->d_box_count.p, h->d_perm.p, pRt(h, tag),
-                   (const T *)BB.XI.p, BB.B[lmax].p, st);
+                   (const T *)BB.XI.p, (const T *)BB.XP.p, BB.B[lmax].p, st);
   for (int l = lmax - 1; l >= lmin; --l)
     launch_batch_ss(h->lev[l].nb, S, R, h->lev[l].child.p, (const T *)BB.B[l + 1].p,
                     pUp(h->lev[l + 1], tag), BB.B[l].p, st);
@@ -464,7 +467,7 @@ int matvec_batch_t(spk_handle *h, int R, const T *const *xs, T *const *ys) {
   launch_batch_fold(LL.nb, S, R, (const T *)BB.D[lmax].p, kd1, kd2, my1, my2, ep1, ep2, BB.W.p, st);
   launch_batch_disagg(h->ntiles, N, S, R, h->d_tile_box.p, h->d_tile_p0.p, h->d_box_first.p,
                       h->d_box_count.p, h->d_perm.p, pRt(h, tag), (const T *)BB.W.p, h->sc1,
-                      h->sc2, BB.YF.p, st);
+                      h->sc2, BB.YP.p, BB.YF.p, st);
   launch_batch_near(h->spmv_wgs, N, R, h->d_row_box.p, h->d_row_r0.p, h->d_box_first.p,
                     h->d_box_count.p, h->d_e2_off.p, h->d_e2_list.p, h->d_near_off.p, h->d_perm.p,
                     pBlk(h, tag), (const T *)BB.XI.p, (const T *)BB.YF.p, ys, st);
@@ -1224,9 +1227,15 @@ static int gmres_mixed_batch(spk_handle *h, int R, const zd *const *b, zd *const
       xp.push_back(x[r]);
       yp.push_back(Tr(r));
     }
-    HIPCHK(hipMemcpy(xs64.p, xp.data(), set.size() * sizeof(void *), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(ys64.p, yp.data(), set.size() * sizeof(void *), hipMemcpyHostToDevice));
-    int e = matvec_batch_t<zd>(h, (int)set.size(), xs64.p, ys64.p);
+    int e = SPK_OK;
+    if (set.size() < 4) {
+      // a few stragglers: the single-vector operator is faster than a narrow batch
+      for (size_t i = 0; i < set.size() && !e; ++i) e = matvec_t<zd>(h, xp[i], yp[i]);
+    } else {
+      HIPCHK(hipMemcpy(xs64.p, xp.data(), set.size() * sizeof(void *), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(ys64.p, yp.data(), set.size() * sizeof(void *), hipMemcpyHostToDevice));
+      e = matvec_batch_t<zd>(h, (int)set.size(), xs64.p, ys64.p);
+    }
     if (e) return e;
     for (int r : set) {
       launch_zsub(n, b[r], Tr(r), Rr(r), h->st);
@@ -1278,9 +1287,14 @@ static int gmres_mixed_batch(spk_handle *h, int R, const zd *const *b, zd *const
       xp.push_back(Vr(r) + (size_t)S[r].j * n);
       yp.push_back(Wr(r));
     }
-    HIPCHK(hipMemcpy(xs32.p, xp.data(), act.size() * sizeof(void *), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(ys32.p, yp.data(), act.size() * sizeof(void *), hipMemcpyHostToDevice));
-    if ((rc = matvec_batch_t<zf>(h, (int)act.size(), xs32.p, ys32.p))) return rc;
+    if (act.size() < 4) {
+      for (size_t i = 0; i < act.size() && !rc; ++i) rc = matvec_t<zf>(h, xp[i], yp[i]);
+    } else {
+      HIPCHK(hipMemcpy(xs32.p, xp.data(), act.size() * sizeof(void *), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(ys32.p, yp.data(), act.size() * sizeof(void *), hipMemcpyHostToDevice));
+      rc = matvec_batch_t<zf>(h, (int)act.size(), xs32.p, ys32.p);
+    }
+    if (rc) return rc;
     // ICGS passes for all active right-hand sides; a second pass only for
     // those whose |w| fell below |h| (the CERFACS test)
     std::vector<int> pass = act;

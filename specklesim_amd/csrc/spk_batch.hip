@@ -29,14 +29,16 @@ __device__ inline T bshfl_down(T v, int off) {
 
 // The R input vectors interleaved: XI[(m * 2 + cur) * RP + r] = x_r[m + cur N]
 // (zero for r >= R), so that a column's 2 RP values are one contiguous run.
+// XP: the same in edge-position order, XP[(p * 2 + cur) * RP + r] = x_r[perm[p] + cur N].
 template <class T, int RP>
-__global__ void kb_interleave(int N, int R, const T *const *xs, T *XI) {
+__global__ void kb_interleave(int N, int R, const int *perm, const T *const *xs, T *XI, T *XP) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (int64_t)2 * N * RP) return;
   const int r = (int)(t % RP);
   const int64_t mc = t / RP;
   const int m = (int)(mc >> 1), cur = (int)(mc & 1);
   XI[t] = r < R ? xs[r][m + (int64_t)cur * N] : T{0, 0};
+  XP[t] = r < R ? xs[r][perm[m] + (int64_t)cur * N] : T{0, 0};
 }
 
 // Aggregation: C[b][it][s][r] = sum_{edges of b} conj(RT[p][q][s]) x_r(cur),
@@ -46,8 +48,8 @@ __global__ void kb_interleave(int N, int R, const T *const *xs, T *XI) {
 // power of two (lanes beyond R are zero and not written).
 template <class T, int RP>
 __global__ void __launch_bounds__(256) kb_agg(int S, int N, int R, const int *box_first,
-                                              const int *box_count, const int *perm, const T *rt,
-                                              const T *XI, T *C) {
+                                              const int *box_count, const T *rt, const T *XP,
+                                              T *C) {
   __shared__ T xl[64 * 2 * RP];
   const int s = blockIdx.x * 256 + threadIdx.x;
   const int b = blockIdx.y, q = blockIdx.z, k = q >> 1, comp = q & 1;
@@ -58,10 +60,8 @@ __global__ void __launch_bounds__(256) kb_agg(int S, int N, int R, const int *bo
   for (int e0 = 0; e0 < cnt; e0 += 64) {
     const int ne = min(64, cnt - e0);
     __syncthreads();
-    for (int i = threadIdx.x; i < ne * 2 * RP; i += 256) {
-      const int e = i / (2 * RP), w = i % (2 * RP);
-      xl[i] = XI[(size_t)perm[p0 + e0 + e] * 2 * RP + w];
-    }
+    // the box's edges are consecutive positions: one contiguous run of XP
+    for (int i = threadIdx.x; i < ne * 2 * RP; i += 256) xl[i] = XP[(size_t)(p0 + e0) * 2 * RP + i];
     __syncthreads();
     if (s < S) {
       for (int e = 0; e < ne; ++e) {
@@ -186,7 +186,8 @@ template <class T, int RP>
 __global__ void __launch_bounds__(256) kb_disagg(int N, int S, int R, const int *tile_box,
                                                  const int *tile_p0, const int *box_first,
                                                  const int *box_count, const int *perm,
-                                                 const T *rt, const T *W, T sc1, T sc2, T *yf) {
+                                                 const T *rt, const T *W, T sc1, T sc2, int sblk,
+                                                 T *yp) {
   constexpr int NT = 16 * RP, EPT = 4;
   __shared__ T wl[8 * kBatchSC * RP];
   __shared__ T fl[kBatchTile * 4 * kBatchSC];
@@ -198,8 +199,11 @@ __global__ void __launch_bounds__(256) kb_disagg(int N, int S, int R, const int 
   T se[EPT], sh[EPT];
 #pragma unroll
   for (int e = 0; e < EPT; ++e) se[e] = sh[e] = T{0, 0};
-  for (int s0 = 0; s0 < S; s0 += kBatchSC) {
-    const int ns = min(kBatchSC, S - s0);
+  // this workgroup's sample range; its partial sums go to slice blockIdx.y
+  const int s_lo = blockIdx.y * sblk, s_hi = min(S, s_lo + sblk);
+  T *yf = yp + (size_t)blockIdx.y * R * 2 * N;
+  for (int s0 = s_lo; s0 < s_hi; s0 += kBatchSC) {
+    const int ns = min(kBatchSC, s_hi - s0);
     __syncthreads();
     // W[j][s0 + i][r] -> wl[(j * kBatchSC + i) * RP + r]
     for (int t = threadIdx.x; t < 8 * kBatchSC * RP; t += NT) {
@@ -323,13 +327,14 @@ __global__ void __launch_bounds__(256) kb_near(int N, int R, const int *row_box,
 inline int batch_pad(int R) { return R <= 1 ? 1 : R <= 2 ? 2 : R <= 4 ? 4 : R <= 8 ? 8 : 16; }
 
 template <class T>
-void launch_batch_interleave(int N, int R, const T *const *xs, T *XI, hipStream_t st) {
+void launch_batch_interleave(int N, int R, const int *perm, const T *const *xs, T *XI, T *XP,
+                             hipStream_t st) {
   const int RP = batch_pad(R);
   const unsigned g = bnblk((int64_t)2 * N * RP, 256);
   switch (RP) {
 #define SPK_IL_CASE(P)                                                                          \
   case P:                                                                                       \
-    hipLaunchKernelGGL((kb_interleave<T, P>), dim3(g), dim3(256), 0, st, N, R, xs, XI);          \
+    hipLaunchKernelGGL((kb_interleave<T, P>), dim3(g), dim3(256), 0, st, N, R, perm, xs, XI, XP); \
     break;
     SPK_IL_CASE(1) SPK_IL_CASE(2) SPK_IL_CASE(4) SPK_IL_CASE(8) SPK_IL_CASE(16)
 #undef SPK_IL_CASE
@@ -337,14 +342,17 @@ void launch_batch_interleave(int N, int R, const T *const *xs, T *XI, hipStream_
 }
 template <class T>
 void launch_batch_agg(int nbl, int S, int N, int R, const int *box_first, const int *box_count,
-                      const int *perm, const T *rt, const T *XI, T *C, hipStream_t st) {
+                      const int *perm, const T *rt, const T *XI, const T *XP, T *C,
+                      hipStream_t st) {
   if (nbl <= 0) return;
+  (void)XI;
+  (void)perm;
   const dim3 g(bnblk(S, 256), nbl, 4);
   switch (batch_pad(R)) {
 #define SPK_AGG_CASE(RP)                                                                       \
   case RP:                                                                                     \
     hipLaunchKernelGGL((kb_agg<T, RP>), g, dim3(256), 0, st, S, N, R, box_first, box_count,     \
-                       perm, rt, XI, C);                                                       \
+                       rt, XP, C);                                                             \
     break;
     SPK_AGG_CASE(1) SPK_AGG_CASE(2) SPK_AGG_CASE(4) SPK_AGG_CASE(8) SPK_AGG_CASE(16)
 #undef SPK_AGG_CASE
@@ -381,21 +389,36 @@ void launch_batch_fold(int nbl, int S, int R, const T *D, zd kd1, zd kd2, zd my1
                      cvt<T>(kd1), cvt<T>(kd2), cvt<T>(my1), cvt<T>(my2), cvt<T>(ep1), cvt<T>(ep2),
                      W);
 }
+// yf = sum over the kBatchSplit slices of yp, in slice order
+template <class T>
+__global__ void kb_sum_split(int64_t len, const T *yp, T *yf) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= len) return;
+  T acc = yp[i];
+  for (int sp = 1; sp < kBatchSplit; ++sp) acc = acc + yp[(size_t)sp * len + i];
+  yf[i] = acc;
+}
+
 template <class T>
 void launch_batch_disagg(int ntiles, int N, int S, int R, const int *tile_box, const int *tile_p0,
                          const int *box_first, const int *box_count, const int *perm, const T *rt,
-                         const T *W, zd sc1, zd sc2, T *yf, hipStream_t st) {
+                         const T *W, zd sc1, zd sc2, T *yp, T *yf, hipStream_t st) {
   if (ntiles <= 0) return;
+  // samples split over kBatchSplit workgroups per tile (multiples of 16 samples)
+  const int sblk = (int)bnblk(bnblk(S, 16), kBatchSplit) * 16;
+  const dim3 g(ntiles, kBatchSplit);
   switch (batch_pad(R)) {
 #define SPK_DIS_CASE(RP)                                                                        \
   case RP:                                                                                      \
-    hipLaunchKernelGGL((kb_disagg<T, RP>), dim3(ntiles), dim3(16 * RP), 0, st, N, S, R,         \
-                       tile_box, tile_p0, box_first, box_count, perm, rt, W, cvt<T>(sc1),        \
-                       cvt<T>(sc2), yf);                                                        \
+    hipLaunchKernelGGL((kb_disagg<T, RP>), g, dim3(16 * RP), 0, st, N, S, R, tile_box, tile_p0, \
+                       box_first, box_count, perm, rt, W, cvt<T>(sc1), cvt<T>(sc2), sblk, yp);  \
     break;
     SPK_DIS_CASE(1) SPK_DIS_CASE(2) SPK_DIS_CASE(4) SPK_DIS_CASE(8) SPK_DIS_CASE(16)
 #undef SPK_DIS_CASE
   }
+  const int64_t len = (int64_t)R * 2 * N;
+  hipLaunchKernelGGL(kb_sum_split<T>, dim3(bnblk(len, 256)), dim3(256), 0, st, len, (const T *)yp,
+                     yf);
 }
 template <class T>
 void launch_batch_near(int nwg, int N, int R, const int *row_box, const int *row_r0,
@@ -417,9 +440,10 @@ void launch_batch_near(int nwg, int N, int R, const int *row_box, const int *row
 }
 
 #define SPK_BATCH_INSTANTIATE(T)                                                                  \
-  template void launch_batch_interleave<T>(int, int, const T *const *, T *, hipStream_t);         \
+  template void launch_batch_interleave<T>(int, int, const int *, const T *const *, T *, T *,     \
+                                           hipStream_t);                                          \
   template void launch_batch_agg<T>(int, int, int, int, const int *, const int *, const int *,     \
-                                    const T *, const T *, T *, hipStream_t);                       \
+                                    const T *, const T *, const T *, T *, hipStream_t);            \
   template void launch_batch_ss<T>(int, int, int, const int *, const T *, const T *, T *,          \
                                    hipStream_t);                                                   \
   template void launch_batch_m2l<T>(int, int, int, const int *, const int *, const int *,          \
@@ -430,7 +454,7 @@ void launch_batch_near(int nwg, int N, int R, const int *row_box, const int *row
                                      hipStream_t);                                                 \
   template void launch_batch_disagg<T>(int, int, int, int, const int *, const int *, const int *,  \
                                        const int *, const int *, const T *, const T *, zd, zd,     \
-                                       T *, hipStream_t);                                          \
+                                       T *, T *, hipStream_t);                                     \
   template void launch_batch_near<T>(int, int, int, const int *, const int *, const int *,         \
                                      const int *, const int *, const int *, const int64_t *,       \
                                      const int *, const T *, const T *, const T *, T *const *,     \

@@ -134,6 +134,8 @@ constexpr int kDisaggEdges = 4;
 // disaggregation tile
 constexpr int kBatchMax = 16;
 constexpr int kBatchTile = 64;
+// workgroups per disaggregation tile, each summing a share of the samples
+constexpr int kBatchSplit = 4;
 // arbitrary (m, n) pairs, one thread per pair (parity probe)
 void launch_near_pairs(const NearArgs &a, int npairs, const int *m, const int *n, zd *out,
                        hipStream_t st);
@@ -205,11 +207,14 @@ size_t gemv_n_partial_elems(int64_t n, int kmax);
 // the matvec for R <= kBatchMax right-hand sides (spk_batch.hip); xs, ys:
 // device arrays of R vector pointers; k-space arrays [box][it][s][r]
 // XI[(m * 2 + cur) * RP + r] = xs[r][m + cur N], RP = R rounded up to a power of two
+// and XP[(p * 2 + cur) * RP + r] = xs[r][perm[p] + cur N] (edge-position order)
 template <class T>
-void launch_batch_interleave(int N, int R, const T *const *xs, T *XI, hipStream_t st);
+void launch_batch_interleave(int N, int R, const int *perm, const T *const *xs, T *XI, T *XP,
+                             hipStream_t st);
 template <class T>
 void launch_batch_agg(int nbl, int S, int N, int R, const int *box_first, const int *box_count,
-                      const int *perm, const T *rt, const T *XI, T *C, hipStream_t st);
+                      const int *perm, const T *rt, const T *XI, const T *XP, T *C,
+                      hipStream_t st);
 template <class T>
 void launch_batch_ss(int nb, int S, int R, const int *child, const T *Cc, const T *up, T *P,
                      hipStream_t st);
@@ -223,10 +228,11 @@ template <class T>
 void launch_batch_fold(int nbl, int S, int R, const T *D, zd kd1, zd kd2, zd my1, zd my2, zd ep1,
                        zd ep2, T *W, hipStream_t st);
 // tiles of up to kBatchTile consecutive edges of one leaf box
+// yp: kBatchSplit partial slices of R x 2N, summed into yf
 template <class T>
 void launch_batch_disagg(int ntiles, int N, int S, int R, const int *tile_box, const int *tile_p0,
                          const int *box_first, const int *box_count, const int *perm, const T *rt,
-                         const T *W, zd sc1, zd sc2, T *yf, hipStream_t st);
+                         const T *W, zd sc1, zd sc2, T *yp, T *yf, hipStream_t st);
 template <class T>
 void launch_batch_near(int nwg, int N, int R, const int *row_box, const int *row_r0,
                        const int *box_first, const int *box_count, const int *e2_off,
