@@ -1184,6 +1184,14 @@ static int gmres_mixed_batch(spk_handle *h, int R, const zd *const *b, zd *const
   zd *partial_nrm = h->g_partial.p + gc_elems;
   HIPCHK(h->g_pn.alloc(gemv_n_partial_elems(n, m + 1)));
   HIPCHK(h->pin.ensure((size_t)R * ldh));
+  // arguments of the batched Gram-Schmidt passes: basis, vector and
+  // coefficient pointers, column counts, partial sums
+  DBuf<uint64_t> orth_args;
+  DBuf<int> orth_ks;
+  DBuf<zd> orth_partial;
+  HIPCHK(orth_args.alloc(4 * (size_t)kBatchMax));
+  HIPCHK(orth_ks.alloc(kBatchMax));
+  HIPCHK(orth_partial.alloc(batch_orth_partial_elems(n, R, m + 1)));
   auto Vr = [&](int r) { return V.p + (size_t)r * (m + 1) * n; };
   auto Wr = [&](int r) { return W.p + (size_t)r * n; };
   auto Rr = [&](int r) { return Rv.p + (size_t)r * n; };
@@ -1300,14 +1308,27 @@ static int gmres_mixed_batch(spk_handle *h, int R, const zd *const *b, zd *const
     std::vector<int> pass = act;
     std::vector<double> dnormw(R, 0.0);
     for (int np_ = 1; !pass.empty() && np_ <= 3; ++np_) {
-      for (int r : pass) {
-        const int j = S[r].j;
-        zd *hr = Hd.p + (size_t)r * ldh;
-        launch_gemv_c(n, j + 1, (const zf *)Vr(r), n, (const zf *)Wr(r), (const zf *)nullptr,
-                      h->g_partial.p, hr, h->st);
-        launch_gemv_n_sub(n, j + 1, (const zf *)Vr(r), n, hr, Wr(r), h->g_pn.p, h->st);
-        launch_zdotc(n, (const zf *)Wr(r), (const zf *)Wr(r), partial_nrm, hr + (j + 1), h->st);
+      // one batched ICGS pass for all right-hand sides in `pass`
+      const int P = (int)pass.size();
+      std::vector<uint64_t> arg(4 * (size_t)kBatchMax, 0);
+      int maxk = 0;
+      for (int i = 0; i < P; ++i) {
+        const int r = pass[i], k = S[r].j + 1;
+        maxk = std::max(maxk, k);
+        arg[i] = (uint64_t)(uintptr_t)Vr(r);
+        arg[kBatchMax + i] = (uint64_t)(uintptr_t)Wr(r);
+        arg[2 * kBatchMax + i] = (uint64_t)(uintptr_t)(Hd.p + (size_t)r * ldh);
+        arg[3 * kBatchMax + i] = (uint64_t)k;
       }
+      std::vector<int> kk(P);
+      for (int i = 0; i < P; ++i) kk[i] = (int)arg[3 * kBatchMax + i];
+      HIPCHK(hipMemcpy(orth_args.p, arg.data(), 3 * kBatchMax * sizeof(uint64_t),
+                       hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(orth_ks.p, kk.data(), P * sizeof(int), hipMemcpyHostToDevice));
+      const zf *const *Vs = (const zf *const *)orth_args.p;
+      zf *const *Ws = (zf *const *)(orth_args.p + kBatchMax);
+      zd *const *Hs = (zd *const *)(orth_args.p + 2 * kBatchMax);
+      launch_batch_orth(n, P, maxk, orth_ks.p, Vs, Ws, Hs, orth_partial.p, h->st);
       HIPCHK(hipMemcpyAsync(h->pin.p, Hd.p, (size_t)R * ldh * sizeof(zd), hipMemcpyDeviceToHost,
                             h->st));
       HIPCHK(hipStreamSynchronize(h->st));

@@ -10,6 +10,8 @@
 // The arithmetic of each kernel is that of its single-vector counterpart in
 // spk_kernels.hip; only the order of the near-field and disaggregation sums
 // differs.
+#include <algorithm>
+
 #include "spk_device.h"
 
 namespace spk {
@@ -326,6 +328,151 @@ __global__ void __launch_bounds__(256) kb_near(int N, int R, const int *row_box,
 // padded batch width: R rounded up to a power of two
 inline int batch_pad(int R) { return R <= 1 ? 1 : R <= 2 ? 2 : R <= 4 ? 4 : R <= 8 ? 8 : 16; }
 
+// ---------------------------------------------------------------------------
+// Gram-Schmidt products of P independent Krylov bases in one launch each
+// (the batched solves' orthogonalisation): basis r has k_r columns of n
+// values at Vs[r] + j*n, its new vector is ws[r], its coefficients go to
+// hs[r][0..k_r) and |w_r|^2 to hs[r][k_r].  Products accumulate in
+// complex128, partial sums are added in a fixed order.
+// ---------------------------------------------------------------------------
+namespace {
+template <class T>
+__device__ inline zd bwiden(T v) {
+  return zd{(double)v.x, (double)v.y};
+}
+__device__ inline zd bwave_sum(zd v) {
+  for (int off = 32; off > 0; off >>= 1) {
+    v.x += __shfl_down(v.x, off, 64);
+    v.y += __shfl_down(v.y, off, 64);
+  }
+  return v;
+}
+}  // namespace
+
+constexpr int kOrthCT = 16;
+// partial[(r * ldp + j) * Rc + chunk] = <V_r[:, j], w_r> over the chunk's rows
+template <class T>
+__global__ void __launch_bounds__(256) kb_gemv_c(int64_t n, int64_t rows, int ldp, const int *ks,
+                                                 const T *const *Vs, const T *const *ws,
+                                                 zd *partial) {
+  __shared__ zd red[4][kOrthCT];
+  const int r = blockIdx.z, k = ks[r];
+  const int j0 = blockIdx.y * kOrthCT;
+  if (j0 >= k) return;  // uniform for the workgroup
+  const int nt = min(kOrthCT, k - j0), Rc = gridDim.x;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const T *V = Vs[r], *w = ws[r];
+  const int64_t i0 = (int64_t)blockIdx.x * rows, i1 = min<int64_t>(n, i0 + rows);
+  zd acc[kOrthCT];
+#pragma unroll
+  for (int t = 0; t < kOrthCT; ++t) acc[t] = zd{0, 0};
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) {
+    const zd x = bwiden(w[i]);
+#pragma unroll
+    for (int t = 0; t < kOrthCT; ++t)
+      if (t < nt) {
+        const zd v = bwiden(V[(size_t)(j0 + t) * n + i]);
+        acc[t] = acc[t] + conj(v) * x;
+      }
+  }
+#pragma unroll
+  for (int t = 0; t < kOrthCT; ++t) {
+    const zd v = bwave_sum(acc[t]);
+    if (lane == 0) red[wv][t] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < nt) {
+    const int t = threadIdx.x;
+    partial[((size_t)r * ldp + j0 + t) * Rc + blockIdx.x] = red[0][t] + red[1][t] + red[2][t] + red[3][t];
+  }
+}
+// hs[r][j] = sum of the Rc partials, one wave per (r, j)
+__global__ void kb_gemv_c_final(int P, int ldp, int Rc, const int *ks, const zd *partial,
+                                zd *const *hs) {
+  const int o = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (o >= P * ldp) return;
+  const int r = o / ldp, j = o % ldp;
+  if (j >= ks[r]) return;
+  zd acc = {0, 0};
+  for (int c = lane; c < Rc; c += 64) acc = acc + partial[(size_t)o * Rc + c];
+  acc = bwave_sum(acc);
+  if (lane == 0) hs[r][j] = acc;
+}
+// w_r -= V_r h_r (thread per row, all k_r columns)
+template <class T>
+__global__ void __launch_bounds__(256) kb_gemv_n_sub(int64_t n, const int *ks, const T *const *Vs,
+                                                     const zd *const *hs, T *const *ws) {
+  const int r = blockIdx.y;
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int k = ks[r];
+  const T *V = Vs[r];
+  const zd *h = hs[r];
+  zd acc = {0, 0};
+  int j = 0;
+  for (; j + 8 <= k; j += 8) {
+    zd v[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) v[t] = bwiden(V[(size_t)(j + t) * n + i]);
+#pragma unroll
+    for (int t = 0; t < 8; ++t) acc = acc + v[t] * h[j + t];
+  }
+  for (; j < k; ++j) acc = acc + bwiden(V[(size_t)j * n + i]) * h[j];
+  const zd wv = bwiden(ws[r][i]) - acc;
+  ws[r][i] = cvt<T>(wv);
+}
+// |w_r|^2: partial[r * G + block], then hs[r][k_r] = sum (imaginary part 0)
+template <class T>
+__global__ void __launch_bounds__(256) kb_norm2(int64_t n, T *const *ws, zd *partial) {
+  __shared__ double red[4];
+  const int r = blockIdx.y, G = gridDim.x;
+  const T *w = ws[r];
+  double acc = 0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)G * 256) {
+    const zd v = bwiden(w[i]);
+    acc += v.x * v.x + v.y * v.y;
+  }
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[(size_t)r * G + blockIdx.x] = zd{red[0] + red[1] + red[2] + red[3], 0.0};
+}
+__global__ void kb_norm2_final(int G, const int *ks, const zd *partial, zd *const *hs) {
+  const int r = blockIdx.x, lane = threadIdx.x;
+  zd acc = {0, 0};
+  for (int c = lane; c < G; c += 64) acc = acc + partial[(size_t)r * G + c];
+  acc = bwave_sum(acc);
+  if (lane == 0) hs[r][ks[r]] = acc;
+}
+
+constexpr int kOrthNormBlocks = 64;
+size_t batch_orth_partial_elems(int64_t n, int P, int maxk) {
+  const size_t c = (size_t)P * (size_t)maxk * 256;
+  const size_t d = (size_t)P * kOrthNormBlocks;
+  return c > d ? c : d;
+}
+
+template <class T>
+void launch_batch_orth(int64_t n, int P, int maxk, const int *ks, const T *const *Vs, T *const *ws,
+                       zd *const *hs, zd *partial, hipStream_t st) {
+  if (P <= 0 || maxk <= 0) return;
+  const int ctiles = (maxk + kOrthCT - 1) / kOrthCT;
+  // row chunks: at least 4 rows per thread, about 2048 workgroups in all
+  int64_t rc = std::max<int64_t>(1, std::min<int64_t>((n + 1023) / 1024,
+                                                     (2048 + (int64_t)ctiles * P - 1) / ((int64_t)ctiles * P)));
+  rc = std::min<int64_t>(rc, 256);
+  const int64_t rows = (n + rc - 1) / rc;
+  hipLaunchKernelGGL(kb_gemv_c<T>, dim3((unsigned)rc, ctiles, P), dim3(256), 0, st, n, rows, maxk,
+                     ks, Vs, (const T *const *)ws, partial);
+  hipLaunchKernelGGL(kb_gemv_c_final, dim3((unsigned)(((int64_t)P * maxk + 3) / 4)), dim3(256), 0,
+                     st, P, maxk, (int)rc, ks, (const zd *)partial, hs);
+  hipLaunchKernelGGL(kb_gemv_n_sub<T>, dim3(bnblk(n, 256), P), dim3(256), 0, st, n, ks, Vs,
+                     (const zd *const *)hs, ws);
+  hipLaunchKernelGGL(kb_norm2<T>, dim3(kOrthNormBlocks, P), dim3(256), 0, st, n, ws, partial);
+  hipLaunchKernelGGL(kb_norm2_final, dim3(P), dim3(64), 0, st, kOrthNormBlocks, ks,
+                     (const zd *)partial, hs);
+}
+
 template <class T>
 void launch_batch_interleave(int N, int R, const int *perm, const T *const *xs, T *XI, T *XP,
                              hipStream_t st) {
@@ -461,5 +608,7 @@ void launch_batch_near(int nwg, int N, int R, const int *row_box, const int *row
                                      hipStream_t);
 SPK_BATCH_INSTANTIATE(zd)
 SPK_BATCH_INSTANTIATE(zf)
+template void launch_batch_orth<zf>(int64_t, int, int, const int *, const zf *const *, zf *const *,
+                                    zd *const *, zd *, hipStream_t);
 
 }  // namespace spk

@@ -240,6 +240,13 @@ void launch_batch_near(int nwg, int N, int R, const int *row_box, const int *row
                        const T *blocks, const T *XI, const T *yf, T *const *ys,
                        hipStream_t st);
 
+// one ICGS pass over P bases (coefficients into hs[r][0..ks[r]), |w_r|^2
+// into hs[r][ks[r]]); partial: batch_orth_partial_elems(n, P, maxk)
+template <class T>
+void launch_batch_orth(int64_t n, int P, int maxk, const int *ks, const T *const *Vs, T *const *ws,
+                       zd *const *hs, zd *partial, hipStream_t st);
+size_t batch_orth_partial_elems(int64_t n, int P, int maxk);
+
 // streaming read of a buffer (bandwidth probe)
 void launch_stream_read(const void *a, size_t bytes, double *out, hipStream_t st);
 // one-reduce MGS: row k1 of T = (I + L)^-1 from tg = T^H g
