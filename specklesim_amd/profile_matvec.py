@@ -6,9 +6,14 @@ around it.
 
     python -m specklesim_amd.profile_matvec [--cells 100] [--size 10] [--reps 20]
     python -m specklesim_amd.profile_matvec --copy-mib 1024
+    python -m specklesim_amd.profile_matvec --batch 16
 
 ``--copy-mib`` instead runs five plain device-to-device copies of that size
 (torch's copy kernel), the calibration for FETCH_SIZE / WRITE_SIZE counters.
+``--batch R`` instead runs one batched mixed-precision solve of R plane
+waves (polar angles uniform in 0-60 degrees, TM, tolerance 1e-6) after the
+precalculation, and prints each right-hand side's iterations and backward
+error.
 """
 from __future__ import annotations
 
@@ -29,6 +34,24 @@ def copy_calibration(mib: int) -> dict:
     return {"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 5}
 
 
+def batched_solve(s, R: int) -> dict:
+    import numpy as np
+
+    from . import DeviceBuffer, plane_wave, synchronize
+
+    nb = 2 * s.N * 16
+    bs = [DeviceBuffer(nb) for _ in range(R)]
+    xs = [DeviceBuffer(nb) for _ in range(R)]
+    for r, b in enumerate(bs):
+        k, e = plane_wave(np.radians(60.0 * r / max(R - 1, 1)), 0.0, 2, "TM")
+        s.get_vector_b_tri_device(k, e, b.ptr)
+    infos = s.gmres_run_batch_device([b.ptr for b in bs], [x.ptr for x in xs],
+                                     max_iterations=1300, tolerance=1e-6,
+                                     use_initial_guess=False)
+    synchronize(0)
+    return {"batch": R, "iters": [i["iters"] for i in infos], "be": [i["be"] for i in infos]}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--cells", type=int, default=100)
@@ -39,6 +62,7 @@ def main(argv=None):
     ap.add_argument("--precision", default="fp64")
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--copy-mib", type=int, default=0)
+    ap.add_argument("--batch", type=int, default=0)
     args = ap.parse_args(argv)
     if args.copy_mib:
         print(json.dumps(copy_calibration(args.copy_mib)), flush=True)
@@ -50,8 +74,12 @@ def main(argv=None):
     pp, tt = meshes.rough_surface(args.cells, args.cells, args.size, args.size, 0.1, 0.5,
                                   args.seed, lam)
     s = SieTriMlfmm(lam, args.eps_r2, formulation=args.formulation, normal_convention=2,
-                    tree_s_opt=250, truncation_number=19, precision=args.precision)
+                    tree_s_opt=250, truncation_number=19,
+                    precision="mixed" if args.batch else args.precision)
     s.precalculation_tri(pp, tt)
+    if args.batch:
+        print(json.dumps(batched_solve(s, args.batch)), flush=True)
+        return
     c64 = args.precision == "mixed"
     prof = s.profile_matvec(reps=args.reps, c64=c64)
     st = s.stats()
