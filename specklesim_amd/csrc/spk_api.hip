@@ -115,6 +115,9 @@ struct spk_handle {
   // batched matvec: disaggregation tiles (box, first position), buffers
   DBuf<int> d_tile_box, d_tile_p0;
   int ntiles = 0;
+  // matrix-core disaggregation tiles: (box, first position) of up to 128 edges
+  DBuf<int> d_dx_box, d_dx_p0;
+  int dx_tiles = 0;
   BatchBufs<zd> bd;
   BatchBufs<zf> bf;
   int spmv_wgs = 0;
@@ -467,7 +470,7 @@ int matvec_batch_t(spk_handle *h, int R, const T *const *xs, T *const *ys) {
   launch_batch_fold(LL.nb, S, R, (const T *)BB.D[lmax].p, kd1, kd2, my1, my2, ep1, ep2, BB.W.p, st);
   launch_batch_disagg(h->ntiles, N, S, R, h->d_tile_box.p, h->d_tile_p0.p, h->d_box_first.p,
                       h->d_box_count.p, h->d_perm.p, pRt(h, tag), (const T *)BB.W.p, h->sc1,
-                      h->sc2, BB.YP.p, BB.YF.p, st);
+                      h->sc2, BB.YP.p, BB.YF.p, h->dx_tiles, h->d_dx_box.p, h->d_dx_p0.p, st);
   launch_batch_near(h->spmv_wgs, N, R, h->d_row_box.p, h->d_row_r0.p, h->d_box_first.p,
                     h->d_box_count.p, h->d_e2_off.p, h->d_e2_list.p, h->d_near_off.p, h->d_perm.p,
                     pBlk(h, tag), (const T *)BB.XI.p, (const T *)BB.YF.p, ys, st);
@@ -628,6 +631,15 @@ int spk_setup_tri(spk_handle *h, int np, const double *pp, int nt, const int *tt
     h->ntiles = (int)tb.size();
     HIPCHK(h->d_tile_box.upload(tb));
     HIPCHK(h->d_tile_p0.upload(tp));
+    std::vector<int> xb, xp;
+    for (size_t b = 0; b < t.box_count.size(); ++b)
+      for (int r = 0; r < t.box_count[b]; r += 128) {
+        xb.push_back((int)b);
+        xp.push_back(t.box_first[b] + r);
+      }
+    h->dx_tiles = (int)xb.size();
+    HIPCHK(h->d_dx_box.upload(xb));
+    HIPCHK(h->d_dx_p0.upload(xp));
     h->bd = BatchBufs<zd>();
     h->bf = BatchBufs<zf>();
   }

@@ -9,8 +9,12 @@
 // pointers (each a 2N complex vector in the reference order).
 // The arithmetic of each kernel is that of its single-vector counterpart in
 // spk_kernels.hip; only the order of the near-field and disaggregation sums
-// differs.
+// differs.  Complex64 batches of 9..16 run the near field, aggregation and
+// disaggregation on the matrix cores (v_mfma_f32_16x16x4f32, which rounds as
+// a chain of four fused multiply-adds): the aggregation and disaggregation
+// sum each group of 16 terms from zero before adding it to the running sum.
 #include <algorithm>
+#include <type_traits>
 
 #include "spk_device.h"
 
@@ -86,6 +90,122 @@ __global__ void __launch_bounds__(256) kb_agg(int S, int N, int R, const int *bo
         Cb[((size_t)(comp * 4 + 2 + k) * S + s) * R + r] = ah[r];
       }
   }
+}
+
+typedef float mxf4 __attribute__((ext_vector_type(4)));
+#define SPK_MFMA(a, b, c) __builtin_amdgcn_mfma_f32_16x16x4f32((a), (b), (c), 0, 0, 0)
+
+// Aggregation for 9..16 right-hand sides in complex64 on the matrix cores:
+// per (leaf box, q) C[s][(cur, r)] = sum_p conj(RT[p][q][s]) x_r(p, cur) is a
+// product of the box's pattern rows (samples x edges) and its x values
+// (edges x 32).  Workgroup = (256 samples, box, q), wave = 64 samples as 4
+// tiles of 16: tile 2h + par holds the samples sw + 32 h + 2 i + par, so a
+// lane's one 16-byte load gives the A values of two tiles.  A k-step covers
+// 4 edges (lane group g = edge), B[g][n] = x_n of that edge from LDS.  With
+// a = conj(f): re += fr xr + fi xi, im += fr xi - fi xr, for cur = 0 and 1.
+// Samples and edges past the end are clamped on load (their x is zero in
+// LDS, their results are not written).
+__global__ void __launch_bounds__(256) kb_agg_mx(int S, int R, const int *box_first,
+                                                 const int *box_count, const zf *rt,
+                                                 const zf *XP, zf *C) {
+  __shared__ float4 xl[64 * 16];
+  const int b = blockIdx.y, q = blockIdx.z, k = q >> 1, comp = q & 1;
+  const int p0 = box_first[b], cnt = box_count[b];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int m = lane & 15, g = lane >> 4;
+  const int sw = blockIdx.x * 256 + 64 * wv;  // first sample of this wave
+  const bool active = sw < S;                 // uniform over the wave
+  // the two sample pairs of this lane (16-byte aligned: S is even)
+  const int sa = min(sw + 2 * m, S - 2), sb = min(sw + 32 + 2 * m, S - 2);
+  mxf4 acc[4][4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc[t][u] = mxf4{0.f, 0.f, 0.f, 0.f};
+  float4 fa[4], fb[4];
+  auto load = [&](int e0, float4 *a, float4 *bb) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int e = min(e0 + 4 * j + g, cnt - 1);
+      const zf *f = rt + ((size_t)(p0 + e) * 4 + q) * S;
+      a[j] = *reinterpret_cast<const float4 *>(f + sa);
+      bb[j] = *reinterpret_cast<const float4 *>(f + sb);
+    }
+  };
+  if (active && cnt > 0) load(0, fa, fb);
+  for (int e0 = 0; e0 < cnt; e0 += 64) {
+    const int ne = min(64, cnt - e0);
+    __syncthreads();
+    {
+      const int n = threadIdx.x & 15;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int e = (int)(threadIdx.x + 256 * u) >> 4;
+        const size_t pe = (size_t)(p0 + e0 + min(e, ne - 1)) * 2;
+        const zf x0 = XP[pe * 16 + n], x1 = XP[(pe + 1) * 16 + n];
+        xl[threadIdx.x + 256 * u] = e < ne ? float4{x0.x, x0.y, x1.x, x1.y} : float4{0.f, 0.f, 0.f, 0.f};
+      }
+    }
+    __syncthreads();
+    if (!active) continue;
+    for (int i = 0; i < ne; i += 16) {
+      float4 na[4], nb[4];
+      load(e0 + i + 16, na, nb);
+      __builtin_amdgcn_sched_barrier(0);
+      // the group's 16 edges are summed from zero and then added to the
+      // running sums, so the running sums take one rounding per group
+      mxf4 gs[4][4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) gs[t][u] = mxf4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const float4 x = xl[(i + 4 * j + g) * 16 + m];
+        const float n0 = -x.x, n1 = -x.z;
+        const float ar[4] = {fa[j].x, fa[j].z, fb[j].x, fb[j].z};
+        const float ai[4] = {fa[j].y, fa[j].w, fb[j].y, fb[j].w};
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          gs[t][0] = SPK_MFMA(ar[t], x.x, gs[t][0]);
+          gs[t][1] = SPK_MFMA(ar[t], x.y, gs[t][1]);
+          gs[t][2] = SPK_MFMA(ar[t], x.z, gs[t][2]);
+          gs[t][3] = SPK_MFMA(ar[t], x.w, gs[t][3]);
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          gs[t][0] = SPK_MFMA(ai[t], x.y, gs[t][0]);
+          gs[t][1] = SPK_MFMA(ai[t], n0, gs[t][1]);
+          gs[t][2] = SPK_MFMA(ai[t], x.w, gs[t][2]);
+          gs[t][3] = SPK_MFMA(ai[t], n1, gs[t][3]);
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc[t][u] += gs[t][u];
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        fa[j] = na[j];
+        fb[j] = nb[j];
+      }
+    }
+  }
+  // accumulator element v of lane (g, n) in tile 2h + par: sample sw + 32h + 2(4g + v) + par
+  const int n = m;
+  if (!active || n >= R) return;
+  zf *Cb = C + (size_t)b * 8 * S * R;
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int s = sw + 32 * (t >> 1) + 2 * (4 * g + v) + (t & 1);
+      if (s < S) {
+        Cb[((size_t)(comp * 4 + 0 + k) * S + s) * R + n] = zf{acc[t][0][v], acc[t][1][v]};
+        Cb[((size_t)(comp * 4 + 2 + k) * S + s) * R + n] = zf{acc[t][2][v], acc[t][3][v]};
+      }
+    }
 }
 
 // translation_SS over u = s*R + r
@@ -248,6 +368,127 @@ __global__ void __launch_bounds__(256) kb_disagg(int N, int S, int R, const int 
   }
 }
 
+// Disaggregation for 9..16 right-hand sides in complex64 on the matrix
+// cores: per leaf box y_r(p, e) = sum_{k,s} fa W(k,e,a) + fb W(k,e,b) with
+// fa = RT[p][2k][s], fb = RT[p][2k+1][s] is a product of the box's pattern
+// rows (edges x (k, s)) and its folded W ((k, s) x r).  Workgroup = (tile of
+// up to 128 consecutive edges of one box, sample part blockIdx.y), wave = 16
+// edges.  A k-step covers 4 samples: lane group g takes sample s + 4g + j in
+// step j, so a lane's two 16-byte loads per pattern row serve 4 steps.  W of
+// kMxSC samples is staged in LDS as [k][sample][half][r][2]: half 0 = the e
+// pair (a, b), half 1 = the h pair.  Edges and samples past the end are
+// clamped on load (their W is zero in LDS, their results are not written).
+// yf slice blockIdx.y: [r][m] = sc1 * sum (e part), [r][m + N] = sc2 * sum (h part).
+constexpr int kMxSC = 32;
+__global__ void __launch_bounds__(512) kb_disagg_mx(int N, int S, int R, const int *tile_box,
+                                                    const int *tile_p0, const int *box_first,
+                                                    const int *box_count, const int *perm,
+                                                    const zf *rt, const zf *W, zf sc1, zf sc2,
+                                                    int sblk, zf *yp) {
+  __shared__ float4 wl[2 * kMxSC * 2 * 16];
+  const int b = tile_box[blockIdx.x], p0 = tile_p0[blockIdx.x];
+  const int cnt = min(128, box_first[b] + box_count[b] - p0);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int m = lane & 15, g = lane >> 4;
+  const int ew = 16 * wv;
+  const bool active = ew < cnt;  // uniform over the wave
+  const zf *F = rt + (size_t)(p0 + min(ew + m, cnt - 1)) * 4 * S;
+  const size_t SR = (size_t)S * R;
+  const zf *Wb = W + (size_t)b * 8 * SR;
+  const int s_lo = blockIdx.y * sblk, s_hi = min(S, s_lo + sblk);
+  mxf4 er = {0.f, 0.f, 0.f, 0.f}, ei = er, hr = er, hi = er;
+  // pattern rows 2k and 2k+1 at samples s + 4g .. s + 4g + 3 (pairs clamped, S is even)
+  auto load = [&](int s, int k, float4 *a) {
+    const int s1 = min(s + 4 * g, S - 2), s2 = min(s + 4 * g + 2, S - 2);
+    const zf *fa = F + (size_t)(2 * k) * S, *fb = fa + S;
+    a[0] = *reinterpret_cast<const float4 *>(fa + s1);
+    a[1] = *reinterpret_cast<const float4 *>(fa + s2);
+    a[2] = *reinterpret_cast<const float4 *>(fb + s1);
+    a[3] = *reinterpret_cast<const float4 *>(fb + s2);
+  };
+  float4 cur[4];
+  if (active && s_lo < s_hi) load(s_lo, 0, cur);
+  for (int s0 = s_lo; s0 < s_hi; s0 += kMxSC) {
+    __syncthreads();
+    {
+      // W[it][s0 + i][r] -> wl as complex [((k * kMxSC + i) * 2 + half) * 16 + r][jj], it = 4k + 2half + jj
+      zf *wc = reinterpret_cast<zf *>(wl);
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int t = threadIdx.x + 512 * u;
+        const int r = t & 15, i = (t >> 4) & (kMxSC - 1), it = t >> 9;
+        const int s = s0 + i;
+        const bool ok = r < R && s < s_hi;
+        const zf v = Wb[((size_t)it * S + min(s, S - 1)) * R + min(r, R - 1)];
+        const int k = it >> 2, half = (it >> 1) & 1, jj = it & 1;
+        wc[(((k * kMxSC + i) * 2 + half) * 16 + r) * 2 + jj] = ok ? v : zf{0.f, 0.f};
+      }
+    }
+    __syncthreads();
+    if (!active) continue;
+    for (int sg = 0; sg < kMxSC && s0 + sg < s_hi; sg += 16) {
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        float4 nx[4];
+        if (k == 0)
+          load(s0 + sg, 1, nx);
+        else
+          load(s0 + sg + 16, 0, nx);
+        __builtin_amdgcn_sched_barrier(0);
+        const float fa[8] = {cur[0].x, cur[0].y, cur[0].z, cur[0].w, cur[1].x, cur[1].y, cur[1].z, cur[1].w};
+        const float fb[8] = {cur[2].x, cur[2].y, cur[2].z, cur[2].w, cur[3].x, cur[3].y, cur[3].z, cur[3].w};
+        // the group's 16 samples are summed from zero and then added to the
+        // running sums, so the running sums take one rounding per group
+        mxf4 gr = {0.f, 0.f, 0.f, 0.f}, gi = gr, gh = gr, gj = gr;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int i = sg + 4 * g + j;
+          const float4 we = wl[((k * kMxSC + i) * 2 + 0) * 16 + m];  // (e,a), (e,b)
+          const float4 wh = wl[((k * kMxSC + i) * 2 + 1) * 16 + m];  // (h,a), (h,b)
+          const float ar = fa[2 * j], ai = fa[2 * j + 1], br = fb[2 * j], bi = fb[2 * j + 1];
+          const float nai = -ai, nbi = -bi;
+          gr = SPK_MFMA(ar, we.x, gr);
+          gi = SPK_MFMA(ar, we.y, gi);
+          gh = SPK_MFMA(ar, wh.x, gh);
+          gj = SPK_MFMA(ar, wh.y, gj);
+          gr = SPK_MFMA(nai, we.y, gr);
+          gi = SPK_MFMA(ai, we.x, gi);
+          gh = SPK_MFMA(nai, wh.y, gh);
+          gj = SPK_MFMA(ai, wh.x, gj);
+          gr = SPK_MFMA(br, we.z, gr);
+          gi = SPK_MFMA(br, we.w, gi);
+          gh = SPK_MFMA(br, wh.z, gh);
+          gj = SPK_MFMA(br, wh.w, gj);
+          gr = SPK_MFMA(nbi, we.w, gr);
+          gi = SPK_MFMA(bi, we.z, gi);
+          gh = SPK_MFMA(nbi, wh.w, gh);
+          gj = SPK_MFMA(bi, wh.z, gj);
+        }
+        er += gr;
+        ei += gi;
+        hr += gh;
+        hi += gj;
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) cur[u] = nx[u];
+      }
+    }
+  }
+  // accumulator element v of lane (g, n): edge ew + 4g + v, right-hand side n
+  const int n = m;
+  if (!active || n >= R) return;
+  zf *yf = yp + (size_t)blockIdx.y * R * 2 * N;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int e = ew + 4 * g + v;
+    if (e < cnt) {
+      const int mm = perm[p0 + e];
+      yf[(size_t)n * 2 * N + mm] = zf{er[v], ei[v]} * sc1;
+      yf[(size_t)n * 2 * N + mm + N] = zf{hr[v], hi[v]} * sc2;
+    }
+  }
+}
+
 // Near field for R vectors: workgroup = (leaf box, 16 rows), 4 rows per
 // wave processed together.  A wave's lanes = CL column lanes x RG groups of
 // right-hand sides, each lane holding RPL = RP / RG of them: a lane loads the
@@ -321,6 +562,141 @@ __global__ void __launch_bounds__(256) kb_near(int N, int R, const int *row_box,
         ys[r][m] = se;
         ys[r][m + N] = sh;
       }
+    }
+  }
+}
+
+// Near field for 9..16 right-hand sides in complex64 on the matrix cores.
+// Per leaf box the near operator is dense: rows = the box's edges, columns
+// = its E2 edges, each entry a 2x2 complex block.  Workgroup = (box, up to
+// 16 consecutive rows), its 4 waves take every fourth group of 16 columns.
+// One v_mfma_f32_16x16x4f32 multiplies A[m][g] (a real part of the block of
+// row m and column c + g) by B[g][n] (a real part of x_n at column c + g),
+// so each complex product is four real ones into four accumulators:
+//   e_re += Z0r xer - Z0i xei + Z1r xhr - Z1i xhi,  e_im += Z0r xei + Z0i xer + ...
+//   h_re, h_im the same with Z2, Z3.
+// A lane loads the whole 2x2 block of its (row, column), 32 bytes; the
+// columns' (xe, xh) of the 16 right-hand sides are staged in LDS, 128
+// columns at a time, and a wave loads its next 16 columns' blocks before it
+// multiplies the current ones.  Rows and columns past the end are clamped on
+// load (their x is zero in LDS, their results are not written).  The four
+// waves' sums are added in wave order through LDS.
+constexpr int kMxCols = 128;
+__global__ void __launch_bounds__(256) kb_near_mx(int N, int R, const int *row_box,
+                                                  const int *row_r0, const int *box_first,
+                                                  const int *box_count, const int *e2_off,
+                                                  const int *e2_list, const int64_t *near_off,
+                                                  const int *perm, const zf *blocks, const zf *XI,
+                                                  const zf *yf, zf *const *ys) {
+  constexpr int PER = kMxCols * 16 / 256;
+  __shared__ float4 xsl[kMxCols * 16];
+  const int b = row_box[blockIdx.x];
+  const int pf = box_first[b], nrow = box_count[b];
+  const int c0 = e2_off[b], cols = e2_off[b + 1] - c0;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int m = lane & 15, g = lane >> 4;
+  const int rw = row_r0[blockIdx.x];  // first row of the tile (rw < nrow)
+  const int row = min(rw + m, nrow - 1);
+  const float4 *Zr = reinterpret_cast<const float4 *>(blocks + 4 * (near_off[b] + (int64_t)row * cols));
+  mxf4 er = {0.f, 0.f, 0.f, 0.f}, ei = er, hr = er, hi = er;
+  float4 za[4], zb[4];
+  auto load = [&](int cg, float4 *a, float4 *bb) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int c = min(cg + 4 * j + g, cols - 1);
+      a[j] = Zr[2 * c];
+      bb[j] = Zr[2 * c + 1];
+    }
+  };
+  if (cols > 0) load(16 * wv, za, zb);
+  for (int cc = 0; cc < cols; cc += kMxCols) {
+    const int nc = min(kMxCols, cols - cc);
+    __syncthreads();
+    {
+      // unconditional (clamped) loads so that all of them are in flight
+      // together; XI is zero for right-hand sides R..15
+      const int n = threadIdx.x & 15;
+      int e[PER];
+#pragma unroll
+      for (int k = 0; k < PER; ++k)
+        e[k] = e2_list[c0 + cc + min((int)(threadIdx.x + 256 * k) >> 4, nc - 1)];
+#pragma unroll
+      for (int k = 0; k < PER; ++k) {
+        const zf xe = XI[((size_t)e[k] * 2 + 0) * 16 + n], xh = XI[((size_t)e[k] * 2 + 1) * 16 + n];
+        const bool ok = ((int)(threadIdx.x + 256 * k) >> 4) < nc;
+        xsl[threadIdx.x + 256 * k] = ok ? float4{xe.x, xe.y, xh.x, xh.y} : float4{0.f, 0.f, 0.f, 0.f};
+      }
+    }
+    __syncthreads();
+    for (int off = 16 * wv; off < nc; off += 64) {
+      float4 na[4], nb[4];
+      load(cc + off + 64, na, nb);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const float4 x = xsl[(off + 4 * j + g) * 16 + m];
+        const float4 z = za[j], w = zb[j];
+        const float nxi = -x.y, nhi = -x.w;
+        er = SPK_MFMA(z.x, x.x, er);
+        ei = SPK_MFMA(z.x, x.y, ei);
+        hr = SPK_MFMA(w.x, x.x, hr);
+        hi = SPK_MFMA(w.x, x.y, hi);
+        er = SPK_MFMA(z.y, nxi, er);
+        ei = SPK_MFMA(z.y, x.x, ei);
+        hr = SPK_MFMA(w.y, nxi, hr);
+        hi = SPK_MFMA(w.y, x.x, hi);
+        er = SPK_MFMA(z.z, x.z, er);
+        ei = SPK_MFMA(z.z, x.w, ei);
+        hr = SPK_MFMA(w.z, x.z, hr);
+        hi = SPK_MFMA(w.z, x.w, hi);
+        er = SPK_MFMA(z.w, nhi, er);
+        ei = SPK_MFMA(z.w, x.z, ei);
+        hr = SPK_MFMA(w.w, nhi, hr);
+        hi = SPK_MFMA(w.w, x.z, hi);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        za[j] = na[j];
+        zb[j] = nb[j];
+      }
+    }
+  }
+  // waves 1..3 hand their sums to wave 0 through LDS
+  __syncthreads();
+  mxf4 *red = reinterpret_cast<mxf4 *>(xsl);
+  if (wv > 0) {
+    mxf4 *o = red + ((size_t)(wv - 1) * 64 + lane) * 4;
+    o[0] = er;
+    o[1] = ei;
+    o[2] = hr;
+    o[3] = hi;
+  }
+  __syncthreads();
+  if (wv > 0) return;
+#pragma unroll
+  for (int p = 0; p < 3; ++p) {
+    const mxf4 *o = red + ((size_t)p * 64 + lane) * 4;
+    er += o[0];
+    ei += o[1];
+    hr += o[2];
+    hi += o[3];
+  }
+  // accumulator element i of lane (g, n) is row 4g + i of the tile, right-hand side n
+  const int n = m;
+  if (n >= R) return;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int r = rw + 4 * g + i;
+    if (r < nrow) {
+      const int mm = perm[pf + r];
+      zf se = {er[i], ei[i]}, sh = {hr[i], hi[i]};
+      if (yf) {
+        se = se + yf[(size_t)n * 2 * N + mm];
+        sh = sh + yf[(size_t)n * 2 * N + mm + N];
+      }
+      ys[n][mm] = se;
+      ys[n][mm + N] = sh;
     }
   }
 }
@@ -495,6 +871,12 @@ void launch_batch_agg(int nbl, int S, int N, int R, const int *box_first, const 
   (void)XI;
   (void)perm;
   const dim3 g(bnblk(S, 256), nbl, 4);
+  if constexpr (std::is_same<T, zf>::value) {
+    if (batch_pad(R) == 16) {
+      hipLaunchKernelGGL(kb_agg_mx, g, dim3(256), 0, st, S, R, box_first, box_count, rt, XP, C);
+      return;
+    }
+  }
   switch (batch_pad(R)) {
 #define SPK_AGG_CASE(RP)                                                                       \
   case RP:                                                                                     \
@@ -549,12 +931,22 @@ __global__ void kb_sum_split(int64_t len, const T *yp, T *yf) {
 template <class T>
 void launch_batch_disagg(int ntiles, int N, int S, int R, const int *tile_box, const int *tile_p0,
                          const int *box_first, const int *box_count, const int *perm, const T *rt,
-                         const T *W, zd sc1, zd sc2, T *yp, T *yf, hipStream_t st) {
+                         const T *W, zd sc1, zd sc2, T *yp, T *yf, int mx_tiles,
+                         const int *mx_box, const int *mx_p0, hipStream_t st) {
   if (ntiles <= 0) return;
   // samples split over kBatchSplit workgroups per tile (multiples of 16 samples)
   const int sblk = (int)bnblk(bnblk(S, 16), kBatchSplit) * 16;
   const dim3 g(ntiles, kBatchSplit);
-  switch (batch_pad(R)) {
+  bool done = false;
+  if constexpr (std::is_same<T, zf>::value) {
+    if (batch_pad(R) == 16 && mx_tiles > 0) {
+      hipLaunchKernelGGL(kb_disagg_mx, dim3(mx_tiles, kBatchSplit), dim3(512), 0, st, N, S, R,
+                         mx_box, mx_p0, box_first, box_count, perm, rt, W, cvt<T>(sc1),
+                         cvt<T>(sc2), sblk, yp);
+      done = true;
+    }
+  }
+  if (!done) switch (batch_pad(R)) {
 #define SPK_DIS_CASE(RP)                                                                        \
   case RP:                                                                                      \
     hipLaunchKernelGGL((kb_disagg<T, RP>), g, dim3(16 * RP), 0, st, N, S, R, tile_box, tile_p0, \
@@ -574,6 +966,13 @@ void launch_batch_near(int nwg, int N, int R, const int *row_box, const int *row
                        const T *blocks, const T *XI, const T *yf, T *const *ys,
                        hipStream_t st) {
   if (nwg <= 0) return;
+  if constexpr (std::is_same<T, zf>::value) {
+    if (batch_pad(R) == 16) {
+      hipLaunchKernelGGL(kb_near_mx, dim3(nwg), dim3(256), 0, st, N, R, row_box, row_r0,
+                         box_first, box_count, e2_off, e2_list, near_off, perm, blocks, XI, yf, ys);
+      return;
+    }
+  }
   switch (batch_pad(R)) {
 #define SPK_NEAR_CASE(RP)                                                                       \
   case RP:                                                                                      \
@@ -601,7 +1000,7 @@ void launch_batch_near(int nwg, int N, int R, const int *row_box, const int *row
                                      hipStream_t);                                                 \
   template void launch_batch_disagg<T>(int, int, int, int, const int *, const int *, const int *,  \
                                        const int *, const int *, const T *, const T *, zd, zd,     \
-                                       T *, T *, hipStream_t);                                     \
+                                       T *, T *, int, const int *, const int *, hipStream_t);      \
   template void launch_batch_near<T>(int, int, int, const int *, const int *, const int *,         \
                                      const int *, const int *, const int *, const int64_t *,       \
                                      const int *, const T *, const T *, const T *, T *const *,     \

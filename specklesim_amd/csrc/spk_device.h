@@ -227,12 +227,16 @@ void launch_batch_rr(int nb, int S, int R, const int *parent, const int *octant,
 template <class T>
 void launch_batch_fold(int nbl, int S, int R, const T *D, zd kd1, zd kd2, zd my1, zd my2, zd ep1,
                        zd ep2, T *W, hipStream_t st);
-// tiles of up to kBatchTile consecutive edges of one leaf box
+// tiles of up to kBatchTile consecutive edges of one leaf box (mx tiles: up
+// to 128, for complex64 with 9..16 right-hand sides on the matrix cores)
 // yp: kBatchSplit partial slices of R x 2N, summed into yf
 template <class T>
 void launch_batch_disagg(int ntiles, int N, int S, int R, const int *tile_box, const int *tile_p0,
                          const int *box_first, const int *box_count, const int *perm, const T *rt,
-                         const T *W, zd sc1, zd sc2, T *yp, T *yf, hipStream_t st);
+                         const T *W, zd sc1, zd sc2, T *yp, T *yf, int mx_tiles,
+                         const int *mx_box, const int *mx_p0, hipStream_t st);
+// tiles (box, first row) of kSpmvRowsPerWG rows; complex64 with 9..16
+// right-hand sides runs on the matrix cores
 template <class T>
 void launch_batch_near(int nwg, int N, int R, const int *row_box, const int *row_r0,
                        const int *box_first, const int *box_count, const int *e2_off,

@@ -4,7 +4,8 @@ another order, and the far field amplifies rounding by the cancellation of
 the K = 19 plane-wave sum at the 0.31-0.62 lambda leaves (about 1e4 here):
 complex128 agrees to 1e-10 relative L2 per vector (1.3e-12 measured),
 complex64 to 3e-3 against the FP64 operator, as the single complex64
-operator does."""
+operator does.  Complex64 batches of 9..16 run the matrix-core kernels
+(near field, aggregation, disaggregation); smaller ones the vector kernels."""
 import numpy as np
 import pytest
 
@@ -34,7 +35,7 @@ def test_batched_matvec_fp64(solver, R):
         assert rel(Y[r], solver.calculate_vector_b_tri(X[r])) < 1e-10, r
 
 
-@pytest.mark.parametrize("R", [2, 16])
+@pytest.mark.parametrize("R", [2, 11, 16])
 def test_batched_matvec_c64(solver, R):
     rng = np.random.default_rng(20 + R)
     X = (rng.standard_normal((R, 2 * solver.N)) + 1j * rng.standard_normal((R, 2 * solver.N))
@@ -84,3 +85,26 @@ def test_batched_mixed_solve(solver):
         _, I0 = farfield.brdf(solver, x1, np.deg2rad(angles[r]), (-1.2, 1.2), 41, 0.0, 1e3 * LAM)
         _, I1 = farfield.brdf(solver, x, np.deg2rad(angles[r]), (-1.2, 1.2), 41, 0.0, 1e3 * LAM)
         assert np.max(np.abs(I1 - I0)) / np.max(np.abs(I0)) < 1e-4, r
+
+
+def test_batched_mixed_solve_wide(solver):
+    """Twelve incidence angles (the matrix-core batch width) reach the FP64
+    tolerance, each in its own FP64 residual."""
+    from specklesim_amd import DeviceBuffer, plane_wave
+
+    angles = np.linspace(0.0, 55.0, 12)
+    nb = 2 * solver.N * 16
+    bs, xs, rhs = [], [], []
+    for a in angles:
+        k, e = plane_wave(np.deg2rad(a), 0.0, 2, "TM")
+        b = solver.get_vector_b_tri(k, e)
+        rhs.append(b)
+        bs.append(DeviceBuffer(nb).upload(b))
+        xs.append(DeviceBuffer(nb))
+    infos = solver.gmres_run_batch_device([d.ptr for d in bs], [d.ptr for d in xs],
+                                          max_iterations=1300, tolerance=1e-6,
+                                          use_initial_guess=False)
+    for r, (b, inf) in enumerate(zip(rhs, infos)):
+        assert inf["info"] == 0 and inf["be"] <= 1e-6, (r, inf)
+        res = np.linalg.norm(b - solver.calculate_vector_b_tri(xs[r].download())) / np.linalg.norm(b)
+        assert res <= 1e-6 and abs(res - inf["be"]) < 1e-9, (r, res, inf["be"])
