@@ -70,13 +70,24 @@ __global__ void __launch_bounds__(256) kb_agg(int S, int N, int R, const int *bo
     for (int i = threadIdx.x; i < ne * 2 * RP; i += 256) xl[i] = XP[(size_t)(p0 + e0) * 2 * RP + i];
     __syncthreads();
     if (s < S) {
-      for (int e = 0; e < ne; ++e) {
-        const T f = conj(rt[((size_t)(p0 + e0 + e) * 4 + q) * S + s]);
-        const T *xe = xl + e * 2 * RP;
+      for (int g0 = 0; g0 < ne; g0 += kSumBlock<T>) {
+        T ge[RP], gh[RP];
+#pragma unroll
+        for (int r = 0; r < RP; ++r) ge[r] = gh[r] = T{0, 0};
+        const int g1 = min(ne, g0 + kSumBlock<T>);
+        for (int e = g0; e < g1; ++e) {
+          const T f = conj(rt[((size_t)(p0 + e0 + e) * 4 + q) * S + s]);
+          const T *xe = xl + e * 2 * RP;
+#pragma unroll
+          for (int r = 0; r < RP; ++r) {
+            ge[r] = ge[r] + f * xe[r];
+            gh[r] = gh[r] + f * xe[RP + r];
+          }
+        }
 #pragma unroll
         for (int r = 0; r < RP; ++r) {
-          ae[r] = ae[r] + f * xe[r];
-          ah[r] = ah[r] + f * xe[RP + r];
+          ae[r] = ae[r] + ge[r];
+          ah[r] = ah[r] + gh[r];
         }
       }
     }
@@ -248,12 +259,20 @@ __global__ void kb_m2l(int S, int R, int nb, int gx, const int *i4_off, const in
   T acc[8];
   for (int it = 0; it < 8; ++it) acc[it] = T{0, 0};
   const int q0 = i4_off[a], q1 = i4_off[a + 1];
-  for (int q = q0; q < q1; ++q) {
-    const int b = i4_box[q], slot = i4_slot[q];
-    const T t0 = tlt[((size_t)slot * 2 + 0) * S + s], t1 = tlt[((size_t)slot * 2 + 1) * S + s];
-    const T *src = B + (size_t)b * 8 * SR;
+  for (int g0 = q0; g0 < q1; g0 += kSumBlock<T>) {
+    T gs[8];
 #pragma unroll
-    for (int it = 0; it < 8; ++it) acc[it] = acc[it] + src[(size_t)it * SR + u] * ((it & 1) ? t1 : t0);
+    for (int it = 0; it < 8; ++it) gs[it] = T{0, 0};
+    const int g1 = min(q1, g0 + kSumBlock<T>);
+    for (int q = g0; q < g1; ++q) {
+      const int b = i4_box[q], slot = i4_slot[q];
+      const T t0 = tlt[((size_t)slot * 2 + 0) * S + s], t1 = tlt[((size_t)slot * 2 + 1) * S + s];
+      const T *src = B + (size_t)b * 8 * SR;
+#pragma unroll
+      for (int it = 0; it < 8; ++it) gs[it] = gs[it] + src[(size_t)it * SR + u] * ((it & 1) ? t1 : t0);
+    }
+#pragma unroll
+    for (int it = 0; it < 8; ++it) acc[it] = acc[it] + gs[it];
   }
   T *dst = Dt + (size_t)a * 8 * SR;
   for (int it = 0; it < 8; ++it) dst[(size_t)it * SR + u] = acc[it];
@@ -339,6 +358,12 @@ __global__ void __launch_bounds__(256) kb_disagg(int N, int S, int R, const int 
     }
     __syncthreads();
     if (eg * EPT < cnt) {
+      // complex64: the chunk's samples are summed from zero, then added
+      constexpr bool blk = kSumBlock<T> <= kBatchSC;
+      T ge[EPT], gh[EPT];
+#pragma unroll
+      for (int e = 0; e < EPT; ++e) ge[e] = gh[e] = T{0, 0};
+      T *te = blk ? ge : se, *th = blk ? gh : sh;
 #pragma unroll
       for (int k = 0; k < 2; ++k) {
         const T *w1a = wl + (k * 4 + 0) * kBatchSC * RP + r, *w1b = wl + (k * 4 + 1) * kBatchSC * RP + r;
@@ -350,9 +375,16 @@ __global__ void __launch_bounds__(256) kb_disagg(int N, int S, int R, const int 
           for (int e = 0; e < EPT; ++e) {
             const T *f = fl + ((eg * EPT + e) * 4 + k * 2) * kBatchSC;
             const T a = f[i], bb = f[kBatchSC + i];
-            se[e] = se[e] + a * wa + bb * wb;
-            sh[e] = sh[e] + a * wc + bb * wd;
+            te[e] = te[e] + a * wa + bb * wb;
+            th[e] = th[e] + a * wc + bb * wd;
           }
+        }
+      }
+      if constexpr (blk) {
+#pragma unroll
+        for (int e = 0; e < EPT; ++e) {
+          se[e] = se[e] + ge[e];
+          sh[e] = sh[e] + gh[e];
         }
       }
     }

@@ -128,6 +128,16 @@ void launch_to_zf(int64_t n, const zd *a, zf *b, hipStream_t st);
 void launch_tl_level(int S, int K, const double *khat, const double *trf, const double *rhat,
                      const zd *coef, const int *live, zd *tlt, hipStream_t st);
 constexpr int kSpmvRowsPerWG = 16;
+// Complex64 sums of many terms (aggregation over a box's edges, M2L over
+// the I4 list, disaggregation over samples) add each group of kSumBlock
+// terms from zero before adding it to the running sum: the far field
+// amplifies the rounding of these sums, and the running sum then takes one
+// rounding per group instead of one per term.  Complex128 keeps the plain
+// sequential order (one group).
+template <class T>
+inline constexpr int kSumBlock = 1 << 30;
+template <>
+inline constexpr int kSumBlock<zf> = 16;
 // edges per disaggregation task (one wavefront, one leaf box)
 constexpr int kDisaggEdges = 4;
 // right-hand sides per batched matvec launch; edges per batched

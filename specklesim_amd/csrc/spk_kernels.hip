@@ -826,14 +826,20 @@ __global__ void k_agg(int S, int N, const int *box_first, const int *box_count, 
   const int k = q >> 1, comp = q & 1;
   T ae = {0, 0}, ah = {0, 0};
   const int p0 = box_first[b], cnt = box_count[b];
+  for (int e0 = 0; e0 < cnt; e0 += kSumBlock<T>) {
+    T ge = {0, 0}, gh = {0, 0};
+    const int e1 = min(cnt, e0 + kSumBlock<T>);
 #pragma unroll 4
-  for (int e = 0; e < cnt; ++e) {
-    const int p = p0 + e;
-    const int m = perm[p];
-    const T f = conj(rt[((size_t)p * 4 + q) * S + s]);
-    const T xe = x[m], xh = x[m + N];
-    ae = ae + f * xe;
-    ah = ah + f * xh;
+    for (int e = e0; e < e1; ++e) {
+      const int p = p0 + e;
+      const int m = perm[p];
+      const T f = conj(rt[((size_t)p * 4 + q) * S + s]);
+      const T xe = x[m], xh = x[m + N];
+      ge = ge + f * xe;
+      gh = gh + f * xh;
+    }
+    ae = ae + ge;
+    ah = ah + gh;
   }
   T *Cb = C + (size_t)b * 8 * S;
   Cb[(size_t)(comp * 4 + 0 * 2 + k) * S + s] = ae;
@@ -884,12 +890,20 @@ __global__ void k_m2l(int S, const int *i4_off, const int *i4_box, const int *i4
   T acc[8];
   for (int it = 0; it < 8; ++it) acc[it] = T{0, 0};
   const int q0 = i4_off[a], q1 = i4_off[a + 1];
-  for (int q = q0; q < q1; ++q) {
-    const int b = i4_box[q], slot = i4_slot[q];
-    const T t0 = tlt[((size_t)slot * 2 + 0) * S + s], t1 = tlt[((size_t)slot * 2 + 1) * S + s];
-    const T *src = B + (size_t)b * 8 * S;
+  for (int g0 = q0; g0 < q1; g0 += kSumBlock<T>) {
+    T gs[8];
 #pragma unroll
-    for (int it = 0; it < 8; ++it) acc[it] = acc[it] + src[(size_t)it * S + s] * ((it & 1) ? t1 : t0);
+    for (int it = 0; it < 8; ++it) gs[it] = T{0, 0};
+    const int g1 = min(q1, g0 + kSumBlock<T>);
+    for (int q = g0; q < g1; ++q) {
+      const int b = i4_box[q], slot = i4_slot[q];
+      const T t0 = tlt[((size_t)slot * 2 + 0) * S + s], t1 = tlt[((size_t)slot * 2 + 1) * S + s];
+      const T *src = B + (size_t)b * 8 * S;
+#pragma unroll
+      for (int it = 0; it < 8; ++it) gs[it] = gs[it] + src[(size_t)it * S + s] * ((it & 1) ? t1 : t0);
+    }
+#pragma unroll
+    for (int it = 0; it < 8; ++it) acc[it] = acc[it] + gs[it];
   }
   T *dst = Dt + (size_t)a * 8 * S;
   for (int it = 0; it < 8; ++it) dst[(size_t)it * S + s] = acc[it];
