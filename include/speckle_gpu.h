@@ -82,10 +82,15 @@ typedef struct spk_gmres_params {
   int preconditioning;           /* icntl(4): SPK_PRECOND_NONE or SPK_PRECOND_LEFT_JACOBI
                                     (precondition_left, src/lib_sie_solver_gmres_interface.f90:115;
                                     diagonal of the self blocks, lib_sie_tri_mlfmm_interface.f90:94-109);
-                                    FP64 only */
+                                    FP64 only.  SPK_PRECOND_NEAR_FIELD (no counterpart in the
+                                    reference): flexible GMRES on the FP64 operator, right-
+                                    preconditioned by inner_iterations GMRES steps on the
+                                    near-field operator (complex64 on an SPK_MIXED handle) */
+  int inner_iterations;          /* SPK_PRECOND_NEAR_FIELD: inner steps per outer step (0: 20) */
 } spk_gmres_params;
 #define SPK_PRECOND_NONE 0
 #define SPK_PRECOND_LEFT_JACOBI 1
+#define SPK_PRECOND_NEAR_FIELD 2
 
 typedef struct spk_gmres_info {
   int info;                      /* info(1): 0 converged, -4 not converged */

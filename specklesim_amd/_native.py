@@ -48,7 +48,7 @@ class GmresParams(C.Structure):
     _fields_ = [("max_iterations", C.c_int), ("restart", C.c_int), ("tolerance", C.c_double),
                 ("orthogonalization", C.c_int), ("use_initial_guess", C.c_int),
                 ("refinement_steps", C.c_int), ("refinement_tolerance", C.c_double),
-                ("preconditioning", C.c_int)]
+                ("preconditioning", C.c_int), ("inner_iterations", C.c_int)]
 
 
 class GmresInfo(C.Structure):

@@ -440,8 +440,9 @@ static int batch_ensure(spk_handle *h, int R) {
 }
 
 // ys[r] = Z xs[r] for r < R <= kBatchMax; xs, ys: device arrays of R pointers.
+// far_on = false: the near-field part only.
 template <class T>
-int matvec_batch_t(spk_handle *h, int R, const T *const *xs, T *const *ys) {
+int matvec_batch_t(spk_handle *h, int R, const T *const *xs, T *const *ys, bool far_on = true) {
   T *tag = nullptr;
   if (R < 1 || R > kBatchMax) return fail(SPK_ERR_ARG, "batch size must be 1..16");
   int rc = batch_ensure<T>(h, R);
@@ -452,6 +453,13 @@ int matvec_batch_t(spk_handle *h, int R, const T *const *xs, T *const *ys) {
   hipStream_t st = h->st;
   LevelDev &LL = h->lev[lmax];
   launch_batch_interleave(N, R, h->d_perm.p, xs, BB.XI.p, BB.XP.p, st);
+  if (!far_on) {
+    launch_batch_near(h->spmv_wgs, N, R, h->d_row_box.p, h->d_row_r0.p, h->d_box_first.p,
+                      h->d_box_count.p, h->d_e2_off.p, h->d_e2_list.p, h->d_near_off.p, h->d_perm.p,
+                      pBlk(h, tag), (const T *)BB.XI.p, (const T *)nullptr, ys, st);
+    HIPCHK(hipGetLastError());
+    return SPK_OK;
+  }
   launch_batch_agg(LL.nb, S, N, R, h->d_box_first.p, h->d_box_count.p, h->d_perm.p, pRt(h, tag),
                    (const T *)BB.XI.p, (const T *)BB.XP.p, BB.B[lmax].p, st);
   for (int l = lmax - 1; l >= lmin; --l)
@@ -1425,15 +1433,379 @@ static int gmres_mixed_batch(spk_handle *h, int R, const zd *const *b, zd *const
 
 extern "C" {
 
+extern "C++" {
+// Flexible GMRES on the FP64 operator for R <= kBatchMax right-hand sides,
+// right-preconditioned by kin GMRES steps (from zero) on the near-field
+// operator: complex64 on an SPK_MIXED handle (T = zf), FP64 otherwise.  The
+// near field holds most of the operator's conditioning, so a short inner
+// solve with it replaces most outer steps, each of which costs a full FP64
+// product (near field plus the MLFMM far field).
+//  - Inner: the active right-hand sides' inner steps share one (batched)
+//    near-field product; their Arnoldi runs on the device (two classical
+//    Gram-Schmidt passes, coefficients and norms stay in HBM) and the host
+//    solves the small least-squares problems once per outer step.
+//  - Outer: z_r = U_r y_r is kept, w = Z z_r in FP64, ICGS with the CERFACS
+//    reorthogonalisation test, Givens rotations on the host; the Arnoldi
+//    backward error is the stopping test, confirmed by the true residual
+//    b - Z x (a new cycle starts from it otherwise, as after m steps).
+template <class T>
+static int fgmres_near(spk_handle *h, int R, const zd *const *b, zd *const *x,
+                       const spk_gmres_params *gp, spk_gmres_info *info) {
+  if (R < 1 || R > kBatchMax) return fail(SPK_ERR_ARG, "batch size must be 1..16");
+  if (std::is_same<T, zf>::value && !h->mixed_ready)
+    return fail(SPK_ERR_STATE, "handle was not created with SPK_MIXED");
+  const double t_start = now_ms();
+  const int64_t n = 2 * (int64_t)h->N;
+  const int max_it = std::max(1, gp->max_iterations);
+  const int m = std::min(max_it, gp->restart > 0 ? gp->restart : 100);
+  const int kin = gp->inner_iterations > 0 ? std::min(gp->inner_iterations, 64) : 20;
+  const double tol = gp->tolerance;
+  const size_t ldh = (size_t)m + 2, ldhi = (size_t)kin + 2, hik = (size_t)(kin + 1) * kin;
+  const size_t ldu = (size_t)(kin + 1) * n;
+  DBuf<zd> V, Z, W, Hd, Rv, Tv, Dv, Yo, Yi, h1, h2, Hi, dots;
+  DBuf<T> U, Wi;
+  HIPCHK(V.alloc((size_t)R * (m + 1) * n));
+  HIPCHK(Z.alloc((size_t)R * m * n));
+  HIPCHK(W.alloc((size_t)R * n));
+  HIPCHK(Rv.alloc((size_t)R * n));
+  HIPCHK(Tv.alloc((size_t)R * n));
+  HIPCHK(Dv.alloc(n));
+  HIPCHK(Hd.alloc((size_t)R * ldh));
+  HIPCHK(Yo.alloc(ldh));
+  HIPCHK(Yi.alloc((size_t)R * kin));
+  HIPCHK(U.alloc((size_t)R * ldu));
+  HIPCHK(Wi.alloc((size_t)R * n));
+  HIPCHK(h1.alloc((size_t)R * ldhi));
+  HIPCHK(h2.alloc((size_t)R * ldhi));
+  HIPCHK(Hi.alloc((size_t)R * hik));
+  HIPCHK(dots.alloc(R));
+  const int kmax = std::max(m, kin) + 1;
+  DBuf<zd> part;
+  HIPCHK(part.alloc(std::max(batch_orth_partial_elems(n, R, kmax), (size_t)reduce_blocks(n))));
+  HIPCHK(h->g_pn.alloc(gemv_n_partial_elems(n, kmax)));
+  HIPCHK(h->pin.ensure(std::max((size_t)R * ldh, (size_t)R * hik)));
+  // pointer tables: inner slots s -> U_s, W_s, h1_s, h2_s, the inner
+  // products' inputs U_s[:, j] and ks[j][s] = j + 1 (fixed for the solve);
+  // outer: V_r, W_r, H_r, k_r and the products' inputs / outputs (per step)
+  constexpr int B = kBatchMax;
+  // (separate arrays: a table is rewritten only after the kernels that
+  // read its last contents have completed)
+  DBuf<uint64_t> tin, tprod, torth;
+  DBuf<int> kin_tab, kout;
+  HIPCHK(tin.alloc((size_t)(4 + kin) * B));
+  HIPCHK(kin_tab.alloc((size_t)kin * B));
+  HIPCHK(tprod.alloc(2 * (size_t)B));
+  HIPCHK(torth.alloc(3 * (size_t)B));
+  HIPCHK(kout.alloc(B));
+  {
+    std::vector<uint64_t> t((size_t)(4 + kin) * B, 0);
+    std::vector<int> kt((size_t)kin * B);
+    for (int s = 0; s < B; ++s) {
+      t[s] = (uint64_t)(uintptr_t)(U.p + (s < R ? s * ldu : 0));
+      t[B + s] = (uint64_t)(uintptr_t)(Wi.p + (size_t)(s < R ? s : 0) * n);
+      t[2 * B + s] = (uint64_t)(uintptr_t)(h1.p + (size_t)(s < R ? s : 0) * ldhi);
+      t[3 * B + s] = (uint64_t)(uintptr_t)(h2.p + (size_t)(s < R ? s : 0) * ldhi);
+      for (int j = 0; j < kin; ++j) {
+        t[(size_t)(4 + j) * B + s] = (uint64_t)(uintptr_t)(U.p + (s < R ? s * ldu : 0) + (size_t)j * n);
+        kt[(size_t)j * B + s] = j + 1;
+      }
+    }
+    HIPCHK(hipMemcpy(tin.p, t.data(), t.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(kin_tab.p, kt.data(), kt.size() * sizeof(int), hipMemcpyHostToDevice));
+  }
+  const T *const *tU = (const T *const *)tin.p;
+  T *const *tW = (T *const *)(tin.p + B);
+  zd *const *tH1 = (zd *const *)(tin.p + 2 * B);
+  zd *const *tH2 = (zd *const *)(tin.p + 3 * B);
+  auto Vr = [&](int r) { return V.p + (size_t)r * (m + 1) * n; };
+  auto Zr = [&](int r) { return Z.p + (size_t)r * m * n; };
+  auto Wr = [&](int r) { return W.p + (size_t)r * n; };
+  auto Rr = [&](int r) { return Rv.p + (size_t)r * n; };
+  auto Tr = [&](int r) { return Tv.p + (size_t)r * n; };
+  std::vector<RhsState> S(R);
+  for (auto &st : S) {
+    st.H.assign((size_t)(m + 1) * (m + 1), 0.0);
+    st.g.assign(m + 1, 0.0);
+    st.rs.assign(m + 1, 0.0);
+    st.yc.assign(m + 1, 0.0);
+    st.rcs.assign(m + 1, 0.0);
+  }
+  if (R == 1) h->hist.clear();
+  // |u|^2 for a list of right-hand sides, one transfer
+  auto norms2 = [&](const std::vector<int> &rr, auto vec_of, std::vector<double> *out) -> int {
+    for (size_t i = 0; i < rr.size(); ++i) {
+      const zd *u = vec_of(rr[i]);
+      launch_zdotc(n, u, u, part.p, dots.p + i, h->st);
+    }
+    HIPCHK(hipMemcpyAsync(h->pin.p, dots.p, rr.size() * sizeof(zd), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    out->resize(rr.size());
+    for (size_t i = 0; i < rr.size(); ++i) (*out)[i] = std::max(h->pin.p[i].x, 0.0);
+    return SPK_OK;
+  };
+  // FP64 products ys[i] = Z xs[i]: batched from 4 vectors up
+  auto products = [&](const std::vector<const zd *> &xs, const std::vector<zd *> &ys) -> int {
+    if (xs.size() < 4) {
+      for (size_t i = 0; i < xs.size(); ++i) {
+        const int e = matvec_t<zd>(h, xs[i], ys[i]);
+        if (e) return e;
+      }
+      return SPK_OK;
+    }
+    // the previous product on these tables has completed (a synchronisation follows each use)
+    HIPCHK(hipMemcpy(tprod.p, xs.data(), xs.size() * sizeof(void *), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(tprod.p + B, ys.data(), ys.size() * sizeof(void *), hipMemcpyHostToDevice));
+    return matvec_batch_t<zd>(h, (int)xs.size(), (const zd *const *)tprod.p, (zd *const *)(tprod.p + B));
+  };
+  int rc = SPK_OK;
+  std::vector<int> all(R);
+  for (int r = 0; r < R; ++r) all[r] = r;
+  std::vector<double> nv;
+  if ((rc = norms2(all, [&](int r) { return b[r]; }, &nv))) return rc;
+  for (int r = 0; r < R; ++r) {
+    S[r].bn = std::sqrt(nv[r]);
+    if (!gp->use_initial_guess) HIPCHK(hipMemsetAsync(x[r], 0, n * sizeof(zd), h->st));
+  }
+  bool first = true;
+  // true residuals r = b - Z x of a set; each finishes or starts a cycle
+  auto residuals = [&](const std::vector<int> &set) -> int {
+    if (set.empty()) return SPK_OK;
+    int e = SPK_OK;
+    if (first && !gp->use_initial_guess) {
+      for (int r : set) HIPCHK(hipMemcpyAsync(Rr(r), b[r], n * sizeof(zd), hipMemcpyDeviceToDevice, h->st));
+    } else {
+      std::vector<const zd *> xs;
+      std::vector<zd *> ys;
+      for (int r : set) {
+        xs.push_back(x[r]);
+        ys.push_back(Tr(r));
+      }
+      if ((e = products(xs, ys))) return e;
+      for (int r : set) {
+        launch_zsub(n, b[r], Tr(r), Rr(r), h->st);
+        ++S[r].matvecs;
+      }
+    }
+    first = false;
+    std::vector<double> rr2;
+    if ((e = norms2(set, Rr, &rr2))) return e;
+    for (size_t i = 0; i < set.size(); ++i) {
+      const int r = set[i];
+      RhsState &st = S[r];
+      st.rn = std::sqrt(rr2[i]);
+      st.be = st.bn > 0 ? st.rn / st.bn : 0.0;
+      if (st.outer == 0) st.bea = st.be;
+      if (st.bn == 0.0 || st.be <= tol || st.total >= max_it) {
+        st.phase = 2;
+        continue;
+      }
+      st.phase = 0;
+      launch_zscal_copy(n, zd{1.0 / st.rn, 0.0}, Rr(r), Vr(r), h->st);
+      std::fill(st.H.begin(), st.H.end(), zc(0.0));
+      std::fill(st.g.begin(), st.g.end(), zc(0.0));
+      st.g[0] = st.rn;
+      st.beta = st.rn;
+      st.j = 0;
+    }
+    return SPK_OK;
+  };
+  if ((rc = residuals(all))) return rc;
+  std::vector<int> act;
+  for (;;) {
+    act.clear();
+    for (int r = 0; r < R; ++r)
+      if (S[r].phase == 0) act.push_back(r);
+    if (act.empty()) break;
+    const int P = (int)act.size();
+    // ---- inner near-field solves: slot s works on right-hand side act[s]
+    for (int s = 0; s < P; ++s) {
+      const zd *v = Vr(act[s]) + (size_t)S[act[s]].j * n;
+      if constexpr (std::is_same<T, zf>::value)
+        launch_scale_down(n, 1.0, v, (zf *)(U.p + s * ldu), h->st);
+      else
+        HIPCHK(hipMemcpyAsync(U.p + s * ldu, v, n * sizeof(zd), hipMemcpyDeviceToDevice, h->st));
+    }
+    for (int j = 0; j < kin; ++j) {
+      if (P < 4) {
+        for (int s = 0; s < P && !rc; ++s)
+          rc = matvec_t<T>(h, U.p + s * ldu + (size_t)j * n, Wi.p + (size_t)s * n, true, false);
+      } else {
+        rc = matvec_batch_t<T>(h, P, (const T *const *)(tin.p + (size_t)(4 + j) * B), tW, false);
+      }
+      if (rc) return rc;
+      const int *ks = kin_tab.p + (size_t)j * B;
+      launch_batch_orth<T>(n, P, j + 1, ks, tU, tW, tH1, part.p, h->st);
+      launch_batch_orth<T>(n, P, j + 1, ks, tU, tW, tH2, part.p, h->st);
+      launch_fg_step<T>(n, P, j, kin, (int)ldhi, h1.p, h2.p, Wi.p, ldu, U.p, Hi.p, h->st);
+    }
+    HIPCHK(hipMemcpyAsync(h->pin.p, Hi.p, (size_t)P * hik * sizeof(zd), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    // least squares min |e_0 - H y| per slot (Givens), up to a breakdown
+    std::vector<zd> yall((size_t)R * kin, zd{0, 0});
+    std::vector<int> klen(P);
+    for (int s = 0; s < P; ++s) {
+      const zd *Hs = h->pin.p + (size_t)s * hik;
+      std::vector<zc> Hc((size_t)(kin + 1) * kin), gg(kin + 1, 0.0), yy(kin, 0.0);
+      for (size_t i = 0; i < Hc.size(); ++i) Hc[i] = zc(Hs[i].x, Hs[i].y);
+      auto HC = [&](int i, int c) -> zc & { return Hc[(size_t)c * (kin + 1) + i]; };
+      int kk = kin;
+      for (int c = 0; c < kin; ++c)
+        if (std::abs(HC(c + 1, c)) == 0.0) {
+          kk = c + 1;
+          break;
+        }
+      gg[0] = 1.0;
+      std::vector<double> cs(kk);
+      std::vector<zc> sn(kk);
+      for (int c = 0; c < kk; ++c) {
+        for (int i = 0; i < c; ++i) zrot1(&HC(i, c), &HC(i + 1, c), cs[i], sn[i]);
+        zc a = HC(c, c);
+        zrotg_(&a, HC(c + 1, c), &cs[c], &sn[c]);
+        zrot1(&gg[c], &gg[c + 1], cs[c], sn[c]);
+        zrot1(&HC(c, c), &HC(c + 1, c), cs[c], sn[c]);
+        HC(c + 1, c) = 0.0;
+      }
+      while (kk > 0 && std::abs(HC(kk - 1, kk - 1)) == 0.0) --kk;
+      for (int c = kk - 1; c >= 0; --c) {
+        zc sum = gg[c];
+        for (int i = c + 1; i < kk; ++i) sum -= HC(c, i) * yy[i];
+        yy[c] = sum / HC(c, c);
+      }
+      for (int c = 0; c < kk; ++c) yall[(size_t)s * kin + c] = zd{yy[c].real(), yy[c].imag()};
+      klen[s] = kk;
+    }
+    // from pinned memory, which is next written by a later copy on this stream
+    std::copy(yall.begin(), yall.begin() + (size_t)P * kin, h->pin.p);
+    HIPCHK(hipMemcpyAsync(Yi.p, h->pin.p, (size_t)P * kin * sizeof(zd), hipMemcpyHostToDevice, h->st));
+    std::vector<const zd *> zin;
+    std::vector<zd *> wout;
+    for (int s = 0; s < P; ++s) {
+      const int r = act[s];
+      zd *z = Zr(r) + (size_t)S[r].j * n;
+      if (klen[s] > 0)
+        launch_gemv_n(n, klen[s], (const T *)(U.p + s * ldu), n, Yi.p + (size_t)s * kin, z, h->g_pn.p,
+                      h->st);
+      else  // no usable inner step: the identity preconditioner
+        HIPCHK(hipMemcpyAsync(z, Vr(r) + (size_t)S[r].j * n, n * sizeof(zd), hipMemcpyDeviceToDevice,
+                              h->st));
+      zin.push_back(z);
+      wout.push_back(Wr(r));
+    }
+    // ---- outer step: w_r = Z z_r, ICGS against V_r
+    if ((rc = products(zin, wout))) return rc;
+    std::vector<int> pass = act;
+    std::vector<double> dnormw(R, 0.0);
+    for (int np_ = 1; !pass.empty() && np_ <= 3; ++np_) {
+      const int Pp = (int)pass.size();
+      std::vector<uint64_t> arg(3 * (size_t)B, 0);
+      std::vector<int> kk(Pp);
+      int maxk = 0;
+      for (int i = 0; i < Pp; ++i) {
+        const int r = pass[i];
+        kk[i] = S[r].j + 1;
+        maxk = std::max(maxk, kk[i]);
+        arg[i] = (uint64_t)(uintptr_t)Vr(r);
+        arg[B + i] = (uint64_t)(uintptr_t)Wr(r);
+        arg[2 * B + i] = (uint64_t)(uintptr_t)(Hd.p + (size_t)r * ldh);
+      }
+      HIPCHK(hipMemcpy(torth.p, arg.data(), 3 * B * sizeof(uint64_t), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(kout.p, kk.data(), Pp * sizeof(int), hipMemcpyHostToDevice));
+      launch_batch_orth<zd>(n, Pp, maxk, kout.p, (const zd *const *)torth.p, (zd *const *)(torth.p + B),
+                            (zd *const *)(torth.p + 2 * B), part.p, h->st);
+      HIPCHK(hipMemcpyAsync(h->pin.p, Hd.p, (size_t)R * ldh * sizeof(zd), hipMemcpyDeviceToHost, h->st));
+      HIPCHK(hipStreamSynchronize(h->st));
+      std::vector<int> again;
+      for (int r : pass) {
+        RhsState &st = S[r];
+        const int j = st.j;
+        const zd *hv = h->pin.p + (size_t)r * ldh;
+        double dloo = 0.0;
+        for (int i = 0; i <= j; ++i) {
+          const zc hi(hv[i].x, hv[i].y);
+          st.H[(size_t)j * (m + 1) + i] += hi;
+          dloo += std::norm(hi);
+        }
+        dnormw[r] = std::sqrt(std::max(hv[j + 1].x, 0.0));
+        if (dnormw[r] <= std::sqrt(dloo) && np_ < 3) again.push_back(r);
+      }
+      pass.swap(again);
+    }
+    std::vector<int> fin;
+    for (int r : act) {
+      RhsState &st = S[r];
+      const int j = st.j;
+      auto HH = [&](int i, int c) -> zc & { return st.H[(size_t)c * (m + 1) + i]; };
+      HH(j + 1, j) = dnormw[r];
+      if (dnormw[r] > 0.0) launch_zscal_copy(n, zd{1.0 / dnormw[r], 0.0}, Wr(r), Vr(r) + (size_t)(j + 1) * n, h->st);
+      for (int i = 0; i < j; ++i) zrot1(&HH(i, j), &HH(i + 1, j), st.rcs[i], st.rs[i]);
+      zc a = HH(j, j);
+      double cc;
+      zc ss;
+      zrotg_(&a, HH(j + 1, j), &cc, &ss);
+      st.rcs[j] = cc;
+      st.rs[j] = ss;
+      zrot1(&st.g[j], &st.g[j + 1], cc, ss);
+      zrot1(&HH(j, j), &HH(j + 1, j), cc, ss);
+      HH(j + 1, j) = 0.0;
+      st.est = std::abs(st.g[j + 1]) / st.bn;
+      st.bea = st.est;
+      ++st.total;
+      ++st.matvecs;
+      st.j = j + 1;
+      if (R == 1) h->hist.push_back(st.est);
+      if (st.est <= tol || dnormw[r] == 0.0 || st.j >= m || st.total >= max_it) fin.push_back(r);
+    }
+    // finished cycles: x += Z y, then their true residuals together
+    for (int r : fin) {
+      RhsState &st = S[r];
+      const int k = st.j;
+      auto HH = [&](int i, int c) -> zc & { return st.H[(size_t)c * (m + 1) + i]; };
+      for (int j = k - 1; j >= 0; --j) {
+        zc sum = st.g[j];
+        for (int i = j + 1; i < k; ++i) sum -= HH(j, i) * st.yc[i];
+        st.yc[j] = sum / HH(j, j);
+      }
+      std::vector<zd> yd(k);
+      for (int j = 0; j < k; ++j) yd[j] = zd{st.yc[j].real(), st.yc[j].imag()};
+      HIPCHK(hipMemcpyAsync(Yo.p, yd.data(), k * sizeof(zd), hipMemcpyHostToDevice, h->st));
+      HIPCHK(hipStreamSynchronize(h->st));
+      launch_gemv_n(n, k, (const zd *)Zr(r), n, Yo.p, Dv.p, h->g_pn.p, h->st);
+      launch_zaxpy(n, zd{1.0, 0.0}, Dv.p, x[r], h->st);
+      ++st.outer;
+      st.phase = 1;
+    }
+    if ((rc = residuals(fin))) return rc;
+  }
+  HIPCHK(hipStreamSynchronize(h->st));
+  HIPCHK(hipGetLastError());
+  const double secs = (now_ms() - t_start) / 1e3;
+  for (int r = 0; r < R; ++r) {
+    const RhsState &st = S[r];
+    info[r].info = st.be <= tol ? 0 : -4;
+    info[r].iterations = st.total;
+    info[r].arnoldi_backward_error = st.bea;
+    info[r].backward_error = st.be;
+    info[r].matvecs = st.matvecs;
+    info[r].seconds = secs;
+    info[r].true_backward_error = st.be;
+  }
+  return SPK_OK;
+}
+}  // extern "C++"
+
 int spk_gmres_batch_device(spk_handle *h, int R, const void *const *d_b, void *const *d_x,
                            const spk_gmres_params *gp, spk_gmres_info *info) {
   if (!h || !d_b || !d_x || !gp || !info) return fail(SPK_ERR_ARG, "null argument");
   if (!h->ready) return fail(SPK_ERR_STATE, "spk_setup_tri has not completed");
-  if (h->p.precision != SPK_MIXED)
-    return fail(SPK_ERR_STATE, "the batched solve runs on an SPK_MIXED handle");
   if (R < 1 || R > kBatchMax) return fail(SPK_ERR_ARG, "batch size must be 1..16");
   for (int r = 0; r < R; ++r)
     if (!d_b[r] || !d_x[r]) return fail(SPK_ERR_ARG, "null vector in the batch");
+  if (gp->preconditioning == SPK_PRECOND_NEAR_FIELD)
+    return h->p.precision == SPK_MIXED
+               ? fgmres_near<zf>(h, R, (const zd *const *)d_b, (zd *const *)d_x, gp, info)
+               : fgmres_near<zd>(h, R, (const zd *const *)d_b, (zd *const *)d_x, gp, info);
+  if (h->p.precision != SPK_MIXED)
+    return fail(SPK_ERR_STATE, "the batched refinement solve runs on an SPK_MIXED handle");
   return gmres_mixed_batch(h, R, (const zd *const *)d_b, (zd *const *)d_x, gp, info);
 }
 
@@ -1441,6 +1813,15 @@ int spk_gmres_device(spk_handle *h, const void *d_b, void *d_x, const spk_gmres_
                      spk_gmres_info *info) {
   if (!h || !d_b || !d_x || !gp) return fail(SPK_ERR_ARG, "null argument");
   if (!h->ready) return fail(SPK_ERR_STATE, "spk_setup_tri has not completed");
+  if (gp->preconditioning == SPK_PRECOND_NEAR_FIELD) {
+    const zd *bb[1] = {(const zd *)d_b};
+    zd *xx[1] = {(zd *)d_x};
+    spk_gmres_info tmp;
+    const int e = h->p.precision == SPK_MIXED ? fgmres_near<zf>(h, 1, bb, xx, gp, &tmp)
+                                              : fgmres_near<zd>(h, 1, bb, xx, gp, &tmp);
+    if (!e && info) *info = tmp;
+    return e;
+  }
   if (h->p.precision == SPK_MIXED) return gmres_mixed(h, (const zd *)d_b, (zd *)d_x, gp, info);
   const double t_start = now_ms();
   const int64_t n = 2 * (int64_t)h->N;

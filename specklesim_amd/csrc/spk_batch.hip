@@ -881,6 +881,42 @@ void launch_batch_orth(int64_t n, int P, int maxk, const int *ks, const T *const
                      (const zd *)partial, hs);
 }
 
+// ---------------------------------------------------------------------------
+// Device-side Arnoldi of the near-field inner solves (FGMRES preconditioner):
+// P slots, slot s holds its basis U_s = U + s * ldu (column j at j * n), its
+// work vector W + s * n, the two Gram-Schmidt passes' coefficients h1, h2 +
+// s * ldh (|w|^2 after the second pass in h2[j + 1]) and its Hessenberg
+// matrix H + s * (kin + 1) * kin, column-major with leading dimension kin + 1.
+// ---------------------------------------------------------------------------
+// H_s[:, j] = h1 + h2 (rows 0..j), H_s[j + 1, j] = |w|
+__global__ void k_fg_hcol(int j, int kin, int ldh, const zd *h1, const zd *h2, zd *H) {
+  const int s = blockIdx.x, i = threadIdx.x;
+  if (i > j + 1) return;
+  const zd *a = h1 + (size_t)s * ldh, *b = h2 + (size_t)s * ldh;
+  const zd v = i <= j ? a[i] + b[i] : zd{sqrt(fmax(b[j + 1].x, 0.0)), 0.0};
+  H[(size_t)s * (kin + 1) * kin + (size_t)j * (kin + 1) + i] = v;
+}
+// U_s[:, j + 1] = w_s / |w_s| (zero when |w_s| = 0)
+template <class T>
+__global__ void k_fg_scale(int64_t n, int j, int ldh, const T *W, const zd *h2, size_t ldu, T *U) {
+  const int s = blockIdx.y;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const double w2 = h2[(size_t)s * ldh + j + 1].x;
+  const double inv = w2 > 0.0 ? 1.0 / sqrt(w2) : 0.0;
+  const T v = W[(size_t)s * n + t];
+  using F = decltype(v.x);
+  U[(size_t)s * ldu + (size_t)(j + 1) * n + t] = T{(F)(v.x * inv), (F)(v.y * inv)};
+}
+template <class T>
+void launch_fg_step(int64_t n, int P, int j, int kin, int ldh, const zd *h1, const zd *h2,
+                    const T *W, size_t ldu, T *U, zd *H, hipStream_t st) {
+  if (P <= 0) return;
+  hipLaunchKernelGGL(k_fg_hcol, dim3(P), dim3(128), 0, st, j, kin, ldh, h1, h2, H);
+  hipLaunchKernelGGL(k_fg_scale<T>, dim3(bnblk(n, 256), P), dim3(256), 0, st, n, j, ldh, W, h2, ldu,
+                     U);
+}
+
 template <class T>
 void launch_batch_interleave(int N, int R, const int *perm, const T *const *xs, T *XI, T *XP,
                              hipStream_t st) {
@@ -1041,5 +1077,11 @@ SPK_BATCH_INSTANTIATE(zd)
 SPK_BATCH_INSTANTIATE(zf)
 template void launch_batch_orth<zf>(int64_t, int, int, const int *, const zf *const *, zf *const *,
                                     zd *const *, zd *, hipStream_t);
+template void launch_batch_orth<zd>(int64_t, int, int, const int *, const zd *const *, zd *const *,
+                                    zd *const *, zd *, hipStream_t);
+template void launch_fg_step<zf>(int64_t, int, int, int, int, const zd *, const zd *, const zf *,
+                                 size_t, zf *, zd *, hipStream_t);
+template void launch_fg_step<zd>(int64_t, int, int, int, int, const zd *, const zd *, const zd *,
+                                 size_t, zd *, zd *, hipStream_t);
 
 }  // namespace spk

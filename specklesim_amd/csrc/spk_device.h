@@ -260,6 +260,13 @@ template <class T>
 void launch_batch_orth(int64_t n, int P, int maxk, const int *ks, const T *const *Vs, T *const *ws,
                        zd *const *hs, zd *partial, hipStream_t st);
 size_t batch_orth_partial_elems(int64_t n, int P, int maxk);
+// one Arnoldi step of P near-field inner solves after their two
+// Gram-Schmidt passes: H_s[:, j] = h1 + h2, H_s[j+1, j] = |w_s| (|w|^2 in
+// h2[s * ldh + j + 1]), U_s[:, j+1] = w_s / |w_s|; slot s: U + s * ldu,
+// W + s * n, H + s * (kin + 1) * kin (column-major, leading dimension kin + 1)
+template <class T>
+void launch_fg_step(int64_t n, int P, int j, int kin, int ldh, const zd *h1, const zd *h2,
+                    const T *W, size_t ldu, T *U, zd *H, hipStream_t st);
 
 // streaming read of a buffer (bandwidth probe)
 void launch_stream_read(const void *a, size_t bytes, double *out, hipStream_t st);

@@ -27,7 +27,7 @@ module spk_gpu_binding
       real(c_double) :: tolerance
       integer(c_int) :: orthogonalization, use_initial_guess, refinement_steps
       real(c_double) :: refinement_tolerance
-      integer(c_int) :: preconditioning
+      integer(c_int) :: preconditioning, inner_iterations
    end type
 
    type, bind(c) :: spk_gmres_info
@@ -254,6 +254,7 @@ contains
       p%refinement_steps = 0
       p%refinement_tolerance = 0d0
       p%preconditioning = 0
+      p%inner_iterations = 0
       call spk_check(spk_gmres(handle, vector_b, vector_x, p, info), 'spk_gmres')
       iterations = info%iterations
       backward_error = info%backward_error

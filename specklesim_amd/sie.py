@@ -35,6 +35,20 @@ def _dp(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
+PRECOND_NEAR_FIELD = 2
+
+
+def _gmres_params(max_iterations, restart, tolerance, orthogonalization, use_initial_guess,
+                  refinement_steps, refinement_tolerance, precondition_left, near_field,
+                  inner_iterations):
+    if precondition_left and near_field:
+        raise ValueError("precondition_left and near_field exclude each other")
+    prec = PRECOND_NEAR_FIELD if near_field else int(bool(precondition_left))
+    return nat.GmresParams(max_iterations, restart, tolerance, ORTHOGONALIZATION[orthogonalization],
+                           int(use_initial_guess), refinement_steps, refinement_tolerance, prec,
+                           inner_iterations)
+
+
 def plane_wave(theta_in=0.0, phi_in=0.0, k_direction=1, polarization="TM"):
     """Incident plane wave of set_illumination_parameter (main.f90:245-297):
     k_in = (sin t cos p, sin t sin p, sign cos t) with sign = +1 for
@@ -195,17 +209,21 @@ class SieTriMlfmm:
     # -- lib_math_solver_gmres_run ------------------------------------------
     def gmres_run(self, vector_b, x0=None, max_iterations=650, restart=0, tolerance=1e-3,
                   orthogonalization="IMGS", use_initial_guess=True, refinement_steps=0,
-                  refinement_tolerance=0.0, precondition_left=False):
+                  refinement_tolerance=0.0, precondition_left=False, near_field=False,
+                  inner_iterations=0):
         """CERFACS GMRES on the FP64 operator (precondition_left: the
         reference's Jacobi left preconditioner); on a precision="mixed"
         handle, FP64 iterative refinement around complex64 inner GMRES
         solves (refinement_tolerance: inner relative residual of the first
-        solve, default 3e-4; later solves stop at tolerance / (2 |r|/|b|))."""
+        solve, default 3e-4; later solves stop at tolerance / (2 |r|/|b|)).
+        near_field: flexible GMRES on the FP64 operator, right-preconditioned
+        by inner_iterations (default 20) GMRES steps on the near-field
+        operator (complex64 on a mixed handle); restart defaults to 100."""
         b = np.ascontiguousarray(vector_b, dtype=np.complex128)
         x = self.get_init_vector_x_tri() if x0 is None else np.array(x0, dtype=np.complex128)
-        gp = nat.GmresParams(max_iterations, restart, tolerance,
-                             ORTHOGONALIZATION[orthogonalization], int(use_initial_guess),
-                             refinement_steps, refinement_tolerance, int(precondition_left))
+        gp = _gmres_params(max_iterations, restart, tolerance, orthogonalization,
+                           use_initial_guess, refinement_steps, refinement_tolerance,
+                           precondition_left, near_field, inner_iterations)
         info = nat.GmresInfo()
         nat.check(nat.lib().spk_gmres(self._h, _dp(b.view(np.float64)), _dp(x.view(np.float64)),
                                       C.byref(gp), C.byref(info)))
@@ -216,10 +234,11 @@ class SieTriMlfmm:
 
     def gmres_run_device(self, d_b, d_x, max_iterations=650, restart=0, tolerance=1e-3,
                          orthogonalization="IMGS", use_initial_guess=True, refinement_steps=0,
-                         refinement_tolerance=0.0, precondition_left=False):
-        gp = nat.GmresParams(max_iterations, restart, tolerance,
-                             ORTHOGONALIZATION[orthogonalization], int(use_initial_guess),
-                             refinement_steps, refinement_tolerance, int(precondition_left))
+                         refinement_tolerance=0.0, precondition_left=False, near_field=False,
+                         inner_iterations=0):
+        gp = _gmres_params(max_iterations, restart, tolerance, orthogonalization,
+                           use_initial_guess, refinement_steps, refinement_tolerance,
+                           precondition_left, near_field, inner_iterations)
         info = nat.GmresInfo()
         nat.check(nat.lib().spk_gmres_device(self._h, C.c_void_p(int(d_b)), C.c_void_p(int(d_x)),
                                              C.byref(gp), C.byref(info)))
@@ -229,16 +248,21 @@ class SieTriMlfmm:
 
     def gmres_run_batch_device(self, d_bs, d_xs, max_iterations=650, tolerance=1e-3,
                                use_initial_guess=True, refinement_steps=0,
-                               refinement_tolerance=0.0):
-        """Mixed-precision solves of R <= 16 right-hand sides (device
-        pointers to 2N complex128 vectors) on a precision="mixed" handle,
-        their inner complex64 steps sharing one batched matvec.  Returns one
-        info dict per right-hand side."""
+                               refinement_tolerance=0.0, near_field=False, inner_iterations=0,
+                               restart=0):
+        """Solves of R <= 16 right-hand sides (device pointers to 2N
+        complex128 vectors) sharing batched products.  Default: mixed
+        precision refinement on a precision="mixed" handle, the inner
+        complex64 steps sharing one batched matvec.  near_field: flexible
+        GMRES on the FP64 operator preconditioned by inner_iterations GMRES
+        steps on the near-field operator (any handle).  Returns one info dict
+        per right-hand side."""
         R = len(d_bs)
         if len(d_xs) != R:
             raise ValueError("d_bs and d_xs must have the same length")
-        gp = nat.GmresParams(max_iterations, 0, tolerance, ORTHOGONALIZATION["ICGS"],
-                             int(use_initial_guess), refinement_steps, refinement_tolerance, 0)
+        gp = _gmres_params(max_iterations, restart, tolerance, "ICGS", use_initial_guess,
+                           refinement_steps, refinement_tolerance, False, near_field,
+                           inner_iterations)
         infos = (nat.GmresInfo * R)()
         bs = (C.c_void_p * R)(*[int(p) for p in d_bs])
         xs = (C.c_void_p * R)(*[int(p) for p in d_xs])
