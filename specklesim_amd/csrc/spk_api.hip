@@ -115,6 +115,8 @@ struct spk_handle {
   // batched matvec: disaggregation tiles (box, first position), buffers
   DBuf<int> d_tile_box, d_tile_p0;
   int ntiles = 0;
+  // bf16 copy of the complex64 near blocks (near-field preconditioner), built on first use
+  DBuf<uint32_t> d_blk_bf;
   // matrix-core disaggregation tiles: (box, first position) of up to 128 edges
   DBuf<int> d_dx_box, d_dx_p0;
   int dx_tiles = 0;
@@ -442,7 +444,8 @@ static int batch_ensure(spk_handle *h, int R) {
 // ys[r] = Z xs[r] for r < R <= kBatchMax; xs, ys: device arrays of R pointers.
 // far_on = false: the near-field part only.
 template <class T>
-int matvec_batch_t(spk_handle *h, int R, const T *const *xs, T *const *ys, bool far_on = true) {
+int matvec_batch_t(spk_handle *h, int R, const T *const *xs, T *const *ys, bool far_on = true,
+                   const uint32_t *blocks_bf = nullptr) {
   T *tag = nullptr;
   if (R < 1 || R > kBatchMax) return fail(SPK_ERR_ARG, "batch size must be 1..16");
   int rc = batch_ensure<T>(h, R);
@@ -456,7 +459,7 @@ int matvec_batch_t(spk_handle *h, int R, const T *const *xs, T *const *ys, bool 
   if (!far_on) {
     launch_batch_near(h->spmv_wgs, N, R, h->d_row_box.p, h->d_row_r0.p, h->d_box_first.p,
                       h->d_box_count.p, h->d_e2_off.p, h->d_e2_list.p, h->d_near_off.p, h->d_perm.p,
-                      pBlk(h, tag), (const T *)BB.XI.p, (const T *)nullptr, ys, st);
+                      pBlk(h, tag), (const T *)BB.XI.p, (const T *)nullptr, ys, st, blocks_bf);
     HIPCHK(hipGetLastError());
     return SPK_OK;
   }
@@ -1454,6 +1457,16 @@ static int fgmres_near(spk_handle *h, int R, const zd *const *b, zd *const *x,
   if (R < 1 || R > kBatchMax) return fail(SPK_ERR_ARG, "batch size must be 1..16");
   if (std::is_same<T, zf>::value && !h->mixed_ready)
     return fail(SPK_ERR_STATE, "handle was not created with SPK_MIXED");
+  // complex64 inner solves run on the bf16 copy of the near field: the
+  // preconditioner only has to approximate the near-field solve
+  const uint32_t *bf = nullptr;
+  if constexpr (std::is_same<T, zf>::value) {
+    if (h->d_blk_bf.n != h->f_blocks.n) {
+      HIPCHK(h->d_blk_bf.alloc(h->f_blocks.n));
+      launch_to_bf((int64_t)h->f_blocks.n, h->f_blocks.p, h->d_blk_bf.p, h->st);
+    }
+    bf = h->d_blk_bf.p;
+  }
   const double t_start = now_ms();
   const int64_t n = 2 * (int64_t)h->N;
   const int max_it = std::max(1, gp->max_iterations);
@@ -1627,10 +1640,20 @@ static int fgmres_near(spk_handle *h, int R, const zd *const *b, zd *const *x,
     }
     for (int j = 0; j < kin; ++j) {
       if (P < 4) {
-        for (int s = 0; s < P && !rc; ++s)
-          rc = matvec_t<T>(h, U.p + s * ldu + (size_t)j * n, Wi.p + (size_t)s * n, true, false);
+        for (int s = 0; s < P && !rc; ++s) {
+          T *ui = U.p + s * ldu + (size_t)j * n, *wo = Wi.p + (size_t)s * n;
+          if constexpr (std::is_same<T, zf>::value) {
+            if (bf) {
+              launch_near_spmv_bf(h->spmv_wgs, h->N, h->d_row_box.p, h->d_row_r0.p, h->d_box_first.p,
+                                  h->d_box_count.p, h->d_e2_off.p, h->d_e2_list.p, h->d_near_off.p,
+                                  h->d_perm.p, bf, ui, wo, h->st);
+              continue;
+            }
+          }
+          rc = matvec_t<T>(h, ui, wo, true, false);
+        }
       } else {
-        rc = matvec_batch_t<T>(h, P, (const T *const *)(tin.p + (size_t)(4 + j) * B), tW, false);
+        rc = matvec_batch_t<T>(h, P, (const T *const *)(tin.p + (size_t)(4 + j) * B), tW, false, bf);
       }
       if (rc) return rc;
       const int *ks = kin_tab.p + (size_t)j * B;

@@ -613,12 +613,15 @@ __global__ void __launch_bounds__(256) kb_near(int N, int R, const int *row_box,
 // multiplies the current ones.  Rows and columns past the end are clamped on
 // load (their x is zero in LDS, their results are not written).  The four
 // waves' sums are added in wave order through LDS.
+// BF: the blocks are the bf16 copy (one 16-byte load per block, widened to
+// fp32 in registers).
 constexpr int kMxCols = 128;
+template <bool BF>
 __global__ void __launch_bounds__(256) kb_near_mx(int N, int R, const int *row_box,
                                                   const int *row_r0, const int *box_first,
                                                   const int *box_count, const int *e2_off,
                                                   const int *e2_list, const int64_t *near_off,
-                                                  const int *perm, const zf *blocks, const zf *XI,
+                                                  const int *perm, const void *blocks, const zf *XI,
                                                   const zf *yf, zf *const *ys) {
   constexpr int PER = kMxCols * 16 / 256;
   __shared__ float4 xsl[kMxCols * 16];
@@ -629,15 +632,25 @@ __global__ void __launch_bounds__(256) kb_near_mx(int N, int R, const int *row_b
   const int m = lane & 15, g = lane >> 4;
   const int rw = row_r0[blockIdx.x];  // first row of the tile (rw < nrow)
   const int row = min(rw + m, nrow - 1);
-  const float4 *Zr = reinterpret_cast<const float4 *>(blocks + 4 * (near_off[b] + (int64_t)row * cols));
+  const int64_t roff = near_off[b] + (int64_t)row * cols;  // first block of the row
+  const float4 *Zr = reinterpret_cast<const float4 *>(blocks) + 2 * roff;
+  const uint4 *Zb = reinterpret_cast<const uint4 *>(blocks) + roff;
   mxf4 er = {0.f, 0.f, 0.f, 0.f}, ei = er, hr = er, hi = er;
   float4 za[4], zb[4];
   auto load = [&](int cg, float4 *a, float4 *bb) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int c = min(cg + 4 * j + g, cols - 1);
-      a[j] = Zr[2 * c];
-      bb[j] = Zr[2 * c + 1];
+      if constexpr (BF) {
+        const uint4 v = Zb[c];
+        a[j] = float4{__uint_as_float(v.x << 16), __uint_as_float(v.x & 0xFFFF0000u),
+                      __uint_as_float(v.y << 16), __uint_as_float(v.y & 0xFFFF0000u)};
+        bb[j] = float4{__uint_as_float(v.z << 16), __uint_as_float(v.z & 0xFFFF0000u),
+                       __uint_as_float(v.w << 16), __uint_as_float(v.w & 0xFFFF0000u)};
+      } else {
+        a[j] = Zr[2 * c];
+        bb[j] = Zr[2 * c + 1];
+      }
     }
   };
   if (cols > 0) load(16 * wv, za, zb);
@@ -1032,12 +1045,18 @@ void launch_batch_near(int nwg, int N, int R, const int *row_box, const int *row
                        const int *box_first, const int *box_count, const int *e2_off,
                        const int *e2_list, const int64_t *near_off, const int *perm,
                        const T *blocks, const T *XI, const T *yf, T *const *ys,
-                       hipStream_t st) {
+                       hipStream_t st, const uint32_t *blocks_bf) {
   if (nwg <= 0) return;
   if constexpr (std::is_same<T, zf>::value) {
     if (batch_pad(R) == 16) {
-      hipLaunchKernelGGL(kb_near_mx, dim3(nwg), dim3(256), 0, st, N, R, row_box, row_r0,
-                         box_first, box_count, e2_off, e2_list, near_off, perm, blocks, XI, yf, ys);
+      if (blocks_bf)
+        hipLaunchKernelGGL(kb_near_mx<true>, dim3(nwg), dim3(256), 0, st, N, R, row_box, row_r0,
+                           box_first, box_count, e2_off, e2_list, near_off, perm,
+                           (const void *)blocks_bf, XI, yf, ys);
+      else
+        hipLaunchKernelGGL(kb_near_mx<false>, dim3(nwg), dim3(256), 0, st, N, R, row_box, row_r0,
+                           box_first, box_count, e2_off, e2_list, near_off, perm,
+                           (const void *)blocks, XI, yf, ys);
       return;
     }
   }
@@ -1072,7 +1091,7 @@ void launch_batch_near(int nwg, int N, int R, const int *row_box, const int *row
   template void launch_batch_near<T>(int, int, int, const int *, const int *, const int *,         \
                                      const int *, const int *, const int *, const int64_t *,       \
                                      const int *, const T *, const T *, const T *, T *const *,     \
-                                     hipStream_t);
+                                     hipStream_t, const uint32_t *);
 SPK_BATCH_INSTANTIATE(zd)
 SPK_BATCH_INSTANTIATE(zf)
 template void launch_batch_orth<zf>(int64_t, int, int, const int *, const zf *const *, zf *const *,

@@ -246,13 +246,14 @@ void launch_batch_disagg(int ntiles, int N, int S, int R, const int *tile_box, c
                          const T *W, zd sc1, zd sc2, T *yp, T *yf, int mx_tiles,
                          const int *mx_box, const int *mx_p0, hipStream_t st);
 // tiles (box, first row) of kSpmvRowsPerWG rows; complex64 with 9..16
-// right-hand sides runs on the matrix cores
+// right-hand sides runs on the matrix cores, on the bf16 copy of the
+// blocks when blocks_bf is given
 template <class T>
 void launch_batch_near(int nwg, int N, int R, const int *row_box, const int *row_r0,
                        const int *box_first, const int *box_count, const int *e2_off,
                        const int *e2_list, const int64_t *near_off, const int *perm,
                        const T *blocks, const T *XI, const T *yf, T *const *ys,
-                       hipStream_t st);
+                       hipStream_t st, const uint32_t *blocks_bf = nullptr);
 
 // one ICGS pass over P bases (coefficients into hs[r][0..ks[r]), |w_r|^2
 // into hs[r][ks[r]]); partial: batch_orth_partial_elems(n, P, maxk)
@@ -260,6 +261,14 @@ template <class T>
 void launch_batch_orth(int64_t n, int P, int maxk, const int *ks, const T *const *Vs, T *const *ws,
                        zd *const *hs, zd *partial, hipStream_t st);
 size_t batch_orth_partial_elems(int64_t n, int P, int maxk);
+// the near field in bf16 for the near-field preconditioner: one 32-bit word
+// per complex value (real part in the low half), blocks as in the complex64
+// array; the SpMV takes and returns complex64 vectors (no far-field add)
+void launch_to_bf(int64_t n, const zf *a, uint32_t *b, hipStream_t st);
+void launch_near_spmv_bf(int nwg, int N, const int *row_box, const int *row_r0,
+                         const int *box_first, const int *box_count, const int *e2_off,
+                         const int *e2_list, const int64_t *near_off, const int *perm,
+                         const uint32_t *blocks, const zf *x, zf *y, hipStream_t st);
 // one Arnoldi step of P near-field inner solves after their two
 // Gram-Schmidt passes: H_s[:, j] = h1 + h2, H_s[j+1, j] = |w_s| (|w|^2 in
 // h2[s * ldh + j + 1]), U_s[:, j+1] = w_s / |w_s|; slot s: U + s * ldu,

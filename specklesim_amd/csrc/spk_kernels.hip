@@ -1166,6 +1166,78 @@ void launch_near_spmv_rows(int nwg, int N, const int *row_box, const int *row_r0
 SPK_MATVEC_INSTANTIATE(zd)
 SPK_MATVEC_INSTANTIATE(zf)
 
+// Near field in bf16 (the preconditioner's copy): a complex value is one
+// 32-bit word, real part in the low half; a 2x2 block is 16 bytes.
+__device__ inline uint32_t f32_to_bf16(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even (finite values)
+  return u >> 16;
+}
+__global__ void k_to_bf(int64_t n, const zf *a, uint32_t *b) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = f32_to_bf16(a[i].x) | (f32_to_bf16(a[i].y) << 16);
+}
+void launch_to_bf(int64_t n, const zf *a, uint32_t *b, hipStream_t st) {
+  hipLaunchKernelGGL(k_to_bf, dim3(2048), dim3(256), 0, st, n, a, b);
+}
+// the near SpMV of k_near_spmv (complex64 x and y) on bf16 blocks
+__global__ void __launch_bounds__(256) k_near_spmv_bf(int N, const int *row_box, const int *row_r0,
+                                                      const int *box_first, const int *box_count,
+                                                      const int *e2_off, const int *e2_list,
+                                                      const int64_t *near_off, const int *perm,
+                                                      const uint4 *blocks, const zf *x, zf *y) {
+  __shared__ zf xs[2 * kColChunk];
+  const int b = row_box[blockIdx.x], r0 = row_r0[blockIdx.x];
+  const int rows = min(kRowsPerWG, box_count[b] - r0);
+  const int c0 = e2_off[b], cols = e2_off[b + 1] - c0;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  zf ae[kRowsPerWG / 4], ah[kRowsPerWG / 4];
+  for (int q = 0; q < kRowsPerWG / 4; ++q) ae[q] = ah[q] = zf{0, 0};
+  auto lo = [](uint32_t u) { return __uint_as_float(u << 16); };
+  auto hi = [](uint32_t u) { return __uint_as_float(u & 0xFFFF0000u); };
+  for (int cc = 0; cc < cols; cc += kColChunk) {
+    const int nc = min(kColChunk, cols - cc);
+    __syncthreads();
+    for (int c = threadIdx.x; c < nc; c += blockDim.x) {
+      const int n = e2_list[c0 + cc + c];
+      xs[2 * c] = x[n];
+      xs[2 * c + 1] = x[n + N];
+    }
+    __syncthreads();
+    for (int q = 0; q < kRowsPerWG / 4; ++q) {
+      const int r = r0 + wv + 4 * q;
+      if (r >= r0 + rows) break;
+      const uint4 *Z = blocks + (near_off[b] + (int64_t)r * cols + cc);
+      for (int c = lane; c < nc; c += 64) {
+        const uint4 v = Z[c];
+        const zf z0 = {lo(v.x), hi(v.x)}, z1 = {lo(v.y), hi(v.y)};
+        const zf z2 = {lo(v.z), hi(v.z)}, z3 = {lo(v.w), hi(v.w)};
+        const zf xe = xs[2 * c], xh = xs[2 * c + 1];
+        ae[q] = ae[q] + z0 * xe + z1 * xh;
+        ah[q] = ah[q] + z2 * xe + z3 * xh;
+      }
+    }
+  }
+  for (int q = 0; q < kRowsPerWG / 4; ++q) {
+    const int r = r0 + wv + 4 * q;
+    const zf se = wave_sum(ae[q]), sh = wave_sum(ah[q]);
+    if (r < r0 + rows && lane == 0) {
+      const int m = perm[box_first[b] + r];
+      y[m] = se;
+      y[m + N] = sh;
+    }
+  }
+}
+void launch_near_spmv_bf(int nwg, int N, const int *row_box, const int *row_r0,
+                         const int *box_first, const int *box_count, const int *e2_off,
+                         const int *e2_list, const int64_t *near_off, const int *perm,
+                         const uint32_t *blocks, const zf *x, zf *y, hipStream_t st) {
+  if (nwg <= 0) return;
+  hipLaunchKernelGGL(k_near_spmv_bf, dim3(nwg), dim3(256), 0, st, N, row_box, row_r0, box_first,
+                     box_count, e2_off, e2_list, near_off, perm, (const uint4 *)blocks, x, y);
+}
+
 // complex128 -> complex64 copy (the mixed-precision operator's tables)
 __global__ void k_to_zf(int64_t n, const zd *a, zf *b) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
