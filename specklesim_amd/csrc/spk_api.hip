@@ -1475,7 +1475,7 @@ static int fgmres_near(spk_handle *h, int R, const zd *const *b, zd *const *x,
   const double tol = gp->tolerance;
   const size_t ldh = (size_t)m + 2, ldhi = (size_t)kin + 2, hik = (size_t)(kin + 1) * kin;
   const size_t ldu = (size_t)(kin + 1) * n;
-  DBuf<zd> V, Z, W, Hd, Rv, Tv, Dv, Yo, Yi, h1, h2, Hi, dots;
+  DBuf<zd> V, Z, W, Hd, Rv, Tv, Dv, Yo, Yi, h1, Hi, dots;
   DBuf<T> U, Wi;
   HIPCHK(V.alloc((size_t)R * (m + 1) * n));
   HIPCHK(Z.alloc((size_t)R * m * n));
@@ -1489,47 +1489,37 @@ static int fgmres_near(spk_handle *h, int R, const zd *const *b, zd *const *x,
   HIPCHK(U.alloc((size_t)R * ldu));
   HIPCHK(Wi.alloc((size_t)R * n));
   HIPCHK(h1.alloc((size_t)R * ldhi));
-  HIPCHK(h2.alloc((size_t)R * ldhi));
+  DBuf<zd> apart;
+  HIPCHK(apart.alloc(arnoldi_partial_elems(R, kin)));
   HIPCHK(Hi.alloc((size_t)R * hik));
   HIPCHK(dots.alloc(R));
   const int kmax = std::max(m, kin) + 1;
   DBuf<zd> part;
-  HIPCHK(part.alloc(std::max(batch_orth_partial_elems(n, R, kmax), (size_t)reduce_blocks(n))));
+  HIPCHK(part.alloc(std::max(batch_orth_partial_elems(n, R, m + 1), (size_t)reduce_blocks(n))));
   HIPCHK(h->g_pn.alloc(gemv_n_partial_elems(n, kmax)));
   HIPCHK(h->pin.ensure(std::max((size_t)R * ldh, (size_t)R * hik)));
-  // pointer tables: inner slots s -> U_s, W_s, h1_s, h2_s, the inner
-  // products' inputs U_s[:, j] and ks[j][s] = j + 1 (fixed for the solve);
-  // outer: V_r, W_r, H_r, k_r and the products' inputs / outputs (per step)
+  // pointer tables: the inner products' inputs U_s[:, j] and outputs W_s
+  // (fixed for the solve); the outer products' and Gram-Schmidt passes'
+  // (per step; separate arrays: a table is rewritten only after the kernels
+  // that read its last contents have completed)
   constexpr int B = kBatchMax;
-  // (separate arrays: a table is rewritten only after the kernels that
-  // read its last contents have completed)
   DBuf<uint64_t> tin, tprod, torth;
-  DBuf<int> kin_tab, kout;
-  HIPCHK(tin.alloc((size_t)(4 + kin) * B));
-  HIPCHK(kin_tab.alloc((size_t)kin * B));
+  DBuf<int> kout;
+  HIPCHK(tin.alloc((size_t)(1 + kin) * B));
   HIPCHK(tprod.alloc(2 * (size_t)B));
   HIPCHK(torth.alloc(3 * (size_t)B));
   HIPCHK(kout.alloc(B));
   {
-    std::vector<uint64_t> t((size_t)(4 + kin) * B, 0);
-    std::vector<int> kt((size_t)kin * B);
+    std::vector<uint64_t> t((size_t)(1 + kin) * B, 0);
     for (int s = 0; s < B; ++s) {
-      t[s] = (uint64_t)(uintptr_t)(U.p + (s < R ? s * ldu : 0));
-      t[B + s] = (uint64_t)(uintptr_t)(Wi.p + (size_t)(s < R ? s : 0) * n);
-      t[2 * B + s] = (uint64_t)(uintptr_t)(h1.p + (size_t)(s < R ? s : 0) * ldhi);
-      t[3 * B + s] = (uint64_t)(uintptr_t)(h2.p + (size_t)(s < R ? s : 0) * ldhi);
-      for (int j = 0; j < kin; ++j) {
-        t[(size_t)(4 + j) * B + s] = (uint64_t)(uintptr_t)(U.p + (s < R ? s * ldu : 0) + (size_t)j * n);
-        kt[(size_t)j * B + s] = j + 1;
-      }
+      const int q = s < R ? s : 0;
+      t[s] = (uint64_t)(uintptr_t)(Wi.p + (size_t)q * n);
+      for (int j = 0; j < kin; ++j)
+        t[(size_t)(1 + j) * B + s] = (uint64_t)(uintptr_t)(U.p + q * ldu + (size_t)j * n);
     }
     HIPCHK(hipMemcpy(tin.p, t.data(), t.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(kin_tab.p, kt.data(), kt.size() * sizeof(int), hipMemcpyHostToDevice));
   }
-  const T *const *tU = (const T *const *)tin.p;
-  T *const *tW = (T *const *)(tin.p + B);
-  zd *const *tH1 = (zd *const *)(tin.p + 2 * B);
-  zd *const *tH2 = (zd *const *)(tin.p + 3 * B);
+  T *const *tW = (T *const *)tin.p;
   auto Vr = [&](int r) { return V.p + (size_t)r * (m + 1) * n; };
   auto Zr = [&](int r) { return Z.p + (size_t)r * m * n; };
   auto Wr = [&](int r) { return W.p + (size_t)r * n; };
@@ -1653,13 +1643,10 @@ static int fgmres_near(spk_handle *h, int R, const zd *const *b, zd *const *x,
           rc = matvec_t<T>(h, ui, wo, true, false);
         }
       } else {
-        rc = matvec_batch_t<T>(h, P, (const T *const *)(tin.p + (size_t)(4 + j) * B), tW, false, bf);
+        rc = matvec_batch_t<T>(h, P, (const T *const *)(tin.p + (size_t)(1 + j) * B), tW, false, bf);
       }
       if (rc) return rc;
-      const int *ks = kin_tab.p + (size_t)j * B;
-      launch_batch_orth<T>(n, P, j + 1, ks, tU, tW, tH1, part.p, h->st);
-      launch_batch_orth<T>(n, P, j + 1, ks, tU, tW, tH2, part.p, h->st);
-      launch_fg_step<T>(n, P, j, kin, (int)ldhi, h1.p, h2.p, Wi.p, ldu, U.p, Hi.p, h->st);
+      launch_arnoldi_cgs<T>(n, P, j, kin, U.p, ldu, Wi.p, h1.p, (int)ldhi, Hi.p, apart.p, h->st);
     }
     HIPCHK(hipMemcpyAsync(h->pin.p, Hi.p, (size_t)P * hik * sizeof(zd), hipMemcpyDeviceToHost, h->st));
     HIPCHK(hipStreamSynchronize(h->st));

@@ -895,39 +895,141 @@ void launch_batch_orth(int64_t n, int P, int maxk, const int *ks, const T *const
 }
 
 // ---------------------------------------------------------------------------
-// Device-side Arnoldi of the near-field inner solves (FGMRES preconditioner):
-// P slots, slot s holds its basis U_s = U + s * ldu (column j at j * n), its
-// work vector W + s * n, the two Gram-Schmidt passes' coefficients h1, h2 +
-// s * ldh (|w|^2 after the second pass in h2[j + 1]) and its Hessenberg
-// matrix H + s * (kin + 1) * kin, column-major with leading dimension kin + 1.
+// Device-side Arnoldi step of P near-field inner solves (the FGMRES
+// preconditioner), one classical Gram-Schmidt pass in three launches:
+//   dot:  partial[s][c][g] = sum over the rows of chunk g of conj(U_s[:, c]) w_s
+//   upd:  h_s = sum_g partial (fixed order, in every workgroup), w_s -= U_s h_s,
+//         pnrm[s][g] = |w_s|^2 over chunk g
+//   fin:  |w_s| = sqrt(sum_g pnrm); H_s[:, j] = (h_s, |w_s|); U_s[:, j + 1] = w_s / |w_s|
+// Slot s: U + s * ldu (column c at c * n), w: W + s * n, h: hb + s * ldh,
+// H + s * (kin + 1) * kin (column-major, leading dimension kin + 1).
+// Products accumulate in complex128.
 // ---------------------------------------------------------------------------
-// H_s[:, j] = h1 + h2 (rows 0..j), H_s[j + 1, j] = |w|
-__global__ void k_fg_hcol(int j, int kin, int ldh, const zd *h1, const zd *h2, zd *H) {
-  const int s = blockIdx.x, i = threadIdx.x;
-  if (i > j + 1) return;
-  const zd *a = h1 + (size_t)s * ldh, *b = h2 + (size_t)s * ldh;
-  const zd v = i <= j ? a[i] + b[i] : zd{sqrt(fmax(b[j + 1].x, 0.0)), 0.0};
-  H[(size_t)s * (kin + 1) * kin + (size_t)j * (kin + 1) + i] = v;
-}
-// U_s[:, j + 1] = w_s / |w_s| (zero when |w_s| = 0)
+constexpr int kArnCT = 16;  // columns per pass of the dot kernel
 template <class T>
-__global__ void k_fg_scale(int64_t n, int j, int ldh, const T *W, const zd *h2, size_t ldu, T *U) {
-  const int s = blockIdx.y;
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  const double w2 = h2[(size_t)s * ldh + j + 1].x;
-  const double inv = w2 > 0.0 ? 1.0 / sqrt(w2) : 0.0;
-  const T v = W[(size_t)s * n + t];
-  using F = decltype(v.x);
-  U[(size_t)s * ldu + (size_t)(j + 1) * n + t] = T{(F)(v.x * inv), (F)(v.y * inv)};
+__global__ void __launch_bounds__(256) k_arn_dot(int64_t n, int64_t rows, int k, int G, const T *U,
+                                                 size_t ldu, const T *W, zd *partial) {
+  __shared__ zd red[4][kArnCT];
+  const int s = blockIdx.y, g = blockIdx.x;
+  const T *Us = U + (size_t)s * ldu, *w = W + (size_t)s * n;
+  const int64_t r0 = (int64_t)g * rows, r1 = min(n, r0 + rows);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int c0 = 0; c0 < k; c0 += kArnCT) {
+    const int nc = min(kArnCT, k - c0);
+    zd acc[kArnCT];
+#pragma unroll
+    for (int c = 0; c < kArnCT; ++c) acc[c] = zd{0, 0};
+    for (int64_t i = r0 + threadIdx.x; i < r1; i += 256) {
+      const zd wi = bwiden(w[i]);
+#pragma unroll
+      for (int c = 0; c < kArnCT; ++c)
+        if (c < nc) {
+          const zd u = bwiden(Us[(size_t)(c0 + c) * n + i]);
+          acc[c] = acc[c] + zd{u.x * wi.x + u.y * wi.y, u.x * wi.y - u.y * wi.x};
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < kArnCT; ++c) {
+      const zd v = bwave_sum(acc[c]);
+      if (lane == 0) red[wv][c] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < nc) {
+      const int c = threadIdx.x;
+      partial[((size_t)s * k + c0 + c) * G + g] = red[0][c] + red[1][c] + red[2][c] + red[3][c];
+    }
+    __syncthreads();
+  }
 }
 template <class T>
-void launch_fg_step(int64_t n, int P, int j, int kin, int ldh, const zd *h1, const zd *h2,
-                    const T *W, size_t ldu, T *U, zd *H, hipStream_t st) {
+__global__ void __launch_bounds__(256) k_arn_upd(int64_t n, int64_t rows, int k, int G, const T *U,
+                                                 size_t ldu, T *W, const zd *partial, zd *hb,
+                                                 int ldh, double *pnrm) {
+  __shared__ zd hs[64];
+  __shared__ double red[4];
+  const int s = blockIdx.y, g = blockIdx.x;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  // h_c = sum over the G partials: wave wv takes columns wv, wv + 4, ...;
+  // lanes stride the partials, then a shuffle tree (a fixed order)
+  for (int c = wv; c < k; c += 4) {
+    const zd *p = partial + ((size_t)s * k + c) * G;
+    zd acc = {0, 0};
+    for (int q = lane; q < G; q += 64) acc = acc + p[q];
+    acc = bwave_sum(acc);
+    if (lane == 0) {
+      hs[c] = acc;
+      if (g == 0) hb[(size_t)s * ldh + c] = acc;
+    }
+  }
+  __syncthreads();
+  const T *Us = U + (size_t)s * ldu;
+  T *w = W + (size_t)s * n;
+  const int64_t r0 = (int64_t)g * rows, r1 = min(n, r0 + rows);
+  double nrm = 0.0;
+  for (int64_t i = r0 + threadIdx.x; i < r1; i += 256) {
+    zd wi = bwiden(w[i]);
+    for (int c = 0; c < k; ++c) {
+      const zd u = bwiden(Us[(size_t)c * n + i]), hc = hs[c];
+      wi.x -= u.x * hc.x - u.y * hc.y;
+      wi.y -= u.x * hc.y + u.y * hc.x;
+    }
+    using F = decltype(w[i].x);
+    const T wt = T{(F)wi.x, (F)wi.y};
+    w[i] = wt;
+    const zd ws = bwiden(wt);
+    nrm += ws.x * ws.x + ws.y * ws.y;
+  }
+  for (int off = 32; off > 0; off >>= 1) nrm += __shfl_down(nrm, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = nrm;
+  __syncthreads();
+  if (threadIdx.x == 0) pnrm[(size_t)s * G + g] = red[0] + red[1] + red[2] + red[3];
+}
+template <class T>
+__global__ void __launch_bounds__(256) k_arn_fin(int64_t n, int64_t rows, int j, int kin, int G,
+                                                 const double *pnrm, const zd *hb, int ldh,
+                                                 const T *W, size_t ldu, T *U, zd *H) {
+  __shared__ double nrm;
+  const int s = blockIdx.y, g = blockIdx.x;
+  if (threadIdx.x < 64) {
+    double a = 0.0;
+    for (int q = threadIdx.x; q < G; q += 64) a += pnrm[(size_t)s * G + q];
+    for (int off = 32; off > 0; off >>= 1) a += __shfl_down(a, off, 64);
+    if (threadIdx.x == 0) nrm = sqrt(fmax(a, 0.0));
+  }
+  __syncthreads();
+  const double nv = nrm, inv = nv > 0.0 ? 1.0 / nv : 0.0;
+  if (g == 0 && threadIdx.x <= j + 1) {
+    const int i = threadIdx.x;
+    H[(size_t)s * (kin + 1) * kin + (size_t)j * (kin + 1) + i] =
+        i <= j ? hb[(size_t)s * ldh + i] : zd{nv, 0.0};
+  }
+  const T *w = W + (size_t)s * n;
+  T *u = U + (size_t)s * ldu + (size_t)(j + 1) * n;
+  const int64_t r0 = (int64_t)g * rows, r1 = min(n, r0 + rows);
+  using F = decltype(w[0].x);
+  for (int64_t i = r0 + threadIdx.x; i < r1; i += 256) {
+    const T v = w[i];
+    u[i] = T{(F)(v.x * inv), (F)(v.y * inv)};
+  }
+}
+int arnoldi_chunks(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(256, (n + 511) / 512)); }
+size_t arnoldi_partial_elems(int P, int kin) {
+  return (size_t)P * (kin + 1) * 256 + (size_t)P * 256;
+}
+template <class T>
+void launch_arnoldi_cgs(int64_t n, int P, int j, int kin, T *U, size_t ldu, T *W, zd *hb, int ldh,
+                        zd *H, zd *partial, hipStream_t st) {
   if (P <= 0) return;
-  hipLaunchKernelGGL(k_fg_hcol, dim3(P), dim3(128), 0, st, j, kin, ldh, h1, h2, H);
-  hipLaunchKernelGGL(k_fg_scale<T>, dim3(bnblk(n, 256), P), dim3(256), 0, st, n, j, ldh, W, h2, ldu,
-                     U);
+  const int G = arnoldi_chunks(n);
+  const int64_t rows = (n + G - 1) / G;
+  const int k = j + 1;
+  double *pn = reinterpret_cast<double *>(partial + (size_t)P * (kin + 1) * G);
+  hipLaunchKernelGGL(k_arn_dot<T>, dim3(G, P), dim3(256), 0, st, n, rows, k, G, (const T *)U, ldu,
+                     (const T *)W, partial);
+  hipLaunchKernelGGL(k_arn_upd<T>, dim3(G, P), dim3(256), 0, st, n, rows, k, G, (const T *)U, ldu, W,
+                     (const zd *)partial, hb, ldh, pn);
+  hipLaunchKernelGGL(k_arn_fin<T>, dim3(G, P), dim3(256), 0, st, n, rows, j, kin, G,
+                     (const double *)pn, (const zd *)hb, ldh, (const T *)W, ldu, U, H);
 }
 
 template <class T>
@@ -1098,9 +1200,9 @@ template void launch_batch_orth<zf>(int64_t, int, int, const int *, const zf *co
                                     zd *const *, zd *, hipStream_t);
 template void launch_batch_orth<zd>(int64_t, int, int, const int *, const zd *const *, zd *const *,
                                     zd *const *, zd *, hipStream_t);
-template void launch_fg_step<zf>(int64_t, int, int, int, int, const zd *, const zd *, const zf *,
-                                 size_t, zf *, zd *, hipStream_t);
-template void launch_fg_step<zd>(int64_t, int, int, int, int, const zd *, const zd *, const zd *,
-                                 size_t, zd *, zd *, hipStream_t);
+template void launch_arnoldi_cgs<zf>(int64_t, int, int, int, zf *, size_t, zf *, zd *, int, zd *,
+                                     zd *, hipStream_t);
+template void launch_arnoldi_cgs<zd>(int64_t, int, int, int, zd *, size_t, zd *, zd *, int, zd *,
+                                     zd *, hipStream_t);
 
 }  // namespace spk

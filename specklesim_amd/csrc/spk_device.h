@@ -269,13 +269,16 @@ void launch_near_spmv_bf(int nwg, int N, const int *row_box, const int *row_r0,
                          const int *box_first, const int *box_count, const int *e2_off,
                          const int *e2_list, const int64_t *near_off, const int *perm,
                          const uint32_t *blocks, const zf *x, zf *y, hipStream_t st);
-// one Arnoldi step of P near-field inner solves after their two
-// Gram-Schmidt passes: H_s[:, j] = h1 + h2, H_s[j+1, j] = |w_s| (|w|^2 in
-// h2[s * ldh + j + 1]), U_s[:, j+1] = w_s / |w_s|; slot s: U + s * ldu,
-// W + s * n, H + s * (kin + 1) * kin (column-major, leading dimension kin + 1)
+// one Arnoldi step (one classical Gram-Schmidt pass, three launches) of P
+// near-field inner solves after w_s = A U_s[:, j]: h_s = U_s[:, 0..j]^H w_s
+// (also in hb + s * ldh), w_s -= U_s h_s, H_s[:, j] = (h_s, |w_s|),
+// U_s[:, j+1] = w_s / |w_s|.  Slot s: U + s * ldu (column c at c * n),
+// W + s * n, H + s * (kin + 1) * kin (column-major, leading dimension
+// kin + 1); partial: arnoldi_partial_elems(P, kin); kin <= 64
 template <class T>
-void launch_fg_step(int64_t n, int P, int j, int kin, int ldh, const zd *h1, const zd *h2,
-                    const T *W, size_t ldu, T *U, zd *H, hipStream_t st);
+void launch_arnoldi_cgs(int64_t n, int P, int j, int kin, T *U, size_t ldu, T *W, zd *hb, int ldh,
+                        zd *H, zd *partial, hipStream_t st);
+size_t arnoldi_partial_elems(int P, int kin);
 
 // streaming read of a buffer (bandwidth probe)
 void launch_stream_read(const void *a, size_t bytes, double *out, hipStream_t st);
