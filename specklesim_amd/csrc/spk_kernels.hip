@@ -50,13 +50,14 @@ __device__ inline zd expmikr(zd k, double R) {
   return zd{e * c, -e * s};
 }
 
-// Diver_Green (src/lib_sie_tri_calculation_mod.f90:1421-1432)
-__device__ inline void green(zd k, double R, zd &G, zd &DG) {
+// Diver_Green (src/lib_sie_tri_calculation_mod.f90:1421-1432); rinv = 1 / R
+// (the fill kernels divide once per point pair)
+__device__ inline void green(zd k, double R, double rinv, zd &G, zd &DG) {
   const zd E = expmikr(k, R);
   const zd ikr = zd{-k.y * R, k.x * R};
-  const double inv = 1.0 / (4 * PI * R);
+  const double inv = rinv * (1.0 / (4 * PI));
   G = E * inv;
-  DG = -((zd{1.0 + ikr.x, ikr.y} * E) * (inv / R));
+  DG = -((zd{1.0 + ikr.x, ikr.y} * E) * (inv * rinv));
 }
 // Diver_Green_s (:1435-1451)
 __device__ inline void green_s(zd k, double R, zd &Gs, zd &DGs) {
@@ -249,14 +250,14 @@ __device__ void normal_pair(const FnRec &fm, const FnRec &fn, const NearArgs &A,
         rfq[k] = 0.5 * (rq[k] - fn.c[6 + k]) * fn.df;
         R[k] = ra[k] - rq[k];
       }
-      const double Rl = len3(R);
-      for (int k = 0; k < 3; ++k) Rn[k] = R[k] / Rl;
+      const double Rl = len3(R), rinv = 1.0 / Rl;
+      for (int k = 0; k < 3; ++k) Rn[k] = R[k] * rinv;
       cross3(rfq, Rn, x);
       const double TRFi = A.qw[i] * fn.area;
 #pragma unroll
       for (int md = 0; md < 2; ++md) {
         zd G, DG;
-        green(kk[md], Rl, G, DG);
+        green(kk[md], Rl, rinv, G, DG);
         for (int k = 0; k < 3; ++k) {
           IK[md][k] = IK[md][k] + x[k] * DG * TRFi;
           ITa[md][k] = ITa[md][k] + rfq[k] * G * TRFi;
@@ -405,14 +406,14 @@ __device__ void edge_adjacent_kt(const FnRec &fm, const FnRec &fn, const NearArg
         rfq[k] = 0.5 * (rq[k] - fn.c[6 + k]) * fn.df;
         R[k] = ra[k] - rq[k];
       }
-      const double Rl = len3(R);
-      for (int k = 0; k < 3; ++k) Rn[k] = R[k] / Rl;
+      const double Rl = len3(R), rinv = 1.0 / Rl;
+      for (int k = 0; k < 3; ++k) Rn[k] = R[k] * rinv;
       cross3(rfq, Rn, x);
       const double TRFi = A.qw12[i] * fn.area;
 #pragma unroll
       for (int md = 0; md < 2; ++md) {
         zd G, DG;
-        green(kk[md], Rl, G, DG);
+        green(kk[md], Rl, rinv, G, DG);
         for (int k = 0; k < 3; ++k) Ik[md][k] = Ik[md][k] + x[k] * DG * TRFi;
       }
     }
@@ -1417,10 +1418,10 @@ __global__ void __launch_bounds__(256) k_scattered(ScatArgs a, const FnRec *fn, 
           fm[k] = 0.5 * dfm * (rq - f.c[6 + k]);
           d[k] = ra[k] - rq;
         }
-        const double R = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        const double R = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]), rinv = 1.0 / R;
         zd G, DG;
-        green(a.k1, R, G, DG);
-        const zd gG[3] = {DG * (d[0] / R), DG * (d[1] / R), DG * (d[2] / R)};
+        green(a.k1, R, rinv, G, DG);
+        const zd gG[3] = {DG * (d[0] * rinv), DG * (d[1] * rinv), DG * (d[2] * rinv)};
         const zd Mf[3] = {M * fm[0], M * fm[1], M * fm[2]};
         const zd cE[3] = {Mf[1] * gG[2] - Mf[2] * gG[1], Mf[2] * gG[0] - Mf[0] * gG[2],
                           Mf[0] * gG[1] - Mf[1] * gG[0]};
