@@ -39,13 +39,15 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on the
 # default workload (100 x 100 cells, 10 lambda, ICTF, complex128), each counter
 # scaled by what it reads for a plain 1 GiB device copy (FETCH x 2.0,
-# WRITE x 1.0); profiles/matvec_traffic_rough10x10_ictf.txt
+# WRITE x 1.0); profiles/matvec_traffic_rough10x10_ictf.txt and
+# profiles/near_spmv_bf16_rough10x10_ictf.txt
 MEASURED_TRAFFIC = {("near_spmv", 100, 10.0, "fp64"): 1.8555e9,
                     ("aggregation", 100, 10.0, "fp64"): 1.4732e9,
                     ("disaggregation", 100, 10.0, "fp64"): 1.6627e9,
                     ("near_spmv", 100, 10.0, "c64"): 0.9289e9,
                     ("aggregation", 100, 10.0, "c64"): 0.7573e9,
-                    ("disaggregation", 100, 10.0, "c64"): 0.8181e9}
+                    ("disaggregation", 100, 10.0, "c64"): 0.8181e9,
+                    ("near_spmv_bf16", 100, 10.0, "bf16"): 0.4716e9}
 
 
 def _sync():
@@ -252,9 +254,15 @@ def main():
     # the roofline kernel: the longest HBM-bound stage of the headline path's
     # matvec (the complex64 operator when the mixed path leads)
     pc = paths["mixed"]["solver"].profile_matvec(reps=20, c64=True) if "mixed" in paths else None
-    rprof, rprec = (pc, "c64") if head in ("mixed", "near_fgmres") else (prof, "fp64")
-    stage, (ms, by) = max(((k, v) for k, v in rprof.items() if k != "matvec"),
-                          key=lambda kv: kv[1][0])
+    if head == "near_fgmres":
+        # the near-field preconditioner's bf16 SpMV: about 20 launches per
+        # outer step, the longest-running kernel of this path
+        stage, rprec = "near_spmv_bf16", "bf16"
+        ms, by = H["solver"].profile_near_bf(reps=20)
+    else:
+        rprof, rprec = (pc, "c64") if head == "mixed" else (prof, "fp64")
+        stage, (ms, by) = max(((k, v) for k, v in rprof.items() if k != "matvec"),
+                              key=lambda kv: kv[1][0])
     achieved = by / (ms * 1e-3) / 1e9
     traffic = MEASURED_TRAFFIC.get((stage, args.cells, args.size, rprec))
     from specklesim_amd._native import read_bandwidth
@@ -299,7 +307,9 @@ def main():
         "headline_path": head,
         "mixed_vs_fp64": (paths["mixed"]["solves_per_s"] / paths["fp64"]["solves_per_s"]
                           if "mixed" in paths and "fp64" in paths else None),
-        "roofline": {"kernel": stage, "operator": "complex64" if rprec == "c64" else "complex128",
+        "roofline": {"kernel": stage,
+                     "operator": {"c64": "complex64", "fp64": "complex128",
+                                  "bf16": "bf16 near blocks, complex64 vectors"}[rprec],
                      "bound": "hbm", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic, "algorithmic_bytes": by, "ms": ms,

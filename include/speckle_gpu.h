@@ -273,6 +273,12 @@ int spk_probe_read_bandwidth(int device, size_t bytes, int reps, double *gbps);
 /* Same for the complex64 operator of an SPK_MIXED handle (bytes at 8 B per
  * complex). */
 int spk_profile_matvec_c64(spk_handle *h, int reps, double *ms, double *bytes);
+/* No counterpart in the reference: timing of the near-field preconditioner's
+ * SpMV (bf16 blocks, complex64 vectors; SPK_PRECOND_NEAR_FIELD on an
+ * SPK_MIXED handle) with HIP events on the handle's stream: *ms = mean over
+ * reps launches, *bytes = its algorithmic HBM bytes (16 B per 2x2 block,
+ * x read and y written at 8 B per complex). */
+int spk_profile_near_bf(spk_handle *h, int reps, double *ms, double *bytes);
 
 /* Host-only setup probe (no GPU needed): RWG topology and octree of a mesh
  * exactly as spk_setup_tri builds them.  corner/element: N x 2 (1-based),

@@ -1437,6 +1437,15 @@ static int gmres_mixed_batch(spk_handle *h, int R, const zd *const *b, zd *const
 extern "C" {
 
 extern "C++" {
+// the bf16 copy of the complex64 near blocks (built once per setup)
+static int ensure_near_bf(spk_handle *h) {
+  if (h->d_blk_bf.n != h->f_blocks.n) {
+    HIPCHK(h->d_blk_bf.alloc(h->f_blocks.n));
+    launch_to_bf((int64_t)h->f_blocks.n, h->f_blocks.p, h->d_blk_bf.p, h->st);
+  }
+  return SPK_OK;
+}
+
 // Flexible GMRES on the FP64 operator for R <= kBatchMax right-hand sides,
 // right-preconditioned by kin GMRES steps (from zero) on the near-field
 // operator: complex64 on an SPK_MIXED handle (T = zf), FP64 otherwise.  The
@@ -1461,10 +1470,8 @@ static int fgmres_near(spk_handle *h, int R, const zd *const *b, zd *const *x,
   // preconditioner only has to approximate the near-field solve
   const uint32_t *bf = nullptr;
   if constexpr (std::is_same<T, zf>::value) {
-    if (h->d_blk_bf.n != h->f_blocks.n) {
-      HIPCHK(h->d_blk_bf.alloc(h->f_blocks.n));
-      launch_to_bf((int64_t)h->f_blocks.n, h->f_blocks.p, h->d_blk_bf.p, h->st);
-    }
+    const int e = ensure_near_bf(h);
+    if (e) return e;
     bf = h->d_blk_bf.p;
   }
   const double t_start = now_ms();
@@ -2319,6 +2326,39 @@ int spk_profile_matvec_c64(spk_handle *h, int reps, double *ms, double *bytes) {
   if (!h->ready) return fail(SPK_ERR_STATE, "spk_setup_tri has not completed");
   if (!h->mixed_ready) return fail(SPK_ERR_STATE, "handle was not created with SPK_MIXED");
   return profile_matvec_t<zf>(h, reps, ms, bytes);
+}
+
+int spk_profile_near_bf(spk_handle *h, int reps, double *ms, double *bytes) {
+  if (!h || reps < 1 || !ms || !bytes) return fail(SPK_ERR_ARG, "bad arguments");
+  if (!h->ready) return fail(SPK_ERR_STATE, "spk_setup_tri has not completed");
+  if (!h->mixed_ready) return fail(SPK_ERR_STATE, "handle was not created with SPK_MIXED");
+  const int rc = ensure_near_bf(h);
+  if (rc) return rc;
+  const int64_t n = 2 * (int64_t)h->N;
+  DBuf<zf> x, y;
+  HIPCHK(x.alloc(n));
+  HIPCHK(y.alloc(n));
+  HIPCHK(hipMemsetAsync(x.p, 0, n * sizeof(zf), h->st));
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  auto run = [&]() {
+    launch_near_spmv_bf(h->spmv_wgs, h->N, h->d_row_box.p, h->d_row_r0.p, h->d_box_first.p,
+                        h->d_box_count.p, h->d_e2_off.p, h->d_e2_list.p, h->d_near_off.p,
+                        h->d_perm.p, h->d_blk_bf.p, x.p, y.p, h->st);
+  };
+  run();
+  HIPCHK(hipEventRecord(e0, h->st));
+  for (int r = 0; r < reps; ++r) run();
+  HIPCHK(hipEventRecord(e1, h->st));
+  HIPCHK(hipEventSynchronize(e1));
+  float t = 0.f;
+  HIPCHK(hipEventElapsedTime(&t, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms = (double)t / reps;
+  *bytes = (double)(h->d_blk_bf.n / 4) * 16.0 + 3.0 * (double)n * 8.0;
+  return SPK_OK;
 }
 
 int spk_topology_probe(int np, const double *pp, int nt, const int *tt, int normal_convention,

@@ -7,10 +7,12 @@ around it.
     python -m specklesim_amd.profile_matvec [--cells 100] [--size 10] [--reps 20]
     python -m specklesim_amd.profile_matvec --copy-mib 1024
     python -m specklesim_amd.profile_matvec --batch 16
+    python -m specklesim_amd.profile_matvec --near-bf
 
 ``--copy-mib`` instead runs five plain device-to-device copies of that size
 (torch's copy kernel), the calibration for FETCH_SIZE / WRITE_SIZE counters.
-``--batch R`` instead runs one batched mixed-precision solve of R plane
+``--near-bf`` instead repeats the near-field preconditioner's bf16 SpMV.
+``--batch R`` instead runs one batched near-field preconditioned solve of R plane
 waves (polar angles uniform in 0-60 degrees, TM, tolerance 1e-6) after the
 precalculation, and prints each right-hand side's iterations and backward
 error.
@@ -46,8 +48,8 @@ def batched_solve(s, R: int) -> dict:
         k, e = plane_wave(np.radians(60.0 * r / max(R - 1, 1)), 0.0, 2, "TM")
         s.get_vector_b_tri_device(k, e, b.ptr)
     infos = s.gmres_run_batch_device([b.ptr for b in bs], [x.ptr for x in xs],
-                                     max_iterations=1300, tolerance=1e-6,
-                                     use_initial_guess=False)
+                                     max_iterations=650, tolerance=1e-6,
+                                     use_initial_guess=False, near_field=True)
     synchronize(0)
     return {"batch": R, "iters": [i["iters"] for i in infos], "be": [i["be"] for i in infos]}
 
@@ -63,6 +65,7 @@ def main(argv=None):
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--copy-mib", type=int, default=0)
     ap.add_argument("--batch", type=int, default=0)
+    ap.add_argument("--near-bf", action="store_true")
     args = ap.parse_args(argv)
     if args.copy_mib:
         print(json.dumps(copy_calibration(args.copy_mib)), flush=True)
@@ -75,10 +78,15 @@ def main(argv=None):
                                   args.seed, lam)
     s = SieTriMlfmm(lam, args.eps_r2, formulation=args.formulation, normal_convention=2,
                     tree_s_opt=250, truncation_number=19,
-                    precision="mixed" if args.batch else args.precision)
+                    precision="mixed" if (args.batch or args.near_bf) else args.precision)
     s.precalculation_tri(pp, tt)
     if args.batch:
         print(json.dumps(batched_solve(s, args.batch)), flush=True)
+        return
+    if args.near_bf:
+        ms, by = s.profile_near_bf(reps=args.reps)
+        print(json.dumps({"kernel": "k_near_spmv_bf", "reps": args.reps, "ms": ms, "bytes": by,
+                          "GBps": by / (ms * 1e-3) / 1e9}), flush=True)
         return
     c64 = args.precision == "mixed"
     prof = s.profile_matvec(reps=args.reps, c64=c64)
