@@ -8,10 +8,14 @@ around it.
     python -m specklesim_amd.profile_matvec --copy-mib 1024
     python -m specklesim_amd.profile_matvec --batch 16
     python -m specklesim_amd.profile_matvec --near-bf
+    python -m specklesim_amd.profile_matvec --solve near
 
 ``--copy-mib`` instead runs five plain device-to-device copies of that size
 (torch's copy kernel), the calibration for FETCH_SIZE / WRITE_SIZE counters.
 ``--near-bf`` instead repeats the near-field preconditioner's bf16 SpMV.
+``--solve near|mixed|fp64`` instead runs one solve of the normal-incidence
+TM plane wave to 1e-6 (near: the near-field preconditioned flexible GMRES;
+mixed: the refinement solve; fp64: the reference's GMRES with IMGS).
 ``--batch R`` instead runs one batched near-field preconditioned solve of R plane
 waves (polar angles uniform in 0-60 degrees, TM, tolerance 1e-6) after the
 precalculation, and prints each right-hand side's iterations and backward
@@ -66,6 +70,7 @@ def main(argv=None):
     ap.add_argument("--copy-mib", type=int, default=0)
     ap.add_argument("--batch", type=int, default=0)
     ap.add_argument("--near-bf", action="store_true")
+    ap.add_argument("--solve", default="", choices=["", "near", "mixed", "fp64"])
     args = ap.parse_args(argv)
     if args.copy_mib:
         print(json.dumps(copy_calibration(args.copy_mib)), flush=True)
@@ -78,10 +83,21 @@ def main(argv=None):
                                   args.seed, lam)
     s = SieTriMlfmm(lam, args.eps_r2, formulation=args.formulation, normal_convention=2,
                     tree_s_opt=250, truncation_number=19,
-                    precision="mixed" if (args.batch or args.near_bf) else args.precision)
+                    precision=("fp64" if args.solve == "fp64" else "mixed")
+                    if (args.batch or args.near_bf or args.solve) else args.precision)
     s.precalculation_tri(pp, tt)
     if args.batch:
         print(json.dumps(batched_solve(s, args.batch)), flush=True)
+        return
+    if args.solve:
+        from . import plane_wave
+
+        k, e = plane_wave(0.0, 0.0, 2, "TM")
+        b = s.get_vector_b_tri(k, e)
+        x, inf = s.gmres_run(b, tolerance=1e-6, max_iterations=650, use_initial_guess=False,
+                             near_field=args.solve == "near")
+        print(json.dumps({"solve": args.solve, "iters": inf["iters"], "matvecs": inf["matvecs"],
+                          "be": inf["be"], "seconds": inf["seconds"]}), flush=True)
         return
     if args.near_bf:
         ms, by = s.profile_near_bf(reps=args.reps)
