@@ -1497,7 +1497,7 @@ static int fgmres_near(spk_handle *h, int R, const zd *const *b, zd *const *x,
   HIPCHK(Wi.alloc((size_t)R * n));
   HIPCHK(h1.alloc((size_t)R * ldhi));
   DBuf<zd> apart;
-  HIPCHK(apart.alloc(arnoldi_partial_elems(R, kin)));
+  HIPCHK(apart.alloc(arnoldi_partial_elems(n, R, kin)));
   HIPCHK(Hi.alloc((size_t)R * hik));
   HIPCHK(dots.alloc(R));
   const int kmax = std::max(m, kin) + 1;

@@ -898,6 +898,7 @@ void launch_batch_orth(int64_t n, int P, int maxk, const int *ks, const T *const
 // Device-side Arnoldi step of P near-field inner solves (the FGMRES
 // preconditioner), one classical Gram-Schmidt pass in three launches:
 //   dot:  partial[s][c][g] = sum over the rows of chunk g of conj(U_s[:, c]) w_s
+//         (one workgroup per 1024-row chunk, column and slot)
 //   upd:  h_s = sum_g partial (fixed order, in every workgroup), w_s -= U_s h_s,
 //         pnrm[s][g] = |w_s|^2 over chunk g
 //   fin:  |w_s| = sqrt(sum_g pnrm); H_s[:, j] = (h_s, |w_s|); U_s[:, j + 1] = w_s / |w_s|
@@ -905,46 +906,32 @@ void launch_batch_orth(int64_t n, int P, int maxk, const int *ks, const T *const
 // H + s * (kin + 1) * kin (column-major, leading dimension kin + 1).
 // Products accumulate in complex128.
 // ---------------------------------------------------------------------------
-constexpr int kArnCT = 16;  // columns per pass of the dot kernel
+constexpr int kArnRows = 1024;  // rows per workgroup of the dot kernel
+// partial[s][c][g] = sum over rows [g * kArnRows, (g + 1) * kArnRows) of
+// conj(U_s[:, c]) w_s: one workgroup per (row chunk, column, slot)
 template <class T>
-__global__ void __launch_bounds__(256) k_arn_dot(int64_t n, int64_t rows, int k, int G, const T *U,
-                                                 size_t ldu, const T *W, zd *partial) {
-  __shared__ zd red[4][kArnCT];
-  const int s = blockIdx.y, g = blockIdx.x;
-  const T *Us = U + (size_t)s * ldu, *w = W + (size_t)s * n;
-  const int64_t r0 = (int64_t)g * rows, r1 = min(n, r0 + rows);
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  for (int c0 = 0; c0 < k; c0 += kArnCT) {
-    const int nc = min(kArnCT, k - c0);
-    zd acc[kArnCT];
-#pragma unroll
-    for (int c = 0; c < kArnCT; ++c) acc[c] = zd{0, 0};
-    for (int64_t i = r0 + threadIdx.x; i < r1; i += 256) {
-      const zd wi = bwiden(w[i]);
-#pragma unroll
-      for (int c = 0; c < kArnCT; ++c)
-        if (c < nc) {
-          const zd u = bwiden(Us[(size_t)(c0 + c) * n + i]);
-          acc[c] = acc[c] + zd{u.x * wi.x + u.y * wi.y, u.x * wi.y - u.y * wi.x};
-        }
-    }
-#pragma unroll
-    for (int c = 0; c < kArnCT; ++c) {
-      const zd v = bwave_sum(acc[c]);
-      if (lane == 0) red[wv][c] = v;
-    }
-    __syncthreads();
-    if (threadIdx.x < nc) {
-      const int c = threadIdx.x;
-      partial[((size_t)s * k + c0 + c) * G + g] = red[0][c] + red[1][c] + red[2][c] + red[3][c];
-    }
-    __syncthreads();
+__global__ void __launch_bounds__(256) k_arn_dot(int64_t n, int k, int G, const T *U, size_t ldu,
+                                                 const T *W, zd *partial) {
+  __shared__ zd red[4];
+  const int g = blockIdx.x, c = blockIdx.y, s = blockIdx.z;
+  const T *u = U + (size_t)s * ldu + (size_t)c * n, *w = W + (size_t)s * n;
+  const int64_t r0 = (int64_t)g * kArnRows, r1 = min(n, r0 + kArnRows);
+  zd acc = {0, 0};
+#pragma unroll 4
+  for (int64_t i = r0 + threadIdx.x; i < r1; i += 256) {
+    const zd a = bwiden(u[i]), b = bwiden(w[i]);
+    acc.x += a.x * b.x + a.y * b.y;
+    acc.y += a.x * b.y - a.y * b.x;
   }
+  acc = bwave_sum(acc);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[((size_t)s * k + c) * G + g] = red[0] + red[1] + red[2] + red[3];
 }
 template <class T>
-__global__ void __launch_bounds__(256) k_arn_upd(int64_t n, int64_t rows, int k, int G, const T *U,
-                                                 size_t ldu, T *W, const zd *partial, zd *hb,
-                                                 int ldh, double *pnrm) {
+__global__ void __launch_bounds__(256) k_arn_upd(int64_t n, int64_t rows, int k, int G, int Gd,
+                                                 const T *U, size_t ldu, T *W, const zd *partial,
+                                                 zd *hb, int ldh, double *pnrm) {
   __shared__ zd hs[64];
   __shared__ double red[4];
   const int s = blockIdx.y, g = blockIdx.x;
@@ -952,9 +939,9 @@ __global__ void __launch_bounds__(256) k_arn_upd(int64_t n, int64_t rows, int k,
   // h_c = sum over the G partials: wave wv takes columns wv, wv + 4, ...;
   // lanes stride the partials, then a shuffle tree (a fixed order)
   for (int c = wv; c < k; c += 4) {
-    const zd *p = partial + ((size_t)s * k + c) * G;
+    const zd *p = partial + ((size_t)s * k + c) * Gd;
     zd acc = {0, 0};
-    for (int q = lane; q < G; q += 64) acc = acc + p[q];
+    for (int q = lane; q < Gd; q += 64) acc = acc + p[q];
     acc = bwave_sum(acc);
     if (lane == 0) {
       hs[c] = acc;
@@ -1013,21 +1000,22 @@ __global__ void __launch_bounds__(256) k_arn_fin(int64_t n, int64_t rows, int j,
   }
 }
 int arnoldi_chunks(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(256, (n + 511) / 512)); }
-size_t arnoldi_partial_elems(int P, int kin) {
-  return (size_t)P * (kin + 1) * 256 + (size_t)P * 256;
+static int arnoldi_dot_chunks(int64_t n) { return (int)((n + kArnRows - 1) / kArnRows); }
+size_t arnoldi_partial_elems(int64_t n, int P, int kin) {
+  return (size_t)P * (kin + 1) * (size_t)std::max(256, arnoldi_dot_chunks(n)) + (size_t)P * 256;
 }
 template <class T>
 void launch_arnoldi_cgs(int64_t n, int P, int j, int kin, T *U, size_t ldu, T *W, zd *hb, int ldh,
                         zd *H, zd *partial, hipStream_t st) {
   if (P <= 0) return;
-  const int G = arnoldi_chunks(n);
+  const int G = arnoldi_chunks(n), Gd = arnoldi_dot_chunks(n);
   const int64_t rows = (n + G - 1) / G;
   const int k = j + 1;
-  double *pn = reinterpret_cast<double *>(partial + (size_t)P * (kin + 1) * G);
-  hipLaunchKernelGGL(k_arn_dot<T>, dim3(G, P), dim3(256), 0, st, n, rows, k, G, (const T *)U, ldu,
+  double *pn = reinterpret_cast<double *>(partial + (size_t)P * (kin + 1) * std::max(256, Gd));
+  hipLaunchKernelGGL(k_arn_dot<T>, dim3(Gd, k, P), dim3(256), 0, st, n, k, Gd, (const T *)U, ldu,
                      (const T *)W, partial);
-  hipLaunchKernelGGL(k_arn_upd<T>, dim3(G, P), dim3(256), 0, st, n, rows, k, G, (const T *)U, ldu, W,
-                     (const zd *)partial, hb, ldh, pn);
+  hipLaunchKernelGGL(k_arn_upd<T>, dim3(G, P), dim3(256), 0, st, n, rows, k, G, Gd, (const T *)U,
+                     ldu, W, (const zd *)partial, hb, ldh, pn);
   hipLaunchKernelGGL(k_arn_fin<T>, dim3(G, P), dim3(256), 0, st, n, rows, j, kin, G,
                      (const double *)pn, (const zd *)hb, ldh, (const T *)W, ldu, U, H);
 }

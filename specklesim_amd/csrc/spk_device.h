@@ -274,11 +274,11 @@ void launch_near_spmv_bf(int nwg, int N, const int *row_box, const int *row_r0,
 // (also in hb + s * ldh), w_s -= U_s h_s, H_s[:, j] = (h_s, |w_s|),
 // U_s[:, j+1] = w_s / |w_s|.  Slot s: U + s * ldu (column c at c * n),
 // W + s * n, H + s * (kin + 1) * kin (column-major, leading dimension
-// kin + 1); partial: arnoldi_partial_elems(P, kin); kin <= 64
+// kin + 1); partial: arnoldi_partial_elems(n, P, kin); kin <= 64
 template <class T>
 void launch_arnoldi_cgs(int64_t n, int P, int j, int kin, T *U, size_t ldu, T *W, zd *hb, int ldh,
                         zd *H, zd *partial, hipStream_t st);
-size_t arnoldi_partial_elems(int P, int kin);
+size_t arnoldi_partial_elems(int64_t n, int P, int kin);
 
 // streaming read of a buffer (bandwidth probe)
 void launch_stream_read(const void *a, size_t bytes, double *out, hipStream_t st);
