@@ -235,15 +235,19 @@ contains
       vector_x = x_initial
    end subroutine
 
-   ! lib_math_solver_gmres_run with the GPU matvec and orthogonalisation
+   ! lib_math_solver_gmres_run with the GPU matvec and orthogonalisation;
+   ! near_field (optional): the near-field preconditioned flexible GMRES
+   ! (iterations then counts its outer steps)
    subroutine lib_sie_ml_fmm_gmres_run(vector_b, vector_x, max_iterations, restart, &
-                                       tolerance, orthogonalization, iterations, backward_error)
+                                       tolerance, orthogonalization, iterations, backward_error, &
+                                       near_field)
       double complex, dimension(:), intent(in) :: vector_b
       double complex, dimension(:), intent(inout) :: vector_x
       integer, intent(in) :: max_iterations, restart, orthogonalization
       real(8), intent(in) :: tolerance
       integer, intent(out) :: iterations
       real(8), intent(out) :: backward_error
+      logical, intent(in), optional :: near_field
       type(spk_gmres_params) :: p
       type(spk_gmres_info) :: info
       p%max_iterations = max_iterations
@@ -254,6 +258,9 @@ contains
       p%refinement_steps = 0
       p%refinement_tolerance = 0d0
       p%preconditioning = 0
+      if (present(near_field)) then
+         if (near_field) p%preconditioning = 2   ! SPK_PRECOND_NEAR_FIELD
+      end if
       p%inner_iterations = 0
       call spk_check(spk_gmres(handle, vector_b, vector_x, p, info), 'spk_gmres')
       iterations = info%iterations
