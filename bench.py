@@ -214,7 +214,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch", type=int, default=16,
                     help="incidence angles of the batched path (0: skip it)")
-    ap.add_argument("--inner", type=int, default=20,
+    ap.add_argument("--inner", type=int, default=30,
                     help="inner near-field GMRES steps per flexible GMRES step")
     args = ap.parse_args()
     if args.gpus != 1:

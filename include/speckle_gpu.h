@@ -86,7 +86,7 @@ typedef struct spk_gmres_params {
                                     reference): flexible GMRES on the FP64 operator, right-
                                     preconditioned by inner_iterations GMRES steps on the
                                     near-field operator (complex64 on an SPK_MIXED handle) */
-  int inner_iterations;          /* SPK_PRECOND_NEAR_FIELD: inner steps per outer step (0: 20) */
+  int inner_iterations;          /* SPK_PRECOND_NEAR_FIELD: inner steps per outer step (0: 30) */
 } spk_gmres_params;
 #define SPK_PRECOND_NONE 0
 #define SPK_PRECOND_LEFT_JACOBI 1

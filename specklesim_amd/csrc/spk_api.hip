@@ -1478,7 +1478,7 @@ static int fgmres_near(spk_handle *h, int R, const zd *const *b, zd *const *x,
   const int64_t n = 2 * (int64_t)h->N;
   const int max_it = std::max(1, gp->max_iterations);
   const int m = std::min(max_it, gp->restart > 0 ? gp->restart : 100);
-  const int kin = gp->inner_iterations > 0 ? std::min(gp->inner_iterations, 64) : 20;
+  const int kin = gp->inner_iterations > 0 ? std::min(gp->inner_iterations, 64) : 30;
   const double tol = gp->tolerance;
   const size_t ldh = (size_t)m + 2, ldhi = (size_t)kin + 2, hik = (size_t)(kin + 1) * kin;
   const size_t ldu = (size_t)(kin + 1) * n;

@@ -217,7 +217,7 @@ class SieTriMlfmm:
         solves (refinement_tolerance: inner relative residual of the first
         solve, default 3e-4; later solves stop at tolerance / (2 |r|/|b|)).
         near_field: flexible GMRES on the FP64 operator, right-preconditioned
-        by inner_iterations (default 20) GMRES steps on the near-field
+        by inner_iterations (default 30) GMRES steps on the near-field
         operator (complex64 on a mixed handle); restart defaults to 100."""
         b = np.ascontiguousarray(vector_b, dtype=np.complex128)
         x = self.get_init_vector_x_tri() if x0 is None else np.array(x0, dtype=np.complex128)
