@@ -746,6 +746,326 @@ __global__ void __launch_bounds__(256) kb_near_mx(int N, int R, const int *row_b
   }
 }
 
+// ---------------------------------------------------------------------------
+// The same three products for complex128 batches of 9..16 on the FP64
+// matrix cores (v_mfma_f64_16x16x4f64: A[m][g] in lane g*16 + m and B[g][n]
+// in lane g*16 + n as in the fp32 form, but D[4i + g][n] in element i of lane
+// g*16 + n, as probed on the device).  Each complex value is one 16-byte
+// load; sums run as one running sum (complex128 needs no grouping).
+// ---------------------------------------------------------------------------
+typedef double mxd4 __attribute__((ext_vector_type(4)));
+#define SPK_MFMA64(a, b, c) __builtin_amdgcn_mfma_f64_16x16x4f64((a), (b), (c), 0, 0, 0)
+
+// near field: workgroup = (box, 16 rows), 4 waves take every fourth group of
+// 16 columns, x staged in LDS 64 columns at a time as (xe, xh) per (column, r)
+constexpr int kMx64Cols = 64;
+__global__ void __launch_bounds__(256) kb_near_mx64(int N, int R, const int *row_box,
+                                                    const int *row_r0, const int *box_first,
+                                                    const int *box_count, const int *e2_off,
+                                                    const int *e2_list, const int64_t *near_off,
+                                                    const int *perm, const zd *blocks, const zd *XI,
+                                                    const zd *yf, zd *const *ys) {
+  constexpr int PER = kMx64Cols * 16 / 256;
+  __shared__ double4 xsl[kMx64Cols * 16];
+  const int b = row_box[blockIdx.x];
+  const int pf = box_first[b], nrow = box_count[b];
+  const int c0 = e2_off[b], cols = e2_off[b + 1] - c0;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int m = lane & 15, g = lane >> 4;
+  const int rw = row_r0[blockIdx.x];
+  const int row = min(rw + m, nrow - 1);
+  const double2 *Zr = reinterpret_cast<const double2 *>(blocks) + 4 * (near_off[b] + (int64_t)row * cols);
+  mxd4 er = {0.0, 0.0, 0.0, 0.0}, ei = er, hr = er, hi = er;
+  double2 cur[4][4];
+  auto load = [&](int cg, double2 (*z)[4]) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int c = min(cg + 4 * j + g, cols - 1);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) z[j][e] = Zr[4 * c + e];
+    }
+  };
+  if (cols > 0) load(16 * wv, cur);
+  for (int cc = 0; cc < cols; cc += kMx64Cols) {
+    const int nc = min(kMx64Cols, cols - cc);
+    __syncthreads();
+    {
+      const int n = threadIdx.x & 15;
+      int e[PER];
+#pragma unroll
+      for (int k = 0; k < PER; ++k)
+        e[k] = e2_list[c0 + cc + min((int)(threadIdx.x + 256 * k) >> 4, nc - 1)];
+#pragma unroll
+      for (int k = 0; k < PER; ++k) {
+        const zd xe = XI[((size_t)e[k] * 2 + 0) * 16 + n], xh = XI[((size_t)e[k] * 2 + 1) * 16 + n];
+        const bool ok = ((int)(threadIdx.x + 256 * k) >> 4) < nc;
+        xsl[threadIdx.x + 256 * k] = ok ? double4{xe.x, xe.y, xh.x, xh.y} : double4{0.0, 0.0, 0.0, 0.0};
+      }
+    }
+    __syncthreads();
+    for (int off = 16 * wv; off < nc; off += 64) {
+      double2 nx[4][4];
+      load(cc + off + 64, nx);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const double4 x = xsl[(off + 4 * j + g) * 16 + m];
+        const double2 z0 = cur[j][0], z1 = cur[j][1], z2 = cur[j][2], z3 = cur[j][3];
+        const double nxi = -x.y, nhi = -x.w;
+        er = SPK_MFMA64(z0.x, x.x, er);
+        ei = SPK_MFMA64(z0.x, x.y, ei);
+        hr = SPK_MFMA64(z2.x, x.x, hr);
+        hi = SPK_MFMA64(z2.x, x.y, hi);
+        er = SPK_MFMA64(z0.y, nxi, er);
+        ei = SPK_MFMA64(z0.y, x.x, ei);
+        hr = SPK_MFMA64(z2.y, nxi, hr);
+        hi = SPK_MFMA64(z2.y, x.x, hi);
+        er = SPK_MFMA64(z1.x, x.z, er);
+        ei = SPK_MFMA64(z1.x, x.w, ei);
+        hr = SPK_MFMA64(z3.x, x.z, hr);
+        hi = SPK_MFMA64(z3.x, x.w, hi);
+        er = SPK_MFMA64(z1.y, nhi, er);
+        ei = SPK_MFMA64(z1.y, x.z, ei);
+        hr = SPK_MFMA64(z3.y, nhi, hr);
+        hi = SPK_MFMA64(z3.y, x.z, hi);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) cur[j][e] = nx[j][e];
+    }
+  }
+  __syncthreads();
+  mxd4 *red = reinterpret_cast<mxd4 *>(xsl);
+  if (wv > 0) {
+    mxd4 *o = red + ((size_t)(wv - 1) * 64 + lane) * 4;
+    o[0] = er;
+    o[1] = ei;
+    o[2] = hr;
+    o[3] = hi;
+  }
+  __syncthreads();
+  if (wv > 0) return;
+#pragma unroll
+  for (int p = 0; p < 3; ++p) {
+    const mxd4 *o = red + ((size_t)p * 64 + lane) * 4;
+    er += o[0];
+    ei += o[1];
+    hr += o[2];
+    hi += o[3];
+  }
+  const int n = m;
+  if (n >= R) return;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int r = rw + 4 * i + g;
+    if (r < nrow) {
+      const int mm = perm[pf + r];
+      zd se = {er[i], ei[i]}, sh = {hr[i], hi[i]};
+      if (yf) {
+        se = se + yf[(size_t)n * 2 * N + mm];
+        sh = sh + yf[(size_t)n * 2 * N + mm + N];
+      }
+      ys[n][mm] = se;
+      ys[n][mm + N] = sh;
+    }
+  }
+}
+
+// aggregation: workgroup = (128 samples, box, q), wave = 32 samples as 2
+// tiles of 16 (tile t: samples sw + 16 t + i); k-step = 4 edges
+__global__ void __launch_bounds__(256) kb_agg_mx64(int S, int R, const int *box_first,
+                                                   const int *box_count, const zd *rt,
+                                                   const zd *XP, zd *C) {
+  __shared__ double4 xl[32 * 16];
+  const int b = blockIdx.y, q = blockIdx.z, k = q >> 1, comp = q & 1;
+  const int p0 = box_first[b], cnt = box_count[b];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int m = lane & 15, g = lane >> 4;
+  const int sw = blockIdx.x * 128 + 32 * wv;
+  const bool active = sw < S;
+  const int sa = min(sw + m, S - 1), sb = min(sw + 16 + m, S - 1);
+  mxd4 acc[2][4];
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc[t][u] = mxd4{0.0, 0.0, 0.0, 0.0};
+  double2 fa[4], fb[4];
+  auto load = [&](int e0, double2 *a, double2 *bb) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int e = min(e0 + 4 * j + g, cnt - 1);
+      const double2 *f = reinterpret_cast<const double2 *>(rt + ((size_t)(p0 + e) * 4 + q) * S);
+      a[j] = f[sa];
+      bb[j] = f[sb];
+    }
+  };
+  if (active && cnt > 0) load(0, fa, fb);
+  for (int e0 = 0; e0 < cnt; e0 += 32) {
+    const int ne = min(32, cnt - e0);
+    __syncthreads();
+    {
+      const int n = threadIdx.x & 15;
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int e = (int)(threadIdx.x + 256 * u) >> 4;
+        const size_t pe = (size_t)(p0 + e0 + min(e, ne - 1)) * 2;
+        const zd x0 = XP[pe * 16 + n], x1 = XP[(pe + 1) * 16 + n];
+        xl[threadIdx.x + 256 * u] = e < ne ? double4{x0.x, x0.y, x1.x, x1.y} : double4{0.0, 0.0, 0.0, 0.0};
+      }
+    }
+    __syncthreads();
+    if (!active) continue;
+    for (int i = 0; i < ne; i += 16) {
+      double2 na[4], nb[4];
+      load(e0 + i + 16, na, nb);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const double4 x = xl[(i + 4 * j + g) * 16 + m];
+        const double n0 = -x.x, n1 = -x.z;
+        const double ar[2] = {fa[j].x, fb[j].x}, ai[2] = {fa[j].y, fb[j].y};
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+          acc[t][0] = SPK_MFMA64(ar[t], x.x, acc[t][0]);
+          acc[t][1] = SPK_MFMA64(ar[t], x.y, acc[t][1]);
+          acc[t][2] = SPK_MFMA64(ar[t], x.z, acc[t][2]);
+          acc[t][3] = SPK_MFMA64(ar[t], x.w, acc[t][3]);
+        }
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+          acc[t][0] = SPK_MFMA64(ai[t], x.y, acc[t][0]);
+          acc[t][1] = SPK_MFMA64(ai[t], n0, acc[t][1]);
+          acc[t][2] = SPK_MFMA64(ai[t], x.w, acc[t][2]);
+          acc[t][3] = SPK_MFMA64(ai[t], n1, acc[t][3]);
+        }
+      }
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        fa[j] = na[j];
+        fb[j] = nb[j];
+      }
+    }
+  }
+  const int n = m;
+  if (!active || n >= R) return;
+  zd *Cb = C + (size_t)b * 8 * S * R;
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int s = sw + 16 * t + 4 * v + g;
+      if (s < S) {
+        Cb[((size_t)(comp * 4 + 0 + k) * S + s) * R + n] = zd{acc[t][0][v], acc[t][1][v]};
+        Cb[((size_t)(comp * 4 + 2 + k) * S + s) * R + n] = zd{acc[t][2][v], acc[t][3][v]};
+      }
+    }
+}
+
+// disaggregation: workgroup = (up to 128 edges of a box, sample part),
+// wave = 16 edges; k-step = 4 samples (lane group g takes sample s + 4g + j
+// in step j); W staged in LDS 16 samples at a time as [k][sample][e/h][r]
+constexpr int kMx64SC = 16;
+__global__ void __launch_bounds__(512) kb_disagg_mx64(int N, int S, int R, const int *tile_box,
+                                                      const int *tile_p0, const int *box_first,
+                                                      const int *box_count, const int *perm,
+                                                      const zd *rt, const zd *W, zd sc1, zd sc2,
+                                                      int sblk, zd *yp) {
+  __shared__ double4 wl[2 * kMx64SC * 2 * 16];
+  const int b = tile_box[blockIdx.x], p0 = tile_p0[blockIdx.x];
+  const int cnt = min(128, box_first[b] + box_count[b] - p0);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int m = lane & 15, g = lane >> 4;
+  const int ew = 16 * wv;
+  const bool active = ew < cnt;
+  const double2 *F = reinterpret_cast<const double2 *>(rt + (size_t)(p0 + min(ew + m, cnt - 1)) * 4 * S);
+  const size_t SR = (size_t)S * R;
+  const zd *Wb = W + (size_t)b * 8 * SR;
+  const int s_lo = blockIdx.y * sblk, s_hi = min(S, s_lo + sblk);
+  mxd4 er = {0.0, 0.0, 0.0, 0.0}, ei = er, hr = er, hi = er;
+  // pattern rows 2k and 2k+1 at samples s + 4g .. s + 4g + 3 (clamped)
+  auto load = [&](int s, int k, double2 *a) {
+    const double2 *fa = F + (size_t)(2 * k) * S, *fb = fa + S;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int sj = min(s + 4 * g + j, S - 1);
+      a[j] = fa[sj];
+      a[4 + j] = fb[sj];
+    }
+  };
+  double2 cur[8];
+  if (active && s_lo < s_hi) load(s_lo, 0, cur);
+  for (int s0 = s_lo; s0 < s_hi; s0 += kMx64SC) {
+    __syncthreads();
+    {
+      // W[it][s0 + i][r] -> wl as complex [((k * SC + i) * 2 + half) * 16 + r][jj], it = 4k + 2half + jj
+      zd *wc = reinterpret_cast<zd *>(wl);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int t = threadIdx.x + 512 * u;
+        const int r = t & 15, i = (t >> 4) & (kMx64SC - 1), it = t >> 8;
+        const int s = s0 + i;
+        const bool ok = r < R && s < s_hi;
+        const zd v = Wb[((size_t)it * S + min(s, S - 1)) * R + min(r, R - 1)];
+        const int k = it >> 2, half = (it >> 1) & 1, jj = it & 1;
+        wc[(((k * kMx64SC + i) * 2 + half) * 16 + r) * 2 + jj] = ok ? v : zd{0.0, 0.0};
+      }
+    }
+    __syncthreads();
+    if (!active) continue;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      double2 nx[8];
+      if (k == 0)
+        load(s0, 1, nx);
+      else
+        load(s0 + kMx64SC, 0, nx);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int i = 4 * g + j;
+        const double4 we = wl[((k * kMx64SC + i) * 2 + 0) * 16 + m];  // (e,a), (e,b)
+        const double4 wh = wl[((k * kMx64SC + i) * 2 + 1) * 16 + m];  // (h,a), (h,b)
+        const double ar = cur[j].x, ai = cur[j].y, br = cur[4 + j].x, bi = cur[4 + j].y;
+        const double nai = -ai, nbi = -bi;
+        er = SPK_MFMA64(ar, we.x, er);
+        ei = SPK_MFMA64(ar, we.y, ei);
+        hr = SPK_MFMA64(ar, wh.x, hr);
+        hi = SPK_MFMA64(ar, wh.y, hi);
+        er = SPK_MFMA64(nai, we.y, er);
+        ei = SPK_MFMA64(ai, we.x, ei);
+        hr = SPK_MFMA64(nai, wh.y, hr);
+        hi = SPK_MFMA64(ai, wh.x, hi);
+        er = SPK_MFMA64(br, we.z, er);
+        ei = SPK_MFMA64(br, we.w, ei);
+        hr = SPK_MFMA64(br, wh.z, hr);
+        hi = SPK_MFMA64(br, wh.w, hi);
+        er = SPK_MFMA64(nbi, we.w, er);
+        ei = SPK_MFMA64(bi, we.z, ei);
+        hr = SPK_MFMA64(nbi, wh.w, hr);
+        hi = SPK_MFMA64(bi, wh.z, hi);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int u = 0; u < 8; ++u) cur[u] = nx[u];
+    }
+  }
+  const int n = m;
+  if (!active || n >= R) return;
+  zd *yf = yp + (size_t)blockIdx.y * R * 2 * N;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int e = ew + 4 * v + g;
+    if (e < cnt) {
+      const int mm = perm[p0 + e];
+      yf[(size_t)n * 2 * N + mm] = zd{er[v], ei[v]} * sc1;
+      yf[(size_t)n * 2 * N + mm + N] = zd{hr[v], hi[v]} * sc2;
+    }
+  }
+}
+
 // padded batch width: R rounded up to a power of two
 inline int batch_pad(int R) { return R <= 1 ? 1 : R <= 2 ? 2 : R <= 4 ? 4 : R <= 8 ? 8 : 16; }
 
@@ -1047,6 +1367,12 @@ void launch_batch_agg(int nbl, int S, int N, int R, const int *box_first, const 
       hipLaunchKernelGGL(kb_agg_mx, g, dim3(256), 0, st, S, R, box_first, box_count, rt, XP, C);
       return;
     }
+  } else {
+    if (batch_pad(R) == 16) {
+      hipLaunchKernelGGL(kb_agg_mx64, dim3(bnblk(S, 128), nbl, 4), dim3(256), 0, st, S, R, box_first,
+                         box_count, rt, XP, C);
+      return;
+    }
   }
   switch (batch_pad(R)) {
 #define SPK_AGG_CASE(RP)                                                                       \
@@ -1109,13 +1435,15 @@ void launch_batch_disagg(int ntiles, int N, int S, int R, const int *tile_box, c
   const int sblk = (int)bnblk(bnblk(S, 16), kBatchSplit) * 16;
   const dim3 g(ntiles, kBatchSplit);
   bool done = false;
-  if constexpr (std::is_same<T, zf>::value) {
-    if (batch_pad(R) == 16 && mx_tiles > 0) {
+  if (batch_pad(R) == 16 && mx_tiles > 0) {
+    if constexpr (std::is_same<T, zf>::value)
       hipLaunchKernelGGL(kb_disagg_mx, dim3(mx_tiles, kBatchSplit), dim3(512), 0, st, N, S, R,
                          mx_box, mx_p0, box_first, box_count, perm, rt, W, cvt<T>(sc1),
                          cvt<T>(sc2), sblk, yp);
-      done = true;
-    }
+    else
+      hipLaunchKernelGGL(kb_disagg_mx64, dim3(mx_tiles, kBatchSplit), dim3(512), 0, st, N, S, R,
+                         mx_box, mx_p0, box_first, box_count, perm, rt, W, sc1, sc2, sblk, yp);
+    done = true;
   }
   if (!done) switch (batch_pad(R)) {
 #define SPK_DIS_CASE(RP)                                                                        \
@@ -1137,6 +1465,13 @@ void launch_batch_near(int nwg, int N, int R, const int *row_box, const int *row
                        const T *blocks, const T *XI, const T *yf, T *const *ys,
                        hipStream_t st, const uint32_t *blocks_bf) {
   if (nwg <= 0) return;
+  if constexpr (std::is_same<T, zd>::value) {
+    if (batch_pad(R) == 16) {
+      hipLaunchKernelGGL(kb_near_mx64, dim3(nwg), dim3(256), 0, st, N, R, row_box, row_r0,
+                         box_first, box_count, e2_off, e2_list, near_off, perm, blocks, XI, yf, ys);
+      return;
+    }
+  }
   if constexpr (std::is_same<T, zf>::value) {
     if (batch_pad(R) == 16) {
       if (blocks_bf)
