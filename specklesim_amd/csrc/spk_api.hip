@@ -12,6 +12,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <future>
 #include <thread>
 #include <type_traits>
 #include <vector>
@@ -242,7 +243,27 @@ int near_fill(spk_handle *h) {
   return SPK_OK;
 }
 
-int setup_mlfmm(spk_handle *h) {
+// the host's per-level table inputs: TL slot factors and SS/RR phases
+struct HostLevelTables {
+  std::vector<double> rhat;
+  std::vector<cplx> coef, up, down;
+  std::vector<int> live;
+};
+static std::vector<HostLevelTables> host_level_tables(const Tables &tb, const Material &mat,
+                                                      const Tree &t) {
+  std::vector<HostLevelTables> v(t.l_max + 1);
+  std::vector<std::thread> th;
+  for (int l = t.l_min; l <= t.l_max; ++l)
+    th.emplace_back([&, l]() {
+      tl_coefficients(tb, mat, t, l, &v[l].rhat, &v[l].coef, &v[l].live);
+      make_phase_level(tb, mat, t, l, &v[l].up, &v[l].down);
+    });
+  for (auto &x : th) x.join();
+  return v;
+}
+
+// pre: the host tables (computed while the near fill runs on the device)
+int setup_mlfmm(spk_handle *h, std::future<std::vector<HostLevelTables>> &pre) {
   const Tree &t = h->tree;
   const int S = h->tb.S, N = h->N;
   double t0 = now_ms();
@@ -270,6 +291,7 @@ int setup_mlfmm(spk_handle *h) {
   HIPCHK(hipStreamSynchronize(h->st));
   h->stats.patterns_ms = now_ms() - t0;
   t0 = now_ms();
+  const std::vector<HostLevelTables> ht = pre.get();
   // levels
   h->lev.clear();
   h->lev.resize(t.l_max + 1);
@@ -288,10 +310,9 @@ int setup_mlfmm(spk_handle *h) {
     HIPCHK(D.i4_slot.upload(L.i4_slot));
     {
       // TL on the device from the host's per-slot Hankel factors
-      std::vector<double> rhat;
-      std::vector<cplx> coef;
-      std::vector<int> live;
-      tl_coefficients(h->tb, h->mat, t, l, &rhat, &coef, &live);
+      const std::vector<double> &rhat = ht[l].rhat;
+      const std::vector<cplx> &coef = ht[l].coef;
+      const std::vector<int> &live = ht[l].live;
       std::vector<zd> cz(coef.size());
       for (size_t i = 0; i < coef.size(); ++i) cz[i] = zd{coef[i].re, coef[i].im};
       DBuf<double> d_rhat;
@@ -306,8 +327,7 @@ int setup_mlfmm(spk_handle *h) {
       HIPCHK(hipGetLastError());
       HIPCHK(hipStreamSynchronize(h->st));
     }
-    std::vector<cplx> up, down;
-    make_phase_level(h->tb, h->mat, t, l, &up, &down);
+    const std::vector<cplx> &up = ht[l].up, &down = ht[l].down;
     std::vector<zd> uz(up.size()), dz(down.size());
     for (size_t i = 0; i < up.size(); ++i) {
       uz[i] = zd{up[i].re, up[i].im};
@@ -658,11 +678,15 @@ int spk_setup_tri(spk_handle *h, int np, const double *pp, int nt, const int *tt
   HIPCHK(h->d_y.alloc(2 * (size_t)h->N));
   HIPCHK(h->d_yfar.alloc(2 * (size_t)h->N));
   HIPCHK(h->d_tmp.alloc(2 * (size_t)h->N));
+  // the host part of the level tables runs beside the device's near fill
+  std::future<std::vector<HostLevelTables>> pre =
+      std::async(std::launch::async, host_level_tables, std::cref(h->tb), std::cref(h->mat),
+                 std::cref(h->tree));
   double t1 = now_ms();
   rc = near_fill(h);
   if (rc != SPK_OK) return rc;
   h->stats.near_fill_ms = now_ms() - t1;
-  rc = setup_mlfmm(h);
+  rc = setup_mlfmm(h, pre);
   if (rc != SPK_OK) return rc;
   h->stats.n_edges = h->N;
   h->stats.near_blocks = t.nnz;

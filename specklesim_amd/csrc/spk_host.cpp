@@ -374,22 +374,41 @@ void build_tree(const std::vector<FnRec> &fn, int N, int s_opt, Tree *t) {
       t->pos[m] = p;
     }
   }
-  // E2 lists per leaf box (sorted edge numbers, sort_edge_index)
+  // E2 lists per leaf box (sorted edge numbers, sort_edge_index): the
+  // neighbour boxes and list sizes first, then the boxes' lists filled and
+  // sorted in parallel (each box's segment is its own)
   t->e2_off.assign(nbl + 1, 0);
-  t->e2_list.clear();
   t->near_off.assign(nbl + 1, 0);
+  std::vector<int> nbr(27 * (size_t)nbl, -1);
   for (int b = 0; b < nbl; ++b) {
     int64_t bx[27];
-    int nb = e2_boxes(LL.n[b], l_max, bx);
-    size_t start = t->e2_list.size();
+    const int nb = e2_boxes(LL.n[b], l_max, bx);
+    int cnt = 0, w = 0;
     for (int q = 0; q < nb; ++q) {
-      int p = find_box(LL.n, bx[q]);
+      const int p = find_box(LL.n, bx[q]);
       if (p < 0) continue;
-      for (int k = 0; k < t->box_count[p]; ++k) t->e2_list.push_back(t->perm[t->box_first[p] + k]);
+      nbr[27 * (size_t)b + w++] = p;
+      cnt += t->box_count[p];
     }
-    std::sort(t->e2_list.begin() + start, t->e2_list.end());
-    t->e2_off[b + 1] = (int)t->e2_list.size();
-    t->near_off[b + 1] = t->near_off[b] + (int64_t)t->box_count[b] * (t->e2_off[b + 1] - t->e2_off[b]);
+    t->e2_off[b + 1] = t->e2_off[b] + cnt;
+    t->near_off[b + 1] = t->near_off[b] + (int64_t)t->box_count[b] * cnt;
+  }
+  t->e2_list.assign(t->e2_off[nbl], 0);
+  {
+    auto fill = [&](int b0, int b1) {
+      for (int b = b0; b < b1; ++b) {
+        int *dst = t->e2_list.data() + t->e2_off[b];
+        for (int q = 0; q < 27 && nbr[27 * (size_t)b + q] >= 0; ++q) {
+          const int p = nbr[27 * (size_t)b + q];
+          for (int k = 0; k < t->box_count[p]; ++k) *dst++ = t->perm[t->box_first[p] + k];
+        }
+        std::sort(t->e2_list.data() + t->e2_off[b], dst);
+      }
+    };
+    const int nt = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    for (int w = 0; w < nt; ++w) th.emplace_back(fill, (int)((int64_t)nbl * w / nt), (int)((int64_t)nbl * (w + 1) / nt));
+    for (auto &x : th) x.join();
   }
   t->nnz = t->near_off[nbl];
   // per-level parent/child/I4
