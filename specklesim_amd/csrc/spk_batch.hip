@@ -1066,6 +1066,127 @@ __global__ void __launch_bounds__(512) kb_disagg_mx64(int N, int S, int R, const
   }
 }
 
+// ---------------------------------------------------------------------------
+// The near-field preconditioner's product for 9..16 right-hand sides on the
+// bf16 matrix cores (v_mfma_f32_16x16x16bf16_1k: lane g*16 + m holds
+// A[m][4g .. 4g+3] and B[4g .. 4g+3][n = m], D[4g + i][n] in element i, as
+// probed on the device).  One product covers 4 columns: k = 4 * column +
+// component, A = the bf16 block of (row, column) as stored: (Z0r, Z0i, Z1r,
+// Z1i) for the e rows and (Z2r, Z2i, Z3r, Z3i) for the h rows; B = x in bf16
+// arranged per (column, right-hand side) as (xer, -xei, xhr, -xhi) for the
+// real parts and (xei, xer, xhi, xhr) for the imaginary parts.  Products
+// accumulate in fp32.  Workgroup = (box, 16 rows), the 4 waves take every
+// fourth group of 16 columns, x staged in LDS 128 columns at a time.
+// ---------------------------------------------------------------------------
+typedef short mxs4 __attribute__((ext_vector_type(4)));
+__device__ inline uint32_t bf16_bits(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even (finite values)
+  return u >> 16;
+}
+__device__ inline uint32_t bf16_pair(float lo, float hi) { return bf16_bits(lo) | (bf16_bits(hi) << 16); }
+__device__ inline mxs4 as_s4(uint32_t a, uint32_t b) {
+  mxs4 v;
+  v[0] = (short)(a & 0xFFFFu);
+  v[1] = (short)(a >> 16);
+  v[2] = (short)(b & 0xFFFFu);
+  v[3] = (short)(b >> 16);
+  return v;
+}
+__global__ void __launch_bounds__(256) kb_near_mxb(int N, int R, const int *row_box,
+                                                   const int *row_r0, const int *box_first,
+                                                   const int *box_count, const int *e2_off,
+                                                   const int *e2_list, const int64_t *near_off,
+                                                   const int *perm, const uint4 *blocks, const zf *XI,
+                                                   zf *const *ys) {
+  constexpr int PER = kMxCols * 16 / 256;
+  __shared__ uint4 xsl[kMxCols * 16];
+  const int b = row_box[blockIdx.x];
+  const int pf = box_first[b], nrow = box_count[b];
+  const int c0 = e2_off[b], cols = e2_off[b + 1] - c0;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int m = lane & 15, g = lane >> 4;
+  const int rw = row_r0[blockIdx.x];
+  const int row = min(rw + m, nrow - 1);
+  const uint4 *Zr = blocks + near_off[b] + (int64_t)row * cols;
+  mxf4 er = {0.f, 0.f, 0.f, 0.f}, ei = er, hr = er, hi = er;
+  uint4 cur[4];
+  auto load = [&](int cg, uint4 *z) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) z[j] = Zr[min(cg + 4 * j + g, cols - 1)];
+  };
+  if (cols > 0) load(16 * wv, cur);
+  for (int cc = 0; cc < cols; cc += kMxCols) {
+    const int nc = min(kMxCols, cols - cc);
+    __syncthreads();
+    {
+      const int n = threadIdx.x & 15;
+      int e[PER];
+#pragma unroll
+      for (int k = 0; k < PER; ++k)
+        e[k] = e2_list[c0 + cc + min((int)(threadIdx.x + 256 * k) >> 4, nc - 1)];
+#pragma unroll
+      for (int k = 0; k < PER; ++k) {
+        const zf xe = XI[((size_t)e[k] * 2 + 0) * 16 + n], xh = XI[((size_t)e[k] * 2 + 1) * 16 + n];
+        const bool ok = ((int)(threadIdx.x + 256 * k) >> 4) < nc;
+        const uint4 v = {bf16_pair(xe.x, -xe.y), bf16_pair(xh.x, -xh.y), bf16_pair(xe.y, xe.x),
+                         bf16_pair(xh.y, xh.x)};
+        xsl[threadIdx.x + 256 * k] = ok ? v : uint4{0u, 0u, 0u, 0u};
+      }
+    }
+    __syncthreads();
+    for (int off = 16 * wv; off < nc; off += 64) {
+      uint4 nx[4];
+      load(cc + off + 64, nx);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const uint4 x = xsl[(off + 4 * j + g) * 16 + m];
+        const mxs4 be = as_s4(x.x, x.y), bi = as_s4(x.z, x.w);
+        const mxs4 ae = as_s4(cur[j].x, cur[j].y), ah = as_s4(cur[j].z, cur[j].w);
+        er = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ae, be, er, 0, 0, 0);
+        ei = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ae, bi, ei, 0, 0, 0);
+        hr = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, be, hr, 0, 0, 0);
+        hi = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bi, hi, 0, 0, 0);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) cur[j] = nx[j];
+    }
+  }
+  // waves 1..3 hand their sums to wave 0 through LDS
+  __syncthreads();
+  mxf4 *red = reinterpret_cast<mxf4 *>(xsl);
+  if (wv > 0) {
+    mxf4 *o = red + ((size_t)(wv - 1) * 64 + lane) * 4;
+    o[0] = er;
+    o[1] = ei;
+    o[2] = hr;
+    o[3] = hi;
+  }
+  __syncthreads();
+  if (wv > 0) return;
+#pragma unroll
+  for (int p = 0; p < 3; ++p) {
+    const mxf4 *o = red + ((size_t)p * 64 + lane) * 4;
+    er += o[0];
+    ei += o[1];
+    hr += o[2];
+    hi += o[3];
+  }
+  const int n = m;
+  if (n >= R) return;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int r = rw + 4 * g + i;
+    if (r < nrow) {
+      const int mm = perm[pf + r];
+      ys[n][mm] = zf{er[i], ei[i]};
+      ys[n][mm + N] = zf{hr[i], hi[i]};
+    }
+  }
+}
+
 // padded batch width: R rounded up to a power of two
 inline int batch_pad(int R) { return R <= 1 ? 1 : R <= 2 ? 2 : R <= 4 ? 4 : R <= 8 ? 8 : 16; }
 
@@ -1474,7 +1595,11 @@ void launch_batch_near(int nwg, int N, int R, const int *row_box, const int *row
   }
   if constexpr (std::is_same<T, zf>::value) {
     if (batch_pad(R) == 16) {
-      if (blocks_bf)
+      if (blocks_bf && !yf)  // the preconditioner's product: bf16 operands throughout
+        hipLaunchKernelGGL(kb_near_mxb, dim3(nwg), dim3(256), 0, st, N, R, row_box, row_r0,
+                           box_first, box_count, e2_off, e2_list, near_off, perm,
+                           (const uint4 *)blocks_bf, XI, ys);
+      else if (blocks_bf)
         hipLaunchKernelGGL(kb_near_mx<true>, dim3(nwg), dim3(256), 0, st, N, R, row_box, row_r0,
                            box_first, box_count, e2_off, e2_list, near_off, perm,
                            (const void *)blocks_bf, XI, yf, ys);
