@@ -279,6 +279,10 @@ int spk_profile_matvec_c64(spk_handle *h, int reps, double *ms, double *bytes);
  * reps launches, *bytes = its algorithmic HBM bytes (16 B per 2x2 block,
  * x read and y written at 8 B per complex). */
 int spk_profile_near_bf(spk_handle *h, int reps, double *ms, double *bytes);
+/* No counterpart in the reference: the same timing for the single-vector
+ * preconditioner's fp8 copy (OCP E4M3FN, 8 B per 2x2 block plus 16 B of
+ * per-row component scales per edge). */
+int spk_profile_near_f8(spk_handle *h, int reps, double *ms, double *bytes);
 
 /* Host-only setup probe (no GPU needed): RWG topology and octree of a mesh
  * exactly as spk_setup_tri builds them.  corner/element: N x 2 (1-based),

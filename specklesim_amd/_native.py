@@ -27,7 +27,8 @@ EXPORTS = [
     "spk_device_alloc", "spk_device_free", "spk_device_copy", "spk_device_synchronize",
     "spk_gmres", "spk_gmres_device", "spk_gmres_history", "spk_get_rwg", "spk_get_tree",
     "spk_get_i4", "spk_get_near_csr", "spk_near_blocks", "spk_get_patterns",
-    "spk_profile_matvec", "spk_profile_matvec_c64", "spk_profile_near_bf", "spk_topology_probe",
+    "spk_profile_matvec", "spk_profile_matvec_c64", "spk_profile_near_bf", "spk_profile_near_f8",
+    "spk_topology_probe",
 ]
 STAGES = ["aggregation", "ss", "m2l", "rr", "fold", "disaggregation", "near_spmv", "matvec"]
 
@@ -132,6 +133,7 @@ def lib():
     L.spk_get_patterns.argtypes = [vp, C.c_int, C.c_int, dp]
     L.spk_profile_matvec.argtypes = [vp, C.c_int, dp, dp]
     L.spk_profile_matvec_c64.argtypes = [vp, C.c_int, dp, dp]
+    L.spk_profile_near_f8.argtypes = [vp, C.c_int, dp, dp]
     L.spk_profile_near_bf.argtypes = [vp, C.c_int, dp, dp]
     L.spk_topology_probe.argtypes = [C.c_int, dp, C.c_int, ip, C.c_int, C.c_int, i64p, ip, ip,
                                      i64p, ip, ip, dp, dp]

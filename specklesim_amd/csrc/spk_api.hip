@@ -118,6 +118,9 @@ struct spk_handle {
   int ntiles = 0;
   // bf16 copy of the complex64 near blocks (near-field preconditioner), built on first use
   DBuf<uint32_t> d_blk_bf;
+  // fp8 (E4M3FN) copy with per-row, per-component scales (single-vector preconditioner)
+  DBuf<uint32_t> d_blk_f8;
+  DBuf<float> d_f8_scale;
   // matrix-core disaggregation tiles: (box, first position) of up to 128 edges
   DBuf<int> d_dx_box, d_dx_p0;
   int dx_tiles = 0;
@@ -609,6 +612,9 @@ int spk_setup_tri(spk_handle *h, int np, const double *pp, int nt, const int *tt
   h->ready = false;
   h->mixed_ready = false;
   h->diag_ready = false;
+  // the low-precision near-field copies belong to the previous operator
+  h->d_blk_bf.reset();
+  h->d_blk_f8.reset();
   h->stats = spk_stats{};
   const double t0 = now_ms();
   std::string err;
@@ -1470,6 +1476,18 @@ static int ensure_near_bf(spk_handle *h) {
   return SPK_OK;
 }
 
+// the fp8 copy of the complex64 near blocks (built once per setup)
+static int ensure_near_f8(spk_handle *h) {
+  if (h->d_blk_f8.n != h->f_blocks.n / 2) {
+    HIPCHK(h->d_blk_f8.alloc(h->f_blocks.n / 2));
+    HIPCHK(h->d_f8_scale.alloc(4 * (size_t)h->N));
+    launch_near_to_f8(h->N, h->d_leafbox_of_pos.p, h->d_box_first.p, h->d_e2_off.p,
+                      h->d_near_off.p, h->f_blocks.p, h->d_blk_f8.p, h->d_f8_scale.p, h->st);
+    HIPCHK(hipGetLastError());
+  }
+  return SPK_OK;
+}
+
 // Flexible GMRES on the FP64 operator for R <= kBatchMax right-hand sides,
 // right-preconditioned by kin GMRES steps (from zero) on the near-field
 // operator: complex64 on an SPK_MIXED handle (T = zf), FP64 otherwise.  The
@@ -1492,11 +1510,17 @@ static int fgmres_near(spk_handle *h, int R, const zd *const *b, zd *const *x,
     return fail(SPK_ERR_STATE, "handle was not created with SPK_MIXED");
   // complex64 inner solves run on the bf16 copy of the near field: the
   // preconditioner only has to approximate the near-field solve
-  const uint32_t *bf = nullptr;
+  const uint32_t *bf = nullptr, *f8 = nullptr;
   if constexpr (std::is_same<T, zf>::value) {
     const int e = ensure_near_bf(h);
     if (e) return e;
     bf = h->d_blk_bf.p;
+    const char *ev = getenv("SPK_NEAR_F8");
+    if (R < 4 && !(ev && ev[0] == '0')) {
+      const int e8 = ensure_near_f8(h);
+      if (e8) return e8;
+      f8 = h->d_blk_f8.p;
+    }
   }
   const double t_start = now_ms();
   const int64_t n = 2 * (int64_t)h->N;
@@ -1664,6 +1688,12 @@ static int fgmres_near(spk_handle *h, int R, const zd *const *b, zd *const *x,
         for (int s = 0; s < P && !rc; ++s) {
           T *ui = U.p + s * ldu + (size_t)j * n, *wo = Wi.p + (size_t)s * n;
           if constexpr (std::is_same<T, zf>::value) {
+            if (f8) {
+              launch_near_spmv_f8(h->spmv_wgs, h->N, h->d_row_box.p, h->d_row_r0.p, h->d_box_first.p,
+                                  h->d_box_count.p, h->d_e2_off.p, h->d_e2_list.p, h->d_near_off.p,
+                                  h->d_perm.p, f8, h->d_f8_scale.p, ui, wo, h->st);
+              continue;
+            }
             if (bf) {
               launch_near_spmv_bf(h->spmv_wgs, h->N, h->d_row_box.p, h->d_row_r0.p, h->d_box_first.p,
                                   h->d_box_count.p, h->d_e2_off.p, h->d_e2_list.p, h->d_near_off.p,
@@ -2352,11 +2382,13 @@ int spk_profile_matvec_c64(spk_handle *h, int reps, double *ms, double *bytes) {
   return profile_matvec_t<zf>(h, reps, ms, bytes);
 }
 
-int spk_profile_near_bf(spk_handle *h, int reps, double *ms, double *bytes) {
+} // extern "C"
+// times the preconditioner's near SpMV on the bf16 (f8 false) or fp8 copy
+static int profile_near_lp(spk_handle *h, int reps, double *ms, double *bytes, bool f8) {
   if (!h || reps < 1 || !ms || !bytes) return fail(SPK_ERR_ARG, "bad arguments");
   if (!h->ready) return fail(SPK_ERR_STATE, "spk_setup_tri has not completed");
   if (!h->mixed_ready) return fail(SPK_ERR_STATE, "handle was not created with SPK_MIXED");
-  const int rc = ensure_near_bf(h);
+  const int rc = f8 ? ensure_near_f8(h) : ensure_near_bf(h);
   if (rc) return rc;
   const int64_t n = 2 * (int64_t)h->N;
   DBuf<zf> x, y;
@@ -2367,9 +2399,14 @@ int spk_profile_near_bf(spk_handle *h, int reps, double *ms, double *bytes) {
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
   auto run = [&]() {
-    launch_near_spmv_bf(h->spmv_wgs, h->N, h->d_row_box.p, h->d_row_r0.p, h->d_box_first.p,
-                        h->d_box_count.p, h->d_e2_off.p, h->d_e2_list.p, h->d_near_off.p,
-                        h->d_perm.p, h->d_blk_bf.p, x.p, y.p, h->st);
+    if (f8)
+      launch_near_spmv_f8(h->spmv_wgs, h->N, h->d_row_box.p, h->d_row_r0.p, h->d_box_first.p,
+                          h->d_box_count.p, h->d_e2_off.p, h->d_e2_list.p, h->d_near_off.p,
+                          h->d_perm.p, h->d_blk_f8.p, h->d_f8_scale.p, x.p, y.p, h->st);
+    else
+      launch_near_spmv_bf(h->spmv_wgs, h->N, h->d_row_box.p, h->d_row_r0.p, h->d_box_first.p,
+                          h->d_box_count.p, h->d_e2_off.p, h->d_e2_list.p, h->d_near_off.p,
+                          h->d_perm.p, h->d_blk_bf.p, x.p, y.p, h->st);
   };
   run();
   HIPCHK(hipEventRecord(e0, h->st));
@@ -2381,8 +2418,19 @@ int spk_profile_near_bf(spk_handle *h, int reps, double *ms, double *bytes) {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   *ms = (double)t / reps;
-  *bytes = (double)(h->d_blk_bf.n / 4) * 16.0 + 3.0 * (double)n * 8.0;
+  const double nblocks = (double)(h->f_blocks.n / 4);
+  *bytes = f8 ? nblocks * 8.0 + 16.0 * (double)h->N + 3.0 * (double)n * 8.0
+              : nblocks * 16.0 + 3.0 * (double)n * 8.0;
   return SPK_OK;
+}
+extern "C" {
+
+int spk_profile_near_bf(spk_handle *h, int reps, double *ms, double *bytes) {
+  return profile_near_lp(h, reps, ms, bytes, false);
+}
+
+int spk_profile_near_f8(spk_handle *h, int reps, double *ms, double *bytes) {
+  return profile_near_lp(h, reps, ms, bytes, true);
 }
 
 int spk_topology_probe(int np, const double *pp, int nt, const int *tt, int normal_convention,

@@ -269,6 +269,18 @@ void launch_near_spmv_bf(int nwg, int N, const int *row_box, const int *row_r0,
                          const int *box_first, const int *box_count, const int *e2_off,
                          const int *e2_list, const int64_t *near_off, const int *perm,
                          const uint32_t *blocks, const zf *x, zf *y, hipStream_t st);
+// the near field in fp8 (OCP E4M3FN) for the single-vector preconditioner:
+// 8 bytes per 2x2 block (Z0r, Z0i, Z1r, Z1i, Z2r, Z2i, Z3r, Z3i), scale[4 p + k]
+// = the scale of component k of the row at position p (its largest |re|,
+// |im| maps to 448); one workgroup per row position
+void launch_near_to_f8(int N, const int *leaf_box_of_pos, const int *box_first, const int *e2_off,
+                       const int64_t *near_off, const zf *blocks, uint32_t *out, float *scale,
+                       hipStream_t st);
+void launch_near_spmv_f8(int nwg, int N, const int *row_box, const int *row_r0,
+                         const int *box_first, const int *box_count, const int *e2_off,
+                         const int *e2_list, const int64_t *near_off, const int *perm,
+                         const uint32_t *blocks, const float *scale, const zf *x, zf *y,
+                         hipStream_t st);
 // one Arnoldi step (one classical Gram-Schmidt pass, three launches) of P
 // near-field inner solves after w_s = A U_s[:, j]: h_s = U_s[:, 0..j]^H w_s
 // (also in hb + s * ldh), w_s -= U_s h_s, H_s[:, j] = (h_s, |w_s|),

@@ -1239,6 +1239,157 @@ void launch_near_spmv_bf(int nwg, int N, const int *row_box, const int *row_r0,
                      box_count, e2_off, e2_list, near_off, perm, (const uint4 *)blocks, x, y);
 }
 
+// Near field in fp8 (OCP E4M3FN, the single-vector preconditioner's copy):
+// a 2x2 block is 8 bytes (Z0r, Z0i, Z1r, Z1i, Z2r, Z2i, Z3r, Z3i); every
+// scalar row and component has its own scale (the row's largest |re|, |im|
+// of that component maps to 448), scale[p] = (s0, s1, s2, s3) for position p.
+__device__ inline uint32_t f32_to_e4m3(float x) {
+  const uint32_t sg = x < 0.f ? 0x80u : 0u;
+  const float a = fabsf(x);
+  if (!(a < 448.f)) return sg | 0x7Eu;  // saturate (finite inputs)
+  if (a < 0.015625f) return sg | (uint32_t)rintf(a * 512.f);  // subnormal (8 -> 0x08, the least normal)
+  int E;
+  const float f = frexpf(a, &E);  // a = f 2^E, f in [0.5, 1)
+  int e = E - 1;
+  uint32_t mnt = (uint32_t)rintf((f * 2.f - 1.f) * 8.f);
+  if (mnt == 8u) {
+    mnt = 0u;
+    ++e;
+  }
+  if (e + 7 > 15 || (e + 7 == 15 && mnt == 7u)) return sg | 0x7Eu;
+  return sg | ((uint32_t)(e + 7) << 3) | mnt;
+}
+// one workgroup per row position: the row's component maxima, then its blocks encoded
+__global__ void __launch_bounds__(256) k_near_to_f8(int N, const int *leaf_box_of_pos,
+                                                    const int *box_first, const int *e2_off,
+                                                    const int64_t *near_off, const zf *blocks,
+                                                    uint2 *out, float4 *scale) {
+  __shared__ float red[4][4];
+  __shared__ float inv[4];
+  const int p = blockIdx.x;
+  const int b = leaf_box_of_pos[p], r = p - box_first[b];
+  const int cols = e2_off[b + 1] - e2_off[b];
+  const int64_t g0 = near_off[b] + (int64_t)r * cols;
+  float mx[4] = {0.f, 0.f, 0.f, 0.f};
+  for (int c = threadIdx.x; c < cols; c += 256)
+    for (int k = 0; k < 4; ++k) {
+      const zf z = blocks[4 * (g0 + c) + k];
+      mx[k] = fmaxf(mx[k], fmaxf(fabsf(z.x), fabsf(z.y)));
+    }
+  for (int k = 0; k < 4; ++k) {
+    float v = mx[k];
+    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_down(v, off, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][k] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    const int k = threadIdx.x;
+    const float m = fmaxf(fmaxf(red[0][k], red[1][k]), fmaxf(red[2][k], red[3][k]));
+    const float sc = m > 0.f ? m / 448.f : 1.f;
+    inv[k] = 1.f / sc;
+    reinterpret_cast<float *>(scale + p)[k] = sc;
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < cols; c += 256) {
+    uint32_t w[2] = {0u, 0u};
+    for (int k = 0; k < 4; ++k) {
+      const zf z = blocks[4 * (g0 + c) + k];
+      const uint32_t lo = f32_to_e4m3(z.x * inv[k]), hi = f32_to_e4m3(z.y * inv[k]);
+      w[k >> 1] |= (lo | (hi << 8)) << (16 * (k & 1));
+    }
+    out[g0 + c] = uint2{w[0], w[1]};
+  }
+}
+void launch_near_to_f8(int N, const int *leaf_box_of_pos, const int *box_first, const int *e2_off,
+                       const int64_t *near_off, const zf *blocks, uint32_t *out, float *scale,
+                       hipStream_t st) {
+  if (N <= 0) return;
+  hipLaunchKernelGGL(k_near_to_f8, dim3(N), dim3(256), 0, st, N, leaf_box_of_pos, box_first, e2_off,
+                     near_off, blocks, (uint2 *)out, (float4 *)scale);
+}
+// the near SpMV of k_near_spmv (complex64 x and y) on the fp8 blocks: the
+// four components are summed apart and scaled once per row.  A wave owns rows
+// r0 + wv + 4q (q < 4) and walks their columns together, two 64-column groups
+// per step, so eight 8-byte loads are in flight per lane and each x value read
+// from LDS serves four rows (rows past the box end read row r0 and are not
+// written; a column past the chunk end reads the last column with x = 0).
+__global__ void __launch_bounds__(256) k_near_spmv_f8(int N, const int *row_box, const int *row_r0,
+                                                      const int *box_first, const int *box_count,
+                                                      const int *e2_off, const int *e2_list,
+                                                      const int64_t *near_off, const int *perm,
+                                                      const uint2 *blocks, const float4 *scale,
+                                                      const zf *x, zf *y) {
+  __shared__ zf xs[2 * kColChunk];
+  const int b = row_box[blockIdx.x], r0 = row_r0[blockIdx.x];
+  const int rows = min(kRowsPerWG, box_count[b] - r0);
+  const int c0 = e2_off[b], cols = e2_off[b + 1] - c0;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  constexpr int NQ = kRowsPerWG / 4;
+  const uint2 *Zq[NQ];
+  for (int q = 0; q < NQ; ++q) {
+    const int r = r0 + wv + 4 * q;
+    Zq[q] = blocks + (near_off[b] + (int64_t)(r < r0 + rows ? r : r0) * cols);
+  }
+  zf a0[NQ], a1[NQ], a2[NQ], a3[NQ];
+  for (int q = 0; q < NQ; ++q) a0[q] = a1[q] = a2[q] = a3[q] = zf{0, 0};
+  auto acc = [&](int q, uint2 v, zf xe, zf xh) {
+    const auto q0 = __builtin_amdgcn_cvt_pk_f32_fp8((int)v.x, false);
+    const auto q1 = __builtin_amdgcn_cvt_pk_f32_fp8((int)v.x, true);
+    const auto q2 = __builtin_amdgcn_cvt_pk_f32_fp8((int)v.y, false);
+    const auto q3 = __builtin_amdgcn_cvt_pk_f32_fp8((int)v.y, true);
+    a0[q] = a0[q] + zf{q0[0], q0[1]} * xe;
+    a1[q] = a1[q] + zf{q1[0], q1[1]} * xh;
+    a2[q] = a2[q] + zf{q2[0], q2[1]} * xe;
+    a3[q] = a3[q] + zf{q3[0], q3[1]} * xh;
+  };
+  for (int cc = 0; cc < cols; cc += kColChunk) {
+    const int nc = min(kColChunk, cols - cc);
+    __syncthreads();
+    for (int c = threadIdx.x; c < nc; c += blockDim.x) {
+      const int n = e2_list[c0 + cc + c];
+      xs[2 * c] = x[n];
+      xs[2 * c + 1] = x[n + N];
+    }
+    __syncthreads();
+    for (int c = lane; c < nc; c += 128) {
+      const int c1 = min(c + 64, nc - 1);
+      uint2 v0[NQ], v1[NQ];
+      for (int q = 0; q < NQ; ++q) {
+        v0[q] = Zq[q][cc + c];
+        v1[q] = Zq[q][cc + c1];
+      }
+      const zf xe0 = xs[2 * c], xh0 = xs[2 * c + 1];
+      zf xe1 = xs[2 * c1], xh1 = xs[2 * c1 + 1];
+      if (c + 64 >= nc) xe1 = xh1 = zf{0, 0};
+      for (int q = 0; q < NQ; ++q) {
+        acc(q, v0[q], xe0, xh0);
+        acc(q, v1[q], xe1, xh1);
+      }
+    }
+  }
+  for (int q = 0; q < NQ; ++q) {
+    const int r = r0 + wv + 4 * q;
+    const zf s0 = wave_sum(a0[q]), s1 = wave_sum(a1[q]), s2 = wave_sum(a2[q]), s3 = wave_sum(a3[q]);
+    if (r < r0 + rows && lane == 0) {
+      const int p = box_first[b] + r;
+      const float4 sc = scale[p];
+      const int m = perm[p];
+      y[m] = s0 * sc.x + s1 * sc.y;
+      y[m + N] = s2 * sc.z + s3 * sc.w;
+    }
+  }
+}
+void launch_near_spmv_f8(int nwg, int N, const int *row_box, const int *row_r0,
+                         const int *box_first, const int *box_count, const int *e2_off,
+                         const int *e2_list, const int64_t *near_off, const int *perm,
+                         const uint32_t *blocks, const float *scale, const zf *x, zf *y,
+                         hipStream_t st) {
+  if (nwg <= 0) return;
+  hipLaunchKernelGGL(k_near_spmv_f8, dim3(nwg), dim3(256), 0, st, N, row_box, row_r0, box_first,
+                     box_count, e2_off, e2_list, near_off, perm, (const uint2 *)blocks,
+                     (const float4 *)scale, x, y);
+}
+
 // complex128 -> complex64 copy (the mixed-precision operator's tables)
 __global__ void k_to_zf(int64_t n, const zd *a, zf *b) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;

@@ -8,11 +8,13 @@ around it.
     python -m specklesim_amd.profile_matvec --copy-mib 1024
     python -m specklesim_amd.profile_matvec --batch 16
     python -m specklesim_amd.profile_matvec --near-bf
+    python -m specklesim_amd.profile_matvec --near-f8
     python -m specklesim_amd.profile_matvec --solve near
 
 ``--copy-mib`` instead runs five plain device-to-device copies of that size
 (torch's copy kernel), the calibration for FETCH_SIZE / WRITE_SIZE counters.
-``--near-bf`` instead repeats the near-field preconditioner's bf16 SpMV.
+``--near-bf`` instead repeats the near-field preconditioner's bf16 SpMV,
+``--near-f8`` its fp8 (E4M3FN) SpMV.
 ``--solve near|mixed|fp64`` instead runs one solve of the normal-incidence
 TM plane wave to 1e-6 (near: the near-field preconditioned flexible GMRES;
 mixed: the refinement solve; fp64: the reference's GMRES with IMGS).
@@ -70,6 +72,7 @@ def main(argv=None):
     ap.add_argument("--copy-mib", type=int, default=0)
     ap.add_argument("--batch", type=int, default=0)
     ap.add_argument("--near-bf", action="store_true")
+    ap.add_argument("--near-f8", action="store_true")
     ap.add_argument("--solve", default="", choices=["", "near", "mixed", "fp64"])
     args = ap.parse_args(argv)
     if args.copy_mib:
@@ -84,7 +87,7 @@ def main(argv=None):
     s = SieTriMlfmm(lam, args.eps_r2, formulation=args.formulation, normal_convention=2,
                     tree_s_opt=250, truncation_number=19,
                     precision=("fp64" if args.solve == "fp64" else "mixed")
-                    if (args.batch or args.near_bf or args.solve) else args.precision)
+                    if (args.batch or args.near_bf or args.near_f8 or args.solve) else args.precision)
     s.precalculation_tri(pp, tt)
     if args.batch:
         print(json.dumps(batched_solve(s, args.batch)), flush=True)
@@ -99,9 +102,10 @@ def main(argv=None):
         print(json.dumps({"solve": args.solve, "iters": inf["iters"], "matvecs": inf["matvecs"],
                           "be": inf["be"], "seconds": inf["seconds"]}), flush=True)
         return
-    if args.near_bf:
-        ms, by = s.profile_near_bf(reps=args.reps)
-        print(json.dumps({"kernel": "k_near_spmv_bf", "reps": args.reps, "ms": ms, "bytes": by,
+    if args.near_bf or args.near_f8:
+        ms, by = s.profile_near_bf(reps=args.reps, fp8=args.near_f8)
+        print(json.dumps({"kernel": "k_near_spmv_f8" if args.near_f8 else "k_near_spmv_bf",
+                          "reps": args.reps, "ms": ms, "bytes": by,
                           "GBps": by / (ms * 1e-3) / 1e9}), flush=True)
         return
     c64 = args.precision == "mixed"

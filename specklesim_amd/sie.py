@@ -330,11 +330,13 @@ class SieTriMlfmm:
                                             n.ctypes.data_as(C.POINTER(C.c_int)), _dp(out)))
         return (out[0::2] + 1j * out[1::2]).reshape(-1, 2, 2)
 
-    def profile_near_bf(self, reps=20):
+    def profile_near_bf(self, reps=20, fp8=False):
         """Mean device time [ms] and algorithmic HBM bytes of the near-field
-        preconditioner's SpMV (bf16 blocks; precision="mixed" handle)."""
+        preconditioner's SpMV (bf16 blocks, or the fp8 copy with fp8=True;
+        precision="mixed" handle)."""
         ms, by = np.zeros(1), np.zeros(1)
-        nat.check(nat.lib().spk_profile_near_bf(self._h, reps, _dp(ms), _dp(by)))
+        fn = nat.lib().spk_profile_near_f8 if fp8 else nat.lib().spk_profile_near_bf
+        nat.check(fn(self._h, reps, _dp(ms), _dp(by)))
         return float(ms[0]), float(by[0])
 
     def profile_matvec(self, reps=20, c64=False):

@@ -46,9 +46,14 @@ def reference(solvers):
     return out
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_near_field_fgmres_single(solvers, reference, precision):
-    s = solvers[precision]
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "mixed-bf16"])
+def test_near_field_fgmres_single(solvers, reference, precision, monkeypatch):
+    """mixed: the inner products of one right-hand side run on the fp8
+    (E4M3FN, per-row component scales) copy of the near field; mixed-bf16:
+    on the bf16 copy (SPK_NEAR_F8=0), as batches of four or more do."""
+    if precision == "mixed-bf16":
+        monkeypatch.setenv("SPK_NEAR_F8", "0")
+    s = solvers[precision.split("-")[0]]
     a, b, x0, it0 = reference[0]
     x, inf = s.gmres_run(b, tolerance=1e-6, max_iterations=650, use_initial_guess=False,
                          near_field=True)
