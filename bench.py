@@ -40,14 +40,15 @@ HBM_PEAK_GBS = 8000.0
 # default workload (100 x 100 cells, 10 lambda, ICTF, complex128), each counter
 # scaled by what it reads for a plain 1 GiB device copy (FETCH x 2.0,
 # WRITE x 1.0); profiles/matvec_traffic_rough10x10_ictf.txt and
-# profiles/near_spmv_bf16_rough10x10_ictf.txt
+# profiles/near_spmv_bf16_rough10x10_ictf.txt, profiles/near_spmv_fp8_rough10x10_ictf.txt
 MEASURED_TRAFFIC = {("near_spmv", 100, 10.0, "fp64"): 1.8555e9,
                     ("aggregation", 100, 10.0, "fp64"): 1.4732e9,
                     ("disaggregation", 100, 10.0, "fp64"): 1.6627e9,
                     ("near_spmv", 100, 10.0, "c64"): 0.9289e9,
                     ("aggregation", 100, 10.0, "c64"): 0.7573e9,
                     ("disaggregation", 100, 10.0, "c64"): 0.8181e9,
-                    ("near_spmv_bf16", 100, 10.0, "bf16"): 0.4716e9}
+                    ("near_spmv_bf16", 100, 10.0, "bf16"): 0.4716e9,
+                    ("near_spmv_fp8", 100, 10.0, "fp8"): 0.2462e9}
 
 
 def _sync():
@@ -255,10 +256,10 @@ def main():
     # matvec (the complex64 operator when the mixed path leads)
     pc = paths["mixed"]["solver"].profile_matvec(reps=20, c64=True) if "mixed" in paths else None
     if head == "near_fgmres":
-        # the near-field preconditioner's bf16 SpMV: about 20 launches per
-        # outer step, the longest-running kernel of this path
-        stage, rprec = "near_spmv_bf16", "bf16"
-        ms, by = H["solver"].profile_near_bf(reps=20)
+        # the near-field preconditioner's fp8 SpMV (one right-hand side): 30
+        # launches per outer step, the longest-running kernel of this path
+        stage, rprec = "near_spmv_fp8", "fp8"
+        ms, by = H["solver"].profile_near_bf(reps=20, fp8=True)
     else:
         rprof, rprec = (pc, "c64") if head == "mixed" else (prof, "fp64")
         stage, (ms, by) = max(((k, v) for k, v in rprof.items() if k != "matvec"),
@@ -309,7 +310,9 @@ def main():
                           if "mixed" in paths and "fp64" in paths else None),
         "roofline": {"kernel": stage,
                      "operator": {"c64": "complex64", "fp64": "complex128",
-                                  "bf16": "bf16 near blocks, complex64 vectors"}[rprec],
+                                  "bf16": "bf16 near blocks, complex64 vectors",
+                                  "fp8": "fp8 (E4M3FN) near blocks with per-row component "
+                                         "scales, complex64 vectors"}[rprec],
                      "bound": "hbm", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic, "algorithmic_bytes": by, "ms": ms,

@@ -1508,18 +1508,22 @@ static int fgmres_near(spk_handle *h, int R, const zd *const *b, zd *const *x,
   if (R < 1 || R > kBatchMax) return fail(SPK_ERR_ARG, "batch size must be 1..16");
   if (std::is_same<T, zf>::value && !h->mixed_ready)
     return fail(SPK_ERR_STATE, "handle was not created with SPK_MIXED");
-  // complex64 inner solves run on the bf16 copy of the near field: the
-  // preconditioner only has to approximate the near-field solve
+  // complex64 inner solves run on a low-precision copy of the near field:
+  // the preconditioner only has to approximate the near-field solve.  Fewer
+  // than four right-hand sides use the fp8 copy, one SpMV per vector (the
+  // bf16 one with SPK_NEAR_F8=0); four or more share the bf16 matrix-core
+  // product.
   const uint32_t *bf = nullptr, *f8 = nullptr;
   if constexpr (std::is_same<T, zf>::value) {
-    const int e = ensure_near_bf(h);
-    if (e) return e;
-    bf = h->d_blk_bf.p;
     const char *ev = getenv("SPK_NEAR_F8");
     if (R < 4 && !(ev && ev[0] == '0')) {
       const int e8 = ensure_near_f8(h);
       if (e8) return e8;
       f8 = h->d_blk_f8.p;
+    } else {
+      const int e = ensure_near_bf(h);
+      if (e) return e;
+      bf = h->d_blk_bf.p;
     }
   }
   const double t_start = now_ms();

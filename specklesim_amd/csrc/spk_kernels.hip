@@ -11,6 +11,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include "spk_device.h"
+#include "spk_fp8.h"
 
 namespace spk {
 
@@ -1242,23 +1243,8 @@ void launch_near_spmv_bf(int nwg, int N, const int *row_box, const int *row_r0,
 // Near field in fp8 (OCP E4M3FN, the single-vector preconditioner's copy):
 // a 2x2 block is 8 bytes (Z0r, Z0i, Z1r, Z1i, Z2r, Z2i, Z3r, Z3i); every
 // scalar row and component has its own scale (the row's largest |re|, |im|
-// of that component maps to 448), scale[p] = (s0, s1, s2, s3) for position p.
-__device__ inline uint32_t f32_to_e4m3(float x) {
-  const uint32_t sg = x < 0.f ? 0x80u : 0u;
-  const float a = fabsf(x);
-  if (!(a < 448.f)) return sg | 0x7Eu;  // saturate (finite inputs)
-  if (a < 0.015625f) return sg | (uint32_t)rintf(a * 512.f);  // subnormal (8 -> 0x08, the least normal)
-  int E;
-  const float f = frexpf(a, &E);  // a = f 2^E, f in [0.5, 1)
-  int e = E - 1;
-  uint32_t mnt = (uint32_t)rintf((f * 2.f - 1.f) * 8.f);
-  if (mnt == 8u) {
-    mnt = 0u;
-    ++e;
-  }
-  if (e + 7 > 15 || (e + 7 == 15 && mnt == 7u)) return sg | 0x7Eu;
-  return sg | ((uint32_t)(e + 7) << 3) | mnt;
-}
+// of that component maps to 448), scale[p] = (s0, s1, s2, s3) for position p;
+// f32_to_e4m3 (spk_fp8.h) encodes.
 // one workgroup per row position: the row's component maxima, then its blocks encoded
 __global__ void __launch_bounds__(256) k_near_to_f8(int N, const int *leaf_box_of_pos,
                                                     const int *box_first, const int *e2_off,

@@ -70,6 +70,22 @@ def test_near_field_fgmres_single(solvers, reference, precision, monkeypatch):
     assert len(hist) == inf["iters"] and hist[-1] <= 1e-6 and np.all(np.diff(hist) <= 1e-12)
 
 
+def test_near_field_fgmres_fp8_against_bf16(solvers, reference, monkeypatch):
+    """The fp8 copy of the near field (one right-hand side) preconditions
+    about as well as the bf16 one: at most two more outer steps, and both
+    solutions agree with plain GMRES's."""
+    s = solvers["mixed"]
+    _, b, x0, _ = reference[2]
+    x8, i8 = s.gmres_run(b, tolerance=1e-6, max_iterations=650, use_initial_guess=False,
+                         near_field=True)
+    monkeypatch.setenv("SPK_NEAR_F8", "0")
+    xb, ib = s.gmres_run(b, tolerance=1e-6, max_iterations=650, use_initial_guess=False,
+                         near_field=True)
+    assert i8["info"] == 0 and ib["info"] == 0
+    assert i8["iters"] <= ib["iters"] + 2, (i8["iters"], ib["iters"])
+    assert rel(x8, x0) < 3e-5 and rel(xb, x0) < 3e-5
+
+
 @pytest.mark.parametrize("inner", [1, 8, 100])
 def test_near_field_fgmres_inner_lengths(solvers, reference, inner):
     """Any inner length converges (100 is clamped to 64); one inner step is
