@@ -468,7 +468,7 @@ static int batch_ensure(spk_handle *h, int R) {
 // far_on = false: the near-field part only.
 template <class T>
 int matvec_batch_t(spk_handle *h, int R, const T *const *xs, T *const *ys, bool far_on = true,
-                   const uint32_t *blocks_bf = nullptr) {
+                   const uint32_t *blocks_bf = nullptr, const float *f8_scale = nullptr) {
   T *tag = nullptr;
   if (R < 1 || R > kBatchMax) return fail(SPK_ERR_ARG, "batch size must be 1..16");
   int rc = batch_ensure<T>(h, R);
@@ -480,9 +480,19 @@ int matvec_batch_t(spk_handle *h, int R, const T *const *xs, T *const *ys, bool 
   LevelDev &LL = h->lev[lmax];
   launch_batch_interleave(N, R, h->d_perm.p, xs, BB.XI.p, BB.XP.p, st);
   if (!far_on) {
+    if constexpr (std::is_same<T, zf>::value) {
+      if (f8_scale && R > 8) {
+        launch_batch_near_f8(h->spmv_wgs, N, R, h->d_row_box.p, h->d_row_r0.p, h->d_box_first.p,
+                             h->d_box_count.p, h->d_e2_off.p, h->d_e2_list.p, h->d_near_off.p,
+                             h->d_perm.p, blocks_bf, f8_scale, (const zf *)BB.XI.p, ys, st);
+        HIPCHK(hipGetLastError());
+        return SPK_OK;
+      }
+    }
     launch_batch_near(h->spmv_wgs, N, R, h->d_row_box.p, h->d_row_r0.p, h->d_box_first.p,
                       h->d_box_count.p, h->d_e2_off.p, h->d_e2_list.p, h->d_near_off.p, h->d_perm.p,
-                      pBlk(h, tag), (const T *)BB.XI.p, (const T *)nullptr, ys, st, blocks_bf);
+                      pBlk(h, tag), (const T *)BB.XI.p, (const T *)nullptr, ys, st,
+                      f8_scale ? nullptr : blocks_bf);
     HIPCHK(hipGetLastError());
     return SPK_OK;
   }
@@ -1509,14 +1519,14 @@ static int fgmres_near(spk_handle *h, int R, const zd *const *b, zd *const *x,
   if (std::is_same<T, zf>::value && !h->mixed_ready)
     return fail(SPK_ERR_STATE, "handle was not created with SPK_MIXED");
   // complex64 inner solves run on a low-precision copy of the near field:
-  // the preconditioner only has to approximate the near-field solve.  Fewer
-  // than four right-hand sides use the fp8 copy, one SpMV per vector (the
-  // bf16 one with SPK_NEAR_F8=0); four or more share the bf16 matrix-core
-  // product.
+  // the preconditioner only has to approximate the near-field solve.  The
+  // fp8 copy by default (one SpMV per vector below 4 active right-hand
+  // sides, the matrix-core product from 9, complex64 blocks between), the
+  // bf16 copy with SPK_NEAR_F8=0.
   const uint32_t *bf = nullptr, *f8 = nullptr;
   if constexpr (std::is_same<T, zf>::value) {
     const char *ev = getenv("SPK_NEAR_F8");
-    if (R < 4 && !(ev && ev[0] == '0')) {
+    if (!(ev && ev[0] == '0')) {
       const int e8 = ensure_near_f8(h);
       if (e8) return e8;
       f8 = h->d_blk_f8.p;
@@ -1708,7 +1718,8 @@ static int fgmres_near(spk_handle *h, int R, const zd *const *b, zd *const *x,
           rc = matvec_t<T>(h, ui, wo, true, false);
         }
       } else {
-        rc = matvec_batch_t<T>(h, P, (const T *const *)(tin.p + (size_t)(1 + j) * B), tW, false, bf);
+        rc = matvec_batch_t<T>(h, P, (const T *const *)(tin.p + (size_t)(1 + j) * B), tW, false,
+                               f8 ? f8 : bf, f8 ? h->d_f8_scale.p : nullptr);
       }
       if (rc) return rc;
       launch_arnoldi_cgs<T>(n, P, j, kin, U.p, ldu, Wi.p, h1.p, (int)ldhi, Hi.p, apart.p, h->st);

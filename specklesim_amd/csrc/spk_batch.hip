@@ -1187,6 +1187,172 @@ __global__ void __launch_bounds__(256) kb_near_mxb(int N, int R, const int *row_
   }
 }
 
+// The same product on the fp8 copy of the blocks (E4M3FN, 8 bytes per 2x2
+// block, per-row component scales).  The scales differ per component, so each
+// component gets its own accumulators: one product covers 8 columns of one
+// component, k = 2 * column + (re, im); lane g * 16 + m holds A = (Zc_r, Zc_i)
+// of row m at columns 2g and 2g + 1 of the 8 (decoded by v_cvt_pk_f32_fp8;
+// an fp8 value has 3 mantissa bits, so its fp32 form is exactly bf16 and the
+// high halves are the operand) and B = the staged x words of those columns.
+// The row's scales multiply the component sums before the waves' sums are
+// added.  A workgroup covers RT row tiles of 16 rows (one x staging for all).
+template <bool HI>
+__device__ inline mxs4 fp8_comp_pair(uint32_t w0, uint32_t w1) {
+  const auto a = __builtin_amdgcn_cvt_pk_f32_fp8((int)w0, HI);
+  const auto b = __builtin_amdgcn_cvt_pk_f32_fp8((int)w1, HI);
+  return as_s4((__float_as_uint(a[0]) >> 16) | (__float_as_uint(a[1]) & 0xFFFF0000u),
+               (__float_as_uint(b[0]) >> 16) | (__float_as_uint(b[1]) & 0xFFFF0000u));
+}
+template <int RT>
+__global__ void __launch_bounds__(256) kb_near_mxf8(int N, int R, const int *row_box,
+                                                    const int *row_r0, const int *box_first,
+                                                    const int *box_count, const int *e2_off,
+                                                    const int *e2_list, const int64_t *near_off,
+                                                    const int *perm, const uint2 *blocks,
+                                                    const float4 *scale, const zf *XI,
+                                                    zf *const *ys) {
+  constexpr int PER = kMxCols * 16 / 256;
+  __shared__ uint4 xsl[kMxCols * 16];
+  const int b = row_box[blockIdx.x];
+  const int pf = box_first[b], nrow = box_count[b];
+  const int c0 = e2_off[b], cols = e2_off[b + 1] - c0;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int m = lane & 15, g = lane >> 4;
+  const int rw = row_r0[blockIdx.x];
+  const uint2 *Zr[RT];
+#pragma unroll
+  for (int t = 0; t < RT; ++t) Zr[t] = blocks + near_off[b] + (int64_t)min(rw + 16 * t + m, nrow - 1) * cols;
+  // acc[t][c][0] real, acc[t][c][1] imaginary part of component c's sum, row tile t
+  mxf4 acc[RT][4][2];
+#pragma unroll
+  for (int t = 0; t < RT; ++t)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[t][c][0] = acc[t][c][1] = mxf4{0.f, 0.f, 0.f, 0.f};
+  // a lane's blocks of one 16-column step: columns cg + 8 jj + 2g + u
+  uint2 cur[RT][4];
+  auto load = [&](int cg, uint2 (*z)[4]) {
+#pragma unroll
+    for (int t = 0; t < RT; ++t)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) z[t][j] = Zr[t][min(cg + 8 * (j >> 1) + 2 * g + (j & 1), cols - 1)];
+  };
+  if (cols > 0) load(16 * wv, cur);
+  for (int cc = 0; cc < cols; cc += kMxCols) {
+    const int nc = min(kMxCols, cols - cc);
+    __syncthreads();
+    {
+      const int n = threadIdx.x & 15;
+      int e[PER];
+#pragma unroll
+      for (int k = 0; k < PER; ++k)
+        e[k] = e2_list[c0 + cc + min((int)(threadIdx.x + 256 * k) >> 4, nc - 1)];
+#pragma unroll
+      for (int k = 0; k < PER; ++k) {
+        const zf xe = XI[((size_t)e[k] * 2 + 0) * 16 + n], xh = XI[((size_t)e[k] * 2 + 1) * 16 + n];
+        const bool ok = ((int)(threadIdx.x + 256 * k) >> 4) < nc;
+        const uint4 v = {bf16_pair(xe.x, -xe.y), bf16_pair(xh.x, -xh.y), bf16_pair(xe.y, xe.x),
+                         bf16_pair(xh.y, xh.x)};
+        xsl[threadIdx.x + 256 * k] = ok ? v : uint4{0u, 0u, 0u, 0u};
+      }
+    }
+    __syncthreads();
+    for (int off = 16 * wv; off < nc; off += 64) {
+      uint2 nx[RT][4];
+      load(cc + off + 64, nx);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int jj = 0; jj < 2; ++jj) {
+        const int col = off + 8 * jj + 2 * g;
+        const uint4 x0 = xsl[col * 16 + m], x1 = xsl[(col + 1) * 16 + m];
+        const mxs4 ber = as_s4(x0.x, x1.x), bhr = as_s4(x0.y, x1.y);
+        const mxs4 bei = as_s4(x0.z, x1.z), bhi = as_s4(x0.w, x1.w);
+#pragma unroll
+        for (int t = 0; t < RT; ++t) {
+          const uint2 z0 = cur[t][2 * jj], z1 = cur[t][2 * jj + 1];
+          const mxs4 a0 = fp8_comp_pair<false>(z0.x, z1.x), a1 = fp8_comp_pair<true>(z0.x, z1.x);
+          const mxs4 a2 = fp8_comp_pair<false>(z0.y, z1.y), a3 = fp8_comp_pair<true>(z0.y, z1.y);
+          mxf4(&A)[4][2] = acc[t];
+          A[0][0] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a0, ber, A[0][0], 0, 0, 0);
+          A[0][1] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a0, bei, A[0][1], 0, 0, 0);
+          A[1][0] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1, bhr, A[1][0], 0, 0, 0);
+          A[1][1] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1, bhi, A[1][1], 0, 0, 0);
+          A[2][0] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2, ber, A[2][0], 0, 0, 0);
+          A[2][1] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2, bei, A[2][1], 0, 0, 0);
+          A[3][0] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a3, bhr, A[3][0], 0, 0, 0);
+          A[3][1] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a3, bhi, A[3][1], 0, 0, 0);
+        }
+      }
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int t = 0; t < RT; ++t)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) cur[t][j] = nx[t][j];
+    }
+  }
+  // element i of this lane in tile t is output row rw + 16t + 4g + i: scale,
+  // then sum the waves (tile by tile through LDS)
+  mxf4 *red = reinterpret_cast<mxf4 *>(xsl);
+#pragma unroll
+  for (int t = 0; t < RT; ++t) {
+    mxf4 er, ei, hr, hi;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int r = min(rw + 16 * t + 4 * g + i, nrow - 1);
+      const float4 sc = scale[pf + r];
+      er[i] = sc.x * acc[t][0][0][i] + sc.y * acc[t][1][0][i];
+      ei[i] = sc.x * acc[t][0][1][i] + sc.y * acc[t][1][1][i];
+      hr[i] = sc.z * acc[t][2][0][i] + sc.w * acc[t][3][0][i];
+      hi[i] = sc.z * acc[t][2][1][i] + sc.w * acc[t][3][1][i];
+    }
+    __syncthreads();
+    if (wv > 0) {
+      mxf4 *o = red + ((size_t)(wv - 1) * 64 + lane) * 4;
+      o[0] = er;
+      o[1] = ei;
+      o[2] = hr;
+      o[3] = hi;
+    }
+    __syncthreads();
+    if (wv == 0) {
+#pragma unroll
+      for (int p = 0; p < 3; ++p) {
+        const mxf4 *o = red + ((size_t)p * 64 + lane) * 4;
+        er += o[0];
+        ei += o[1];
+        hr += o[2];
+        hi += o[3];
+      }
+      const int n = m;
+      if (n < R) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int r = rw + 16 * t + 4 * g + i;
+          if (r < nrow) {
+            const int mm = perm[pf + r];
+            ys[n][mm] = zf{er[i], ei[i]};
+            ys[n][mm + N] = zf{hr[i], hi[i]};
+          }
+        }
+      }
+    }
+  }
+}
+
+// the preconditioner's product for 9..16 right-hand sides on the fp8 copy
+// (blocks, scale as k_near_spmv_f8 reads them); workgroups = row tiles of
+// 16 rows.  Two tiles per workgroup (x staged once for 32 rows, 156 VGPRs)
+// measured the same, 105 us at 16 vectors on the 10 x 10 lambda patch.
+void launch_batch_near_f8(int nwg, int N, int R, const int *row_box, const int *row_r0,
+                          const int *box_first, const int *box_count, const int *e2_off,
+                          const int *e2_list, const int64_t *near_off, const int *perm,
+                          const uint32_t *blocks, const float *scale, const zf *XI,
+                          zf *const *ys, hipStream_t st) {
+  if (nwg <= 0) return;
+  hipLaunchKernelGGL(kb_near_mxf8<1>, dim3(nwg), dim3(256), 0, st, N, R, row_box, row_r0,
+                     box_first, box_count, e2_off, e2_list, near_off, perm, (const uint2 *)blocks,
+                     (const float4 *)scale, XI, ys);
+}
+
 // padded batch width: R rounded up to a power of two
 inline int batch_pad(int R) { return R <= 1 ? 1 : R <= 2 ? 2 : R <= 4 ? 4 : R <= 8 ? 8 : 16; }
 
