@@ -85,7 +85,11 @@ typedef struct spk_gmres_params {
                                     FP64 only.  SPK_PRECOND_NEAR_FIELD (no counterpart in the
                                     reference): flexible GMRES on the FP64 operator, right-
                                     preconditioned by inner_iterations GMRES steps on the
-                                    near-field operator (complex64 on an SPK_MIXED handle) */
+                                    near-field operator (complex64 vectors and fp8 blocks on
+                                    an SPK_MIXED handle, where the outer products also switch
+                                    to the complex64 operator once the residual estimate is
+                                    at or below 200 x tolerance; the true residual that ends
+                                    the solve is FP64 in every case) */
   int inner_iterations;          /* SPK_PRECOND_NEAR_FIELD: inner steps per outer step (0: 30) */
 } spk_gmres_params;
 #define SPK_PRECOND_NONE 0

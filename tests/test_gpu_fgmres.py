@@ -86,6 +86,26 @@ def test_near_field_fgmres_fp8_against_bf16(solvers, reference, monkeypatch):
     assert rel(x8, x0) < 3e-5 and rel(xb, x0) < 3e-5
 
 
+def test_near_field_fgmres_relaxed_products(solvers, reference, monkeypatch):
+    """On a mixed handle the outer products switch to the complex64 operator
+    once the residual estimate is at or below 2e-4 (tolerance 1e-6): the
+    solve still meets the tolerance in the FP64 true residual, in as many
+    outer steps as with every product in FP64 (SPK_RELAX=0), and both agree
+    with plain GMRES."""
+    s = solvers["mixed"]
+    _, b, x0, _ = reference[1]
+    xr, ir = s.gmres_run(b, tolerance=1e-6, max_iterations=650, use_initial_guess=False,
+                         near_field=True)
+    monkeypatch.setenv("SPK_RELAX", "0")
+    xf, iff = s.gmres_run(b, tolerance=1e-6, max_iterations=650, use_initial_guess=False,
+                          near_field=True)
+    assert ir["info"] == 0 and iff["info"] == 0
+    res = np.linalg.norm(b - solvers["fp64"].calculate_vector_b_tri(xr)) / np.linalg.norm(b)
+    assert res <= 1e-6 and abs(res - ir["be"]) < 1e-9, (res, ir["be"])
+    assert ir["iters"] <= iff["iters"] + 1, (ir["iters"], iff["iters"])
+    assert rel(xr, x0) < 3e-5 and rel(xf, x0) < 3e-5
+
+
 @pytest.mark.parametrize("inner", [1, 8, 100])
 def test_near_field_fgmres_inner_lengths(solvers, reference, inner):
     """Any inner length converges (100 is clamped to 64); one inner step is
