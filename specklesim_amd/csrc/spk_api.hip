@@ -46,21 +46,33 @@ double now_ms() {
       .count();
 }
 
+// device buffer of n elements; alloc keeps the allocation when the request
+// fits its capacity (a handle set up again for a mesh of the same size
+// allocates nothing), so the contents after alloc are undefined either way
 template <class T>
 struct DBuf {
   T *p = nullptr;
-  size_t n = 0;
+  size_t n = 0, cap = 0;
   ~DBuf() { reset(); }
   void reset() {
     if (p) (void)hipFree(p);
     p = nullptr;
-    n = 0;
+    n = cap = 0;
   }
   hipError_t alloc(size_t count) {
+    if (p && count <= cap) {
+      n = count;
+      return hipSuccess;
+    }
     reset();
-    n = count;
     if (count == 0) return hipSuccess;
-    return hipMalloc(&p, count * sizeof(T));
+    const hipError_t e = hipMalloc(&p, count * sizeof(T));
+    if (e != hipSuccess) {
+      p = nullptr;
+      return e;
+    }
+    n = cap = count;
+    return hipSuccess;
   }
   hipError_t upload(const std::vector<T> &v) {
     hipError_t e = alloc(v.size());
@@ -121,6 +133,7 @@ struct spk_handle {
   // fp8 (E4M3FN) copy with per-row, per-component scales (single-vector preconditioner)
   DBuf<uint32_t> d_blk_f8;
   DBuf<float> d_f8_scale;
+  bool near_bf_ok = false, near_f8_ok = false;  // the copies match the current blocks
   // matrix-core disaggregation tiles: (box, first position) of up to 128 edges
   DBuf<int> d_dx_box, d_dx_p0;
   int dx_tiles = 0;
@@ -623,8 +636,7 @@ int spk_setup_tri(spk_handle *h, int np, const double *pp, int nt, const int *tt
   h->mixed_ready = false;
   h->diag_ready = false;
   // the low-precision near-field copies belong to the previous operator
-  h->d_blk_bf.reset();
-  h->d_blk_f8.reset();
+  h->near_bf_ok = h->near_f8_ok = false;
   h->stats = spk_stats{};
   const double t0 = now_ms();
   std::string err;
@@ -1480,21 +1492,24 @@ extern "C" {
 extern "C++" {
 // the bf16 copy of the complex64 near blocks (built once per setup)
 static int ensure_near_bf(spk_handle *h) {
-  if (h->d_blk_bf.n != h->f_blocks.n) {
+  if (!h->near_bf_ok) {
     HIPCHK(h->d_blk_bf.alloc(h->f_blocks.n));
     launch_to_bf((int64_t)h->f_blocks.n, h->f_blocks.p, h->d_blk_bf.p, h->st);
+    HIPCHK(hipGetLastError());
+    h->near_bf_ok = true;
   }
   return SPK_OK;
 }
 
 // the fp8 copy of the complex64 near blocks (built once per setup)
 static int ensure_near_f8(spk_handle *h) {
-  if (h->d_blk_f8.n != h->f_blocks.n / 2) {
+  if (!h->near_f8_ok) {
     HIPCHK(h->d_blk_f8.alloc(h->f_blocks.n / 2));
     HIPCHK(h->d_f8_scale.alloc(4 * (size_t)h->N));
     launch_near_to_f8(h->N, h->d_leafbox_of_pos.p, h->d_box_first.p, h->d_e2_off.p,
                       h->d_near_off.p, h->f_blocks.p, h->d_blk_f8.p, h->d_f8_scale.p, h->st);
     HIPCHK(hipGetLastError());
+    h->near_f8_ok = true;
   }
   return SPK_OK;
 }
