@@ -1606,7 +1606,12 @@ __global__ void __launch_bounds__(256) k_arn_fin(int64_t n, int64_t rows, int j,
     u[i] = T{(F)(v.x * inv), (F)(v.y * inv)};
   }
 }
-int arnoldi_chunks(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(256, (n + 511) / 512)); }
+// row chunks of the update and normalisation (at most 256): 256 rows for
+// fewer than 4 slots (more workgroups for a short grid), 512 otherwise
+static int arnoldi_chunks(int64_t n, int P) {
+  const int64_t rows = P < 4 ? 256 : 512;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(256, (n + rows - 1) / rows));
+}
 static int arnoldi_dot_chunks(int64_t n) { return (int)((n + kArnRows - 1) / kArnRows); }
 size_t arnoldi_partial_elems(int64_t n, int P, int kin) {
   return (size_t)P * (kin + 1) * (size_t)std::max(256, arnoldi_dot_chunks(n)) + (size_t)P * 256;
@@ -1615,7 +1620,7 @@ template <class T>
 void launch_arnoldi_cgs(int64_t n, int P, int j, int kin, T *U, size_t ldu, T *W, zd *hb, int ldh,
                         zd *H, zd *partial, hipStream_t st) {
   if (P <= 0) return;
-  const int G = arnoldi_chunks(n), Gd = arnoldi_dot_chunks(n);
+  const int G = arnoldi_chunks(n, P), Gd = arnoldi_dot_chunks(n);
   const int64_t rows = (n + G - 1) / G;
   const int k = j + 1;
   double *pn = reinterpret_cast<double *>(partial + (size_t)P * (kin + 1) * std::max(256, Gd));
