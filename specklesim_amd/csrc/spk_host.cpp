@@ -140,31 +140,53 @@ int build_rwg(int np, const double *pp, int nt, const int *tt, int normal_conven
     }
   corner->clear();
   element->clear();
-  std::vector<int> cand;
-  for (int i = 0; i < nt; ++i) {
-    cand.clear();
-    for (int k = 0; k < 3; ++k) {
-      int v = tt[3 * i + k] - 1;
-      for (int a = deg[v]; a < deg[v + 1]; ++a)
-        if (adj[a] > i) cand.push_back(adj[a]);
-    }
-    std::sort(cand.begin(), cand.end());
-    cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
-    for (int j : cand) {
-      int com[3], ne = 0;
-      for (int a = 0; a < 3; ++a)
-        for (int b = 0; b < 3; ++b)
-          if (tt[3 * i + a] == tt[3 * j + b] && ne < 3) com[ne++] = tt[3 * i + a];
-      if (ne == 2) {
-        corner->push_back(com[0]);
-        corner->push_back(com[1]);
-        element->push_back(i + 1);
-        element->push_back(j + 1);
-      } else if (ne == 3) {
-        *err = "duplicate triangle in mesh";
-        return SPK_ERR_MESH;
+  // pairs (i < j) in order of i, then j: triangle ranges scanned on threads,
+  // their lists concatenated in range order
+  const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  const int nch = std::max(1, std::min(nth, nt / 2048));
+  std::vector<std::vector<int>> pcor(nch), pelm(nch);
+  std::vector<char> dup(nch, 0);
+  auto scan = [&](int c) {
+    const int i0 = (int)((int64_t)nt * c / nch), i1 = (int)((int64_t)nt * (c + 1) / nch);
+    std::vector<int> cand;
+    for (int i = i0; i < i1 && !dup[c]; ++i) {
+      cand.clear();
+      for (int k = 0; k < 3; ++k) {
+        int v = tt[3 * i + k] - 1;
+        for (int a = deg[v]; a < deg[v + 1]; ++a)
+          if (adj[a] > i) cand.push_back(adj[a]);
+      }
+      std::sort(cand.begin(), cand.end());
+      cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+      for (int j : cand) {
+        int com[3], ne = 0;
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b)
+            if (tt[3 * i + a] == tt[3 * j + b] && ne < 3) com[ne++] = tt[3 * i + a];
+        if (ne == 2) {
+          pcor[c].push_back(com[0]);
+          pcor[c].push_back(com[1]);
+          pelm[c].push_back(i + 1);
+          pelm[c].push_back(j + 1);
+        } else if (ne == 3) {
+          dup[c] = 1;
+        }
       }
     }
+  };
+  {
+    std::vector<std::thread> th;
+    for (int c = 1; c < nch; ++c) th.emplace_back(scan, c);
+    scan(0);
+    for (auto &t : th) t.join();
+  }
+  for (int c = 0; c < nch; ++c) {
+    if (dup[c]) {
+      *err = "duplicate triangle in mesh";
+      return SPK_ERR_MESH;
+    }
+    corner->insert(corner->end(), pcor[c].begin(), pcor[c].end());
+    element->insert(element->end(), pelm[c].begin(), pelm[c].end());
   }
   const int N = (int)element->size() / 2;
   std::vector<double> cent(3 * (size_t)nt);
@@ -176,7 +198,8 @@ int build_rwg(int np, const double *pp, int nt, const int *tt, int normal_conven
   fn->assign(2 * (size_t)N, FnRec{});
   er->assign(2 * (size_t)N, ElemRec{});
   rref->assign(N, 0.0);
-  for (int m = 0; m < N; ++m) {
+  auto records = [&](int m0, int m1) {
+  for (int m = m0; m < m1; ++m) {
     for (int ot = 1; ot <= 2; ++ot) {
       int e = (*element)[2 * m + ot - 1] - 1;
       const int *cr = &tt[3 * e];
@@ -241,6 +264,15 @@ int build_rwg(int np, const double *pp, int nt, const int *tt, int normal_conven
     double d[3] = {cent[3 * p] - cent[3 * q], cent[3 * p + 1] - cent[3 * q + 1],
                    cent[3 * p + 2] - cent[3 * q + 2]};
     (*rref)[m] = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  }
+  };
+  {
+    const int nr = std::max(1, std::min(nth, N / 2048));
+    std::vector<std::thread> th;
+    for (int c = 1; c < nr; ++c)
+      th.emplace_back(records, (int)((int64_t)N * c / nr), (int)((int64_t)N * (c + 1) / nr));
+    records(0, (int)((int64_t)N / nr));
+    for (auto &t : th) t.join();
   }
   return SPK_OK;
 }
