@@ -88,6 +88,7 @@ struct BatchBufs {
   std::vector<DBuf<T>> B, D;  // per level, [box][it][s][r]
   DBuf<T> W, YF, YP, XI, XP;  // folded leaf coefficients; far part [r][2N] and its partial
                               // slices; interleaved x
+  DBuf<uint32_t> XB;          // XI in bf16 (the fp8 preconditioner product)
   DBuf<const T *> xs;
   DBuf<T *> ys;
 };
@@ -470,6 +471,7 @@ static int batch_ensure(spk_handle *h, int R) {
   HIPCHK(BB.YF.alloc((size_t)R * 2 * h->N));
   HIPCHK(BB.YP.alloc((size_t)kBatchSplit * R * 2 * h->N));
   HIPCHK(BB.XI.alloc((size_t)kBatchMax * 2 * h->N));
+  HIPCHK(BB.XB.alloc((size_t)kBatchMax * 2 * h->N));
   HIPCHK(BB.XP.alloc((size_t)kBatchMax * 2 * h->N));
   HIPCHK(BB.xs.alloc(kBatchMax));
   HIPCHK(BB.ys.alloc(kBatchMax));
@@ -497,7 +499,8 @@ int matvec_batch_t(spk_handle *h, int R, const T *const *xs, T *const *ys, bool 
       if (f8_scale && R > 8) {
         launch_batch_near_f8(h->spmv_wgs, N, R, h->d_row_box.p, h->d_row_r0.p, h->d_box_first.p,
                              h->d_box_count.p, h->d_e2_off.p, h->d_e2_list.p, h->d_near_off.p,
-                             h->d_perm.p, blocks_bf, f8_scale, (const zf *)BB.XI.p, ys, st);
+                             h->d_perm.p, blocks_bf, f8_scale, (const zf *)BB.XI.p, BB.XB.p, ys,
+                             st);
         HIPCHK(hipGetLastError());
         return SPK_OK;
       }

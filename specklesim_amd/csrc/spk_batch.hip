@@ -1209,7 +1209,7 @@ __global__ void __launch_bounds__(256) kb_near_mxf8(int N, int R, const int *row
                                                     const int *box_count, const int *e2_off,
                                                     const int *e2_list, const int64_t *near_off,
                                                     const int *perm, const uint2 *blocks,
-                                                    const float4 *scale, const zf *XI,
+                                                    const float4 *scale, const uint32_t *XB,
                                                     zf *const *ys) {
   constexpr int PER = kMxCols * 16 / 256;
   __shared__ uint4 xsl[kMxCols * 16];
@@ -1248,10 +1248,12 @@ __global__ void __launch_bounds__(256) kb_near_mxf8(int N, int R, const int *row
         e[k] = e2_list[c0 + cc + min((int)(threadIdx.x + 256 * k) >> 4, nc - 1)];
 #pragma unroll
       for (int k = 0; k < PER; ++k) {
-        const zf xe = XI[((size_t)e[k] * 2 + 0) * 16 + n], xh = XI[((size_t)e[k] * 2 + 1) * 16 + n];
+        // x in bf16 (re low, im high), rounded once by kb_x_to_bf: negating a
+        // half is its sign bit, (im, re) is the halves swapped
+        const uint32_t xe = XB[((size_t)e[k] * 2 + 0) * 16 + n], xh = XB[((size_t)e[k] * 2 + 1) * 16 + n];
         const bool ok = ((int)(threadIdx.x + 256 * k) >> 4) < nc;
-        const uint4 v = {bf16_pair(xe.x, -xe.y), bf16_pair(xh.x, -xh.y), bf16_pair(xe.y, xe.x),
-                         bf16_pair(xh.y, xh.x)};
+        const uint4 v = {xe ^ 0x80000000u, xh ^ 0x80000000u, (xe >> 16) | (xe << 16),
+                         (xh >> 16) | (xh << 16)};
         xsl[threadIdx.x + 256 * k] = ok ? v : uint4{0u, 0u, 0u, 0u};
       }
     }
@@ -1342,15 +1344,23 @@ __global__ void __launch_bounds__(256) kb_near_mxf8(int N, int R, const int *row
 // (blocks, scale as k_near_spmv_f8 reads them); workgroups = row tiles of
 // 16 rows.  Two tiles per workgroup (x staged once for 32 rows, 156 VGPRs)
 // measured the same, 105 us at 16 vectors on the 10 x 10 lambda patch.
+// XB[i] = XI[i] in bf16 (re in the low half, im in the high half), rounded
+// to nearest even as bf16_pair rounds
+__global__ void __launch_bounds__(256) kb_x_to_bf(int64_t len, const zf *XI, uint32_t *XB) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < len) XB[i] = bf16_pair(XI[i].x, XI[i].y);
+}
 void launch_batch_near_f8(int nwg, int N, int R, const int *row_box, const int *row_r0,
                           const int *box_first, const int *box_count, const int *e2_off,
                           const int *e2_list, const int64_t *near_off, const int *perm,
                           const uint32_t *blocks, const float *scale, const zf *XI,
-                          zf *const *ys, hipStream_t st) {
+                          uint32_t *XB, zf *const *ys, hipStream_t st) {
   if (nwg <= 0) return;
+  const int64_t len = (int64_t)N * 2 * 16;
+  hipLaunchKernelGGL(kb_x_to_bf, dim3(bnblk(len, 256)), dim3(256), 0, st, len, XI, XB);
   hipLaunchKernelGGL(kb_near_mxf8<1>, dim3(nwg), dim3(256), 0, st, N, R, row_box, row_r0,
                      box_first, box_count, e2_off, e2_list, near_off, perm, (const uint2 *)blocks,
-                     (const float4 *)scale, XI, ys);
+                     (const float4 *)scale, (const uint32_t *)XB, ys);
 }
 
 // padded batch width: R rounded up to a power of two

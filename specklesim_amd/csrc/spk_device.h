@@ -255,13 +255,14 @@ void launch_batch_near(int nwg, int N, int R, const int *row_box, const int *row
                        const T *blocks, const T *XI, const T *yf, T *const *ys,
                        hipStream_t st, const uint32_t *blocks_bf = nullptr);
 // the near-field preconditioner's product for 9..16 right-hand sides (XI as
-// launch_batch_interleave writes it) on the fp8 copy of the blocks, on the
-// matrix cores; workgroups = the row tiles (box, first row) of kSpmvRowsPerWG rows
+// launch_batch_interleave writes it; XB: scratch of N * 32 words for x in
+// bf16) on the fp8 copy of the blocks, on the matrix cores; workgroups = the
+// row tiles (box, first row) of kSpmvRowsPerWG rows
 void launch_batch_near_f8(int nwg, int N, int R, const int *row_box, const int *row_r0,
                           const int *box_first, const int *box_count, const int *e2_off,
                           const int *e2_list, const int64_t *near_off, const int *perm,
                           const uint32_t *blocks, const float *scale, const zf *XI,
-                          zf *const *ys, hipStream_t st);
+                          uint32_t *XB, zf *const *ys, hipStream_t st);
 
 // one ICGS pass over P bases (coefficients into hs[r][0..ks[r]), |w_r|^2
 // into hs[r][ks[r]]); partial: batch_orth_partial_elems(n, P, maxk)
