@@ -66,20 +66,41 @@ def workload(args):
     return lam, pp, tt
 
 
-def one_solve(args, lam, pp, tt, precision, orth=None, near=False):
+class _Path:
+    """One solver handle and its device vectors, kept for every step of a
+    path: each step sets the handle up again from the host mesh (the whole
+    precalculation runs again; its device allocations are reused, as for
+    the reference's loop over surface realisations)."""
+
+    def __init__(self, args, lam, precision, R=1):
+        from specklesim_amd import DeviceBuffer, SieTriMlfmm
+
+        self.s = SieTriMlfmm(lam, args.eps_r2, formulation=args.formulation, normal_convention=2,
+                             tree_s_opt=250, truncation_number=19, precision=precision)
+        self.R = R
+        self.bs, self.xs = [], []
+        self._DeviceBuffer = DeviceBuffer
+
+    def vectors(self, N):
+        nb = 2 * N * 16
+        if not self.bs or self.bs[0].nbytes != nb:
+            self.bs = [self._DeviceBuffer(nb) for _ in range(self.R)]
+            self.xs = [self._DeviceBuffer(nb) for _ in range(self.R)]
+        return self.bs, self.xs
+
+
+def one_solve(args, path, pp, tt, orth=None, near=False):
     """Precalculation from the host mesh, then the right-hand side and GMRES
     on device buffers (b and x stay in HBM).  near: flexible GMRES
     preconditioned by inner GMRES on the near-field operator."""
-    from specklesim_amd import DeviceBuffer, SieTriMlfmm, plane_wave
+    from specklesim_amd import plane_wave
 
     t0 = time.perf_counter()
-    s = SieTriMlfmm(lam, args.eps_r2, formulation=args.formulation, normal_convention=2,
-                    tree_s_opt=250, truncation_number=19, precision=precision)
+    s = path.s
     s.precalculation_tri(pp, tt)
     t1 = time.perf_counter()
     k, e = plane_wave(0.0, 0.0, 2, "TM")
-    d_b = DeviceBuffer(2 * s.N * 16)
-    d_x = DeviceBuffer(2 * s.N * 16)
+    (d_b,), (d_x,) = path.vectors(s.N)
     s.get_vector_b_tri_device(k, e, d_b.ptr)
     info = s.gmres_run_device(d_b.ptr, d_x.ptr, tolerance=args.tol,
                               max_iterations=args.maxit, orthogonalization=orth or args.orth,
@@ -90,15 +111,23 @@ def one_solve(args, lam, pp, tt, precision, orth=None, near=False):
                         be=info["be"], info=info["info"], matvecs=info["matvecs"])
 
 
+def spread(runs, key="total_s"):
+    """min / median / max of one per-step time, in ms."""
+    v = np.array([r[key] for r in runs]) * 1e3
+    return {"min": round(float(v.min()), 3), "median": round(float(np.median(v)), 3),
+            "max": round(float(v.max()), 3)}
+
+
 def timed_path(args, lam, pp, tt, precision, orth=None, near=False):
     """W untimed solves, then exactly K solves between two device syncs."""
+    path = _Path(args, lam, precision)
     for _ in range(args.warmup):
-        one_solve(args, lam, pp, tt, precision, orth, near)
+        one_solve(args, path, pp, tt, orth, near)
     _sync()
     t0 = time.perf_counter()
     runs = []
     for _ in range(args.steps):
-        s, d_x, r = one_solve(args, lam, pp, tt, precision, orth, near)
+        s, d_x, r = one_solve(args, path, pp, tt, orth, near)
         runs.append(r)
     _sync()
     elapsed = time.perf_counter() - t0
@@ -112,21 +141,18 @@ def batch_angles(R):
     return [np.radians(60.0 * r / max(R - 1, 1)) for r in range(R)]
 
 
-def one_batch(args, lam, pp, tt, R):
+def one_batch(args, path, pp, tt):
     """Precalculation, R right-hand sides on the device and one batched
     near-field preconditioned solve of all of them (FP64 outer steps,
     complex64 inner near-field steps)."""
-    from specklesim_amd import DeviceBuffer, SieTriMlfmm, plane_wave
+    from specklesim_amd import plane_wave
 
     t0 = time.perf_counter()
-    s = SieTriMlfmm(lam, args.eps_r2, formulation=args.formulation, normal_convention=2,
-                    tree_s_opt=250, truncation_number=19, precision="mixed")
+    s = path.s
     s.precalculation_tri(pp, tt)
     t1 = time.perf_counter()
-    nb = 2 * s.N * 16
-    bs = [DeviceBuffer(nb) for _ in range(R)]
-    xs = [DeviceBuffer(nb) for _ in range(R)]
-    for b, th in zip(bs, batch_angles(R)):
+    bs, xs = path.vectors(s.N)
+    for b, th in zip(bs, batch_angles(path.R)):
         k, e = plane_wave(th, 0.0, 2, "TM")
         s.get_vector_b_tri_device(k, e, b.ptr)
     infos = s.gmres_run_batch_device([b.ptr for b in bs], [x.ptr for x in xs],
@@ -134,17 +160,18 @@ def one_batch(args, lam, pp, tt, R):
                                      use_initial_guess=False, near_field=True,
                                      inner_iterations=args.inner)
     t2 = time.perf_counter()
-    return s, xs, dict(setup_s=t1 - t0, gmres_s=t2 - t1, infos=infos)
+    return s, xs, dict(setup_s=t1 - t0, gmres_s=t2 - t1, total_s=t2 - t0, infos=infos)
 
 
 def timed_batch(args, lam, pp, tt, R):
+    path = _Path(args, lam, "mixed", R)
     for _ in range(args.warmup):
-        one_batch(args, lam, pp, tt, R)
+        one_batch(args, path, pp, tt)
     _sync()
     t0 = time.perf_counter()
     runs = []
     for _ in range(args.steps):
-        s, xs, r = one_batch(args, lam, pp, tt, R)
+        s, xs, r = one_batch(args, path, pp, tt)
         runs.append(r)
     _sync()
     elapsed = time.perf_counter() - t0
@@ -301,7 +328,9 @@ def main():
                       "iterations": v["runs"][-1]["iters"], "matvecs": v["runs"][-1]["matvecs"],
                       "backward_error": v["runs"][-1]["be"], "info": v["runs"][-1]["info"],
                       "setup_ms": v["runs"][-1]["setup_s"] * 1e3,
-                      "gmres_ms": v["runs"][-1]["gmres_s"] * 1e3}
+                      "gmres_ms": v["runs"][-1]["gmres_s"] * 1e3,
+                      "step_ms": spread(v["runs"]), "setup_step_ms": spread(v["runs"], "setup_s"),
+                      "gmres_step_ms": spread(v["runs"], "gmres_s")}
                   for k, v in paths.items()},
         "farfield_rel_err_mixed_vs_fp64": ff_err,
         "farfield_rel_err_vs_fp64": ff,
@@ -354,6 +383,8 @@ def main():
             "right_hand_sides": args.batch,
             "setup_ms": batch["runs"][-1]["setup_s"] * 1e3,
             "gmres_ms": batch["runs"][-1]["gmres_s"] * 1e3,
+            "step_ms": spread(batch["runs"]), "setup_step_ms": spread(batch["runs"], "setup_s"),
+            "gmres_step_ms": spread(batch["runs"], "gmres_s"),
             "iterations": [min(i["iters"] for i in infos), max(i["iters"] for i in infos)],
             "backward_error_max": max(i["be"] for i in infos),
             "all_converged": all(i["info"] == 0 for i in infos),

@@ -179,6 +179,20 @@ int spk_quad_topology_probe(double Lx, double Ly, int nx, int ny, const double *
 int spk_gmres_batch_device(spk_handle *h, int R, const void *const *d_b, void *const *d_x,
                            const spk_gmres_params *params, spk_gmres_info *info);
 
+/* No counterpart in the reference (a probe of the near-field preconditioner
+ * of SPK_PRECOND_NEAR_FIELD): the near-field part of the operator of an
+ * SPK_MIXED handle on R <= 16 complex64 device vectors (2N interleaved float
+ * pairs each), with the blocks in complex64 (SPK_NEAR_C64), in the bf16 copy
+ * (SPK_NEAR_BF16) or in the fp8 E4M3FN copy with per-row component scales
+ * (SPK_NEAR_FP8).  batched = 0: one single-vector launch per vector;
+ * batched = 1: the batched launch the inner solves of a batch use (for fp8,
+ * the matrix-core product from R = 9, complex64 blocks below). */
+#define SPK_NEAR_C64 0
+#define SPK_NEAR_BF16 1
+#define SPK_NEAR_FP8 2
+int spk_near_product_lowp(spk_handle *h, int format, int batched, int R, const void *const *d_x,
+                          void *const *d_y);
+
 /* Device memory for the *_device entry points (no counterpart in the
  * reference).  A caller that has its own HIP allocations may pass those
  * instead; they must come from the same HIP runtime as this library. */

@@ -28,7 +28,7 @@ EXPORTS = [
     "spk_gmres", "spk_gmres_device", "spk_gmres_history", "spk_get_rwg", "spk_get_tree",
     "spk_get_i4", "spk_get_near_csr", "spk_near_blocks", "spk_get_patterns",
     "spk_profile_matvec", "spk_profile_matvec_c64", "spk_profile_near_bf", "spk_profile_near_f8",
-    "spk_topology_probe",
+    "spk_topology_probe", "spk_near_product_lowp",
 ]
 STAGES = ["aggregation", "ss", "m2l", "rr", "fold", "disaggregation", "near_spmv", "matvec"]
 
@@ -72,6 +72,43 @@ class Stats(C.Structure):
 _lib = None
 
 
+def hip_runtimes(maps_text: str) -> list:
+    """The distinct HIP runtime files (libamdhip64.so*) mapped into a process,
+    from the text of its /proc/<pid>/maps: one entry per (device, inode),
+    named by the first path seen for it."""
+    seen = {}
+    for line in maps_text.splitlines():
+        parts = line.split(None, 5)
+        if len(parts) < 6:
+            continue
+        path = parts[5].strip()
+        if not os.path.basename(path).startswith("libamdhip64.so"):
+            continue
+        seen.setdefault((parts[3], parts[4]), path)
+    return sorted(seen.values())
+
+
+def check_single_hip_runtime(maps_text: str | None = None) -> None:
+    """Raise SpeckleError when two HIP runtimes are mapped into this process.
+
+    PyTorch wheels carry their own libamdhip64.so, whose soname equals the
+    ROCm one but whose file name does not: when this library is loaded
+    first, a later ``import torch`` maps a second runtime.  Two runtimes own
+    two device contexts; pointers from one are invalid in the other, and the
+    pair has been seen to crash at process exit.  Importing torch before
+    this library keeps one runtime (the loader then matches the soname)."""
+    if maps_text is None:
+        try:
+            with open("/proc/self/maps") as f:
+                maps_text = f.read()
+        except OSError:
+            return
+    rts = hip_runtimes(maps_text)
+    if len(rts) > 1:
+        raise SpeckleError("two HIP runtimes are loaded in this process (" + ", ".join(rts) +
+                           "): import torch before specklesim_amd, or do not import torch")
+
+
 def lib():
     """Load libspeckle_gpu.so (raises if it was not built)."""
     global _lib
@@ -113,6 +150,7 @@ def lib():
     vpp = C.POINTER(C.c_void_p)
     L.spk_matvec_batch_device.argtypes = [vp, C.c_int, vpp, vpp]
     L.spk_matvec_batch_c64_device.argtypes = [vp, C.c_int, vpp, vpp]
+    L.spk_near_product_lowp.argtypes = [vp, C.c_int, C.c_int, C.c_int, vpp, vpp]
     L.spk_gmres_batch_device.argtypes = [vp, C.c_int, vpp, vpp, C.POINTER(GmresParams),
                                          C.POINTER(GmresInfo)]
     L.spk_quad_topology_probe.argtypes = [C.c_double, C.c_double, C.c_int, C.c_int, dp, ip, ip, ip,
@@ -137,6 +175,7 @@ def lib():
     L.spk_profile_near_bf.argtypes = [vp, C.c_int, dp, dp]
     L.spk_topology_probe.argtypes = [C.c_int, dp, C.c_int, ip, C.c_int, C.c_int, i64p, ip, ip,
                                      i64p, ip, ip, dp, dp]
+    check_single_hip_runtime()
     _lib = L
     return L
 

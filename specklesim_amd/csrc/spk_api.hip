@@ -53,6 +53,24 @@ template <class T>
 struct DBuf {
   T *p = nullptr;
   size_t n = 0, cap = 0;
+  DBuf() = default;
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
+  DBuf(DBuf &&o) noexcept : p(o.p), n(o.n), cap(o.cap) {
+    o.p = nullptr;
+    o.n = o.cap = 0;
+  }
+  DBuf &operator=(DBuf &&o) noexcept {
+    if (this != &o) {
+      reset();
+      p = o.p;
+      n = o.n;
+      cap = o.cap;
+      o.p = nullptr;
+      o.n = o.cap = 0;
+    }
+    return *this;
+  }
   ~DBuf() { reset(); }
   void reset() {
     if (p) (void)hipFree(p);
@@ -97,6 +115,9 @@ struct LevelDev {
   DBuf<zd> B, D, tlt, up, down;
   DBuf<zf> fB, fD, ftlt, fup, fdown;  // complex64 mirror (mixed precision)
   DBuf<int> child, parent, octant, i4_off, i4_box, i4_slot;
+  DBuf<double> rhat;  // TL inputs: slot directions, per-slot Hankel factors, live slots
+  DBuf<zd> coef;
+  DBuf<int> live;
   int nb = 0;
 };
 
@@ -178,6 +199,16 @@ struct spk_handle {
       return e;
     }
   } pin;
+  // near-field preconditioned flexible GMRES workspace, kept between solves
+  // (DBuf reuses an allocation that fits): outer bases V, Z, inner bases U
+  // (complex64 on mixed handles, complex128 otherwise), pointer tables
+  struct FgWork {
+    DBuf<zd> V, Z, W, Hd, Rv, Tv, Dv, Yo, Yi, h1, Hi, dots, apart, part;
+    DBuf<zf> Uf, Wif;
+    DBuf<zd> Ud, Wid;
+    DBuf<uint64_t> tin, tprod, torth;
+    DBuf<int> kout;
+  } fg;
   DBuf<unsigned long long> g_ctl;    // MGS sweep abort flag
   DBuf<unsigned char> g_slots;       // MGS sweep exchange slots
   unsigned long long sweep_seq = 0;  // sequence number of the last sweep launch
@@ -185,6 +216,14 @@ struct spk_handle {
   std::vector<double> hist;
   void *scan_tmp = nullptr;
   size_t scan_bytes = 0;
+  // near fill scratch: block classes, task counts and offsets, task lists,
+  // edge-adjacent K terms and singular-pair partial blocks
+  struct NearWork {
+    DBuf<uint8_t> cls;
+    DBuf<int> ca, cb, ba, bb;
+    DBuf<NearTask> ta, tb;
+    DBuf<zd> kt, zz;
+  } nw;
   spk_stats stats{};
   ~spk_handle() {
     if (scan_tmp) (void)hipFree(scan_tmp);
@@ -227,8 +266,9 @@ int near_fill(spk_handle *h) {
   HIPCHK(h->d_blocks.alloc(4 * (size_t)nnz));
   if (nnz == 0) return SPK_OK;
   NearArgs a = near_args(h);
-  DBuf<uint8_t> cls;
-  DBuf<int> ca, cb, ba, bb;
+  auto &NW = h->nw;
+  DBuf<uint8_t> &cls = NW.cls;
+  DBuf<int> &ca = NW.ca, &cb = NW.cb, &ba = NW.ba, &bb = NW.bb;
   HIPCHK(cls.alloc(nnz));
   HIPCHK(ca.alloc(nnz + 1));
   HIPCHK(cb.alloc(nnz + 1));
@@ -237,15 +277,15 @@ int near_fill(spk_handle *h) {
   HIPCHK(hipMemsetAsync(ca.p + nnz, 0, sizeof(int), h->st));
   HIPCHK(hipMemsetAsync(cb.p + nnz, 0, sizeof(int), h->st));
   launch_near_classify(a, nnz, cls.p, ca.p, cb.p, h->st);
-  exclusive_scan_int(ca.p, ba.p, nnz + 1, &h->scan_tmp, &h->scan_bytes, h->st);
-  exclusive_scan_int(cb.p, bb.p, nnz + 1, &h->scan_tmp, &h->scan_bytes, h->st);
+  HIPCHK(exclusive_scan_int(ca.p, ba.p, nnz + 1, &h->scan_tmp, &h->scan_bytes, h->st));
+  HIPCHK(exclusive_scan_int(cb.p, bb.p, nnz + 1, &h->scan_tmp, &h->scan_bytes, h->st));
   int na = 0, nb = 0;
   HIPCHK(hipMemcpyAsync(&na, ba.p + nnz, sizeof(int), hipMemcpyDeviceToHost, h->st));
   HIPCHK(hipMemcpyAsync(&nb, bb.p + nnz, sizeof(int), hipMemcpyDeviceToHost, h->st));
   HIPCHK(hipStreamSynchronize(h->st));
   HIPCHK(hipGetLastError());
-  DBuf<NearTask> ta, tb;
-  DBuf<zd> kt, zz;
+  DBuf<NearTask> &ta = NW.ta, &tb = NW.tb;
+  DBuf<zd> &kt = NW.kt, &zz = NW.zz;
   HIPCHK(ta.alloc(std::max(na, 1)));
   HIPCHK(tb.alloc(std::max(nb, 1)));
   HIPCHK(kt.alloc(2 * (size_t)std::max(nb, 1)));
@@ -310,8 +350,7 @@ int setup_mlfmm(spk_handle *h, std::future<std::vector<HostLevelTables>> &pre) {
   t0 = now_ms();
   const std::vector<HostLevelTables> ht = pre.get();
   // levels
-  h->lev.clear();
-  h->lev.resize(t.l_max + 1);
+  if (h->lev.size() < (size_t)t.l_max + 1) h->lev.resize(t.l_max + 1);
   const int nthreads = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
   for (int l = t.l_min; l <= t.l_max; ++l) {
     const Level &L = t.lev[l];
@@ -332,9 +371,9 @@ int setup_mlfmm(spk_handle *h, std::future<std::vector<HostLevelTables>> &pre) {
       const std::vector<int> &live = ht[l].live;
       std::vector<zd> cz(coef.size());
       for (size_t i = 0; i < coef.size(); ++i) cz[i] = zd{coef[i].re, coef[i].im};
-      DBuf<double> d_rhat;
-      DBuf<zd> d_coef;
-      DBuf<int> d_live;
+      DBuf<double> &d_rhat = D.rhat;
+      DBuf<zd> &d_coef = D.coef;
+      DBuf<int> &d_live = D.live;
       HIPCHK(d_rhat.upload(rhat));
       HIPCHK(d_coef.upload(cz));
       HIPCHK(d_live.upload(live));
@@ -459,10 +498,12 @@ static int batch_ensure(spk_handle *h, int R) {
   if (BB.R >= R) return SPK_OK;
   const Tree &t = h->tree;
   const size_t S = h->tb.S;
-  BB.B.clear();
-  BB.D.clear();
-  BB.B.resize(t.l_max + 1);
-  BB.D.resize(t.l_max + 1);
+  // allocations that fit are reused (a handle set up again for a mesh of
+  // the same size allocates nothing)
+  if (BB.B.size() < (size_t)t.l_max + 1) {
+    BB.B.resize(t.l_max + 1);
+    BB.D.resize(t.l_max + 1);
+  }
   for (int l = t.l_min; l <= t.l_max; ++l) {
     HIPCHK(BB.B[l].alloc((size_t)h->lev[l].nb * 8 * S * R));
     HIPCHK(BB.D[l].alloc((size_t)h->lev[l].nb * 8 * S * R));
@@ -702,8 +743,9 @@ int spk_setup_tri(spk_handle *h, int np, const double *pp, int nt, const int *tt
     h->dx_tiles = (int)xb.size();
     HIPCHK(h->d_dx_box.upload(xb));
     HIPCHK(h->d_dx_p0.upload(xp));
-    h->bd = BatchBufs<zd>();
-    h->bf = BatchBufs<zf>();
+    // the batch buffers are resized for this mesh on their next use
+    h->bd.R = 0;
+    h->bf.R = 0;
   }
   HIPCHK(h->d_x.alloc(2 * (size_t)h->N));
   HIPCHK(h->d_y.alloc(2 * (size_t)h->N));
@@ -923,7 +965,7 @@ static int rhs_plane_dev(spk_handle *h, const double k_in[3], const double e_in[
 int spk_rhs_plane_wave_device(spk_handle *h, const double k_in[3], const double e_in[3],
                               void *d_b) {
   if (!h || !k_in || !e_in || !d_b) return fail(SPK_ERR_ARG, "null argument");
-  if (h->N <= 0 || !h->d_fn.p) return fail(SPK_ERR_STATE, "no mesh");
+  if (!h->ready || h->N <= 0 || !h->d_fn.p) return fail(SPK_ERR_STATE, "spk_setup_tri has not completed");
   int rc = rhs_plane_dev(h, k_in, e_in, (zd *)d_b);
   if (rc) return rc;
   HIPCHK(hipStreamSynchronize(h->st));
@@ -932,7 +974,7 @@ int spk_rhs_plane_wave_device(spk_handle *h, const double k_in[3], const double 
 
 int spk_rhs_plane_wave(spk_handle *h, const double k_in[3], const double e_in[3], double *bout) {
   if (!h || !k_in || !e_in || !bout) return fail(SPK_ERR_ARG, "null argument");
-  if (h->N <= 0 || !h->d_fn.p) return fail(SPK_ERR_STATE, "no mesh");
+  if (!h->ready || h->N <= 0 || !h->d_fn.p) return fail(SPK_ERR_STATE, "spk_setup_tri has not completed");
   DBuf<zd> db;
   HIPCHK(db.alloc(2 * (size_t)h->N));
   int rc = rhs_plane_dev(h, k_in, e_in, db.p);
@@ -978,7 +1020,7 @@ static int scattered_dev(spk_handle *h, int npts, const double *r, const zd *d_x
 int spk_scattered_field_device(spk_handle *h, int npts, const double *r, const void *d_x,
                                double *E, double *H) {
   if (!h || !r || !d_x || !E || !H || npts < 0) return fail(SPK_ERR_ARG, "bad argument");
-  if (h->N <= 0 || !h->d_fn.p) return fail(SPK_ERR_STATE, "no mesh");
+  if (!h->ready || h->N <= 0 || !h->d_fn.p) return fail(SPK_ERR_STATE, "spk_setup_tri has not completed");
   if (npts == 0) return SPK_OK;
   return scattered_dev(h, npts, r, (const zd *)d_x, E, H);
 }
@@ -986,7 +1028,7 @@ int spk_scattered_field_device(spk_handle *h, int npts, const double *r, const v
 int spk_scattered_field(spk_handle *h, int npts, const double *r, const double *x, double *E,
                         double *H) {
   if (!h || !r || !x || !E || !H || npts < 0) return fail(SPK_ERR_ARG, "bad argument");
-  if (h->N <= 0 || !h->d_fn.p) return fail(SPK_ERR_STATE, "no mesh");
+  if (!h->ready || h->N <= 0 || !h->d_fn.p) return fail(SPK_ERR_STATE, "spk_setup_tri has not completed");
   if (npts == 0) return SPK_OK;
   DBuf<zd> dx;
   HIPCHK(dx.alloc(2 * (size_t)h->N));
@@ -1517,6 +1559,11 @@ static int ensure_near_f8(spk_handle *h) {
   return SPK_OK;
 }
 
+static DBuf<zf> &fgU(spk_handle::FgWork &F, zf *) { return F.Uf; }
+static DBuf<zd> &fgU(spk_handle::FgWork &F, zd *) { return F.Ud; }
+static DBuf<zf> &fgWi(spk_handle::FgWork &F, zf *) { return F.Wif; }
+static DBuf<zd> &fgWi(spk_handle::FgWork &F, zd *) { return F.Wid; }
+
 // Flexible GMRES on the FP64 operator for R <= kBatchMax right-hand sides,
 // right-preconditioned by kin GMRES steps (from zero) on the near-field
 // operator: complex64 on an SPK_MIXED handle (T = zf), FP64 otherwise.  The
@@ -1572,8 +1619,10 @@ static int fgmres_near(spk_handle *h, int R, const zd *const *b, zd *const *x,
   constexpr double kRelaxDelta = 5e-3;
   const size_t ldh = (size_t)m + 2, ldhi = (size_t)kin + 2, hik = (size_t)(kin + 1) * kin;
   const size_t ldu = (size_t)(kin + 1) * n;
-  DBuf<zd> V, Z, W, Hd, Rv, Tv, Dv, Yo, Yi, h1, Hi, dots;
-  DBuf<T> U, Wi;
+  auto &F = h->fg;
+  DBuf<zd> &V = F.V, &Z = F.Z, &W = F.W, &Hd = F.Hd, &Rv = F.Rv, &Tv = F.Tv, &Dv = F.Dv,
+           &Yo = F.Yo, &Yi = F.Yi, &h1 = F.h1, &Hi = F.Hi, &dots = F.dots;
+  DBuf<T> &U = fgU(F, (T *)nullptr), &Wi = fgWi(F, (T *)nullptr);
   HIPCHK(V.alloc((size_t)R * (m + 1) * n));
   HIPCHK(Z.alloc((size_t)R * m * n));
   HIPCHK(W.alloc((size_t)R * n));
@@ -1586,12 +1635,12 @@ static int fgmres_near(spk_handle *h, int R, const zd *const *b, zd *const *x,
   HIPCHK(U.alloc((size_t)R * ldu));
   HIPCHK(Wi.alloc((size_t)R * n));
   HIPCHK(h1.alloc((size_t)R * ldhi));
-  DBuf<zd> apart;
+  DBuf<zd> &apart = F.apart;
   HIPCHK(apart.alloc(arnoldi_partial_elems(n, R, kin)));
   HIPCHK(Hi.alloc((size_t)R * hik));
   HIPCHK(dots.alloc(R));
   const int kmax = std::max(m, kin) + 1;
-  DBuf<zd> part;
+  DBuf<zd> &part = F.part;
   HIPCHK(part.alloc(std::max(batch_orth_partial_elems(n, R, m + 1), (size_t)reduce_blocks(n))));
   HIPCHK(h->g_pn.alloc(gemv_n_partial_elems(n, kmax)));
   HIPCHK(h->pin.ensure(std::max((size_t)R * ldh, (size_t)R * hik)));
@@ -1600,8 +1649,8 @@ static int fgmres_near(spk_handle *h, int R, const zd *const *b, zd *const *x,
   // (per step; separate arrays: a table is rewritten only after the kernels
   // that read its last contents have completed)
   constexpr int B = kBatchMax;
-  DBuf<uint64_t> tin, tprod, torth;
-  DBuf<int> kout;
+  DBuf<uint64_t> &tin = F.tin, &tprod = F.tprod, &torth = F.torth;
+  DBuf<int> &kout = F.kout;
   HIPCHK(tin.alloc((size_t)(1 + kin) * B));
   HIPCHK(tprod.alloc(2 * (size_t)B));
   HIPCHK(torth.alloc(3 * (size_t)B));
@@ -1944,6 +1993,54 @@ static int fgmres_near(spk_handle *h, int R, const zd *const *b, zd *const *x,
   return SPK_OK;
 }
 }  // extern "C++"
+
+// The near-field part of the operator on an SPK_MIXED handle's complex64
+// blocks or on the preconditioner's bf16 / fp8 copies, for R complex64
+// vectors: one launch per vector (the single-vector kernels) or one batched
+// launch (the kernels the batched inner solves use).
+int spk_near_product_lowp(spk_handle *h, int format, int batched, int R, const void *const *d_x,
+                          void *const *d_y) {
+  if (!h || !d_x || !d_y) return fail(SPK_ERR_ARG, "null argument");
+  if (!h->ready) return fail(SPK_ERR_STATE, "spk_setup_tri has not completed");
+  if (!h->mixed_ready) return fail(SPK_ERR_STATE, "handle was not created with SPK_MIXED");
+  if (R < 1 || R > kBatchMax) return fail(SPK_ERR_ARG, "batch size must be 1..16");
+  if (format < SPK_NEAR_C64 || format > SPK_NEAR_FP8) return fail(SPK_ERR_ARG, "unknown format");
+  for (int r = 0; r < R; ++r)
+    if (!d_x[r] || !d_y[r]) return fail(SPK_ERR_ARG, "null vector in the batch");
+  HIPCHK(hipSetDevice(h->p.device));
+  int rc = SPK_OK;
+  if (format == SPK_NEAR_BF16 && (rc = ensure_near_bf(h))) return rc;
+  if (format == SPK_NEAR_FP8 && (rc = ensure_near_f8(h))) return rc;
+  const uint32_t *bf = format == SPK_NEAR_BF16 ? h->d_blk_bf.p : nullptr;
+  const uint32_t *f8 = format == SPK_NEAR_FP8 ? h->d_blk_f8.p : nullptr;
+  if (!batched) {
+    for (int r = 0; r < R; ++r) {
+      const zf *x = (const zf *)d_x[r];
+      zf *y = (zf *)d_y[r];
+      if (f8)
+        launch_near_spmv_f8(h->spmv_wgs, h->N, h->d_row_box.p, h->d_row_r0.p, h->d_box_first.p,
+                            h->d_box_count.p, h->d_e2_off.p, h->d_e2_list.p, h->d_near_off.p,
+                            h->d_perm.p, f8, h->d_f8_scale.p, x, y, h->st);
+      else if (bf)
+        launch_near_spmv_bf(h->spmv_wgs, h->N, h->d_row_box.p, h->d_row_r0.p, h->d_box_first.p,
+                            h->d_box_count.p, h->d_e2_off.p, h->d_e2_list.p, h->d_near_off.p,
+                            h->d_perm.p, bf, x, y, h->st);
+      else if ((rc = matvec_t<zf>(h, x, y, true, false)))
+        return rc;
+    }
+  } else {
+    if ((rc = batch_ensure<zf>(h, R))) return rc;
+    BatchBufs<zf> &BB = h->bf;
+    HIPCHK(hipMemcpy(BB.xs.p, d_x, R * sizeof(void *), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(BB.ys.p, d_y, R * sizeof(void *), hipMemcpyHostToDevice));
+    if ((rc = matvec_batch_t<zf>(h, R, BB.xs.p, BB.ys.p, false, f8 ? f8 : bf,
+                                 f8 ? h->d_f8_scale.p : nullptr)))
+      return rc;
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(h->st));
+  return SPK_OK;
+}
 
 int spk_gmres_batch_device(spk_handle *h, int R, const void *const *d_b, void *const *d_x,
                            const spk_gmres_params *gp, spk_gmres_info *info) {
@@ -2294,6 +2391,7 @@ int spk_gmres_history(const spk_handle *h, double *hist, int cap, int *count) {
 
 int spk_get_rwg(const spk_handle *h, int *corner, int *element) {
   if (!h) return fail(SPK_ERR_ARG, "null argument");
+  if (!h->ready) return fail(SPK_ERR_STATE, "spk_setup_tri has not completed");
   if (corner) std::memcpy(corner, h->corner.data(), h->corner.size() * sizeof(int));
   if (element) std::memcpy(element, h->element.data(), h->element.size() * sizeof(int));
   return SPK_OK;
@@ -2302,6 +2400,7 @@ int spk_get_rwg(const spk_handle *h, int *corner, int *element) {
 int spk_get_tree(const spk_handle *h, int64_t *leaf, int *l_min, int *l_max, double *D,
                  double *xmin3) {
   if (!h) return fail(SPK_ERR_ARG, "null argument");
+  if (!h->ready) return fail(SPK_ERR_STATE, "spk_setup_tri has not completed");
   const Tree &t = h->tree;
   if (leaf) std::memcpy(leaf, t.leaf.data(), t.leaf.size() * sizeof(int64_t));
   if (l_min) *l_min = t.l_min;
@@ -2313,6 +2412,7 @@ int spk_get_tree(const spk_handle *h, int64_t *leaf, int *l_min, int *l_max, dou
 
 int spk_get_i4(const spk_handle *h, int level, int64_t n, int64_t *out, int cap, int *count) {
   if (!h || !count) return fail(SPK_ERR_ARG, "null argument");
+  if (!h->ready) return fail(SPK_ERR_STATE, "spk_setup_tri has not completed");
   const Tree &t = h->tree;
   if (level < t.l_min || level > t.l_max) return fail(SPK_ERR_ARG, "level outside [l_min, l_max]");
   const Level &L = t.lev[level];
@@ -2356,7 +2456,7 @@ int spk_get_near_csr(const spk_handle *h, int64_t *ia, int32_t *ja, double *abcd
 
 int spk_near_blocks(spk_handle *h, int npairs, const int *m, const int *n, double *out) {
   if (!h || !m || !n || !out || npairs < 0) return fail(SPK_ERR_ARG, "null argument");
-  if (h->N <= 0 || !h->d_fn.p) return fail(SPK_ERR_STATE, "no mesh");
+  if (!h->ready || h->N <= 0 || !h->d_fn.p) return fail(SPK_ERR_STATE, "spk_setup_tri has not completed");
   for (int i = 0; i < npairs; ++i)
     if (m[i] < 0 || m[i] >= h->N || n[i] < 0 || n[i] >= h->N)
       return fail(SPK_ERR_ARG, "edge index out of range");

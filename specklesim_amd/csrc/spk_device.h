@@ -87,7 +87,7 @@ void launch_near_singular(const NearArgs &a, int ntask, const NearTask *ta, cons
                           const int *base_b, const zd *kt, zd *zz, hipStream_t st);
 void launch_near_regular(const NearArgs &a, int64_t nnz, const uint8_t *cls, const int *base_a,
                          const zd *zz, zd *blocks, hipStream_t st);
-void exclusive_scan_int(const int *in, int *out, int64_t n, void **tmp, size_t *tmp_bytes,
+hipError_t exclusive_scan_int(const int *in, int *out, int64_t n, void **tmp, size_t *tmp_bytes,
                         hipStream_t st);
 
 // MLFMM (FP64)

@@ -735,16 +735,23 @@ void launch_near_regular(const NearArgs &a, int64_t nnz, const uint8_t *cls, con
                        base_a, zz, blocks);
 }
 
-void exclusive_scan_int(const int *in, int *out, int64_t n, void **tmp, size_t *tmp_bytes,
-                        hipStream_t st) {
+hipError_t exclusive_scan_int(const int *in, int *out, int64_t n, void **tmp, size_t *tmp_bytes,
+                              hipStream_t st) {
   size_t need = 0;
-  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, need, in, out, (int)n, st);
-  if (need > *tmp_bytes) {
+  hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, need, in, out, (int)n, st);
+  if (e != hipSuccess) return e;
+  if (need > *tmp_bytes || !*tmp) {
     if (*tmp) (void)hipFree(*tmp);
-    (void)hipMalloc(tmp, need);
+    *tmp = nullptr;
+    *tmp_bytes = 0;
+    e = hipMalloc(tmp, need ? need : 1);
+    if (e != hipSuccess) {
+      *tmp = nullptr;
+      return e;
+    }
     *tmp_bytes = need;
   }
-  (void)hipcub::DeviceScan::ExclusiveSum(*tmp, need, in, out, (int)n, st);
+  return hipcub::DeviceScan::ExclusiveSum(*tmp, need, in, out, (int)n, st);
 }
 
 // ---------------------------------------------------------------------------

@@ -71,6 +71,7 @@ class SieTriMlfmm:
                  normal_convention=1, ng=3, tree_s_opt=250, truncation_number=19,
                  precision="fp64", device=0):
         L = nat.lib()
+        nat.check_single_hip_runtime()
         f = FORMULATIONS[formulation] if isinstance(formulation, str) else int(formulation)
         prec = {"fp64": 0, "mixed": 1}[precision]
         eps_r1, eps_r2 = complex(eps_r1), complex(eps_r2)
@@ -338,6 +339,25 @@ class SieTriMlfmm:
         fn = nat.lib().spk_profile_near_f8 if fp8 else nat.lib().spk_profile_near_bf
         nat.check(fn(self._h, reps, _dp(ms), _dp(by)))
         return float(ms[0]), float(by[0])
+
+    def near_product_lowp(self, X, fmt="fp8", batched=False):
+        """The near-field part of a precision="mixed" handle's operator on the
+        rows of X (R <= 16 complex64 vectors of length 2N), with the blocks in
+        complex64 ("c64") or in the preconditioner's "bf16" / "fp8" copy;
+        batched: the batched launch of the inner solves instead of one
+        single-vector launch per vector."""
+        from ._native import DeviceBuffer
+
+        X = np.ascontiguousarray(np.atleast_2d(X), dtype=np.complex64)
+        R = X.shape[0]
+        nb = 2 * self.N * 8
+        xb = [DeviceBuffer(nb).upload(X[r]) for r in range(R)]
+        yb = [DeviceBuffer(nb) for _ in range(R)]
+        xs = (C.c_void_p * R)(*[b.ptr for b in xb])
+        ys = (C.c_void_p * R)(*[b.ptr for b in yb])
+        f = {"c64": 0, "bf16": 1, "fp8": 2}[fmt]
+        nat.check(nat.lib().spk_near_product_lowp(self._h, f, int(bool(batched)), R, xs, ys))
+        return np.stack([yb[r].download(np.complex64) for r in range(R)])
 
     def profile_matvec(self, reps=20, c64=False):
         """Mean device time [ms] and algorithmic HBM bytes of each matvec

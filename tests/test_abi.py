@@ -51,3 +51,24 @@ def test_topology_probe_runs_without_gpu():
                               tt.shape[0], tt.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), 1,
                               250, ctypes.byref(n), None, None, None, None, None, None, None)
     assert rc == 0 and n.value == 120  # 80 triangles -> 120 edges on a closed surface
+
+
+def test_two_hip_runtimes_are_refused():
+    """A process that maps two different libamdhip64 files (a PyTorch wheel's
+    and ROCm's) is refused; two mappings of one file are one runtime."""
+    one = ("7f0000000000-7f0000100000 r-xp 00000000 fd:01 1234 /opt/rocm/lib/libamdhip64.so.7.2.0\n"
+           "7f0000100000-7f0000200000 r--p 00100000 fd:01 1234 /opt/rocm/lib/libamdhip64.so.7.2.0\n"
+           "7f0000300000-7f0000400000 r-xp 00000000 fd:01 99 /usr/lib/libc.so.6\n")
+    two = one + ("7f0000500000-7f0000600000 r-xp 00000000 fd:02 777 "
+                 "/usr/local/lib/python3.10/dist-packages/torch/lib/libamdhip64.so\n")
+    assert _native.hip_runtimes(one) == ["/opt/rocm/lib/libamdhip64.so.7.2.0"]
+    assert len(_native.hip_runtimes(two)) == 2
+    _native.check_single_hip_runtime(one)
+    import pytest
+
+    with pytest.raises(_native.SpeckleError, match="two HIP runtimes"):
+        _native.check_single_hip_runtime(two)
+    # this process, after loading the library
+    _native.lib()
+    with open("/proc/self/maps") as f:
+        assert len(_native.hip_runtimes(f.read())) <= 1

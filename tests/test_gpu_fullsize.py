@@ -1,0 +1,166 @@
+"""GPU parity at the full sizes of the two CPU-reachable workloads, through
+the C-ABI, against the restatement on the same inputs.
+
+- Rough patch 10 lambda x 10 lambda, 100 x 100 cells, 20,000 triangles,
+  N = 29,800 RWG edges, levels 2-4 (the benchmark workload, ICTF, n = 1.5,
+  seed 1): every integer structure bit-exact (RWG pairs, leaves, CSR, the
+  I4 list of every non-empty box at every level), every near block to 1e-10,
+  the matvec to 1e-10, a 60-step IMGS history to 1e-6, and the benchmark's
+  near-field preconditioned solve checked under the restatement's operator.
+- Icosphere level 3 (1280 triangles, N = 1920), r = lambda/2, PMCHWT with the
+  7-point rule: near blocks to 1e-10, the right-hand side to 1e-13, and the
+  matvec and a converged GMRES solution to the conditioning bound of the
+  K = 19 sum on 0.25 lambda leaves (1e-5).
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from specklesim_amd import DeviceBuffer, SieTriMlfmm, meshes, plane_wave
+
+pytestmark = pytest.mark.gpu
+LAM = 633e-9
+NT = 16
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.fixture(scope="module")
+def c2():
+    pp, tt = meshes.rough_surface(100, 100, 10.0, 10.0, 0.1, 0.5, 1, LAM)
+    print("c2: restatement tree, CSR and near fill", flush=True)
+    o = O.Oracle(pp, tt, LAM, 2.25, formulation="ICTF", normal_convention=2)
+    to = o.tree()
+    ia, ja, v = o.near_csr(nthreads=NT)
+    o.mlfmm_setup(NT)
+    print("c2: GPU precalculation", flush=True)
+    g = SieTriMlfmm(LAM, 2.25, formulation="ICTF", normal_convention=2).precalculation_tri(pp, tt)
+    return pp, tt, o, to, (ia, ja, v), g
+
+
+def test_c2_sizes(c2):
+    _, _, o, to, (ia, ja, v), g = c2
+    st = g.stats()
+    assert o.N == g.N == 29800
+    assert (st["l_min"], st["l_max"]) == (to["l_min"], to["l_max"]) == (2, 4)
+    assert st["near_blocks"] == ja.size
+
+
+def test_c2_integer_structures_bit_exact(c2):
+    _, _, o, to, (ia, ja, _), g = c2
+    c1, e1 = g.rwg()
+    c2_, e2 = o.pairs()
+    assert np.array_equal(c1, c2_) and np.array_equal(e1, e2)
+    tg = g.tree()
+    assert np.array_equal(tg["leaf"], to["leaf"])
+    assert tg["D"] == to["D"] and np.array_equal(tg["xmin"], to["xmin"])
+    ia2, ja2, _ = g.near_csr(values=False)
+    assert np.array_equal(ia2, ia) and np.array_equal(ja2, ja)
+    lmax = to["l_max"]
+    nboxes = 0
+    for level in range(to["l_min"], lmax + 1):
+        boxes = np.unique(to["leaf"] >> (3 * (lmax - level)))
+        live = set(boxes.tolist())
+        for box in boxes:
+            ref = sorted(b for b in o.i4(level, int(box)) if b in live)
+            assert np.array_equal(g.i4(level, int(box)), np.array(ref, np.int64)), (level, box)
+            nboxes += 1
+    assert nboxes > 300
+
+
+def test_c2_near_blocks(c2):
+    _, _, _, _, (_, _, v), g = c2
+    _, _, vg = g.near_csr()
+    for k in range(4):
+        assert rel(vg[:, k], v[:, k]) < 1e-10, k
+    scale = np.abs(v).max(axis=1)
+    assert np.max(np.abs(vg - v).max(axis=1) / scale) < 1e-9
+
+
+def test_c2_matvec(c2):
+    _, _, o, _, _, g = c2
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal(2 * o.N) + 1j * rng.standard_normal(2 * o.N)
+    yo = o.matvec(x, NT)
+    yg = g.calculate_vector_b_tri(x)
+    assert rel(yg[:o.N], yo[:o.N]) < 1e-10
+    assert rel(yg[o.N:], yo[o.N:]) < 1e-10
+
+
+def test_c2_gmres_history(c2):
+    """60 IMGS Arnoldi steps at normal incidence: the history to 1e-6 (log
+    ratio) and the iterate to 1e-6."""
+    _, _, o, _, _, g = c2
+    k, e = plane_wave(0.0, 0.0, 2, "TM")
+    b = g.get_vector_b_tri(k, e)
+    bo = o.rhs_plane(k, e)
+    assert rel(b, bo) < 1e-13
+    x, info = g.gmres_run(b, tolerance=1e-12, max_iterations=60, orthogonalization="IMGS")
+    xo, io = o.gmres(bo, tol=1e-12, maxit=60, orth=1, nthreads=NT)
+    assert info["iters"] == io["iters"] == 60
+    assert np.max(np.abs(np.log(info["hist"] / io["hist"]))) < 1e-6
+    assert rel(x, xo) < 1e-6
+
+
+def test_c2_headline_solve_under_restatement_operator(c2):
+    """The benchmark's headline solve (flexible GMRES on the FP64 operator,
+    fp8 near-field inner solves, relaxed complex64 outer products, SPK_MIXED
+    handle) meets the tolerance under the restatement's operator."""
+    pp, tt, o, _, _, _ = c2
+    s = SieTriMlfmm(LAM, 2.25, formulation="ICTF", normal_convention=2,
+                    precision="mixed").precalculation_tri(pp, tt)
+    k, e = plane_wave(0.0, 0.0, 2, "TM")
+    d_b, d_x = DeviceBuffer(2 * s.N * 16), DeviceBuffer(2 * s.N * 16)
+    s.get_vector_b_tri_device(k, e, d_b.ptr)
+    info = s.gmres_run_device(d_b.ptr, d_x.ptr, tolerance=1e-6, max_iterations=650,
+                              orthogonalization="ICGS", use_initial_guess=False, near_field=True)
+    assert info["info"] == 0 and info["iters"] < 40, info
+    x = d_x.download()
+    bo = o.rhs_plane(k, e)
+    res = np.linalg.norm(bo - o.matvec(x, NT)) / np.linalg.norm(bo)
+    assert res <= 1.01e-6, res
+
+
+@pytest.fixture(scope="module")
+def c1():
+    pp, tt = meshes.icosphere(3, 0.5 * LAM)
+    o = O.Oracle(pp, tt, LAM, 2.25, formulation="PMCHWT", ngp=7)
+    o.tree()
+    ia, ja, v = o.near_csr(nthreads=NT)
+    o.mlfmm_setup(NT)
+    g = SieTriMlfmm(LAM, 2.25, formulation="PMCHWT", ng=7).precalculation_tri(pp, tt)
+    return pp, tt, o, (ia, ja, v), g
+
+
+def test_c1_seven_point_rule(c1):
+    _, tt, o, (ia, ja, v), g = c1
+    assert tt.shape[0] == 1280 and o.N == g.N == 1920
+    c1_, e1 = g.rwg()
+    c2_, e2 = o.pairs()
+    assert np.array_equal(c1_, c2_) and np.array_equal(e1, e2)
+    ia2, ja2, vg = g.near_csr()
+    assert np.array_equal(ia2, ia) and np.array_equal(ja2, ja)
+    for k in range(4):
+        assert rel(vg[:, k], v[:, k]) < 1e-10, k
+    # the 7-point rule changes the blocks (the 3-point oracle differs)
+    o3 = O.Oracle(*c1[:2], LAM, 2.25, formulation="PMCHWT", ngp=3)
+    z3 = o3.near_block(0, 5)
+    assert np.linalg.norm(z3 - o.near_block(0, 5)) > 1e-6 * np.linalg.norm(z3)
+    k, e = plane_wave(0.0, 0.0, 2, "TM")
+    assert rel(g.get_vector_b_tri(k, e), o.rhs_plane(k, e)) < 1e-13
+
+
+def test_c1_matvec_and_solution(c1):
+    _, _, o, _, g = c1
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal(2 * o.N) + 1j * rng.standard_normal(2 * o.N)
+    assert rel(g.calculate_vector_b_tri(x), o.matvec(x, NT)) < 1e-5
+    k, e = plane_wave(0.0, 0.0, 2, "TM")
+    b = g.get_vector_b_tri(k, e)
+    xg, ig = g.gmres_run(b, tolerance=1e-6, max_iterations=650)
+    xo, io = o.gmres(o.rhs_plane(k, e), tol=1e-6, maxit=650, nthreads=NT)
+    assert ig["info"] == io["info"] == 0
+    assert abs(ig["iters"] - io["iters"]) <= 2
+    assert rel(xg, xo) < 1e-4
