@@ -43,6 +43,7 @@ def lib():
         L.spo_get_pairs.argtypes = [C.c_void_p, ip, ip]
         L.spo_get_fn.argtypes = [C.c_void_p, C.c_int, C.c_int, dp]
         L.spo_near_block.argtypes = [C.c_void_p, C.c_int, C.c_int, dp]
+        L.spo_edge_adjacent_kt.argtypes = [C.c_void_p] + [C.c_int] * 5 + [dp]
         L.spo_tree_build.argtypes = [C.c_void_p]
         L.spo_tree_info.argtypes = [C.c_void_p, ip, ip, dp, dp]
         L.spo_edge_leaf.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
@@ -150,6 +151,13 @@ class Oracle:
         o = np.zeros(8)
         lib().spo_near_block(self.h, m, n, _d(o))
         return (o[0::2] + 1j * o[1::2]).reshape(2, 2)
+
+    def edge_adjacent_kt(self, m, t, n, s, md):
+        """The 12x12-rule K of side t of edge m against side s of edge n in
+        medium md (1 outside, 2 inside): (K, its double integral, x_nt)."""
+        o = np.zeros(5)
+        lib().spo_edge_adjacent_kt(self.h, m, t, n, s, md, _d(o))
+        return o[0] + 1j * o[1], o[2] + 1j * o[3], o[4]
 
     def tree(self):
         L = lib()

@@ -702,7 +702,8 @@ static relem outer(const spo_ctx *c, const fnrwg *fm, const fnrwg *fn, int ngp, 
 
 /* tri_outer_integration_edge_adjacent (:2165-2197) with
  * tri_inner_integration_edge_adjacent (:2011-2044), 12x12 rule. */
-static zc outer_edge_adjacent(const spo_ctx *c, const fnrwg *fm, const fnrwg *fn, int md) {
+static zc outer_edge_adjacent_parts(const spo_ctx *c, const fnrwg *fm, const fnrwg *fn, int md,
+                                    zc *kint, double *xnt) {
   double qa[20], qb[20], qw[20];
   const int ngp = 12;
   spo_quadrature_tri(ngp, qa, qb, qw);
@@ -737,7 +738,15 @@ static zc outer_edge_adjacent(const spo_ctx *c, const fnrwg *fm, const fnrwg *fn
     x_nt = x_nt - vdot(rfn, nx) * TRF;
   }
   double sg = (md == 1) ? -0.5 : 0.5;
+  *kint = Kt;
+  *xnt = x_nt;
   return Kt + sg * (double)(float)x_nt;
+}
+
+static zc outer_edge_adjacent(const spo_ctx *c, const fnrwg *fm, const fnrwg *fn, int md) {
+  zc kint;
+  double xnt;
+  return outer_edge_adjacent_parts(c, fm, fn, md, &kint, &xnt);
 }
 
 /* edge_adjacent_tri: :2544-2569 (2 common corners) */
@@ -860,6 +869,21 @@ static void impedance_edges(const spo_ctx *c, int m, int n, double r_ref, zc *D)
   if (form == SPO_PMCHWT_S) {
     D[1] = D[1] * c->eta1; D[2] = D[2] * c->eta1; D[3] = D[3] * (c->eta1 * c->eta1);
   }
+}
+
+/* Probe: the 12x12 edge-adjacent K (outer_edge_adjacent) of side tt of edge m
+ * against side ll of edge n in medium md (1 outside, 2 inside):
+ * out = (K re, K im, integral part re, im, x_nt before its float rounding). */
+void spo_edge_adjacent_kt(const spo_ctx *c, int m, int tt, int n, int ll, int md, double *out) {
+  zc kint;
+  double xnt;
+  zc k = outer_edge_adjacent_parts(c, &c->fn[2 * m + tt - 1], &c->fn[2 * n + ll - 1], md, &kint,
+                                   &xnt);
+  out[0] = creal(k);
+  out[1] = cimag(k);
+  out[2] = creal(kint);
+  out[3] = cimag(kint);
+  out[4] = xnt;
 }
 
 void spo_near_block(const spo_ctx *c, int m, int n, double *out) {

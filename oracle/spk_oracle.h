@@ -53,6 +53,8 @@ void spo_get_fn(const spo_ctx *c, int m, int ot, double *out);
 /* 2x2 complex impedance block Z(m,n) with r_ref of row m
  * (lib_sie_tri_get_impedance_edges).  out = D11,D12,D21,D22 as re/im. */
 void spo_near_block(const spo_ctx *c, int m, int n, double *out);
+/* the 12x12 edge-adjacent K of side tt of m against side ll of n, medium md */
+void spo_edge_adjacent_kt(const spo_ctx *c, int m, int tt, int n, int ll, int md, double *out);
 
 /* Octree (lib_tree.f90): returns l_max. */
 int spo_tree_build(spo_ctx *c);

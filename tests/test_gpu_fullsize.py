@@ -153,14 +153,38 @@ def test_c1_seven_point_rule(c1):
 
 
 def test_c1_matvec_and_solution(c1):
+    """The sphere's PMCHWT system under unpreconditioned GMRES stalls near a
+    backward error of 4e-4 (the oracle takes 650 and 2000 steps without
+    reaching 5e-4), so the two solvers are compared at 3e-3, which both
+    reach in about 53 steps.  At 0.25 lambda leaf boxes the K = 19 far-field
+    sum amplifies rounding (GPU and oracle matvecs agree to about 1e-6, the
+    near field to 1e-10).  GMRES on this ill-conditioned system amplifies
+    that 1e-6 operator difference: the backward-error histories differ by up
+    to 4.6e-2 at single steps (measured), so they are compared to 0.1, and
+    the GPU solution is judged by its residual under the oracle's operator
+    (2.980e-3 against the oracle solution's 2.997e-3, measured)."""
     _, _, o, _, g = c1
     rng = np.random.default_rng(7)
     x = rng.standard_normal(2 * o.N) + 1j * rng.standard_normal(2 * o.N)
-    assert rel(g.calculate_vector_b_tri(x), o.matvec(x, NT)) < 1e-5
+    em = rel(g.calculate_vector_b_tri(x), o.matvec(x, NT))
+    print(f"C1 ng=7 matvec rel err vs oracle {em:.2e}")
+    assert em < 1e-5
     k, e = plane_wave(0.0, 0.0, 2, "TM")
     b = g.get_vector_b_tri(k, e)
-    xg, ig = g.gmres_run(b, tolerance=1e-6, max_iterations=650)
-    xo, io = o.gmres(o.rhs_plane(k, e), tol=1e-6, maxit=650, nthreads=NT)
+    bo = o.rhs_plane(k, e)
+    xg, ig = g.gmres_run(b, tolerance=3e-3, max_iterations=200)
+    xo, io = o.gmres(bo, tol=3e-3, maxit=200, nthreads=NT)
     assert ig["info"] == io["info"] == 0
-    assert abs(ig["iters"] - io["iters"]) <= 2
-    assert rel(xg, xo) < 1e-4
+    assert abs(ig["iters"] - io["iters"]) <= 2, (ig["iters"], io["iters"])
+    n = min(len(ig["hist"]), len(io["hist"]))
+    hg, ho = np.asarray(ig["hist"][:n]), np.asarray(io["hist"][:n])
+    eh = np.max(np.abs(hg - ho) / ho)
+    rg = rel(o.matvec(xg, NT), bo)
+    ro = rel(o.matvec(xo, NT), bo)
+    ex = rel(xg, xo)
+    print(f"C1 ng=7 GMRES {ig['iters']}/{io['iters']} steps, history rel diff {eh:.2e} "
+          f"(steps 1-10: {np.max(np.abs(hg[:10] - ho[:10]) / ho[:10]):.2e}), "
+          f"solution {ex:.2e}, residual under the oracle operator {rg:.3e} (oracle's {ro:.3e})")
+    assert eh < 0.1
+    assert rg < 1.05 * ro
+    assert ex < 1e-3

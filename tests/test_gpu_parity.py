@@ -172,6 +172,21 @@ def _patch24(form, s_opt):
     return o, g, g.get_vector_b_tri(k, e), o.rhs_plane(k, e)
 
 
+@pytest.mark.parametrize("form", FORMS)
+def test_all_formulations_matvec(form):
+    """The whole matvec of every formulation: its far part carries the
+    formulation's scaling of the disaggregated sums
+    (src/lib_sie_tri_mlfmm.f90:478-495)."""
+    o, g, _, _ = _patch24(form, 250)
+    rng = np.random.default_rng(11)
+    x = rng.standard_normal(2 * o.N) + 1j * rng.standard_normal(2 * o.N)
+    yo = o.matvec(x, 16)
+    yg = g.calculate_vector_b_tri(x)
+    N = o.N
+    assert rel(yg[:N], yo[:N]) < 1e-10, form
+    assert rel(yg[N:], yo[N:]) < 1e-10, form
+
+
 def test_gmres_converged_solution():
     """ICTF on the 2.4 lambda patch (N = 1680, one far-field level, 0.58
     lambda leaves): GMRES converges to 1e-8 in ~360 iterations on both."""
