@@ -69,6 +69,8 @@ def lib():
         L.spq_elements.argtypes = [C.c_void_p, ip]
         L.spq_edges.argtypes = [C.c_void_p, ip]
         L.spq_edge_element.argtypes = [C.c_void_p, C.c_int, C.c_int, dp]
+        L.spq_self_point.argtypes = [C.c_void_p, dp, C.c_int, C.c_double, C.c_double, C.c_int,
+                                     dp]
         L.spq_assemble.argtypes = [C.c_void_p, C.c_int, dp, C.c_int]
         L.spq_geometry_probe.argtypes = [C.c_double, C.c_double, dp, dp]
         L.spq_regular_pair.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, dp]
@@ -337,6 +339,20 @@ class QuadOracle:
         o = np.zeros(60)
         lib().spq_edge_element(self.h, int(s), int(t), _d(o))
         return (o[0::2] + 1j * o[1::2]).reshape(3, 10)
+
+    def element_nodes(self, e):
+        """The 8 nodes (8, 3) of element e (0-based), in the element's order."""
+        return self.nodes()[self.elements()[e] - 1]
+
+    def self_point(self, El, pI, mu, nu, ngp=15):
+        """Integration_KABCD_Taylor_ST_4FEdge at (mu, nu) of element El (8, 3)
+        for test function pI (1-based): (raw (5, 10): KA, KBE, KBH with their
+        log terms, KCE, KCH; combined (3, 10))."""
+        El = np.ascontiguousarray(El, dtype=np.float64).reshape(24)
+        o = np.zeros(160)
+        lib().spq_self_point(self.h, _d(El), int(pI), float(mu), float(nu), int(ngp), _d(o))
+        z = o[0::2] + 1j * o[1::2]
+        return z[:50].reshape(5, 10), z[50:].reshape(3, 10)
 
     def regular_pair(self, s, t, ngp):
         o = np.zeros(60)

@@ -163,6 +163,9 @@ void spq_assemble(const spq_ctx *c, int formulation, double *MM, int nthreads);
 /* probes: element geometry at (k, e) (68 doubles, see spk_quad.c) and the
  * regular rule of any order */
 void spq_geometry_probe(double k, double e, const double *El24, double *out);
+/* the self-term inner integral at (mu, nu) of element El24 for test function pI */
+void spq_self_point(const spq_ctx *c, const double *El24, int pI, double mu, double nu, int ngp,
+                    double *out);
 void spq_regular_pair(const spq_ctx *c, int s, int t, int ngp, double *out);
 /* plane-wave right-hand side b[2M] (complex) for PMCHWT (1), MCTF (2), ICTF (3) */
 void spq_rhs_plane(const spq_ctx *c, int formulation, const double *kin, const double *ein,

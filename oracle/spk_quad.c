@@ -478,8 +478,8 @@ static void int_normal(const spq_ctx *c, const double Ea[8][3], const double Eq[
 /* ------------------------------------------------------------------------ */
 /* self term: Integration_KABCD_Taylor_ST_4FEdge                              */
 /* ------------------------------------------------------------------------ */
-static void taylor_st(const spq_ctx *c, int pI, double mu, double nu, const double El[8][3],
-                      int ngp, zc rv[3][10]) {
+static void taylor_st_raw(const spq_ctx *c, int pI, double mu, double nu, const double El[8][3],
+                          int ngp, zc rv[3][10], zc raw[5][10]) {
   double x1[64], w1[64], x2[64], w2[64];
   spo_gauss_legendre(ngp, x1, w1);
   spo_gauss_legendre(5, x2, w2);
@@ -556,6 +556,19 @@ static void taylor_st(const spq_ctx *c, int pI, double mu, double nu, const doub
     rv[1][n] = (KBEa[n] + KBEb[n]) / iwe - (-iwm * KCE[n]);
     rv[2][n] = (KBHa[n] + KBHb[n]) / (-iwm) - iwe * KCH[n];
   }
+  if (raw)
+    for (int n = 0; n < 10; ++n) {
+      raw[0][n] = KAa[n] + KAb[n];
+      raw[1][n] = KBEa[n] + KBEb[n];
+      raw[2][n] = KBHa[n] + KBHb[n];
+      raw[3][n] = KCE[n];
+      raw[4][n] = KCH[n];
+    }
+}
+
+static void taylor_st(const spq_ctx *c, int pI, double mu, double nu, const double El[8][3],
+                      int ngp, zc rv[3][10]) {
+  taylor_st_raw(c, pI, mu, nu, El, ngp, rv, NULL);
 }
 
 static void int_self(const spq_ctx *c, const double Ea[8][3], int pI, int ngp, zc sum[3][10]) {
@@ -867,6 +880,23 @@ void spq_geometry_probe(double k, double e, const double *El24, double *out) {
   memcpy(out + 25, d.nq, 3 * sizeof(double));
   memcpy(out + 28, d.N, 30 * sizeof(double));
   memcpy(out + 58, d.nrot, 10 * sizeof(double));
+}
+
+/* The self-term inner integral at one observation point (mu, nu) of element
+ * El[24], test function pI (1-based), ngp angular points per sector: the five
+ * singularity-subtracted sums (KA, KB-E, KB-H with their log terms, KC-E,
+ * KC-H), then the combined (A, DE, DH) rows; out = 80 complex. */
+void spq_self_point(const spq_ctx *c, const double *El24, int pI, double mu, double nu, int ngp,
+                    double *out) {
+  double El[8][3];
+  memcpy(El, El24, sizeof(El));
+  zc rv[3][10], raw[5][10];
+  taylor_st_raw(c, pI, mu, nu, El, ngp, rv, raw);
+  zc *o = (zc *)out;
+  for (int k = 0; k < 5; ++k)
+    for (int n = 0; n < 10; ++n) o[k * 10 + n] = raw[k][n];
+  for (int k = 0; k < 3; ++k)
+    for (int n = 0; n < 10; ++n) o[50 + k * 10 + n] = rv[k][n];
 }
 
 /* Plane-wave right-hand side (get_vector_b_quad with incident_int,
