@@ -146,6 +146,8 @@ struct spk_handle {
   DBuf<int64_t> d_near_off;
   DBuf<zd> d_blocks;
   DBuf<int> d_row_box, d_row_r0, d_disagg_task;
+  DBuf<int> d_mx_box, d_mx_r0;  // 64-row groups of a box
+  int mx_groups = 0;
   int disagg_tasks = 0;
   // batched matvec: disaggregation tiles (box, first position), buffers
   DBuf<int> d_tile_box, d_tile_p0;
@@ -538,7 +540,8 @@ int matvec_batch_t(spk_handle *h, int R, const T *const *xs, T *const *ys, bool 
   if (!far_on) {
     if constexpr (std::is_same<T, zf>::value) {
       if (f8_scale && R > 8) {
-        launch_batch_near_f8(h->spmv_wgs, N, R, h->d_row_box.p, h->d_row_r0.p, h->d_box_first.p,
+        launch_batch_near_f8(h->spmv_wgs, N, R, h->d_row_box.p, h->d_row_r0.p, h->mx_groups,
+                             h->d_mx_box.p, h->d_mx_r0.p, h->d_box_first.p,
                              h->d_box_count.p, h->d_e2_off.p, h->d_e2_list.p, h->d_near_off.p,
                              h->d_perm.p, blocks_bf, f8_scale, (const zf *)BB.XI.p, BB.XB.p, ys,
                              st);
@@ -716,6 +719,16 @@ int spk_setup_tri(spk_handle *h, int np, const double *pp, int nt, const int *tt
   h->spmv_wgs = (int)rb.size();
   HIPCHK(h->d_row_box.upload(rb));
   HIPCHK(h->d_row_r0.upload(r0));
+  // groups of four 16-row tiles of one box (the batched fp8 product)
+  std::vector<int> gb, g0;
+  for (size_t b = 0; b < t.box_count.size(); ++b)
+    for (int r = 0; r < t.box_count[b]; r += 64) {
+      gb.push_back((int)b);
+      g0.push_back(r);
+    }
+  h->mx_groups = (int)gb.size();
+  HIPCHK(h->d_mx_box.upload(gb));
+  HIPCHK(h->d_mx_r0.upload(g0));
   // disaggregation tasks: up to kDisaggEdges consecutive positions of one box
   if (h->N >= (1 << 28)) return fail(SPK_ERR_ARG, "more than 2^28 RWG edges");
   std::vector<int> dt;

@@ -16,6 +16,8 @@
 #include <algorithm>
 #include <type_traits>
 
+#include <cstdlib>
+
 #include "spk_device.h"
 
 namespace spk {
@@ -1340,6 +1342,105 @@ __global__ void __launch_bounds__(256) kb_near_mxf8(int N, int R, const int *row
   }
 }
 
+// The same product with a box's row tiles shared out among the waves: a
+// workgroup covers 64 rows of one box (wave w: rows r0 + 16 w .. + 16), the
+// x of each column chunk is staged once for all four tiles and every wave
+// runs over all of the box's columns.  A quarter of the x gathers of one
+// 16-row tile per workgroup, the same registers per wave, and no sums across
+// waves (each wave owns whole rows).  Waves without rows (the box's last
+// group) take part in the staging only.
+__global__ void __launch_bounds__(256) kb_near_mxf8g(int N, int R, const int *grp_box,
+                                                     const int *grp_r0, const int *box_first,
+                                                     const int *box_count, const int *e2_off,
+                                                     const int *e2_list, const int64_t *near_off,
+                                                     const int *perm, const uint2 *blocks,
+                                                     const float4 *scale, const uint32_t *XB,
+                                                     zf *const *ys) {
+  constexpr int PER = kMxCols * 16 / 256;
+  __shared__ uint4 xsl[kMxCols * 16];
+  const int b = grp_box[blockIdx.x];
+  const int pf = box_first[b], nrow = box_count[b];
+  const int c0 = e2_off[b], cols = e2_off[b + 1] - c0;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int m = lane & 15, g = lane >> 4;
+  const int rw = grp_r0[blockIdx.x] + 16 * wv;
+  const bool active = rw < nrow;
+  const uint2 *Zr = blocks + near_off[b] + (int64_t)min(rw + m, nrow - 1) * cols;
+  // acc[c][0] real, acc[c][1] imaginary part of component c's sum
+  mxf4 acc[4][2];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) acc[c][0] = acc[c][1] = mxf4{0.f, 0.f, 0.f, 0.f};
+  // a lane's blocks of one 16-column step: columns cg + 8 jj + 2g + u
+  uint2 cur[4];
+  auto load = [&](int cg, uint2 *z) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) z[j] = Zr[min(cg + 8 * (j >> 1) + 2 * g + (j & 1), cols - 1)];
+  };
+  if (active && cols > 0) load(0, cur);
+  for (int cc = 0; cc < cols; cc += kMxCols) {
+    const int nc = min(kMxCols, cols - cc);
+    __syncthreads();
+    {
+      const int n = threadIdx.x & 15;
+      int e[PER];
+#pragma unroll
+      for (int k = 0; k < PER; ++k)
+        e[k] = e2_list[c0 + cc + min((int)(threadIdx.x + 256 * k) >> 4, nc - 1)];
+#pragma unroll
+      for (int k = 0; k < PER; ++k) {
+        const uint32_t xe = XB[((size_t)e[k] * 2 + 0) * 16 + n], xh = XB[((size_t)e[k] * 2 + 1) * 16 + n];
+        const bool ok = ((int)(threadIdx.x + 256 * k) >> 4) < nc;
+        const uint4 v = {xe ^ 0x80000000u, xh ^ 0x80000000u, (xe >> 16) | (xe << 16),
+                         (xh >> 16) | (xh << 16)};
+        xsl[threadIdx.x + 256 * k] = ok ? v : uint4{0u, 0u, 0u, 0u};
+      }
+    }
+    __syncthreads();
+    if (active) {
+      for (int off = 0; off < nc; off += 16) {
+        uint2 nx[4];
+        load(cc + off + 16, nx);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int jj = 0; jj < 2; ++jj) {
+          const int col = off + 8 * jj + 2 * g;
+          const uint4 x0 = xsl[col * 16 + m], x1 = xsl[(col + 1) * 16 + m];
+          const mxs4 ber = as_s4(x0.x, x1.x), bhr = as_s4(x0.y, x1.y);
+          const mxs4 bei = as_s4(x0.z, x1.z), bhi = as_s4(x0.w, x1.w);
+          const uint2 z0 = cur[2 * jj], z1 = cur[2 * jj + 1];
+          const mxs4 a0 = fp8_comp_pair<false>(z0.x, z1.x), a1 = fp8_comp_pair<true>(z0.x, z1.x);
+          const mxs4 a2 = fp8_comp_pair<false>(z0.y, z1.y), a3 = fp8_comp_pair<true>(z0.y, z1.y);
+          acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a0, ber, acc[0][0], 0, 0, 0);
+          acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a0, bei, acc[0][1], 0, 0, 0);
+          acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1, bhr, acc[1][0], 0, 0, 0);
+          acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1, bhi, acc[1][1], 0, 0, 0);
+          acc[2][0] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2, ber, acc[2][0], 0, 0, 0);
+          acc[2][1] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2, bei, acc[2][1], 0, 0, 0);
+          acc[3][0] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a3, bhr, acc[3][0], 0, 0, 0);
+          acc[3][1] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a3, bhi, acc[3][1], 0, 0, 0);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) cur[j] = nx[j];
+      }
+    }
+  }
+  if (!active) return;
+  // element i of this lane is output row rw + 4g + i of vector m
+  const int n = m;
+  if (n >= R) return;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int r = rw + 4 * g + i;
+    if (r < nrow) {
+      const float4 sc = scale[pf + r];
+      const int mm = perm[pf + r];
+      ys[n][mm] = zf{sc.x * acc[0][0][i] + sc.y * acc[1][0][i], sc.x * acc[0][1][i] + sc.y * acc[1][1][i]};
+      ys[n][mm + N] = zf{sc.z * acc[2][0][i] + sc.w * acc[3][0][i], sc.z * acc[2][1][i] + sc.w * acc[3][1][i]};
+    }
+  }
+}
+
 // the preconditioner's product for 9..16 right-hand sides on the fp8 copy
 // (blocks, scale as k_near_spmv_f8 reads them); workgroups = row tiles of
 // 16 rows.  Two tiles per workgroup (x staged once for 32 rows, 156 VGPRs)
@@ -1351,6 +1452,7 @@ __global__ void __launch_bounds__(256) kb_x_to_bf(int64_t len, const zf *XI, uin
   if (i < len) XB[i] = bf16_pair(XI[i].x, XI[i].y);
 }
 void launch_batch_near_f8(int nwg, int N, int R, const int *row_box, const int *row_r0,
+                          int ngrp, const int *grp_box, const int *grp_r0,
                           const int *box_first, const int *box_count, const int *e2_off,
                           const int *e2_list, const int64_t *near_off, const int *perm,
                           const uint32_t *blocks, const float *scale, const zf *XI,
@@ -1358,9 +1460,20 @@ void launch_batch_near_f8(int nwg, int N, int R, const int *row_box, const int *
   if (nwg <= 0) return;
   const int64_t len = (int64_t)N * 2 * 16;
   hipLaunchKernelGGL(kb_x_to_bf, dim3(bnblk(len, 256)), dim3(256), 0, st, len, XI, XB);
-  hipLaunchKernelGGL(kb_near_mxf8<1>, dim3(nwg), dim3(256), 0, st, N, R, row_box, row_r0,
-                     box_first, box_count, e2_off, e2_list, near_off, perm, (const uint2 *)blocks,
-                     (const float4 *)scale, (const uint32_t *)XB, ys);
+  // SPK_NEAR_MXF8_TILE=1: one 16-row tile per workgroup (the waves split the
+  // columns) instead of four tiles sharing the staged x
+  static const bool per_tile = [] {
+    const char *v = std::getenv("SPK_NEAR_MXF8_TILE");
+    return v && v[0] == '1';
+  }();
+  if (per_tile || ngrp <= 0)
+    hipLaunchKernelGGL(kb_near_mxf8<1>, dim3(nwg), dim3(256), 0, st, N, R, row_box, row_r0,
+                       box_first, box_count, e2_off, e2_list, near_off, perm, (const uint2 *)blocks,
+                       (const float4 *)scale, (const uint32_t *)XB, ys);
+  else
+    hipLaunchKernelGGL(kb_near_mxf8g, dim3(ngrp), dim3(256), 0, st, N, R, grp_box, grp_r0,
+                       box_first, box_count, e2_off, e2_list, near_off, perm, (const uint2 *)blocks,
+                       (const float4 *)scale, (const uint32_t *)XB, ys);
 }
 
 // padded batch width: R rounded up to a power of two
