@@ -248,6 +248,17 @@ int spk_scattered_field(spk_handle *h, int npts, const double *r, const double *
 /* Same with x in device memory. */
 int spk_scattered_field_device(spk_handle *h, int npts, const double *r, const void *d_x,
                                double *E, double *H);
+/* spk_scattered_field_device without allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The points and the E/H staging live in a library-owned device array of
+ * 1.9 MB per GPU, held until the process ends; npts above
+ * SPK_SCATTERED_NOALLOC_MAX_POINTS is rejected with SPK_ERR_ARG.  Every call
+ * on one GPU shares that array: calls that overlap on one GPU (from several
+ * host threads or handles) race.  Results equal spk_scattered_field_device's. */
+#define SPK_SCATTERED_NOALLOC_MAX_POINTS 16384
+int spk_scattered_field_device_noalloc(spk_handle *h, int npts, const double *r,
+                                       const void *d_x, double *E, double *H);
 
 /* lib_math_solver_gmres_run (src/lib/libmath/src/math/lib_math_solver_GMRES.f90:159-379)
  * with the GPU matvec; x holds the initial guess on entry. */

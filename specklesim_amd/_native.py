@@ -19,6 +19,7 @@ EXPORTS = [
     "spk_num_edges", "spk_matvec", "spk_matvec_device", "spk_matvec_c64",
     "spk_matvec_c64_device", "spk_matvec_parts", "spk_rhs_plane_wave",
     "spk_rhs_plane_wave_device", "spk_scattered_field", "spk_scattered_field_device",
+    "spk_scattered_field_device_noalloc",
     "spk_quad_create", "spk_quad_destroy", "spk_quad_setup", "spk_quad_sizes",
     "spk_quad_get_mesh", "spk_quad_get_edges", "spk_quad_edge_element", "spk_quad_fill",
     "spk_quad_fill_device", "spk_quad_topology_probe", "spk_quad_rhs_plane_wave",
@@ -160,6 +161,7 @@ def lib():
     L.spk_device_copy.argtypes = [vp, vp, C.c_size_t, C.c_int]
     L.spk_device_synchronize.argtypes = [C.c_int]
     L.spk_scattered_field_device.argtypes = [vp, C.c_int, dp, vp, dp, dp]
+    L.spk_scattered_field_device_noalloc.argtypes = [vp, C.c_int, dp, vp, dp, dp]
     L.spk_gmres.argtypes = [vp, dp, dp, C.POINTER(GmresParams), C.POINTER(GmresInfo)]
     L.spk_gmres_device.argtypes = [vp, vp, vp, C.POINTER(GmresParams), C.POINTER(GmresInfo)]
     L.spk_gmres_history.argtypes = [vp, dp, C.c_int, ip]

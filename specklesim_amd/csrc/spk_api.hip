@@ -11,6 +11,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <future>
 #include <thread>
@@ -645,6 +646,8 @@ extern "C" {
 
 const char *spk_last_error(void) { return spk::g_err.c_str(); }
 
+static int scratch_init(int device);
+
 int spk_create(const spk_params *params, spk_handle **out) {
   if (!params || !out) return fail(SPK_ERR_ARG, "null argument");
   if (params->lambda <= 0) return fail(SPK_ERR_ARG, "lambda must be positive");
@@ -660,6 +663,7 @@ int spk_create(const spk_params *params, spk_handle **out) {
   if (params->device < 0 || params->device >= ndev)
     return fail(SPK_ERR_ARG, "device ordinal out of range");
   HIPCHK(hipSetDevice(params->device));
+  if (int rc = scratch_init(params->device)) return rc;
   std::unique_ptr<spk_handle> h(new spk_handle());
   h->p = *params;
   if (h->p.tree_s_opt <= 0) h->p.tree_s_opt = 250;
@@ -997,8 +1001,7 @@ int spk_rhs_plane_wave(spk_handle *h, const double k_in[3], const double e_in[3]
   return SPK_OK;
 }
 
-static int scattered_dev(spk_handle *h, int npts, const double *r, const zd *d_x, double *E,
-                         double *H) {
+static ScatArgs scat_args(const spk_handle *h, int npts) {
   ScatArgs a{};
   a.N = h->N;
   a.npts = npts;
@@ -1016,17 +1019,65 @@ static int scattered_dev(spk_handle *h, int npts, const double *r, const zd *d_x
   a.iwmy = zd{iwmy.real(), iwmy.imag()};
   a.iweps = zd{iweps.real(), iweps.imag()};
   a.inv_iwmy = zd{inv_iwmy.real(), inv_iwmy.imag()};
+  return a;
+}
+
+// One field evaluation through device staging: the points dr[3*npts] are
+// copied in, one workgroup per point writes every component of dE, dH
+// [3*npts], which are copied out.
+static int scattered_staged(spk_handle *h, int npts, const double *r, const zd *d_x, double *E,
+                            double *H, double *dr, zd *dE, zd *dH) {
+  const ScatArgs a = scat_args(h, npts);
+  HIPCHK(hipMemcpyAsync(dr, r, 3 * (size_t)npts * sizeof(double), hipMemcpyHostToDevice, h->st));
+  launch_scattered(a, h->d_fn.p, d_x, dr, dE, dH, h->st);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(E, dE, 3 * (size_t)npts * sizeof(zd), hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipMemcpyAsync(H, dH, 3 * (size_t)npts * sizeof(zd), hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  return SPK_OK;
+}
+
+static int scattered_dev(spk_handle *h, int npts, const double *r, const zd *d_x, double *E,
+                         double *H) {
   DBuf<double> dr;
   DBuf<zd> dE, dH;
   HIPCHK(dr.alloc(3 * (size_t)npts));
   HIPCHK(dE.alloc(3 * (size_t)npts));
   HIPCHK(dH.alloc(3 * (size_t)npts));
-  HIPCHK(hipMemcpyAsync(dr.p, r, 3 * (size_t)npts * sizeof(double), hipMemcpyHostToDevice, h->st));
-  launch_scattered(a, h->d_fn.p, d_x, dr.p, dE.p, dH.p, h->st);
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(E, dE.p, 3 * (size_t)npts * sizeof(zd), hipMemcpyDeviceToHost, h->st));
-  HIPCHK(hipMemcpyAsync(H, dH.p, 3 * (size_t)npts * sizeof(zd), hipMemcpyDeviceToHost, h->st));
-  HIPCHK(hipStreamSynchronize(h->st));
+  return scattered_staged(h, npts, r, d_x, E, H, dr.p, dE.p, dH.p);
+}
+
+// ---------------------------------------------------------------------------
+// Staging for the scattered field without allocation: a file-scope device
+// array, one copy per GPU, held until the process ends.  Host code never
+// passes the array's name to a kernel or a copy: spk_create takes its
+// device address once per device (hipGetSymbolAddress, with the device
+// current) and the entry point uses that address.  Layout: points
+// [3 * kScatScratchPoints] double, then E and H [3 * kScatScratchPoints]
+// complex128 each (every offset a multiple of 256 bytes).
+// ---------------------------------------------------------------------------
+constexpr int kScatScratchPoints = 16384;
+constexpr size_t kScatOffE = 3 * (size_t)kScatScratchPoints * sizeof(double);
+constexpr size_t kScatOffH = kScatOffE + 3 * (size_t)kScatScratchPoints * sizeof(zd);
+constexpr size_t kScatScratchBytes = kScatOffH + 3 * (size_t)kScatScratchPoints * sizeof(zd);
+static_assert(kScatOffE % 256 == 0 && kScatOffH % 256 == 0, "scratch offsets");
+static __device__ __align__(256) unsigned char scratch[kScatScratchBytes];
+
+constexpr int kScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[kScratchDevices];
+
+// The device address of `scratch` on `device`, which must be current.
+static int scratch_init(int device) {
+  if (device < 0 || device >= kScratchDevices)
+    return fail(SPK_ERR_ARG, "device ordinal beyond the staging table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return SPK_OK;
+  void *ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    return fail(SPK_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch_ptr[device] = ptr;
   return SPK_OK;
 }
 
@@ -1036,6 +1087,25 @@ int spk_scattered_field_device(spk_handle *h, int npts, const double *r, const v
   if (!h->ready || h->N <= 0 || !h->d_fn.p) return fail(SPK_ERR_STATE, "spk_setup_tri has not completed");
   if (npts == 0) return SPK_OK;
   return scattered_dev(h, npts, r, (const zd *)d_x, E, H);
+}
+
+int spk_scattered_field_device_noalloc(spk_handle *h, int npts, const double *r,
+                                       const void *d_x, double *E, double *H) {
+  if (!h || !r || !d_x || !E || !H || npts < 0) return fail(SPK_ERR_ARG, "bad argument");
+  if (!h->ready || h->N <= 0 || !h->d_fn.p) return fail(SPK_ERR_STATE, "spk_setup_tri has not completed");
+  if (npts > kScatScratchPoints)
+    return fail(SPK_ERR_ARG, "npts exceeds the no-allocation maximum of " +
+                                 std::to_string(kScatScratchPoints) + " points");
+  if (npts == 0) return SPK_OK;
+  HIPCHK(hipSetDevice(h->p.device));
+  unsigned char *base = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    base = (unsigned char *)g_scratch_ptr[h->p.device];
+  }
+  if (!base) return fail(SPK_ERR_STATE, "no staging address for this device");
+  return scattered_staged(h, npts, r, (const zd *)d_x, E, H, (double *)base,
+                          (zd *)(base + kScatOffE), (zd *)(base + kScatOffH));
 }
 
 int spk_scattered_field(spk_handle *h, int npts, const double *r, const double *x, double *E,
