@@ -147,7 +147,7 @@ struct spk_handle {
   DBuf<int64_t> d_near_off;
   DBuf<zd> d_blocks;
   DBuf<int> d_row_box, d_row_r0, d_disagg_task;
-  DBuf<int> d_mx_box, d_mx_r0;  // 64-row groups of a box
+  DBuf<int> d_mx_box, d_mx_r0;  // groups of row tiles of a box (batched fp8 product)
   int mx_groups = 0;
   int disagg_tasks = 0;
   // batched matvec: disaggregation tiles (box, first position), buffers
@@ -723,10 +723,12 @@ int spk_setup_tri(spk_handle *h, int np, const double *pp, int nt, const int *tt
   h->spmv_wgs = (int)rb.size();
   HIPCHK(h->d_row_box.upload(rb));
   HIPCHK(h->d_row_r0.upload(r0));
-  // groups of four 16-row tiles of one box (the batched fp8 product)
+  // groups of near_mx_tiles_per_group() 16-row tiles of one box (the
+  // batched fp8 product)
+  const int grp_rows = 16 * near_mx_tiles_per_group();
   std::vector<int> gb, g0;
   for (size_t b = 0; b < t.box_count.size(); ++b)
-    for (int r = 0; r < t.box_count[b]; r += 64) {
+    for (int r = 0; r < t.box_count[b]; r += grp_rows) {
       gb.push_back((int)b);
       g0.push_back(r);
     }

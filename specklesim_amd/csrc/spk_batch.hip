@@ -1342,13 +1342,14 @@ __global__ void __launch_bounds__(256) kb_near_mxf8(int N, int R, const int *row
   }
 }
 
-// The same product with a box's row tiles shared out among the waves: a
-// workgroup covers 64 rows of one box (wave w: rows r0 + 16 w .. + 16), the
-// x of each column chunk is staged once for all four tiles and every wave
-// runs over all of the box's columns.  A quarter of the x gathers of one
-// 16-row tile per workgroup, the same registers per wave, and no sums across
-// waves (each wave owns whole rows).  Waves without rows (the box's last
-// group) take part in the staging only.
+// The same product with TPW row tiles of a box per workgroup sharing one x
+// staging: wave w works on tile w / CS over the columns c = 16 (w % CS) +
+// 16 CS k of each chunk (CS = 4 / TPW column splits), and the CS waves of a
+// tile add their sums through LDS at the end.  TPW = 1 is one 16-row tile
+// per workgroup; TPW = 4 stages x once for 64 rows but leaves a quarter of
+// the workgroups.  Waves without rows (a box's last group) take part in the
+// staging only.
+template <int TPW>
 __global__ void __launch_bounds__(256) kb_near_mxf8g(int N, int R, const int *grp_box,
                                                      const int *grp_r0, const int *box_first,
                                                      const int *box_count, const int *e2_off,
@@ -1356,14 +1357,16 @@ __global__ void __launch_bounds__(256) kb_near_mxf8g(int N, int R, const int *gr
                                                      const int *perm, const uint2 *blocks,
                                                      const float4 *scale, const uint32_t *XB,
                                                      zf *const *ys) {
+  constexpr int CS = 4 / TPW;
   constexpr int PER = kMxCols * 16 / 256;
   __shared__ uint4 xsl[kMxCols * 16];
   const int b = grp_box[blockIdx.x];
   const int pf = box_first[b], nrow = box_count[b];
   const int c0 = e2_off[b], cols = e2_off[b + 1] - c0;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int tile = wv / CS, split = wv % CS;
   const int m = lane & 15, g = lane >> 4;
-  const int rw = grp_r0[blockIdx.x] + 16 * wv;
+  const int rw = grp_r0[blockIdx.x] + 16 * tile;
   const bool active = rw < nrow;
   const uint2 *Zr = blocks + near_off[b] + (int64_t)min(rw + m, nrow - 1) * cols;
   // acc[c][0] real, acc[c][1] imaginary part of component c's sum
@@ -1376,7 +1379,7 @@ __global__ void __launch_bounds__(256) kb_near_mxf8g(int N, int R, const int *gr
 #pragma unroll
     for (int j = 0; j < 4; ++j) z[j] = Zr[min(cg + 8 * (j >> 1) + 2 * g + (j & 1), cols - 1)];
   };
-  if (active && cols > 0) load(0, cur);
+  if (active && cols > 0) load(16 * split, cur);
   for (int cc = 0; cc < cols; cc += kMxCols) {
     const int nc = min(kMxCols, cols - cc);
     __syncthreads();
@@ -1397,9 +1400,9 @@ __global__ void __launch_bounds__(256) kb_near_mxf8g(int N, int R, const int *gr
     }
     __syncthreads();
     if (active) {
-      for (int off = 0; off < nc; off += 16) {
+      for (int off = 16 * split; off < nc; off += 16 * CS) {
         uint2 nx[4];
-        load(cc + off + 16, nx);
+        load(cc + off + 16 * CS, nx);
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int jj = 0; jj < 2; ++jj) {
@@ -1425,18 +1428,49 @@ __global__ void __launch_bounds__(256) kb_near_mxf8g(int N, int R, const int *gr
       }
     }
   }
+  // element i of this lane is output row rw + 4g + i of vector m: the row's
+  // scales, then the splits' sums in a fixed order (split 0 adds 1..CS-1)
+  mxf4 er, ei, hr, hi;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int r = min(rw + 4 * g + i, nrow - 1);
+    const float4 sc = scale[pf + r];
+    er[i] = sc.x * acc[0][0][i] + sc.y * acc[1][0][i];
+    ei[i] = sc.x * acc[0][1][i] + sc.y * acc[1][1][i];
+    hr[i] = sc.z * acc[2][0][i] + sc.w * acc[3][0][i];
+    hi[i] = sc.z * acc[2][1][i] + sc.w * acc[3][1][i];
+  }
+  if constexpr (CS > 1) {
+    mxf4 *red = reinterpret_cast<mxf4 *>(xsl);
+    __syncthreads();
+    if (split > 0) {
+      mxf4 *o = red + (((size_t)tile * (CS - 1) + split - 1) * 64 + lane) * 4;
+      o[0] = er;
+      o[1] = ei;
+      o[2] = hr;
+      o[3] = hi;
+    }
+    __syncthreads();
+    if (split > 0) return;
+#pragma unroll
+    for (int p = 0; p < CS - 1; ++p) {
+      const mxf4 *o = red + (((size_t)tile * (CS - 1) + p) * 64 + lane) * 4;
+      er += o[0];
+      ei += o[1];
+      hr += o[2];
+      hi += o[3];
+    }
+  }
   if (!active) return;
-  // element i of this lane is output row rw + 4g + i of vector m
   const int n = m;
   if (n >= R) return;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const int r = rw + 4 * g + i;
     if (r < nrow) {
-      const float4 sc = scale[pf + r];
       const int mm = perm[pf + r];
-      ys[n][mm] = zf{sc.x * acc[0][0][i] + sc.y * acc[1][0][i], sc.x * acc[0][1][i] + sc.y * acc[1][1][i]};
-      ys[n][mm + N] = zf{sc.z * acc[2][0][i] + sc.w * acc[3][0][i], sc.z * acc[2][1][i] + sc.w * acc[3][1][i]};
+      ys[n][mm] = zf{er[i], ei[i]};
+      ys[n][mm + N] = zf{hr[i], hi[i]};
     }
   }
 }
@@ -1460,20 +1494,34 @@ void launch_batch_near_f8(int nwg, int N, int R, const int *row_box, const int *
   if (nwg <= 0) return;
   const int64_t len = (int64_t)N * 2 * 16;
   hipLaunchKernelGGL(kb_x_to_bf, dim3(bnblk(len, 256)), dim3(256), 0, st, len, XI, XB);
-  // SPK_NEAR_MXF8_TILE=1: one 16-row tile per workgroup (the waves split the
-  // columns) instead of four tiles sharing the staged x
-  static const bool per_tile = [] {
-    const char *v = std::getenv("SPK_NEAR_MXF8_TILE");
-    return v && v[0] == '1';
-  }();
-  if (per_tile || ngrp <= 0)
+  // tpw row tiles per workgroup (near_mx_tiles_per_group); 1 runs the
+  // original kernel on the 16-row tile map
+  const int tpw = near_mx_tiles_per_group();
+  if (tpw == 1 || ngrp <= 0)
     hipLaunchKernelGGL(kb_near_mxf8<1>, dim3(nwg), dim3(256), 0, st, N, R, row_box, row_r0,
                        box_first, box_count, e2_off, e2_list, near_off, perm, (const uint2 *)blocks,
                        (const float4 *)scale, (const uint32_t *)XB, ys);
-  else
-    hipLaunchKernelGGL(kb_near_mxf8g, dim3(ngrp), dim3(256), 0, st, N, R, grp_box, grp_r0,
+  else if (tpw == 2)
+    hipLaunchKernelGGL(kb_near_mxf8g<2>, dim3(ngrp), dim3(256), 0, st, N, R, grp_box, grp_r0,
                        box_first, box_count, e2_off, e2_list, near_off, perm, (const uint2 *)blocks,
                        (const float4 *)scale, (const uint32_t *)XB, ys);
+  else
+    hipLaunchKernelGGL(kb_near_mxf8g<4>, dim3(ngrp), dim3(256), 0, st, N, R, grp_box, grp_r0,
+                       box_first, box_count, e2_off, e2_list, near_off, perm, (const uint2 *)blocks,
+                       (const float4 *)scale, (const uint32_t *)XB, ys);
+}
+
+// Row tiles per workgroup of the batched fp8 near product: 1 by default,
+// SPK_NEAR_MXF8_TPW = 1, 2 or 4 to choose (read once per process).  On the
+// 10 x 10 lambda patch at 16 vectors: 82.9, 84.2 and 94.1 us, with HBM
+// traffic 1.75x, 1.28x and 1.2x the algorithmic bytes.
+int near_mx_tiles_per_group() {
+  static const int tpw = [] {
+    const char *v = std::getenv("SPK_NEAR_MXF8_TPW");
+    const int t = v ? std::atoi(v) : 1;
+    return (t == 2 || t == 4) ? t : 1;
+  }();
+  return tpw;
 }
 
 // padded batch width: R rounded up to a power of two

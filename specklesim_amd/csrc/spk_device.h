@@ -258,6 +258,7 @@ void launch_batch_near(int nwg, int N, int R, const int *row_box, const int *row
 // launch_batch_interleave writes it; XB: scratch of N * 32 words for x in
 // bf16) on the fp8 copy of the blocks, on the matrix cores; workgroups = the
 // row tiles (box, first row) of kSpmvRowsPerWG rows
+int near_mx_tiles_per_group();
 void launch_batch_near_f8(int nwg, int N, int R, const int *row_box, const int *row_r0,
                           int ngrp, const int *grp_box, const int *grp_r0,
                           const int *box_first, const int *box_count, const int *e2_off,
