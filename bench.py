@@ -179,6 +179,75 @@ def timed_batch(args, lam, pp, tt, R):
                 solves_per_s=R * args.steps / elapsed, ms_per_step=elapsed / args.steps * 1e3)
 
 
+def ensemble_meshes(M, lam, seed0=1):
+    """M seeded realisations of the speckle-ensemble statistics: patch side
+    L ~ U(5, 15) lambda, RMS height ~ U(0.05, 0.15) lambda, correlation length
+    ~ U(0.3, 1.0) lambda, lambda/10 cells (2 triangles each)."""
+    from specklesim_amd import meshes
+
+    rng = np.random.default_rng(12345)
+    out, sizes = [], []
+    for i in range(M):
+        L, rms, corr = rng.uniform(5, 15), rng.uniform(0.05, 0.15), rng.uniform(0.3, 1.0)
+        c = int(round(10 * L))
+        out.append(meshes.rough_surface(c, c, c / 10, c / 10, rms, corr, seed0 + i, lam))
+        sizes.append(c)
+    return out, sizes
+
+
+def timed_ensemble(args, lam):
+    """The speckle ensemble: M realisations x 16 incidence angles (0-60 deg,
+    TM), near-field preconditioned flexible GMRES on SPK_MIXED handles in
+    batches of 16, intensity at 181 far-field points of the plane of
+    incidence; worker threads with their own handles.  Then the plain FP64
+    GMRES (the reference's algorithm, one solve at a time) on every
+    realisation at normal incidence, for the FP64 comparison."""
+    from specklesim_amd import SieTriMlfmm, ensemble_intensity, plane_wave
+
+    ms, cells = ensemble_meshes(args.ensemble, lam)
+    angles = [plane_wave(th, 0.0, 2, "TM") for th in batch_angles(16)]
+    th = np.radians(np.linspace(-80, 80, 181))
+    pts = 1e3 * lam * np.stack([np.sin(th), np.zeros_like(th), np.cos(th)], axis=1)
+    kw = dict(formulation=args.formulation, tolerance=args.tol, max_iterations=args.maxit,
+              inner_iterations=args.inner, workers=args.workers)
+    ensemble_intensity(lam, args.eps_r2, ms[:2], angles, pts, **kw)  # warm-up
+    _sync()
+    t0 = time.perf_counter()
+    I, info = ensemble_intensity(lam, args.eps_r2, ms, angles, pts, **kw)
+    _sync()
+    el = time.perf_counter() - t0
+    its = [i["iters"] for row in info for i in row]
+    # plain FP64 GMRES, one realisation at a time, normal incidence
+    s64 = SieTriMlfmm(lam, args.eps_r2, formulation=args.formulation, normal_convention=2,
+                      precision="fp64")
+    k, e = plane_wave(0.0, 0.0, 2, "TM")
+    t0 = time.perf_counter()
+    it64 = []
+    for pp, tt in ms:
+        s64.precalculation_tri(pp, tt)
+        _, i64 = s64.gmres_run(s64.get_vector_b_tri(k, e), tolerance=args.tol,
+                               max_iterations=args.maxit, orthogonalization=args.orth,
+                               use_initial_guess=False)
+        it64.append(i64["iters"])
+    _sync()
+    el64 = time.perf_counter() - t0
+    n_edges = [3 * c * c - 2 * c for c in cells]  # interior edges of c x c cells, 2 triangles each
+    return {
+        "workload": f"speckle ensemble: {args.ensemble} seeded realisations (L ~ U(5,15) lambda, "
+                    f"RMS ~ U(0.05,0.15) lambda, correlation ~ U(0.3,1.0) lambda, lambda/10 "
+                    f"cells) x 16 incidence angles (0-60 deg, TM), {args.formulation}, n=1.5, "
+                    f"tol {args.tol:g}, intensity at 181 far-field points; {args.workers} worker "
+                    f"threads with their own handles (spk_ensemble_intensity)",
+        "realisations": args.ensemble, "angles": 16,
+        "unknowns": [2 * min(n_edges), 2 * max(n_edges)],
+        "solves_per_s": len(ms) * 16 / el, "ms_per_ensemble": el * 1e3,
+        "iterations": [min(its), max(its)],
+        "all_converged": all(i["info"] == 0 for row in info for i in row),
+        "fp64_plain_solves_per_s": len(ms) / el64, "fp64_plain_iterations": [min(it64), max(it64)],
+        "vs_fp64_plain": (len(ms) * 16 / el) / (len(ms) / el64),
+    }
+
+
 def brdf_curve(res, lam):
     """BRDF of a solution in the plane of incidence, 181 angles in
     (-80, 80) degrees on a sphere of 1000 wavelengths."""
@@ -259,6 +328,10 @@ def main():
                     help="incidence angles of the batched path (0: skip it)")
     ap.add_argument("--inner", type=int, default=30,
                     help="inner near-field GMRES steps per flexible GMRES step")
+    ap.add_argument("--ensemble", type=int, default=16,
+                    help="realisations of the speckle-ensemble object (0: skip it)")
+    ap.add_argument("--workers", type=int, default=3,
+                    help="worker threads (handles) of the ensemble")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1",
@@ -279,9 +352,22 @@ def main():
         # solves on the FP64 near blocks): separates the gain of the algorithm
         # from the gain of the low-precision copies
         paths["near_fgmres_fp64"] = timed_path(args, lam, pp, tt, "fp64", "ICGS", near=True)
-    batch = None
+        # the same as near_fgmres with the inner solve's kernels launched one
+        # by one instead of replaying its captured graph
+        os.environ["SPK_INNER_GRAPH"] = "0"
+        try:
+            paths["near_fgmres_eager"] = timed_path(args, lam, pp, tt, "mixed", "ICGS", near=True)
+        finally:
+            del os.environ["SPK_INNER_GRAPH"]
+    batch = batch_eager = None
     if args.batch and args.precision == "both":
         batch = timed_batch(args, lam, pp, tt, args.batch)
+        os.environ["SPK_INNER_GRAPH"] = "0"
+        try:
+            batch_eager = timed_batch(args, lam, pp, tt, args.batch)
+        finally:
+            del os.environ["SPK_INNER_GRAPH"]
+    ens = timed_ensemble(args, lam) if args.ensemble and args.precision == "both" else None
     ff_err = None
     if "fp64" in paths and "mixed" in paths:
         b64, b32 = brdf_curve(paths["fp64"], lam), brdf_curve(paths["mixed"], lam)
@@ -416,7 +502,12 @@ def main():
             "vs_fp64_single": batch["solves_per_s"] / paths["fp64"]["solves_per_s"],
             "vs_mixed_single": batch["solves_per_s"] / paths["mixed"]["solves_per_s"],
             "vs_headline_single": batch["solves_per_s"] / H["solves_per_s"],
+            "inner_graph": "captured inner near-field solve replayed per outer step",
+            "eager_ms_per_batch": batch_eager["ms_per_step"],
+            "eager_step_ms": spread(batch_eager["runs"]),
         }
+    if ens is not None:
+        rec["ensemble"] = ens
     if not args.no_cpu_baseline:
         it64 = (paths["fp64"] if "fp64" in paths else H)["runs"][-1]["iters"]
         cb = cpu_baseline(args, lam, pp, tt, it64)

@@ -170,6 +170,31 @@ int spk_quad_rhs_plane_wave_device(spk_quad *q, const double k_in[3], const doub
 int spk_quad_topology_probe(double Lx, double Ly, int nx, int ny, const double *z, int *n_edges,
                             int *n_elements, int *n_nodes, double *pp, int *vert8, int *edge_p5);
 
+/* One surface realisation: np nodes pp[3*np] (m) and nt triangles tt[3*nt]
+ * (1-based node numbers), as spk_setup_tri takes them. */
+typedef struct spk_mesh {
+  int np;
+  const double *pp;
+  int nt;
+  const int *tt;
+} spk_mesh;
+
+/* A speckle ensemble: n_mesh realisations, each solved for n_angles plane
+ * waves (k_in[3*a], e_in[3*a], as spk_rhs_plane_wave) with gp (in batches of
+ * up to 16 right-hand sides, spk_gmres_batch_device), then the scattered
+ * field's intensity sum_c |E_c|^2 at npts points r[3*npts] (m):
+ * intensity[(m * n_angles + a) * npts + p], info[m * n_angles + a].
+ * n_workers (1..16) host threads, each with its own handle created from
+ * params and its own HIP stream, take the realisations in order, so one
+ * realisation's host precalculation overlaps another's GPU work.  No
+ * counterpart in the reference, whose angle loop (main.f90:450-463) solves
+ * one realisation and one angle at a time; a failure stops the ensemble and
+ * names the realisation. */
+int spk_ensemble_intensity(const spk_params *params, int n_mesh, const spk_mesh *meshes,
+                           int n_angles, const double *k_in, const double *e_in, int npts,
+                           const double *r, const spk_gmres_params *gp, int n_workers,
+                           double *intensity, spk_gmres_info *info);
+
 /* Mixed-precision solves of R <= 16 right-hand sides of one mesh (incidence
  * angles) on an SPK_MIXED handle: each runs the refinement of spk_gmres
  * (FP64 residuals, complex64 inner GMRES with its own basis); the inner
@@ -212,6 +237,23 @@ int spk_num_edges(const spk_handle *h, int64_t *n_edges);
 int spk_matvec(spk_handle *h, const double *x, double *y);
 /* Same on device buffers (HBM-resident), stream 0 of the handle. */
 int spk_matvec_device(spk_handle *h, const void *d_x, void *d_y);
+/* The handle's operator becomes its transpose Z^T (transposed != 0) or Z
+ * again (0): every matvec, solve and low-precision copy that follows uses
+ * Z^T (the near blocks are transposed in HBM, the MLFMM far field runs its
+ * stages transposed), except the batched products of 2..16 vectors, which
+ * reject it.  For the adjoint system Z^T lambda = dJ/dI.  No counterpart in
+ * the reference.  spk_near_blocks keeps evaluating Z's blocks. */
+int spk_set_transposed(spk_handle *h, int transposed);
+int spk_get_transposed(const spk_handle *h, int *transposed);
+/* The near-field preconditioned solves capture their inner near-field solve
+ * (inner_iterations steps of near-field product and Arnoldi update) once as
+ * a HIP graph and replay it at every outer step (enable = 1, the default),
+ * or launch its kernels one by one (0).  A capture in progress makes HIP
+ * reject synchronous copies issued by other host threads, so a process that
+ * drives several handles from several threads disables it on each of them
+ * (spk_ensemble_intensity does).  The environment variable
+ * SPK_INNER_GRAPH=0 disables it for every handle. */
+int spk_set_inner_graph(spk_handle *h, int enable);
 /* The complex64 operator of an SPK_MIXED handle (the inner operator of the
  * mixed solve): y = Z x with near blocks, patterns and tables rounded to
  * complex64; vectors of 2N interleaved float pairs.  No counterpart in the
@@ -259,6 +301,23 @@ int spk_scattered_field_device(spk_handle *h, int npts, const double *r, const v
 #define SPK_SCATTERED_NOALLOC_MAX_POINTS 16384
 int spk_scattered_field_device_noalloc(spk_handle *h, int npts, const double *r,
                                        const void *d_x, double *E, double *H);
+
+/* Adjoint gradient of a far-field intensity with respect to mesh-node
+ * heights.  J = sum_p w[p] |E(r_p)|^2, E the scattered field (as
+ * spk_scattered_field) at npts points r[3*npts] of the solution of Z I = b,
+ * b the plane wave (k_in, e_in); grad[k] = dJ/dz of node nodes[k] (0-based
+ * indices into the pp of spk_setup_tri).  One solve of Z I = b and one of
+ * Z^T nu = F^T W conj(E) with gp, then for every node the central
+ * differences (node moved by +-dz along z) of the near blocks, radiation
+ * patterns, right-hand-side entries and detector fields it changes,
+ * contracted with nu, I and the leaf fields: dJ = 2 Re[E^H W dF I +
+ * nu^T (db - dZ I)].  Near blocks keep the singular / regular branch of the
+ * unmoved mesh.  J is returned too; info[2] (may be NULL): the two solves.
+ * No counterpart in the reference. */
+int spk_adjoint_height_gradient(spk_handle *h, const double k_in[3], const double e_in[3],
+                                int npts, const double *r, const double *w,
+                                const spk_gmres_params *gp, int n_nodes, const int *nodes,
+                                double dz, double *J, double *grad, spk_gmres_info *info);
 
 /* lib_math_solver_gmres_run (src/lib/libmath/src/math/lib_math_solver_GMRES.f90:159-379)
  * with the GPU matvec; x holds the initial guess on entry. */

@@ -16,7 +16,9 @@ LIB_PATH = os.path.join(HERE, "libspeckle_gpu.so")
 # Exported entry points, in include/speckle_gpu.h order.
 EXPORTS = [
     "spk_create", "spk_destroy", "spk_last_error", "spk_setup_tri", "spk_get_stats",
-    "spk_num_edges", "spk_matvec", "spk_matvec_device", "spk_matvec_c64",
+    "spk_num_edges", "spk_matvec", "spk_matvec_device", "spk_set_transposed",
+    "spk_get_transposed", "spk_set_inner_graph", "spk_adjoint_height_gradient",
+    "spk_matvec_c64",
     "spk_matvec_c64_device", "spk_matvec_parts", "spk_rhs_plane_wave",
     "spk_rhs_plane_wave_device", "spk_scattered_field", "spk_scattered_field_device",
     "spk_scattered_field_device_noalloc",
@@ -24,7 +26,7 @@ EXPORTS = [
     "spk_quad_get_mesh", "spk_quad_get_edges", "spk_quad_edge_element", "spk_quad_fill",
     "spk_quad_fill_device", "spk_quad_topology_probe", "spk_quad_rhs_plane_wave",
     "spk_quad_rhs_plane_wave_device", "spk_probe_read_bandwidth", "spk_matvec_batch_device",
-    "spk_matvec_batch_c64_device", "spk_gmres_batch_device",
+    "spk_matvec_batch_c64_device", "spk_ensemble_intensity", "spk_gmres_batch_device",
     "spk_device_alloc", "spk_device_free", "spk_device_copy", "spk_device_synchronize",
     "spk_gmres", "spk_gmres_device", "spk_gmres_history", "spk_get_rwg", "spk_get_tree",
     "spk_get_i4", "spk_get_near_csr", "spk_near_blocks", "spk_get_patterns",
@@ -58,6 +60,11 @@ class GmresInfo(C.Structure):
                 ("arnoldi_backward_error", C.c_double), ("backward_error", C.c_double),
                 ("matvecs", C.c_int), ("seconds", C.c_double),
                 ("true_backward_error", C.c_double)]
+
+
+class Mesh(C.Structure):
+    _fields_ = [("np", C.c_int), ("pp", C.POINTER(C.c_double)), ("nt", C.c_int),
+                ("tt", C.POINTER(C.c_int))]
 
 
 class Stats(C.Structure):
@@ -152,8 +159,17 @@ def lib():
     L.spk_matvec_batch_device.argtypes = [vp, C.c_int, vpp, vpp]
     L.spk_matvec_batch_c64_device.argtypes = [vp, C.c_int, vpp, vpp]
     L.spk_near_product_lowp.argtypes = [vp, C.c_int, C.c_int, C.c_int, vpp, vpp]
+    L.spk_set_transposed.argtypes = [vp, C.c_int]
+    L.spk_set_inner_graph.argtypes = [vp, C.c_int]
+    L.spk_adjoint_height_gradient.argtypes = [vp, dp, dp, C.c_int, dp, dp, C.POINTER(GmresParams),
+                                              C.c_int, ip, C.c_double, dp, dp,
+                                              C.POINTER(GmresInfo)]
+    L.spk_get_transposed.argtypes = [vp, C.POINTER(C.c_int)]
     L.spk_gmres_batch_device.argtypes = [vp, C.c_int, vpp, vpp, C.POINTER(GmresParams),
                                          C.POINTER(GmresInfo)]
+    L.spk_ensemble_intensity.argtypes = [C.POINTER(Params), C.c_int, C.POINTER(Mesh), C.c_int,
+                                         dp, dp, C.c_int, dp, C.POINTER(GmresParams), C.c_int,
+                                         dp, C.POINTER(GmresInfo)]
     L.spk_quad_topology_probe.argtypes = [C.c_double, C.c_double, C.c_int, C.c_int, dp, ip, ip, ip,
                                           dp, ip, ip]
     L.spk_device_alloc.argtypes = [C.c_int, C.c_size_t, C.POINTER(vp)]

@@ -17,7 +17,7 @@ LIB = os.path.join(HERE, "libspeckle_gpu.so")
 BUILD_DIR = os.path.join(HERE, "_build")
 
 HIP_SOURCES = ["spk_kernels.hip", "spk_api.hip", "spk_quad.hip", "spk_batch.hip"]
-CXX_SOURCES = ["spk_host.cpp", "spk_quad_host.cpp"]
+CXX_SOURCES = ["spk_host.cpp", "spk_quad_host.cpp", "spk_ensemble.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SPK_OFFLOAD_ARCH", "gfx950")

@@ -116,6 +116,7 @@ struct LevelDev {
   DBuf<zd> B, D, tlt, up, down;
   DBuf<zf> fB, fD, ftlt, fup, fdown;  // complex64 mirror (mixed precision)
   DBuf<int> child, parent, octant, i4_off, i4_box, i4_slot;
+  DBuf<int> i4_slotm;  // mirrored offset slots (342 - slot): the transposed M2L
   DBuf<double> rhat;  // TL inputs: slot directions, per-slot Hankel factors, live slots
   DBuf<zd> coef;
   DBuf<int> live;
@@ -169,6 +170,13 @@ struct spk_handle {
   DBuf<double> d_khat, d_tth, d_tph, d_q, d_leaf_centre, d_trf;
   std::vector<LevelDev> lev;
   DBuf<zd> d_x, d_y, d_yfar, d_tmp;
+  // the transposed operator (spk_set_transposed): flag, edge -> position map,
+  // scaled input of the transposed far field
+  bool transposed = false;
+  DBuf<int> d_pos;
+  std::vector<double> pp_host;  // the mesh nodes of the last setup (the adjoint moves them)
+  DBuf<zd> d_xt;
+  DBuf<zf> f_xt;
   zd sc1{1, 0}, sc2{1, 0};
   // complex64 mirror of the operator (precision == SPK_MIXED): near blocks,
   // patterns, folded W and the far-part scratch
@@ -212,6 +220,22 @@ struct spk_handle {
     DBuf<uint64_t> tin, tprod, torth;
     DBuf<int> kout;
   } fg;
+  // the inner near-field solve (inner_iterations steps of near-field product
+  // and Arnoldi update, no host synchronisation inside) captured once as a
+  // HIP graph and replayed every outer step; rebuilt when its launch
+  // configuration or any buffer address changes
+  struct InnerGraph {
+    std::vector<uint64_t> key;
+    hipGraphExec_t exec = nullptr;
+    void reset() {
+      if (exec) (void)hipGraphExecDestroy(exec);
+      exec = nullptr;
+      key.clear();
+    }
+    ~InnerGraph() { reset(); }
+  } ig;
+  uint64_t setup_serial = 0;  // spk_setup_tri calls completed or started
+  bool graph_ok = true;       // spk_set_inner_graph
   DBuf<unsigned long long> g_ctl;    // MGS sweep abort flag
   DBuf<unsigned char> g_slots;       // MGS sweep exchange slots
   unsigned long long sweep_seq = 0;  // sequence number of the last sweep launch
@@ -368,6 +392,11 @@ int setup_mlfmm(spk_handle *h, std::future<std::vector<HostLevelTables>> &pre) {
     HIPCHK(D.i4_box.upload(L.i4_box));
     HIPCHK(D.i4_slot.upload(L.i4_slot));
     {
+      std::vector<int> mir(L.i4_slot.size());
+      for (size_t q = 0; q < mir.size(); ++q) mir[q] = 342 - L.i4_slot[q];
+      HIPCHK(D.i4_slotm.upload(mir));
+    }
+    {
       // TL on the device from the host's per-slot Hankel factors
       const std::vector<double> &rhat = ht[l].rhat;
       const std::vector<cplx> &coef = ht[l].coef;
@@ -437,6 +466,8 @@ static const zd *pBlk(spk_handle *h, zd *) { return h->d_blocks.p; }
 static const zf *pBlk(spk_handle *h, zf *) { return h->f_blocks.p; }
 static zd *pYf(spk_handle *h, zd *) { return h->d_yfar.p; }
 static zf *pYf(spk_handle *h, zf *) { return h->f_yfar.p; }
+static DBuf<zd> &pXt(spk_handle *h, zd *) { return h->d_xt; }
+static DBuf<zf> &pXt(spk_handle *h, zf *) { return h->f_xt; }
 
 // y = Z x on device vectors (2N complex each), far part into yfar first.
 // T = zd: the FP64 operator; T = zf: its complex64 mirror.
@@ -453,7 +484,47 @@ int matvec_t(spk_handle *h, const T *x, T *y, bool near_on = true, bool far_on =
     if (ev) (void)hipEventRecord(ev[i], st);
   };
   mark(0);
-  if (far_on) {
+  if (far_on && h->transposed) {
+    // Z_far^T x: the stages in reverse, each transposed: the disaggregation
+    // as an aggregation of [sc1 x_e; sc2 x_h] with the patterns themselves;
+    // the fold's transpose; RR's transpose upwards (children's sums times
+    // the down phases); M2L with each I4 pair's mirrored offset; SS's
+    // transpose downwards (parents times the up phases, added to M2L's
+    // output); the aggregation's transpose as a disaggregation with the
+    // conjugated patterns (k_fold with kd = 0, my = ep = 1 reorders the
+    // sums into its layout)
+    LevelDev &LL = h->lev[lmax];
+    DBuf<T> &xt = pXt(h, tag);
+    HIPCHK(xt.alloc(2 * (size_t)N));
+    launch_scale_halves(N, h->sc1, h->sc2, x, xt.p, st);
+    launch_agg(LL.nb, S, N, h->d_box_first.p, h->d_box_count.p, h->d_perm.p, pRt(h, tag),
+               (const T *)xt.p, pB(LL, tag), st, false);
+    mark(1);
+    const zd I = {0, 1};
+    const zd kd1 = I * zcx(h->mat.k1), kd2 = I * zcx(h->mat.k2);
+    const zd my1 = zd{h->mat.my1, 0} * I * h->mat.omega, my2 = zd{h->mat.my2, 0} * I * h->mat.omega;
+    const zd ep1 = zcx(h->mat.eps1) * I * h->mat.omega, ep2 = zcx(h->mat.eps2) * I * h->mat.omega;
+    launch_fold_t(LL.nb, S, (const T *)pB(LL, tag), kd1, kd2, my1, my2, ep1, ep2, pD(LL, tag), st);
+    for (int l = lmax - 1; l >= lmin; --l)
+      launch_ss(h->lev[l].nb, S, h->lev[l].child.p, (const T *)pD(h->lev[l + 1], tag),
+                pDn(h->lev[l + 1], tag), pD(h->lev[l], tag), st);
+    mark(2);
+    for (int l = lmin; l <= lmax; ++l)
+      launch_m2l(h->lev[l].nb, S, h->lev[l].i4_off.p, h->lev[l].i4_box.p, h->lev[l].i4_slotm.p,
+                 (const T *)pD(h->lev[l], tag), pT(h->lev[l], tag), pB(h->lev[l], tag), st);
+    mark(3);
+    for (int l = lmin + 1; l <= lmax; ++l)
+      launch_rr(h->lev[l].nb, S, h->lev[l].parent.p, h->lev[l].octant.p,
+                (const T *)pB(h->lev[l - 1], tag), pUp(h->lev[l], tag), pB(h->lev[l], tag), st);
+    mark(4);
+    launch_fold(LL.nb, S, (const T *)pB(LL, tag), zd{0, 0}, zd{0, 0}, zd{1, 0}, zd{1, 0},
+                zd{1, 0}, zd{1, 0}, pW(h, tag), st);
+    mark(5);
+    launch_disagg(h->disagg_tasks, N, S, h->d_disagg_task.p, h->d_perm.p, h->d_leafbox_of_pos.p,
+                  pRt(h, tag), (const T *)pW(h, tag), zd{1, 0}, zd{1, 0}, near_on ? yfar : y, st,
+                  true);
+    mark(6);
+  } else if (far_on) {
     LevelDev &LL = h->lev[lmax];
     launch_agg(LL.nb, S, N, h->d_box_first.p, h->d_box_count.p, h->d_perm.p, pRt(h, tag), x,
                pB(LL, tag), st);
@@ -530,6 +601,8 @@ int matvec_batch_t(spk_handle *h, int R, const T *const *xs, T *const *ys, bool 
                    const uint32_t *blocks_bf = nullptr, const float *f8_scale = nullptr) {
   T *tag = nullptr;
   if (R < 1 || R > kBatchMax) return fail(SPK_ERR_ARG, "batch size must be 1..16");
+  if (far_on && h->transposed)
+    return fail(SPK_ERR_STATE, "the batched products do not take the transposed operator");
   int rc = batch_ensure<T>(h, R);
   if (rc) return rc;
   BatchBufs<T> &BB = pBB(h, tag);
@@ -686,6 +759,8 @@ int spk_setup_tri(spk_handle *h, int np, const double *pp, int nt, const int *tt
   h->ready = false;
   h->mixed_ready = false;
   h->diag_ready = false;
+  ++h->setup_serial;
+  h->ig.reset();
   // the low-precision near-field copies belong to the previous operator
   h->near_bf_ok = h->near_f8_ok = false;
   h->stats = spk_stats{};
@@ -711,6 +786,9 @@ int spk_setup_tri(spk_handle *h, int np, const double *pp, int nt, const int *tt
   HIPCHK(h->d_e2_list.upload(t.e2_list));
   HIPCHK(h->d_near_off.upload(t.near_off));
   HIPCHK(h->d_leaf_box.upload(t.leaf_box));
+  HIPCHK(h->d_pos.upload(t.pos));
+  h->pp_host.assign(pp, pp + 3 * (size_t)np);
+  h->transposed = false;
   std::vector<int> lbp(h->N);
   for (int p = 0; p < h->N; ++p) lbp[p] = t.leaf_box[t.perm[p]];
   HIPCHK(h->d_leafbox_of_pos.upload(lbp));
@@ -841,6 +919,59 @@ int spk_num_edges(const spk_handle *h, int64_t *n) {
   return SPK_OK;
 }
 
+// Switch the handle between Z and its transpose: the near blocks are
+// transposed in HBM (through a temporary copy), the far field runs its
+// transposed stages.  The low-precision near-field copies of the
+// preconditioner are rebuilt from the transposed blocks on their next use.
+int spk_set_transposed(spk_handle *h, int transposed) {
+  if (!h) return fail(SPK_ERR_ARG, "null argument");
+  if (!h->ready) return fail(SPK_ERR_STATE, "spk_setup_tri has not completed");
+  const bool want = transposed != 0;
+  if (want == h->transposed) return SPK_OK;
+  HIPCHK(hipSetDevice(h->p.device));
+  const Tree &t = h->tree;
+  const int nbl = (int)t.box_count.size();
+  int max_rows = 0;
+  for (int c : t.box_count) max_rows = std::max(max_rows, c);
+  {
+    DBuf<zd> tmp;
+    HIPCHK(tmp.alloc(h->d_blocks.n));
+    launch_near_transpose(nbl, max_rows, h->d_box_first.p, h->d_box_count.p, h->d_e2_off.p,
+                          h->d_e2_list.p, h->d_near_off.p, h->d_perm.p, h->d_pos.p,
+                          h->d_leaf_box.p, (const zd *)h->d_blocks.p, tmp.p, h->st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->st));
+    std::swap(h->d_blocks, tmp);
+  }
+  if (h->mixed_ready && h->f_blocks.p) {
+    DBuf<zf> tmp;
+    HIPCHK(tmp.alloc(h->f_blocks.n));
+    launch_near_transpose(nbl, max_rows, h->d_box_first.p, h->d_box_count.p, h->d_e2_off.p,
+                          h->d_e2_list.p, h->d_near_off.p, h->d_perm.p, h->d_pos.p,
+                          h->d_leaf_box.p, (const zf *)h->f_blocks.p, tmp.p, h->st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->st));
+    std::swap(h->f_blocks, tmp);
+  }
+  h->near_bf_ok = h->near_f8_ok = false;
+  h->ig.reset();
+  h->transposed = want;
+  return SPK_OK;
+}
+
+int spk_set_inner_graph(spk_handle *h, int enable) {
+  if (!h) return fail(SPK_ERR_ARG, "null argument");
+  h->graph_ok = enable != 0;
+  if (!h->graph_ok) h->ig.reset();
+  return SPK_OK;
+}
+
+int spk_get_transposed(const spk_handle *h, int *transposed) {
+  if (!h || !transposed) return fail(SPK_ERR_ARG, "null argument");
+  *transposed = h->transposed ? 1 : 0;
+  return SPK_OK;
+}
+
 int spk_matvec_device(spk_handle *h, const void *d_x, void *d_y) {
   if (!h || !d_x || !d_y) return fail(SPK_ERR_ARG, "null argument");
   if (!h->ready) return fail(SPK_ERR_STATE, "spk_setup_tri has not completed");
@@ -935,7 +1066,7 @@ int spk_matvec_parts(spk_handle *h, const double *x, double *y_near, double *y_f
 }
 
 // get_vector_b_tri / incident_field_tri_general, plane wave
-static int rhs_plane_dev(spk_handle *h, const double k_in[3], const double e_in[3], zd *d_b) {
+static RhsArgs rhs_args(const spk_handle *h, const double k_in[3], const double e_in[3]) {
   RhsArgs a{};
   a.N = h->N;
   a.ng = h->ngp;
@@ -976,6 +1107,11 @@ static int rhs_plane_dev(spk_handle *h, const double k_in[3], const double e_in[
     a.ca[q] = zd{ca[q].real(), ca[q].imag()};
     a.cb[q] = zd{cb[q].real(), cb[q].imag()};
   }
+  return a;
+}
+
+static int rhs_plane_dev(spk_handle *h, const double k_in[3], const double e_in[3], zd *d_b) {
+  const RhsArgs a = rhs_args(h, k_in, e_in);
   launch_rhs_plane(a, h->d_fn.p, d_b, h->st);
   HIPCHK(hipGetLastError());
   return SPK_OK;
@@ -1702,6 +1838,10 @@ static int fgmres_near(spk_handle *h, int R, const zd *const *b, zd *const *x,
   // 5 x 5 lambda patch, 2e-3 and 5e-3 did not (and 5e-4 left a 16-angle batch
   // at 9.95e-7 of the 1e-6 tolerance)
   constexpr double kRelaxDelta = 5e-3;
+  // SPK_INNER_GRAPH=0: launch the inner steps one by one instead of
+  // replaying their captured graph (read at each solve)
+  const char *gv = std::getenv("SPK_INNER_GRAPH");
+  const bool use_graph = h->graph_ok && !(gv && gv[0] == '0');
   const size_t ldh = (size_t)m + 2, ldhi = (size_t)kin + 2, hik = (size_t)(kin + 1) * kin;
   const size_t ldu = (size_t)(kin + 1) * n;
   auto &F = h->fg;
@@ -1889,6 +2029,9 @@ static int fgmres_near(spk_handle *h, int R, const zd *const *b, zd *const *x,
       else
         HIPCHK(hipMemcpyAsync(U.p + s * ldu, v, n * sizeof(zd), hipMemcpyDeviceToDevice, h->st));
     }
+    // the inner steps' launches (near-field products, Arnoldi updates)
+    auto inner_steps = [&]() -> int {
+    int rc = SPK_OK;
     for (int j = 0; j < kin; ++j) {
       if (P < 4) {
         for (int s = 0; s < P && !rc; ++s) {
@@ -1915,6 +2058,37 @@ static int fgmres_near(spk_handle *h, int R, const zd *const *b, zd *const *x,
       }
       if (rc) return rc;
       launch_arnoldi_cgs<T>(n, P, j, kin, U.p, ldu, Wi.p, h1.p, (int)ldhi, Hi.p, apart.p, h->st);
+    }
+    return rc;
+    };
+    if (use_graph && (f8 || bf)) {
+      // batch buffers sized before the capture (no allocation inside it)
+      if (P >= 4 && (rc = batch_ensure<T>(h, P))) return rc;
+      const std::vector<uint64_t> key = {
+          (uint64_t)P, (uint64_t)kin, (uint64_t)n, (uint64_t)(uintptr_t)f8,
+          (uint64_t)(uintptr_t)bf, (uint64_t)(uintptr_t)U.p, (uint64_t)(uintptr_t)Wi.p,
+          (uint64_t)(uintptr_t)h1.p, (uint64_t)(uintptr_t)Hi.p, (uint64_t)(uintptr_t)apart.p,
+          (uint64_t)(uintptr_t)tin.p, (uint64_t)(uintptr_t)pBB(h, (T *)nullptr).XI.p,
+          (uint64_t)(uintptr_t)pBB(h, (T *)nullptr).XB.p, (uint64_t)h->setup_serial};
+      if (h->ig.key != key || !h->ig.exec) {
+        h->ig.reset();
+        hipGraph_t graph = nullptr;
+        HIPCHK(hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal));
+        const int rc_cap = inner_steps();
+        const hipError_t e_end = hipStreamEndCapture(h->st, &graph);
+        if (rc_cap) {
+          if (graph) (void)hipGraphDestroy(graph);
+          return rc_cap;
+        }
+        HIPCHK(e_end);
+        const hipError_t e_inst = hipGraphInstantiate(&h->ig.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        HIPCHK(e_inst);
+        h->ig.key = key;
+      }
+      HIPCHK(hipGraphLaunch(h->ig.exec, h->st));
+    } else if ((rc = inner_steps())) {
+      return rc;
     }
     HIPCHK(hipMemcpyAsync(h->pin.p, Hi.p, (size_t)P * hik * sizeof(zd), hipMemcpyDeviceToHost, h->st));
     HIPCHK(hipStreamSynchronize(h->st));
@@ -2463,6 +2637,229 @@ int spk_gmres(spk_handle *h, const double *b, double *x, const spk_gmres_params 
   if (rc) return rc;
   HIPCHK(hipMemcpyAsync(x, dx.p, bytes, hipMemcpyDeviceToHost, h->st));
   HIPCHK(hipStreamSynchronize(h->st));
+  return SPK_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint gradient of J = sum_p w_p |E(r_p)|^2 (the scattered field of the
+// plane-wave solution I of Z I = b) with respect to the heights (z) of mesh
+// nodes.  dJ = 2 Re[E^H W dF I + nu^T (db - dZ I)] with Z^T nu = F^T W conj(E):
+// one forward solve, one transposed solve, then for every requested node the
+// central differences of the local quantities it moves (spk_kernels.hip
+// k_adj_*), contracted at once.  No counterpart in the reference.
+// ---------------------------------------------------------------------------
+int spk_adjoint_height_gradient(spk_handle *h, const double k_in[3], const double e_in[3],
+                                int npts, const double *r, const double *w,
+                                const spk_gmres_params *gp, int n_nodes, const int *nodes,
+                                double dz, double *J, double *grad, spk_gmres_info *info) {
+  if (!h || !k_in || !e_in || !r || !w || !gp || !J || npts < 1 || n_nodes < 0 ||
+      (n_nodes > 0 && (!nodes || !grad)) || !(dz > 0))
+    return fail(SPK_ERR_ARG, "bad argument");
+  if (!h->ready) return fail(SPK_ERR_STATE, "spk_setup_tri has not completed");
+  if (h->transposed) return fail(SPK_ERR_STATE, "the handle holds the transposed operator");
+  const int np = (int)(h->pp_host.size() / 3);
+  for (int i = 0; i < n_nodes; ++i)
+    if (nodes[i] < 0 || nodes[i] >= np) return fail(SPK_ERR_ARG, "node index out of range");
+  HIPCHK(hipSetDevice(h->p.device));
+  const int N = h->N;
+  const int64_t n = 2 * (int64_t)N;
+  hipStream_t st = h->st;
+  int rc = SPK_OK;
+  DBuf<zd> b, I, nu, c, E, H, ytmp;
+  DBuf<double> dr, dw;
+  HIPCHK(b.alloc(n));
+  HIPCHK(I.alloc(n));
+  HIPCHK(nu.alloc(n));
+  HIPCHK(c.alloc(n));
+  HIPCHK(ytmp.alloc(n));
+  HIPCHK(E.alloc(3 * (size_t)npts));
+  HIPCHK(H.alloc(3 * (size_t)npts));
+  HIPCHK(dr.upload(std::vector<double>(r, r + 3 * (size_t)npts)));
+  HIPCHK(dw.upload(std::vector<double>(w, w + (size_t)npts)));
+  spk_gmres_info inf[2] = {};
+  // forward solve and its field
+  if ((rc = rhs_plane_dev(h, k_in, e_in, b.p))) return rc;
+  if ((rc = spk_gmres_device(h, b.p, I.p, gp, &inf[0]))) return rc;
+  const ScatArgs sa = scat_args(h, npts);
+  launch_scattered(sa, h->d_fn.p, I.p, dr.p, E.p, H.p, st);
+  HIPCHK(hipGetLastError());
+  std::vector<zd> Eh(3 * (size_t)npts);
+  HIPCHK(hipMemcpyAsync(Eh.data(), E.p, Eh.size() * sizeof(zd), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  double Jv = 0.0;
+  for (int p = 0; p < npts; ++p)
+    for (int k = 0; k < 3; ++k)
+      Jv += w[p] * (Eh[3 * p + k].x * Eh[3 * p + k].x + Eh[3 * p + k].y * Eh[3 * p + k].y);
+  *J = Jv;
+  // the adjoint's right-hand side and the leaf field W of I
+  launch_scattered_t(sa, h->d_fn.p, dr.p, dw.p, E.p, c.p, st);
+  HIPCHK(hipGetLastError());
+  if ((rc = matvec_t<zd>(h, I.p, ytmp.p))) return rc;
+  const Tree &t = h->tree;
+  const size_t S = h->tb.S, nleaf = h->lev[t.l_max].nb;
+  DBuf<zd> WI, Bt;
+  HIPCHK(WI.alloc(8 * S * nleaf));
+  HIPCHK(Bt.alloc(8 * S * nleaf));
+  HIPCHK(hipMemcpyAsync(WI.p, h->d_W.p, 8 * S * nleaf * sizeof(zd), hipMemcpyDeviceToDevice, st));
+  // the transposed solve and its leaf sums
+  if ((rc = spk_set_transposed(h, 1))) return rc;
+  rc = spk_gmres_device(h, c.p, nu.p, gp, &inf[1]);
+  if (!rc) rc = matvec_t<zd>(h, nu.p, ytmp.p);
+  if (!rc) {
+    const hipError_t e = hipMemcpyAsync(Bt.p, h->lev[t.l_max].B.p, 8 * S * nleaf * sizeof(zd),
+                                        hipMemcpyDeviceToDevice, st);
+    if (e != hipSuccess) rc = fail(SPK_ERR_HIP, std::string("hipMemcpyAsync: ") + hipGetErrorString(e));
+  }
+  const int rc_back = spk_set_transposed(h, 0);
+  if (rc) return rc;
+  if (rc_back) return rc_back;
+  if (info) {
+    info[0] = inf[0];
+    info[1] = inf[1];
+  }
+  if (n_nodes == 0) return SPK_OK;
+  // the edges each requested node moves (either side's triangle holds it)
+  std::vector<std::vector<int>> node_edges(np);
+  for (int m = 0; m < N; ++m)
+    for (int q = 0; q < 2; ++q)
+      for (int v = 0; v < 3; ++v) {
+        auto &lst = node_edges[h->er[2 * m + q].node[v] - 1];
+        if (lst.empty() || lst.back() != m) lst.push_back(m);
+      }
+  std::vector<int> aoff(n_nodes + 1, 0), aedge, anode;
+  std::vector<int64_t> toff;
+  for (int k = 0; k < n_nodes; ++k) {
+    std::vector<int> lst = node_edges[nodes[k]];
+    std::sort(lst.begin(), lst.end());
+    lst.erase(std::unique(lst.begin(), lst.end()), lst.end());
+    int64_t acc = 0;
+    for (int m : lst) {
+      aedge.push_back(m);
+      anode.push_back(k);
+      toff.push_back(acc);
+      const int bx = t.leaf_box[m];
+      acc += 2 * (int64_t)(t.e2_off[bx + 1] - t.e2_off[bx]);
+    }
+    toff.push_back(acc);  // the node's total (its last slot's end)
+    aoff[k + 1] = (int)aedge.size();
+  }
+  // task_off is indexed per slot with one extra entry per node: rebase so
+  // that node k's slots start at aoff[k] + k
+  std::vector<int> aoff_t(n_nodes + 1);
+  for (int k = 0; k <= n_nodes; ++k) aoff_t[k] = aoff[k] + k;
+  // per-edge side nodes (p1, p2, free vertex, element nodes), 0-based
+  std::vector<int> sn(12 * (size_t)N);
+  for (int m = 0; m < N; ++m)
+    for (int q = 0; q < 2; ++q) {
+      const int *cr = h->er[2 * m + q].node;
+      int nv = 0, ipc = 0;
+      for (int j = 0; j < 3; ++j)
+        if (cr[j] != h->corner[2 * m] && cr[j] != h->corner[2 * m + 1]) {
+          nv = j + 1;
+          ipc = cr[j];
+        }
+      int i1, i2;
+      if (nv == 1) {
+        i1 = cr[1];
+        i2 = cr[2];
+      } else if (nv == 2) {
+        i1 = cr[2];
+        i2 = cr[0];
+      } else {
+        i1 = cr[0];
+        i2 = cr[1];
+      }
+      int *o = &sn[6 * (2 * (size_t)m + q)];
+      o[0] = i1 - 1;
+      o[1] = i2 - 1;
+      o[2] = ipc - 1;
+      o[3] = cr[0] - 1;
+      o[4] = cr[1] - 1;
+      o[5] = cr[2] - 1;
+    }
+  DBuf<double> d_pp;
+  DBuf<int> d_sn, d_nodes, d_aoff, d_aedge, d_anode;
+  DBuf<int64_t> d_toff;
+  HIPCHK(d_pp.upload(h->pp_host));
+  HIPCHK(d_sn.upload(sn));
+  HIPCHK(d_nodes.upload(std::vector<int>(nodes, nodes + n_nodes)));
+  HIPCHK(d_aoff.upload(aoff_t));
+  HIPCHK(d_aedge.upload(aedge));
+  HIPCHK(d_anode.upload(anode));
+  HIPCHK(d_toff.upload(toff));
+  const int nslot = (int)aedge.size();
+  AdjArgs G{};
+  G.N = N;
+  G.nnode = n_nodes;
+  G.nslot = nslot;
+  G.normal_convention = h->p.normal_convention;
+  G.dz = dz;
+  G.pp = d_pp.p;
+  G.side_nodes = d_sn.p;
+  G.leaf_box = h->d_leaf_box.p;
+  G.node = d_nodes.p;
+  // the near kernel reads slots through aff_off (rebased, with the per-node
+  // end entries of task_off); the per-slot kernels read aff_edge / aff_node
+  G.aff_off = d_aoff.p;
+  G.aff_node = d_anode.p;
+  G.task_off = d_toff.p;
+  // aff_edge as the near kernel sees it: one placeholder entry per node end
+  std::vector<int> aedge_t;
+  for (int k = 0; k < n_nodes; ++k) {
+    for (int q = aoff[k]; q < aoff[k + 1]; ++q) aedge_t.push_back(aedge[q]);
+    aedge_t.push_back(-1);
+  }
+  DBuf<int> d_aedge_t;
+  HIPCHK(d_aedge_t.upload(aedge_t));
+  const NearArgs A = near_args(h);
+  constexpr int kGroups = 64;
+  DBuf<zd> near_part, far_out, rhs_out, fld_out;
+  HIPCHK(near_part.alloc((size_t)n_nodes * kGroups));
+  HIPCHK(far_out.alloc(std::max(nslot, 1)));
+  HIPCHK(rhs_out.alloc(std::max(nslot, 1)));
+  HIPCHK(fld_out.alloc(std::max(nslot, 1)));
+  {
+    AdjArgs Gn = G;
+    Gn.aff_edge = d_aedge_t.p;
+    launch_adj_near(A, Gn, kGroups, nu.p, I.p, near_part.p, st);
+  }
+  G.aff_edge = d_aedge.p;
+  AdjFar F{};
+  F.S = (int)S;
+  F.ngp = h->ngp;
+  F.khat = h->d_khat.p;
+  F.tth = h->d_tth.p;
+  F.tph = h->d_tph.p;
+  F.leaf_centre = h->d_leaf_centre.p;
+  for (int i = 0; i < 12; ++i) {
+    F.qa[i] = h->qa[i];
+    F.qb[i] = h->qb[i];
+    F.qw[i] = h->qw[i];
+  }
+  F.k1 = zcx(h->mat.k1);
+  F.k2 = zcx(h->mat.k2);
+  F.sc1 = h->sc1;
+  F.sc2 = h->sc2;
+  launch_adj_far(G, F, nu.p, I.p, WI.p, Bt.p, far_out.p, st);
+  launch_adj_rhs(G, rhs_args(h, k_in, e_in), nu.p, rhs_out.p, st);
+  launch_adj_field(G, sa, dr.p, dw.p, E.p, I.p, fld_out.p, st);
+  HIPCHK(hipGetLastError());
+  std::vector<zd> hn((size_t)n_nodes * kGroups), hf(std::max(nslot, 1)), hr(std::max(nslot, 1)),
+      hq(std::max(nslot, 1));
+  HIPCHK(hipMemcpyAsync(hn.data(), near_part.p, hn.size() * sizeof(zd), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(hf.data(), far_out.p, hf.size() * sizeof(zd), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(hr.data(), rhs_out.p, hr.size() * sizeof(zd), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(hq.data(), fld_out.p, hq.size() * sizeof(zd), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  for (int k = 0; k < n_nodes; ++k) {
+    zc tz(0.0, 0.0), tf(0.0, 0.0);
+    for (int g = 0; g < kGroups; ++g) tz += zc(hn[(size_t)k * kGroups + g].x, hn[(size_t)k * kGroups + g].y);
+    for (int q = aoff[k]; q < aoff[k + 1]; ++q) {
+      tz += zc(hf[q].x, hf[q].y);
+      tf += zc(hq[q].x, hq[q].y) + zc(hr[q].x, hr[q].y);
+    }
+    grad[k] = 2.0 * (tf - tz).real() / (2.0 * dz);
+  }
   return SPK_OK;
 }
 

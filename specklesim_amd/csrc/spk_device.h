@@ -100,9 +100,21 @@ void launch_patterns(int N, int S, int ngp, const FnRec *fn, const int *perm, co
                      const double *leaf_centre, const double *khat, const double *tth,
                      const double *tph, const double *qa, const double *qb, const double *qw,
                      zd k1, zd k2, zd *rt, hipStream_t st);
+// conj_rt = false: the transposed operator's aggregation (patterns not conjugated)
 template <class T>
 void launch_agg(int nbl, int S, int N, const int *box_first, const int *box_count,
-                const int *perm, const T *rt, const T *x, T *C, hipStream_t st);
+                const int *perm, const T *rt, const T *x, T *C, hipStream_t st,
+                bool conj_rt = true);
+template <class T>
+void launch_fold_t(int nbl, int S, const T *C, zd kd1, zd kd2, zd my1, zd my2, zd ep1, zd ep2,
+                   T *D, hipStream_t st);
+template <class T>
+void launch_scale_halves(int N, zd sc1, zd sc2, const T *x, T *out, hipStream_t st);
+template <class T>
+void launch_near_transpose(int nbl, int max_rows, const int *box_first, const int *box_count,
+                           const int *e2_off, const int *e2_list, const int64_t *near_off,
+                           const int *perm, const int *pos, const int *leaf_box, const T *in,
+                           T *out, hipStream_t st);
 template <class T>
 void launch_ss(int nb, int S, const int *child, const T *Cc, const T *up, T *P, hipStream_t st);
 template <class T>
@@ -114,10 +126,11 @@ void launch_rr(int nb, int S, const int *parent, const int *octant, const T *Dp,
 template <class T>
 void launch_fold(int nbl, int S, const T *D, zd kd1, zd kd2, zd my1, zd my2, zd ep1, zd ep2,
                  T *W, hipStream_t st);
+// conj_rt: the transposed operator's disaggregation (conjugated patterns)
 template <class T>
 void launch_disagg(int ntask, int N, int S, const int *task, const int *perm,
                    const int *leaf_box_of_pos, const T *rt, const T *W, zd sc1, zd sc2, T *y,
-                   hipStream_t st);
+                   hipStream_t st, bool conj_rt = false);
 template <class T>
 void launch_near_spmv_rows(int nwg, int N, const int *row_box, const int *row_r0,
                            const int *box_first, const int *box_count, const int *e2_off,
@@ -171,6 +184,39 @@ struct ScatArgs {
 };
 void launch_scattered(const ScatArgs &a, const FnRec *fn, const zd *x, const double *r, zd *E,
                       zd *H, hipStream_t st);
+
+// adjoint gradient with respect to node heights (central differences of
+// the local quantities, node moved by +-dz along z)
+struct AdjArgs {
+  int N, nnode, nslot, normal_convention;
+  double dz;
+  const double *pp;        // node coordinates [3 * np]
+  const int *side_nodes;   // per RWG side: p1, p2, free vertex, element nodes (0-based) [6 * 2N]
+  const int *leaf_box;     // edge -> leaf box
+  const int *node;         // requested node ids (0-based) [nnode]
+  // near kernel: node k's slots are aff_off[k] .. aff_off[k + 1] - 2 of
+  // aff_edge / task_off, followed by one end entry (task_off = the node's
+  // task count); the per-slot kernels index aff_edge / aff_node by slot
+  const int *aff_off;
+  const int *aff_edge;
+  const int *aff_node;
+  const int64_t *task_off;
+};
+struct AdjFar {
+  int S, ngp;
+  const double *khat, *tth, *tph, *leaf_centre;
+  double qa[12], qb[12], qw[12];
+  zd k1, k2, sc1, sc2;
+};
+void launch_adj_near(const NearArgs &A, const AdjArgs &G, int groups, const zd *nu, const zd *I,
+                     zd *partial, hipStream_t st);
+void launch_adj_far(const AdjArgs &G, const AdjFar &F, const zd *nu, const zd *I, const zd *W,
+                    const zd *Bt, zd *out, hipStream_t st);
+void launch_adj_rhs(const AdjArgs &G, const RhsArgs &a, const zd *nu, zd *out, hipStream_t st);
+void launch_scattered_t(const ScatArgs &a, const FnRec *fn, const double *r, const double *w,
+                        const zd *E, zd *c, hipStream_t st);
+void launch_adj_field(const AdjArgs &G, const ScatArgs &a, const double *r, const double *w,
+                      const zd *E, const zd *I, zd *out, hipStream_t st);
 
 // Jacobi preconditioner: diag = [Z_mm(1,1); Z_mm(2,2)] (reference order), y = x / diag
 void launch_jacobi_diag(int N, const int *perm, const int *leaf_box_of_pos, const int *box_first,

@@ -826,7 +826,9 @@ __device__ inline zf2 ld2(const zf *p) {
   return zf2{zf{v.x, v.y}, zf{v.z, v.w}};
 }
 
-template <class T>
+// CONJ = false: the patterns themselves (the transposed operator's
+// disaggregation, aggregated)
+template <class T, bool CONJ>
 __global__ void k_agg(int S, int N, const int *box_first, const int *box_count, const int *perm,
                       const T *rt, const T *x, T *C) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -842,7 +844,8 @@ __global__ void k_agg(int S, int N, const int *box_first, const int *box_count, 
     for (int e = e0; e < e1; ++e) {
       const int p = p0 + e;
       const int m = perm[p];
-      const T f = conj(rt[((size_t)p * 4 + q) * S + s]);
+      const T r = rt[((size_t)p * 4 + q) * S + s];
+      const T f = CONJ ? conj(r) : r;
       const T xe = x[m], xh = x[m + N];
       ge = ge + f * xe;
       gh = gh + f * xh;
@@ -857,10 +860,14 @@ __global__ void k_agg(int S, int N, const int *box_first, const int *box_count, 
 
 template <class T>
 void launch_agg(int nbl, int S, int N, const int *box_first, const int *box_count,
-                const int *perm, const T *rt, const T *x, T *C, hipStream_t st) {
+                const int *perm, const T *rt, const T *x, T *C, hipStream_t st, bool conj_rt) {
   if (nbl <= 0) return;
-  hipLaunchKernelGGL(k_agg<T>, dim3(nblk(S, 256), nbl, 4), dim3(256), 0, st, S, N, box_first,
-                     box_count, perm, rt, x, C);
+  if (conj_rt)
+    hipLaunchKernelGGL((k_agg<T, true>), dim3(nblk(S, 256), nbl, 4), dim3(256), 0, st, S, N,
+                       box_first, box_count, perm, rt, x, C);
+  else
+    hipLaunchKernelGGL((k_agg<T, false>), dim3(nblk(S, 256), nbl, 4), dim3(256), 0, st, S, N,
+                       box_first, box_count, perm, rt, x, C);
 }
 
 // translation_SS (:466-528): P[it][s] = sum_o C_child[it][s] * up[o][k][s]
@@ -980,6 +987,99 @@ void launch_fold(int nbl, int S, const T *D, zd kd1, zd kd2, zd my1, zd my2, zd 
                      cvt<T>(kd2), cvt<T>(my1), cvt<T>(my2), cvt<T>(ep1), cvt<T>(ep2), W);
 }
 
+// The transpose of k_fold per box and sample: from the aggregated adjoint
+// sums C (layout [comp*4 + cur*2 + k], comp = theta/phi, cur = e/h) to the
+// adjoint of D (the same layout):
+//   D_ae = my C_ae + kd C_bh,  D_ah = -kd C_be + ep C_ah,
+//   D_be = my C_be - kd C_ah,  D_bh = kd C_ae + ep C_bh.
+template <class T>
+__global__ void k_fold_t(int S, const T *C, T kd1, T kd2, T my1, T my2, T ep1, T ep2, T *D) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  if (s >= S) return;
+  const T *Cb = C + (size_t)b * 8 * S;
+  T *Db = D + (size_t)b * 8 * S;
+  for (int k = 0; k < 2; ++k) {
+    const T kd = k ? kd2 : kd1, my = k ? my2 : my1, ep = k ? ep2 : ep1;
+    const T Cae = Cb[(size_t)(0 + k) * S + s], Cah = Cb[(size_t)(2 + k) * S + s];
+    const T Cbe = Cb[(size_t)(4 + k) * S + s], Cbh = Cb[(size_t)(6 + k) * S + s];
+    Db[(size_t)(0 + k) * S + s] = Cae * my + Cbh * kd;
+    Db[(size_t)(2 + k) * S + s] = -(Cbe * kd) + Cah * ep;
+    Db[(size_t)(4 + k) * S + s] = Cbe * my - Cah * kd;
+    Db[(size_t)(6 + k) * S + s] = Cae * kd + Cbh * ep;
+  }
+}
+
+template <class T>
+void launch_fold_t(int nbl, int S, const T *C, zd kd1, zd kd2, zd my1, zd my2, zd ep1, zd ep2,
+                   T *D, hipStream_t st) {
+  if (nbl <= 0) return;
+  hipLaunchKernelGGL(k_fold_t<T>, dim3(nblk(S, 256), nbl), dim3(256), 0, st, S, C, cvt<T>(kd1),
+                     cvt<T>(kd2), cvt<T>(my1), cvt<T>(my2), cvt<T>(ep1), cvt<T>(ep2), D);
+}
+
+// out = [sc1 x_e; sc2 x_h]
+template <class T>
+__global__ void k_scale_halves(int N, T sc1, T sc2, const T *x, T *out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 2 * (int64_t)N) return;
+  out[i] = x[i] * (i < N ? sc1 : sc2);
+}
+
+template <class T>
+void launch_scale_halves(int N, zd sc1, zd sc2, const T *x, T *out, hipStream_t st) {
+  if (N <= 0) return;
+  hipLaunchKernelGGL(k_scale_halves<T>, dim3(nblk(2 * (int64_t)N, 256)), dim3(256), 0, st, N,
+                     cvt<T>(sc1), cvt<T>(sc2), x, out);
+}
+
+// The transpose of the near blocks: block (m, n) of the output is block
+// (n, m) of the input with its 2 x 2 components transposed.  One workgroup
+// per (leaf box, row); the column's own row lives in its leaf box, found
+// through pos, and m among that box's E2 columns (sorted by edge number)
+// by bisection.
+template <class T>
+__global__ void k_near_transpose(int nbl, const int *box_first, const int *box_count,
+                                 const int *e2_off, const int *e2_list, const int64_t *near_off,
+                                 const int *perm, const int *pos, const int *leaf_box,
+                                 const T *in, T *out) {
+  const int b = blockIdx.y;
+  const int r = blockIdx.x;
+  if (b >= nbl || r >= box_count[b]) return;
+  const int m = perm[box_first[b] + r];
+  const int c0 = e2_off[b], cols = e2_off[b + 1] - c0;
+  for (int c = threadIdx.x; c < cols; c += blockDim.x) {
+    const int n = e2_list[c0 + c];
+    const int bq = leaf_box[n];
+    const int rq = pos[n] - box_first[bq];
+    const int q0 = e2_off[bq], qc = e2_off[bq + 1] - q0;
+    int lo = 0, hi = qc - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (e2_list[q0 + mid] < m)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    const T *src = in + 4 * (near_off[bq] + (int64_t)rq * qc + lo);
+    T *dst = out + 4 * (near_off[b] + (int64_t)r * cols + c);
+    dst[0] = src[0];
+    dst[1] = src[2];
+    dst[2] = src[1];
+    dst[3] = src[3];
+  }
+}
+
+template <class T>
+void launch_near_transpose(int nbl, int max_rows, const int *box_first, const int *box_count,
+                           const int *e2_off, const int *e2_list, const int64_t *near_off,
+                           const int *perm, const int *pos, const int *leaf_box, const T *in,
+                           T *out, hipStream_t st) {
+  if (nbl <= 0 || max_rows <= 0) return;
+  hipLaunchKernelGGL(k_near_transpose<T>, dim3(max_rows, nbl), dim3(256), 0, st, nbl, box_first,
+                     box_count, e2_off, e2_list, near_off, perm, pos, leaf_box, in, out);
+}
+
 __device__ inline zd wave_sum(zd v) {
   for (int off = 32; off > 0; off >>= 1) {
     v.x += __shfl_down(v.x, off, 64);
@@ -1001,7 +1101,9 @@ __device__ inline zf wave_sum(zf v) {
 // coefficient array), the patterns of the group's edges stream from HBM.
 // task[t] = p0 * 8 + count.  Per edge the sums run in the same order as one
 // wavefront per edge would (media, then lane-strided samples).
-template <class T>
+// CONJ: the conjugated patterns (the transposed operator's aggregation,
+// disaggregated)
+template <class T, bool CONJ>
 __global__ void __launch_bounds__(256) k_disagg(int ntask, int N, int S, const int *task,
                                                 const int *perm, const int *leaf_box_of_pos,
                                                 const T *rt, const T *W, T sc1, T sc2, T *y) {
@@ -1032,6 +1134,10 @@ __global__ void __launch_bounds__(256) k_disagg(int ntask, int N, int S, const i
         for (int e = 0; e < E; ++e) {
           a[e] = ld2(f[e] + s);
           bb[e] = ld2(f[e] + S + s);
+          if (CONJ) {
+            a[e] = zf2{conj(a[e].a), conj(a[e].b)};
+            bb[e] = zf2{conj(bb[e].a), conj(bb[e].b)};
+          }
         }
 #pragma unroll
         for (int e = 0; e < E; ++e)
@@ -1046,8 +1152,8 @@ __global__ void __launch_bounds__(256) k_disagg(int ntask, int N, int S, const i
         T a[E], bb[E];
 #pragma unroll
         for (int e = 0; e < E; ++e) {
-          a[e] = f[e][s];
-          bb[e] = f[e][S + s];
+          a[e] = CONJ ? conj(f[e][s]) : f[e][s];
+          bb[e] = CONJ ? conj(f[e][S + s]) : f[e][S + s];
         }
 #pragma unroll
         for (int e = 0; e < E; ++e)
@@ -1072,10 +1178,14 @@ __global__ void __launch_bounds__(256) k_disagg(int ntask, int N, int S, const i
 template <class T>
 void launch_disagg(int ntask, int N, int S, const int *task, const int *perm,
                    const int *leaf_box_of_pos, const T *rt, const T *W, zd sc1, zd sc2, T *y,
-                   hipStream_t st) {
+                   hipStream_t st, bool conj_rt) {
   if (ntask <= 0) return;
-  hipLaunchKernelGGL(k_disagg<T>, dim3(nblk(ntask, 4)), dim3(256), 0, st, ntask, N, S, task, perm,
-                     leaf_box_of_pos, rt, W, cvt<T>(sc1), cvt<T>(sc2), y);
+  if (conj_rt)
+    hipLaunchKernelGGL((k_disagg<T, true>), dim3(nblk(ntask, 4)), dim3(256), 0, st, ntask, N, S,
+                       task, perm, leaf_box_of_pos, rt, W, cvt<T>(sc1), cvt<T>(sc2), y);
+  else
+    hipLaunchKernelGGL((k_disagg<T, false>), dim3(nblk(ntask, 4)), dim3(256), 0, st, ntask, N, S,
+                       task, perm, leaf_box_of_pos, rt, W, cvt<T>(sc1), cvt<T>(sc2), y);
 }
 
 // Near SpMV over the box-dense block matrices.  Workgroup = (leaf box,
@@ -1159,7 +1269,12 @@ void launch_near_spmv_rows(int nwg, int N, const int *row_box, const int *row_r0
 
 #define SPK_MATVEC_INSTANTIATE(T)                                                               \
   template void launch_agg<T>(int, int, int, const int *, const int *, const int *, const T *,   \
-                              const T *, T *, hipStream_t);                                      \
+                              const T *, T *, hipStream_t, bool);                                \
+  template void launch_fold_t<T>(int, int, const T *, zd, zd, zd, zd, zd, zd, T *, hipStream_t); \
+  template void launch_scale_halves<T>(int, zd, zd, const T *, T *, hipStream_t);                \
+  template void launch_near_transpose<T>(int, int, const int *, const int *, const int *,        \
+                                         const int *, const int64_t *, const int *, const int *, \
+                                         const int *, const T *, T *, hipStream_t);              \
   template void launch_ss<T>(int, int, const int *, const T *, const T *, T *, hipStream_t);     \
   template void launch_m2l<T>(int, int, const int *, const int *, const int *, const T *,        \
                               const T *, T *, hipStream_t);                                      \
@@ -1167,7 +1282,7 @@ void launch_near_spmv_rows(int nwg, int N, const int *row_box, const int *row_r0
                              hipStream_t);                                                       \
   template void launch_fold<T>(int, int, const T *, zd, zd, zd, zd, zd, zd, T *, hipStream_t);   \
   template void launch_disagg<T>(int, int, int, const int *, const int *, const int *, const T *, \
-                                 const T *, zd, zd, T *, hipStream_t);                           \
+                                 const T *, zd, zd, T *, hipStream_t, bool);                     \
   template void launch_near_spmv_rows<T>(int, int, const int *, const int *, const int *,        \
                                          const int *, const int *, const int *, const int64_t *, \
                                          const int *, const T *, const T *, const T *, T *,      \
@@ -2086,6 +2201,384 @@ void launch_scattered(const ScatArgs &a, const FnRec *fn, const zd *x, const dou
                       zd *H, hipStream_t st) {
   if (a.npts <= 0) return;
   hipLaunchKernelGGL(k_scattered, dim3(a.npts), dim3(256), 0, st, a, fn, x, r, E, H);
+}
+
+
+// ---------------------------------------------------------------------------
+// Adjoint gradient of a far-field intensity with respect to node heights.
+// Every geometry-dependent local quantity (an RWG side's record, a near
+// block, a radiation pattern, a right-hand-side entry, a side's field at a
+// detector point) is evaluated with one node moved by +dz and by -dz along
+// z, and the central difference is contracted at once with the adjoint
+// vector, the currents or the leaf fields.  Near blocks keep the singular /
+// regular branch choice of the unmoved geometry.
+// ---------------------------------------------------------------------------
+
+// The record of RWG side `side` (0, 1) from node coordinates, node pn moved
+// by dz along z; sn = (p1, p2, free vertex, element nodes in tt order),
+// 0-based; the arithmetic of build_rwg (fn_parameter_type, norm_vec_r_new).
+__device__ FnRec fn_rebuild(const double *pp, const int *sn, int side, int normal_convention,
+                            int pn, double dz) {
+  FnRec f;
+  auto P = [&](int v, int d) -> double { return pp[3 * v + d] + ((v == pn && d == 2) ? dz : 0.0); };
+  for (int d = 0; d < 3; ++d) {
+    f.c[d] = P(sn[0], d);
+    f.c[3 + d] = P(sn[1], d);
+    f.c[6 + d] = P(sn[2], d);
+  }
+  const double dv[3] = {f.c[0] - f.c[3], f.c[1] - f.c[4], f.c[2] - f.c[5]};
+  f.length = sqrt(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
+  const double a1[3] = {f.c[0] - f.c[6], f.c[1] - f.c[7], f.c[2] - f.c[8]};
+  const double a2[3] = {f.c[3] - f.c[6], f.c[4] - f.c[7], f.c[5] - f.c[8]};
+  double x[3];
+  cross3(a1, a2, x);
+  f.area = 0.5 * sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+  const double sg = side == 0 ? 1.0 : -1.0;
+  f.sign = sg;
+  f.df = sg * f.length / f.area;
+  for (int d = 0; d < 3; ++d) f.mid[d] = (P(sn[3], d) + P(sn[4], d) + P(sn[5], d)) / 3.0;
+  double v1[3], v2[3];
+  for (int d = 0; d < 3; ++d) {
+    if (normal_convention == 1) {
+      v1[d] = f.c[d] - f.c[3 + d];
+      v2[d] = f.c[6 + d] - f.c[3 + d];
+    } else {
+      v1[d] = f.c[6 + d] - f.c[d];
+      v2[d] = f.c[3 + d] - f.c[d];
+    }
+  }
+  double nx[3];
+  cross3(v1, v2, nx);
+  const double nl = len3(nx);
+  for (int d = 0; d < 3; ++d) {
+    f.nq[d] = nx[d] / nl;
+    f.nqf[d] = (double)(float)f.nq[d];
+  }
+  f.pad[0] = f.pad[1] = 0.0;
+  return f;
+}
+
+__device__ inline bool in_list(const int *v, int n, int x) {
+  for (int i = 0; i < n; ++i)
+    if (v[i] == x) return true;
+  return false;
+}
+
+// near block (m, n) from the given side records, with the branch choices
+// (singular test, self element, edge adjacency) of the unmoved geometry
+__device__ void block_frozen(const NearArgs &A, int m, int n, const FnRec *fm2, const FnRec *fn2,
+                             zd *D) {
+  const double rr = 1.5 * A.rref[m];
+  D[0] = D[1] = D[2] = D[3] = zd{0, 0};
+  for (int p = 0; p < 4; ++p) {
+    const int t = p >> 1, s = p & 1;
+    const FnRec &bm = A.fn[2 * m + t], &bq = A.fn[2 * n + s];
+    const double d[3] = {bm.mid[0] - bq.mid[0], bm.mid[1] - bq.mid[1], bm.mid[2] - bq.mid[2]};
+    const ElemRec ea = A.er[2 * m + t], eb = A.er[2 * n + s];
+    Rel rv[2];
+    if (!(len3(d) > rr)) {
+      const bool iqs = ea.elem == eb.elem;
+      singular_pair(fm2[t], fn2[s], iqs, A, rv);
+      if (A.edge_swap && !iqs && edge_adj(ea, eb)) {
+        zd kt[2];
+        edge_adjacent_kt(fm2[t], fn2[s], A, kt);
+        rv[0].Kt = kt[0];
+        rv[1].Kt = kt[1];
+      }
+    } else {
+      normal_pair<true>(fm2[t], fn2[s], A, rv);
+    }
+    zd z[4];
+    assemble_zz(A, rv, z);
+    accumulate_D(A, D, z);
+  }
+  finish_D(A, D);
+}
+
+// sum over the requested nodes' near tasks of nu_m^T (Z_mn(+dz) - Z_mn(-dz)) I_n:
+// for slot a (affected edge e_a) of node k, tasks j < cols: (e_a, E2 column j);
+// cols <= j < 2 cols: (E2 column j - cols, e_a) unless that row is affected
+// too (then it is counted among its own slot's rows)
+__global__ void __launch_bounds__(256) k_adj_near(NearArgs A, AdjArgs G, const zd *nu, const zd *I,
+                                                  zd *partial) {
+  const int k = blockIdx.y;
+  const int pn = G.node[k];
+  // node k's slots: aff_off[k] .. aff_off[k + 1] - 2, then its end entry
+  const int s0 = G.aff_off[k], ns = G.aff_off[k + 1] - s0 - 1;
+  const int *aff = G.aff_edge + s0;
+  const int64_t *toff = G.task_off + s0;
+  const int64_t ntask = toff[ns];
+  zd acc = {0, 0};
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < ntask;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    int a = 0;
+    while (a + 1 < ns && toff[a + 1] <= t) ++a;
+    const int ea = aff[a];
+    const int b = G.leaf_box[ea];
+    const int c0 = A.e2_off[b], cols = A.e2_off[b + 1] - c0;
+    const int64_t loc = t - toff[a];
+    const int other = A.e2_list[c0 + (int)(loc % cols)];
+    int m, n;
+    if (loc < cols) {
+      m = ea;
+      n = other;
+    } else {
+      if (in_list(aff, ns, other)) continue;
+      m = other;
+      n = ea;
+    }
+    const bool am = in_list(aff, ns, m), an = in_list(aff, ns, n);
+    zd Dp[4], Dm[4];
+    for (int sgn = 0; sgn < 2; ++sgn) {
+      const double dz = sgn ? -G.dz : G.dz;
+      FnRec fm2[2], fn2[2];
+      for (int q = 0; q < 2; ++q) {
+        fm2[q] = am ? fn_rebuild(G.pp, G.side_nodes + 6 * (2 * m + q), q, G.normal_convention, pn, dz)
+                    : A.fn[2 * m + q];
+        fn2[q] = an ? fn_rebuild(G.pp, G.side_nodes + 6 * (2 * n + q), q, G.normal_convention, pn, dz)
+                    : A.fn[2 * n + q];
+      }
+      block_frozen(A, m, n, fm2, fn2, sgn ? Dm : Dp);
+    }
+    const zd d0 = Dp[0] - Dm[0], d1 = Dp[1] - Dm[1], d2 = Dp[2] - Dm[2], d3 = Dp[3] - Dm[3];
+    const zd ie = I[n], ih = I[n + G.N];
+    acc = acc + nu[m] * (d0 * ie + d1 * ih) + nu[m + G.N] * (d2 * ie + d3 * ih);
+  }
+  acc = block_sum(acc);
+  if (threadIdx.x == 0) partial[(size_t)k * gridDim.x + blockIdx.x] = acc;
+}
+
+// the four pattern components of edge m (sides fs[2]) at sample s: theta
+// and phi projections for media 1 and 2 (k_patterns' integrand)
+__device__ void pattern_sample(const FnRec *fs, const double *Rc, const double *kh,
+                               const double *tt3, const double *tp3, int ngp, const double *qa,
+                               const double *qb, const double *qw, zd k1, zd k2, zd *o) {
+  o[0] = o[1] = o[2] = o[3] = zd{0, 0};
+  for (int ll = 0; ll < 2; ++ll) {
+    const FnRec &f = fs[ll];
+    const double dfm = f.sign * f.length;
+    zd sa[3] = {{0, 0}, {0, 0}, {0, 0}}, sb[3] = {{0, 0}, {0, 0}, {0, 0}};
+    for (int j = 0; j < ngp; ++j) {
+      double Ro[3], fm[3], Roc[3];
+      simplex(qa[j], qb[j], f.c, Ro);
+      for (int kk = 0; kk < 3; ++kk) {
+        fm[kk] = 0.5 * (Ro[kk] - f.c[6 + kk]) * dfm;
+        Roc[kk] = Ro[kk] - Rc[kk];
+      }
+      const double tmp = dot3(kh, Roc);
+      const zd ff1 = expmikr(k1, tmp) * qw[j];
+      const zd ff2 = expmikr(k2, tmp) * qw[j];
+      for (int kk = 0; kk < 3; ++kk) {
+        sa[kk] = sa[kk] + fm[kk] * ff1;
+        sb[kk] = sb[kk] + fm[kk] * ff2;
+      }
+    }
+    o[0] = o[0] + dot_rc(tt3, sa);
+    o[1] = o[1] + dot_rc(tp3, sa);
+    o[2] = o[2] + dot_rc(tt3, sb);
+    o[3] = o[3] + dot_rc(tp3, sb);
+  }
+}
+
+// per (node, affected edge m): the far field's share, through the change of
+// m's pattern: as a receiver nu_m against the leaf field W of I (the folded
+// coefficients), as a source I_m against the transposed pipeline's leaf sums
+// Bt of nu
+__global__ void __launch_bounds__(256) k_adj_far(AdjArgs G, AdjFar F, const zd *nu, const zd *I,
+                                                 const zd *W, const zd *Bt, zd *out) {
+  const int g = blockIdx.x;
+  const int pn = G.node[G.aff_node[g]];
+  const int m = G.aff_edge[g];
+  const int b = G.leaf_box[m];
+  const int S = F.S;
+  FnRec fp[2], fq[2];
+  for (int q = 0; q < 2; ++q) {
+    fp[q] = fn_rebuild(G.pp, G.side_nodes + 6 * (2 * m + q), q, G.normal_convention, pn, G.dz);
+    fq[q] = fn_rebuild(G.pp, G.side_nodes + 6 * (2 * m + q), q, G.normal_convention, pn, -G.dz);
+  }
+  const double *Rc = F.leaf_centre + 3 * b;
+  const zd *Wb = W + (size_t)b * 8 * S, *Bb = Bt + (size_t)b * 8 * S;
+  const zd ne = nu[m] * F.sc1, nh = nu[m + G.N] * F.sc2, ie = I[m], ih = I[m + G.N];
+  zd acc = {0, 0};
+  for (int s = threadIdx.x; s < S; s += blockDim.x) {
+    zd op[4], om[4];
+    pattern_sample(fp, Rc, F.khat + 3 * s, F.tth + 3 * s, F.tph + 3 * s, F.ngp, F.qa, F.qb, F.qw,
+                   F.k1, F.k2, op);
+    pattern_sample(fq, Rc, F.khat + 3 * s, F.tth + 3 * s, F.tph + 3 * s, F.ngp, F.qa, F.qb, F.qw,
+                   F.k1, F.k2, om);
+    for (int kk = 0; kk < 2; ++kk) {
+      const zd da = op[kk * 2 + 0] - om[kk * 2 + 0], db = op[kk * 2 + 1] - om[kk * 2 + 1];
+      acc = acc + ne * (da * Wb[(size_t)(kk * 4 + 0) * S + s] + db * Wb[(size_t)(kk * 4 + 1) * S + s]);
+      acc = acc + nh * (da * Wb[(size_t)(kk * 4 + 2) * S + s] + db * Wb[(size_t)(kk * 4 + 3) * S + s]);
+      // source side: C[comp*4 + cur*2 + k] = conj(rt[k*2 + comp]) x_cur
+      acc = acc + ie * (conj(da) * Bb[(size_t)(0 * 4 + 0 * 2 + kk) * S + s] +
+                        conj(db) * Bb[(size_t)(1 * 4 + 0 * 2 + kk) * S + s]);
+      acc = acc + ih * (conj(da) * Bb[(size_t)(0 * 4 + 1 * 2 + kk) * S + s] +
+                        conj(db) * Bb[(size_t)(1 * 4 + 1 * 2 + kk) * S + s]);
+    }
+  }
+  acc = block_sum(acc);
+  if (threadIdx.x == 0) out[g] = acc;
+}
+
+// b_e, b_h of edge m from its side records (k_rhs_plane's integrand)
+__device__ void rhs_edge(const RhsArgs &a, const FnRec *fs, zd &be, zd &bh) {
+  zd S4[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+  for (int ot = 0; ot < 2; ++ot) {
+    const FnRec &f = fs[ot];
+    for (int i = 0; i < a.ng; ++i) {
+      double Ra[3], rfm[3];
+      const double w0 = 1 - a.qa[i] - a.qb[i];
+      for (int d = 0; d < 3; ++d) {
+        Ra[d] = w0 * f.c[d] + a.qa[i] * f.c[3 + d] + a.qb[i] * f.c[6 + d];
+        rfm[d] = (Ra[d] - f.c[6 + d]) * f.df * 0.5;
+      }
+      const zd ph = (a.k1 * a.kin[0]) * Ra[0] + (a.k1 * a.kin[1]) * Ra[1] + (a.k1 * a.kin[2]) * Ra[2];
+      double sn, cs;
+      sincos(ph.x, &sn, &cs);
+      const double mag = exp(ph.y);
+      const zd e = {mag * cs, -mag * sn};
+      const zd E[3] = {e * a.ein[0], e * a.ein[1], e * a.ein[2]};
+      const zd H[3] = {(E[2] * a.kin[1] - E[1] * a.kin[2]) * (1.0 / a.c0my),
+                       (E[0] * a.kin[2] - E[2] * a.kin[0]) * (1.0 / a.c0my),
+                       (E[1] * a.kin[0] - E[0] * a.kin[1]) * (1.0 / a.c0my)};
+      const double *nf = f.nqf;
+      const zd nE[3] = {E[2] * nf[1] - E[1] * nf[2], E[0] * nf[2] - E[2] * nf[0],
+                        E[1] * nf[0] - E[0] * nf[1]};
+      const zd nH[3] = {H[2] * nf[1] - H[1] * nf[2], H[0] * nf[2] - H[2] * nf[0],
+                        H[1] * nf[0] - H[0] * nf[1]};
+      const double wa = a.qw[i] * f.area;
+      S4[0] = S4[0] + (E[0] * rfm[0] + E[1] * rfm[1] + E[2] * rfm[2]) * wa;
+      S4[1] = S4[1] + (H[0] * rfm[0] + H[1] * rfm[1] + H[2] * rfm[2]) * wa;
+      S4[2] = S4[2] + (nE[0] * rfm[0] + nE[1] * rfm[1] + nE[2] * rfm[2]) * wa;
+      S4[3] = S4[3] + (nH[0] * rfm[0] + nH[1] * rfm[1] + nH[2] * rfm[2]) * wa;
+    }
+  }
+  be = bh = zd{0, 0};
+  for (int q = 0; q < 4; ++q) {
+    be = be + a.ca[q] * S4[q];
+    bh = bh + a.cb[q] * S4[q];
+  }
+}
+
+// per (node, affected edge): nu_m^T (b_m(+dz) - b_m(-dz))
+__global__ void k_adj_rhs(AdjArgs G, RhsArgs a, const zd *nu, zd *out) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= G.nslot) return;
+  const int pn = G.node[G.aff_node[g]];
+  const int m = G.aff_edge[g];
+  FnRec fp[2], fq[2];
+  for (int q = 0; q < 2; ++q) {
+    fp[q] = fn_rebuild(G.pp, G.side_nodes + 6 * (2 * m + q), q, G.normal_convention, pn, G.dz);
+    fq[q] = fn_rebuild(G.pp, G.side_nodes + 6 * (2 * m + q), q, G.normal_convention, pn, -G.dz);
+  }
+  zd bpe, bph, bme, bmh;
+  rhs_edge(a, fp, bpe, bph);
+  rhs_edge(a, fq, bme, bmh);
+  out[g] = nu[m] * (bpe - bme) + nu[m + G.N] * (bph - bmh);
+}
+
+// E at ra of unit J (Aj) and unit M (Am) on one side (k_scattered's
+// integrand; the charge term of E is dropped as there)
+__device__ void scat_side(const ScatArgs &a, const FnRec &f, const double *ra, zd *Aj, zd *Am) {
+  const double dfm = f.sign * f.length;
+  for (int k = 0; k < 3; ++k) Aj[k] = Am[k] = zd{0, 0};
+  for (int i = 0; i < 3; ++i) {
+    const double w0 = 1 - a.qa[i] - a.qb[i];
+    double fm[3], d[3];
+    for (int k = 0; k < 3; ++k) {
+      const double rq = w0 * f.c[k] + a.qa[i] * f.c[3 + k] + a.qb[i] * f.c[6 + k];
+      fm[k] = 0.5 * dfm * (rq - f.c[6 + k]);
+      d[k] = ra[k] - rq;
+    }
+    const double R = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]), rinv = 1.0 / R;
+    zd Gf, DG;
+    green(a.k1, R, rinv, Gf, DG);
+    const zd gG[3] = {DG * (d[0] * rinv), DG * (d[1] * rinv), DG * (d[2] * rinv)};
+    const zd cx[3] = {fm[1] * gG[2] - fm[2] * gG[1], fm[2] * gG[0] - fm[0] * gG[2],
+                      fm[0] * gG[1] - fm[1] * gG[0]};
+    for (int k = 0; k < 3; ++k) {
+      Aj[k] = Aj[k] - (a.iwmy * fm[k] * Gf) * a.qw[i];
+      Am[k] = Am[k] + cx[k] * a.qw[i];
+    }
+  }
+}
+
+// c = F^T (w conj(E)): c_m = sum_p w_p A_pm . conj(E_p), one workgroup per edge
+__global__ void __launch_bounds__(256) k_scattered_t(ScatArgs a, const FnRec *fn, const double *r,
+                                                     const double *w, const zd *E, zd *c) {
+  const int m = blockIdx.x;
+  zd ce = {0, 0}, ch = {0, 0};
+  for (int p = threadIdx.x; p < a.npts; p += blockDim.x) {
+    const double ra[3] = {r[3 * p], r[3 * p + 1], r[3 * p + 2]};
+    const zd Ep[3] = {conj(E[3 * p]) * w[p], conj(E[3 * p + 1]) * w[p], conj(E[3 * p + 2]) * w[p]};
+    for (int t = 0; t < 2; ++t) {
+      zd Aj[3], Am[3];
+      scat_side(a, fn[2 * m + t], ra, Aj, Am);
+      ce = ce + Aj[0] * Ep[0] + Aj[1] * Ep[1] + Aj[2] * Ep[2];
+      ch = ch + Am[0] * Ep[0] + Am[1] * Ep[1] + Am[2] * Ep[2];
+    }
+  }
+  ce = block_sum(ce);
+  ch = block_sum(ch);
+  if (threadIdx.x == 0) {
+    c[m] = ce;
+    c[m + a.N] = ch;
+  }
+}
+
+// per (node, affected edge m): sum_p w_p conj(E_p) . (E_p,m(+dz) - E_p,m(-dz)),
+// E_p,m the field of m's currents I_m at point p
+__global__ void __launch_bounds__(256) k_adj_field(AdjArgs G, ScatArgs a, const double *r,
+                                                   const double *w, const zd *E, const zd *I,
+                                                   zd *out) {
+  const int g = blockIdx.x;
+  const int pn = G.node[G.aff_node[g]];
+  const int m = G.aff_edge[g];
+  FnRec fp[2], fq[2];
+  for (int q = 0; q < 2; ++q) {
+    fp[q] = fn_rebuild(G.pp, G.side_nodes + 6 * (2 * m + q), q, G.normal_convention, pn, G.dz);
+    fq[q] = fn_rebuild(G.pp, G.side_nodes + 6 * (2 * m + q), q, G.normal_convention, pn, -G.dz);
+  }
+  const zd ie = I[m], ih = I[m + G.N];
+  zd acc = {0, 0};
+  for (int p = threadIdx.x; p < a.npts; p += blockDim.x) {
+    const double ra[3] = {r[3 * p], r[3 * p + 1], r[3 * p + 2]};
+    for (int t = 0; t < 2; ++t) {
+      zd Ajp[3], Amp[3], Ajm[3], Amm[3];
+      scat_side(a, fp[t], ra, Ajp, Amp);
+      scat_side(a, fq[t], ra, Ajm, Amm);
+      for (int k = 0; k < 3; ++k)
+        acc = acc + conj(E[3 * p + k]) * w[p] * ((Ajp[k] - Ajm[k]) * ie + (Amp[k] - Amm[k]) * ih);
+    }
+  }
+  acc = block_sum(acc);
+  if (threadIdx.x == 0) out[g] = acc;
+}
+
+void launch_adj_near(const NearArgs &A, const AdjArgs &G, int groups, const zd *nu, const zd *I,
+                     zd *partial, hipStream_t st) {
+  if (G.nnode <= 0) return;
+  hipLaunchKernelGGL(k_adj_near, dim3(groups, G.nnode), dim3(256), 0, st, A, G, nu, I, partial);
+}
+void launch_adj_far(const AdjArgs &G, const AdjFar &F, const zd *nu, const zd *I, const zd *W,
+                    const zd *Bt, zd *out, hipStream_t st) {
+  if (G.nslot <= 0) return;
+  hipLaunchKernelGGL(k_adj_far, dim3(G.nslot), dim3(256), 0, st, G, F, nu, I, W, Bt, out);
+}
+void launch_adj_rhs(const AdjArgs &G, const RhsArgs &a, const zd *nu, zd *out, hipStream_t st) {
+  if (G.nslot <= 0) return;
+  hipLaunchKernelGGL(k_adj_rhs, dim3(nblk(G.nslot, 64)), dim3(64), 0, st, G, a, nu, out);
+}
+void launch_scattered_t(const ScatArgs &a, const FnRec *fn, const double *r, const double *w,
+                        const zd *E, zd *c, hipStream_t st) {
+  if (a.N <= 0) return;
+  hipLaunchKernelGGL(k_scattered_t, dim3(a.N), dim3(256), 0, st, a, fn, r, w, E, c);
+}
+void launch_adj_field(const AdjArgs &G, const ScatArgs &a, const double *r, const double *w,
+                      const zd *E, const zd *I, zd *out, hipStream_t st) {
+  if (G.nslot <= 0 || a.npts <= 0) return;
+  hipLaunchKernelGGL(k_adj_field, dim3(G.nslot), dim3(256), 0, st, G, a, r, w, E, I, out);
 }
 
 }  // namespace spk

@@ -182,6 +182,44 @@ class SieTriMlfmm:
     def get_init_vector_x_tri(self, x_initial=0.0):
         return np.full(2 * self.N, complex(x_initial), dtype=np.complex128)
 
+    # -- the transposed operator (the adjoint system) ------------------------
+    def set_transposed(self, transposed=True):
+        """Every later matvec and solve uses Z^T (or Z again)."""
+        nat.check(nat.lib().spk_set_transposed(self._h, int(bool(transposed))))
+        return self
+
+    @property
+    def transposed(self):
+        t = C.c_int()
+        nat.check(nat.lib().spk_get_transposed(self._h, C.byref(t)))
+        return bool(t.value)
+
+    # -- adjoint gradient with respect to node heights ------------------------
+    def adjoint_height_gradient(self, k_in, e_in, points, weights, nodes, dz=1e-6,
+                                tolerance=1e-10, max_iterations=650, near_field=True,
+                                inner_iterations=0):
+        """J = sum_p weights[p] |E(points[p])|^2 of the plane-wave solution
+        and dJ/dz of the mesh nodes ``nodes`` (0-based), by one forward and
+        one transposed solve and the nodes' central differences (dz in
+        metres).  Returns (J, grad, (forward info, adjoint info))."""
+        pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+        w = np.ascontiguousarray(weights, dtype=np.float64).reshape(-1)
+        if w.shape[0] != pts.shape[0]:
+            raise ValueError("one weight per point")
+        nd = np.ascontiguousarray(nodes, dtype=np.int32).reshape(-1)
+        k = np.ascontiguousarray(k_in, dtype=np.float64)
+        e = np.ascontiguousarray(e_in, dtype=np.float64)
+        gp = _gmres_params(max_iterations, 0, tolerance, "ICGS", False, 0, 0.0, False, near_field,
+                           inner_iterations)
+        J = np.zeros(1)
+        grad = np.zeros(max(nd.size, 1))
+        infos = (nat.GmresInfo * 2)()
+        nat.check(nat.lib().spk_adjoint_height_gradient(
+            self._h, _dp(k), _dp(e), pts.shape[0], _dp(pts), _dp(w), C.byref(gp), nd.size,
+            nd.ctypes.data_as(C.POINTER(C.c_int)), float(dz), _dp(J), _dp(grad), infos))
+        info = tuple(dict(info=i.info, iters=i.iterations, be=i.backward_error) for i in infos)
+        return float(J[0]), grad[:nd.size], info
+
     # -- Integration_scattered_field_tri -------------------------------------
     def scattered_field(self, points, vector_x):
         """Scattered (E, H) at points (m) in the exterior medium from the
@@ -374,3 +412,42 @@ class SieTriMlfmm:
         out = np.zeros(count * 8 * S)
         nat.check(nat.lib().spk_get_patterns(self._h, first, count, _dp(out)))
         return (out[0::2] + 1j * out[1::2]).reshape(count, 4, S)
+
+
+def ensemble_intensity(wavelength, eps_r2, meshes, angles, points, eps_r1=1.0 + 0.0j,
+                       formulation="ICTF", normal_convention=2, tree_s_opt=250,
+                       truncation_number=19, precision="mixed", device=0, tolerance=1e-6,
+                       max_iterations=650, inner_iterations=0, workers=3):
+    """A speckle ensemble (spk_ensemble_intensity): every mesh (pp, tt) in
+    ``meshes`` is solved for every plane wave (k_in, e_in) in ``angles`` by
+    the near-field preconditioned flexible GMRES, in batches of up to 16, and
+    the scattered intensity sum |E_c|^2 is evaluated at ``points`` (npts, 3).
+    ``workers`` host threads with their own handles take the meshes in turn.
+    Returns (intensity (n_mesh, n_angles, npts), info dicts (n_mesh, n_angles))."""
+    nat.check_single_hip_runtime()
+    f = FORMULATIONS[formulation] if isinstance(formulation, str) else int(formulation)
+    prec = {"fp64": 0, "mixed": 1}[precision]
+    eps_r1, eps_r2 = complex(eps_r1), complex(eps_r2)
+    params = nat.Params(wavelength, eps_r1.real, eps_r1.imag, eps_r2.real, eps_r2.imag, f,
+                        normal_convention, 3, tree_s_opt, truncation_number, prec, device)
+    keep = []
+    ms = (nat.Mesh * len(meshes))()
+    for i, (pp, tt) in enumerate(meshes):
+        pp = np.ascontiguousarray(pp, dtype=np.float64).reshape(-1, 3)
+        tt = np.ascontiguousarray(tt, dtype=np.int32).reshape(-1, 3)
+        keep += [pp, tt]
+        ms[i] = nat.Mesh(pp.shape[0], _dp(pp), tt.shape[0], tt.ctypes.data_as(C.POINTER(C.c_int)))
+    kin = np.ascontiguousarray([a[0] for a in angles], dtype=np.float64).reshape(-1, 3)
+    ein = np.ascontiguousarray([a[1] for a in angles], dtype=np.float64).reshape(-1, 3)
+    pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+    nm, na, npts = len(meshes), kin.shape[0], pts.shape[0]
+    out = np.zeros((nm, na, npts))
+    infos = (nat.GmresInfo * max(1, nm * na))()
+    gp = _gmres_params(max_iterations, 0, tolerance, "ICGS", False, 0, 0.0, False, True,
+                       inner_iterations)
+    nat.check(nat.lib().spk_ensemble_intensity(C.byref(params), nm, ms, na, _dp(kin), _dp(ein),
+                                               npts, _dp(pts), C.byref(gp), int(workers),
+                                               _dp(out), infos))
+    info = [[dict(info=infos[m * na + a].info, iters=infos[m * na + a].iterations,
+                  be=infos[m * na + a].backward_error) for a in range(na)] for m in range(nm)]
+    return out, info

@@ -159,3 +159,46 @@ def test_near_field_fgmres_batch(solvers, reference, precision, R, monkeypatch):
         assert res <= 1e-6 and abs(res - inf["be"]) < 1e-9, (r, res, inf["be"])
     _, b0, x0, _ = reference[0]
     assert rel(xs[0].download(), x0) < 3e-5
+
+
+@pytest.mark.parametrize("R", [1, 5, 12])
+def test_inner_solve_graph_replays_equal_eager(solvers, R, monkeypatch):
+    """The inner near-field solve is captured once as a HIP graph and
+    replayed at every outer step (19 replays a solve, with the outer step's
+    eager launches and host synchronisations between them): the solutions
+    are bit-identical to launching the inner steps one by one, on two solves
+    in a row (the second reuses the captured graph) and after the handle is
+    set up again for another mesh and back (the graph is rebuilt)."""
+    s = solvers["mixed"]
+    angles = np.linspace(0.0, 50.0, R)
+
+    def solve():
+        nb = 2 * s.N * 16
+        bs, xs = [], []
+        for a in angles:
+            k, e = plane_wave(np.deg2rad(a), 0.0, 2, "TM")
+            bs.append(DeviceBuffer(nb).upload(s.get_vector_b_tri(k, e)))
+            xs.append(DeviceBuffer(nb))
+        infos = s.gmres_run_batch_device([d.ptr for d in bs], [d.ptr for d in xs],
+                                         max_iterations=650, tolerance=1e-6,
+                                         use_initial_guess=False, near_field=True)
+        return [x.download() for x in xs], [i["iters"] for i in infos]
+
+    monkeypatch.setenv("SPK_INNER_GRAPH", "0")
+    x_eager, it_eager = solve()
+    monkeypatch.delenv("SPK_INNER_GRAPH")
+    for _ in range(2):
+        x_g, it_g = solve()
+        assert it_g == it_eager
+        for a, b in zip(x_g, x_eager):
+            assert np.array_equal(a, b)
+    pp2, tt2 = meshes.rough_surface(30, 30, 3.0, 3.0, 0.1, 0.5, 2, LAM)
+    s.precalculation_tri(pp2, tt2)
+    _, it_other = solve()
+    assert all(i < 60 for i in it_other)
+    pp, tt = meshes.rough_surface(40, 40, 5.0, 5.0, 0.1, 0.5, 1, LAM)
+    s.precalculation_tri(pp, tt)
+    x_back, it_back = solve()
+    assert it_back == it_eager
+    for a, b in zip(x_back, x_eager):
+        assert np.array_equal(a, b)
