@@ -312,12 +312,15 @@ int spk_scattered_field_device_noalloc(spk_handle *h, int npts, const double *r,
  * patterns, right-hand-side entries and detector fields it changes,
  * contracted with nu, I and the leaf fields: dJ = 2 Re[E^H W dF I +
  * nu^T (db - dZ I)].  Near blocks keep the singular / regular branch of the
- * unmoved mesh.  J is returned too; info[2] (may be NULL): the two solves.
- * No counterpart in the reference. */
+ * unmoved mesh.  J is returned too; terms (may be NULL): 4 * n_nodes
+ * values, node k's gradient split into the detector-field, right-hand-side,
+ * near-block and far-field shares (their sum is grad[k]); info[2] (may be
+ * NULL): the two solves.  No counterpart in the reference. */
 int spk_adjoint_height_gradient(spk_handle *h, const double k_in[3], const double e_in[3],
                                 int npts, const double *r, const double *w,
                                 const spk_gmres_params *gp, int n_nodes, const int *nodes,
-                                double dz, double *J, double *grad, spk_gmres_info *info);
+                                double dz, double *J, double *grad, double *terms,
+                                spk_gmres_info *info);
 
 /* lib_math_solver_gmres_run (src/lib/libmath/src/math/lib_math_solver_GMRES.f90:159-379)
  * with the GPU matvec; x holds the initial guess on entry. */

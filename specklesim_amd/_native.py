@@ -162,7 +162,7 @@ def lib():
     L.spk_set_transposed.argtypes = [vp, C.c_int]
     L.spk_set_inner_graph.argtypes = [vp, C.c_int]
     L.spk_adjoint_height_gradient.argtypes = [vp, dp, dp, C.c_int, dp, dp, C.POINTER(GmresParams),
-                                              C.c_int, ip, C.c_double, dp, dp,
+                                              C.c_int, ip, C.c_double, dp, dp, dp,
                                               C.POINTER(GmresInfo)]
     L.spk_get_transposed.argtypes = [vp, C.POINTER(C.c_int)]
     L.spk_gmres_batch_device.argtypes = [vp, C.c_int, vpp, vpp, C.POINTER(GmresParams),

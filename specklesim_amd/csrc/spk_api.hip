@@ -2651,7 +2651,8 @@ int spk_gmres(spk_handle *h, const double *b, double *x, const spk_gmres_params 
 int spk_adjoint_height_gradient(spk_handle *h, const double k_in[3], const double e_in[3],
                                 int npts, const double *r, const double *w,
                                 const spk_gmres_params *gp, int n_nodes, const int *nodes,
-                                double dz, double *J, double *grad, spk_gmres_info *info) {
+                                double dz, double *J, double *grad, double *terms,
+                                spk_gmres_info *info) {
   if (!h || !k_in || !e_in || !r || !w || !gp || !J || npts < 1 || n_nodes < 0 ||
       (n_nodes > 0 && (!nodes || !grad)) || !(dz > 0))
     return fail(SPK_ERR_ARG, "bad argument");
@@ -2859,6 +2860,19 @@ int spk_adjoint_height_gradient(spk_handle *h, const double k_in[3], const doubl
       tf += zc(hq[q].x, hq[q].y) + zc(hr[q].x, hr[q].y);
     }
     grad[k] = 2.0 * (tf - tz).real() / (2.0 * dz);
+    if (terms) {
+      zc tn(0.0, 0.0), tfar(0.0, 0.0), tfld(0.0, 0.0), trhs(0.0, 0.0);
+      for (int g = 0; g < kGroups; ++g) tn += zc(hn[(size_t)k * kGroups + g].x, hn[(size_t)k * kGroups + g].y);
+      for (int q = aoff[k]; q < aoff[k + 1]; ++q) {
+        tfar += zc(hf[q].x, hf[q].y);
+        tfld += zc(hq[q].x, hq[q].y);
+        trhs += zc(hr[q].x, hr[q].y);
+      }
+      terms[4 * k + 0] = tfld.real() / dz;
+      terms[4 * k + 1] = trhs.real() / dz;
+      terms[4 * k + 2] = -tn.real() / dz;
+      terms[4 * k + 3] = -tfar.real() / dz;
+    }
   }
   return SPK_OK;
 }

@@ -197,11 +197,13 @@ class SieTriMlfmm:
     # -- adjoint gradient with respect to node heights ------------------------
     def adjoint_height_gradient(self, k_in, e_in, points, weights, nodes, dz=1e-6,
                                 tolerance=1e-10, max_iterations=650, near_field=True,
-                                inner_iterations=0):
+                                inner_iterations=0, return_terms=False):
         """J = sum_p weights[p] |E(points[p])|^2 of the plane-wave solution
         and dJ/dz of the mesh nodes ``nodes`` (0-based), by one forward and
         one transposed solve and the nodes' central differences (dz in
-        metres).  Returns (J, grad, (forward info, adjoint info))."""
+        metres).  Returns (J, grad, (forward info, adjoint info)), and with
+        return_terms an (n, 4) array of each node's detector-field,
+        right-hand-side, near-block and far-field shares of grad."""
         pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
         w = np.ascontiguousarray(weights, dtype=np.float64).reshape(-1)
         if w.shape[0] != pts.shape[0]:
@@ -213,11 +215,15 @@ class SieTriMlfmm:
                            inner_iterations)
         J = np.zeros(1)
         grad = np.zeros(max(nd.size, 1))
+        terms = np.zeros((max(nd.size, 1), 4))
         infos = (nat.GmresInfo * 2)()
         nat.check(nat.lib().spk_adjoint_height_gradient(
             self._h, _dp(k), _dp(e), pts.shape[0], _dp(pts), _dp(w), C.byref(gp), nd.size,
-            nd.ctypes.data_as(C.POINTER(C.c_int)), float(dz), _dp(J), _dp(grad), infos))
+            nd.ctypes.data_as(C.POINTER(C.c_int)), float(dz), _dp(J), _dp(grad), _dp(terms),
+            infos))
         info = tuple(dict(info=i.info, iters=i.iterations, be=i.backward_error) for i in infos)
+        if return_terms:
+            return float(J[0]), grad[:nd.size], info, terms[:nd.size]
         return float(J[0]), grad[:nd.size], info
 
     # -- Integration_scattered_field_tri -------------------------------------
