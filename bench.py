@@ -248,6 +248,91 @@ def timed_ensemble(args, lam):
     }
 
 
+def timed_adjoint(args, lam, pp, tt):
+    """C5: dJ/dz of every node height of the benchmark mesh, J the sum of
+    |E|^2 over a 5 x 5 detector window of +-5 degrees around the specular
+    direction (normal incidence, TM, points at 1000 lambda), by one forward
+    and one transposed near-field preconditioned solve (tolerance 1e-8) and
+    the per-node shares (local central differences of step 1e-5 lambda) on a
+    SPK_MIXED handle; the FP64 handle's gradient as
+    the reference; central differences of J at --adjoint-nodes seeded
+    interior nodes from FP64 solves to 1e-11, at a step of 1e-5 lambda (the
+    check) and of 1e-4 lambda (the step the C5 definition names; J is not
+    smooth enough at that scale for a 1e-3 check, see the record)."""
+    from specklesim_amd import SieTriMlfmm, plane_wave
+    from specklesim_amd.adjoint import HeightIntensity, detector_window
+
+    k, e = plane_wave(0.0, 0.0, 2, "TM")
+    pts = detector_window(0.0, 5.0, 5, 1e3 * lam)
+    z0 = pp[:, 2].copy()
+    nodes = np.arange(pp.shape[0])
+    kw = dict(formulation=args.formulation, normal_convention=2)
+
+    def problem(prec, tol):
+        s = SieTriMlfmm(lam, args.eps_r2, precision=prec, **kw)
+        return HeightIntensity(s, pp, tt, k, e, pts, tolerance=tol, dz=1e-5 * lam)
+
+    pm = problem("mixed", 1e-8)
+    pm.gradient(z0, nodes[:8])  # warm-up
+    runs = []
+    for _ in range(max(1, args.steps)):
+        _sync()
+        t0 = time.perf_counter()
+        J, g, info = pm.gradient(z0, nodes)
+        _sync()
+        runs.append(time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    _, g64, info64 = problem("fp64", 1e-8).gradient(z0, nodes)
+    _sync()
+    t64 = time.perf_counter() - t0
+    # central differences at seeded interior nodes
+    x, y = pp[:, 0], pp[:, 1]
+    inner = np.nonzero((np.abs(x) < 0.99 * np.abs(x).max()) &
+                       (np.abs(y) < 0.99 * np.abs(y).max()))[0]
+    fdn = np.sort(np.random.default_rng(7).choice(inner, args.adjoint_nodes, replace=False))
+    pf = problem("fp64", 1e-11)
+    pf.value(z0)
+    sing0 = pf.s.stats()["singular_pairs"]
+
+    def central(h):
+        fd, flips = [], 0
+        t0 = time.perf_counter()
+        for i in fdn:
+            zp, zm = z0.copy(), z0.copy()
+            zp[i] += h
+            zm[i] -= h
+            Jp = pf.value(zp)
+            flips += int(pf.s.stats()["singular_pairs"] != sing0)
+            Jm = pf.value(zm)
+            flips += int(pf.s.stats()["singular_pairs"] != sing0)
+            fd.append((Jp - Jm) / (2 * h))
+        fd = np.array(fd)
+        return {"nodes": int(len(fdn)), "step_lambda": h / lam,
+                "max_err_of_max_fd": float(np.max(np.abs(g[fdn] - fd)) / np.max(np.abs(fd))),
+                "branch_switches_in_stencils": flips, "forward_solves": "FP64 handle, to 1e-11",
+                "seconds": time.perf_counter() - t0}
+
+    fd5, fd4 = central(1e-5 * lam), central(1e-4 * lam)
+    ms = np.array(runs) * 1e3
+    return {
+        "workload": f"C5: the benchmark mesh's {len(nodes)} node heights, J = sum |E|^2 over a "
+                    f"5 x 5 detector window of +-5 deg around specular (normal incidence, TM), "
+                    f"{args.formulation}; forward + transposed near-field preconditioned "
+                    f"solves to 1e-8 and the per-node shares (spk_adjoint_height_gradient) on "
+                    f"a mixed handle",
+        "design_variables": len(nodes), "adjoint_step_lambda": 1e-5,
+        "gradient_ms": float(np.median(ms)),
+        "gradient_step_ms": {"min": float(ms.min()), "median": float(np.median(ms)),
+                             "max": float(ms.max())},
+        "solve_iterations": [info[0]["iters"], info[1]["iters"]],
+        "J": J,
+        "fp64_gradient_ms": t64 * 1e3,
+        "rel_err_vs_fp64_gradient": float(np.linalg.norm(g - g64) / np.linalg.norm(g64)),
+        "fd_check": fd5,
+        "fd_at_1e-4_lambda": fd4,
+    }
+
+
 def brdf_curve(res, lam):
     """BRDF of a solution in the plane of incidence, 181 angles in
     (-80, 80) degrees on a sphere of 1000 wavelengths."""
@@ -330,6 +415,8 @@ def main():
                     help="inner near-field GMRES steps per flexible GMRES step")
     ap.add_argument("--ensemble", type=int, default=16,
                     help="realisations of the speckle-ensemble object (0: skip it)")
+    ap.add_argument("--adjoint-nodes", type=int, default=32,
+                    help="nodes of the adjoint object's finite-difference check (0: skip it)")
     ap.add_argument("--workers", type=int, default=3,
                     help="worker threads (handles) of the ensemble")
     args = ap.parse_args()
@@ -368,6 +455,8 @@ def main():
         finally:
             del os.environ["SPK_INNER_GRAPH"]
     ens = timed_ensemble(args, lam) if args.ensemble and args.precision == "both" else None
+    adj = (timed_adjoint(args, lam, pp, tt) if args.adjoint_nodes and args.precision == "both"
+           else None)
     ff_err = None
     if "fp64" in paths and "mixed" in paths:
         b64, b32 = brdf_curve(paths["fp64"], lam), brdf_curve(paths["mixed"], lam)
@@ -508,6 +597,8 @@ def main():
         }
     if ens is not None:
         rec["ensemble"] = ens
+    if adj is not None:
+        rec["adjoint"] = adj
     if not args.no_cpu_baseline:
         it64 = (paths["fp64"] if "fp64" in paths else H)["runs"][-1]["iters"]
         cb = cpu_baseline(args, lam, pp, tt, it64)

@@ -2264,35 +2264,55 @@ __device__ inline bool in_list(const int *v, int n, int x) {
   return false;
 }
 
-// near block (m, n) from the given side records, with the branch choices
-// (singular test, self element, edge adjacency) of the unmoved geometry
-__device__ void block_frozen(const NearArgs &A, int m, int n, const FnRec *fm2, const FnRec *fn2,
-                             zd *D) {
+// The pair (m side t, n side s) of a near block from the given side
+// records, with the branch choice (singular test, self element, edge
+// adjacency) of the unmoved geometry; zz as assemble_zz gives it.
+__device__ void pair_frozen(const NearArgs &A, int m, int n, int t, int s, const FnRec &fa,
+                            const FnRec &fb, zd *zz) {
   const double rr = 1.5 * A.rref[m];
+  const FnRec &bm = A.fn[2 * m + t], &bq = A.fn[2 * n + s];
+  const double d[3] = {bm.mid[0] - bq.mid[0], bm.mid[1] - bq.mid[1], bm.mid[2] - bq.mid[2]};
+  const ElemRec ea = A.er[2 * m + t], eb = A.er[2 * n + s];
+  Rel rv[2];
+  if (!(len3(d) > rr)) {
+    const bool iqs = ea.elem == eb.elem;
+    singular_pair(fa, fb, iqs, A, rv);
+    if (A.edge_swap && !iqs && edge_adj(ea, eb)) {
+      zd kt[2];
+      edge_adjacent_kt(fa, fb, A, kt);
+      rv[0].Kt = kt[0];
+      rv[1].Kt = kt[1];
+    }
+  } else {
+    normal_pair<true>(fa, fb, A, rv);
+  }
+  assemble_zz(A, rv, zz);
+}
+
+// Z_mn(+dz) - Z_mn(-dz).  The block is linear in its four triangle pairs'
+// zz (accumulate_D, finish_D), so the difference is accumulated from the
+// pairs' differences; a pair with neither triangle holding the moved node
+// (moved_m[t], moved_n[s] false) contributes an exact zero and is skipped.
+__device__ void block_frozen_diff(const NearArgs &A, int m, int n, const FnRec *fmp,
+                                  const FnRec *fmm, const FnRec *fnp, const FnRec *fnm,
+                                  const bool *moved_m, const bool *moved_n, zd *D) {
   D[0] = D[1] = D[2] = D[3] = zd{0, 0};
   for (int p = 0; p < 4; ++p) {
     const int t = p >> 1, s = p & 1;
-    const FnRec &bm = A.fn[2 * m + t], &bq = A.fn[2 * n + s];
-    const double d[3] = {bm.mid[0] - bq.mid[0], bm.mid[1] - bq.mid[1], bm.mid[2] - bq.mid[2]};
-    const ElemRec ea = A.er[2 * m + t], eb = A.er[2 * n + s];
-    Rel rv[2];
-    if (!(len3(d) > rr)) {
-      const bool iqs = ea.elem == eb.elem;
-      singular_pair(fm2[t], fn2[s], iqs, A, rv);
-      if (A.edge_swap && !iqs && edge_adj(ea, eb)) {
-        zd kt[2];
-        edge_adjacent_kt(fm2[t], fn2[s], A, kt);
-        rv[0].Kt = kt[0];
-        rv[1].Kt = kt[1];
-      }
-    } else {
-      normal_pair<true>(fm2[t], fn2[s], A, rv);
+    zd dz[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    if (moved_m[t] || moved_n[s]) {
+      zd zp[4], zm[4];
+      pair_frozen(A, m, n, t, s, fmp[t], fnp[s], zp);
+      pair_frozen(A, m, n, t, s, fmm[t], fnm[s], zm);
+      for (int k = 0; k < 4; ++k) dz[k] = zp[k] - zm[k];
     }
-    zd z[4];
-    assemble_zz(A, rv, z);
-    accumulate_D(A, D, z);
+    accumulate_D(A, D, dz);
   }
   finish_D(A, D);
+}
+
+__device__ inline bool side_holds(const int *sn, int pn) {
+  return sn[3] == pn || sn[4] == pn || sn[5] == pn;
 }
 
 // sum over the requested nodes' near tasks of nu_m^T (Z_mn(+dz) - Z_mn(-dz)) I_n:
@@ -2328,21 +2348,29 @@ __global__ void __launch_bounds__(256) k_adj_near(NearArgs A, AdjArgs G, const z
       n = ea;
     }
     const bool am = in_list(aff, ns, m), an = in_list(aff, ns, n);
-    zd Dp[4], Dm[4];
-    for (int sgn = 0; sgn < 2; ++sgn) {
-      const double dz = sgn ? -G.dz : G.dz;
-      FnRec fm2[2], fn2[2];
-      for (int q = 0; q < 2; ++q) {
-        fm2[q] = am ? fn_rebuild(G.pp, G.side_nodes + 6 * (2 * m + q), q, G.normal_convention, pn, dz)
-                    : A.fn[2 * m + q];
-        fn2[q] = an ? fn_rebuild(G.pp, G.side_nodes + 6 * (2 * n + q), q, G.normal_convention, pn, dz)
-                    : A.fn[2 * n + q];
+    FnRec fmp[2], fmm[2], fnp[2], fnm[2];
+    bool mvm[2], mvn[2];
+    for (int q = 0; q < 2; ++q) {
+      const int *snm = G.side_nodes + 6 * (2 * m + q), *snn = G.side_nodes + 6 * (2 * n + q);
+      mvm[q] = am && side_holds(snm, pn);
+      mvn[q] = an && side_holds(snn, pn);
+      if (mvm[q]) {
+        fmp[q] = fn_rebuild(G.pp, snm, q, G.normal_convention, pn, G.dz);
+        fmm[q] = fn_rebuild(G.pp, snm, q, G.normal_convention, pn, -G.dz);
+      } else {
+        fmp[q] = fmm[q] = A.fn[2 * m + q];
       }
-      block_frozen(A, m, n, fm2, fn2, sgn ? Dm : Dp);
+      if (mvn[q]) {
+        fnp[q] = fn_rebuild(G.pp, snn, q, G.normal_convention, pn, G.dz);
+        fnm[q] = fn_rebuild(G.pp, snn, q, G.normal_convention, pn, -G.dz);
+      } else {
+        fnp[q] = fnm[q] = A.fn[2 * n + q];
+      }
     }
-    const zd d0 = Dp[0] - Dm[0], d1 = Dp[1] - Dm[1], d2 = Dp[2] - Dm[2], d3 = Dp[3] - Dm[3];
+    zd d[4];
+    block_frozen_diff(A, m, n, fmp, fmm, fnp, fnm, mvm, mvn, d);
     const zd ie = I[n], ih = I[n + G.N];
-    acc = acc + nu[m] * (d0 * ie + d1 * ih) + nu[m + G.N] * (d2 * ie + d3 * ih);
+    acc = acc + nu[m] * (d[0] * ie + d[1] * ih) + nu[m + G.N] * (d[2] * ie + d[3] * ih);
   }
   acc = block_sum(acc);
   if (threadIdx.x == 0) partial[(size_t)k * gridDim.x + blockIdx.x] = acc;

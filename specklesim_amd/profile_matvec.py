@@ -74,6 +74,9 @@ def main(argv=None):
     ap.add_argument("--near-bf", action="store_true")
     ap.add_argument("--near-f8", action="store_true")
     ap.add_argument("--solve", default="", choices=["", "near", "mixed", "fp64"])
+    ap.add_argument("--adjoint", action="store_true",
+                    help="one adjoint gradient over every node height (detector window of "
+                         "+-5 deg around specular, solves to 1e-8)")
     args = ap.parse_args(argv)
     if args.copy_mib:
         print(json.dumps(copy_calibration(args.copy_mib)), flush=True)
@@ -87,7 +90,24 @@ def main(argv=None):
     s = SieTriMlfmm(lam, args.eps_r2, formulation=args.formulation, normal_convention=2,
                     tree_s_opt=250, truncation_number=19,
                     precision=("fp64" if args.solve == "fp64" else "mixed")
-                    if (args.batch or args.near_bf or args.near_f8 or args.solve) else args.precision)
+                    if (args.batch or args.near_bf or args.near_f8 or args.solve or args.adjoint)
+                    else args.precision)
+    if args.adjoint:
+        import time
+
+        import numpy as np
+
+        from . import plane_wave
+        from .adjoint import HeightIntensity, detector_window
+
+        k, e = plane_wave(0.0, 0.0, 2, "TM")
+        prob = HeightIntensity(s, pp, tt, k, e, detector_window(0.0, 5.0, 5, 1e3 * lam),
+                               tolerance=1e-8, dz=1e-6 * lam)
+        t0 = time.perf_counter()
+        J, _, info = prob.gradient(pp[:, 2], np.arange(pp.shape[0]))
+        print(json.dumps({"adjoint": pp.shape[0], "J": J, "info": info,
+                          "seconds": time.perf_counter() - t0}), flush=True)
+        return
     s.precalculation_tri(pp, tt)
     if args.batch:
         print(json.dumps(batched_solve(s, args.batch)), flush=True)
