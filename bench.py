@@ -464,8 +464,12 @@ def main():
     # far-field (BRDF) error of every path against the FP64 path's solution
     ff = {k: float(np.linalg.norm(brdf_curve(v, lam) - b64) / np.linalg.norm(b64))
           for k, v in paths.items() if k != "fp64"} if "fp64" in paths else {}
+    # the headline is the fastest product path within the far-field bound;
+    # near_fgmres_eager (the same solver with its inner solve launched
+    # kernel by kernel) is a diagnostic variant, not a candidate
     ok = {k: v for k, v in paths.items()
-          if k == "fp64" or (ff.get(k, 1.0) <= 1e-4 and v["runs"][-1]["info"] == 0)}
+          if k == "fp64" or (not k.endswith("_eager") and ff.get(k, 1.0) <= 1e-4
+                             and v["runs"][-1]["info"] == 0)}
     head = max(ok, key=lambda k: ok[k]["solves_per_s"]) if ok else next(iter(paths))
     H = paths[head]
     runs, elapsed, ms_per_step = H["runs"], H["elapsed"], H["ms_per_step"]
