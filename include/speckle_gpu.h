@@ -189,7 +189,10 @@ typedef struct spk_mesh {
  * realisation's host precalculation overlaps another's GPU work.  No
  * counterpart in the reference, whose angle loop (main.f90:450-463) solves
  * one realisation and one angle at a time; a failure stops the ensemble and
- * names the realisation. */
+ * names the realisation.  gp->inner_iterations < 0 (near-field
+ * preconditioned solves) first tunes on realisation 0 (spk_autotune with
+ * its first min(16, n_angles) angles) and applies the result to every
+ * worker's handle. */
 int spk_ensemble_intensity(const spk_params *params, int n_mesh, const spk_mesh *meshes,
                            int n_angles, const double *k_in, const double *e_in, int npts,
                            const double *r, const spk_gmres_params *gp, int n_workers,
@@ -321,6 +324,28 @@ int spk_adjoint_height_gradient(spk_handle *h, const double k_in[3], const doubl
                                 const spk_gmres_params *gp, int n_nodes, const int *nodes,
                                 double dz, double *J, double *grad, double *terms,
                                 spk_gmres_info *info);
+
+/* Run-time tuning of the near-field preconditioned solver on the handle's
+ * current mesh (no counterpart in the reference).  Times the batched fp8
+ * near product with 1, 2 and 4 row tiles per workgroup (SPK_MIXED handles;
+ * 16 vectors, HIP events) and one batched solve of the R incidences
+ * (k_in[3R], e_in[3R]) with gp for each inner length 20, 25, 30, 36, 40;
+ * keeps the fastest of each in the handle (used while
+ * spk_gmres_params.inner_iterations is 0, and across spk_setup_tri calls).
+ * out reports the times (inner_ms < 0: that length did not converge). */
+typedef struct spk_tuning {
+  int tiles_per_group;   /* chosen: 1, 2 or 4 */
+  int inner_iterations;  /* chosen inner near-field GMRES steps */
+  double tile_us[3];     /* the batched fp8 product with 1, 2, 4 tiles (0: not timed) */
+  int n_inner;           /* inner lengths tried */
+  int inner_tried[8];
+  double inner_ms[8];    /* one batched solve with each */
+} spk_tuning;
+int spk_autotune(spk_handle *h, int R, const double *k_in, const double *e_in,
+                 const spk_gmres_params *gp, spk_tuning *out);
+/* Applies tuned values (e.g. spk_autotune's on another handle of a similar
+ * mesh); 0 restores a default.  No counterpart in the reference. */
+int spk_set_tuning(spk_handle *h, int tiles_per_group, int inner_iterations);
 
 /* lib_math_solver_gmres_run (src/lib/libmath/src/math/lib_math_solver_GMRES.f90:159-379)
  * with the GPU matvec; x holds the initial guess on entry. */

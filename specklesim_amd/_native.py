@@ -18,6 +18,7 @@ EXPORTS = [
     "spk_create", "spk_destroy", "spk_last_error", "spk_setup_tri", "spk_get_stats",
     "spk_num_edges", "spk_matvec", "spk_matvec_device", "spk_set_transposed",
     "spk_get_transposed", "spk_set_inner_graph", "spk_adjoint_height_gradient",
+    "spk_autotune", "spk_set_tuning",
     "spk_matvec_c64",
     "spk_matvec_c64_device", "spk_matvec_parts", "spk_rhs_plane_wave",
     "spk_rhs_plane_wave_device", "spk_scattered_field", "spk_scattered_field_device",
@@ -65,6 +66,12 @@ class GmresInfo(C.Structure):
 class Mesh(C.Structure):
     _fields_ = [("np", C.c_int), ("pp", C.POINTER(C.c_double)), ("nt", C.c_int),
                 ("tt", C.POINTER(C.c_int))]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("tiles_per_group", C.c_int), ("inner_iterations", C.c_int),
+                ("tile_us", C.c_double * 3), ("n_inner", C.c_int), ("inner_tried", C.c_int * 8),
+                ("inner_ms", C.c_double * 8)]
 
 
 class Stats(C.Structure):
@@ -161,6 +168,8 @@ def lib():
     L.spk_near_product_lowp.argtypes = [vp, C.c_int, C.c_int, C.c_int, vpp, vpp]
     L.spk_set_transposed.argtypes = [vp, C.c_int]
     L.spk_set_inner_graph.argtypes = [vp, C.c_int]
+    L.spk_autotune.argtypes = [vp, C.c_int, dp, dp, C.POINTER(GmresParams), C.POINTER(Tuning)]
+    L.spk_set_tuning.argtypes = [vp, C.c_int, C.c_int]
     L.spk_adjoint_height_gradient.argtypes = [vp, dp, dp, C.c_int, dp, dp, C.POINTER(GmresParams),
                                               C.c_int, ip, C.c_double, dp, dp, dp,
                                               C.POINTER(GmresInfo)]

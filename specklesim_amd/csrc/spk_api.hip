@@ -150,6 +150,10 @@ struct spk_handle {
   DBuf<int> d_row_box, d_row_r0, d_disagg_task;
   DBuf<int> d_mx_box, d_mx_r0;  // groups of row tiles of a box (batched fp8 product)
   int mx_groups = 0;
+  int mx_tpw = 1;  // row tiles per group of that map
+  // tuned launch parameters (spk_autotune, spk_set_tuning; 0: the defaults),
+  // kept across spk_setup_tri calls (an ensemble's meshes are alike)
+  int tune_tpw = 0, tune_inner = 0;
   int disagg_tasks = 0;
   // batched matvec: disaggregation tiles (box, first position), buffers
   DBuf<int> d_tile_box, d_tile_p0;
@@ -618,7 +622,7 @@ int matvec_batch_t(spk_handle *h, int R, const T *const *xs, T *const *ys, bool 
                              h->d_mx_box.p, h->d_mx_r0.p, h->d_box_first.p,
                              h->d_box_count.p, h->d_e2_off.p, h->d_e2_list.p, h->d_near_off.p,
                              h->d_perm.p, blocks_bf, f8_scale, (const zf *)BB.XI.p, BB.XB.p, ys,
-                             st);
+                             h->mx_tpw, st);
         HIPCHK(hipGetLastError());
         return SPK_OK;
       }
@@ -753,6 +757,24 @@ int spk_create(const spk_params *params, spk_handle **out) {
 
 void spk_destroy(spk_handle *h) { delete h; }
 
+// the map of groups of tpw 16-row tiles of one leaf box (the batched fp8
+// near product's workgroups)
+static int build_mx_groups(spk_handle *h, int tpw) {
+  const Tree &t = h->tree;
+  const int grp_rows = 16 * tpw;
+  std::vector<int> gb, g0;
+  for (size_t b = 0; b < t.box_count.size(); ++b)
+    for (int r = 0; r < t.box_count[b]; r += grp_rows) {
+      gb.push_back((int)b);
+      g0.push_back(r);
+    }
+  h->mx_groups = (int)gb.size();
+  h->mx_tpw = tpw;
+  HIPCHK(h->d_mx_box.upload(gb));
+  HIPCHK(h->d_mx_r0.upload(g0));
+  return SPK_OK;
+}
+
 int spk_setup_tri(spk_handle *h, int np, const double *pp, int nt, const int *tt) {
   if (!h || !pp || !tt || np < 3 || nt < 1) return fail(SPK_ERR_ARG, "bad mesh arguments");
   HIPCHK(hipSetDevice(h->p.device));
@@ -801,18 +823,8 @@ int spk_setup_tri(spk_handle *h, int np, const double *pp, int nt, const int *tt
   h->spmv_wgs = (int)rb.size();
   HIPCHK(h->d_row_box.upload(rb));
   HIPCHK(h->d_row_r0.upload(r0));
-  // groups of near_mx_tiles_per_group() 16-row tiles of one box (the
-  // batched fp8 product)
-  const int grp_rows = 16 * near_mx_tiles_per_group();
-  std::vector<int> gb, g0;
-  for (size_t b = 0; b < t.box_count.size(); ++b)
-    for (int r = 0; r < t.box_count[b]; r += grp_rows) {
-      gb.push_back((int)b);
-      g0.push_back(r);
-    }
-  h->mx_groups = (int)gb.size();
-  HIPCHK(h->d_mx_box.upload(gb));
-  HIPCHK(h->d_mx_r0.upload(g0));
+  if ((rc = build_mx_groups(h, h->tune_tpw > 0 ? h->tune_tpw : near_mx_tiles_per_group())))
+    return rc;
   // disaggregation tasks: up to kDisaggEdges consecutive positions of one box
   if (h->N >= (1 << 28)) return fail(SPK_ERR_ARG, "more than 2^28 RWG edges");
   std::vector<int> dt;
@@ -1827,7 +1839,9 @@ static int fgmres_near(spk_handle *h, int R, const zd *const *b, zd *const *x,
   const int64_t n = 2 * (int64_t)h->N;
   const int max_it = std::max(1, gp->max_iterations);
   const int m = std::min(max_it, gp->restart > 0 ? gp->restart : 100);
-  const int kin = gp->inner_iterations > 0 ? std::min(gp->inner_iterations, 64) : 30;
+  const int kin = gp->inner_iterations > 0 ? std::min(gp->inner_iterations, 64)
+                  : h->tune_inner > 0 ? h->tune_inner
+                                      : 30;
   const double tol = gp->tolerance;
   // relaxed outer products on a mixed handle (SPK_RELAX=0 keeps every one in FP64)
   const char *erx = getenv("SPK_RELAX");
@@ -2315,6 +2329,137 @@ int spk_gmres_batch_device(spk_handle *h, int R, const void *const *d_b, void *c
   if (h->p.precision != SPK_MIXED)
     return fail(SPK_ERR_STATE, "the batched refinement solve runs on an SPK_MIXED handle");
   return gmres_mixed_batch(h, R, (const zd *const *)d_b, (zd *const *)d_x, gp, info);
+}
+
+// Run-time tuning of the near-field preconditioned solver on the handle's
+// mesh: the batched fp8 near product's row tiles per workgroup (1, 2, 4;
+// each timed over 20 launches with HIP events at 16 vectors) and the inner
+// solve length (20, 25, 30, 36, 40; each timed as one batched solve of the
+// R incidences with gp).  The fastest of each is kept in the handle.
+int spk_autotune(spk_handle *h, int R, const double *k_in, const double *e_in,
+                 const spk_gmres_params *gp, spk_tuning *out) {
+  if (!h || !k_in || !e_in || !gp || !out) return fail(SPK_ERR_ARG, "null argument");
+  if (!h->ready) return fail(SPK_ERR_STATE, "spk_setup_tri has not completed");
+  if (R < 1 || R > kBatchMax) return fail(SPK_ERR_ARG, "batch size must be 1..16");
+  if (h->transposed) return fail(SPK_ERR_STATE, "the handle holds the transposed operator");
+  HIPCHK(hipSetDevice(h->p.device));
+  *out = spk_tuning{};
+  const int64_t n = 2 * (int64_t)h->N;
+  int rc = SPK_OK;
+  // row tiles of the batched fp8 product (mixed handles: the only ones
+  // whose preconditioner runs it)
+  const int tiles[3] = {1, 2, 4};
+  int best_tpw = h->tune_tpw > 0 ? h->tune_tpw : near_mx_tiles_per_group();
+  if (h->mixed_ready) {
+    if ((rc = ensure_near_f8(h))) return rc;
+    constexpr int kR = 16;
+    if ((rc = batch_ensure<zf>(h, kR))) return rc;
+    DBuf<zf> X, Y;
+    {
+      std::vector<zf> hx((size_t)kR * n);
+      for (size_t i = 0; i < hx.size(); ++i)
+        hx[i] = zf{(float)std::cos(0.37 * (double)i), (float)std::sin(0.59 * (double)i)};
+      HIPCHK(X.upload(hx));
+    }
+    HIPCHK(Y.alloc((size_t)kR * n));
+    std::vector<const zf *> xp(kR);
+    std::vector<zf *> yp(kR);
+    for (int r = 0; r < kR; ++r) {
+      xp[r] = X.p + r * n;
+      yp[r] = Y.p + r * n;
+    }
+    BatchBufs<zf> &BB = h->bf;
+    HIPCHK(hipMemcpy(BB.xs.p, xp.data(), kR * sizeof(void *), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(BB.ys.p, yp.data(), kR * sizeof(void *), hipMemcpyHostToDevice));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    double best = 1e300;
+    for (int i = 0; i < 3 && !rc; ++i) {
+      if ((rc = build_mx_groups(h, tiles[i]))) break;
+      constexpr int kWarm = 3, kReps = 20;
+      for (int k = 0; k < kWarm + kReps && !rc; ++k) {
+        if (k == kWarm) (void)hipEventRecord(e0, h->st);
+        rc = matvec_batch_t<zf>(h, kR, BB.xs.p, BB.ys.p, false, h->d_blk_f8.p, h->d_f8_scale.p);
+      }
+      if (rc) break;
+      (void)hipEventRecord(e1, h->st);
+      if (hipEventSynchronize(e1) != hipSuccess) {
+        rc = fail(SPK_ERR_HIP, "event synchronisation failed");
+        break;
+      }
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      out->tile_us[i] = 1e3 * ms / kReps;
+      if (out->tile_us[i] < best) {
+        best = out->tile_us[i];
+        best_tpw = tiles[i];
+      }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    h->tune_tpw = best_tpw;
+    if ((rc = build_mx_groups(h, best_tpw))) return rc;
+  }
+  out->tiles_per_group = best_tpw;
+  // inner solve length: one batched preconditioned solve per candidate
+  std::vector<DBuf<zd>> B(R), X(R);
+  std::vector<const void *> bp(R);
+  std::vector<void *> xp(R);
+  for (int r = 0; r < R; ++r) {
+    HIPCHK(B[r].alloc(n));
+    HIPCHK(X[r].alloc(n));
+    if ((rc = rhs_plane_dev(h, k_in + 3 * r, e_in + 3 * r, B[r].p))) return rc;
+    bp[r] = B[r].p;
+    xp[r] = X[r].p;
+  }
+  spk_gmres_params g = *gp;
+  g.preconditioning = SPK_PRECOND_NEAR_FIELD;
+  g.use_initial_guess = 0;
+  std::vector<spk_gmres_info> inf(R);
+  const int cand[5] = {20, 25, 30, 36, 40};
+  g.inner_iterations = 30;
+  if ((rc = spk_gmres_batch_device(h, R, bp.data(), xp.data(), &g, inf.data()))) return rc;
+  double best = 1e300;
+  int best_in = 30;
+  for (int i = 0; i < 5; ++i) {
+    g.inner_iterations = cand[i];
+    const double t0 = now_ms();
+    if ((rc = spk_gmres_batch_device(h, R, bp.data(), xp.data(), &g, inf.data()))) return rc;
+    const double ms = now_ms() - t0;
+    bool conv = true;
+    for (int r = 0; r < R; ++r) conv = conv && inf[r].info == 0;
+    out->inner_tried[i] = cand[i];
+    out->inner_ms[i] = conv ? ms : -ms;
+    if (conv && ms < best) {
+      best = ms;
+      best_in = cand[i];
+    }
+  }
+  out->n_inner = 5;
+  h->tune_inner = best_in;
+  out->inner_iterations = best_in;
+  return SPK_OK;
+}
+
+// Applies tuned values (spk_autotune on another handle of a similar mesh):
+// row tiles per workgroup of the batched fp8 near product (1, 2 or 4) and
+// the inner solve length used when spk_gmres_params.inner_iterations is 0;
+// 0 restores a default.
+int spk_set_tuning(spk_handle *h, int tiles_per_group, int inner_iterations) {
+  if (!h) return fail(SPK_ERR_ARG, "null argument");
+  if (tiles_per_group != 0 && tiles_per_group != 1 && tiles_per_group != 2 && tiles_per_group != 4)
+    return fail(SPK_ERR_ARG, "tiles_per_group must be 0, 1, 2 or 4");
+  if (inner_iterations < 0 || inner_iterations > 64)
+    return fail(SPK_ERR_ARG, "inner_iterations must be 0..64");
+  h->tune_tpw = tiles_per_group;
+  h->tune_inner = inner_iterations;
+  if (h->ready) {
+    HIPCHK(hipSetDevice(h->p.device));
+    return build_mx_groups(h, tiles_per_group > 0 ? tiles_per_group : near_mx_tiles_per_group());
+  }
+  return SPK_OK;
 }
 
 int spk_gmres_device(spk_handle *h, const void *d_b, void *d_x, const spk_gmres_params *gp,

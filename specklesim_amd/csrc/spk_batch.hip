@@ -1490,13 +1490,12 @@ void launch_batch_near_f8(int nwg, int N, int R, const int *row_box, const int *
                           const int *box_first, const int *box_count, const int *e2_off,
                           const int *e2_list, const int64_t *near_off, const int *perm,
                           const uint32_t *blocks, const float *scale, const zf *XI,
-                          uint32_t *XB, zf *const *ys, hipStream_t st) {
+                          uint32_t *XB, zf *const *ys, int tpw, hipStream_t st) {
   if (nwg <= 0) return;
   const int64_t len = (int64_t)N * 2 * 16;
   hipLaunchKernelGGL(kb_x_to_bf, dim3(bnblk(len, 256)), dim3(256), 0, st, len, XI, XB);
-  // tpw row tiles per workgroup (near_mx_tiles_per_group); 1 runs the
-  // original kernel on the 16-row tile map
-  const int tpw = near_mx_tiles_per_group();
+  // tpw row tiles per workgroup (grp_box / grp_r0 are the map of groups of
+  // tpw 16-row tiles); 1 runs the original kernel on the 16-row tile map
   if (tpw == 1 || ngrp <= 0)
     hipLaunchKernelGGL(kb_near_mxf8<1>, dim3(nwg), dim3(256), 0, st, N, R, row_box, row_r0,
                        box_first, box_count, e2_off, e2_list, near_off, perm, (const uint2 *)blocks,
@@ -1512,7 +1511,9 @@ void launch_batch_near_f8(int nwg, int N, int R, const int *row_box, const int *
 }
 
 // Row tiles per workgroup of the batched fp8 near product: 1 by default,
-// SPK_NEAR_MXF8_TPW = 1, 2 or 4 to choose (read once per process).  On the
+// The process default of the row tiles per workgroup (a handle's tuned
+// value, spk_autotune, takes precedence); SPK_NEAR_MXF8_TPW = 1, 2 or 4 to
+// choose (read once per process).  On the
 // 10 x 10 lambda patch at 16 vectors: 82.9, 84.2 and 94.1 us, with HBM
 // traffic 1.75x, 1.28x and 1.2x the algorithmic bytes.
 int near_mx_tiles_per_group() {

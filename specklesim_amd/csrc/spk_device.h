@@ -310,7 +310,7 @@ void launch_batch_near_f8(int nwg, int N, int R, const int *row_box, const int *
                           const int *box_first, const int *box_count, const int *e2_off,
                           const int *e2_list, const int64_t *near_off, const int *perm,
                           const uint32_t *blocks, const float *scale, const zf *XI,
-                          uint32_t *XB, zf *const *ys, hipStream_t st);
+                          uint32_t *XB, zf *const *ys, int tpw, hipStream_t st);
 
 // one ICGS pass over P bases (coefficients into hs[r][0..ks[r]), |w_r|^2
 // into hs[r][ks[r]]); partial: batch_orth_partial_elems(n, P, maxk)

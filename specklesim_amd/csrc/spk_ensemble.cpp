@@ -111,6 +111,21 @@ extern "C" int spk_ensemble_intensity(const spk_params *params, int n_mesh, cons
     // synchronous copies (HIP ties them to the legacy stream)
     if ((rc = spk_set_inner_graph(w.h, 0))) return rc;
   }
+  // tuning on realisation 0 (gp->inner_iterations < 0), applied to every
+  // worker; the solves then use the tuned inner length
+  spk_gmres_params g = *gp;
+  if (gp->inner_iterations < 0) {
+    if (gp->preconditioning != SPK_PRECOND_NEAR_FIELD)
+      return spk::set_error(SPK_ERR_ARG, "tuning needs the near-field preconditioned solver");
+    g.inner_iterations = 0;
+    int rc = spk_setup_tri(workers[0].h, meshes[0].np, meshes[0].pp, meshes[0].nt, meshes[0].tt);
+    spk_tuning tu;
+    if (!rc) rc = spk_autotune(workers[0].h, std::min(kMaxBatch, n_angles), k_in, e_in, &g, &tu);
+    for (auto &w : workers)
+      if (!rc) rc = spk_set_tuning(w.h, tu.tiles_per_group, tu.inner_iterations);
+    if (rc) return spk::set_error(rc, std::string("tuning on realisation 0: ") + spk_last_error());
+  }
+  gp = &g;
   std::atomic<int> next(0);
   std::atomic<bool> stop(false);
   std::mutex mu;

@@ -194,6 +194,28 @@ class SieTriMlfmm:
         nat.check(nat.lib().spk_get_transposed(self._h, C.byref(t)))
         return bool(t.value)
 
+    # -- run-time tuning --------------------------------------------------------
+    def autotune(self, angles, tolerance=1e-6, max_iterations=650):
+        """Tunes the near-field preconditioned solver on the current mesh
+        (spk_autotune): the batched fp8 near product's row tiles per
+        workgroup and the inner solve length, timed on a batched solve of
+        ``angles`` (up to 16 (k_in, e_in) pairs).  The choice stays in the
+        handle; returns it with the measured times."""
+        kin = np.ascontiguousarray([a[0] for a in angles], dtype=np.float64).reshape(-1, 3)
+        ein = np.ascontiguousarray([a[1] for a in angles], dtype=np.float64).reshape(-1, 3)
+        gp = _gmres_params(max_iterations, 0, tolerance, "ICGS", False, 0, 0.0, False, True, 0)
+        t = nat.Tuning()
+        nat.check(nat.lib().spk_autotune(self._h, kin.shape[0], _dp(kin), _dp(ein), C.byref(gp),
+                                         C.byref(t)))
+        return dict(tiles_per_group=t.tiles_per_group, inner_iterations=t.inner_iterations,
+                    tile_us=list(t.tile_us),
+                    inner_ms={t.inner_tried[i]: t.inner_ms[i] for i in range(t.n_inner)})
+
+    def set_tuning(self, tiles_per_group=0, inner_iterations=0):
+        """Applies tuned values (0: the default)."""
+        nat.check(nat.lib().spk_set_tuning(self._h, int(tiles_per_group), int(inner_iterations)))
+        return self
+
     # -- adjoint gradient with respect to node heights ------------------------
     def adjoint_height_gradient(self, k_in, e_in, points, weights, nodes, dz=1e-6,
                                 tolerance=1e-10, max_iterations=650, near_field=True,
@@ -429,6 +451,8 @@ def ensemble_intensity(wavelength, eps_r2, meshes, angles, points, eps_r1=1.0 + 
     the near-field preconditioned flexible GMRES, in batches of up to 16, and
     the scattered intensity sum |E_c|^2 is evaluated at ``points`` (npts, 3).
     ``workers`` host threads with their own handles take the meshes in turn.
+    ``inner_iterations="tune"`` tunes on the first mesh first (spk_autotune)
+    and applies the result to every worker.
     Returns (intensity (n_mesh, n_angles, npts), info dicts (n_mesh, n_angles))."""
     nat.check_single_hip_runtime()
     f = FORMULATIONS[formulation] if isinstance(formulation, str) else int(formulation)
@@ -450,7 +474,7 @@ def ensemble_intensity(wavelength, eps_r2, meshes, angles, points, eps_r1=1.0 + 
     out = np.zeros((nm, na, npts))
     infos = (nat.GmresInfo * max(1, nm * na))()
     gp = _gmres_params(max_iterations, 0, tolerance, "ICGS", False, 0, 0.0, False, True,
-                       inner_iterations)
+                       -1 if inner_iterations == "tune" else inner_iterations)
     nat.check(nat.lib().spk_ensemble_intensity(C.byref(params), nm, ms, na, _dp(kin), _dp(ein),
                                                npts, _dp(pts), C.byref(gp), int(workers),
                                                _dp(out), infos))
