@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libspeckle_gpu.so")
 BUILD_DIR = os.path.join(HERE, "_build")
 
-HIP_SOURCES = ["spk_kernels.hip", "spk_api.hip", "spk_quad.hip", "spk_batch.hip"]
+HIP_SOURCES = ["spk_kernels.hip", "spk_api.hip", "spk_quad.hip", "spk_batch.hip", "spk_tree.hip"]
 CXX_SOURCES = ["spk_host.cpp", "spk_quad_host.cpp", "spk_ensemble.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -19,6 +19,8 @@
 #include <vector>
 
 #include "spk_device.h"
+#include "spk_dbuf.h"
+#include "spk_tree.h"
 
 using namespace spk;
 
@@ -46,59 +48,6 @@ double now_ms() {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch())
       .count();
 }
-
-// device buffer of n elements; alloc keeps the allocation when the request
-// fits its capacity (a handle set up again for a mesh of the same size
-// allocates nothing), so the contents after alloc are undefined either way
-template <class T>
-struct DBuf {
-  T *p = nullptr;
-  size_t n = 0, cap = 0;
-  DBuf() = default;
-  DBuf(const DBuf &) = delete;
-  DBuf &operator=(const DBuf &) = delete;
-  DBuf(DBuf &&o) noexcept : p(o.p), n(o.n), cap(o.cap) {
-    o.p = nullptr;
-    o.n = o.cap = 0;
-  }
-  DBuf &operator=(DBuf &&o) noexcept {
-    if (this != &o) {
-      reset();
-      p = o.p;
-      n = o.n;
-      cap = o.cap;
-      o.p = nullptr;
-      o.n = o.cap = 0;
-    }
-    return *this;
-  }
-  ~DBuf() { reset(); }
-  void reset() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    n = cap = 0;
-  }
-  hipError_t alloc(size_t count) {
-    if (p && count <= cap) {
-      n = count;
-      return hipSuccess;
-    }
-    reset();
-    if (count == 0) return hipSuccess;
-    const hipError_t e = hipMalloc(&p, count * sizeof(T));
-    if (e != hipSuccess) {
-      p = nullptr;
-      return e;
-    }
-    n = cap = count;
-    return hipSuccess;
-  }
-  hipError_t upload(const std::vector<T> &v) {
-    hipError_t e = alloc(v.size());
-    if (e != hipSuccess || v.empty()) return e;
-    return hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
-  }
-};
 
 // buffers of the batched matvec (R right-hand sides), one set per precision
 template <class T>
@@ -154,6 +103,7 @@ struct spk_handle {
   // tuned launch parameters (spk_autotune, spk_set_tuning; 0: the defaults),
   // kept across spk_setup_tri calls (an ensemble's meshes are alike)
   int tune_tpw = 0, tune_inner = 0;
+  TreeWork tw;  // scratch of the device octree build
   int disagg_tasks = 0;
   // batched matvec: disaggregation tiles (box, first position), buffers
   DBuf<int> d_tile_box, d_tile_p0;
@@ -793,27 +743,44 @@ int spk_setup_tri(spk_handle *h, int np, const double *pp, int nt, const int *tt
   if (rc != SPK_OK) return fail(rc, err);
   h->N = (int)h->rref.size();
   if (h->N < 1) return fail(SPK_ERR_MESH, "mesh has no interior edges (no RWG pairs)");
-  build_tree(h->fn, h->N, h->p.tree_s_opt, &h->tree);
+  // the octree and its lists: on the device (the host builder with
+  // SPK_HOST_TREE=1; both give the same lists)
+  const char *ht = std::getenv("SPK_HOST_TREE");
+  const bool host_tree = ht && ht[0] == '1';
+  HIPCHK(h->d_fn.upload(h->fn));
+  if (host_tree) {
+    build_tree(h->fn, h->N, h->p.tree_s_opt, &h->tree);
+  } else {
+    const TreeDevOut out{&h->d_perm,      &h->d_pos,       &h->d_leaf_box,
+                         &h->d_leafbox_of_pos, &h->d_box_first, &h->d_box_count,
+                         &h->d_e2_off,    &h->d_e2_list,   &h->d_near_off};
+    const char *what = "";
+    const hipError_t e =
+        build_tree_device(h->d_fn.p, h->N, h->p.tree_s_opt, &h->tree, h->tw, out, h->st, &what);
+    if (e != hipSuccess)
+      return fail(SPK_ERR_HIP, std::string("device octree (") + what + "): " + hipGetErrorString(e));
+  }
   make_tables(h->p.truncation_number, &h->tb);
   const Tree &t = h->tree;
   h->stats.tree_ms = now_ms() - t0;
   // upload topology
-  HIPCHK(h->d_fn.upload(h->fn));
   HIPCHK(h->d_er.upload(h->er));
   HIPCHK(h->d_rref.upload(h->rref));
-  HIPCHK(h->d_perm.upload(t.perm));
-  HIPCHK(h->d_box_first.upload(t.box_first));
-  HIPCHK(h->d_box_count.upload(t.box_count));
-  HIPCHK(h->d_e2_off.upload(t.e2_off));
-  HIPCHK(h->d_e2_list.upload(t.e2_list));
-  HIPCHK(h->d_near_off.upload(t.near_off));
-  HIPCHK(h->d_leaf_box.upload(t.leaf_box));
-  HIPCHK(h->d_pos.upload(t.pos));
+  if (host_tree) {
+    HIPCHK(h->d_perm.upload(t.perm));
+    HIPCHK(h->d_box_first.upload(t.box_first));
+    HIPCHK(h->d_box_count.upload(t.box_count));
+    HIPCHK(h->d_e2_off.upload(t.e2_off));
+    HIPCHK(h->d_e2_list.upload(t.e2_list));
+    HIPCHK(h->d_near_off.upload(t.near_off));
+    HIPCHK(h->d_leaf_box.upload(t.leaf_box));
+    HIPCHK(h->d_pos.upload(t.pos));
+    std::vector<int> lbp(h->N);
+    for (int p = 0; p < h->N; ++p) lbp[p] = t.leaf_box[t.perm[p]];
+    HIPCHK(h->d_leafbox_of_pos.upload(lbp));
+  }
   h->pp_host.assign(pp, pp + 3 * (size_t)np);
   h->transposed = false;
-  std::vector<int> lbp(h->N);
-  for (int p = 0; p < h->N; ++p) lbp[p] = t.leaf_box[t.perm[p]];
-  HIPCHK(h->d_leafbox_of_pos.upload(lbp));
   std::vector<int> rb, r0;
   for (size_t b = 0; b < t.box_count.size(); ++b)
     for (int r = 0; r < t.box_count[b]; r += kSpmvRowsPerWG) {
