@@ -56,6 +56,9 @@ def lib():
         L.spo_near_values.argtypes = [C.c_void_p, dp]
         L.spo_mlfmm_setup.argtypes = [C.c_void_p, C.c_int]
         L.spo_matvec.argtypes = [C.c_void_p, dp, dp, C.c_int]
+        L.spo_lagrange_interp.argtypes = [C.c_int, C.c_int, C.c_int, dp, dp]
+        L.spo_set_level_K.argtypes = [C.c_void_p, C.c_int, ip, C.c_int]
+        L.spo_mlfmm_setup_levels.argtypes = [C.c_void_p, C.c_int]
         L.spo_rhs_plane.argtypes = [C.c_void_p, dp, dp, dp]
         L.spo_scattered_field.argtypes = [C.c_void_p, C.c_int, dp, dp, dp, dp]
         L.spo_gmres.argtypes = [C.c_void_p, dp, dp, C.c_int, C.c_int, C.c_double, C.c_int,
@@ -210,11 +213,26 @@ class Oracle:
         lib().spo_dense_fill(self.h, _d(Z.view(np.float64)), nthreads)
         return Z
 
-    def mlfmm_setup(self, nthreads=0):
-        lib().spo_mlfmm_setup(self.h, nthreads)
+    def mlfmm_setup(self, nthreads=0, level_K=None, rr="adjoint"):
+        """Patterns and TL tables; level_K: truncation number per level
+        (index = level; 0 or missing = the default K), non-increasing from
+        the coarsest level to the leaves; rr: "reference" (RR interpolates
+        the local coefficients, as the reference's code would) or "adjoint"
+        (RR applies the transpose of SS's interpolation)."""
+        if level_K is not None:
+            k = np.ascontiguousarray(level_K, dtype=np.int32)
+            lib().spo_set_level_K(self.h, k.size, _i(k), {"reference": 0, "adjoint": 1}[rr])
+        if lib().spo_mlfmm_setup_levels(self.h, nthreads) != 0:
+            raise ValueError("a level's truncation number exceeds its parent's")
+        self.level_K = None if level_K is None else list(level_K)
 
     def pattern(self, m):
-        S = 2 * self.params.K * self.params.K + 2
+        K = self.params.K
+        if getattr(self, "level_K", None) is not None:
+            lmax = self.tree()["l_max"]
+            if lmax < len(self.level_K) and self.level_K[lmax] > 0:
+                K = self.level_K[lmax]
+        S = 2 * K * K + 2
         o = np.zeros(8 * S)
         lib().spo_get_pattern(self.h, m, _d(o))
         return (o[0::2] + 1j * o[1::2]).reshape(4, S)
@@ -262,6 +280,39 @@ class Oracle:
         info = lib().spo_gmres(self.h, _d(b.view(np.float64)), _d(x.view(np.float64)), maxit,
                                restart, tol, orth, nthreads, _d(hist), C.byref(it), _d(be))
         return x, dict(info=info, iters=it.value, be=be[0], hist=hist[:it.value])
+
+
+def lagrange_interp(values, Km, Kn, p=3):
+    """Lagrange_Interpolation_New (src/lib_sie_mlfmm.f90:268-390) for one
+    current: values (4, 2 Km^2 + 2) complex, channels (theta k1, theta k2,
+    phi k1, phi k2), the last two samples the Cartesian x / y components at
+    theta = 0 and pi; returns (4, 2 Kn^2 + 2)."""
+    v = np.ascontiguousarray(values, dtype=np.complex128).reshape(4, 2 * Km * Km + 2)
+    out = np.zeros((4, 2 * Kn * Kn + 2), np.complex128)
+    lib().spo_lagrange_interp(int(p), int(Km), int(Kn), _d(v.view(np.float64)),
+                              _d(out.view(np.float64)))
+    return out
+
+
+def sample_grid(K):
+    """Khat_and_TRF_ExtendedK (src/lib_sie_mlfmm.f90:190-249): (khat, theta
+    unit vectors, phi unit vectors, weights) of the 2K^2 + 2 samples."""
+    x, w = np.polynomial.legendre.leggauss(K)
+    S = 2 * K * K + 2
+    kh, tt, tp, trf = np.zeros((S, 3)), np.zeros((S, 3)), np.zeros((S, 3)), np.zeros(S)
+    dphi = np.pi / K
+    for j in range(K):
+        st = np.sin(np.arccos(x[j]))
+        for k in range(2 * K):
+            ph, i = k * dphi, k + j * 2 * K
+            kh[i] = [st * np.cos(ph), st * np.sin(ph), x[j]]
+            tt[i] = [x[j] * np.cos(ph), x[j] * np.sin(ph), -st]
+            tp[i] = [-np.sin(ph), np.cos(ph), 0.0]
+            trf[i] = w[j] * dphi
+    kh[S - 2], kh[S - 1] = [0, 0, 1], [0, 0, -1]
+    tt[S - 2] = tt[S - 1] = [1, 0, 0]
+    tp[S - 2] = tp[S - 1] = [0, 1, 0]
+    return kh, tt, tp, trf
 
 
 def gmres_dense(A, b, maxit, restart=0, tol=1e-6, orth=1):

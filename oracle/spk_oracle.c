@@ -165,8 +165,17 @@ struct spo_ctx {
   int S;                /* 2K^2+2 */
   double *khat, *ttheta, *tphi, *trf; /* S*3, S*3, S*3, S */
   zc *rt;               /* N * 4 * S : [m][k*2+comp][i] (comp 0 = a/theta) */
-  zc **tl;              /* per level: 343 * 2 * S */
+  zc **tl;              /* per level: 343 * 2 * S_l */
   int mlfmm_ready;
+  /* per-level truncation numbers (the reference's truncation_number(l),
+   * lib_sie_mlfmm.f90:631-701; all equal to p.K unless spo_set_level_K):
+   * requested values, and per level l_min..l_max: K, samples, grid */
+  int Kreq[32], nKreq;
+  int rr_adjoint;   /* RR to a smaller K: 0 the reference's interpolation of
+                       the local coefficients, 1 the transpose of SS's
+                       interpolation (anterpolation) */
+  int Kl[32], Sl[32];
+  double *khl[32], *trfl[32];
 };
 
 static void *xmalloc(size_t n) {
@@ -334,6 +343,7 @@ void spo_destroy(spo_ctx *c) {
   }
   free(c->ia); free(c->ja); free(c->za);
   free(c->khat); free(c->ttheta); free(c->tphi); free(c->trf); free(c->rt);
+  for (int l = 0; l < 32; ++l) { free(c->khl[l]); free(c->trfl[l]); }
   if (c->tl) {
     for (int l = 0; l <= c->l_max; ++l) free(c->tl[l]);
     free(c->tl);
@@ -1236,6 +1246,170 @@ static void khat_trf(int K, double *kh, double *tth, double *tph, double *trf) {
   free(xx); free(ww);
 }
 
+/* ------------------------------------------------------------------------ */
+/* Lagrange interpolation of a sampled far-field pattern between the sample  */
+/* grids of two truncation numbers: Lagrange_Interpolation_New and its       */
+/* helpers (src/lib_sie_mlfmm.f90:251-493, 573-610), restated loop by loop   */
+/* with the reference's 1-based arrays (macros below).  Default reals of the */
+/* reference are single precision: the stencil indices (it-1.0)/K_n*K_m are */
+/* evaluated in float.                                                       */
+/* ------------------------------------------------------------------------ */
+
+/* theta_phi_extended (:573-610): a_theta(1..3K+2) from 2pi down to -pi
+ * across both poles, a_phi(1..6K) from -2pi in steps of pi/K */
+static void theta_phi_ext(int Km, double *a_phi, double *a_theta) {
+  int m_ph = 3 * (2 * Km);
+  double d_phi = 6 * SPO_PI / m_ph;
+  double *xx = xmalloc(sizeof(double) * Km), *ww = xmalloc(sizeof(double) * Km);
+  spo_gauss_legendre(Km, xx, ww);
+#define AT(i) a_theta[(i) - 1]
+#define XX(i) xx[(i) - 1]
+  for (int j = 1; j <= Km; ++j) {
+    AT(j) = 2 * SPO_PI - acos(XX(Km - (j - 1)));
+    AT(j + Km + 1) = acos(XX(j));
+    AT(2 * Km + 2 + j) = -acos(XX(Km - (j - 1)));
+  }
+  AT(Km + 1) = SPO_PI;
+  AT(2 * Km + 2) = 0.0;
+#undef AT
+#undef XX
+  for (int k = 1; k <= m_ph; ++k) a_phi[k - 1] = -2 * SPO_PI + d_phi * (k - 1);
+  free(xx); free(ww);
+}
+
+/* Lagrange_Interpolation_Weights (:392-412): the 2p weights of x on the
+ * nodes ext(K_arr+t+1-p .. K_arr+t+p) */
+static void lagrange_w(double x, int t, int p, int K_arr, const double *ext, double *wk) {
+  double xx[16];
+  int kt = K_arr + t;
+  for (int j = 1; j <= 2 * p; ++j) xx[j - 1] = ext[kt + 1 - p + (j - 1) - 1];
+  for (int j = 1; j <= 2 * p; ++j) {
+    wk[j - 1] = 1.0;
+    for (int m = 1; m <= 2 * p; ++m)
+      if (m != j) wk[j - 1] = (x - xx[m - 1]) / (xx[j - 1] - xx[m - 1]) * wk[j - 1];
+  }
+}
+
+/* Sorting_theta_phi_LenReduced_Arr_New (:414-493): the K_m grid extended by
+ * p rows beyond each pole (reflection: theta -> -theta, phi -> phi + pi,
+ * both components negated), the two pole rows from the Cartesian pole
+ * samples, and p columns beyond 0 and 2pi (periodic).  ff_in(i, j, q):
+ * interior node i (ascending cos theta), phi index j, channel q; fxy[q2][4]:
+ * (fx, fy) at theta = 0 then at theta = pi for medium q2. */
+static void sort_ext(int p, int Km, const zc *ff_in, const zc fxy[2][4], zc *ff_out,
+                     const double *a_phi, const double *a_theta) {
+  int Kn = 2 * Km, n_phi = Kn + 2 * p + 1, n_theta = Km + 2 * p + 2;
+#define FO(m, n, q) ff_out[((size_t)((q) - 1) * n_theta + ((m) - 1)) * n_phi + ((n) - 1)]
+#define FI(i, j, q) ff_in[((size_t)((q) - 1) * Km + ((i) - 1)) * Kn + ((j) - 1)]
+#define AP(i) a_phi[(i) - 1]
+#define AT(i) a_theta[(i) - 1]
+  for (size_t i = 0; i < (size_t)4 * n_theta * n_phi; ++i) ff_out[i] = 0.0;
+  for (int m = 1; m <= n_phi; ++m) {
+    int n = m + Kn - p;
+    zc Mt[2] = {cos(AP(n)), sin(AP(n))};
+    zc Mp[2] = {-sin(AP(n)), cos(AP(n))};
+    for (int j = 1; j <= 2; ++j) {
+      const zc *f0 = &fxy[j - 1][0], *f1 = &fxy[j - 1][2];
+      FO(p + 1, m, j) = -(conj(Mt[0]) * f1[0] + conj(Mt[1]) * f1[1]);
+      FO(Km + p + 2, m, j) = conj(Mt[0]) * f0[0] + conj(Mt[1]) * f0[1];
+      FO(p + 1, m, j + 2) = conj(Mp[0]) * f1[0] + conj(Mp[1]) * f1[1];
+      FO(Km + p + 2, m, j + 2) = conj(Mp[0]) * f0[0] + conj(Mp[1]) * f0[1];
+    }
+  }
+  for (int q = 1; q <= 4; ++q)
+    for (int i = 1; i <= Km; ++i)
+      for (int j = 1; j <= Kn; ++j) FO(p + 1 + i, p + j, q) = FI(i, j, q);
+  for (int m = 1; m <= n_theta; ++m) {
+    double xt = AT(m - p + Km + 1);
+    for (int n = 1; n <= n_phi; ++n) {
+      double xp = AP(n - p + Kn);
+      int src = 0, dn = 0;
+      if (xt < 0.0) {
+        src = 2 * (Km + p + 2) - m;
+      } else if (xt >= SPO_PI) {
+        src = 2 * (p + 1) - m;
+      }
+      if (src) {
+        if (xp < SPO_PI && xp >= 0.0) dn = Kn / 2;
+        else if (xp < 2 * SPO_PI && xp >= SPO_PI) dn = -Kn / 2;
+        if (dn)
+          for (int q = 1; q <= 4; ++q) FO(m, n, q) = -FO(src, n + dn, q);
+      }
+    }
+  }
+  for (int m = 1; m <= n_theta; ++m)
+    for (int n = 1; n <= n_phi; ++n) {
+      double xp = AP(n - p + Kn);
+      if (xp >= 2 * SPO_PI) {
+        for (int q = 1; q <= 4; ++q) FO(m, n, q) = FO(m, n - Kn, q);
+      } else if (xp < 0) {
+        for (int q = 1; q <= 4; ++q) FO(m, n, q) = FO(m, n + Kn, q);
+      }
+    }
+#undef FO
+#undef FI
+#undef AP
+#undef AT
+}
+
+/* Lagrange_Interpolation_New (:268-390) for one current: in / out hold the
+ * four channels (theta k1, theta k2, phi k1, phi k2), each of 2K^2 + 2
+ * samples (the last two: the Cartesian x / y components at theta = 0 and
+ * pi, copied) as interleaved complex doubles. */
+void spo_lagrange_interp(int p, int Km, int Kn, const double *in, double *out) {
+  const zc *fin = (const zc *)in;
+  zc *fout = (zc *)out;
+  int Sm = 2 * Km * Km + 2, Sn = 2 * Kn * Kn + 2, Ns_m = 2 * Km * Km, Ns_n = 2 * Kn * Kn;
+  int n_phi = 2 * Kn, m_phi = 2 * Km;
+  zc *ff_in = xmalloc(sizeof(zc) * 4 * (size_t)Km * 2 * Km);
+  for (int q = 0; q < 4; ++q)
+    for (int i = 0; i < Km; ++i)
+      for (int j = 0; j < 2 * Km; ++j) ff_in[((size_t)q * Km + i) * 2 * Km + j] = fin[(size_t)q * Sm + i * 2 * Km + j];
+  /* fx_fy (:251-266): channel 1/2 hold x, 3/4 hold y at the poles */
+  zc fxy[2][4];
+  for (int j = 0; j < 2; ++j) {
+    fxy[j][0] = fin[(size_t)j * Sm + Ns_m];         /* fx, theta = 0 */
+    fxy[j][1] = fin[(size_t)(j + 2) * Sm + Ns_m];   /* fy, theta = 0 */
+    fxy[j][2] = fin[(size_t)j * Sm + Ns_m + 1];     /* fx, theta = pi */
+    fxy[j][3] = fin[(size_t)(j + 2) * Sm + Ns_m + 1];
+  }
+  double *a_phi = xmalloc(sizeof(double) * 6 * Km), *a_theta = xmalloc(sizeof(double) * (3 * Km + 2));
+  theta_phi_ext(Km, a_phi, a_theta);
+  int n_theta_e = Km + 2 * p + 2, n_phi_e = 2 * Km + 2 * p + 1;
+  zc *ff_out = xmalloc(sizeof(zc) * 4 * (size_t)n_theta_e * n_phi_e);
+  sort_ext(p, Km, ff_in, (const zc(*)[4])fxy, ff_out, a_phi, a_theta);
+  double *xt = xmalloc(sizeof(double) * Kn), *xw = xmalloc(sizeof(double) * Kn);
+  spo_gauss_legendre(Kn, xt, xw);
+  double wkt[16], wkp[16];
+  for (int it = 1; it <= Kn; ++it) {
+    double theta = acos(xt[it - 1]);
+    int t = (int)floorf((float)(it - 1.0f) / (float)Kn * (float)Km) + 1;
+    int kt = p + t;
+    lagrange_w(theta, t, p, Km, a_theta, wkt);
+    for (int jp = 1; jp <= n_phi; ++jp) {
+      int tt = (it - 1) * n_phi + jp;
+      zc acc[4] = {0, 0, 0, 0};
+      double phi = (jp - 1) * 2 * SPO_PI / n_phi;
+      int s = (int)floorf((float)(jp - 1.0f) / (float)Kn * (float)Km) + 1;
+      int kp = p + s;
+      lagrange_w(phi, s, p, m_phi, a_phi, wkp);
+      for (int i = 1; i <= 2 * p; ++i)
+        for (int j = 1; j <= 2 * p; ++j) {
+          double w = wkp[j - 1] * wkt[i - 1];
+          int m = kt + i - p, n = kp + j - p;
+          for (int q = 0; q < 4; ++q)
+            acc[q] = acc[q] + w * ff_out[((size_t)q * n_theta_e + (m - 1)) * n_phi_e + (n - 1)];
+        }
+      for (int q = 0; q < 4; ++q) fout[(size_t)q * Sn + tt - 1] = acc[q];
+    }
+  }
+  for (int q = 0; q < 4; ++q) {
+    fout[(size_t)q * Sn + Ns_n] = fin[(size_t)q * Sm + Ns_m];
+    fout[(size_t)q * Sn + Ns_n + 1] = fin[(size_t)q * Sm + Ns_m + 1];
+  }
+  free(ff_in); free(ff_out); free(a_phi); free(a_theta); free(xt); free(xw);
+}
+
 /* TL_km_ml_fmm: src/lib_sie_mlfmm.f90:129-188 (one medium). */
 static void tl_eval(int K, zc kd, const double *r, const double *kh, int S, zc *out) {
   double rl = vlen(r);
@@ -1326,11 +1500,13 @@ static void patterns(spo_ctx *c, int nthreads) {
  * src/lib_sie_mlfmm.f90:27-125, grid of lib_sie_lookup_table.f90:204-213,
  * 631-656.  Table slot = (ix*7 + iy)*7 + iz, offsets -3..3 per axis. */
 static void tl_tables(spo_ctx *c, int nthreads) {
-  int K = c->p.K, S = c->S;
   c->tl = xmalloc(sizeof(zc *) * (c->l_max + 1));
+  for (int l = 0; l <= c->l_max; ++l) c->tl[l] = NULL;
   int np_ = 7;
   for (int l = c->l_min; l <= c->l_max; ++l) {
     if (l < 1) continue;
+    const int K = c->Kl[l], S = c->Sl[l];
+    const double *khat = c->khl[l];
     c->tl[l] = xmalloc(sizeof(zc) * 343 * 2 * S);
     double d_c = (1.0 / (double)(1LL << l)) * c->D;
     double x_min = -((np_ - 1) / 2 * d_c);
@@ -1346,16 +1522,109 @@ static void tl_tables(spo_ctx *c, int nthreads) {
       zc *o = &c->tl[l][(size_t)t * 2 * S];
       if (i >= 2 && i <= 4 && j >= 2 && j <= 4 && k >= 2 && k <= 4) continue; /* E2 cube */
       double r[3] = {x_min + i * dx, x_min + j * dx, x_min + k * dx};
-      tl_eval(K, c->k1, r, c->khat, S, o);
-      if (!lossy_skip) tl_eval(K, c->k2, r, c->khat, S, o + S);
+      tl_eval(K, c->k1, r, khat, S, o);
+      if (!lossy_skip) tl_eval(K, c->k2, r, khat, S, o + S);
     }
   }
   (void)nthreads;
 }
 
-void spo_mlfmm_setup(spo_ctx *c, int nthreads) {
+/* Per-level truncation numbers for the next spo_mlfmm_setup: K[l] for
+ * levels l < n (0: the default p.K). */
+void spo_set_level_K(spo_ctx *c, int n, const int *K, int rr_adjoint) {
+  c->nKreq = n < 32 ? n : 32;
+  for (int l = 0; l < c->nKreq; ++l) c->Kreq[l] = K[l];
+  c->rr_adjoint = rr_adjoint;
+}
+
+/* The interpolation from the K_m to the K_n grid as a sparse matrix per
+ * medium: its columns are spo_lagrange_interp of unit samples (channel
+ * theta or phi of one medium, one sample), so rows / cols index
+ * (component * S + sample).  The transpose is the anterpolation. */
+typedef struct { int nnz; int *row, *col; double *val; } spmat;
+static void interp_matrix(int Km, int Kn, spmat *M) {
+  int Sm = 2 * Km * Km + 2, Sn = 2 * Kn * Kn + 2;
+  double *in = xmalloc(sizeof(double) * 2 * 4 * Sm), *out = xmalloc(sizeof(double) * 2 * 4 * Sn);
+  int cap = 1 << 16;
+  M->nnz = 0;
+  M->row = xmalloc(sizeof(int) * cap); M->col = xmalloc(sizeof(int) * cap);
+  M->val = xmalloc(sizeof(double) * cap);
+  memset(in, 0, sizeof(double) * 2 * 4 * Sm);
+  for (int cin = 0; cin < 2; ++cin)
+    for (int si = 0; si < Sm; ++si) {
+      int q = cin * 2; /* channel (theta or phi) of medium 1 */
+      in[2 * ((size_t)q * Sm + si)] = 1.0;
+      spo_lagrange_interp(3, Km, Kn, in, out);
+      in[2 * ((size_t)q * Sm + si)] = 0.0;
+      for (int cout = 0; cout < 2; ++cout)
+        for (int so = 0; so < Sn; ++so) {
+          double v = out[2 * ((size_t)(cout * 2) * Sn + so)];
+          if (v == 0.0) continue;
+          if (M->nnz == cap) {
+            cap *= 2;
+            M->row = realloc(M->row, sizeof(int) * cap); M->col = realloc(M->col, sizeof(int) * cap);
+            M->val = realloc(M->val, sizeof(double) * cap);
+          }
+          M->row[M->nnz] = cout * Sn + so; M->col[M->nnz] = cin * Sm + si; M->val[M->nnz] = v;
+          ++M->nnz;
+        }
+    }
+  free(in); free(out);
+}
+
+/* the same on folded coefficients W (items k*4 + {1a, 1b, 2a, 2b}: the
+ * (theta, phi) pairs (k*4, k*4+1) and (k*4+2, k*4+3) of medium k) */
+static void anterp_w(const spmat *M, int Km, int Kn, const zc *x, zc *y) {
+  int Sm = 2 * Km * Km + 2, Sn = 2 * Kn * Kn + 2;
+  for (int k = 0; k < 2; ++k)
+    for (int g = 0; g < 2; ++g) {
+      const int it_c[2] = {k * 4 + 2 * g, k * 4 + 2 * g + 1};
+      for (int e = 0; e < M->nnz; ++e) {
+        int co = M->row[e] / Sn, so = M->row[e] % Sn, ci = M->col[e] / Sm, si = M->col[e] % Sm;
+        y[(size_t)it_c[ci] * Sm + si] += M->val[e] * x[(size_t)it_c[co] * Sn + so];
+      }
+    }
+}
+
+/* y[items] (+)= M^T x[items] on the 8 coefficient items of a box, for the
+ * two currents and two media; M maps the K_m grid to the K_n grid, so the
+ * transpose maps K_n samples (x) to K_m samples (y) */
+static void anterp_items(const spmat *M, int Km, int Kn, const zc *x, zc *y) {
+  int Sm = 2 * Km * Km + 2, Sn = 2 * Kn * Kn + 2;
+  for (int cur = 0; cur < 2; ++cur)
+    for (int k = 0; k < 2; ++k) {
+      const int it_c[2] = {0 * 4 + cur * 2 + k, 1 * 4 + cur * 2 + k};
+      for (int e = 0; e < M->nnz; ++e) {
+        int co = M->row[e] / Sn, so = M->row[e] % Sn, ci = M->col[e] / Sm, si = M->col[e] % Sm;
+        y[(size_t)it_c[ci] * Sm + si] += M->val[e] * x[(size_t)it_c[co] * Sn + so];
+      }
+    }
+}
+
+int spo_mlfmm_setup_levels(spo_ctx *c, int nthreads);
+
+void spo_mlfmm_setup(spo_ctx *c, int nthreads) { (void)spo_mlfmm_setup_levels(c, nthreads); }
+
+/* returns 0, or -1 if a level's K exceeds its parent's (the reference's SS
+ * and RR interpolate only from a smaller to a larger K going up) */
+int spo_mlfmm_setup_levels(spo_ctx *c, int nthreads) {
   if (!c->tree_built) spo_tree_build(c);
-  int K = c->p.K;
+  for (int l = 0; l <= c->l_max && l < 32; ++l) {
+    c->Kl[l] = (l < c->nKreq && c->Kreq[l] > 0) ? c->Kreq[l] : c->p.K;
+    c->Sl[l] = 2 * c->Kl[l] * c->Kl[l] + 2;
+  }
+  for (int l = c->l_min; l < c->l_max; ++l)
+    if (c->Kl[l] < c->Kl[l + 1]) return -1;
+  for (int l = c->l_min; l <= c->l_max; ++l) {
+    free(c->khl[l]); free(c->trfl[l]);
+    const int S = c->Sl[l];
+    c->khl[l] = xmalloc(sizeof(double) * 3 * S);
+    c->trfl[l] = xmalloc(sizeof(double) * S);
+    double *tth = xmalloc(sizeof(double) * 3 * S), *tph = xmalloc(sizeof(double) * 3 * S);
+    khat_trf(c->Kl[l], c->khl[l], tth, tph, c->trfl[l]);
+    free(tth); free(tph);
+  }
+  int K = c->Kl[c->l_max];
   c->S = 2 * K * K + 2;
   int S = c->S;
   free(c->khat); free(c->ttheta); free(c->tphi); free(c->trf);
@@ -1367,6 +1636,22 @@ void spo_mlfmm_setup(spo_ctx *c, int nthreads) {
   patterns(c, nthreads);
   tl_tables(c, nthreads);
   c->mlfmm_ready = 1;
+  return 0;
+}
+
+/* Lagrange_Interpolation_New on the two currents of a box's 8 coefficient
+ * items (it = comp*4 + cur*2 + k) from the K_m to the K_n grid */
+static void interp_items(int Km, int Kn, const zc *in, zc *out) {
+  int Sm = 2 * Km * Km + 2, Sn = 2 * Kn * Kn + 2;
+  zc *a = xmalloc(sizeof(zc) * 4 * Sm), *b = xmalloc(sizeof(zc) * 4 * Sn);
+  for (int cur = 0; cur < 2; ++cur) {
+    const int items[4] = {0 * 4 + cur * 2 + 0, 0 * 4 + cur * 2 + 1, 1 * 4 + cur * 2 + 0,
+                          1 * 4 + cur * 2 + 1};
+    for (int q = 0; q < 4; ++q) memcpy(&a[(size_t)q * Sm], &in[(size_t)items[q] * Sm], sizeof(zc) * Sm);
+    spo_lagrange_interp(3, Km, Kn, (const double *)a, (double *)b);
+    for (int q = 0; q < 4; ++q) memcpy(&out[(size_t)items[q] * Sn], &b[(size_t)q * Sn], sizeof(zc) * Sn);
+  }
+  free(a); free(b);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1384,8 +1669,8 @@ void spo_matvec(spo_ctx *c, const double *xin, double *yout, int nthreads) {
   zc **B = xmalloc(sizeof(zc *) * (lmax + 1));
   zc **Dt = xmalloc(sizeof(zc *) * (lmax + 1));
   for (int l = lmin; l <= lmax; ++l) {
-    B[l] = xmalloc(sizeof(zc) * (size_t)c->lev[l].nbox * 8 * S);
-    Dt[l] = xmalloc(sizeof(zc) * (size_t)c->lev[l].nbox * 8 * S);
+    B[l] = xmalloc(sizeof(zc) * (size_t)c->lev[l].nbox * 8 * c->Sl[l]);
+    Dt[l] = xmalloc(sizeof(zc) * (size_t)c->lev[l].nbox * 8 * c->Sl[l]);
   }
   const level_t *LL = &c->lev[lmax];
   /* upward pass step 1: lib_sie_ml_fmm_get_c */
@@ -1408,16 +1693,22 @@ void spo_matvec(spo_ctx *c, const double *xin, double *yout, int nthreads) {
         }
     }
   }
-  /* upward pass step 2: translation SS (:466-528, equal-K branch) */
+  /* upward pass step 2: translation SS (:466-528): a child of a smaller K
+   * is interpolated to the parent's grid first (Lagrange_Interpolation_New,
+   * stencil 2p = 6), then shifted with the parent grid's directions */
   for (int l = lmax - 1; l >= lmin; --l) {
     const level_t *L = &c->lev[l], *Lc = &c->lev[l + 1];
+    const int Sp = c->Sl[l], Sc = c->Sl[l + 1];
+    const int interp = c->Kl[l] > c->Kl[l + 1];
+    const double *khp = c->khl[l];
 #ifdef _OPENMP
 #pragma omp parallel for schedule(dynamic, 1)
 #endif
     for (int b = 0; b < L->nbox; ++b) {
       double xp[3];
       box_centre(c, L->n[b], l, xp);
-      zc *P = &B[l][(size_t)b * 8 * S];
+      zc *P = &B[l][(size_t)b * 8 * Sp];
+      zc *Ci = interp ? xmalloc(sizeof(zc) * 8 * Sp) : NULL;
       for (int o = 0; o < 8; ++o) {
         int64_t cn = (L->n[b] << 3) | o;
         int cp = level_find(Lc, cn);
@@ -1425,14 +1716,20 @@ void spo_matvec(spo_ctx *c, const double *xin, double *yout, int nthreads) {
         double xc[3], r[3];
         box_centre(c, cn, l + 1, xc);
         vsub(xc, xp, r);
-        const zc *Cc = &B[l + 1][(size_t)cp * 8 * S];
+        const zc *Cc = &B[l + 1][(size_t)cp * 8 * Sc];
+        if (interp) {
+          interp_items(c->Kl[l + 1], c->Kl[l], Cc, Ci);
+          Cc = Ci;
+        }
+        const int S = Sp;
         for (int i = 0; i < S; ++i) {
-          double tmp = vdot(&c->khat[3 * i], r);
+          double tmp = vdot(&khp[3 * i], r);
           zc t[2] = {cexp(ZCMPLX(-cimag(c->k1) * tmp, creal(c->k1) * tmp)),
                      cexp(ZCMPLX(-cimag(c->k2) * tmp, creal(c->k2) * tmp))};
           for (int it = 0; it < 8; ++it) P[it * S + i] = P[it * S + i] + Cc[it * S + i] * t[it & 1];
         }
       }
+      free(Ci);
     }
   }
   /* downward pass step 1: translation SR over I4 (:547-586) */
@@ -1446,6 +1743,8 @@ void spo_matvec(spo_ctx *c, const double *xin, double *yout, int nthreads) {
       int n4 = spo_i4(c, l, L->n[a], lst, 216);
       int64_t xa[3];
       demorton3((uint64_t)L->n[a], l, xa);
+      const int S = c->Sl[l];
+      const double *trf = c->trfl[l];
       zc *Da = &Dt[l][(size_t)a * 8 * S];
       for (int q = 0; q < n4; ++q) {
         int bp = level_find(L, lst[q]);
@@ -1457,14 +1756,56 @@ void spo_matvec(spo_ctx *c, const double *xin, double *yout, int nthreads) {
         const zc *Bb = &B[l][(size_t)bp * 8 * S];
         for (int it = 0; it < 8; ++it) {
           const zc *Tk = T + (it & 1) * S;
-          for (int i = 0; i < S; ++i) Da[it * S + i] = Da[it * S + i] + Bb[it * S + i] * c->trf[i] * Tk[i];
+          for (int i = 0; i < S; ++i) Da[it * S + i] = Da[it * S + i] + Bb[it * S + i] * trf[i] * Tk[i];
         }
       }
     }
   }
-  /* downward pass step 2: translation RR (:600-655) */
+  /* downward pass step 2: translation RR (:600-655): to a child of a
+   * smaller K the parent's coefficients are shifted on the parent's grid,
+   * then interpolated to the child's grid (Lagrange_Interpolation_New), or
+   * (rr_adjoint) mapped by the transpose of SS's child-to-parent
+   * interpolation */
+  spmat Mup[32];
+  memset(Mup, 0, sizeof Mup);
+  int multik = 0;
+  for (int l = lmin + 1; l <= lmax; ++l) multik |= c->Kl[l] != c->Kl[l - 1];
+  const int wspace = multik && c->rr_adjoint;
+  zc eps_d[2] = {c->eps1 * I * c->omega, c->eps2 * I * c->omega};
+  zc my_d[2] = {c->my1 * I * c->omega, c->my2 * I * c->omega};
+  zc kdv[2] = {c->k1 * I, c->k2 * I};
+  if (wspace) {
+    /* anterpolation: every level's M2L sums are folded first (the per-sample
+     * map of MLFMM_tri_summation_New, src/lib_sie_tri_mlfmm.f90:416-497),
+     * then shifted down and mapped by the transpose of SS's interpolation,
+     * so that the final sum over the leaf grid equals the sum over each
+     * coarser grid with the receiving patterns interpolated up to it */
+    for (int l = lmin; l <= lmax; ++l) {
+      const int Sl = c->Sl[l];
+      for (int b = 0; b < c->lev[l].nbox; ++b) {
+        zc *Db = &Dt[l][(size_t)b * 8 * Sl];
+        for (int i = 0; i < Sl; ++i) {
+          zc w[8];
+          for (int k = 0; k < 2; ++k) {
+            zc Dae = Db[(0 * 4 + 0 * 2 + k) * Sl + i], Dah = Db[(0 * 4 + 1 * 2 + k) * Sl + i];
+            zc Dbe = Db[(1 * 4 + 0 * 2 + k) * Sl + i], Dbh = Db[(1 * 4 + 1 * 2 + k) * Sl + i];
+            w[k * 4 + 0] = Dbh * kdv[k] + Dae * my_d[k];
+            w[k * 4 + 1] = -(Dah * kdv[k]) + Dbe * my_d[k];
+            w[k * 4 + 2] = -(Dbe * kdv[k]) + Dah * eps_d[k];
+            w[k * 4 + 3] = Dae * kdv[k] + Dbh * eps_d[k];
+          }
+          for (int it = 0; it < 8; ++it) Db[it * Sl + i] = w[it];
+        }
+      }
+    }
+    for (int l = lmin + 1; l <= lmax; ++l)
+      if (c->Kl[l] < c->Kl[l - 1]) interp_matrix(c->Kl[l], c->Kl[l - 1], &Mup[l]);
+  }
   for (int l = lmin + 1; l <= lmax; ++l) {
     const level_t *L = &c->lev[l], *Lp = &c->lev[l - 1];
+    const int Sc = c->Sl[l], Sp = c->Sl[l - 1];
+    const int interp = c->Kl[l] < c->Kl[l - 1];
+    const double *khp = c->khl[l - 1];
 #ifdef _OPENMP
 #pragma omp parallel for schedule(dynamic, 1)
 #endif
@@ -1474,23 +1815,40 @@ void spo_matvec(spo_ctx *c, const double *xin, double *yout, int nthreads) {
       box_centre(c, L->n[b], l, xc);
       box_centre(c, L->n[b] >> 3, l - 1, xp);
       vsub(xc, xp, r);
-      zc *Dc = &Dt[l][(size_t)b * 8 * S];
-      const zc *Dp = &Dt[l - 1][(size_t)pp * 8 * S];
+      zc *Dc = &Dt[l][(size_t)b * 8 * Sc];
+      const zc *Dp = &Dt[l - 1][(size_t)pp * 8 * Sp];
+      if (interp) {
+        zc *A = xmalloc(sizeof(zc) * 8 * Sp), *Ai = xmalloc(sizeof(zc) * 8 * Sc);
+        for (int i = 0; i < Sp; ++i) {
+          double tmp = vdot(&khp[3 * i], r);
+          zc t[2] = {zexp_mikr(c->k1, tmp), zexp_mikr(c->k2, tmp)};
+          for (int it = 0; it < 8; ++it) A[it * Sp + i] = Dp[it * Sp + i] * t[wspace ? it >> 2 : it & 1];
+        }
+        if (wspace) {
+          for (size_t i = 0; i < (size_t)8 * Sc; ++i) Ai[i] = 0.0;
+          anterp_w(&Mup[l], c->Kl[l], c->Kl[l - 1], A, Ai);
+        } else {
+          interp_items(c->Kl[l - 1], c->Kl[l], A, Ai);
+        }
+        for (size_t i = 0; i < (size_t)8 * Sc; ++i) Dc[i] = Dc[i] + Ai[i];
+        free(A); free(Ai);
+        continue;
+      }
+      const int S = Sc;
       for (int i = 0; i < S; ++i) {
-        double tmp = vdot(&c->khat[3 * i], r);
+        double tmp = vdot(&khp[3 * i], r);
         zc t[2] = {zexp_mikr(c->k1, tmp), zexp_mikr(c->k2, tmp)};
-        for (int it = 0; it < 8; ++it) Dc[it * S + i] = Dc[it * S + i] + Dp[it * S + i] * t[it & 1];
+        for (int it = 0; it < 8; ++it)
+          Dc[it * S + i] = Dc[it * S + i] + Dp[it * S + i] * t[wspace ? it >> 2 : it & 1];
       }
     }
   }
+  for (int l = 0; l < 32; ++l) { free(Mup[l].row); free(Mup[l].col); free(Mup[l].val); }
   /* final summation: lib_sie_ml_fmm_get_v_y_j (:690-751) with
    * MLFMM_tri_summation_New (src/lib_sie_tri_mlfmm.f90:416-497) and the
    * near-field CSR products (:755-777). */
   zc a_lc[2], d_lc[2];
   form_coef(c, a_lc, d_lc);
-  zc eps_d[2] = {c->eps1 * I * c->omega, c->eps2 * I * c->omega};
-  zc my_d[2] = {c->my1 * I * c->omega, c->my2 * I * c->omega};
-  zc kdv[2] = {c->k1 * I, c->k2 * I};
   int form = c->p.formulation;
 #ifdef _OPENMP
 #pragma omp parallel for schedule(dynamic, 16)
@@ -1500,7 +1858,15 @@ void spo_matvec(spo_ctx *c, const double *xin, double *yout, int nthreads) {
     const zc *Db = &Dt[lmax][(size_t)b * 8 * S];
     const zc *rt = &c->rt[(size_t)j * 4 * S];
     zc s1 = 0.0, s2 = 0.0;
-    for (int k = 0; k < 2; ++k) {
+    for (int k = 0; k < 2 && wspace; ++k) {
+      const zc *fa = rt + (k * 2 + 0) * S, *fb = rt + (k * 2 + 1) * S;
+      const zc *W = Db + (size_t)k * 4 * S;
+      for (int i = 0; i < S; ++i) {
+        s1 = s1 + (fa[i] * W[0 * S + i] + fb[i] * W[1 * S + i]);
+        s2 = s2 + (fa[i] * W[2 * S + i] + fb[i] * W[3 * S + i]);
+      }
+    }
+    for (int k = 0; k < 2 && !wspace; ++k) {
       const zc *fa = rt + (k * 2 + 0) * S, *fb = rt + (k * 2 + 1) * S;
       const zc *Dae = Db + (0 * 4 + 0 * 2 + k) * S, *Dbe = Db + (1 * 4 + 0 * 2 + k) * S;
       const zc *Dah = Db + (0 * 4 + 1 * 2 + k) * S, *Dbh = Db + (1 * 4 + 1 * 2 + k) * S;

@@ -76,6 +76,13 @@ void spo_near_values(const spo_ctx *c, double *abcd /* nnz*8 */);
 
 /* MLFMM: patterns + TL tables, then y = Z x. x,y: 2N complex (re,im). */
 void spo_mlfmm_setup(spo_ctx *c, int nthreads);
+/* per-level truncation numbers (K[l] for l < n; 0 = the default K) for the
+ * next setup; rr_adjoint: RR to a smaller K by the transpose of SS's
+ * interpolation instead of the reference's interpolation of the local
+ * coefficients; spo_mlfmm_setup_levels returns -1 if a level's K exceeds
+ * its parent's */
+void spo_set_level_K(spo_ctx *c, int n, const int *K, int rr_adjoint);
+int spo_mlfmm_setup_levels(spo_ctx *c, int nthreads);
 void spo_matvec(spo_ctx *c, const double *x, double *y, int nthreads);
 /* Dense far-field operator check: y = near + exact (non-MLFMM) far field is
  * not part of the reference; only the MLFMM matvec is. */
@@ -138,6 +145,10 @@ void spo_legendre_p(int n, double x, double *out);
 /* Transfer function table entry TL_k(r) for K terms at the K sampling dirs:
  * out[2*(2K^2+2)] complex for medium kd (re,im). */
 void spo_tl(int K, double kre, double kim, const double *r, double *out);
+/* Lagrange_Interpolation_New (src/lib_sie_mlfmm.f90:268-390) for one
+ * current: 4 channels (theta k1, theta k2, phi k1, phi k2) of 2K^2+2
+ * complex samples each, from the K_m grid to the K_n grid, stencil 2p. */
+void spo_lagrange_interp(int p, int Km, int Kn, const double *in, double *out);
 
 /* ------------------------------------------------------------------------ */
 /* 8-node quadrilaterals with 10 Huber edge functions (spk_quad.c)           */
