@@ -96,6 +96,26 @@ typedef struct spk_gmres_params {
 #define SPK_PRECOND_LEFT_JACOBI 1
 #define SPK_PRECOND_NEAR_FIELD 2
 
+/* Per-level truncation numbers for the next spk_setup_tri: K[l] for levels
+ * l < n (0: spk_params.truncation_number), non-increasing from the
+ * coarsest level to the leaves.  The reference computes per-level values
+ * in Truncation_number_calculation (src/lib_sie_mlfmm.f90:631-701) and then
+ * sets every level to truncation_number_default; its SS and RR translations
+ * (src/lib_sie_tri_mlfmm_interface.f90:466-528, 600-655) interpolate
+ * between the levels' sample grids (Lagrange_Interpolation_New,
+ * src/lib_sie_mlfmm.f90:268-390) where they differ.  SS interpolates a
+ * child's coefficients to the parent's grid.  RR: SPK_RR_REFERENCE shifts
+ * the parent's local coefficients and interpolates them to the child's
+ * grid, as the reference's code does (an operator far from the dense one:
+ * the local coefficients carry the translation's bandwidth and the
+ * parent's quadrature weights); SPK_RR_ANTERPOLATE folds every level's
+ * coefficients and applies the transpose of SS's interpolation (the
+ * operator then stays within the interpolation error of the dense one).
+ * FP64 handles only; the batched, transposed and complex64 products reject
+ * such a handle.  No counterpart as a routine. */
+#define SPK_RR_REFERENCE 0
+#define SPK_RR_ANTERPOLATE 1
+
 typedef struct spk_gmres_info {
   int info;                      /* info(1): 0 converged, -4 not converged */
   int iterations;                /* info(2) (sum over refinement sweeps) */
@@ -343,6 +363,13 @@ typedef struct spk_tuning {
 } spk_tuning;
 int spk_autotune(spk_handle *h, int R, const double *k_in, const double *e_in,
                  const spk_gmres_params *gp, spk_tuning *out);
+int spk_set_level_truncation(spk_handle *h, int n, const int *K, int rr_mode);
+/* The interpolation between the K_m and K_n sample grids as the library
+ * builds it, applied on the host (no GPU) to one current's four channels
+ * (theta k1, theta k2, phi k1, phi k2) of 2K^2 + 2 complex samples each:
+ * a check of the library's construction.  No counterpart as a routine. */
+int spk_interp_probe(int Km, int Kn, const double *in, double *out);
+
 /* Applies tuned values (e.g. spk_autotune's on another handle of a similar
  * mesh); 0 restores a default.  No counterpart in the reference. */
 int spk_set_tuning(spk_handle *h, int tiles_per_group, int inner_iterations);

@@ -18,7 +18,7 @@ EXPORTS = [
     "spk_create", "spk_destroy", "spk_last_error", "spk_setup_tri", "spk_get_stats",
     "spk_num_edges", "spk_matvec", "spk_matvec_device", "spk_set_transposed",
     "spk_get_transposed", "spk_set_inner_graph", "spk_adjoint_height_gradient",
-    "spk_autotune", "spk_set_tuning",
+    "spk_autotune", "spk_set_tuning", "spk_set_level_truncation", "spk_interp_probe",
     "spk_matvec_c64",
     "spk_matvec_c64_device", "spk_matvec_parts", "spk_rhs_plane_wave",
     "spk_rhs_plane_wave_device", "spk_scattered_field", "spk_scattered_field_device",
@@ -170,6 +170,8 @@ def lib():
     L.spk_set_inner_graph.argtypes = [vp, C.c_int]
     L.spk_autotune.argtypes = [vp, C.c_int, dp, dp, C.POINTER(GmresParams), C.POINTER(Tuning)]
     L.spk_set_tuning.argtypes = [vp, C.c_int, C.c_int]
+    L.spk_set_level_truncation.argtypes = [vp, C.c_int, ip, C.c_int]
+    L.spk_interp_probe.argtypes = [C.c_int, C.c_int, dp, dp]
     L.spk_adjoint_height_gradient.argtypes = [vp, dp, dp, C.c_int, dp, dp, C.POINTER(GmresParams),
                                               C.c_int, ip, C.c_double, dp, dp, dp,
                                               C.POINTER(GmresInfo)]

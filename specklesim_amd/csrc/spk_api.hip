@@ -70,6 +70,15 @@ struct LevelDev {
   DBuf<zd> coef;
   DBuf<int> live;
   int nb = 0;
+  // the level's truncation number, samples and grid (directions, weights)
+  int K = 0, S = 0;
+  DBuf<double> khat, trf;
+  // where the parent's K differs: the interpolation from this level's grid
+  // to the parent's on one (theta, phi) pair (SS), and the map of the
+  // downward pass to this level's grid (RR: its transpose, or the
+  // interpolation from the parent's grid); CSR
+  DBuf<int> up_row, up_col, dn_row, dn_col;
+  DBuf<double> up_val, dn_val;
 };
 
 }  // namespace
@@ -103,6 +112,14 @@ struct spk_handle {
   // tuned launch parameters (spk_autotune, spk_set_tuning; 0: the defaults),
   // kept across spk_setup_tri calls (an ensemble's meshes are alike)
   int tune_tpw = 0, tune_inner = 0;
+  // per-level truncation numbers (spk_set_level_truncation): requested
+  // values by level, the downward mode, the values of the current tree
+  std::vector<int> levK_req;
+  int rr_mode = SPK_RR_ANTERPOLATE;
+  std::vector<int> levK;
+  bool multik = false;
+  std::vector<Tables> tbl;  // per level (all equal to tb without multik)
+  DBuf<zd> lvl_tmp;         // coefficients of one level on its parent's grid
   TreeWork tw;  // scratch of the device octree build
   int disagg_tasks = 0;
   // batched matvec: disaggregation tiles (box, first position), buffers
@@ -287,14 +304,16 @@ struct HostLevelTables {
   std::vector<cplx> coef, up, down;
   std::vector<int> live;
 };
-static std::vector<HostLevelTables> host_level_tables(const Tables &tb, const Material &mat,
-                                                      const Tree &t) {
+// tbl: the sample grid of every level; SS/RR phases of child level l on
+// its parent's grid
+static std::vector<HostLevelTables> host_level_tables(const std::vector<Tables> &tbl,
+                                                      const Material &mat, const Tree &t) {
   std::vector<HostLevelTables> v(t.l_max + 1);
   std::vector<std::thread> th;
   for (int l = t.l_min; l <= t.l_max; ++l)
     th.emplace_back([&, l]() {
-      tl_coefficients(tb, mat, t, l, &v[l].rhat, &v[l].coef, &v[l].live);
-      make_phase_level(tb, mat, t, l, &v[l].up, &v[l].down);
+      tl_coefficients(tbl[l], mat, t, l, &v[l].rhat, &v[l].coef, &v[l].live);
+      make_phase_level(tbl[l > t.l_min ? l - 1 : l], mat, t, l, &v[l].up, &v[l].down);
     });
   for (auto &x : th) x.join();
   return v;
@@ -333,12 +352,36 @@ int setup_mlfmm(spk_handle *h, std::future<std::vector<HostLevelTables>> &pre) {
   // levels
   if (h->lev.size() < (size_t)t.l_max + 1) h->lev.resize(t.l_max + 1);
   const int nthreads = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  size_t tmp_need = 0;
   for (int l = t.l_min; l <= t.l_max; ++l) {
     const Level &L = t.lev[l];
     LevelDev &D = h->lev[l];
     D.nb = (int)L.n.size();
-    HIPCHK(D.B.alloc((size_t)D.nb * 8 * S));
-    HIPCHK(D.D.alloc((size_t)D.nb * 8 * S));
+    const Tables &tl = h->tbl[l];
+    D.K = tl.K;
+    D.S = tl.S;
+    HIPCHK(D.khat.upload(tl.khat));
+    HIPCHK(D.trf.upload(tl.trf));
+    HIPCHK(D.B.alloc((size_t)D.nb * 8 * D.S));
+    HIPCHK(D.D.alloc((size_t)D.nb * 8 * D.S));
+    if (l > t.l_min && h->levK[l - 1] != h->levK[l]) {
+      // interpolation to the parent's grid, and the downward map
+      std::vector<int> r, c, tr, tc;
+      std::vector<double> v, tv;
+      interp_pair_csr(3, h->levK[l], h->levK[l - 1], &r, &c, &v);
+      HIPCHK(D.up_row.upload(r));
+      HIPCHK(D.up_col.upload(c));
+      HIPCHK(D.up_val.upload(v));
+      if (h->rr_mode == SPK_RR_ANTERPOLATE) {
+        csr_transpose(2 * D.S, r, c, v, &tr, &tc, &tv);
+      } else {
+        interp_pair_csr(3, h->levK[l - 1], h->levK[l], &tr, &tc, &tv);
+      }
+      HIPCHK(D.dn_row.upload(tr));
+      HIPCHK(D.dn_col.upload(tc));
+      HIPCHK(D.dn_val.upload(tv));
+      tmp_need = std::max(tmp_need, (size_t)D.nb * 8 * h->tbl[l - 1].S);
+    }
     if (l < t.l_max) HIPCHK(D.child.upload(L.child));
     HIPCHK(D.parent.upload(L.parent));
     HIPCHK(D.octant.upload(L.octant));
@@ -363,9 +406,8 @@ int setup_mlfmm(spk_handle *h, std::future<std::vector<HostLevelTables>> &pre) {
       HIPCHK(d_rhat.upload(rhat));
       HIPCHK(d_coef.upload(cz));
       HIPCHK(d_live.upload(live));
-      HIPCHK(D.tlt.alloc(343 * 2 * (size_t)S));
-      launch_tl_level(S, h->tb.K, h->d_khat.p, h->d_trf.p, d_rhat.p, d_coef.p, d_live.p, D.tlt.p,
-                      h->st);
+      HIPCHK(D.tlt.alloc(343 * 2 * (size_t)D.S));
+      launch_tl_level(D.S, D.K, D.khat.p, D.trf.p, d_rhat.p, d_coef.p, d_live.p, D.tlt.p, h->st);
       HIPCHK(hipGetLastError());
       HIPCHK(hipStreamSynchronize(h->st));
     }
@@ -381,6 +423,7 @@ int setup_mlfmm(spk_handle *h, std::future<std::vector<HostLevelTables>> &pre) {
     h->stats.i4_pairs_per_level[l] = (int64_t)L.i4_box.size();
   }
   HIPCHK(h->d_W.alloc((size_t)LL.n.size() * 8 * S));
+  if (tmp_need) HIPCHK(h->lvl_tmp.alloc(tmp_need));
   h->stats.tables_ms = now_ms() - t0;
   // formulation scaling of the far field (MLFMM_tri_summation_New :478-495)
   const zc e1(h->mat.eta1.re, h->mat.eta1.im), e2(h->mat.eta2.re, h->mat.eta2.im);
@@ -423,6 +466,94 @@ static zf *pYf(spk_handle *h, zf *) { return h->f_yfar.p; }
 static DBuf<zd> &pXt(spk_handle *h, zd *) { return h->d_xt; }
 static DBuf<zf> &pXt(spk_handle *h, zf *) { return h->f_xt; }
 
+// y = Z x with per-level truncation numbers (complex128).  Upward: where a
+// parent's K exceeds its children's, the children's coefficients are
+// interpolated to the parent's grid (up CSR) before SS.  Downward, with
+// SPK_RR_REFERENCE, RR shifts on the parent's grid and interpolates to the
+// child's (dn CSR); with SPK_RR_ANTERPOLATE every level's M2L sums are
+// folded first and RR works on folded coefficients, applying the transpose
+// of the upward interpolation (dn CSR) where K changes; the leaf's folded
+// coefficients then go straight to the disaggregation.
+static const int kPairsC[8] = {0 * 4 + 0 * 2 + 0, 1 * 4 + 0 * 2 + 0, 0 * 4 + 0 * 2 + 1,
+                               1 * 4 + 0 * 2 + 1, 0 * 4 + 1 * 2 + 0, 1 * 4 + 1 * 2 + 0,
+                               0 * 4 + 1 * 2 + 1, 1 * 4 + 1 * 2 + 1};
+static const int kPairsW[8] = {0, 1, 2, 3, 4, 5, 6, 7};
+
+static int matvec_multik(spk_handle *h, const zd *x, zd *y, bool near_on, bool far_on) {
+  const Tree &t = h->tree;
+  const int N = h->N, lmax = t.l_max, lmin = t.l_min;
+  hipStream_t st = h->st;
+  zd *yfar = h->d_yfar.p;
+  if (far_on) {
+    LevelDev &LL = h->lev[lmax];
+    const int SL = LL.S;
+    launch_agg(LL.nb, SL, N, h->d_box_first.p, h->d_box_count.p, h->d_perm.p, h->d_rt.p, x, LL.B.p,
+               st, true);
+    for (int l = lmax - 1; l >= lmin; --l) {
+      LevelDev &P = h->lev[l], &Cl = h->lev[l + 1];
+      const zd *src = Cl.B.p;
+      if (Cl.K != P.K) {
+        launch_resample(Cl.nb, Cl.S, P.S, Cl.up_row.p, Cl.up_col.p, Cl.up_val.p, kPairsC, Cl.B.p,
+                        h->lvl_tmp.p, false, st);
+        src = h->lvl_tmp.p;
+      }
+      launch_ss(P.nb, P.S, P.child.p, src, (const zd *)Cl.up.p, P.B.p, st);
+    }
+    for (int l = lmin; l <= lmax; ++l)
+      launch_m2l(h->lev[l].nb, h->lev[l].S, h->lev[l].i4_off.p, h->lev[l].i4_box.p,
+                 h->lev[l].i4_slot.p, (const zd *)h->lev[l].B.p, (const zd *)h->lev[l].tlt.p,
+                 h->lev[l].D.p, st);
+    const zd I = {0, 1};
+    const zd kd1 = I * zcx(h->mat.k1), kd2 = I * zcx(h->mat.k2);
+    const zd my1 = zd{h->mat.my1, 0} * I * h->mat.omega, my2 = zd{h->mat.my2, 0} * I * h->mat.omega;
+    const zd ep1 = zcx(h->mat.eps1) * I * h->mat.omega, ep2 = zcx(h->mat.eps2) * I * h->mat.omega;
+    const zd *W = nullptr;
+    if (h->rr_mode == SPK_RR_REFERENCE) {
+      for (int l = lmin + 1; l <= lmax; ++l) {
+        LevelDev &Cl = h->lev[l], &P = h->lev[l - 1];
+        if (Cl.K != P.K) {
+          launch_shift_write(Cl.nb, P.S, Cl.parent.p, Cl.octant.p, P.D.p, (const zd *)Cl.down.p,
+                             h->lvl_tmp.p, false, st);
+          launch_resample(Cl.nb, P.S, Cl.S, Cl.dn_row.p, Cl.dn_col.p, Cl.dn_val.p, kPairsC,
+                          h->lvl_tmp.p, Cl.D.p, true, st);
+        } else {
+          launch_rr(Cl.nb, Cl.S, Cl.parent.p, Cl.octant.p, (const zd *)P.D.p,
+                    (const zd *)Cl.down.p, Cl.D.p, st);
+        }
+      }
+      launch_fold(LL.nb, SL, (const zd *)LL.D.p, kd1, kd2, my1, my2, ep1, ep2, h->d_W.p, st);
+      W = h->d_W.p;
+    } else {
+      // folded coefficients of every level into its (consumed) B buffer
+      for (int l = lmin; l <= lmax; ++l)
+        launch_fold(h->lev[l].nb, h->lev[l].S, (const zd *)h->lev[l].D.p, kd1, kd2, my1, my2, ep1,
+                    ep2, h->lev[l].B.p, st);
+      for (int l = lmin + 1; l <= lmax; ++l) {
+        LevelDev &Cl = h->lev[l], &P = h->lev[l - 1];
+        if (Cl.K != P.K) {
+          launch_shift_write(Cl.nb, P.S, Cl.parent.p, Cl.octant.p, P.B.p, (const zd *)Cl.down.p,
+                             h->lvl_tmp.p, true, st);
+          launch_resample(Cl.nb, P.S, Cl.S, Cl.dn_row.p, Cl.dn_col.p, Cl.dn_val.p, kPairsW,
+                          h->lvl_tmp.p, Cl.B.p, true, st);
+        } else {
+          launch_rr_folded(Cl.nb, Cl.S, Cl.parent.p, Cl.octant.p, (const zd *)P.B.p,
+                           (const zd *)Cl.down.p, Cl.B.p, st);
+        }
+      }
+      W = LL.B.p;
+    }
+    launch_disagg(h->disagg_tasks, N, SL, h->d_disagg_task.p, h->d_perm.p, h->d_leafbox_of_pos.p,
+                  h->d_rt.p, W, h->sc1, h->sc2, near_on ? yfar : y, st, false);
+  }
+  if (near_on)
+    launch_near_spmv_rows(h->spmv_wgs, N, h->d_row_box.p, h->d_row_r0.p, h->d_box_first.p,
+                          h->d_box_count.p, h->d_e2_off.p, h->d_e2_list.p, h->d_near_off.p,
+                          h->d_perm.p, (const zd *)h->d_blocks.p, x, far_on ? (const zd *)yfar : nullptr, y,
+                          st);
+  HIPCHK(hipGetLastError());
+  return SPK_OK;
+}
+
 // y = Z x on device vectors (2N complex each), far part into yfar first.
 // T = zd: the FP64 operator; T = zf: its complex64 mirror.
 // ev (optional, 8 events): recorded before stage 0 and after stages 0..6.
@@ -430,6 +561,14 @@ template <class T>
 int matvec_t(spk_handle *h, const T *x, T *y, bool near_on = true, bool far_on = true,
              hipEvent_t *ev = nullptr) {
   T *tag = nullptr;
+  if (h->multik) {
+    if constexpr (std::is_same<T, zd>::value) {
+      if (h->transposed) return fail(SPK_ERR_STATE, "per-level truncation numbers: no transposed operator");
+      return matvec_multik(h, x, y, near_on, far_on);
+    } else {
+      return fail(SPK_ERR_STATE, "per-level truncation numbers run on FP64 handles");
+    }
+  }
   const Tree &t = h->tree;
   const int S = h->tb.S, N = h->N, lmax = t.l_max, lmin = t.l_min;
   hipStream_t st = h->st;
@@ -557,6 +696,8 @@ int matvec_batch_t(spk_handle *h, int R, const T *const *xs, T *const *ys, bool 
   if (R < 1 || R > kBatchMax) return fail(SPK_ERR_ARG, "batch size must be 1..16");
   if (far_on && h->transposed)
     return fail(SPK_ERR_STATE, "the batched products do not take the transposed operator");
+  if (far_on && h->multik)
+    return fail(SPK_ERR_STATE, "the batched products do not take per-level truncation numbers");
   int rc = batch_ensure<T>(h, R);
   if (rc) return rc;
   BatchBufs<T> &BB = pBB(h, tag);
@@ -760,7 +901,31 @@ int spk_setup_tri(spk_handle *h, int np, const double *pp, int nt, const int *tt
     if (e != hipSuccess)
       return fail(SPK_ERR_HIP, std::string("device octree (") + what + "): " + hipGetErrorString(e));
   }
-  make_tables(h->p.truncation_number, &h->tb);
+  {
+    // truncation number of every level (the reference sets all of them to
+    // truncation_number_default, lib_sie_mlfmm.f90:696-698)
+    const Tree &tr = h->tree;
+    h->levK.assign(tr.l_max + 1, h->p.truncation_number);
+    for (int l = 0; l <= tr.l_max; ++l)
+      if (l < (int)h->levK_req.size() && h->levK_req[l] > 0) h->levK[l] = h->levK_req[l];
+    h->multik = false;
+    for (int l = tr.l_min; l < tr.l_max; ++l) {
+      if (h->levK[l] < h->levK[l + 1])
+        return fail(SPK_ERR_ARG, "a level's truncation number exceeds its parent's (level " +
+                                     std::to_string(l + 1) + ")");
+      if (h->levK[l] != h->levK[l + 1]) h->multik = true;
+    }
+    if (h->multik && h->p.precision == SPK_MIXED)
+      return fail(SPK_ERR_ARG, "per-level truncation numbers run on FP64 handles");
+    make_tables(h->levK[tr.l_max], &h->tb);
+    h->tbl.assign(tr.l_max + 1, Tables{});
+    for (int l = tr.l_min; l <= tr.l_max; ++l) {
+      if (h->levK[l] == h->levK[tr.l_max])
+        h->tbl[l] = h->tb;
+      else
+        make_tables(h->levK[l], &h->tbl[l]);
+    }
+  }
   const Tree &t = h->tree;
   h->stats.tree_ms = now_ms() - t0;
   // upload topology
@@ -829,7 +994,7 @@ int spk_setup_tri(spk_handle *h, int np, const double *pp, int nt, const int *tt
   HIPCHK(h->d_tmp.alloc(2 * (size_t)h->N));
   // the host part of the level tables runs beside the device's near fill
   std::future<std::vector<HostLevelTables>> pre =
-      std::async(std::launch::async, host_level_tables, std::cref(h->tb), std::cref(h->mat),
+      std::async(std::launch::async, host_level_tables, std::cref(h->tbl), std::cref(h->mat),
                  std::cref(h->tree));
   double t1 = now_ms();
   rc = near_fill(h);
@@ -907,6 +1072,8 @@ int spk_set_transposed(spk_handle *h, int transposed) {
   if (!h->ready) return fail(SPK_ERR_STATE, "spk_setup_tri has not completed");
   const bool want = transposed != 0;
   if (want == h->transposed) return SPK_OK;
+  if (want && h->multik)
+    return fail(SPK_ERR_STATE, "per-level truncation numbers: no transposed operator");
   HIPCHK(hipSetDevice(h->p.device));
   const Tree &t = h->tree;
   const int nbl = (int)t.box_count.size();
@@ -2410,6 +2577,19 @@ int spk_autotune(spk_handle *h, int R, const double *k_in, const double *e_in,
   return SPK_OK;
 }
 
+// Per-level truncation numbers for the next spk_setup_tri.
+int spk_set_level_truncation(spk_handle *h, int n, const int *K, int rr_mode) {
+  if (!h || (n > 0 && !K)) return fail(SPK_ERR_ARG, "null argument");
+  if (n < 0 || n > 32) return fail(SPK_ERR_ARG, "n must be 0..32");
+  if (rr_mode != SPK_RR_REFERENCE && rr_mode != SPK_RR_ANTERPOLATE)
+    return fail(SPK_ERR_ARG, "rr_mode must be SPK_RR_REFERENCE or SPK_RR_ANTERPOLATE");
+  for (int l = 0; l < n; ++l)
+    if (K[l] < 0 || K[l] > 120) return fail(SPK_ERR_ARG, "truncation numbers must be 0..120");
+  h->levK_req.assign(K, K + n);
+  h->rr_mode = rr_mode;
+  return SPK_OK;
+}
+
 // Applies tuned values (spk_autotune on another handle of a similar mesh):
 // row tiles per workgroup of the batched fp8 near product (1, 2 or 4) and
 // the inner solve length used when spk_gmres_params.inner_iterations is 0;
@@ -3161,6 +3341,7 @@ int spk_probe_read_bandwidth(int device, size_t bytes, int reps, double *gbps) {
 int spk_profile_matvec(spk_handle *h, int reps, double *ms, double *bytes) {
   if (!h || reps < 1) return fail(SPK_ERR_ARG, "bad arguments");
   if (!h->ready) return fail(SPK_ERR_STATE, "spk_setup_tri has not completed");
+  if (h->multik) return fail(SPK_ERR_STATE, "stage profile: uniform truncation numbers only");
   return profile_matvec_t<zd>(h, reps, ms, bytes);
 }
 
@@ -3220,6 +3401,33 @@ int spk_profile_near_bf(spk_handle *h, int reps, double *ms, double *bytes) {
 
 int spk_profile_near_f8(spk_handle *h, int reps, double *ms, double *bytes) {
   return profile_near_lp(h, reps, ms, bytes, true);
+}
+
+// The interpolation between two sample grids as this library builds it
+// (interp_pair_csr), applied on the host to one current's four channels
+// (theta k1, theta k2, phi k1, phi k2; 2K^2 + 2 complex samples each).
+int spk_interp_probe(int Km, int Kn, const double *in, double *out) {
+  if (!in || !out || Km < 4 || Kn < 4 || Km > 120 || Kn > 120)
+    return fail(SPK_ERR_ARG, "bad arguments");
+  std::vector<int> r, c;
+  std::vector<double> v;
+  interp_pair_csr(3, Km, Kn, &r, &c, &v);
+  const int Sm = 2 * Km * Km + 2, Sn = 2 * Kn * Kn + 2;
+  for (int k = 0; k < 2; ++k) {  // medium: channels (k, 2 + k)
+    const int ch[2] = {k, 2 + k};
+    for (int row = 0; row < 2 * Sn; ++row) {
+      double re = 0, im = 0;
+      for (int e = r[row]; e < r[row + 1]; ++e) {
+        const int ci = c[e] < Sm ? 0 : 1, si = c[e] - ci * Sm;
+        re += in[2 * ((size_t)ch[ci] * Sm + si)] * v[e];
+        im += in[2 * ((size_t)ch[ci] * Sm + si) + 1] * v[e];
+      }
+      const int co = row < Sn ? 0 : 1, so = row - co * Sn;
+      out[2 * ((size_t)ch[co] * Sn + so)] = re;
+      out[2 * ((size_t)ch[co] * Sn + so) + 1] = im;
+    }
+  }
+  return SPK_OK;
 }
 
 int spk_topology_probe(int np, const double *pp, int nt, const int *tt, int normal_convention,

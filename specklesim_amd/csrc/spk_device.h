@@ -305,6 +305,15 @@ void launch_batch_near(int nwg, int N, int R, const int *row_box, const int *row
 // bf16) on the fp8 copy of the blocks, on the matrix cores; workgroups = the
 // row tiles (box, first row) of kSpmvRowsPerWG rows
 int near_mx_tiles_per_group();
+// levels of different truncation numbers (spk_kernels.hip)
+void launch_resample(int nbox, int Sin, int Sout, const int *rowptr, const int *col,
+                     const double *val, const int *pairs, const zd *X, zd *Y, bool accumulate,
+                     hipStream_t st);
+void launch_shift_write(int nb, int S, const int *parent, const int *octant, const zd *P,
+                        const zd *down, zd *T, bool folded, hipStream_t st);
+void launch_rr_folded(int nb, int S, const int *parent, const int *octant, const zd *Wp,
+                      const zd *down, zd *Wc, hipStream_t st);
+
 void launch_batch_near_f8(int nwg, int N, int R, const int *row_box, const int *row_r0,
                           int ngrp, const int *grp_box, const int *grp_r0,
                           const int *box_first, const int *box_count, const int *e2_off,

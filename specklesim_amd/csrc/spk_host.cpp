@@ -714,4 +714,170 @@ void make_phase_level(const Tables &tb, const Material &mat, const Tree &t, int 
   }
 }
 
+// ---------------------------------------------------------------------------
+// Interpolation between the sample grids of two truncation numbers
+// (Lagrange_Interpolation_New, src/lib_sie_mlfmm.f90:268-493, with
+// theta_phi_extended :573-610 and Sorting_theta_phi_LenReduced_Arr_New
+// :414-493) as a sparse matrix on one (theta, phi) component pair of one
+// medium.  The extended grid is built symbolically: every cell records
+// where its value comes from (an interior sample, a pole row's Cartesian
+// formula at its column's angle, or zero) and its sign, by the same
+// sequence of fills, reflections across the poles and periodic copies,
+// in the same order, as the reference builds the values.  Stencil
+// indices use the float arithmetic of the reference's default reals.
+// ---------------------------------------------------------------------------
+namespace {
+struct ExtCell {
+  int kind = 0;  // 0 zero, 1 interior sample, 2 pole theta = 0, 3 pole theta = pi
+  int src = 0;   // interior sample index
+  double c = 0, s = 0, sign = 1;
+};
+}  // namespace
+
+void interp_pair_csr(int p, int Km, int Kn, std::vector<int> *rowptr, std::vector<int> *col,
+                     std::vector<double> *val) {
+  const int Kx = 2 * Km;  // phi samples of the source grid
+  const int Sm = 2 * Km * Km + 2, Sn = 2 * Kn * Kn + 2, Nsm = 2 * Km * Km, Nsn = 2 * Kn * Kn;
+  const int nth = Km + 2 * p + 2, nph = Kx + 2 * p + 1;
+  // theta_phi_extended: a_theta(1..3Km+2), a_phi(1..6Km), 1-based
+  std::vector<double> xx(Km), ww(Km), at(3 * Km + 3), ap(6 * Km + 1);
+  gauss_legendre(Km, xx.data(), ww.data());
+  const double d_phi = 6 * kPi / (3 * (2 * Km));
+  for (int j = 1; j <= Km; ++j) {
+    at[j] = 2 * kPi - std::acos(xx[Km - (j - 1) - 1]);
+    at[j + Km + 1] = std::acos(xx[j - 1]);
+    at[2 * Km + 2 + j] = -std::acos(xx[Km - (j - 1) - 1]);
+  }
+  at[Km + 1] = kPi;
+  at[2 * Km + 2] = 0.0;
+  for (int k = 1; k <= 6 * Km; ++k) ap[k] = -2 * kPi + d_phi * (k - 1);
+  // the extended grid (1-based rows m, columns n)
+  std::vector<ExtCell> E((size_t)(nth + 1) * (nph + 1));
+  auto cell = [&](int m, int n) -> ExtCell & { return E[(size_t)m * (nph + 1) + n]; };
+  for (int m = 1; m <= nph; ++m) {
+    const int n = m + Kx - p;
+    ExtCell pi_row, zero_row;
+    pi_row.kind = 3;
+    zero_row.kind = 2;
+    pi_row.c = zero_row.c = std::cos(ap[n]);
+    pi_row.s = zero_row.s = std::sin(ap[n]);
+    cell(p + 1, m) = pi_row;
+    cell(Km + p + 2, m) = zero_row;
+  }
+  for (int i = 1; i <= Km; ++i)
+    for (int j = 1; j <= Kx; ++j) {
+      ExtCell e;
+      e.kind = 1;
+      e.src = (i - 1) * Kx + (j - 1);
+      cell(p + 1 + i, p + j) = e;
+    }
+  for (int m = 1; m <= nth; ++m) {
+    const double xt = at[m - p + Km + 1];
+    for (int n = 1; n <= nph; ++n) {
+      const double xp = ap[n - p + Kx];
+      int src = 0, dn = 0;
+      if (xt < 0.0)
+        src = 2 * (Km + p + 2) - m;
+      else if (xt >= kPi)
+        src = 2 * (p + 1) - m;
+      if (!src) continue;
+      if (xp < kPi && xp >= 0.0)
+        dn = Kx / 2;
+      else if (xp < 2 * kPi && xp >= kPi)
+        dn = -Kx / 2;
+      if (!dn) continue;
+      ExtCell e = cell(src, n + dn);
+      e.sign = -e.sign;
+      cell(m, n) = e;
+    }
+  }
+  for (int m = 1; m <= nth; ++m)
+    for (int n = 1; n <= nph; ++n) {
+      const double xp = ap[n - p + Kx];
+      if (xp >= 2 * kPi)
+        cell(m, n) = cell(m, n - Kx);
+      else if (xp < 0)
+        cell(m, n) = cell(m, n + Kx);
+    }
+  // Lagrange_Interpolation_Weights on the 2p nodes ext(K_arr+t+1-p ..)
+  auto weights = [&](double x, int t, int K_arr, const std::vector<double> &ext, double *wk) {
+    double nodes[16];
+    const int kt = K_arr + t;
+    for (int j = 1; j <= 2 * p; ++j) nodes[j - 1] = ext[kt + 1 - p + (j - 1)];
+    for (int j = 1; j <= 2 * p; ++j) {
+      wk[j - 1] = 1.0;
+      for (int m = 1; m <= 2 * p; ++m)
+        if (m != j) wk[j - 1] = (x - nodes[m - 1]) / (nodes[j - 1] - nodes[m - 1]) * wk[j - 1];
+    }
+  };
+  std::vector<double> xt(Kn), wt(Kn);
+  gauss_legendre(Kn, xt.data(), wt.data());
+  rowptr->assign(2 * (size_t)Sn + 1, 0);
+  col->clear();
+  val->clear();
+  std::vector<std::vector<std::pair<int, double>>> rows(2 * (size_t)Sn);
+  double wkt[16], wkp[16];
+  for (int it = 1; it <= Kn; ++it) {
+    const double theta = std::acos(xt[it - 1]);
+    const int t = (int)std::floor((float)((float)(it - 1.0f) / (float)Kn) * (float)Km) + 1;
+    weights(theta, t, Km, at, wkt);
+    for (int jp = 1; jp <= 2 * Kn; ++jp) {
+      const int tt = (it - 1) * 2 * Kn + (jp - 1);
+      const double phi = (jp - 1) * 2 * kPi / (2 * Kn);
+      const int sidx = (int)std::floor((float)((float)(jp - 1.0f) / (float)Kn) * (float)Km) + 1;
+      weights(phi, sidx, Kx, ap, wkp);
+      for (int i = 1; i <= 2 * p; ++i)
+        for (int j = 1; j <= 2 * p; ++j) {
+          const double w = wkp[j - 1] * wkt[i - 1];
+          const ExtCell &e = cell(p + t + i - p, p + sidx + j - p);
+          for (int co = 0; co < 2; ++co) {
+            auto &r = rows[(size_t)co * Sn + tt];
+            if (e.kind == 1) {
+              r.push_back({co * Sm + e.src, w * e.sign});
+            } else if (e.kind == 2 || e.kind == 3) {
+              const int ps = e.kind == 2 ? Nsm : Nsm + 1;
+              const double eps = e.kind == 2 ? 1.0 : -1.0;  // theta row sign at the pole
+              const double ws = w * e.sign;
+              if (co == 0) {  // theta: eps (c fx + s fy)
+                r.push_back({0 * Sm + ps, ws * eps * e.c});
+                r.push_back({1 * Sm + ps, ws * eps * e.s});
+              } else {  // phi: -s fx + c fy
+                r.push_back({0 * Sm + ps, -ws * e.s});
+                r.push_back({1 * Sm + ps, ws * e.c});
+              }
+            }
+          }
+        }
+    }
+  }
+  for (int co = 0; co < 2; ++co)
+    for (int q = 0; q < 2; ++q) rows[(size_t)co * Sn + Nsn + q].push_back({co * Sm + Nsm + q, 1.0});
+  for (size_t r = 0; r < rows.size(); ++r) {
+    for (auto &e : rows[r]) {
+      col->push_back(e.first);
+      val->push_back(e.second);
+    }
+    (*rowptr)[r + 1] = (int)col->size();
+  }
+}
+
+// the transpose of a CSR matrix with ncols columns
+void csr_transpose(int ncols, const std::vector<int> &rowptr, const std::vector<int> &col,
+                   const std::vector<double> &val, std::vector<int> *trow, std::vector<int> *tcol,
+                   std::vector<double> *tval) {
+  const int nrows = (int)rowptr.size() - 1;
+  trow->assign(ncols + 1, 0);
+  for (int c : col) (*trow)[c + 1]++;
+  for (int c = 0; c < ncols; ++c) (*trow)[c + 1] += (*trow)[c];
+  tcol->assign(col.size(), 0);
+  tval->assign(col.size(), 0.0);
+  std::vector<int> fill(trow->begin(), trow->end() - 1);
+  for (int r = 0; r < nrows; ++r)
+    for (int e = rowptr[r]; e < rowptr[r + 1]; ++e) {
+      const int q = fill[col[e]]++;
+      (*tcol)[q] = r;
+      (*tval)[q] = val[e];
+    }
+}
+
 }  // namespace spk

@@ -117,5 +117,13 @@ void quadrature_tri(int ngp, double *a, double *b, double *w);
 int set_error(int code, const std::string &msg);
 void gauss_legendre(int n, double *x, double *w);
 void hankel2_sph(int n, double zre, double zim, cplx *out);
+// Lagrange interpolation from the K_m to the K_n sample grid (stencil 2p)
+// on one (theta, phi) component pair of one medium, as CSR: rows
+// comp * S_n + sample, columns comp * S_m + sample
+void interp_pair_csr(int p, int Km, int Kn, std::vector<int> *rowptr, std::vector<int> *col,
+                     std::vector<double> *val);
+void csr_transpose(int ncols, const std::vector<int> &rowptr, const std::vector<int> &col,
+                   const std::vector<double> &val, std::vector<int> *trow, std::vector<int> *tcol,
+                   std::vector<double> *tval);
 
 }  // namespace spk

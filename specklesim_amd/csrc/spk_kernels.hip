@@ -1267,6 +1267,84 @@ void launch_near_spmv_rows(int nwg, int N, const int *row_box, const int *row_r0
                      box_count, e2_off, e2_list, near_off, perm, blocks, x, yadd, y);
 }
 
+// ---------------------------------------------------------------------------
+// Levels of different truncation numbers.
+// ---------------------------------------------------------------------------
+// Y[box][pair item][s_out] (+)= sum over the CSR row of val * X[box][item
+// of the column's component][s_in]: the interpolation (or its transpose)
+// between two sample grids on the four (theta, phi) component pairs of a
+// box's 8 coefficient items; pairs[2q + c] = item of component c of pair q.
+// One thread per (output row, box, pair).
+__global__ void k_resample(int Sin, int Sout, const int *rowptr, const int *col, const double *val,
+                           int4 pairs_lo, int4 pairs_hi, const zd *X, zd *Y, int accumulate) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y, q = blockIdx.z;
+  if (r >= 2 * Sout) return;
+  const int pr[8] = {pairs_lo.x, pairs_lo.y, pairs_lo.z, pairs_lo.w,
+                     pairs_hi.x, pairs_hi.y, pairs_hi.z, pairs_hi.w};
+  const int it0 = pr[2 * q], it1 = pr[2 * q + 1];
+  const zd *x0 = X + ((size_t)b * 8 + it0) * Sin, *x1 = X + ((size_t)b * 8 + it1) * Sin;
+  zd acc = {0, 0};
+  for (int e = rowptr[r]; e < rowptr[r + 1]; ++e) {
+    const int c = col[e];
+    const zd v = c < Sin ? x0[c] : x1[c - Sin];
+    acc = acc + v * val[e];
+  }
+  const int co = r < Sout ? 0 : 1, so = r - co * Sout;
+  zd *y = Y + ((size_t)b * 8 + (co ? it1 : it0)) * Sout + so;
+  *y = accumulate ? *y + acc : acc;
+}
+
+void launch_resample(int nbox, int Sin, int Sout, const int *rowptr, const int *col,
+                     const double *val, const int *pairs, const zd *X, zd *Y, bool accumulate,
+                     hipStream_t st) {
+  if (nbox <= 0) return;
+  hipLaunchKernelGGL(k_resample, dim3(nblk(2 * (int64_t)Sout, 256), nbox, 4), dim3(256), 0, st, Sin,
+                     Sout, rowptr, col, val, make_int4(pairs[0], pairs[1], pairs[2], pairs[3]),
+                     make_int4(pairs[4], pairs[5], pairs[6], pairs[7]), X, Y, accumulate ? 1 : 0);
+}
+
+// T[b][it][s] = P[parent[b]][it][s] * down[octant[b]][medium][s] on the
+// parent's grid; medium = it & 1 (coefficient items) or it >> 2 (folded)
+__global__ void k_shift_write(int S, const int *parent, const int *octant, const zd *P,
+                              const zd *down, zd *T, int folded) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  if (s >= S) return;
+  const int p = parent[b], o = octant[b];
+  const zd d0 = down[((size_t)o * 2 + 0) * S + s], d1 = down[((size_t)o * 2 + 1) * S + s];
+  for (int it = 0; it < 8; ++it) {
+    const int k = folded ? it >> 2 : it & 1;
+    T[((size_t)b * 8 + it) * S + s] = P[((size_t)p * 8 + it) * S + s] * (k ? d1 : d0);
+  }
+}
+void launch_shift_write(int nb, int S, const int *parent, const int *octant, const zd *P,
+                        const zd *down, zd *T, bool folded, hipStream_t st) {
+  if (nb <= 0) return;
+  hipLaunchKernelGGL(k_shift_write, dim3(nblk(S, 256), nb), dim3(256), 0, st, S, parent, octant, P,
+                     down, T, folded ? 1 : 0);
+}
+
+// RR on folded coefficients (items k*4 + {1a, 1b, 2a, 2b}, medium it >> 2)
+__global__ void k_rr_folded(int S, const int *parent, const int *octant, const zd *Wp,
+                            const zd *down, zd *Wc) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  if (s >= S) return;
+  const int p = parent[b], o = octant[b];
+  const zd d0 = down[((size_t)o * 2 + 0) * S + s], d1 = down[((size_t)o * 2 + 1) * S + s];
+  for (int it = 0; it < 8; ++it) {
+    zd &w = Wc[((size_t)b * 8 + it) * S + s];
+    w = w + Wp[((size_t)p * 8 + it) * S + s] * ((it >> 2) ? d1 : d0);
+  }
+}
+void launch_rr_folded(int nb, int S, const int *parent, const int *octant, const zd *Wp,
+                      const zd *down, zd *Wc, hipStream_t st) {
+  if (nb <= 0) return;
+  hipLaunchKernelGGL(k_rr_folded, dim3(nblk(S, 256), nb), dim3(256), 0, st, S, parent, octant, Wp,
+                     down, Wc);
+}
+
 #define SPK_MATVEC_INSTANTIATE(T)                                                               \
   template void launch_agg<T>(int, int, int, const int *, const int *, const int *, const T *,   \
                               const T *, T *, hipStream_t, bool);                                \

@@ -194,6 +194,19 @@ class SieTriMlfmm:
         nat.check(nat.lib().spk_get_transposed(self._h, C.byref(t)))
         return bool(t.value)
 
+    # -- per-level truncation numbers ----------------------------------------
+    def set_level_truncation(self, K_by_level, rr="anterpolate"):
+        """Truncation number per octree level (index = level, 0 = the
+        handle's default) for the next precalculation_tri; rr: "reference"
+        (RR interpolates the parent's local coefficients, as the reference's
+        code does) or "anterpolate" (RR applies the transpose of SS's
+        interpolation to folded coefficients).  FP64 handles."""
+        k = np.ascontiguousarray(K_by_level, dtype=np.int32).reshape(-1)
+        mode = {"reference": 0, "anterpolate": 1}[rr]
+        nat.check(nat.lib().spk_set_level_truncation(self._h, k.size,
+                                                     k.ctypes.data_as(C.POINTER(C.c_int)), mode))
+        return self
+
     # -- run-time tuning --------------------------------------------------------
     def autotune(self, angles, tolerance=1e-6, max_iterations=650):
         """Tunes the near-field preconditioned solver on the current mesh

@@ -132,3 +132,23 @@ def test_level_K_must_not_grow_towards_the_leaves(patch3):
     pp, tt, *_ = patch3
     with pytest.raises(ValueError):
         _mk(pp, tt, [0, 0, 12, 16], "adjoint")
+
+
+@pytest.mark.parametrize("Km,Kn", [(12, 17), (16, 12), (19, 20), (21, 16)])
+def test_library_interpolation_equals_the_oracle(Km, Kn):
+    """The library's construction (a symbolic replay of the extended grid,
+    applied as a sparse matrix; host probe, no GPU) against the oracle's
+    literal restatement, on random samples."""
+    import ctypes as C
+
+    from specklesim_amd import _native as nat
+
+    rng = np.random.default_rng(Km + 7 * Kn)
+    vals = rng.standard_normal((4, 2 * Km * Km + 2)) + 1j * rng.standard_normal((4, 2 * Km * Km + 2))
+    ref = O.lagrange_interp(vals, Km, Kn)
+    out = np.zeros((4, 2 * Kn * Kn + 2), complex)
+    dp = C.POINTER(C.c_double)
+    vin = np.ascontiguousarray(vals)
+    nat.check(nat.lib().spk_interp_probe(Km, Kn, vin.view(np.float64).ctypes.data_as(dp),
+                                         out.view(np.float64).ctypes.data_as(dp)))
+    assert np.abs(out - ref).max() < 1e-13 * np.abs(vals).max() * 36
