@@ -249,6 +249,8 @@ int spk_near_product_lowp(spk_handle *h, int format, int batched, int R, const v
 #define SPK_COPY_D2D 3
 int spk_device_alloc(int device, size_t bytes, void **d_ptr);
 int spk_device_free(void *d_ptr);
+/* Waits for all work on the current device (every handle's stream), then
+ * copies. */
 int spk_device_copy(void *dst, const void *src, size_t bytes, int direction);
 int spk_device_synchronize(int device);
 

@@ -1041,6 +1041,10 @@ int spk_device_copy(void *dst, const void *src, size_t bytes, int direction) {
                                                 : hipMemcpyDefault;
   if (direction < SPK_COPY_H2D || direction > SPK_COPY_D2D)
     return fail(SPK_ERR_ARG, "unknown copy direction");
+  // the handles' streams do not synchronise with the null stream: finish
+  // their work first, so that a copy after an asynchronous product (e.g.
+  // spk_matvec_device) sees its result
+  HIPCHK(hipDeviceSynchronize());
   HIPCHK(hipMemcpy(dst, src, bytes, k));
   return SPK_OK;
 }
