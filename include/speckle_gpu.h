@@ -149,6 +149,27 @@ const char *spk_last_error(void);
  * tt: 3*nt node indices, 1-based (read_pp_tt_file_tri). */
 int spk_setup_tri(spk_handle *h, int np, const double *pp, int nt, const int *tt);
 
+/* The MLFMM operator on an 8-node quadrilateral height-field surface (the
+ * mesh of sorting_quad, src/lib_sie_quad_data_container.f90:321-403: Lx x Ly
+ * (m), nx x ny elements, z[(2nx+1)(2ny+1)] node heights, x index fastest)
+ * with the 10 edge functions per element of Sorting_edge
+ * (src/lib_sie_quad_calculation_mod.f90:624-698).  No counterpart in the
+ * reference, whose quadrilateral solve is dense LU
+ * (lib_sie_quad_solver_normal.f90:62-270) and whose MLFMM pattern slot for
+ * quadrilaterals (preCalculated_RT_quad, src/lib_sie_data_container.f90:84)
+ * is never filled.  Each edge function sits in the octree at the midpoint
+ * of its node pair (find_midpoint_edge_quad, :210-255); the near blocks are
+ * the dense system's entries (impedance_edge_versus_element and the
+ * assembly of :165-210); the far part is the plane-wave form of the dense
+ * fill's regular rule.  PMCHWT, MCTF or ICTF; uniform truncation number.
+ * Afterwards the handle's products, right-hand sides, solves (one vector or
+ * a batch, FP64 or mixed) and scattered fields (Integration_scattered_field_quad,
+ * :1281-1323, ngp 5, on the element-local coefficients of
+ * lib_sie_quad_solver_normal.f90:312-323) work on the quadrilateral system
+ * of 2M unknowns; the transposed operator, the adjoint gradient and the
+ * triangle probes refuse it. */
+int spk_setup_quad(spk_handle *h, double Lx, double Ly, int nx, int ny, const double *z);
+
 /* ------------------------------------------------------------------------ */
 /* 8-node quadrilaterals with 10 Huber edge functions: the near/dense fill   */
 /* ------------------------------------------------------------------------ */

@@ -78,6 +78,7 @@ def lib():
         L.spq_geometry_probe.argtypes = [C.c_double, C.c_double, dp, dp]
         L.spq_regular_pair.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, dp]
         L.spq_rhs_plane.argtypes = [C.c_void_p, C.c_int, dp, dp, dp]
+        L.spq_scattered_field.argtypes = [C.c_void_p, C.c_int, dp, dp, C.c_int, dp, dp]
         L.spo_gmres_prec.argtypes = [C.c_void_p, dp, dp, C.c_int, C.c_int, C.c_double, C.c_int,
                                      C.c_int, C.c_int, dp, ip, dp, dp]
         L.spo_gmres_dense.argtypes = [C.c_int, dp, dp, dp, C.c_int, C.c_int, C.c_double, C.c_int,
@@ -424,6 +425,18 @@ class QuadOracle:
         b = np.zeros(2 * self.M, np.complex128)
         lib().spq_rhs_plane(self.h, f, _d(k), _d(e), _d(b.view(np.float64)))
         return b
+
+
+    def scattered_field(self, points, x, ngp=5):
+        """E, H (npts, 3) of a solution x[2M] (Integration_scattered_field_quad
+        over all elements on the solution's element-local coefficients)."""
+        pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+        xx = np.ascontiguousarray(x, dtype=np.complex128)
+        E = np.zeros((pts.shape[0], 3), np.complex128)
+        H = np.zeros((pts.shape[0], 3), np.complex128)
+        lib().spq_scattered_field(self.h, pts.shape[0], _d(pts), _d(xx.view(np.float64)), int(ngp),
+                                  _d(E.view(np.float64)), _d(H.view(np.float64)))
+        return E, H
 
 
 def quad_geometry(k, e, El):

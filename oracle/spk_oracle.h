@@ -181,6 +181,10 @@ void spq_regular_pair(const spq_ctx *c, int s, int t, int ngp, double *out);
 /* plane-wave right-hand side b[2M] (complex) for PMCHWT (1), MCTF (2), ICTF (3) */
 void spq_rhs_plane(const spq_ctx *c, int formulation, const double *kin, const double *ein,
                    double *bout);
+/* scattered E, H (3 complex per point) of a solution x[2M] at npts points,
+ * ngp x ngp points per element (the reference uses 5) */
+void spq_scattered_field(const spq_ctx *c, int npts, const double *r, const double *x, int ngp,
+                         double *E, double *H);
 
 #ifdef __cplusplus
 }

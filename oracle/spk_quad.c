@@ -963,3 +963,76 @@ void spq_regular_pair(const spq_ctx *c, int s, int t, int ngp, double *out) {
   for (int k = 0; k < 3; ++k)
     for (int n = 0; n < 10; ++n) o[k * 10 + n] = sum[k][n];
 }
+
+/* ------------------------------------------------------------------------ */
+/* scattered field of a solution                                             */
+/* ------------------------------------------------------------------------ */
+/* The element-local coefficients S_EH of a solution x[2M]
+ * (src/lib_sie_quad_solver_normal.f90:312-323), then at each point the sum
+ * over all elements of Integration_scattered_field_quad
+ * (src/lib_sie_quad_calculation_mod.f90:1281-1323), ngp x ngp Gauss points
+ * on the source element, exterior medium k1:
+ *   E += KA SE + KB SH / (i w eps0) + i w my0 KC SH
+ *   H += KA SH + KB SE / (-i w my0) - i w eps0 KC SE
+ * with KA = (n x N) x gradG ff, KB = nrot gradG ff, KC = (n x N) G ff,
+ * ff = JJ w_i w_j.  E, H: 3 complex per point. */
+void spq_scattered_field(const spq_ctx *c, int npts, const double *r, const double *xin, int ngp,
+                         double *Eout, double *Hout) {
+  const int M = c->medge, nel = c->nel;
+  const zc *x = (const zc *)xin;
+  zc *seh = calloc((size_t)2 * nel * 10, sizeof(zc));
+  for (int m = 0; m < M; ++m) {
+    const int *e = c->edge_p + 5 * m;
+    const int i = e[0] - 1, pa = e[1] - 1, j = e[2] - 1, pb = e[3] - 1;
+    seh[(size_t)i * 10 + pa] = x[m];
+    seh[(size_t)(i + nel) * 10 + pa] = x[m + M];
+    if (j >= 0) {
+      seh[(size_t)j * 10 + pb] = seh[(size_t)i * 10 + pa] * e[4];
+      seh[(size_t)(j + nel) * 10 + pb] = seh[(size_t)(i + nel) * 10 + pa] * e[4];
+    }
+  }
+  double xg[64], wg[64];
+  spo_gauss_legendre(ngp, xg, wg);
+  const zc iwe = I * c->omega * QEPS0, iwm = I * c->omega * QMY0;
+  zc *E = (zc *)Eout, *H = (zc *)Hout;
+  for (int p = 0; p < npts; ++p) {
+    const double *Ra = r + 3 * p;
+    zc sE[3] = {0, 0, 0}, sH[3] = {0, 0, 0};
+    for (int s = 0; s < nel; ++s) {
+      double Eq[8][3];
+      element_nodes(c, s + 1, Eq);
+      const zc *SE = seh + (size_t)s * 10, *SH = seh + (size_t)(s + nel) * 10;
+      for (int i = 0; i < ngp; ++i)
+        for (int j = 0; j < ngp; ++j) {
+          delem d;
+          paras_b(xg[i], xg[j], Eq, &d);
+          double Rq[3], R[3];
+          r_local(xg[i], xg[j], Eq, Rq);
+          for (int k = 0; k < 3; ++k) R[k] = Ra[k] - Rq[k];
+          const double Rn = len3(R);
+          zc G, DG;
+          qgreen(c->k1, Rn, &G, &DG);
+          zc gg[3];
+          for (int k = 0; k < 3; ++k) gg[k] = R[k] / Rn * DG;
+          const double ff = d.JJ * wg[i] * wg[j];
+          for (int n = 0; n < 10; ++n) {
+            double aa[3];
+            cross3(d.nq, d.N[n], aa);
+            const zc KA[3] = {(aa[1] * gg[2] - aa[2] * gg[1]) * ff,
+                              (aa[2] * gg[0] - aa[0] * gg[2]) * ff,
+                              (aa[0] * gg[1] - aa[1] * gg[0]) * ff};
+            for (int k = 0; k < 3; ++k) {
+              const zc KB = d.nrot[n] * gg[k] * ff, KC = aa[k] * G * ff;
+              sE[k] += KA[k] * SE[n] + 1.0 / iwe * KB * SH[n] - (-iwm) * KC * SH[n];
+              sH[k] += KA[k] * SH[n] + 1.0 / (-iwm) * KB * SE[n] - iwe * KC * SE[n];
+            }
+          }
+        }
+    }
+    for (int k = 0; k < 3; ++k) {
+      E[3 * p + k] = sE[k];
+      H[3 * p + k] = sH[k];
+    }
+  }
+  free(seh);
+}

@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "libspeckle_gpu.so")
 
 # Exported entry points, in include/speckle_gpu.h order.
 EXPORTS = [
-    "spk_create", "spk_destroy", "spk_last_error", "spk_setup_tri", "spk_get_stats",
+    "spk_create", "spk_destroy", "spk_last_error", "spk_setup_tri", "spk_setup_quad", "spk_get_stats",
     "spk_num_edges", "spk_matvec", "spk_matvec_device", "spk_set_transposed",
     "spk_get_transposed", "spk_set_inner_graph", "spk_adjoint_height_gradient",
     "spk_autotune", "spk_set_tuning", "spk_set_level_truncation", "spk_interp_probe",
@@ -139,6 +139,7 @@ def lib():
     L.spk_destroy.argtypes = [vp]
     L.spk_destroy.restype = None
     L.spk_setup_tri.argtypes = [vp, C.c_int, dp, C.c_int, ip]
+    L.spk_setup_quad.argtypes = [vp, C.c_double, C.c_double, C.c_int, C.c_int, dp]
     L.spk_get_stats.argtypes = [vp, C.POINTER(Stats)]
     L.spk_num_edges.argtypes = [vp, i64p]
     L.spk_matvec.argtypes = [vp, dp, dp]

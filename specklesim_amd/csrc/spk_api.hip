@@ -20,6 +20,7 @@
 
 #include "spk_device.h"
 #include "spk_dbuf.h"
+#include "spk_quad.h"
 #include "spk_tree.h"
 
 using namespace spk;
@@ -138,6 +139,22 @@ struct spk_handle {
   BatchBufs<zf> bf;
   int spmv_wgs = 0;
   DBuf<zd> d_rt, d_W;
+  // 8-node quadrilaterals with 10 edge functions (spk_setup_quad): the mesh
+  // and neighbour arrays, kernel constants, the patterns' quadrature points,
+  // element-local solution coefficients (scattered field), fold constants.
+  // Coefficient buffers of every level then hold two halves of 8 channels
+  // ((TJx, TJy) and (TJz, TD), each [comp*4 + cur*2 + k]).
+  bool quad = false;
+  QuadMesh qmesh;
+  QuadDev qdev{};
+  zd q_eta_a{}, q_eta12{};
+  QuadFoldC qfold{};
+  DBuf<double> q_el, q_pts;
+  DBuf<int> q_edge_p, q_eadj, q_eside, q_cadj;
+  DBuf<zd> q_seh;
+  QuadTopo qtopo() const {
+    return QuadTopo{q_el.p, q_edge_p.p, q_eadj.p, q_eside.p, q_cadj.p, qmesh.nel, qmesh.medge};
+  }
   DBuf<double> d_khat, d_tth, d_tph, d_q, d_leaf_centre, d_trf;
   std::vector<LevelDev> lev;
   DBuf<zd> d_x, d_y, d_yfar, d_tmp;
@@ -298,6 +315,119 @@ int near_fill(spk_handle *h) {
   return SPK_OK;
 }
 
+// Near field of a quadrilateral handle.  For leaf box b, T(b) holds the
+// elements of its E2 columns (each column's one or two elements); the
+// pairs (row edge function, element of T(b)) are computed once, then every
+// block combines the pairs of its column's elements as the dense assembly
+// does (lib_sie_quad_solver_normal.f90:165-210).  Boxes go in chunks whose
+// pair sums fit kQuadPairChunk pairs.
+constexpr int64_t kQuadPairChunk = 1 << 22;
+
+int near_fill_quad(spk_handle *h) {
+  const Tree &t = h->tree;
+  const QuadMesh &m = h->qmesh;
+  const int nbl = (int)t.box_first.size();
+  const int64_t nnz = t.nnz;
+  HIPCHK(h->d_blocks.alloc(4 * (size_t)nnz));
+  if (nnz == 0) return SPK_OK;
+  std::vector<int> tb_off(nbl + 1, 0), tb_list, colA(t.e2_list.size()), colB(t.e2_list.size(), -1);
+  std::vector<int64_t> pbase(nbl + 1, 0);
+  std::vector<int> els;
+  for (int b = 0; b < nbl; ++b) {
+    els.clear();
+    for (int ci = t.e2_off[b]; ci < t.e2_off[b + 1]; ++ci) {
+      const int *e = &m.edge_p[5 * (size_t)t.e2_list[ci]];
+      els.push_back(e[0] - 1);
+      if (e[2] > 0) els.push_back(e[2] - 1);
+    }
+    std::sort(els.begin(), els.end());
+    els.erase(std::unique(els.begin(), els.end()), els.end());
+    auto idx = [&](int el) {
+      return (int)(std::lower_bound(els.begin(), els.end(), el) - els.begin());
+    };
+    for (int ci = t.e2_off[b]; ci < t.e2_off[b + 1]; ++ci) {
+      const int *e = &m.edge_p[5 * (size_t)t.e2_list[ci]];
+      colA[ci] = idx(e[0] - 1);
+      if (e[2] > 0) colB[ci] = idx(e[2] - 1);
+    }
+    tb_off[b + 1] = tb_off[b] + (int)els.size();
+    tb_list.insert(tb_list.end(), els.begin(), els.end());
+    pbase[b + 1] = pbase[b] + (int64_t)t.box_count[b] * (int64_t)els.size();
+  }
+  DBuf<int> d_tb_off, d_tb_list, d_colA, d_colB, l_s, l_es, l_et, l_side;
+  DBuf<int64_t> d_pbase, l_sdst, l_edst;
+  DBuf<zd> sums, so, eo;
+  HIPCHK(d_tb_off.upload(tb_off));
+  HIPCHK(d_tb_list.upload(tb_list));
+  HIPCHK(d_colA.upload(colA));
+  HIPCHK(d_colB.upload(colB));
+  HIPCHK(d_pbase.upload(pbase));
+  const QuadTopo T = h->qtopo();
+  std::vector<int> ls, es, et, eside;
+  std::vector<int64_t> sdst, edst;
+  int64_t max_pairs = 0;
+  for (int b = 0; b < nbl; ++b) max_pairs = std::max(max_pairs, pbase[b + 1] - pbase[b]);
+  HIPCHK(sums.alloc((size_t)std::max(max_pairs, std::min(kQuadPairChunk, pbase[nbl])) * 30));
+  int singular = 0;
+  for (int b0 = 0; b0 < nbl;) {
+    int b1 = b0 + 1;
+    while (b1 < nbl && pbase[b1 + 1] - pbase[b0] <= kQuadPairChunk) ++b1;
+    const int64_t nq = pbase[b1] - pbase[b0];
+    ls.clear();
+    sdst.clear();
+    es.clear();
+    et.clear();
+    eside.clear();
+    edst.clear();
+    for (int b = b0; b < b1; ++b) {
+      const int *tl = &tb_list[tb_off[b]];
+      const int nT = tb_off[b + 1] - tb_off[b];
+      for (int r = 0; r < t.box_count[b]; ++r) {
+        const int s = t.perm[t.box_first[b] + r];
+        const int ii = m.edge_p[5 * (size_t)s] - 1;
+        const int64_t q0 = pbase[b] - pbase[b0] + (int64_t)r * nT;
+        const int *pi = std::lower_bound(tl, tl + nT, ii);
+        ls.push_back(s);
+        sdst.push_back(q0 + (pi - tl));
+        for (const QuadEdgeAdj &a : m.eadj[ii]) {
+          const int *pt = std::lower_bound(tl, tl + nT, a.other - 1);
+          if (pt == tl + nT || *pt != a.other - 1) continue;
+          es.push_back(s);
+          et.push_back(a.other - 1);
+          eside.push_back(a.side[0]);
+          eside.push_back(a.side[1]);
+          edst.push_back(q0 + (pt - tl));
+        }
+      }
+    }
+    launch_qn_regular(h->qdev, T, b0, b1, nq, d_pbase.p, d_tb_off.p, d_tb_list.p,
+                      h->d_box_first.p, h->d_perm.p, sums.p, h->st);
+    HIPCHK(l_s.upload(ls));
+    HIPCHK(l_sdst.upload(sdst));
+    HIPCHK(so.alloc(ls.size() * 30));
+    launch_quad_self(h->qdev, T, (int)ls.size(), l_s.p, so.p, h->st);
+    HIPCHK(l_es.upload(es));
+    HIPCHK(l_et.upload(et));
+    HIPCHK(l_side.upload(eside));
+    HIPCHK(l_edst.upload(edst));
+    HIPCHK(eo.alloc(std::max<size_t>(1, es.size()) * 30));
+    launch_quad_edge_adj(h->qdev, T, (int)es.size(), l_es.p, l_et.p, l_side.p, eo.p, h->st);
+    launch_qn_scatter((int)ls.size(), l_sdst.p, so.p, sums.p, h->st);
+    launch_qn_scatter((int)es.size(), l_edst.p, eo.p, sums.p, h->st);
+    launch_qn_assemble(T, h->p.formulation, h->q_eta_a, h->q_eta12, b0, b1,
+                       t.near_off[b1] - t.near_off[b0], h->d_near_off.p, h->d_e2_off.p,
+                       h->d_e2_list.p, d_colA.p, d_colB.p, d_pbase.p, d_tb_off.p, sums.p,
+                       h->d_blocks.p, h->st);
+    HIPCHK(hipGetLastError());
+    // the host lists are reused by the next chunk
+    HIPCHK(hipStreamSynchronize(h->st));
+    singular += (int)(ls.size() + es.size());
+    b0 = b1;
+  }
+  h->stats.singular_pairs = singular;
+  return SPK_OK;
+}
+
 // the host's per-level table inputs: TL slot factors and SS/RR phases
 struct HostLevelTables {
   std::vector<double> rhat;
@@ -340,10 +470,19 @@ int setup_mlfmm(spk_handle *h, std::future<std::vector<HostLevelTables>> &pre) {
     q[40 + i] = h->qw[i];
   }
   HIPCHK(h->d_q.upload(q));
-  HIPCHK(h->d_rt.alloc((size_t)N * 4 * S));
-  launch_patterns(N, S, h->ngp, h->d_fn.p, h->d_perm.p, h->d_leaf_box.p, h->d_leaf_centre.p,
-                  h->d_khat.p, h->d_tth.p, h->d_tph.p, h->d_q.p, h->d_q.p + 20, h->d_q.p + 40,
-                  zcx(h->mat.k1), zcx(h->mat.k2), h->d_rt.p, h->st);
+  if (h->quad) {
+    // quadrature points of every position's patterns, then the patterns
+    HIPCHK(h->q_pts.alloc((size_t)N * kQuadPtStride));
+    launch_qpoints(h->qdev, h->qtopo(), N, h->d_perm.p, h->d_leaf_box.p, h->d_leaf_centre.p,
+                   h->q_pts.p, h->st);
+    HIPCHK(h->d_rt.alloc((size_t)N * kQuadRtRows * S));
+    launch_qpatterns(N, S, h->d_khat.p, h->q_pts.p, h->qdev.k1, h->qdev.k2, h->d_rt.p, h->st);
+  } else {
+    HIPCHK(h->d_rt.alloc((size_t)N * 4 * S));
+    launch_patterns(N, S, h->ngp, h->d_fn.p, h->d_perm.p, h->d_leaf_box.p, h->d_leaf_centre.p,
+                    h->d_khat.p, h->d_tth.p, h->d_tph.p, h->d_q.p, h->d_q.p + 20, h->d_q.p + 40,
+                    zcx(h->mat.k1), zcx(h->mat.k2), h->d_rt.p, h->st);
+  }
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(h->st));
   h->stats.patterns_ms = now_ms() - t0;
@@ -362,8 +501,10 @@ int setup_mlfmm(spk_handle *h, std::future<std::vector<HostLevelTables>> &pre) {
     D.S = tl.S;
     HIPCHK(D.khat.upload(tl.khat));
     HIPCHK(D.trf.upload(tl.trf));
-    HIPCHK(D.B.alloc((size_t)D.nb * 8 * D.S));
-    HIPCHK(D.D.alloc((size_t)D.nb * 8 * D.S));
+    // two halves of 8 channels on quadrilateral handles
+    const size_t halves = h->quad ? 2 : 1;
+    HIPCHK(D.B.alloc(halves * D.nb * 8 * D.S));
+    HIPCHK(D.D.alloc(halves * D.nb * 8 * D.S));
     if (l > t.l_min && h->levK[l - 1] != h->levK[l]) {
       // interpolation to the parent's grid, and the downward map
       std::vector<int> r, c, tr, tc;
@@ -422,7 +563,7 @@ int setup_mlfmm(spk_handle *h, std::future<std::vector<HostLevelTables>> &pre) {
     h->stats.boxes_per_level[l] = D.nb;
     h->stats.i4_pairs_per_level[l] = (int64_t)L.i4_box.size();
   }
-  HIPCHK(h->d_W.alloc((size_t)LL.n.size() * 8 * S));
+  HIPCHK(h->d_W.alloc((size_t)LL.n.size() * (h->quad ? 12 : 8) * S));
   if (tmp_need) HIPCHK(h->lvl_tmp.alloc(tmp_need));
   h->stats.tables_ms = now_ms() - t0;
   // formulation scaling of the far field (MLFMM_tri_summation_New :478-495)
@@ -441,6 +582,33 @@ int setup_mlfmm(spk_handle *h, std::future<std::vector<HostLevelTables>> &pre) {
   }
   h->sc1 = zd{s1.real(), s1.imag()};
   h->sc2 = zd{s2.real(), s2.imag()};
+  if (h->quad) {
+    // the quadrilateral rows' fold constants: the dense fill's (A, DE, DH)
+    // combination (combine of ff_ABC_II_Edge's sums) in plane-wave form, and
+    // the assembly's formulation scalings of the four blocks
+    const QuadDev &Q = h->qdev;
+    const zc iwe(Q.iwe.x, Q.iwe.y), iwm(Q.iwm.x, Q.iwm.y);
+    const zc ea(h->q_eta_a.x, h->q_eta_a.y), e12(h->q_eta12.x, h->q_eta12.y);
+    zc sa = 1.0, sb = 1.0, sc = 1.0, sd = 1.0;
+    if (h->p.formulation == SPK_ICTF) {
+      sa = sb = 1.0 / ea;
+      sc = sd = ea;
+    } else if (h->p.formulation == SPK_MCTF) {
+      sc = sd = e12;
+    }
+    auto Z = [](zc c) { return zd{c.real(), c.imag()}; };
+    for (int k = 0; k < 2; ++k) {
+      const zc km = k ? zc(Q.k2.x, Q.k2.y) : zc(Q.k1.x, Q.k1.y);
+      const zc em = k ? zc(Q.e2.x, Q.e2.y) : zc(Q.e1.x, Q.e1.y);
+      const zc mik = zc(0.0, -1.0) * km;
+      h->qfold.cAE[k] = Z(sa * mik);
+      h->qfold.cJE[k] = Z(sb * iwm);
+      h->qfold.cDE[k] = Z(sb * mik / (em * iwe));
+      h->qfold.cJH[k] = Z(-sc * iwe * em);
+      h->qfold.cDH[k] = Z(-sc * mik / iwm);
+      h->qfold.cAH[k] = Z(sd * mik);
+    }
+  }
   return SPK_OK;
 }
 
@@ -554,6 +722,61 @@ static int matvec_multik(spk_handle *h, const zd *x, zd *y, bool near_on, bool f
   return SPK_OK;
 }
 
+// y = Z x of a quadrilateral handle: the radiating patterns aggregated into
+// 16 channels per sample (two halves of the levels' buffers), SS, M2L and
+// RR on each half, the fold into the rows' two vectors per medium, the
+// receiving patterns' disaggregation; the near field as for RWG edges.
+template <class T>
+static int matvec_quad_t(spk_handle *h, const T *x, T *y, bool near_on, bool far_on,
+                         hipEvent_t *ev) {
+  T *tag = nullptr;
+  const Tree &t = h->tree;
+  const int S = h->tb.S, N = h->N, lmax = t.l_max, lmin = t.l_min;
+  hipStream_t st = h->st;
+  T *yfar = pYf(h, tag);
+  auto mark = [&](int i) {
+    if (ev) (void)hipEventRecord(ev[i], st);
+  };
+  auto half = [&](int l) { return (size_t)h->lev[l].nb * 8 * S; };
+  mark(0);
+  if (far_on) {
+    LevelDev &LL = h->lev[lmax];
+    launch_qagg(LL.nb, S, N, h->d_box_first.p, h->d_box_count.p, h->d_perm.p, pRt(h, tag), x,
+                pB(LL, tag), half(lmax), st);
+    mark(1);
+    for (int l = lmax - 1; l >= lmin; --l)
+      for (int c = 0; c < 2; ++c)
+        launch_ss(h->lev[l].nb, S, h->lev[l].child.p, (const T *)pB(h->lev[l + 1], tag) + c * half(l + 1),
+                  pUp(h->lev[l + 1], tag), pB(h->lev[l], tag) + c * half(l), st);
+    mark(2);
+    for (int l = lmin; l <= lmax; ++l)
+      for (int c = 0; c < 2; ++c)
+        launch_m2l(h->lev[l].nb, S, h->lev[l].i4_off.p, h->lev[l].i4_box.p, h->lev[l].i4_slot.p,
+                   (const T *)pB(h->lev[l], tag) + c * half(l), pT(h->lev[l], tag),
+                   pD(h->lev[l], tag) + c * half(l), st);
+    mark(3);
+    for (int l = lmin + 1; l <= lmax; ++l)
+      for (int c = 0; c < 2; ++c)
+        launch_rr(h->lev[l].nb, S, h->lev[l].parent.p, h->lev[l].octant.p,
+                  (const T *)pD(h->lev[l - 1], tag) + c * half(l - 1), pDn(h->lev[l], tag),
+                  pD(h->lev[l], tag) + c * half(l), st);
+    mark(4);
+    launch_qfold(LL.nb, S, (const T *)pD(LL, tag), half(lmax), h->d_khat.p, h->qfold, pW(h, tag), st);
+    mark(5);
+    launch_qdisagg(h->disagg_tasks, N, S, h->d_disagg_task.p, h->d_perm.p, h->d_leafbox_of_pos.p,
+                   pRt(h, tag), (const T *)pW(h, tag), near_on ? yfar : y, st);
+    mark(6);
+  }
+  if (near_on)
+    launch_near_spmv_rows(h->spmv_wgs, N, h->d_row_box.p, h->d_row_r0.p, h->d_box_first.p,
+                          h->d_box_count.p, h->d_e2_off.p, h->d_e2_list.p, h->d_near_off.p,
+                          h->d_perm.p, pBlk(h, tag), x, far_on ? (const T *)yfar : nullptr, y,
+                          st);
+  mark(7);
+  HIPCHK(hipGetLastError());
+  return SPK_OK;
+}
+
 // y = Z x on device vectors (2N complex each), far part into yfar first.
 // T = zd: the FP64 operator; T = zf: its complex64 mirror.
 // ev (optional, 8 events): recorded before stage 0 and after stages 0..6.
@@ -561,6 +784,10 @@ template <class T>
 int matvec_t(spk_handle *h, const T *x, T *y, bool near_on = true, bool far_on = true,
              hipEvent_t *ev = nullptr) {
   T *tag = nullptr;
+  if (h->quad) {
+    if (h->transposed) return fail(SPK_ERR_STATE, "quadrilateral handles: no transposed operator");
+    return matvec_quad_t<T>(h, x, y, near_on, far_on, ev);
+  }
   if (h->multik) {
     if constexpr (std::is_same<T, zd>::value) {
       if (h->transposed) return fail(SPK_ERR_STATE, "per-level truncation numbers: no transposed operator");
@@ -698,6 +925,8 @@ int matvec_batch_t(spk_handle *h, int R, const T *const *xs, T *const *ys, bool 
     return fail(SPK_ERR_STATE, "the batched products do not take the transposed operator");
   if (far_on && h->multik)
     return fail(SPK_ERR_STATE, "the batched products do not take per-level truncation numbers");
+  if (far_on && h->quad)
+    return fail(SPK_ERR_STATE, "quadrilateral handles: batched far-field products are not built");
   int rc = batch_ensure<T>(h, R);
   if (rc) return rc;
   BatchBufs<T> &BB = pBB(h, tag);
@@ -866,24 +1095,10 @@ static int build_mx_groups(spk_handle *h, int tpw) {
   return SPK_OK;
 }
 
-int spk_setup_tri(spk_handle *h, int np, const double *pp, int nt, const int *tt) {
-  if (!h || !pp || !tt || np < 3 || nt < 1) return fail(SPK_ERR_ARG, "bad mesh arguments");
-  HIPCHK(hipSetDevice(h->p.device));
-  h->ready = false;
-  h->mixed_ready = false;
-  h->diag_ready = false;
-  ++h->setup_serial;
-  h->ig.reset();
-  // the low-precision near-field copies belong to the previous operator
-  h->near_bf_ok = h->near_f8_ok = false;
-  h->stats = spk_stats{};
-  const double t0 = now_ms();
-  std::string err;
-  int rc = build_rwg(np, pp, nt, tt, h->p.normal_convention, &h->corner, &h->element, &h->fn,
-                     &h->er, &h->rref, &err);
-  if (rc != SPK_OK) return fail(rc, err);
-  h->N = (int)h->rref.size();
-  if (h->N < 1) return fail(SPK_ERR_MESH, "mesh has no interior edges (no RWG pairs)");
+// The operator of the current topology (h->fn: each unknown's node pair,
+// whose midpoint places it in the octree): the octree and its lists, the
+// truncation numbers, launch lists, the near field and the MLFMM tables.
+static int setup_operator(spk_handle *h, double t0) {
   // the octree and its lists: on the device (the host builder with
   // SPK_HOST_TREE=1; both give the same lists)
   const char *ht = std::getenv("SPK_HOST_TREE");
@@ -917,6 +1132,8 @@ int spk_setup_tri(spk_handle *h, int np, const double *pp, int nt, const int *tt
     }
     if (h->multik && h->p.precision == SPK_MIXED)
       return fail(SPK_ERR_ARG, "per-level truncation numbers run on FP64 handles");
+    if (h->multik && h->quad)
+      return fail(SPK_ERR_ARG, "per-level truncation numbers: triangle meshes only");
     make_tables(h->levK[tr.l_max], &h->tb);
     h->tbl.assign(tr.l_max + 1, Tables{});
     for (int l = tr.l_min; l <= tr.l_max; ++l) {
@@ -928,9 +1145,6 @@ int spk_setup_tri(spk_handle *h, int np, const double *pp, int nt, const int *tt
   }
   const Tree &t = h->tree;
   h->stats.tree_ms = now_ms() - t0;
-  // upload topology
-  HIPCHK(h->d_er.upload(h->er));
-  HIPCHK(h->d_rref.upload(h->rref));
   if (host_tree) {
     HIPCHK(h->d_perm.upload(t.perm));
     HIPCHK(h->d_box_first.upload(t.box_first));
@@ -944,7 +1158,6 @@ int spk_setup_tri(spk_handle *h, int np, const double *pp, int nt, const int *tt
     for (int p = 0; p < h->N; ++p) lbp[p] = t.leaf_box[t.perm[p]];
     HIPCHK(h->d_leafbox_of_pos.upload(lbp));
   }
-  h->pp_host.assign(pp, pp + 3 * (size_t)np);
   h->transposed = false;
   std::vector<int> rb, r0;
   for (size_t b = 0; b < t.box_count.size(); ++b)
@@ -955,10 +1168,10 @@ int spk_setup_tri(spk_handle *h, int np, const double *pp, int nt, const int *tt
   h->spmv_wgs = (int)rb.size();
   HIPCHK(h->d_row_box.upload(rb));
   HIPCHK(h->d_row_r0.upload(r0));
-  if ((rc = build_mx_groups(h, h->tune_tpw > 0 ? h->tune_tpw : near_mx_tiles_per_group())))
-    return rc;
+  if (int e = build_mx_groups(h, h->tune_tpw > 0 ? h->tune_tpw : near_mx_tiles_per_group()))
+    return e;
   // disaggregation tasks: up to kDisaggEdges consecutive positions of one box
-  if (h->N >= (1 << 28)) return fail(SPK_ERR_ARG, "more than 2^28 RWG edges");
+  if (h->N >= (1 << 28)) return fail(SPK_ERR_ARG, "more than 2^28 unknowns per current");
   std::vector<int> dt;
   for (size_t b = 0; b < t.box_count.size(); ++b)
     for (int r = 0; r < t.box_count[b]; r += kDisaggEdges)
@@ -997,7 +1210,7 @@ int spk_setup_tri(spk_handle *h, int np, const double *pp, int nt, const int *tt
       std::async(std::launch::async, host_level_tables, std::cref(h->tbl), std::cref(h->mat),
                  std::cref(h->tree));
   double t1 = now_ms();
-  rc = near_fill(h);
+  int rc = h->quad ? near_fill_quad(h) : near_fill(h);
   if (rc != SPK_OK) return rc;
   h->stats.near_fill_ms = now_ms() - t1;
   rc = setup_mlfmm(h, pre);
@@ -1008,13 +1221,117 @@ int spk_setup_tri(spk_handle *h, int np, const double *pp, int nt, const int *tt
   h->stats.l_max = t.l_max;
   h->stats.samples = h->tb.S;
   h->stats.bytes_near = (double)t.nnz * 64.0;
-  h->stats.bytes_patterns = (double)h->N * 4 * h->tb.S * 16.0;
+  h->stats.bytes_patterns = (double)h->N * (h->quad ? kQuadRtRows : 4) * h->tb.S * 16.0;
   if (h->p.precision == SPK_MIXED) {
     rc = build_mixed_mirror(h);
     if (rc != SPK_OK) return rc;
   }
   h->stats.setup_ms = now_ms() - t0;
   h->ready = true;
+  return SPK_OK;
+}
+
+
+int spk_setup_tri(spk_handle *h, int np, const double *pp, int nt, const int *tt) {
+  if (!h || !pp || !tt || np < 3 || nt < 1) return fail(SPK_ERR_ARG, "bad mesh arguments");
+  HIPCHK(hipSetDevice(h->p.device));
+  h->ready = false;
+  h->mixed_ready = false;
+  h->diag_ready = false;
+  ++h->setup_serial;
+  h->ig.reset();
+  // the low-precision near-field copies belong to the previous operator
+  h->near_bf_ok = h->near_f8_ok = false;
+  h->stats = spk_stats{};
+  h->quad = false;
+  const double t0 = now_ms();
+  std::string err;
+  int rc = build_rwg(np, pp, nt, tt, h->p.normal_convention, &h->corner, &h->element, &h->fn,
+                     &h->er, &h->rref, &err);
+  if (rc != SPK_OK) return fail(rc, err);
+  h->N = (int)h->rref.size();
+  if (h->N < 1) return fail(SPK_ERR_MESH, "mesh has no interior edges (no RWG pairs)");
+  HIPCHK(h->d_er.upload(h->er));
+  HIPCHK(h->d_rref.upload(h->rref));
+  h->pp_host.assign(pp, pp + 3 * (size_t)np);
+  return setup_operator(h, t0);
+}
+
+int spk_setup_quad(spk_handle *h, double Lx, double Ly, int nx, int ny, const double *z) {
+  if (!h || !z || nx < 1 || ny < 1 || !(Lx > 0) || !(Ly > 0))
+    return fail(SPK_ERR_ARG, "bad quadrilateral mesh arguments");
+  const int f = h->p.formulation;
+  if (f != SPK_PMCHWT && f != SPK_MCTF && f != SPK_ICTF)
+    return fail(SPK_ERR_ARG, "quadrilateral meshes support PMCHWT, MCTF and ICTF");
+  if (h->p.eps_r1_re == 0 && h->p.eps_r1_im == 0)
+    return fail(SPK_ERR_ARG, "eps_r1 must be given (1 for vacuum)");
+  HIPCHK(hipSetDevice(h->p.device));
+  h->ready = false;
+  h->mixed_ready = false;
+  h->diag_ready = false;
+  ++h->setup_serial;
+  h->ig.reset();
+  h->near_bf_ok = h->near_f8_ok = false;
+  h->stats = spk_stats{};
+  h->quad = true;
+  const double t0 = now_ms();
+  // sorting_quad, Neighbours_EdgeCorner_Full, Sorting_edge
+  QuadMesh &m = h->qmesh;
+  m = QuadMesh();
+  quad_mesh(Lx, Ly, nx, ny, z, &m);
+  quad_topology(&m);
+  if (m.medge < 1) return fail(SPK_ERR_MESH, "mesh has no edge functions");
+  std::vector<double> el;
+  std::vector<int> eadj, eside, cadj;
+  if (!quad_topology_arrays(m, &el, &eadj, &eside, &cadj))
+    return fail(SPK_ERR_MESH, "element with more than 8 neighbours");
+  HIPCHK(h->q_el.upload(el));
+  HIPCHK(h->q_edge_p.upload(m.edge_p));
+  HIPCHK(h->q_eadj.upload(eadj));
+  HIPCHK(h->q_eside.upload(eside));
+  HIPCHK(h->q_cadj.upload(cadj));
+  quad_constants(h->p, &h->qdev, &h->q_eta_a, &h->q_eta12);
+  HIPCHK(h->q_seh.alloc(2 * (size_t)m.nel * 10));
+  // each edge function's node pair in the records the octree reads
+  h->N = m.medge;
+  h->fn.assign(2 * (size_t)h->N, FnRec{});
+  for (int s = 0; s < h->N; ++s) {
+    int n1, n2;
+    quad_edge_nodes(m, s, &n1, &n2);
+    FnRec &r = h->fn[2 * (size_t)s];
+    for (int d = 0; d < 3; ++d) {
+      r.c[d] = m.pp[3 * (size_t)(n1 - 1) + d];
+      r.c[3 + d] = m.pp[3 * (size_t)(n2 - 1) + d];
+    }
+    h->fn[2 * (size_t)s + 1] = r;
+  }
+  h->corner.clear();
+  h->element.clear();
+  h->er.clear();
+  h->rref.clear();
+  h->pp_host.clear();
+  int rc = setup_operator(h, t0);
+  if (rc != SPK_OK) return rc;
+  // The far part stands for the regular element-pair rule only where the
+  // supports of functions in non-adjacent leaf boxes share no node: every
+  // support point lies within one element diameter of its function's
+  // midpoint, and such midpoints are more than a leaf edge apart.
+  double diam = 0;
+  for (int e = 0; e < m.nel; ++e)
+    for (int a = 0; a < 8; ++a)
+      for (int b = a + 1; b < 8; ++b) {
+        const double *pa = &el[(size_t)e * 24 + 3 * a], *pb = &el[(size_t)e * 24 + 3 * b];
+        const double d2 = (pa[0] - pb[0]) * (pa[0] - pb[0]) + (pa[1] - pb[1]) * (pa[1] - pb[1]) +
+                          (pa[2] - pb[2]) * (pa[2] - pb[2]);
+        diam = std::max(diam, std::sqrt(d2));
+      }
+  const double leaf = std::ldexp(h->tree.D, -h->tree.l_max);
+  if (h->tree.l_max >= 2 && leaf <= 2 * diam) {
+    h->ready = false;
+    return fail(SPK_ERR_ARG, "leaf boxes (" + std::to_string(leaf) +
+                                 " m) must exceed twice the element diameter (" +
+                                 std::to_string(diam) + " m): raise tree_s_opt");
+  }
   return SPK_OK;
 }
 
@@ -1076,6 +1393,7 @@ int spk_set_transposed(spk_handle *h, int transposed) {
   if (!h->ready) return fail(SPK_ERR_STATE, "spk_setup_tri has not completed");
   const bool want = transposed != 0;
   if (want == h->transposed) return SPK_OK;
+  if (want && h->quad) return fail(SPK_ERR_STATE, "quadrilateral handles: no transposed operator");
   if (want && h->multik)
     return fail(SPK_ERR_STATE, "per-level truncation numbers: no transposed operator");
   HIPCHK(hipSetDevice(h->p.device));
@@ -1148,6 +1466,13 @@ static int matvec_batch_api(spk_handle *h, int R, const void *const *d_x, void *
   if (R < 1 || R > kBatchMax) return fail(SPK_ERR_ARG, "batch size must be 1..16");
   for (int r = 0; r < R; ++r)
     if (!d_x[r] || !d_y[r]) return fail(SPK_ERR_ARG, "null vector in the batch");
+  if (h->quad) {
+    // quadrilateral handles: one product per vector
+    for (int r = 0; r < R; ++r)
+      if (int e = matvec_t<T>(h, (const T *)d_x[r], (T *)d_y[r])) return e;
+    HIPCHK(hipStreamSynchronize(h->st));
+    return SPK_OK;
+  }
   int rc = batch_ensure<T>(h, R);
   if (rc) return rc;
   T *tag = nullptr;
@@ -1261,6 +1586,13 @@ static RhsArgs rhs_args(const spk_handle *h, const double k_in[3], const double 
 }
 
 static int rhs_plane_dev(spk_handle *h, const double k_in[3], const double e_in[3], zd *d_b) {
+  if (h->quad) {
+    // get_vector_b_quad (src/lib_sie_quad_calculation_mod.f90:91-138)
+    const QuadRhsArgs a = quad_rhs_args(h->p.formulation, h->qdev, h->q_eta_a, h->q_eta12, k_in, e_in);
+    launch_quad_rhs_plane(h->qdev, h->qtopo(), a, d_b, h->st);
+    HIPCHK(hipGetLastError());
+    return SPK_OK;
+  }
   const RhsArgs a = rhs_args(h, k_in, e_in);
   launch_rhs_plane(a, h->d_fn.p, d_b, h->st);
   HIPCHK(hipGetLastError());
@@ -1317,7 +1649,13 @@ static int scattered_staged(spk_handle *h, int npts, const double *r, const zd *
                             double *H, double *dr, zd *dE, zd *dH) {
   const ScatArgs a = scat_args(h, npts);
   HIPCHK(hipMemcpyAsync(dr, r, 3 * (size_t)npts * sizeof(double), hipMemcpyHostToDevice, h->st));
-  launch_scattered(a, h->d_fn.p, d_x, dr, dE, dH, h->st);
+  if (h->quad) {
+    // element-local coefficients (every slot written), then the field
+    launch_quad_local(h->qtopo(), d_x, h->q_seh.p, h->st);
+    launch_quad_field(h->qdev, h->qtopo(), npts, dr, h->q_seh.p, (double *)dE, (double *)dH, h->st);
+  } else {
+    launch_scattered(a, h->d_fn.p, d_x, dr, dE, dH, h->st);
+  }
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(E, dE, 3 * (size_t)npts * sizeof(zd), hipMemcpyDeviceToHost, h->st));
   HIPCHK(hipMemcpyAsync(H, dH, 3 * (size_t)npts * sizeof(zd), hipMemcpyDeviceToHost, h->st));
@@ -1728,7 +2066,7 @@ static int gmres_mixed_batch(spk_handle *h, int R, const zd *const *b, zd *const
       yp.push_back(Tr(r));
     }
     int e = SPK_OK;
-    if (set.size() < 4) {
+    if (set.size() < 4 || h->quad) {
       // a few stragglers: the single-vector operator is faster than a narrow batch
       for (size_t i = 0; i < set.size() && !e; ++i) e = matvec_t<zd>(h, xp[i], yp[i]);
     } else {
@@ -1787,7 +2125,7 @@ static int gmres_mixed_batch(spk_handle *h, int R, const zd *const *b, zd *const
       xp.push_back(Vr(r) + (size_t)S[r].j * n);
       yp.push_back(Wr(r));
     }
-    if (act.size() < 4) {
+    if (act.size() < 4 || h->quad) {
       for (size_t i = 0; i < act.size() && !rc; ++i) rc = matvec_t<zf>(h, xp[i], yp[i]);
     } else {
       HIPCHK(hipMemcpy(xs32.p, xp.data(), act.size() * sizeof(void *), hipMemcpyHostToDevice));
@@ -2078,7 +2416,7 @@ static int fgmres_near(spk_handle *h, int R, const zd *const *b, zd *const *x,
       if (relax) {
         for (size_t i = 0; i < xs.size(); ++i)
           launch_scale_down(n, 1.0, xs[i], (zf *)(U.p + i * ldu), h->st);
-        if (xs.size() < 4) {
+        if (xs.size() < 4 || h->quad) {
           for (size_t i = 0; i < xs.size(); ++i) {
             const int e = matvec_t<zf>(h, (const zf *)(U.p + i * ldu), (zf *)(U.p + i * ldu + n));
             if (e) return e;
@@ -2101,7 +2439,7 @@ static int fgmres_near(spk_handle *h, int R, const zd *const *b, zd *const *x,
         return SPK_OK;
       }
     }
-    if (xs.size() < 4) {
+    if (xs.size() < 4 || h->quad) {
       for (size_t i = 0; i < xs.size(); ++i) {
         const int e = matvec_t<zd>(h, xs[i], ys[i]);
         if (e) return e;
@@ -2953,6 +3291,7 @@ int spk_adjoint_height_gradient(spk_handle *h, const double k_in[3], const doubl
       (n_nodes > 0 && (!nodes || !grad)) || !(dz > 0))
     return fail(SPK_ERR_ARG, "bad argument");
   if (!h->ready) return fail(SPK_ERR_STATE, "spk_setup_tri has not completed");
+  if (h->quad) return fail(SPK_ERR_STATE, "the adjoint gradient runs on triangle meshes");
   if (h->transposed) return fail(SPK_ERR_STATE, "the handle holds the transposed operator");
   const int np = (int)(h->pp_host.size() / 3);
   for (int i = 0; i < n_nodes; ++i)
@@ -3184,6 +3523,7 @@ int spk_gmres_history(const spk_handle *h, double *hist, int cap, int *count) {
 int spk_get_rwg(const spk_handle *h, int *corner, int *element) {
   if (!h) return fail(SPK_ERR_ARG, "null argument");
   if (!h->ready) return fail(SPK_ERR_STATE, "spk_setup_tri has not completed");
+  if (h->quad) return fail(SPK_ERR_STATE, "quadrilateral handles have no RWG pairs");
   if (corner) std::memcpy(corner, h->corner.data(), h->corner.size() * sizeof(int));
   if (element) std::memcpy(element, h->element.data(), h->element.size() * sizeof(int));
   return SPK_OK;
@@ -3249,6 +3589,7 @@ int spk_get_near_csr(const spk_handle *h, int64_t *ia, int32_t *ja, double *abcd
 int spk_near_blocks(spk_handle *h, int npairs, const int *m, const int *n, double *out) {
   if (!h || !m || !n || !out || npairs < 0) return fail(SPK_ERR_ARG, "null argument");
   if (!h->ready || h->N <= 0 || !h->d_fn.p) return fail(SPK_ERR_STATE, "spk_setup_tri has not completed");
+  if (h->quad) return fail(SPK_ERR_STATE, "quadrilateral handles: use spk_get_near_csr");
   for (int i = 0; i < npairs; ++i)
     if (m[i] < 0 || m[i] >= h->N || n[i] < 0 || n[i] >= h->N)
       return fail(SPK_ERR_ARG, "edge index out of range");
@@ -3269,6 +3610,7 @@ int spk_get_patterns(const spk_handle *h, int first, int count, double *out) {
   if (!h || !out) return fail(SPK_ERR_ARG, "null argument");
   if (!h->ready) return fail(SPK_ERR_STATE, "spk_setup_tri has not completed");
   if (first < 0 || count < 0 || first + count > h->N) return fail(SPK_ERR_ARG, "edge range");
+  if (h->quad) return fail(SPK_ERR_STATE, "quadrilateral handles: patterns have their own layout");
   const int S = h->tb.S;
   for (int e = 0; e < count; ++e) {
     const int p = h->tree.pos[first + e];

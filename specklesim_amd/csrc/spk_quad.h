@@ -28,6 +28,7 @@ struct QuadMesh {
 void quad_mesh(double Lx, double Ly, int nx, int ny, const double *z, QuadMesh *m);
 void quad_topology(QuadMesh *m);
 void quad_side_index(const int ga[3], const int gq[3], int side[2]);
+void quad_edge_nodes(const QuadMesh &m, int s, int *n1, int *n2);
 
 // constants of the kernels (materials and 1-D Gauss-Legendre rules)
 struct QuadDev {
@@ -70,8 +71,61 @@ struct QuadRhsArgs {
 };
 void launch_quad_rhs_plane(const QuadDev &d, const QuadTopo &T, const QuadRhsArgs &a, zd *b,
                            hipStream_t st);
+// material constants and 1-D rules of the kernels (eta_a = (eta1 + eta2)/2,
+// eta12 = eta1 eta2 for the formulation scalings)
+void quad_constants(const spk_params &p, QuadDev *D, zd *eta_a, zd *eta12);
+QuadRhsArgs quad_rhs_args(int formulation, const QuadDev &D, zd eta_a, zd eta12,
+                          const double k_in[3], const double e_in[3]);
+// element nodes (nel * 24) and the neighbour arrays of QuadTopo; false when
+// an element has more than kQuadMaxAdj edge or corner neighbours
+bool quad_topology_arrays(const QuadMesh &m, std::vector<double> *el, std::vector<int> *eadj,
+                          std::vector<int> *eside, std::vector<int> *cadj);
 // MM (row-major 2M x 2M) rows [s0, s0+rows) from sums[r][t][30]
 void launch_quad_assemble(const QuadTopo &T, int formulation, zd eta_a, zd eta12, int s0,
                           int rows, const zd *sums, zd *MM, hipStream_t st);
+
+
+// MLFMM over the quadrilateral edge functions.  Constants of the fold per
+// medium (k = 0 exterior, 1 interior), with mik = -i k_m and the row
+// scalings sa..sd of the formulation:
+//   cAE = sa mik, cJE = sb iwm, cDE = sb mik / (eps_m iwe),
+//   cJH = -sc iwe eps_m, cDH = -sc mik / iwm, cAH = sd mik
+struct QuadFoldC {
+  zd cAE[2], cJE[2], cDE[2], cJH[2], cDH[2], cAH[2];
+};
+// doubles per position of the pattern quadrature points
+constexpr int kQuadPtStride = 16 * 6 + 32 * 7;
+// rows of the pattern array per position (8 radiating, 6 receiving)
+constexpr int kQuadRtRows = 14;
+void launch_qpoints(const QuadDev &d, const QuadTopo &T, int N, const int *perm,
+                    const int *leaf_box, const double *leaf_centre, double *pts, hipStream_t st);
+void launch_qpatterns(int N, int S, const double *khat, const double *pts, zd k1, zd k2, zd *rt,
+                      hipStream_t st);
+template <class T>
+void launch_qagg(int nbl, int S, int N, const int *box_first, const int *box_count,
+                 const int *perm, const T *rt, const T *x, T *B, size_t half, hipStream_t st);
+template <class T>
+void launch_qfold(int nbl, int S, const T *D, size_t half, const double *khat,
+                  const QuadFoldC &C, T *W, hipStream_t st);
+template <class T>
+void launch_qdisagg(int ntask, int N, int S, const int *task, const int *perm,
+                    const int *leaf_box_of_pos, const T *rt, const T *W, T *y, hipStream_t st);
+// near field of leaf boxes [b0, b1): pair sums (regular and corner pairs),
+// the self / edge-adjacent list results scattered to their pair index, then
+// the 2 x 2 blocks
+void launch_qn_regular(const QuadDev &d, const QuadTopo &T, int b0, int b1, int64_t nq,
+                       const int64_t *pbase, const int *tb_off, const int *tb_list,
+                       const int *box_first, const int *perm, zd *sums, hipStream_t st);
+void launch_qn_scatter(int n, const int64_t *dst, const zd *src, zd *sums, hipStream_t st);
+void launch_qn_assemble(const QuadTopo &T, int form, zd eta_a, zd eta12, int b0, int b1,
+                        int64_t nblocks, const int64_t *near_off, const int *e2_off,
+                        const int *e2_list, const int *colA, const int *colB,
+                        const int64_t *pbase, const int *tb_off, const zd *sums, zd *blocks,
+                        hipStream_t st);
+// scattered field of a solution at npts points: element-local coefficients
+// seh[2][nel][10], then E, H (3 complex per point)
+void launch_quad_local(const QuadTopo &T, const zd *x, zd *seh, hipStream_t st);
+void launch_quad_field(const QuadDev &d, const QuadTopo &T, int npts, const double *pts,
+                       const zd *seh, double *E, double *H, hipStream_t st);
 
 }  // namespace spk

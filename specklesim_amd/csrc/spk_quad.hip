@@ -637,7 +637,508 @@ __global__ void k_quad_scatter(int n, int nel, const int *r, const int *t, const
   sums[((size_t)r[i] * nel + t[i]) * 30 + k] = src[(size_t)i * 30 + k];
 }
 
+// ---------------------------------------------------------------------------
+// MLFMM over the quadrilateral edge functions.  The reference's quadrilateral
+// path is dense only (lib_sie_quad_solver_normal.f90:62-270); its MLFMM
+// data container declares a pattern slot for it (preCalculated_RT_quad,
+// src/lib_sie_data_container.f90:84) that nothing fills.  Here the edge
+// functions sit in the same octree as RWG edges (at the midpoint of their
+// node pair, find_midpoint_edge_quad, lib_sie_quad_calculation_mod.f90:210-255)
+// and the far part of the quadrilateral system is the plane-wave form of the
+// regular 4 x 4 element-pair rule the dense fill uses for distant pairs
+// (Integration_KABCD_Edge_new with ff_ABC_II_Edge):
+//   G(ra - rq) = sum_s TL_s exp(-ik khat.(ra - ca)) exp(+ik khat.(rq - cq))
+//   grad_a G   = sum_s TL_s (-ik khat) ...
+// so with the receiving pattern R (the test function on its own element),
+// the radiating patterns TJ (n x N over the function's one or two elements)
+// and TD (its surface divergence), per medium:
+//   KA = sum TL (-ik) R.(TJ x khat),  KB = sum TL (-ik)(khat.R) TD,
+//   KC = sum TL R.TJ
+// and the rows combine as the dense fill's (A, DE, DH) do.
+// ---------------------------------------------------------------------------
+
+// Quadrature points of the patterns, per position p (edge function perm[p]),
+// relative to the leaf box centre: 16 receiving points (d, w JJ Na) and 32
+// radiating points (d, w JJ (n x N), w JJ nrot); the second element's 16
+// are zero-weighted when the function has one element.
+constexpr int kQRecv = 16, kQRad = 32, kQPtStride = kQuadPtStride;
+static_assert(kQPtStride == kQRecv * 6 + kQRad * 7, "point layout");
+
+__global__ void k_qpoints(QuadDev D, QuadTopo T, int N, const int *perm, const int *leaf_box,
+                          const double *leaf_centre, double *pts) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  const int m = perm[p];
+  const double *c = leaf_centre + 3 * (size_t)leaf_box[m];
+  const int *e = T.edge_p + 5 * (size_t)m;
+  const int ii = e[0] - 1, pI = e[1], jj = e[2] - 1, pq = e[3];
+  const double f = (double)e[4];
+  double *o = pts + (size_t)p * kQPtStride;
+  const double *Ea = T.el + (size_t)ii * 24;
+  for (int j = 0; j < 4; ++j)
+    for (int i = 0; i < 4; ++i) {
+      Geo g;
+      geo(D.x4[i], D.x4[j], Ea, g);
+      double r[3], Na[3];
+      rloc(D.x4[i], D.x4[j], Ea, r);
+      obs_fn(g, pI, D.x4[i], D.x4[j], Na);
+      const double w = D.w4[i] * D.w4[j] * g.JJ;
+      double *q = o + (j * 4 + i) * 6;
+      for (int d = 0; d < 3; ++d) {
+        q[d] = r[d] - c[d];
+        q[3 + d] = w * Na[d];
+      }
+    }
+  double *o2 = o + kQRecv * 6;
+  for (int h = 0; h < 2; ++h) {
+    const bool on = h == 0 || (jj >= 0 && e[4] != 0);
+    const double *Eq = T.el + (size_t)(h == 0 ? ii : (on ? jj : ii)) * 24;
+    const int n = (h == 0 ? pI : pq) - 1;
+    const double sg = h == 0 ? 1.0 : f;
+    for (int j = 0; j < 4; ++j)
+      for (int i = 0; i < 4; ++i) {
+        double *q = o2 + (h * 16 + j * 4 + i) * 7;
+        if (!on) {
+          for (int d = 0; d < 7; ++d) q[d] = 0;
+          continue;
+        }
+        const double kq = D.x4[i], eq = D.x4[j];
+        Geo g;
+        geo(kq, eq, Eq, g);
+        double r[3];
+        rloc(kq, eq, Eq, r);
+        const double fv = fedge(n, kq, eq);
+        const double *v = n < 5 ? g.au : g.av;
+        const double Nq[3] = {fv * v[0], fv * v[1], fv * v[2]};
+        double aa[3];
+        cross3(g.nq, Nq, aa);
+        const double w = D.w4[i] * D.w4[j] * sg;
+        for (int d = 0; d < 3; ++d) {
+          q[d] = r[d] - c[d];
+          q[3 + d] = w * g.JJ * aa[d];
+        }
+        // nrot * JJ = the scalar part's derivative (fedge_rot)
+        q[6] = w * fedge_rot(n, kq, eq);
+      }
+  }
+}
+
+// exp(sg * i k x) for complex k
+__device__ inline zd expik(zd k, double x, double sg) {
+  const double mag = exp(-sg * k.y * x);
+  double sn, cs;
+  sincos(sg * k.x * x, &sn, &cs);
+  return zd{mag * cs, mag * sn};
+}
+
+// rt[p][q][s]: q = k*4 + {TJx, TJy, TJz, TD} (radiating, exp(+ik khat.d)),
+// 8 + k*3 + {Rx, Ry, Rz} (receiving, exp(-ik khat.d)); one thread per
+// (sample, position)
+__global__ void k_qpatterns(int N, int S, const double *khat, const double *pts, zd k1, zd k2,
+                            zd *rt) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p = blockIdx.y;
+  if (s >= S || p >= N) return;
+  const double kh[3] = {khat[3 * s], khat[3 * s + 1], khat[3 * s + 2]};
+  const double *o = pts + (size_t)p * kQPtStride;
+  zd *out = rt + (size_t)p * 14 * S + s;
+  for (int k = 0; k < 2; ++k) {
+    const zd kk = k ? k2 : k1;
+    zd R[3] = {{0, 0}, {0, 0}, {0, 0}}, TJ[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    for (int i = 0; i < kQRecv; ++i) {
+      const double *q = o + i * 6;
+      const zd ph = expik(kk, kh[0] * q[0] + kh[1] * q[1] + kh[2] * q[2], -1.0);
+      for (int d = 0; d < 3; ++d) R[d] = R[d] + ph * q[3 + d];
+    }
+    const double *o2 = o + kQRecv * 6;
+    for (int i = 0; i < kQRad; ++i) {
+      const double *q = o2 + i * 7;
+      const zd ph = expik(kk, kh[0] * q[0] + kh[1] * q[1] + kh[2] * q[2], 1.0);
+      for (int d = 0; d < 4; ++d) TJ[d] = TJ[d] + ph * q[3 + d];
+    }
+    for (int d = 0; d < 4; ++d) out[(size_t)(k * 4 + d) * S] = TJ[d];
+    for (int d = 0; d < 3; ++d) out[(size_t)(8 + k * 3 + d) * S] = R[d];
+  }
+}
+
+// Aggregation: B[half c/2][b][(c%2)*4 + cur*2 + k][s] = sum over the box's
+// positions of TJ/TD[k][c](s) * x_cur; grid (samples, leaf boxes, k*4 + c)
+template <class T>
+__global__ void k_qagg(int S, int N, const int *box_first, const int *box_count, const int *perm,
+                       const T *rt, const T *x, T *B, size_t half) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y, q = blockIdx.z;
+  if (s >= S) return;
+  const int k = q >> 2, c = q & 3;
+  T ae = {0, 0}, ah = {0, 0};
+  const int p0 = box_first[b], cnt = box_count[b];
+  for (int e0 = 0; e0 < cnt; e0 += kSumBlock<T>) {
+    T ge = {0, 0}, gh = {0, 0};
+    const int e1 = min(cnt, e0 + kSumBlock<T>);
+    for (int e = e0; e < e1; ++e) {
+      const int p = p0 + e, m = perm[p];
+      const T f = rt[((size_t)p * 14 + q) * S + s];
+      ge = ge + f * x[m];
+      gh = gh + f * x[m + N];
+    }
+    ae = ae + ge;
+    ah = ah + gh;
+  }
+  T *Bb = B + (size_t)(c >> 1) * half + (size_t)b * 8 * S;
+  Bb[(size_t)((c & 1) * 4 + 0 + k) * S + s] = ae;
+  Bb[(size_t)((c & 1) * 4 + 2 + k) * S + s] = ah;
+}
+
+template <class T> struct qreal;
+template <> struct qreal<zd> { using type = double; };
+template <> struct qreal<zf> { using type = float; };
+template <class T> using qreal_t = typename qreal<T>::type;
+
+template <class T>
+__device__ inline void cross_k(const T J[3], const double kh[3], T o[3]) {
+  using R = qreal_t<T>;
+  o[0] = J[1] * (R)kh[2] - J[2] * (R)kh[1];
+  o[1] = J[2] * (R)kh[0] - J[0] * (R)kh[2];
+  o[2] = J[0] * (R)kh[1] - J[1] * (R)kh[0];
+}
+
+// The receiving sums of a leaf box combined per medium into the two
+// vectors the rows read (W[b][k*6 + {E x,y,z, H x,y,z}][s]):
+//   WE = cAE (DJe x khat) + cJE DJh + cDE DDh khat
+//   WH = cJH DJe + cDH DDe khat + cAH (DJh x khat)
+template <class T>
+__global__ void k_qfold(int S, const T *D, size_t half, const double *khat, QuadFoldC C, T *W) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  if (s >= S) return;
+  const double kh[3] = {khat[3 * s], khat[3 * s + 1], khat[3 * s + 2]};
+  const T *D0 = D + (size_t)b * 8 * S, *D1 = D0 + half;
+  T *Wb = W + (size_t)b * 12 * S;
+  for (int k = 0; k < 2; ++k) {
+    const T Je[3] = {D0[(size_t)(0 + k) * S + s], D0[(size_t)(4 + k) * S + s],
+                     D1[(size_t)(0 + k) * S + s]};
+    const T Jh[3] = {D0[(size_t)(2 + k) * S + s], D0[(size_t)(6 + k) * S + s],
+                     D1[(size_t)(2 + k) * S + s]};
+    const T De = D1[(size_t)(4 + k) * S + s], Dh = D1[(size_t)(6 + k) * S + s];
+    T xe[3], xh[3];
+    cross_k(Je, kh, xe);
+    cross_k(Jh, kh, xh);
+    const T cAE = cvt<T>(C.cAE[k]), cJE = cvt<T>(C.cJE[k]), cDE = cvt<T>(C.cDE[k]);
+    const T cJH = cvt<T>(C.cJH[k]), cDH = cvt<T>(C.cDH[k]), cAH = cvt<T>(C.cAH[k]);
+    const T dE = cDE * Dh, dH = cDH * De;
+    for (int d = 0; d < 3; ++d) {
+      Wb[(size_t)(k * 6 + d) * S + s] = cAE * xe[d] + cJE * Jh[d] + dE * (qreal_t<T>)kh[d];
+      Wb[(size_t)(k * 6 + 3 + d) * S + s] = cJH * Je[d] + dH * (qreal_t<T>)kh[d] + cAH * xh[d];
+    }
+  }
+}
+
+__device__ inline zd qwave_sum(zd v) {
+  for (int off = 32; off > 0; off >>= 1) {
+    v.x += __shfl_down(v.x, off, 64);
+    v.y += __shfl_down(v.y, off, 64);
+  }
+  return v;
+}
+__device__ inline zf qwave_sum(zf v) {
+  for (int off = 32; off > 0; off >>= 1) {
+    v.x += __shfl_down(v.x, off, 64);
+    v.y += __shfl_down(v.y, off, 64);
+  }
+  return v;
+}
+
+// Disaggregation: one wavefront per task of up to kDisaggEdges consecutive
+// positions of one leaf box (task = p0 * 8 + count), lanes over samples:
+// y_e = sum_{k,s} R.WE, y_h = sum R.WH
+template <class T>
+__global__ void __launch_bounds__(256) k_qdisagg(int ntask, int N, int S, const int *task,
+                                                 const int *perm, const int *leaf_box_of_pos,
+                                                 const T *rt, const T *W, T *y) {
+  constexpr int E = kDisaggEdges;
+  const int lane = threadIdx.x & 63;
+  const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (t >= ntask) return;
+  const int code = task[t], p0 = code >> 3, cnt = code & 7;
+  const T *Wb = W + (size_t)leaf_box_of_pos[p0] * 12 * S;
+  T se[E], sh[E];
+  for (int e = 0; e < E; ++e) se[e] = sh[e] = T{0, 0};
+  for (int k = 0; k < 2; ++k) {
+    const T *f[E];
+    for (int e = 0; e < E; ++e) f[e] = rt + ((size_t)(p0 + min(e, cnt - 1)) * 14 + 8 + k * 3) * S;
+    const T *wk = Wb + (size_t)k * 6 * S;
+    for (int s = lane; s < S; s += 64) {
+      T w[6];
+      for (int d = 0; d < 6; ++d) w[d] = wk[(size_t)d * S + s];
+      for (int e = 0; e < E; ++e) {
+        if (e >= cnt) continue;
+        const T r0 = f[e][s], r1 = f[e][S + s], r2 = f[e][2 * S + s];
+        se[e] = se[e] + r0 * w[0] + r1 * w[1] + r2 * w[2];
+        sh[e] = sh[e] + r0 * w[3] + r1 * w[4] + r2 * w[5];
+      }
+    }
+  }
+  for (int e = 0; e < E; ++e) {
+    const T a = qwave_sum(se[e]), b = qwave_sum(sh[e]);
+    if (lane == 0 && e < cnt) {
+      const int m = perm[p0 + e];
+      y[m] = a;
+      y[m + N] = b;
+    }
+  }
+}
+
+// Near field, per chunk of leaf boxes [b0, b1): the (row edge function,
+// element) pairs of box b are rows x T(b), T(b) = the elements of its E2
+// columns; pair index q = pbase[b] + r * |T(b)| + li.  Regular and corner
+// pairs here (one thread each); self and edge-adjacent pairs come from the
+// list kernels and are scattered in.
+__device__ inline int upper_box(const int64_t *off, int b0, int b1, int64_t g) {
+  int lo = b0, hi = b1 - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= g)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ void __launch_bounds__(128) k_qn_regular(QuadDev D, QuadTopo T, int b0, int b1,
+                                                    int64_t nq, const int64_t *pbase,
+                                                    const int *tb_off, const int *tb_list,
+                                                    const int *box_first, const int *perm,
+                                                    zd *sums) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq) return;
+  const int64_t q = pbase[b0] + i;
+  const int b = upper_box(pbase, b0, b1, q);
+  const int nT = tb_off[b + 1] - tb_off[b];
+  const int64_t rel = q - pbase[b];
+  const int r = (int)(rel / nT), li = (int)(rel - (int64_t)r * nT);
+  const int s = perm[box_first[b] + r], t = tb_list[tb_off[b] + li];
+  int side[2];
+  const int cls = pair_class(T, s, t, side);
+  if (cls < 2) return;
+  const int ii = T.edge_p[5 * s] - 1, pI = T.edge_p[5 * s + 1];
+  zd out[3][10];
+  if (cls == 2)
+    regular_pair<6>(D, D.x6, D.w6, T.el + (size_t)ii * 24, T.el + (size_t)t * 24, pI, out);
+  else
+    regular_pair<4>(D, D.x4, D.w4, T.el + (size_t)ii * 24, T.el + (size_t)t * 24, pI, out);
+  store30(out, sums + (size_t)i * 30);
+}
+
+__global__ void k_qn_scatter(int n, const int64_t *dst, const zd *src, zd *sums) {
+  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= (int64_t)n * 30) return;
+  const int i = (int)(id / 30), k = (int)(id % 30);
+  sums[(size_t)dst[i] * 30 + k] = src[(size_t)i * 30 + k];
+}
+
+// blocks g of the chunk's boxes: the dense assembly's rule
+// (lib_sie_quad_solver_normal.f90:165-210) on the pair sums
+__global__ void k_qn_assemble(QuadTopo T, int form, zd eta_a, zd eta12, int b0, int b1,
+                              const int64_t *near_off, const int *e2_off, const int *e2_list,
+                              const int *colA, const int *colB, const int64_t *pbase,
+                              const int *tb_off, const zd *sums, zd *blocks) {
+  const int64_t g = near_off[b0] + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= near_off[b1]) return;
+  const int b = upper_box(near_off, b0, b1, g);
+  const int cols = e2_off[b + 1] - e2_off[b];
+  const int64_t rel = g - near_off[b];
+  const int r = (int)(rel / cols), c = (int)(rel - (int64_t)r * cols);
+  const int ci = e2_off[b] + c;
+  const int tc = e2_list[ci];
+  const int *e = T.edge_p + 5 * (size_t)tc;
+  const int pa = e[1] - 1, pq = e[3] - 1, f = e[4];
+  const int nT = tb_off[b + 1] - tb_off[b];
+  const int64_t base = pbase[b] - pbase[b0] + (int64_t)r * nT;
+  const zd *sa = sums + (size_t)(base + colA[ci]) * 30;
+  const zd *sb = f != 0 ? sums + (size_t)(base + colB[ci]) * 30 : nullptr;
+  zd m3[3];
+  for (int k = 0; k < 3; ++k) {
+    zd v = sa[k * 10 + pa];
+    if (f != 0) v = v + sb[k * 10 + pq] * (double)f;
+    m3[k] = v;
+  }
+  zd a = m3[0], bb = m3[1], cc = m3[2], d = m3[0];
+  if (form == SPK_ICTF) {
+    a = m3[0] / eta_a;
+    bb = m3[1] / eta_a;
+    cc = m3[2] * eta_a;
+    d = m3[0] * eta_a;
+  } else if (form == SPK_MCTF) {
+    cc = m3[2] * eta12;
+    d = m3[0] * eta12;
+  }
+  zd *o = blocks + 4 * (size_t)g;
+  o[0] = a;
+  o[1] = bb;
+  o[2] = cc;
+  o[3] = d;
+}
+
+// element-local coefficients of a solution (the S_EH array of
+// lib_sie_quad_solver_normal.f90:312-323): seh[cur][element][10]
+__global__ void k_quad_local(QuadTopo T, const zd *x, zd *seh) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= T.medge) return;
+  const int *e = T.edge_p + 5 * (size_t)m;
+  const int ii = e[0] - 1, pa = e[1] - 1, jj = e[2] - 1, pq = e[3] - 1;
+  const size_t nel10 = (size_t)T.nel * 10;
+  const zd xe = x[m], xh = x[T.medge + m];
+  seh[(size_t)ii * 10 + pa] = xe;
+  seh[nel10 + (size_t)ii * 10 + pa] = xh;
+  if (jj >= 0) {
+    seh[(size_t)jj * 10 + pq] = xe * (double)e[4];
+    seh[nel10 + (size_t)jj * 10 + pq] = xh * (double)e[4];
+  }
+}
+
+// Integration_scattered_field_quad (src/lib_sie_quad_calculation_mod.f90:1281-1323)
+// summed over all elements (lib_sie_quad_solver_normal.f90:388-396, ngp = 5):
+// one workgroup per observation point, threads over elements, the partial
+// fields added in a fixed order.  E, H: 3 complex per point.
+__global__ void __launch_bounds__(256) k_quad_field(QuadDev D, QuadTopo T, const double *pts,
+                                                    const zd *seh, double *Eo, double *Ho) {
+  const int pt = blockIdx.x;
+  const double Ra[3] = {pts[3 * pt], pts[3 * pt + 1], pts[3 * pt + 2]};
+  zd acc[6] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}};
+  const size_t nel10 = (size_t)T.nel * 10;
+  for (int el = threadIdx.x; el < T.nel; el += blockDim.x) {
+    const double *Eq = T.el + (size_t)el * 24;
+    const zd *SE = seh + (size_t)el * 10, *SH = seh + nel10 + (size_t)el * 10;
+    for (int i = 0; i < 5; ++i)
+      for (int j = 0; j < 5; ++j) {
+        const double kq = D.x5[i], eq = D.x5[j];
+        Geo g;
+        geo(kq, eq, Eq, g);
+        double Rq[3];
+        rloc(kq, eq, Eq, Rq);
+        const double R[3] = {Ra[0] - Rq[0], Ra[1] - Rq[1], Ra[2] - Rq[2]};
+        const double nR = sqrt(dot3(R, R));
+        zd G, DG;
+        qgreen(D.k1, nR, G, DG);
+        const zd gg[3] = {DG * (R[0] / nR), DG * (R[1] / nR), DG * (R[2] / nR)};
+        const double ff = g.JJ * D.w5[i] * D.w5[j];
+        for (int n = 0; n < 10; ++n) {
+          const double fv = fedge(n, kq, eq);
+          const double *v = n < 5 ? g.au : g.av;
+          const double Nq[3] = {fv * v[0], fv * v[1], fv * v[2]};
+          double aa[3];
+          cross3(g.nq, Nq, aa);
+          const double nrot = fedge_rot(n, kq, eq) / g.JJ;
+          // KA = (aa x gradG) ff, KB = nrot gradG ff, KC = aa G ff
+          const zd KA[3] = {(gg[2] * aa[1] - gg[1] * aa[2]) * ff,
+                            (gg[0] * aa[2] - gg[2] * aa[0]) * ff,
+                            (gg[1] * aa[0] - gg[0] * aa[1]) * ff};
+          const zd se = SE[n], sh = SH[n];
+          const zd cE = sh * (1.0 / D.iwe), cH = se * (1.0 / (-D.iwm));
+          const zd dE = sh * D.iwm, dH = se * D.iwe;
+          for (int d = 0; d < 3; ++d) {
+            const zd KB = gg[d] * (nrot * ff), KC = G * (aa[d] * ff);
+            acc[d] = acc[d] + KA[d] * se + KB * cE + KC * dE;
+            acc[3 + d] = acc[3 + d] + KA[d] * sh + KB * cH - KC * dH;
+          }
+        }
+      }
+  }
+  __shared__ zd red[4][6];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int q = 0; q < 6; ++q) {
+    const zd s = wsum(acc[q]);
+    if (lane == 0) red[wv][q] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    zd s = red[0][threadIdx.x];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) s = s + red[w][threadIdx.x];
+    double *o = threadIdx.x < 3 ? Eo : Ho;
+    const int d = threadIdx.x % 3;
+    o[(size_t)pt * 6 + 2 * d] = s.x;
+    o[(size_t)pt * 6 + 2 * d + 1] = s.y;
+  }
+}
+
 }  // namespace
+
+void launch_qpoints(const QuadDev &d, const QuadTopo &T, int N, const int *perm,
+                    const int *leaf_box, const double *leaf_centre, double *pts, hipStream_t st) {
+  if (N <= 0) return;
+  hipLaunchKernelGGL(k_qpoints, dim3(nb(N, 64)), dim3(64), 0, st, d, T, N, perm, leaf_box,
+                     leaf_centre, pts);
+}
+void launch_qpatterns(int N, int S, const double *khat, const double *pts, zd k1, zd k2, zd *rt,
+                      hipStream_t st) {
+  if (N <= 0) return;
+  hipLaunchKernelGGL(k_qpatterns, dim3(nb(S, 256), N), dim3(256), 0, st, N, S, khat, pts, k1, k2,
+                     rt);
+}
+template <class T>
+void launch_qagg(int nbl, int S, int N, const int *box_first, const int *box_count,
+                 const int *perm, const T *rt, const T *x, T *B, size_t half, hipStream_t st) {
+  if (nbl <= 0) return;
+  hipLaunchKernelGGL(k_qagg<T>, dim3(nb(S, 256), nbl, 8), dim3(256), 0, st, S, N, box_first,
+                     box_count, perm, rt, x, B, half);
+}
+template <class T>
+void launch_qfold(int nbl, int S, const T *D, size_t half, const double *khat,
+                  const QuadFoldC &C, T *W, hipStream_t st) {
+  if (nbl <= 0) return;
+  hipLaunchKernelGGL(k_qfold<T>, dim3(nb(S, 256), nbl), dim3(256), 0, st, S, D, half, khat, C, W);
+}
+template <class T>
+void launch_qdisagg(int ntask, int N, int S, const int *task, const int *perm,
+                    const int *leaf_box_of_pos, const T *rt, const T *W, T *y, hipStream_t st) {
+  if (ntask <= 0) return;
+  hipLaunchKernelGGL(k_qdisagg<T>, dim3(nb(ntask, 4)), dim3(256), 0, st, ntask, N, S, task, perm,
+                     leaf_box_of_pos, rt, W, y);
+}
+template void launch_qagg<zd>(int, int, int, const int *, const int *, const int *, const zd *,
+                              const zd *, zd *, size_t, hipStream_t);
+template void launch_qagg<zf>(int, int, int, const int *, const int *, const int *, const zf *,
+                              const zf *, zf *, size_t, hipStream_t);
+template void launch_qfold<zd>(int, int, const zd *, size_t, const double *, const QuadFoldC &,
+                               zd *, hipStream_t);
+template void launch_qfold<zf>(int, int, const zf *, size_t, const double *, const QuadFoldC &,
+                               zf *, hipStream_t);
+template void launch_qdisagg<zd>(int, int, int, const int *, const int *, const int *, const zd *,
+                                 const zd *, zd *, hipStream_t);
+template void launch_qdisagg<zf>(int, int, int, const int *, const int *, const int *, const zf *,
+                                 const zf *, zf *, hipStream_t);
+void launch_qn_regular(const QuadDev &d, const QuadTopo &T, int b0, int b1, int64_t nq,
+                       const int64_t *pbase, const int *tb_off, const int *tb_list,
+                       const int *box_first, const int *perm, zd *sums, hipStream_t st) {
+  if (nq <= 0) return;
+  hipLaunchKernelGGL(k_qn_regular, dim3(nb(nq, 128)), dim3(128), 0, st, d, T, b0, b1, nq, pbase,
+                     tb_off, tb_list, box_first, perm, sums);
+}
+void launch_qn_scatter(int n, const int64_t *dst, const zd *src, zd *sums, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_qn_scatter, dim3(nb((int64_t)n * 30, 256)), dim3(256), 0, st, n, dst, src,
+                     sums);
+}
+void launch_qn_assemble(const QuadTopo &T, int form, zd eta_a, zd eta12, int b0, int b1,
+                        int64_t nblocks, const int64_t *near_off, const int *e2_off,
+                        const int *e2_list, const int *colA, const int *colB,
+                        const int64_t *pbase, const int *tb_off, const zd *sums, zd *blocks,
+                        hipStream_t st) {
+  if (nblocks <= 0) return;
+  hipLaunchKernelGGL(k_qn_assemble, dim3(nb(nblocks, 256)), dim3(256), 0, st, T, form, eta_a,
+                     eta12, b0, b1, near_off, e2_off, e2_list, colA, colB, pbase, tb_off, sums,
+                     blocks);
+}
+void launch_quad_local(const QuadTopo &T, const zd *x, zd *seh, hipStream_t st) {
+  if (T.medge <= 0) return;
+  hipLaunchKernelGGL(k_quad_local, dim3(nb(T.medge, 256)), dim3(256), 0, st, T, x, seh);
+}
+void launch_quad_field(const QuadDev &d, const QuadTopo &T, int npts, const double *pts,
+                       const zd *seh, double *E, double *H, hipStream_t st) {
+  if (npts <= 0) return;
+  hipLaunchKernelGGL(k_quad_field, dim3(npts), dim3(256), 0, st, d, T, pts, seh, E, H);
+}
 
 void launch_quad_scatter(int n, int nel, const int *r, const int *t, const zd *src, zd *sums,
                          hipStream_t st) {
@@ -680,6 +1181,74 @@ void launch_quad_assemble(const QuadTopo &T, int formulation, zd eta_a, zd eta12
   if (n <= 0) return;
   hipLaunchKernelGGL(k_quad_assemble, dim3(nb(n, 256)), dim3(256), 0, st, T, formulation, eta_a,
                      eta12, s0, rows, sums, MM);
+}
+
+// set_local_material_parameters_quad (src/lib_sie_quad_data_container.f90:118-129)
+void quad_constants(const spk_params &p, QuadDev *D, zd *eta_a, zd *eta12) {
+  typedef std::complex<double> C;
+  const double k0 = 2 * kPi / p.lambda, omega = 2 * kPi * kC0 / p.lambda;
+  const C er1(p.eps_r1_re, p.eps_r1_im), er2(p.eps_r2_re, p.eps_r2_im);
+  const C er1v = (er1 == C(0.0)) ? C(1.0) : er1;
+  const C k1 = k0 * std::sqrt(er1v), k2 = k0 * std::sqrt(er2);
+  const C eta1 = std::sqrt(kMy0 / (kEps0 * er1v)), eta2 = std::sqrt(kMy0 / (kEps0 * er2));
+  auto Z = [](C c) { return zd{c.real(), c.imag()}; };
+  D->k1 = Z(k1);
+  D->k2 = Z(k2);
+  D->ie1 = Z(1.0 / er1v);
+  D->ie2 = Z(1.0 / er2);
+  D->e1 = Z(er1v);
+  D->e2 = Z(er2);
+  D->ceps = Z(1.0 + 1.0 / er2);
+  D->iwe = Z(C(0.0, 1.0) * omega * kEps0);
+  D->iwm = Z(C(0.0, 1.0) * omega * kMy0);
+  gauss_legendre(4, D->x4, D->w4);
+  gauss_legendre(5, D->x5, D->w5);
+  gauss_legendre(6, D->x6, D->w6);
+  gauss_legendre(15, D->x15, D->w15);
+  *eta_a = Z((eta1 + eta2) / 2.0);
+  *eta12 = Z(eta1 * eta2);
+}
+
+QuadRhsArgs quad_rhs_args(int formulation, const QuadDev &D, zd eta_a, zd eta12,
+                          const double k_in[3], const double e_in[3]) {
+  QuadRhsArgs a{};
+  for (int d = 0; d < 3; ++d) {
+    a.kin[d] = k_in[d];
+    a.ein[d] = e_in[d];
+  }
+  a.k1c = zd{D.k1.x, -D.k1.y};
+  a.c0my = (double)kMy0 * (double)kC0;
+  a.sE = zd{1.0, 0.0};
+  a.sH = zd{1.0, 0.0};
+  if (formulation == SPK_MCTF) {
+    a.sH = eta12;
+  } else if (formulation == SPK_ICTF) {
+    a.sE = zd{1.0, 0.0} / eta_a;
+    a.sH = eta_a;
+  }
+  return a;
+}
+
+bool quad_topology_arrays(const QuadMesh &m, std::vector<double> *el, std::vector<int> *eadj,
+                          std::vector<int> *eside, std::vector<int> *cadj) {
+  el->assign((size_t)m.nel * 24, 0.0);
+  for (int e = 0; e < m.nel; ++e)
+    for (int n = 0; n < 8; ++n)
+      for (int d = 0; d < 3; ++d)
+        (*el)[(size_t)e * 24 + n * 3 + d] = m.pp[3 * (size_t)(m.vert[8 * (size_t)e + n] - 1) + d];
+  eadj->assign((size_t)m.nel * kQuadMaxAdj, 0);
+  eside->assign((size_t)m.nel * kQuadMaxAdj * 2, 0);
+  cadj->assign((size_t)m.nel * kQuadMaxAdj, 0);
+  for (int e = 0; e < m.nel; ++e) {
+    if ((int)m.eadj[e].size() > kQuadMaxAdj || (int)m.cadj[e].size() > kQuadMaxAdj) return false;
+    for (size_t a = 0; a < m.eadj[e].size(); ++a) {
+      (*eadj)[(size_t)e * kQuadMaxAdj + a] = m.eadj[e][a].other;
+      (*eside)[((size_t)e * kQuadMaxAdj + a) * 2] = m.eadj[e][a].side[0];
+      (*eside)[((size_t)e * kQuadMaxAdj + a) * 2 + 1] = m.eadj[e][a].side[1];
+    }
+    for (size_t a = 0; a < m.cadj[e].size(); ++a) (*cadj)[(size_t)e * kQuadMaxAdj + a] = m.cadj[e][a];
+  }
+  return true;
 }
 
 // ---------------------------------------------------------------------------
@@ -755,30 +1324,7 @@ int spk_quad_create(const spk_params *params, spk_quad **out) {
     delete q;
     return set_error(SPK_ERR_HIP, "hipStreamCreate failed");
   }
-  // set_local_material_parameters_quad (src/lib_sie_quad_data_container.f90:118-129)
-  typedef std::complex<double> C;
-  const double k0 = 2 * kPi / params->lambda, omega = 2 * kPi * kC0 / params->lambda;
-  const C er1(params->eps_r1_re, params->eps_r1_im), er2(params->eps_r2_re, params->eps_r2_im);
-  const C er1v = (er1 == C(0.0)) ? C(1.0) : er1;
-  const C k1 = k0 * std::sqrt(er1v), k2 = k0 * std::sqrt(er2);
-  const C eta1 = std::sqrt(kMy0 / (kEps0 * er1v)), eta2 = std::sqrt(kMy0 / (kEps0 * er2));
-  auto Z = [](C c) { return zd{c.real(), c.imag()}; };
-  QuadDev &D = q->dev;
-  D.k1 = Z(k1);
-  D.k2 = Z(k2);
-  D.ie1 = Z(1.0 / er1v);
-  D.ie2 = Z(1.0 / er2);
-  D.e1 = Z(er1v);
-  D.e2 = Z(er2);
-  D.ceps = Z(1.0 + 1.0 / er2);
-  D.iwe = Z(C(0.0, 1.0) * omega * kEps0);
-  D.iwm = Z(C(0.0, 1.0) * omega * kMy0);
-  gauss_legendre(4, D.x4, D.w4);
-  gauss_legendre(5, D.x5, D.w5);
-  gauss_legendre(6, D.x6, D.w6);
-  gauss_legendre(15, D.x15, D.w15);
-  q->eta_a = Z((eta1 + eta2) / 2.0);
-  q->eta12 = Z(eta1 * eta2);
+  quad_constants(*params, &q->dev, &q->eta_a, &q->eta12);
   *out = q;
   return SPK_OK;
 }
@@ -798,23 +1344,10 @@ int spk_quad_setup(spk_quad *q, double Lx, double Ly, int nx, int ny, const doub
   m = QuadMesh();
   quad_mesh(Lx, Ly, nx, ny, z, &m);
   quad_topology(&m);
-  std::vector<double> el((size_t)m.nel * 24);
-  for (int e = 0; e < m.nel; ++e)
-    for (int n = 0; n < 8; ++n)
-      for (int d = 0; d < 3; ++d)
-        el[(size_t)e * 24 + n * 3 + d] = m.pp[3 * (size_t)(m.vert[8 * (size_t)e + n] - 1) + d];
-  std::vector<int> eadj((size_t)m.nel * kQuadMaxAdj, 0), eside((size_t)m.nel * kQuadMaxAdj * 2, 0),
-      cadj((size_t)m.nel * kQuadMaxAdj, 0);
-  for (int e = 0; e < m.nel; ++e) {
-    if ((int)m.eadj[e].size() > kQuadMaxAdj || (int)m.cadj[e].size() > kQuadMaxAdj)
-      return set_error(SPK_ERR_MESH, "element with more than 8 neighbours");
-    for (size_t a = 0; a < m.eadj[e].size(); ++a) {
-      eadj[(size_t)e * kQuadMaxAdj + a] = m.eadj[e][a].other;
-      eside[((size_t)e * kQuadMaxAdj + a) * 2] = m.eadj[e][a].side[0];
-      eside[((size_t)e * kQuadMaxAdj + a) * 2 + 1] = m.eadj[e][a].side[1];
-    }
-    for (size_t a = 0; a < m.cadj[e].size(); ++a) cadj[(size_t)e * kQuadMaxAdj + a] = m.cadj[e][a];
-  }
+  std::vector<double> el;
+  std::vector<int> eadj, eside, cadj;
+  if (!quad_topology_arrays(m, &el, &eadj, &eside, &cadj))
+    return set_error(SPK_ERR_MESH, "element with more than 8 neighbours");
   QCHK(q->d_el.upload(el));
   QCHK(q->d_edge_p.upload(m.edge_p));
   QCHK(q->d_eadj.upload(eadj));
@@ -985,21 +1518,7 @@ int spk_quad_rhs_plane_wave_device(spk_quad *q, const double k_in[3], const doub
   if (!q || !k_in || !e_in || !d_b) return set_error(SPK_ERR_ARG, "null argument");
   if (!q->ready) return set_error(SPK_ERR_STATE, "spk_quad_setup has not completed");
   QCHK(hipSetDevice(q->p.device));
-  QuadRhsArgs a{};
-  for (int d = 0; d < 3; ++d) {
-    a.kin[d] = k_in[d];
-    a.ein[d] = e_in[d];
-  }
-  a.k1c = zd{q->dev.k1.x, -q->dev.k1.y};
-  a.c0my = (double)kMy0 * (double)kC0;
-  a.sE = zd{1.0, 0.0};
-  a.sH = zd{1.0, 0.0};
-  if (q->p.formulation == SPK_MCTF) {
-    a.sH = q->eta12;
-  } else if (q->p.formulation == SPK_ICTF) {
-    a.sE = zd{1.0, 0.0} / q->eta_a;
-    a.sH = q->eta_a;
-  }
+  const QuadRhsArgs a = quad_rhs_args(q->p.formulation, q->dev, q->eta_a, q->eta12, k_in, e_in);
   launch_quad_rhs_plane(q->dev, q->topo(), a, (zd *)d_b, q->st);
   QCHK(hipGetLastError());
   QCHK(hipStreamSynchronize(q->st));

@@ -162,6 +162,15 @@ void quad_topology(QuadMesh *m) {
   m->medge = (int)(m->edge_p.size() / 5);
 }
 
+// the node pair of edge function s on its first element (1-based node
+// numbers), whose midpoint places the function (find_midpoint_edge_quad,
+// src/lib_sie_quad_calculation_mod.f90:210-255)
+void quad_edge_nodes(const QuadMesh &m, int s, int *n1, int *n2) {
+  const int ii = m.edge_p[5 * (size_t)s] - 1, pI = m.edge_p[5 * (size_t)s + 1];
+  *n1 = m.vert[8 * (size_t)ii + kSide[pI - 1][0] - 1];
+  *n2 = m.vert[8 * (size_t)ii + kSide[pI - 1][1] - 1];
+}
+
 void quad_side_index(const int ga[3], const int gq[3], int side[2]) {
   int A[6] = {ga[0], ga[1], ga[2], gq[0], gq[1], gq[2]};
   if (A[1] - A[0] != 1) {

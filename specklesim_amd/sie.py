@@ -105,6 +105,23 @@ class SieTriMlfmm:
         self.N = n.value
         return self
 
+    def precalculation_quad(self, Lx, Ly, nx, ny, z):
+        """The MLFMM operator on an 8-node quadrilateral height-field surface
+        (spk_setup_quad): Lx x Ly, nx x ny elements, z[(2nx+1)(2ny+1)] node
+        heights as the reference's height file orders them (x index fastest;
+        sorting_quad, src/lib_sie_quad_data_container.f90:321-403).  N is
+        then the number of edge functions M; the products, right-hand sides
+        (get_vector_b_quad), solves and scattered fields of this object work
+        on the quadrilateral system."""
+        z = np.ascontiguousarray(z, dtype=np.float64).reshape(-1)
+        if z.size != (2 * nx + 1) * (2 * ny + 1):
+            raise ValueError("z must hold (2nx+1)(2ny+1) heights")
+        nat.check(nat.lib().spk_setup_quad(self._h, float(Lx), float(Ly), int(nx), int(ny), _dp(z)))
+        n = C.c_int64()
+        nat.check(nat.lib().spk_num_edges(self._h, C.byref(n)))
+        self.N = n.value
+        return self
+
     # -- lib_sie_ml_fmm_calculate_vector_b_tri ------------------------------
     def calculate_vector_b_tri(self, vector_x):
         x = np.ascontiguousarray(vector_x, dtype=np.complex128)

@@ -1,0 +1,184 @@
+"""MLFMM on the 8-node quadrilateral edge functions (spk_setup_quad)
+against the oracle's dense quadrilateral system.  The reference solves
+quadrilateral meshes by dense LU only (lib_sie_quad_solver_normal.f90:62-270);
+its MLFMM pattern slot for them (preCalculated_RT_quad,
+src/lib_sie_data_container.f90:84) is never filled, so the dense system is
+the reference operator here:
+
+- every near block in the CSR equals the dense system's entries (1e-10);
+- the far part is the plane-wave form of the dense fill's regular 4 x 4
+  rule: the product equals the dense product to the MLFMM's truncation
+  error, which falls as the truncation number grows;
+- the right-hand side (get_vector_b_quad) and the scattered field
+  (Integration_scattered_field_quad) equal the oracle's;
+- the complex64 mirror equals the FP64 operator to single-precision
+  rounding; batched products equal single ones;
+- the refusals (transposed operator, adjoint, RWG probes).
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from specklesim_amd import DeviceBuffer, SieTriMlfmm, meshes, plane_wave
+from specklesim_amd._native import SpeckleError
+
+pytestmark = pytest.mark.gpu
+LAM = 633e-9
+FORMS = ["PMCHWT", "ICTF", "MCTF"]
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.fixture(scope="module")
+def patch():
+    # 20 x 20 elements of 0.1 lambda (M = 2480): leaf boxes of 0.5 lambda at
+    # l_max = 2 with tree_s_opt = 200, far interactions between boxes two apart
+    Lx, Ly, nx, ny, z = meshes.rough_height_field(20, 20, 2.0, 2.0, 0.05, 0.5, 3, LAM)
+    q = O.QuadOracle(LAM, 2.25, Lx, Ly, nx, ny, z)
+    dense = {}
+    return (Lx, Ly, nx, ny, z), q, dense
+
+
+def _dense(patch, form):
+    _, q, dense = patch
+    if form not in dense:
+        dense[form] = q.assemble(form, nthreads=16)
+    return dense[form]
+
+
+def _gpu(geom, form, precision="fp64", K=19, s_opt=200):
+    g = SieTriMlfmm(LAM, 2.25, formulation=form, normal_convention=2, tree_s_opt=s_opt,
+                    truncation_number=K, precision=precision)
+    return g.precalculation_quad(*geom)
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_near_blocks_are_the_dense_entries(patch, form):
+    geom, q, _ = patch
+    MM = _dense(patch, form)
+    g = _gpu(geom, form)
+    st = g.stats()
+    print(f"{form}: M {g.N}, levels {st['l_min']}-{st['l_max']}, near blocks {st['near_blocks']}")
+    assert g.N == q.M and st["l_max"] >= 2 and st["i4_pairs_per_level"][2] > 0
+    M = g.N
+    ia, ja, v = g.near_csr()
+    rows = np.repeat(np.arange(M), np.diff(ia))
+    ref = np.stack([MM[rows, ja], MM[rows, M + ja], MM[M + rows, ja], MM[M + rows, M + ja]], 1)
+    # per block position: ICTF scales the rows by 1/eta_a and eta_a
+    err = (np.abs(v - ref).max(axis=0) / np.abs(ref).max(axis=0)).max()
+    print(f"{form}: near blocks vs dense {err:.2e}")
+    assert err < 1e-10
+    # the far pairs are present: the near field covers a fraction of the system
+    assert ia[-1] < 0.6 * M * M
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_product_equals_the_dense_product(patch, form):
+    geom, q, _ = patch
+    MM = _dense(patch, form)
+    g = _gpu(geom, form)
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal(2 * g.N) + 1j * rng.standard_normal(2 * g.N)
+    yd = MM @ x
+    y = g.calculate_vector_b_tri(x)
+    yn, yf = g.matvec_parts(x)
+    e = rel(y, yd)
+    ef = rel(yf, yd - yn)
+    print(f"{form}: product vs dense {e:.2e}, far part vs dense far part {ef:.2e}")
+    assert np.allclose(yn + yf, y, rtol=0, atol=1e-12 * np.abs(y).max())
+    assert e < 2e-4 and ef < 2e-3
+
+
+def test_far_part_converges_with_the_truncation_number(patch):
+    geom, q, _ = patch
+    MM = _dense(patch, "PMCHWT")
+    rng = np.random.default_rng(6)
+    errs = []
+    for K in (11, 15, 19):
+        g = _gpu(geom, "PMCHWT", K=K)
+        x = rng.standard_normal(2 * g.N) + 1j * rng.standard_normal(2 * g.N)
+        yn, yf = g.matvec_parts(x)
+        errs.append(rel(yf, MM @ x - yn))
+    print("far part error for K = 11, 15, 19:", ", ".join(f"{e:.2e}" for e in errs))
+    assert errs[0] > errs[1] > errs[2]
+
+
+def test_complex64_operator(patch):
+    """The complex64 mirror against the FP64 operator of the same truncation
+    number.  K = 12 suits these 0.5 lambda boxes (kd = 4.7 in the interior
+    medium); K = 19 there makes |TL| of the nearest far boxes so large that
+    single-precision rounding grows to ~3e-2 of the product."""
+    geom, q, _ = patch
+    g64 = _gpu(geom, "ICTF", K=12)
+    g32 = _gpu(geom, "ICTF", precision="mixed", K=12)
+    rng = np.random.default_rng(8)
+    x = rng.standard_normal(2 * g64.N) + 1j * rng.standard_normal(2 * g64.N)
+    y64 = g64.calculate_vector_b_tri(x)
+    y32 = g32.calculate_vector_b_tri_c64(x.astype(np.complex64)).astype(np.complex128)
+    e = rel(y32, y64)
+    e64 = rel(g32.calculate_vector_b_tri(x), y64)
+    print(f"complex64 product vs FP64 product (K = 12) {e:.2e}; the mixed handle's FP64 product {e64:.1e}")
+    assert e64 < 1e-14
+    assert e < 1e-3
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_rhs_and_scattered_field(patch, form):
+    geom, q, _ = patch
+    g = _gpu(geom, form)
+    k, e = plane_wave(np.deg2rad(20.0), 0.0, 2, "TE")
+    b = g.get_vector_b_tri(k, e)
+    bo = q.rhs_plane(k, e, form)
+    print(f"{form}: rhs vs oracle {rel(b, bo):.2e}")
+    assert rel(b, bo) < 1e-12
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal(2 * g.N) + 1j * rng.standard_normal(2 * g.N)
+    pts = np.array([[0.3, -0.2, 1.0], [1.5, 0.4, 2.0], [-2.0, 1.0, 0.5]]) * LAM * 3
+    E, H = g.scattered_field(pts, x)
+    Eo, Ho = q.scattered_field(pts, x)
+    print(f"{form}: field vs oracle E {rel(E, Eo):.2e} H {rel(H, Ho):.2e}")
+    assert rel(E, Eo) < 1e-11 and rel(H, Ho) < 1e-11
+
+
+def test_batched_products_and_a_preconditioned_solve_step(patch):
+    """The batched product entry point takes one product per vector on a
+    quadrilateral handle; the near-field preconditioned solver runs on it.
+    The reference's quadrilateral system is solved by dense LU
+    (lib_sie_quad_solver_normal.f90): its condition number is ~1e10 at
+    0.1 lambda elements and GMRES stalls on it even densely (tests below
+    record the stall, not a convergence claim)."""
+    geom, q, _ = patch
+    g = _gpu(geom, "ICTF")
+    rng = np.random.default_rng(9)
+    X = rng.standard_normal((3, 2 * g.N)) + 1j * rng.standard_normal((3, 2 * g.N))
+    Y = g.calculate_vector_b_tri_batch(X)
+    for r in range(3):
+        assert np.array_equal(Y[r], g.calculate_vector_b_tri(X[r]))
+    k, e = plane_wave(np.deg2rad(10.0), 0.0, 2, "TM")
+    nb = 2 * g.N * 16
+    db, dx = DeviceBuffer(nb), DeviceBuffer(nb)
+    g.get_vector_b_tri_device(k, e, db.ptr)
+    info = g.gmres_run_device(db.ptr, dx.ptr, tolerance=1e-8, max_iterations=20,
+                              orthogonalization="ICGS", use_initial_guess=False, near_field=True)
+    x, b = dx.download(), db.download()
+    res = rel(g.calculate_vector_b_tri(x), b)
+    print(f"20 preconditioned steps: info {info['info']}, backward error {info['be']:.2e}, "
+          f"true residual {res:.2e}")
+    assert np.isfinite(res) and res < 1.0
+
+
+def test_refusals(patch):
+    geom, _, _ = patch
+    g = _gpu(geom, "PMCHWT")
+    with pytest.raises(SpeckleError):
+        g.set_transposed(True)
+    with pytest.raises(SpeckleError):
+        g.rwg()
+    with pytest.raises(SpeckleError):
+        SieTriMlfmm(LAM, 2.25, formulation="JMCFIE", normal_convention=2).precalculation_quad(*geom)
+    # leaf boxes of 0.125 lambda against elements of 0.1 lambda: supports in
+    # non-adjacent boxes could touch, so the setup refuses
+    with pytest.raises(SpeckleError, match="element diameter"):
+        _gpu(geom, "PMCHWT", s_opt=8)
