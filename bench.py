@@ -405,6 +405,67 @@ def cpu_baseline(args, lam, pp, tt, iters):
             "matvec_ms": t_mv * 1e3, "solve_s_est": t_solve}
 
 
+def timed_quad(args, lam):
+    """C3: the MLFMM operator on a 20 x 20 lambda rough patch of 200 x 200
+    8-node quadrilaterals (10 edge functions per element, M = 240,800,
+    481,600 unknowns; RMS 0.1 lambda, correlation 0.5 lambda, seed 1) on a
+    mixed-precision handle, ICTF, K = 19, leaf boxes of 0.625 lambda
+    (tree_s_opt 400; the default 250 gives 0.3125 lambda leaves).  Setup
+    (near fill from the quadrilateral pair integrals, patterns, tables),
+    FP64 and complex64 products timed with HIP events, and 40 near-field
+    preconditioned outer steps.  The reference solves this system by dense
+    LU only; its condition number (~1e10 at 0.1 lambda elements, measured on
+    small grids) stalls GMRES, preconditioned or not."""
+    import gc
+
+    from specklesim_amd import DeviceBuffer, SieTriMlfmm, meshes, plane_wave
+
+    n = args.quad_cells
+    L = n / 10.0
+    Lx, Ly, nx, ny, z = meshes.rough_height_field(n, n, L, L, 0.1, 0.5, 1, lam)
+    gc.collect()
+    _sync()
+    t0 = time.perf_counter()
+    g = SieTriMlfmm(lam, args.eps_r2, formulation=args.formulation, normal_convention=2,
+                    tree_s_opt=400, precision="mixed").precalculation_quad(Lx, Ly, nx, ny, z)
+    _sync()
+    setup_s = time.perf_counter() - t0
+    st = g.stats()
+    reps = max(3, args.steps)
+    p64 = g.profile_matvec(reps=reps)
+    p32 = g.profile_matvec(reps=reps, c64=True)
+    m64, m32 = p64["matvec"][0], p32["matvec"][0]
+    # algorithmic bytes of one product: near blocks, both pattern sets, x / y
+    nbytes = st["bytes_near"] + st["bytes_patterns"] + 3 * 2 * g.N * 16.0
+    k, e = plane_wave(np.deg2rad(10.0), 0.0, 2, "TM")
+    nb = 2 * g.N * 16
+    db, dx = DeviceBuffer(nb), DeviceBuffer(nb)
+    g.get_vector_b_tri_device(k, e, db.ptr)
+    _sync()
+    t0 = time.perf_counter()
+    info = g.gmres_run_device(db.ptr, dx.ptr, tolerance=args.tol, max_iterations=40,
+                              orthogonalization="ICGS", use_initial_guess=False, near_field=True)
+    _sync()
+    solve_s = time.perf_counter() - t0
+    rec = {
+        "workload": f"C3: {n} x {n} 8-node quadrilaterals on a {L:g} x {L:g} lambda rough patch "
+                    f"(RMS 0.1, correlation 0.5 lambda), {args.formulation}, K = 19, leaf boxes "
+                    f"of 0.625 lambda, mixed-precision handle",
+        "edge_functions": g.N, "unknowns": 2 * g.N, "near_blocks": st["near_blocks"],
+        "levels": [st["l_min"], st["l_max"]],
+        "setup_s": setup_s, "near_fill_ms": st["near_fill_ms"], "patterns_ms": st["patterns_ms"],
+        "matvec_ms_fp64": m64, "matvec_ms_c64": m32,
+        "matvec_stage_ms_fp64": {kk: round(v[0], 4) for kk, v in p64.items() if kk != "matvec"},
+        "fp64_product_bytes": nbytes, "fp64_product_tb_per_s": nbytes / (m64 * 1e-3) / 1e12,
+        "operator_bytes_fp64_and_c64": 1.5 * (st["bytes_near"] + st["bytes_patterns"]),
+        "preconditioned_outer_steps": {"steps": info["iters"], "backward_error": info["be"],
+                                       "converged": info["info"] == 0, "seconds": solve_s},
+    }
+    del g, db, dx
+    gc.collect()
+    return rec
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -428,6 +489,8 @@ def main():
                     help="realisations of the speckle-ensemble object (0: skip it)")
     ap.add_argument("--adjoint-nodes", type=int, default=32,
                     help="nodes of the adjoint object's finite-difference check (0: skip it)")
+    ap.add_argument("--quad-cells", type=int, default=200,
+                    help="quadrilaterals per side of the C3 object (0: skip it)")
     ap.add_argument("--workers", type=int, default=3,
                     help="worker threads (handles) of the ensemble")
     args = ap.parse_args()
@@ -468,6 +531,7 @@ def main():
     ens = timed_ensemble(args, lam) if args.ensemble and args.precision == "both" else None
     adj = (timed_adjoint(args, lam, pp, tt) if args.adjoint_nodes and args.precision == "both"
            else None)
+    quad = timed_quad(args, lam) if args.quad_cells and args.precision == "both" else None
     ff_err = None
     if "fp64" in paths and "mixed" in paths:
         b64, b32 = brdf_curve(paths["fp64"], lam), brdf_curve(paths["mixed"], lam)
@@ -614,6 +678,8 @@ def main():
         rec["ensemble"] = ens
     if adj is not None:
         rec["adjoint"] = adj
+    if quad is not None:
+        rec["quad_mlfmm"] = quad
     if not args.no_cpu_baseline:
         it64 = (paths["fp64"] if "fp64" in paths else H)["runs"][-1]["iters"]
         cb = cpu_baseline(args, lam, pp, tt, it64)
