@@ -168,10 +168,20 @@ module lib_sie_tri_mlfmm_gpu
    public :: lib_sie_ml_fmm_get_init_vector_x_tri
    public :: lib_sie_ml_fmm_gmres_run
    public :: lib_sie_ml_fmm_gpu_finalize
+   public :: lib_sie_ml_fmm_set_mesh_files
+   public :: lib_sie_ml_fmm_set_mesh
+   public :: lib_sie_ml_fmm_set_illumination
    public :: m_pairs
 
    type(c_ptr), save :: handle = c_null_ptr
    integer, save :: m_pairs = 0
+   ! the state the reference's routines read instead of taking arguments:
+   ! the mesh of the data container (data_import_tri reads it from the
+   ! pp_/tt_ files) and the plane wave of illumination_p
+   real(8), allocatable, save :: mesh_pp(:, :)
+   integer, allocatable, save :: mesh_tt(:, :)
+   real(8), save :: illum_k(3) = 0d0, illum_e(3) = 0d0
+   logical, save :: illum_set = .false.
 
 contains
 
@@ -195,18 +205,87 @@ contains
       call spk_check(spk_create(p, handle), 'spk_create')
    end subroutine
 
-   ! lib_sie_ml_fmm_precalculation_tri (src/lib_sie_tri_mlfmm_interface.f90:59-92);
-   ! pp(3, Np) node coordinates, tt(3, Nt) 1-based node indices as read by
-   ! read_pp_tt_file_tri.
-   subroutine lib_sie_ml_fmm_precalculation_tri(pp, tt)
+   ! data_import_tri / read_pp_tt_file_tri
+   ! (src/lib_sie_tri_data_container.f90:226-261): the mesh files, one node
+   ! (x y z) or one triangle (three 1-based node numbers) per line
+   subroutine lib_sie_ml_fmm_set_mesh_files(pp_file, tt_file)
+      character(len=*), intent(in) :: pp_file, tt_file
+      integer :: np, nt, i
+      real(8) :: buf(3)
+      integer :: ibuf(3)
+      np = count_lines(pp_file)
+      nt = count_lines(tt_file)
+      if (allocated(mesh_pp)) deallocate(mesh_pp)
+      if (allocated(mesh_tt)) deallocate(mesh_tt)
+      allocate(mesh_pp(3, np), mesh_tt(3, nt))
+      open(unit=71, file=pp_file, status='old', action='read')
+      do i = 1, np
+         read(71, *) buf
+         mesh_pp(:, i) = buf
+      end do
+      close(71)
+      open(unit=72, file=tt_file, status='old', action='read')
+      do i = 1, nt
+         read(72, *) ibuf
+         mesh_tt(:, i) = ibuf
+      end do
+      close(72)
+   end subroutine
+
+   ! the mesh given as arrays instead of files
+   subroutine lib_sie_ml_fmm_set_mesh(pp, tt)
       real(8), intent(in) :: pp(:, :)
       integer, intent(in) :: tt(:, :)
+      if (allocated(mesh_pp)) deallocate(mesh_pp)
+      if (allocated(mesh_tt)) deallocate(mesh_tt)
+      allocate(mesh_pp, source=pp)
+      allocate(mesh_tt, source=tt)
+   end subroutine
+
+   ! illumination_p%k_in / E_in of set_illumination_parameter (main.f90:245-297)
+   subroutine lib_sie_ml_fmm_set_illumination(k_in, e_in)
+      real(8), intent(in) :: k_in(3), e_in(3)
+      illum_k = k_in
+      illum_e = e_in
+      illum_set = .true.
+   end subroutine
+
+   ! lib_sie_ml_fmm_precalculation_tri (src/lib_sie_tri_mlfmm_interface.f90:59-92).
+   ! Called without arguments as the reference's is, it takes the mesh of
+   ! lib_sie_ml_fmm_set_mesh_files / _set_mesh; pp(3, Np) node coordinates
+   ! and tt(3, Nt) 1-based node indices may also be passed directly.
+   subroutine lib_sie_ml_fmm_precalculation_tri(pp, tt)
+      real(8), intent(in), optional :: pp(:, :)
+      integer, intent(in), optional :: tt(:, :)
       integer(c_int64_t) :: n
-      call spk_check(spk_setup_tri(handle, int(size(pp, 2), c_int), pp, &
-                                   int(size(tt, 2), c_int), tt), 'spk_setup_tri')
+      if (present(pp) .and. present(tt)) then
+         call spk_check(spk_setup_tri(handle, int(size(pp, 2), c_int), pp, &
+                                      int(size(tt, 2), c_int), tt), 'spk_setup_tri')
+      else
+         if (.not. (allocated(mesh_pp) .and. allocated(mesh_tt))) then
+            write(*, *) 'lib_sie_ml_fmm_precalculation_tri: no mesh (lib_sie_ml_fmm_set_mesh_files)'
+            stop 1
+         end if
+         call spk_check(spk_setup_tri(handle, int(size(mesh_pp, 2), c_int), mesh_pp, &
+                                      int(size(mesh_tt, 2), c_int), mesh_tt), 'spk_setup_tri')
+      end if
       call spk_check(spk_num_edges(handle, n), 'spk_num_edges')
       m_pairs = int(n)
    end subroutine
+
+   integer function count_lines(fname)
+      character(len=*), intent(in) :: fname
+      integer :: st
+      character(len=1024) :: line
+      count_lines = 0
+      open(unit=70, file=fname, status='old', action='read')
+      do
+         read(70, '(a)', iostat=st) line
+         if (st /= 0) exit
+         if (len_trim(line) > 0) count_lines = count_lines + 1
+      end do
+      close(70)
+   end function
 
    ! lib_sie_ml_fmm_calculate_vector_b_tri (:285-319): vector_b allocated by
    ! the caller (lib_math_solver_GMRES.f90:274-299), written in place.
@@ -217,13 +296,27 @@ contains
       call spk_check(spk_matvec(handle, vector_x, vector_b), 'spk_matvec')
    end subroutine
 
-   ! lib_sie_ml_fmm_get_vector_b_tri (:263-276) for a plane wave
+   ! lib_sie_ml_fmm_get_vector_b_tri (:263-276) for a plane wave: called with
+   ! vector_b only, as the reference's is, it takes the wave of
+   ! lib_sie_ml_fmm_set_illumination; k_in, e_in may also be passed
    subroutine lib_sie_ml_fmm_get_vector_b_tri(vector_b, k_in, e_in)
       double complex, dimension(:), allocatable, intent(inout) :: vector_b
-      real(8), intent(in) :: k_in(3), e_in(3)
+      real(8), intent(in), optional :: k_in(3), e_in(3)
+      real(8) :: k(3), e(3)
+      if (present(k_in) .and. present(e_in)) then
+         k = k_in
+         e = e_in
+      else
+         if (.not. illum_set) then
+            write(*, *) 'lib_sie_ml_fmm_get_vector_b_tri: no illumination (lib_sie_ml_fmm_set_illumination)'
+            stop 1
+         end if
+         k = illum_k
+         e = illum_e
+      end if
       if (allocated(vector_b)) deallocate(vector_b)
       allocate(vector_b(2*m_pairs))
-      call spk_check(spk_rhs_plane_wave(handle, k_in, e_in, vector_b), 'spk_rhs_plane_wave')
+      call spk_check(spk_rhs_plane_wave(handle, k, e, vector_b), 'spk_rhs_plane_wave')
    end subroutine
 
    ! lib_sie_ml_fmm_get_init_vector_x_tri (:190-239), x_initial constant
