@@ -89,12 +89,20 @@ typedef struct spk_gmres_params {
                                     an SPK_MIXED handle, where the outer products also switch
                                     to the complex64 operator once the residual estimate is
                                     at or below 200 x tolerance; the true residual that ends
-                                    the solve is FP64 in every case) */
+                                    the solve is FP64 in every case).  SPK_PRECOND_BLOCK_LU (no
+                                    counterpart in the reference): the same flexible GMRES,
+                                    right-preconditioned by the inverse of each group's dense
+                                    near-field matrix (groups = the leaf boxes under one box a
+                                    level up; rocSOLVER getrf/getri once per operator, one
+                                    batched GEMV per step); for systems the inner near-field
+                                    GMRES does not precondition enough (the quadrilateral
+                                    system); set restart (default 100) to the steps it needs */
   int inner_iterations;          /* SPK_PRECOND_NEAR_FIELD: inner steps per outer step (0: 30) */
 } spk_gmres_params;
 #define SPK_PRECOND_NONE 0
 #define SPK_PRECOND_LEFT_JACOBI 1
 #define SPK_PRECOND_NEAR_FIELD 2
+#define SPK_PRECOND_BLOCK_LU 3
 
 /* Per-level truncation numbers for the next spk_setup_tri: K[l] for levels
  * l < n (0: spk_params.truncation_number), non-increasing from the

@@ -56,7 +56,8 @@ def build(verbose: bool = False) -> str:
                 print(o)
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp,
-          *[j[0] for j in jobs], "-lpthread"])
+          *[j[0] for j in jobs], "-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-lpthread",
+          "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, LIB)
     build_fortran()
     return LIB

@@ -21,6 +21,9 @@
 #include "spk_device.h"
 #include "spk_dbuf.h"
 #include "spk_quad.h"
+
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
 #include "spk_tree.h"
 
 using namespace spk;
@@ -152,6 +155,20 @@ struct spk_handle {
   DBuf<double> q_el, q_pts;
   DBuf<int> q_edge_p, q_eadj, q_eside, q_cadj;
   DBuf<zd> q_seh;
+  // block-Jacobi preconditioner (SPK_PRECOND_BLOCK_LU): per group of leaf
+  // boxes (the children of one box a level up) the inverse of its dense
+  // near-field matrix, padded to nmax, applied as one batched GEMV; built on
+  // first use for the current operator
+  struct BlockLU {
+    bool ok = false;
+    int ngroups = 0, nmax = 0;
+    DBuf<zd> A, X, Y;
+    DBuf<int> p0, cnt, ipiv, info;
+    rocblas_handle rb = nullptr;
+    ~BlockLU() {
+      if (rb) (void)rocblas_destroy_handle(rb);
+    }
+  } blu;
   QuadTopo qtopo() const {
     return QuadTopo{q_el.p, q_edge_p.p, q_eadj.p, q_eside.p, q_cadj.p, qmesh.nel, qmesh.medge};
   }
@@ -1099,6 +1116,7 @@ static int build_mx_groups(spk_handle *h, int tpw) {
 // whose midpoint places it in the octree): the octree and its lists, the
 // truncation numbers, launch lists, the near field and the MLFMM tables.
 static int setup_operator(spk_handle *h, double t0) {
+  h->blu.ok = false;
   // the octree and its lists: on the device (the host builder with
   // SPK_HOST_TREE=1; both give the same lists)
   const char *ht = std::getenv("SPK_HOST_TREE");
@@ -2287,6 +2305,88 @@ static DBuf<zd> &fgWi(spk_handle::FgWork &F, zd *) { return F.Wid; }
 //    reorthogonalisation test, Givens rotations on the host; the Arnoldi
 //    backward error is the stopping test, confirmed by the true residual
 //    b - Z x (a new cycle starts from it otherwise, as after m steps).
+// The block-Jacobi preconditioner of the current operator: groups are the
+// occupied boxes one level above the leaves (leaf boxes are in Morton
+// order, so a group's positions are contiguous), each group's dense
+// near-field matrix is factorised and inverted (rocSOLVER getrf / getri,
+// batched over the groups, padded to the largest group).
+static int ensure_block_lu(spk_handle *h) {
+  if (h->blu.ok) return SPK_OK;
+  auto &B = h->blu;
+  const Tree &t = h->tree;
+  const Level &LL = t.lev[t.l_max];
+  const int nbl = (int)LL.n.size();
+  std::vector<int> p0, cnt;
+  for (int b = 0; b < nbl;) {
+    int e = b + 1;
+    while (e < nbl && (LL.n[e] >> 3) == (LL.n[b] >> 3)) ++e;
+    const int first = t.box_first[b], last = t.box_first[e - 1] + t.box_count[e - 1];
+    p0.push_back(first);
+    cnt.push_back(last - first);
+    b = e;
+  }
+  int cmax = 0;
+  for (int c : cnt) cmax = std::max(cmax, c);
+  B.ngroups = (int)p0.size();
+  B.nmax = 2 * cmax;
+  const size_t nn = (size_t)B.nmax * B.nmax;
+  const double gb = (double)B.ngroups * nn * sizeof(zd) / 1e9;
+  if (gb > 64.0)
+    return fail(SPK_ERR_ARG, "block-Jacobi groups too large (" + std::to_string(gb) +
+                                 " GB of dense blocks): use smaller leaf boxes");
+  HIPCHK(B.p0.upload(p0));
+  HIPCHK(B.cnt.upload(cnt));
+  HIPCHK(B.A.alloc((size_t)B.ngroups * nn));
+  HIPCHK(B.X.alloc((size_t)B.ngroups * B.nmax));
+  HIPCHK(B.Y.alloc((size_t)B.ngroups * B.nmax));
+  HIPCHK(B.ipiv.alloc((size_t)B.ngroups * B.nmax));
+  HIPCHK(B.info.alloc(B.ngroups));
+  HIPCHK(hipMemsetAsync(B.A.p, 0, B.A.n * sizeof(zd), h->st));
+  launch_blu_extract(B.ngroups, B.nmax, B.p0.p, B.cnt.p, h->d_perm.p, h->d_pos.p,
+                     h->d_leafbox_of_pos.p, h->d_box_first.p, h->d_e2_off.p, h->d_e2_list.p,
+                     h->d_near_off.p, h->d_blocks.p, B.A.p, h->st);
+  HIPCHK(hipGetLastError());
+  if (!B.rb && rocblas_create_handle(&B.rb) != rocblas_status_success)
+    return fail(SPK_ERR_HIP, "rocblas_create_handle failed");
+  if (rocblas_set_stream(B.rb, h->st) != rocblas_status_success)
+    return fail(SPK_ERR_HIP, "rocblas_set_stream failed");
+  auto *Ar = (rocblas_double_complex *)B.A.p;
+  if (rocsolver_zgetrf_strided_batched(B.rb, B.nmax, B.nmax, Ar, B.nmax, (rocblas_stride)nn,
+                                       B.ipiv.p, B.nmax, B.info.p, B.ngroups) !=
+      rocblas_status_success)
+    return fail(SPK_ERR_HIP, "rocsolver_zgetrf_strided_batched failed");
+  std::vector<int> inf(B.ngroups);
+  HIPCHK(hipMemcpyAsync(inf.data(), B.info.p, B.ngroups * sizeof(int), hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  for (int g = 0; g < B.ngroups; ++g)
+    if (inf[g] != 0)
+      return fail(SPK_ERR_STATE, "block-Jacobi: group " + std::to_string(g) +
+                                       " has a singular near-field matrix");
+  if (rocsolver_zgetri_strided_batched(B.rb, B.nmax, Ar, B.nmax, (rocblas_stride)nn, B.ipiv.p,
+                                       B.nmax, B.info.p, B.ngroups) != rocblas_status_success)
+    return fail(SPK_ERR_HIP, "rocsolver_zgetri_strided_batched failed");
+  HIPCHK(hipStreamSynchronize(h->st));
+  B.ok = true;
+  return SPK_OK;
+}
+
+// z = M^-1 v for the block-Jacobi preconditioner (stream-ordered)
+static int apply_block_lu(spk_handle *h, const zd *v, zd *z) {
+  auto &B = h->blu;
+  launch_blu_gather(B.ngroups, B.nmax, h->N, B.p0.p, B.cnt.p, h->d_perm.p, v, B.X.p, h->st);
+  const rocblas_double_complex one = {1.0, 0.0}, zero = {0.0, 0.0};
+  const rocblas_stride nn = (rocblas_stride)B.nmax * B.nmax;
+  if (rocblas_zgemv_strided_batched(B.rb, rocblas_operation_none, B.nmax, B.nmax, &one,
+                                    (const rocblas_double_complex *)B.A.p, B.nmax, nn,
+                                    (const rocblas_double_complex *)B.X.p, 1, B.nmax, &zero,
+                                    (rocblas_double_complex *)B.Y.p, 1, B.nmax, B.ngroups) !=
+      rocblas_status_success)
+    return fail(SPK_ERR_HIP, "rocblas_zgemv_strided_batched failed");
+  launch_blu_scatter(B.ngroups, B.nmax, h->N, B.p0.p, B.cnt.p, h->d_perm.p, B.Y.p, z, h->st);
+  HIPCHK(hipGetLastError());
+  return SPK_OK;
+}
+
 template <class T>
 static int fgmres_near(spk_handle *h, int R, const zd *const *b, zd *const *x,
                        const spk_gmres_params *gp, spk_gmres_info *info) {
@@ -2299,7 +2399,11 @@ static int fgmres_near(spk_handle *h, int R, const zd *const *b, zd *const *x,
   // sides, the matrix-core product from 9, complex64 blocks between), the
   // bf16 copy with SPK_NEAR_F8=0.
   const uint32_t *bf = nullptr, *f8 = nullptr;
-  if constexpr (std::is_same<T, zf>::value) {
+  // SPK_PRECOND_BLOCK_LU: the inner solve is the block-Jacobi inverse
+  const bool blu = gp->preconditioning == SPK_PRECOND_BLOCK_LU;
+  if (blu) {
+    if (int e = ensure_block_lu(h)) return e;
+  } else if constexpr (std::is_same<T, zf>::value) {
     const char *ev = getenv("SPK_NEAR_F8");
     if (!(ev && ev[0] == '0')) {
       const int e8 = ensure_near_f8(h);
@@ -2321,7 +2425,9 @@ static int fgmres_near(spk_handle *h, int R, const zd *const *b, zd *const *x,
   const double tol = gp->tolerance;
   // relaxed outer products on a mixed handle (SPK_RELAX=0 keeps every one in FP64)
   const char *erx = getenv("SPK_RELAX");
-  const bool relax_on = !(erx && erx[0] == '0');
+  // (not with the block-Jacobi preconditioner: the systems it is for are
+  // ill-conditioned enough that the complex64 operator's error stalls them)
+  const bool relax_on = !(erx && erx[0] == '0') && !blu;
   // kRelaxDelta above the complex64 operator's error leaves room for the
   // residual gap (the relaxed products' errors add up to about kRelaxDelta'
   // times the sum of the later estimates): 1e-3 forced a second cycle on a
@@ -2512,126 +2618,136 @@ static int fgmres_near(spk_handle *h, int R, const zd *const *b, zd *const *x,
     if (act.empty()) break;
     const int P = (int)act.size();
     // ---- inner near-field solves: slot s works on right-hand side act[s]
-    for (int s = 0; s < P; ++s) {
-      const zd *v = Vr(act[s]) + (size_t)S[act[s]].j * n;
-      if constexpr (std::is_same<T, zf>::value)
-        launch_scale_down(n, 1.0, v, (zf *)(U.p + s * ldu), h->st);
-      else
-        HIPCHK(hipMemcpyAsync(U.p + s * ldu, v, n * sizeof(zd), hipMemcpyDeviceToDevice, h->st));
-    }
-    // the inner steps' launches (near-field products, Arnoldi updates)
-    auto inner_steps = [&]() -> int {
-    int rc = SPK_OK;
-    for (int j = 0; j < kin; ++j) {
-      if (P < 4) {
-        for (int s = 0; s < P && !rc; ++s) {
-          T *ui = U.p + s * ldu + (size_t)j * n, *wo = Wi.p + (size_t)s * n;
-          if constexpr (std::is_same<T, zf>::value) {
-            if (f8) {
-              launch_near_spmv_f8(h->spmv_wgs, h->N, h->d_row_box.p, h->d_row_r0.p, h->d_box_first.p,
-                                  h->d_box_count.p, h->d_e2_off.p, h->d_e2_list.p, h->d_near_off.p,
-                                  h->d_perm.p, f8, h->d_f8_scale.p, ui, wo, h->st);
-              continue;
-            }
-            if (bf) {
-              launch_near_spmv_bf(h->spmv_wgs, h->N, h->d_row_box.p, h->d_row_r0.p, h->d_box_first.p,
-                                  h->d_box_count.p, h->d_e2_off.p, h->d_e2_list.p, h->d_near_off.p,
-                                  h->d_perm.p, bf, ui, wo, h->st);
-              continue;
-            }
-          }
-          rc = matvec_t<T>(h, ui, wo, true, false);
-        }
-      } else {
-        rc = matvec_batch_t<T>(h, P, (const T *const *)(tin.p + (size_t)(1 + j) * B), tW, false,
-                               f8 ? f8 : bf, f8 ? h->d_f8_scale.p : nullptr);
-      }
-      if (rc) return rc;
-      launch_arnoldi_cgs<T>(n, P, j, kin, U.p, ldu, Wi.p, h1.p, (int)ldhi, Hi.p, apart.p, h->st);
-    }
-    return rc;
-    };
-    if (use_graph && (f8 || bf)) {
-      // batch buffers sized before the capture (no allocation inside it)
-      if (P >= 4 && (rc = batch_ensure<T>(h, P))) return rc;
-      const std::vector<uint64_t> key = {
-          (uint64_t)P, (uint64_t)kin, (uint64_t)n, (uint64_t)(uintptr_t)f8,
-          (uint64_t)(uintptr_t)bf, (uint64_t)(uintptr_t)U.p, (uint64_t)(uintptr_t)Wi.p,
-          (uint64_t)(uintptr_t)h1.p, (uint64_t)(uintptr_t)Hi.p, (uint64_t)(uintptr_t)apart.p,
-          (uint64_t)(uintptr_t)tin.p, (uint64_t)(uintptr_t)pBB(h, (T *)nullptr).XI.p,
-          (uint64_t)(uintptr_t)pBB(h, (T *)nullptr).XB.p, (uint64_t)h->setup_serial};
-      if (h->ig.key != key || !h->ig.exec) {
-        h->ig.reset();
-        hipGraph_t graph = nullptr;
-        HIPCHK(hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal));
-        const int rc_cap = inner_steps();
-        const hipError_t e_end = hipStreamEndCapture(h->st, &graph);
-        if (rc_cap) {
-          if (graph) (void)hipGraphDestroy(graph);
-          return rc_cap;
-        }
-        HIPCHK(e_end);
-        const hipError_t e_inst = hipGraphInstantiate(&h->ig.exec, graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        HIPCHK(e_inst);
-        h->ig.key = key;
-      }
-      HIPCHK(hipGraphLaunch(h->ig.exec, h->st));
-    } else if ((rc = inner_steps())) {
-      return rc;
-    }
-    HIPCHK(hipMemcpyAsync(h->pin.p, Hi.p, (size_t)P * hik * sizeof(zd), hipMemcpyDeviceToHost, h->st));
-    HIPCHK(hipStreamSynchronize(h->st));
-    // least squares min |e_0 - H y| per slot (Givens), up to a breakdown
-    std::vector<zd> yall((size_t)R * kin, zd{0, 0});
-    std::vector<int> klen(P);
-    for (int s = 0; s < P; ++s) {
-      const zd *Hs = h->pin.p + (size_t)s * hik;
-      std::vector<zc> Hc((size_t)(kin + 1) * kin), gg(kin + 1, 0.0), yy(kin, 0.0);
-      for (size_t i = 0; i < Hc.size(); ++i) Hc[i] = zc(Hs[i].x, Hs[i].y);
-      auto HC = [&](int i, int c) -> zc & { return Hc[(size_t)c * (kin + 1) + i]; };
-      int kk = kin;
-      for (int c = 0; c < kin; ++c)
-        if (std::abs(HC(c + 1, c)) == 0.0) {
-          kk = c + 1;
-          break;
-        }
-      gg[0] = 1.0;
-      std::vector<double> cs(kk);
-      std::vector<zc> sn(kk);
-      for (int c = 0; c < kk; ++c) {
-        for (int i = 0; i < c; ++i) zrot1(&HC(i, c), &HC(i + 1, c), cs[i], sn[i]);
-        zc a = HC(c, c);
-        zrotg_(&a, HC(c + 1, c), &cs[c], &sn[c]);
-        zrot1(&gg[c], &gg[c + 1], cs[c], sn[c]);
-        zrot1(&HC(c, c), &HC(c + 1, c), cs[c], sn[c]);
-        HC(c + 1, c) = 0.0;
-      }
-      while (kk > 0 && std::abs(HC(kk - 1, kk - 1)) == 0.0) --kk;
-      for (int c = kk - 1; c >= 0; --c) {
-        zc sum = gg[c];
-        for (int i = c + 1; i < kk; ++i) sum -= HC(c, i) * yy[i];
-        yy[c] = sum / HC(c, c);
-      }
-      for (int c = 0; c < kk; ++c) yall[(size_t)s * kin + c] = zd{yy[c].real(), yy[c].imag()};
-      klen[s] = kk;
-    }
-    // from pinned memory, which is next written by a later copy on this stream
-    std::copy(yall.begin(), yall.begin() + (size_t)P * kin, h->pin.p);
-    HIPCHK(hipMemcpyAsync(Yi.p, h->pin.p, (size_t)P * kin * sizeof(zd), hipMemcpyHostToDevice, h->st));
     std::vector<const zd *> zin;
     std::vector<zd *> wout;
-    for (int s = 0; s < P; ++s) {
-      const int r = act[s];
-      zd *z = Zr(r) + (size_t)S[r].j * n;
-      if (klen[s] > 0)
-        launch_gemv_n(n, klen[s], (const T *)(U.p + s * ldu), n, Yi.p + (size_t)s * kin, z, h->g_pn.p,
-                      h->st);
-      else  // no usable inner step: the identity preconditioner
-        HIPCHK(hipMemcpyAsync(z, Vr(r) + (size_t)S[r].j * n, n * sizeof(zd), hipMemcpyDeviceToDevice,
-                              h->st));
-      zin.push_back(z);
-      wout.push_back(Wr(r));
+    if (blu) {
+      for (int s = 0; s < P; ++s) {
+        const int r = act[s];
+        zd *z = Zr(r) + (size_t)S[r].j * n;
+        if (int e = apply_block_lu(h, Vr(r) + (size_t)S[r].j * n, z)) return e;
+        zin.push_back(z);
+        wout.push_back(Wr(r));
+      }
+    } else {
+      for (int s = 0; s < P; ++s) {
+        const zd *v = Vr(act[s]) + (size_t)S[act[s]].j * n;
+        if constexpr (std::is_same<T, zf>::value)
+          launch_scale_down(n, 1.0, v, (zf *)(U.p + s * ldu), h->st);
+        else
+          HIPCHK(hipMemcpyAsync(U.p + s * ldu, v, n * sizeof(zd), hipMemcpyDeviceToDevice, h->st));
+      }
+      // the inner steps' launches (near-field products, Arnoldi updates)
+      auto inner_steps = [&]() -> int {
+      int rc = SPK_OK;
+      for (int j = 0; j < kin; ++j) {
+        if (P < 4) {
+          for (int s = 0; s < P && !rc; ++s) {
+            T *ui = U.p + s * ldu + (size_t)j * n, *wo = Wi.p + (size_t)s * n;
+            if constexpr (std::is_same<T, zf>::value) {
+              if (f8) {
+                launch_near_spmv_f8(h->spmv_wgs, h->N, h->d_row_box.p, h->d_row_r0.p, h->d_box_first.p,
+                                    h->d_box_count.p, h->d_e2_off.p, h->d_e2_list.p, h->d_near_off.p,
+                                    h->d_perm.p, f8, h->d_f8_scale.p, ui, wo, h->st);
+                continue;
+              }
+              if (bf) {
+                launch_near_spmv_bf(h->spmv_wgs, h->N, h->d_row_box.p, h->d_row_r0.p, h->d_box_first.p,
+                                    h->d_box_count.p, h->d_e2_off.p, h->d_e2_list.p, h->d_near_off.p,
+                                    h->d_perm.p, bf, ui, wo, h->st);
+                continue;
+              }
+            }
+            rc = matvec_t<T>(h, ui, wo, true, false);
+          }
+        } else {
+          rc = matvec_batch_t<T>(h, P, (const T *const *)(tin.p + (size_t)(1 + j) * B), tW, false,
+                                 f8 ? f8 : bf, f8 ? h->d_f8_scale.p : nullptr);
+        }
+        if (rc) return rc;
+        launch_arnoldi_cgs<T>(n, P, j, kin, U.p, ldu, Wi.p, h1.p, (int)ldhi, Hi.p, apart.p, h->st);
+      }
+      return rc;
+      };
+      if (use_graph && (f8 || bf)) {
+        // batch buffers sized before the capture (no allocation inside it)
+        if (P >= 4 && (rc = batch_ensure<T>(h, P))) return rc;
+        const std::vector<uint64_t> key = {
+            (uint64_t)P, (uint64_t)kin, (uint64_t)n, (uint64_t)(uintptr_t)f8,
+            (uint64_t)(uintptr_t)bf, (uint64_t)(uintptr_t)U.p, (uint64_t)(uintptr_t)Wi.p,
+            (uint64_t)(uintptr_t)h1.p, (uint64_t)(uintptr_t)Hi.p, (uint64_t)(uintptr_t)apart.p,
+            (uint64_t)(uintptr_t)tin.p, (uint64_t)(uintptr_t)pBB(h, (T *)nullptr).XI.p,
+            (uint64_t)(uintptr_t)pBB(h, (T *)nullptr).XB.p, (uint64_t)h->setup_serial};
+        if (h->ig.key != key || !h->ig.exec) {
+          h->ig.reset();
+          hipGraph_t graph = nullptr;
+          HIPCHK(hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal));
+          const int rc_cap = inner_steps();
+          const hipError_t e_end = hipStreamEndCapture(h->st, &graph);
+          if (rc_cap) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc_cap;
+          }
+          HIPCHK(e_end);
+          const hipError_t e_inst = hipGraphInstantiate(&h->ig.exec, graph, nullptr, nullptr, 0);
+          (void)hipGraphDestroy(graph);
+          HIPCHK(e_inst);
+          h->ig.key = key;
+        }
+        HIPCHK(hipGraphLaunch(h->ig.exec, h->st));
+      } else if ((rc = inner_steps())) {
+        return rc;
+      }
+      HIPCHK(hipMemcpyAsync(h->pin.p, Hi.p, (size_t)P * hik * sizeof(zd), hipMemcpyDeviceToHost, h->st));
+      HIPCHK(hipStreamSynchronize(h->st));
+      // least squares min |e_0 - H y| per slot (Givens), up to a breakdown
+      std::vector<zd> yall((size_t)R * kin, zd{0, 0});
+      std::vector<int> klen(P);
+      for (int s = 0; s < P; ++s) {
+        const zd *Hs = h->pin.p + (size_t)s * hik;
+        std::vector<zc> Hc((size_t)(kin + 1) * kin), gg(kin + 1, 0.0), yy(kin, 0.0);
+        for (size_t i = 0; i < Hc.size(); ++i) Hc[i] = zc(Hs[i].x, Hs[i].y);
+        auto HC = [&](int i, int c) -> zc & { return Hc[(size_t)c * (kin + 1) + i]; };
+        int kk = kin;
+        for (int c = 0; c < kin; ++c)
+          if (std::abs(HC(c + 1, c)) == 0.0) {
+            kk = c + 1;
+            break;
+          }
+        gg[0] = 1.0;
+        std::vector<double> cs(kk);
+        std::vector<zc> sn(kk);
+        for (int c = 0; c < kk; ++c) {
+          for (int i = 0; i < c; ++i) zrot1(&HC(i, c), &HC(i + 1, c), cs[i], sn[i]);
+          zc a = HC(c, c);
+          zrotg_(&a, HC(c + 1, c), &cs[c], &sn[c]);
+          zrot1(&gg[c], &gg[c + 1], cs[c], sn[c]);
+          zrot1(&HC(c, c), &HC(c + 1, c), cs[c], sn[c]);
+          HC(c + 1, c) = 0.0;
+        }
+        while (kk > 0 && std::abs(HC(kk - 1, kk - 1)) == 0.0) --kk;
+        for (int c = kk - 1; c >= 0; --c) {
+          zc sum = gg[c];
+          for (int i = c + 1; i < kk; ++i) sum -= HC(c, i) * yy[i];
+          yy[c] = sum / HC(c, c);
+        }
+        for (int c = 0; c < kk; ++c) yall[(size_t)s * kin + c] = zd{yy[c].real(), yy[c].imag()};
+        klen[s] = kk;
+      }
+      // from pinned memory, which is next written by a later copy on this stream
+      std::copy(yall.begin(), yall.begin() + (size_t)P * kin, h->pin.p);
+      HIPCHK(hipMemcpyAsync(Yi.p, h->pin.p, (size_t)P * kin * sizeof(zd), hipMemcpyHostToDevice, h->st));
+      for (int s = 0; s < P; ++s) {
+        const int r = act[s];
+        zd *z = Zr(r) + (size_t)S[r].j * n;
+        if (klen[s] > 0)
+          launch_gemv_n(n, klen[s], (const T *)(U.p + s * ldu), n, Yi.p + (size_t)s * kin, z, h->g_pn.p,
+                        h->st);
+        else  // no usable inner step: the identity preconditioner
+          HIPCHK(hipMemcpyAsync(z, Vr(r) + (size_t)S[r].j * n, n * sizeof(zd), hipMemcpyDeviceToDevice,
+                                h->st));
+        zin.push_back(z);
+        wout.push_back(Wr(r));
+      }
     }
     // ---- outer step: w_r = Z z_r, ICGS against V_r.  Relaxed products:
     // once every active estimate is at or below tol / kRelaxDelta, the
@@ -2798,7 +2914,8 @@ int spk_gmres_batch_device(spk_handle *h, int R, const void *const *d_b, void *c
   if (R < 1 || R > kBatchMax) return fail(SPK_ERR_ARG, "batch size must be 1..16");
   for (int r = 0; r < R; ++r)
     if (!d_b[r] || !d_x[r]) return fail(SPK_ERR_ARG, "null vector in the batch");
-  if (gp->preconditioning == SPK_PRECOND_NEAR_FIELD)
+  if (gp->preconditioning == SPK_PRECOND_NEAR_FIELD ||
+      gp->preconditioning == SPK_PRECOND_BLOCK_LU)
     return h->p.precision == SPK_MIXED
                ? fgmres_near<zf>(h, R, (const zd *const *)d_b, (zd *const *)d_x, gp, info)
                : fgmres_near<zd>(h, R, (const zd *const *)d_b, (zd *const *)d_x, gp, info);
@@ -2955,7 +3072,8 @@ int spk_gmres_device(spk_handle *h, const void *d_b, void *d_x, const spk_gmres_
                      spk_gmres_info *info) {
   if (!h || !d_b || !d_x || !gp) return fail(SPK_ERR_ARG, "null argument");
   if (!h->ready) return fail(SPK_ERR_STATE, "spk_setup_tri has not completed");
-  if (gp->preconditioning == SPK_PRECOND_NEAR_FIELD) {
+  if (gp->preconditioning == SPK_PRECOND_NEAR_FIELD ||
+      gp->preconditioning == SPK_PRECOND_BLOCK_LU) {
     const zd *bb[1] = {(const zd *)d_b};
     zd *xx[1] = {(zd *)d_x};
     spk_gmres_info tmp;

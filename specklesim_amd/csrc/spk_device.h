@@ -364,4 +364,19 @@ void launch_stream_read(const void *a, size_t bytes, double *out, hipStream_t st
 void launch_icwy_row(int k1, zd *Tm, int ldt, const zd *tg, hipStream_t st);
 void launch_zadd(int64_t n, const zd *a, zd *y, hipStream_t st);  // y += a
 
+// Block-Jacobi preconditioner on groups of leaf boxes (the children of one
+// box a level up): group g holds positions [p0[g], p0[g] + cnt[g]); its
+// dense matrix A_g (column-major, leading dimension nmax, unknowns ordered
+// [x_e of the group; x_h of the group], identity beyond 2 cnt[g]) comes
+// from the near blocks.  A must be zeroed before the extraction.
+void launch_blu_extract(int ngroups, int nmax, const int *p0, const int *cnt, const int *perm,
+                        const int *pos, const int *leafbox_of_pos, const int *box_first,
+                        const int *e2_off, const int *e2_list, const int64_t *near_off,
+                        const zd *blocks, zd *A, hipStream_t st);
+// x_g = v gathered into the group layout (zero padded), z = y_g scattered back
+void launch_blu_gather(int ngroups, int nmax, int N, const int *p0, const int *cnt,
+                       const int *perm, const zd *v, zd *X, hipStream_t st);
+void launch_blu_scatter(int ngroups, int nmax, int N, const int *p0, const int *cnt,
+                        const int *perm, const zd *Y, zd *z, hipStream_t st);
+
 }  // namespace spk

@@ -755,6 +755,93 @@ hipError_t exclusive_scan_int(const int *in, int *out, int64_t n, void **tmp, si
 }
 
 // ---------------------------------------------------------------------------
+// Block-Jacobi preconditioner: dense group matrices from the near blocks
+// ---------------------------------------------------------------------------
+// One workgroup per (group, row): the row's near columns that fall inside
+// the group are written into A_g; the padding diagonal is set by the row
+// workgroups beyond 2 cnt.
+__global__ void __launch_bounds__(256) k_blu_extract(int nmax, const int *p0, const int *cnt,
+                                                     const int *perm, const int *pos,
+                                                     const int *leafbox_of_pos,
+                                                     const int *box_first, const int *e2_off,
+                                                     const int *e2_list, const int64_t *near_off,
+                                                     const zd *blocks, zd *A) {
+  const int g = blockIdx.y, r = blockIdx.x;
+  const int c0 = cnt[g], q0 = p0[g];
+  zd *Ag = A + (size_t)g * nmax * nmax;
+  if (r >= c0) {
+    // padding rows/columns 2 c0 .. nmax-1: identity (two per workgroup)
+    const int i0 = 2 * c0 + 2 * (r - c0);
+    if (threadIdx.x < 2 && i0 + (int)threadIdx.x < nmax) {
+      const int i = i0 + threadIdx.x;
+      Ag[(size_t)i * nmax + i] = zd{1.0, 0.0};
+    }
+    return;
+  }
+  const int p = q0 + r;
+  const int b = leafbox_of_pos[p];
+  const int cols = e2_off[b + 1] - e2_off[b];
+  const int rr = p - box_first[b];
+  const zd *row = blocks + 4 * (size_t)(near_off[b] + (int64_t)rr * cols);
+  for (int c = threadIdx.x; c < cols; c += blockDim.x) {
+    const int q = pos[e2_list[e2_off[b] + c]] - q0;
+    if (q < 0 || q >= c0) continue;
+    const zd *bl = row + 4 * (size_t)c;
+    Ag[(size_t)q * nmax + r] = bl[0];
+    Ag[(size_t)(c0 + q) * nmax + r] = bl[1];
+    Ag[(size_t)q * nmax + c0 + r] = bl[2];
+    Ag[(size_t)(c0 + q) * nmax + c0 + r] = bl[3];
+  }
+}
+
+__global__ void k_blu_gather(int nmax, int N, const int *p0, const int *cnt, const int *perm,
+                             const zd *v, zd *X) {
+  const int g = blockIdx.y, i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nmax) return;
+  const int c0 = cnt[g];
+  zd val = {0, 0};
+  if (i < c0)
+    val = v[perm[p0[g] + i]];
+  else if (i < 2 * c0)
+    val = v[N + perm[p0[g] + i - c0]];
+  X[(size_t)g * nmax + i] = val;
+}
+
+__global__ void k_blu_scatter(int nmax, int N, const int *p0, const int *cnt, const int *perm,
+                              const zd *Y, zd *z) {
+  const int g = blockIdx.y, i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c0 = cnt[g];
+  if (i >= 2 * c0) return;
+  const zd val = Y[(size_t)g * nmax + i];
+  if (i < c0)
+    z[perm[p0[g] + i]] = val;
+  else
+    z[N + perm[p0[g] + i - c0]] = val;
+}
+
+void launch_blu_extract(int ngroups, int nmax, const int *p0, const int *cnt, const int *perm,
+                        const int *pos, const int *leafbox_of_pos, const int *box_first,
+                        const int *e2_off, const int *e2_list, const int64_t *near_off,
+                        const zd *blocks, zd *A, hipStream_t st) {
+  if (ngroups <= 0) return;
+  // rows 0 .. nmax/2: real rows below cnt, padding pairs above
+  hipLaunchKernelGGL(k_blu_extract, dim3((nmax + 1) / 2, ngroups), dim3(256), 0, st, nmax, p0, cnt,
+                     perm, pos, leafbox_of_pos, box_first, e2_off, e2_list, near_off, blocks, A);
+}
+void launch_blu_gather(int ngroups, int nmax, int N, const int *p0, const int *cnt,
+                       const int *perm, const zd *v, zd *X, hipStream_t st) {
+  if (ngroups <= 0) return;
+  hipLaunchKernelGGL(k_blu_gather, dim3(nblk(nmax, 256), ngroups), dim3(256), 0, st, nmax, N, p0,
+                     cnt, perm, v, X);
+}
+void launch_blu_scatter(int ngroups, int nmax, int N, const int *p0, const int *cnt,
+                        const int *perm, const zd *Y, zd *z, hipStream_t st) {
+  if (ngroups <= 0) return;
+  hipLaunchKernelGGL(k_blu_scatter, dim3(nblk(nmax, 256), ngroups), dim3(256), 0, st, nmax, N, p0,
+                     cnt, perm, Y, z);
+}
+
+// ---------------------------------------------------------------------------
 // MLFMM kernels
 // ---------------------------------------------------------------------------
 // PreCalculate_RT / RT_function_SCSArr_New (src/lib_sie_tri_mlfmm.f90:503-583):

@@ -36,6 +36,7 @@ def _dp(a):
 
 
 PRECOND_NEAR_FIELD = 2
+PRECOND_BLOCK_LU = 3
 
 
 def _gmres_params(max_iterations, restart, tolerance, orthogonalization, use_initial_guess,
@@ -43,7 +44,14 @@ def _gmres_params(max_iterations, restart, tolerance, orthogonalization, use_ini
                   inner_iterations):
     if precondition_left and near_field:
         raise ValueError("precondition_left and near_field exclude each other")
-    prec = PRECOND_NEAR_FIELD if near_field else int(bool(precondition_left))
+    # near_field: True for the inner near-field GMRES, "block_lu" for the
+    # block-Jacobi inverse of the leaf groups' near-field matrices
+    if near_field == "block_lu":
+        prec = PRECOND_BLOCK_LU
+    elif near_field:
+        prec = PRECOND_NEAR_FIELD
+    else:
+        prec = int(bool(precondition_left))
     return nat.GmresParams(max_iterations, restart, tolerance, ORTHOGONALIZATION[orthogonalization],
                            int(use_initial_guess), refinement_steps, refinement_tolerance, prec,
                            inner_iterations)
