@@ -225,13 +225,15 @@ def timed_ensemble(args, lam):
     _, info_t = ensemble_intensity(lam, args.eps_r2, ms, angles, pts, **kw_t)
     _sync()
     el_t = time.perf_counter() - t0
-    # plain FP64 GMRES, one realisation at a time, normal incidence
+    # plain FP64 GMRES, one realisation at a time, normal incidence, on the
+    # first (at most) 16 realisations: a per-solve rate
     s64 = SieTriMlfmm(lam, args.eps_r2, formulation=args.formulation, normal_convention=2,
                       precision="fp64")
     k, e = plane_wave(0.0, 0.0, 2, "TM")
+    ms64 = ms[:16]
     t0 = time.perf_counter()
     it64 = []
-    for pp, tt in ms:
+    for pp, tt in ms64:
         s64.precalculation_tri(pp, tt)
         _, i64 = s64.gmres_run(s64.get_vector_b_tri(k, e), tolerance=args.tol,
                                max_iterations=args.maxit, orthogonalization=args.orth,
@@ -254,8 +256,9 @@ def timed_ensemble(args, lam):
         "tuned": {"solves_per_s": len(ms) * 16 / el_t, "ms_per_ensemble": el_t * 1e3,
                   "includes": "spk_autotune on realisation 0 (3 tile maps, 6 batched solves)",
                   "all_converged": all(i["info"] == 0 for row in info_t for i in row)},
-        "fp64_plain_solves_per_s": len(ms) / el64, "fp64_plain_iterations": [min(it64), max(it64)],
-        "vs_fp64_plain": (len(ms) * 16 / el) / (len(ms) / el64),
+        "fp64_plain_solves_per_s": len(ms64) / el64, "fp64_plain_realisations": len(ms64),
+        "fp64_plain_iterations": [min(it64), max(it64)],
+        "vs_fp64_plain": (len(ms) * 16 / el) / (len(ms64) / el64),
     }
 
 
@@ -485,7 +488,7 @@ def main():
                     help="incidence angles of the batched path (0: skip it)")
     ap.add_argument("--inner", type=int, default=30,
                     help="inner near-field GMRES steps per flexible GMRES step")
-    ap.add_argument("--ensemble", type=int, default=16,
+    ap.add_argument("--ensemble", type=int, default=64,
                     help="realisations of the speckle-ensemble object (0: skip it)")
     ap.add_argument("--adjoint-nodes", type=int, default=32,
                     help="nodes of the adjoint object's finite-difference check (0: skip it)")
