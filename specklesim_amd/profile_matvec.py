@@ -15,6 +15,8 @@ around it.
 (torch's copy kernel), the calibration for FETCH_SIZE / WRITE_SIZE counters.
 ``--near-bf`` instead repeats the near-field preconditioner's bf16 SpMV,
 ``--near-f8`` its fp8 (E4M3FN) SpMV.
+``--quad N`` instead sets up the MLFMM on an N x N quadrilateral patch and
+times its products (``--precision mixed`` for the complex64 mirror).
 ``--solve near|mixed|fp64`` instead runs one solve of the normal-incidence
 TM plane wave to 1e-6 (near: the near-field preconditioned flexible GMRES;
 mixed: the refinement solve; fp64: the reference's GMRES with IMGS).
@@ -77,6 +79,9 @@ def main(argv=None):
     ap.add_argument("--adjoint", action="store_true",
                     help="one adjoint gradient over every node height (detector window of "
                          "+-5 deg around specular, solves to 1e-8)")
+    ap.add_argument("--quad", type=int, default=0,
+                    help="the MLFMM products on an N x N quadrilateral rough patch of N/10 "
+                         "lambda (RMS 0.1, correlation 0.5 lambda, tree_s_opt 400) instead")
     args = ap.parse_args(argv)
     if args.copy_mib:
         print(json.dumps(copy_calibration(args.copy_mib)), flush=True)
@@ -85,6 +90,22 @@ def main(argv=None):
     from . import SieTriMlfmm, meshes
 
     lam = 633e-9
+    if args.quad:
+        n = args.quad
+        Lx, Ly, nx, ny, z = meshes.rough_height_field(n, n, n / 10.0, n / 10.0, 0.1, 0.5,
+                                                      args.seed, lam)
+        s = SieTriMlfmm(lam, args.eps_r2, formulation=args.formulation, normal_convention=2,
+                        tree_s_opt=400, truncation_number=19, precision=args.precision)
+        s.precalculation_quad(Lx, Ly, nx, ny, z)
+        st = s.stats()
+        prof = s.profile_matvec(reps=args.reps, c64=args.precision == "mixed")
+        print(json.dumps({"workload": f"quadrilateral rough patch {n / 10:g} lambda, {n}x{n} "
+                                      f"elements, M={st['n_edges']}, {args.formulation}",
+                          "operator": "complex64" if args.precision == "mixed" else "complex128",
+                          "setup_ms": st["setup_ms"], "near_fill_ms": st["near_fill_ms"],
+                          "bytes_near": st["bytes_near"], "bytes_patterns": st["bytes_patterns"],
+                          "stages_ms": {k: v[0] for k, v in prof.items()}}), flush=True)
+        return
     pp, tt = meshes.rough_surface(args.cells, args.cells, args.size, args.size, 0.1, 0.5,
                                   args.seed, lam)
     s = SieTriMlfmm(lam, args.eps_r2, formulation=args.formulation, normal_convention=2,
