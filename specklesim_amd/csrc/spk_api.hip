@@ -1412,6 +1412,8 @@ int spk_set_transposed(spk_handle *h, int transposed) {
   const bool want = transposed != 0;
   if (want == h->transposed) return SPK_OK;
   if (want && h->quad) return fail(SPK_ERR_STATE, "quadrilateral handles: no transposed operator");
+  // the block-Jacobi inverses belong to the untransposed blocks
+  h->blu.ok = false;
   if (want && h->multik)
     return fail(SPK_ERR_STATE, "per-level truncation numbers: no transposed operator");
   HIPCHK(hipSetDevice(h->p.device));
@@ -1668,7 +1670,8 @@ static int scattered_staged(spk_handle *h, int npts, const double *r, const zd *
   const ScatArgs a = scat_args(h, npts);
   HIPCHK(hipMemcpyAsync(dr, r, 3 * (size_t)npts * sizeof(double), hipMemcpyHostToDevice, h->st));
   if (h->quad) {
-    // element-local coefficients (every slot written), then the field
+    // element-local coefficients (every slot is some function's), then the field
+    HIPCHK(hipMemsetAsync(h->q_seh.p, 0, h->q_seh.n * sizeof(zd), h->st));
     launch_quad_local(h->qtopo(), d_x, h->q_seh.p, h->st);
     launch_quad_field(h->qdev, h->qtopo(), npts, dr, h->q_seh.p, (double *)dE, (double *)dH, h->st);
   } else {
