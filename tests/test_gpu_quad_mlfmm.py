@@ -197,6 +197,49 @@ def test_block_lu_solve_against_dense_lu(patch):
     assert np.allclose(out["mixed"], out["fp64"], rtol=1e-3)
 
 
+def _dense_rows_pmchwt(q, rows):
+    """Rows s (0-based) of the oracle's assembled PMCHWT system, [MM[s, :];
+    MM[M + s, :]], from impedance_edge_versus_element over every element and
+    the assembly rule (lib_sie_quad_solver_normal.f90:139-208)."""
+    E = q.edges()
+    M, nel = q.M, q.n_el
+    ii, pa, jj, pq, f = E[:, 0] - 1, E[:, 1] - 1, E[:, 2] - 1, E[:, 3] - 1, E[:, 4]
+    top = np.zeros((len(rows), 2 * M), complex)
+    bot = np.zeros((len(rows), 2 * M), complex)
+    for r, s in enumerate(rows):
+        S = np.array([q.edge_element(s + 1, t + 1) for t in range(nel)])  # (nel, 3, 10)
+        m3 = S[ii, :, pa]
+        nb = jj >= 0
+        m3[nb] += S[jj[nb], :, pq[nb]] * f[nb, None]
+        top[r, :M], top[r, M:] = m3[:, 0], m3[:, 1]
+        bot[r, :M], bot[r, M:] = m3[:, 2], m3[:, 0]
+    return top, bot
+
+
+def test_two_level_product_rows_against_the_dense_system():
+    """A 4 lambda patch with leaf boxes of 0.5 lambda at level 3 under level
+    2 (SS, M2L at both levels and RR run on both 8-channel halves): 48
+    random rows of the product against the oracle's dense rows."""
+    Lx, Ly, nx, ny, z = meshes.rough_height_field(40, 40, 4.0, 4.0, 0.1, 0.5, 5, LAM)
+    q = O.QuadOracle(LAM, 2.25, Lx, Ly, nx, ny, z)
+    g = _gpu((Lx, Ly, nx, ny, z), "PMCHWT")
+    st = g.stats()
+    assert (st["l_min"], st["l_max"]) == (2, 3)
+    assert st["i4_pairs_per_level"][2] > 0 and st["i4_pairs_per_level"][3] > 0
+    rng = np.random.default_rng(12)
+    rows = np.sort(rng.choice(g.N, 48, replace=False))
+    top, bot = _dense_rows_pmchwt(q, rows)
+    x = rng.standard_normal(2 * g.N) + 1j * rng.standard_normal(2 * g.N)
+    y = g.calculate_vector_b_tri(x)
+    yn, _ = g.matvec_parts(x)
+    e_top = rel(y[rows], top @ x)
+    e_bot = rel(y[g.N + rows], bot @ x)
+    ef = rel(np.concatenate([y[rows] - yn[rows], y[g.N + rows] - yn[g.N + rows]]),
+             np.concatenate([top @ x - yn[rows], bot @ x - yn[g.N + rows]]))
+    print(f"M {g.N}, levels 2-3: rows vs dense {e_top:.2e} / {e_bot:.2e}, far part {ef:.2e}")
+    assert e_top < 1e-4 and e_bot < 1e-4 and ef < 3e-3
+
+
 def test_refusals(patch):
     geom, _, _ = patch
     g = _gpu(geom, "PMCHWT")
