@@ -214,6 +214,25 @@ int spk_quad_fill_device(spk_quad *q, void *d_MM);
 int spk_quad_rhs_plane_wave(spk_quad *q, const double k_in[3], const double e_in[3], double *b);
 int spk_quad_rhs_plane_wave_device(spk_quad *q, const double k_in[3], const double e_in[3],
                                    void *d_b);
+/* The reference's quadrilateral solve, solver_LU (ZGESV,
+ * src/lib_sie_math.f90:649-676) as lib_sie_quad_solver_normal_I calls it
+ * (src/lib_sie_quad_solver_normal.f90:216-217), by rocSOLVER in FP64: d_MM is
+ * the row-major 2M x 2M system of spk_quad_fill_device and is overwritten by
+ * its LU factors; d_b holds nrhs right-hand sides of 2M (column j at
+ * d_b + 2M j) and is overwritten by the solutions.  An exactly zero pivot
+ * returns SPK_ERR_STATE (where the reference stops). */
+int spk_quad_solve_lu_device(spk_quad *q, void *d_MM, int nrhs, void *d_b);
+/* Fill, plane-wave right-hand side and LU solve in one call (the device
+ * matrix, 16 (2M)^2 bytes, lives for the call only): x[2M] complex, the
+ * V_EH that lib_sie_quad_solver_normal_I writes out (:225-229). */
+int spk_quad_solve_plane_wave(spk_quad *q, const double k_in[3], const double e_in[3], double *x);
+/* lib_sie_quad_solver_normal_II (src/lib_sie_quad_solver_normal.f90:272-335):
+ * the element-local coefficients S_EH of a solution x[2M] and the scattered
+ * E, H (3 complex each per point) at npts points r[3*npts]
+ * (lib_sie_quad_observation_field_calculation, :337-, five-point rule);
+ * arguments in the order of spk_scattered_field. */
+int spk_quad_scattered_field(spk_quad *q, int npts, const double *r, const double *x, double *E,
+                             double *H);
 /* Host-only (no GPU): mesh, sizes and Edge_I of spk_quad_setup; arrays may
  * be NULL to query the sizes first. */
 int spk_quad_topology_probe(double Lx, double Ly, int nx, int ny, const double *z, int *n_edges,

@@ -26,7 +26,8 @@ EXPORTS = [
     "spk_quad_create", "spk_quad_destroy", "spk_quad_setup", "spk_quad_sizes",
     "spk_quad_get_mesh", "spk_quad_get_edges", "spk_quad_edge_element", "spk_quad_fill",
     "spk_quad_fill_device", "spk_quad_topology_probe", "spk_quad_rhs_plane_wave",
-    "spk_quad_rhs_plane_wave_device", "spk_probe_read_bandwidth", "spk_matvec_batch_device",
+    "spk_quad_rhs_plane_wave_device", "spk_quad_solve_lu_device", "spk_quad_solve_plane_wave",
+    "spk_quad_scattered_field", "spk_probe_read_bandwidth", "spk_matvec_batch_device",
     "spk_matvec_batch_c64_device", "spk_ensemble_intensity", "spk_gmres_batch_device",
     "spk_device_alloc", "spk_device_free", "spk_device_copy", "spk_device_synchronize",
     "spk_gmres", "spk_gmres_device", "spk_gmres_history", "spk_get_rwg", "spk_get_tree",
@@ -162,6 +163,9 @@ def lib():
     L.spk_quad_fill_device.argtypes = [vp, vp]
     L.spk_quad_rhs_plane_wave.argtypes = [vp, dp, dp, dp]
     L.spk_quad_rhs_plane_wave_device.argtypes = [vp, dp, dp, vp]
+    L.spk_quad_solve_lu_device.argtypes = [vp, vp, C.c_int, vp]
+    L.spk_quad_solve_plane_wave.argtypes = [vp, dp, dp, dp]
+    L.spk_quad_scattered_field.argtypes = [vp, C.c_int, dp, dp, dp, dp]
     L.spk_probe_read_bandwidth.argtypes = [C.c_int, C.c_size_t, C.c_int, dp]
     vpp = C.POINTER(C.c_void_p)
     L.spk_matvec_batch_device.argtypes = [vp, C.c_int, vpp, vpp]

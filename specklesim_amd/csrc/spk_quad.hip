@@ -20,6 +20,9 @@
 #include <string>
 #include <vector>
 
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
 #include "spk_quad.h"
 
 namespace spk {
@@ -1300,6 +1303,9 @@ struct spk_quad {
   bool ready = false;
   QBuf<double> d_el;
   QBuf<int> d_edge_p, d_eadj, d_eside, d_cadj;
+  // dense solve: rocSOLVER handle on st, pivots and status of the last factorisation
+  rocblas_handle rb = nullptr;
+  QBuf<int64_t> ipiv, info;
   QuadTopo topo() const {
     return QuadTopo{d_el.p, d_edge_p.p, d_eadj.p, d_eside.p, d_cadj.p, mesh.nel, mesh.medge};
   }
@@ -1331,6 +1337,7 @@ int spk_quad_create(const spk_params *params, spk_quad **out) {
 
 void spk_quad_destroy(spk_quad *q) {
   if (!q) return;
+  if (q->rb) (void)rocblas_destroy_handle(q->rb);
   if (q->st) (void)hipStreamDestroy(q->st);
   delete q;
 }
@@ -1546,6 +1553,80 @@ int spk_quad_fill(spk_quad *q, double *MM) {
   int rc = spk_quad_fill_device(q, d.p);
   if (rc) return rc;
   QCHK(hipMemcpy(MM, d.p, n * n * sizeof(zd), hipMemcpyDeviceToHost));
+  return SPK_OK;
+}
+
+int spk_quad_solve_lu_device(spk_quad *q, void *d_MM, int nrhs, void *d_b) {
+  if (!q || !d_MM || !d_b || nrhs < 1) return set_error(SPK_ERR_ARG, "bad arguments");
+  if (!q->ready) return set_error(SPK_ERR_STATE, "spk_quad_setup has not completed");
+  QCHK(hipSetDevice(q->p.device));
+  const int64_t n = 2 * (int64_t)q->mesh.medge;
+  if (!q->rb && rocblas_create_handle(&q->rb) != rocblas_status_success)
+    return set_error(SPK_ERR_HIP, "rocblas_create_handle failed");
+  if (rocblas_set_stream(q->rb, q->st) != rocblas_status_success)
+    return set_error(SPK_ERR_HIP, "rocblas_set_stream failed");
+  if (q->ipiv.n != (size_t)n) QCHK(q->ipiv.alloc(n));
+  if (!q->info.p) QCHK(q->info.alloc(1));
+  // MM is row-major, so the column-major matrix rocSOLVER sees is MM^T:
+  // factor MM^T = P L U, then solve with the transposed factors, MM x = b.
+  auto *A = (rocblas_double_complex *)d_MM;
+  if (rocsolver_zgetrf_64(q->rb, n, n, A, n, q->ipiv.p, q->info.p) != rocblas_status_success)
+    return set_error(SPK_ERR_HIP, "rocsolver_zgetrf_64 failed");
+  int64_t info = 0;
+  QCHK(hipMemcpyAsync(&info, q->info.p, sizeof(int64_t), hipMemcpyDeviceToHost, q->st));
+  QCHK(hipStreamSynchronize(q->st));
+  if (info > 0)
+    return set_error(SPK_ERR_STATE, "U(" + std::to_string(info) + "," + std::to_string(info) +
+                                        ") is zero: the system matrix is singular");
+  if (rocsolver_zgetrs_64(q->rb, rocblas_operation_transpose, n, nrhs, A, n, q->ipiv.p,
+                          (rocblas_double_complex *)d_b, n) != rocblas_status_success)
+    return set_error(SPK_ERR_HIP, "rocsolver_zgetrs_64 failed");
+  QCHK(hipStreamSynchronize(q->st));
+  return SPK_OK;
+}
+
+int spk_quad_solve_plane_wave(spk_quad *q, const double k_in[3], const double e_in[3], double *x) {
+  if (!q || !k_in || !e_in || !x) return set_error(SPK_ERR_ARG, "null argument");
+  if (!q->ready) return set_error(SPK_ERR_STATE, "spk_quad_setup has not completed");
+  QCHK(hipSetDevice(q->p.device));
+  const size_t n = 2 * (size_t)q->mesh.medge;
+  QBuf<zd> MM, b;
+  QCHK(MM.alloc(n * n));
+  QCHK(b.alloc(n));
+  int rc = spk_quad_fill_device(q, MM.p);
+  if (!rc) rc = spk_quad_rhs_plane_wave_device(q, k_in, e_in, b.p);
+  if (!rc) rc = spk_quad_solve_lu_device(q, MM.p, 1, b.p);
+  if (rc) return rc;
+  QCHK(hipMemcpy(x, b.p, n * sizeof(zd), hipMemcpyDeviceToHost));
+  return SPK_OK;
+}
+
+int spk_quad_scattered_field(spk_quad *q, int npts, const double *r, const double *x, double *E,
+                             double *H) {
+  if (!q || !x || npts < 0 || (npts && (!r || !E || !H)))
+    return set_error(SPK_ERR_ARG, "bad arguments");
+  if (!q->ready) return set_error(SPK_ERR_STATE, "spk_quad_setup has not completed");
+  if (!npts) return SPK_OK;
+  QCHK(hipSetDevice(q->p.device));
+  const QuadMesh &m = q->mesh;
+  QBuf<zd> dx, seh, dE, dH;
+  QBuf<double> dr;
+  QCHK(dx.alloc(2 * (size_t)m.medge));
+  QCHK(seh.alloc(2 * (size_t)m.nel * 10));
+  QCHK(dr.alloc(3 * (size_t)npts));
+  QCHK(dE.alloc(3 * (size_t)npts));
+  QCHK(dH.alloc(3 * (size_t)npts));
+  QCHK(hipMemcpyAsync(dx.p, x, dx.n * sizeof(zd), hipMemcpyHostToDevice, q->st));
+  QCHK(hipMemcpyAsync(dr.p, r, dr.n * sizeof(double), hipMemcpyHostToDevice, q->st));
+  // element-local coefficients S_EH, then the field of every element at every point
+  QCHK(hipMemsetAsync(seh.p, 0, seh.n * sizeof(zd), q->st));
+  const QuadTopo T = q->topo();
+  launch_quad_local(T, dx.p, seh.p, q->st);
+  launch_quad_field(q->dev, T, npts, dr.p, seh.p, (double *)dE.p, (double *)dH.p, q->st);
+  QCHK(hipGetLastError());
+  QCHK(hipMemcpyAsync(E, dE.p, dE.n * sizeof(zd), hipMemcpyDeviceToHost, q->st));
+  QCHK(hipMemcpyAsync(H, dH.p, dH.n * sizeof(zd), hipMemcpyDeviceToHost, q->st));
+  QCHK(hipStreamSynchronize(q->st));
   return SPK_OK;
 }
 

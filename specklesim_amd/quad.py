@@ -5,7 +5,9 @@ Mirrors the reference's quad path (src/lib_sie_quad_solver_normal.f90:80-210):
 (sorting_quad, Neighbours_EdgeCorner_Full, Sorting_edge);
 ``impedance_edge_versus_element`` is the reference routine of that name
 (src/lib_sie_quad_singularity.f90:22-98) for (edge function, element) pairs,
-0-based here; ``fill`` returns the assembled 2M x 2M system.
+0-based here; ``fill`` returns the assembled 2M x 2M system,
+``solve_plane_wave`` solves it by LU as the reference does and
+``scattered_field`` evaluates the field of a solution.
 """
 from __future__ import annotations
 
@@ -99,6 +101,37 @@ class QuadFill:
         e = np.ascontiguousarray(e_in, dtype=np.float64)
         nat.check(nat.lib().spk_quad_rhs_plane_wave_device(self._h, _dp(k), _dp(e),
                                                            C.c_void_p(int(d_b))))
+
+    def solve_lu_device(self, d_MM, d_b, nrhs=1):
+        """solver_LU (ZGESV, src/lib_sie_math.f90:649-676) on the device:
+        d_MM (row-major, from fill_device) becomes its LU factors, d_b
+        (nrhs columns of 2M) the solutions."""
+        nat.check(nat.lib().spk_quad_solve_lu_device(self._h, C.c_void_p(int(d_MM)), int(nrhs),
+                                                     C.c_void_p(int(d_b))))
+
+    def solve_plane_wave(self, k_in, e_in):
+        """lib_sie_quad_solver_normal_I (src/lib_sie_quad_solver_normal.f90:80-229):
+        fill, plane-wave right-hand side and LU solve; V_EH[2M] complex."""
+        k = np.ascontiguousarray(k_in, dtype=np.float64)
+        e = np.ascontiguousarray(e_in, dtype=np.float64)
+        x = np.zeros(2 * self.M, np.complex128)
+        nat.check(nat.lib().spk_quad_solve_plane_wave(self._h, _dp(k), _dp(e),
+                                                      _dp(x.view(np.float64))))
+        return x
+
+    def scattered_field(self, points, x):
+        """lib_sie_quad_solver_normal_II (src/lib_sie_quad_solver_normal.f90:272-335):
+        E, H (npts x 3 complex) of the solution x[2M] at the points."""
+        x = np.ascontiguousarray(x, dtype=np.complex128)
+        if x.size != 2 * self.M:
+            raise ValueError("x must hold 2M coefficients")
+        r = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+        E = np.zeros((r.shape[0], 3), np.complex128)
+        H = np.zeros((r.shape[0], 3), np.complex128)
+        nat.check(nat.lib().spk_quad_scattered_field(
+            self._h, r.shape[0], _dp(r), _dp(x.view(np.float64)), _dp(E.view(np.float64)),
+            _dp(H.view(np.float64))))
+        return E, H
 
 
 def topology_probe(Lx, Ly, nx, ny, z):

@@ -86,3 +86,61 @@ def test_plane_wave_rhs(form):
         ref = q.rhs_plane(k, e, form)
         M = q.M
         assert rel(b[:M], ref[:M]) < 1e-12 and rel(b[M:], ref[M:]) < 1e-12
+
+
+@pytest.mark.parametrize("form", ["PMCHWT", "MCTF", "ICTF"])
+def test_lu_solve_and_field(form):
+    """lib_sie_quad_solver_normal_I/II (src/lib_sie_quad_solver_normal.f90:80-335):
+    GPU fill, right-hand side and rocSOLVER LU against LAPACK's ZGESV (the
+    routine solver_LU calls, src/lib_sie_math.f90:649-676) on the oracle's
+    system and right-hand side; then the field of the solution against the
+    oracle's Integration_scattered_field_quad."""
+    Lx, Ly, nx, ny, z = meshes.rough_height_field(5, 4, 1.8, 1.4, 0.05, 0.5, 11, LAM)
+    eps2 = 18.392 - 0.4164j
+    q = O.QuadOracle(LAM, eps2, Lx, Ly, nx, ny, z)
+    g = QuadFill(LAM, eps2, formulation=form).setup(Lx, Ly, nx, ny, z)
+    th = np.deg2rad(30.0)
+    k = np.array([np.sin(th), 0.0, -np.cos(th)])
+    e = np.array([0.0, 1.0, 0.0])
+    x = g.solve_plane_wave(k, e)
+    A = q.assemble(form, nthreads=16)
+    b = q.rhs_plane(k, e, form)
+    xr = np.linalg.solve(A, b)
+    cond = np.linalg.cond(A)
+    res = rel(A @ x, b)
+    ex = rel(x, xr)
+    print(f"{form}: 2M {2 * g.M}, cond {cond:.1e}, residual {res:.1e}, vs ZGESV {ex:.1e}")
+    # backward stable under the oracle's system; forward error within the
+    # conditioning of the system
+    assert res < 1e-12
+    assert ex < max(1e-10, 1e-15 * cond)
+    pts = np.array([[0.3, -0.2, 1.0], [1.5, 0.4, 2.0], [-2.0, 1.0, 0.5]]) * LAM * 3
+    E, H = g.scattered_field(pts, x)
+    Eo, Ho = q.scattered_field(pts, x)
+    assert rel(E, Eo) < 1e-11 and rel(H, Ho) < 1e-11
+
+
+def test_lu_solve_several_right_hand_sides_and_refusals():
+    """Two right-hand sides in one solve equal two single solves; a matrix
+    with a zero column is refused as singular, as the reference stops."""
+    from specklesim_amd import _native as nat
+    Lx, Ly, nx, ny, z = meshes.rough_height_field(3, 3, 1.2, 1.2, 0.05, 0.5, 4, LAM)
+    g = QuadFill(LAM, 2.25, formulation="ICTF").setup(Lx, Ly, nx, ny, z)
+    n = 2 * g.M
+    ks = [np.array([0.0, 0.0, -1.0]), np.array([np.sin(0.4), 0.0, -np.cos(0.4)])]
+    es = [np.array([1.0, 0.0, 0.0]), np.array([0.0, 1.0, 0.0])]
+    single = [g.solve_plane_wave(k, e) for k, e in zip(ks, es)]
+    MM = nat.DeviceBuffer(n * n * 16)
+    B = nat.DeviceBuffer(2 * n * 16)
+    g.fill_device(MM.ptr)
+    for j in range(2):
+        g.get_vector_b_quad_device(ks[j], es[j], B.ptr + j * n * 16)
+    g.solve_lu_device(MM.ptr, B.ptr, nrhs=2)
+    X = B.download(np.complex128).reshape(2, n)
+    for j in range(2):
+        assert rel(X[j], single[j]) < 1e-13
+    A = g.fill()
+    A[:, 5] = 0
+    MM.upload(A)
+    with pytest.raises(nat.SpeckleError, match="singular"):
+        g.solve_lu_device(MM.ptr, B.ptr, nrhs=1)
