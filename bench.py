@@ -469,6 +469,57 @@ def timed_quad(args, lam):
     return rec
 
 
+def timed_quad_dense(args, lam):
+    """The reference's own quadrilateral solve (lib_sie_quad_solver_normal_I,
+    src/lib_sie_quad_solver_normal.f90:80-229): the dense 2M x 2M system
+    filled on the GPU, the plane-wave right-hand side and rocSOLVER's FP64
+    LU (ZGESV in the reference, src/lib_sie_math.f90:649-676), on an n x n
+    rough patch of 0.1 lambda quadrilaterals (RMS 0.1, correlation 0.5
+    lambda, seed 1).  The factorisation is priced at 8/3 (2M)^3 real flops
+    against the FP64 matrix-core peak."""
+    import gc
+
+    from specklesim_amd import DeviceBuffer, QuadFill, meshes, plane_wave
+
+    n = args.quad_dense_cells
+    L = n / 10.0
+    Lx, Ly, nx, ny, z = meshes.rough_height_field(n, n, L, L, 0.1, 0.5, 1, lam)
+    g = QuadFill(lam, args.eps_r2, formulation=args.formulation).setup(Lx, Ly, nx, ny, z)
+    N = 2 * g.M
+    MM, b = DeviceBuffer(N * N * 16), DeviceBuffer(N * 16)
+    k, e = plane_wave(np.deg2rad(10.0), 0.0, 2, "TM")
+    runs = []
+    for _ in range(2):
+        _sync()
+        t0 = time.perf_counter()
+        g.fill_device(MM.ptr)
+        _sync()
+        t1 = time.perf_counter()
+        g.get_vector_b_quad_device(k, e, b.ptr)
+        g.solve_lu_device(MM.ptr, b.ptr)
+        _sync()
+        t2 = time.perf_counter()
+        runs.append({"fill_s": t1 - t0, "lu_s": t2 - t1, "total_s": t2 - t0})
+    best = min(runs, key=lambda r: r["total_s"])
+    flops = 8.0 / 3.0 * float(N) ** 3
+    x = b.download(np.complex128)
+    rec = {
+        "workload": f"{n} x {n} 8-node quadrilaterals on a {L:g} x {L:g} lambda rough patch "
+                    f"(RMS 0.1, correlation 0.5 lambda), {args.formulation}: dense fill, "
+                    f"plane-wave right-hand side and FP64 LU solve, as the reference solves it",
+        "edge_functions": g.M, "unknowns": N, "matrix_gb": N * N * 16 / 1e9,
+        "fill_s": best["fill_s"], "lu_s": best["lu_s"], "solve_s": best["total_s"],
+        "solves_per_s": 1.0 / best["total_s"], "runs": runs,
+        "lu_roofline": {"bound": "mfma", "achieved": flops / best["lu_s"] / 1e12, "peak": 78.6,
+                        "unit": "TFLOP/s", "frac": flops / best["lu_s"] / 1e12 / 78.6,
+                        "flops": flops},
+        "solution_finite": bool(np.isfinite(x).all()),
+    }
+    del MM, b, g
+    gc.collect()
+    return rec
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -494,6 +545,8 @@ def main():
                     help="nodes of the adjoint object's finite-difference check (0: skip it)")
     ap.add_argument("--quad-cells", type=int, default=200,
                     help="quadrilaterals per side of the C3 object (0: skip it)")
+    ap.add_argument("--quad-dense-cells", type=int, default=40,
+                    help="quadrilaterals per side of the dense LU object (0: skip it)")
     ap.add_argument("--workers", type=int, default=3,
                     help="worker threads (handles) of the ensemble")
     args = ap.parse_args()
@@ -535,6 +588,8 @@ def main():
     adj = (timed_adjoint(args, lam, pp, tt) if args.adjoint_nodes and args.precision == "both"
            else None)
     quad = timed_quad(args, lam) if args.quad_cells and args.precision == "both" else None
+    qdense = (timed_quad_dense(args, lam) if args.quad_dense_cells and args.precision == "both"
+              else None)
     ff_err = None
     if "fp64" in paths and "mixed" in paths:
         b64, b32 = brdf_curve(paths["fp64"], lam), brdf_curve(paths["mixed"], lam)
@@ -683,6 +738,8 @@ def main():
         rec["adjoint"] = adj
     if quad is not None:
         rec["quad_mlfmm"] = quad
+    if qdense is not None:
+        rec["quad_dense_lu"] = qdense
     if not args.no_cpu_baseline:
         it64 = (paths["fp64"] if "fp64" in paths else H)["runs"][-1]["iters"]
         cb = cpu_baseline(args, lam, pp, tt, it64)

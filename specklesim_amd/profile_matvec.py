@@ -17,6 +17,8 @@ around it.
 ``--near-f8`` its fp8 (E4M3FN) SpMV.
 ``--quad N`` instead sets up the MLFMM on an N x N quadrilateral patch and
 times its products (``--precision mixed`` for the complex64 mirror).
+``--quad-dense N`` instead fills the dense system of an N x N quadrilateral
+patch and solves it by LU (the reference's quadrilateral solve), twice.
 ``--solve near|mixed|fp64`` instead runs one solve of the normal-incidence
 TM plane wave to 1e-6 (near: the near-field preconditioned flexible GMRES;
 mixed: the refinement solve; fp64: the reference's GMRES with IMGS).
@@ -82,6 +84,9 @@ def main(argv=None):
     ap.add_argument("--quad", type=int, default=0,
                     help="the MLFMM products on an N x N quadrilateral rough patch of N/10 "
                          "lambda (RMS 0.1, correlation 0.5 lambda, tree_s_opt 400) instead")
+    ap.add_argument("--quad-dense", type=int, default=0,
+                    help="dense fill and LU solve of an N x N quadrilateral rough patch of N/10 "
+                         "lambda instead")
     args = ap.parse_args(argv)
     if args.copy_mib:
         print(json.dumps(copy_calibration(args.copy_mib)), flush=True)
@@ -90,6 +95,31 @@ def main(argv=None):
     from . import SieTriMlfmm, meshes
 
     lam = 633e-9
+    if args.quad_dense:
+        import time
+
+        from . import DeviceBuffer, QuadFill, plane_wave
+        from ._native import synchronize
+        n = args.quad_dense
+        Lx, Ly, nx, ny, z = meshes.rough_height_field(n, n, n / 10.0, n / 10.0, 0.1, 0.5,
+                                                      args.seed, lam)
+        g = QuadFill(lam, args.eps_r2, formulation=args.formulation).setup(Lx, Ly, nx, ny, z)
+        N = 2 * g.M
+        MM, b = DeviceBuffer(N * N * 16), DeviceBuffer(N * 16)
+        k, e = plane_wave(0.0, 0.0, 2, "TM")
+        out = []
+        for _ in range(2):
+            t0 = time.perf_counter()
+            g.fill_device(MM.ptr)
+            t1 = time.perf_counter()
+            g.get_vector_b_quad_device(k, e, b.ptr)
+            g.solve_lu_device(MM.ptr, b.ptr)
+            synchronize()
+            out.append({"fill_s": t1 - t0, "lu_s": time.perf_counter() - t1})
+        print(json.dumps({"workload": f"quadrilateral rough patch {n / 10:g} lambda, {n}x{n} "
+                                      f"elements, dense {N} x {N} LU, {args.formulation}",
+                          "runs": out}), flush=True)
+        return
     if args.quad:
         n = args.quad
         Lx, Ly, nx, ny, z = meshes.rough_height_field(n, n, n / 10.0, n / 10.0, 0.1, 0.5,
