@@ -348,7 +348,7 @@ int near_fill_quad(spk_handle *h) {
   HIPCHK(h->d_blocks.alloc(4 * (size_t)nnz));
   if (nnz == 0) return SPK_OK;
   std::vector<int> tb_off(nbl + 1, 0), tb_list, colA(t.e2_list.size()), colB(t.e2_list.size(), -1);
-  std::vector<int64_t> pbase(nbl + 1, 0);
+  std::vector<int64_t> pbase(nbl + 1, 0), toff(nbl + 1, 0);
   std::vector<int> els;
   for (int b = 0; b < nbl; ++b) {
     els.clear();
@@ -370,18 +370,20 @@ int near_fill_quad(spk_handle *h) {
     tb_off[b + 1] = tb_off[b] + (int)els.size();
     tb_list.insert(tb_list.end(), els.begin(), els.end());
     pbase[b + 1] = pbase[b] + (int64_t)t.box_count[b] * (int64_t)els.size();
+    toff[b + 1] = toff[b] + (int64_t)((t.box_count[b] + 63) / 64) * (int64_t)els.size();
   }
-  DBuf<int> d_tb_off, d_tb_list, d_colA, d_colB, l_s, l_es, l_et, l_side;
-  DBuf<int64_t> d_pbase, l_sdst, l_edst;
-  DBuf<zd> sums, so, eo;
+  DBuf<int> d_tb_off, d_tb_list, d_colA, d_colB, l_s, l_es, l_et, l_side, l_cs, l_ct, l_cg;
+  DBuf<int64_t> d_pbase, d_toff, l_sdst, l_edst, l_cdst;
+  DBuf<zd> sums, so, eo, co;
   HIPCHK(d_tb_off.upload(tb_off));
   HIPCHK(d_tb_list.upload(tb_list));
   HIPCHK(d_colA.upload(colA));
   HIPCHK(d_colB.upload(colB));
   HIPCHK(d_pbase.upload(pbase));
+  HIPCHK(d_toff.upload(toff));
   const QuadTopo T = h->qtopo();
-  std::vector<int> ls, es, et, eside;
-  std::vector<int64_t> sdst, edst;
+  std::vector<int> ls, es, et, eside, cs, ct, cg;
+  std::vector<int64_t> sdst, edst, cdst;
   int64_t max_pairs = 0;
   for (int b = 0; b < nbl; ++b) max_pairs = std::max(max_pairs, pbase[b + 1] - pbase[b]);
   HIPCHK(sums.alloc((size_t)std::max(max_pairs, std::min(kQuadPairChunk, pbase[nbl])) * 30));
@@ -389,13 +391,15 @@ int near_fill_quad(spk_handle *h) {
   for (int b0 = 0; b0 < nbl;) {
     int b1 = b0 + 1;
     while (b1 < nbl && pbase[b1 + 1] - pbase[b0] <= kQuadPairChunk) ++b1;
-    const int64_t nq = pbase[b1] - pbase[b0];
     ls.clear();
     sdst.clear();
     es.clear();
     et.clear();
     eside.clear();
     edst.clear();
+    cs.clear();
+    ct.clear();
+    cdst.clear();
     for (int b = b0; b < b1; ++b) {
       const int *tl = &tb_list[tb_off[b]];
       const int nT = tb_off[b + 1] - tb_off[b];
@@ -415,10 +419,22 @@ int near_fill_quad(spk_handle *h) {
           eside.push_back(a.side[1]);
           edst.push_back(q0 + (pt - tl));
         }
+        // corner-adjacent pairs: the 6 x 6 rule (pair_class order: an edge
+        // neighbour is never also taken as a corner neighbour)
+        for (int c : m.cadj[ii]) {
+          bool edge = false;
+          for (const QuadEdgeAdj &a : m.eadj[ii]) edge = edge || a.other == c;
+          if (edge || c - 1 == ii) continue;
+          const int *pt = std::lower_bound(tl, tl + nT, c - 1);
+          if (pt == tl + nT || *pt != c - 1) continue;
+          cs.push_back(s);
+          ct.push_back(c - 1);
+          cdst.push_back(q0 + (pt - tl));
+        }
       }
     }
-    launch_qn_regular(h->qdev, T, b0, b1, nq, d_pbase.p, d_tb_off.p, d_tb_list.p,
-                      h->d_box_first.p, h->d_perm.p, sums.p, h->st);
+    launch_qn_regular(h->qdev, T, b0, b1, toff[b1] - toff[b0], d_toff.p, d_pbase.p, d_tb_off.p,
+                      d_tb_list.p, h->d_box_first.p, h->d_box_count.p, h->d_perm.p, sums.p, h->st);
     HIPCHK(l_s.upload(ls));
     HIPCHK(l_sdst.upload(sdst));
     HIPCHK(so.alloc(ls.size() * 30));
@@ -429,8 +445,16 @@ int near_fill_quad(spk_handle *h) {
     HIPCHK(l_edst.upload(edst));
     HIPCHK(eo.alloc(std::max<size_t>(1, es.size()) * 30));
     launch_quad_edge_adj(h->qdev, T, (int)es.size(), l_es.p, l_et.p, l_side.p, eo.p, h->st);
+    cg.assign(cs.size(), 6);
+    HIPCHK(l_cs.upload(cs));
+    HIPCHK(l_ct.upload(ct));
+    HIPCHK(l_cg.upload(cg));
+    HIPCHK(l_cdst.upload(cdst));
+    HIPCHK(co.alloc(std::max<size_t>(1, cs.size()) * 30));
+    launch_quad_regular_list(h->qdev, T, (int)cs.size(), l_cs.p, l_ct.p, l_cg.p, co.p, h->st);
     launch_qn_scatter((int)ls.size(), l_sdst.p, so.p, sums.p, h->st);
     launch_qn_scatter((int)es.size(), l_edst.p, eo.p, sums.p, h->st);
+    launch_qn_scatter((int)cs.size(), l_cdst.p, co.p, sums.p, h->st);
     launch_qn_assemble(T, h->p.formulation, h->q_eta_a, h->q_eta12, b0, b1,
                        t.near_off[b1] - t.near_off[b0], h->d_near_off.p, h->d_e2_off.p,
                        h->d_e2_list.p, d_colA.p, d_colB.p, d_pbase.p, d_tb_off.p, sums.p,

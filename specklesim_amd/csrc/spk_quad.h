@@ -46,7 +46,8 @@ struct QuadTopo {
   int nel, medge;
 };
 
-// regular pairs of rows [s0, s0+rows) x all elements into sums[r][t][30]
+// regular pairs (no shared node) of rows [s0, s0+rows) x all elements into
+// sums[r][t][30]; other pairs' slots are left as they are
 void launch_quad_regular_grid(const QuadDev &d, const QuadTopo &T, int s0, int rows, zd *sums,
                               hipStream_t st);
 // regular pairs from a list (s, t, ngp), 0-based s and t, out[i*30]
@@ -110,12 +111,14 @@ void launch_qfold(int nbl, int S, const T *D, size_t half, const double *khat,
 template <class T>
 void launch_qdisagg(int ntask, int N, int S, const int *task, const int *perm,
                     const int *leaf_box_of_pos, const T *rt, const T *W, T *y, hipStream_t st);
-// near field of leaf boxes [b0, b1): pair sums (regular and corner pairs),
-// the self / edge-adjacent list results scattered to their pair index, then
-// the 2 x 2 blocks
-void launch_qn_regular(const QuadDev &d, const QuadTopo &T, int b0, int b1, int64_t nq,
-                       const int64_t *pbase, const int *tb_off, const int *tb_list,
-                       const int *box_first, const int *perm, zd *sums, hipStream_t st);
+// near field of leaf boxes [b0, b1): pair sums (regular pairs), the self /
+// edge- / corner-adjacent list results scattered to their pair index, then
+// the 2 x 2 blocks.  Regular pairs by tasks (box, 64-row block, element of
+// T(b)): toff[b] = the first task of box b, ntask = toff[b1] - toff[b0].
+void launch_qn_regular(const QuadDev &d, const QuadTopo &T, int b0, int b1, int64_t ntask,
+                       const int64_t *toff, const int64_t *pbase, const int *tb_off,
+                       const int *tb_list, const int *box_first, const int *box_count,
+                       const int *perm, zd *sums, hipStream_t st);
 void launch_qn_scatter(int n, const int64_t *dst, const zd *src, zd *sums, hipStream_t st);
 void launch_qn_assemble(const QuadTopo &T, int form, zd eta_a, zd eta12, int b0, int b1,
                         int64_t nblocks, const int64_t *near_off, const int *e2_off,

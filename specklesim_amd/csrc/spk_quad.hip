@@ -223,6 +223,75 @@ __device__ void regular_pair(const QuadDev &D, const double *x, const double *w,
   combine(D, S, out);
 }
 
+// Source-element data of the regular 4 x 4 rule at point (a, b), shared by
+// every pair of a workgroup that integrates over the same element: position,
+// rule weight and, per edge function n, JJ (n x N_n) and the n x curl factor
+// fedge_rot (JJ times nrot of acc_ff).
+struct QSrc {
+  double R[3], w;
+  double aJ[10][3];
+  double fr[10];
+};
+__device__ inline void qsrc_point(const QuadDev &D, const double *Eq, int a, int b, QSrc &o) {
+  const double k = D.x4[a], e = D.x4[b];
+  Geo g;
+  geo(k, e, Eq, g);
+  rloc(k, e, Eq, o.R);
+  o.w = D.w4[a] * D.w4[b];
+#pragma unroll
+  for (int n = 0; n < 10; ++n) {
+    const double f = fedge(n, k, e);
+    const double *v = n < 5 ? g.au : g.av;
+    const double Nq[3] = {f * v[0], f * v[1], f * v[2]};
+    double aa[3];
+    cross3(g.nq, Nq, aa);
+    for (int d = 0; d < 3; ++d) o.aJ[n][d] = aa[d] * g.JJ;
+    o.fr[n] = fedge_rot(n, k, e);
+  }
+}
+
+// regular_pair<4> with the source element's 16 points read from src (point
+// a, b at src[4a + b], the inner loop order of regular_pair)
+__device__ void regular_pair_tab(const QuadDev &D, const double *Ea, const QSrc *src, int pI,
+                                 zd out[3][10]) {
+  zd S[5][10];
+  for (int k = 0; k < 5; ++k)
+    for (int n = 0; n < 10; ++n) S[k][n] = zd{0, 0};
+  for (int j = 0; j < 4; ++j)
+    for (int i = 0; i < 4; ++i) {
+      Geo ga;
+      geo(D.x4[i], D.x4[j], Ea, ga);
+      double Na[3], Ra[3];
+      obs_fn(ga, pI, D.x4[i], D.x4[j], Na);
+      rloc(D.x4[i], D.x4[j], Ea, Ra);
+      const double TRF = D.w4[i] * D.w4[j] * ga.JJ;
+      for (int p = 0; p < 16; ++p) {
+        const QSrc &P = src[p];
+        const double R[3] = {Ra[0] - P.R[0], Ra[1] - P.R[1], Ra[2] - P.R[2]};
+        const double nR = sqrt(dot3(R, R));
+        const double rn[3] = {R[0] / nR, R[1] / nR, R[2] / nR};
+        zd Ga, GSa, Gm, GSm;
+        qgreen(D.k1, nR, Ga, GSa);
+        qgreen(D.k2, nR, Gm, GSm);
+        const zd gA = GSa + GSm, gBE = D.ie1 * GSa + D.ie2 * GSm, gC = Ga + Gm;
+        const zd gCH = D.e1 * Ga + D.e2 * Gm;
+        const double Nrn = dot3(Na, rn), w = P.w * TRF;
+#pragma unroll
+        for (int n = 0; n < 10; ++n) {
+          double t[3];
+          cross3(P.aJ[n], rn, t);
+          const double f1 = dot3(Na, t), f2 = Nrn * P.fr[n], f3 = dot3(Na, P.aJ[n]);
+          S[0][n] = S[0][n] + gA * (f1 * w);
+          S[1][n] = S[1][n] + gBE * (f2 * w);
+          S[2][n] = S[2][n] + gA * (f2 * w);
+          S[3][n] = S[3][n] + gC * (f3 * w);
+          S[4][n] = S[4][n] + gCH * (f3 * w);
+        }
+      }
+    }
+  combine(D, S, out);
+}
+
 __device__ inline int pair_class(const QuadTopo &T, int s, int t, int *side) {
   const int ii = T.edge_p[5 * s] - 1;
   if (t == ii) return 0;
@@ -242,21 +311,24 @@ __device__ inline void store30(zd out[3][10], zd *dst) {
     for (int n = 0; n < 10; ++n) dst[k * 10 + n] = out[k][n];
 }
 
-__global__ void __launch_bounds__(128) k_quad_regular_grid(QuadDev D, QuadTopo T, int s0, int rows,
-                                                           zd *sums) {
-  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (id >= (int64_t)rows * T.nel) return;
-  const int r = (int)(id / T.nel), t = (int)(id % T.nel);
+// dense fill, rows [s0, s0 + rows): workgroup = (element t, 64 rows), the
+// regular pairs (no shared node) on the element's source table; self,
+// edge- and corner-adjacent pairs come from the list kernels
+__global__ void __launch_bounds__(64) k_quad_regular_grid(QuadDev D, QuadTopo T, int s0, int rows,
+                                                          int nrb, zd *sums) {
+  __shared__ QSrc src[16];
+  const int t = blockIdx.x / nrb, rb = blockIdx.x - t * nrb;
+  if (threadIdx.x < 16)
+    qsrc_point(D, T.el + (size_t)t * 24, threadIdx.x >> 2, threadIdx.x & 3, src[threadIdx.x]);
+  __syncthreads();
+  const int r = rb * 64 + threadIdx.x;
+  if (r >= rows) return;
   const int s = s0 + r;
   int side[2];
-  const int cls = pair_class(T, s, t, side);
-  if (cls < 2) return;
+  if (pair_class(T, s, t, side) != 3) return;
   const int ii = T.edge_p[5 * s] - 1, pI = T.edge_p[5 * s + 1];
   zd out[3][10];
-  if (cls == 2)
-    regular_pair<6>(D, D.x6, D.w6, T.el + (size_t)ii * 24, T.el + (size_t)t * 24, pI, out);
-  else
-    regular_pair<4>(D, D.x4, D.w4, T.el + (size_t)ii * 24, T.el + (size_t)t * 24, pI, out);
+  regular_pair_tab(D, T.el + (size_t)ii * 24, src, pI, out);
   store30(out, sums + ((size_t)r * T.nel + t) * 30);
 }
 
@@ -893,9 +965,10 @@ __global__ void __launch_bounds__(256) k_qdisagg(int ntask, int N, int S, const 
 
 // Near field, per chunk of leaf boxes [b0, b1): the (row edge function,
 // element) pairs of box b are rows x T(b), T(b) = the elements of its E2
-// columns; pair index q = pbase[b] + r * |T(b)| + li.  Regular and corner
-// pairs here (one thread each); self and edge-adjacent pairs come from the
-// list kernels and are scattered in.
+// columns; pair index q = pbase[b] + r * |T(b)| + li.  Regular pairs here
+// (one thread each, a workgroup sharing one element's source table); self,
+// edge- and corner-adjacent pairs come from the list kernels and are
+// scattered in.
 __device__ inline int upper_box(const int64_t *off, int b0, int b1, int64_t g) {
   int lo = b0, hi = b1 - 1;
   while (lo < hi) {
@@ -908,29 +981,32 @@ __device__ inline int upper_box(const int64_t *off, int b0, int b1, int64_t g) {
   return lo;
 }
 
-__global__ void __launch_bounds__(128) k_qn_regular(QuadDev D, QuadTopo T, int b0, int b1,
-                                                    int64_t nq, const int64_t *pbase,
-                                                    const int *tb_off, const int *tb_list,
-                                                    const int *box_first, const int *perm,
-                                                    zd *sums) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nq) return;
-  const int64_t q = pbase[b0] + i;
-  const int b = upper_box(pbase, b0, b1, q);
+__global__ void __launch_bounds__(64) k_qn_regular(QuadDev D, QuadTopo T, int b0, int b1,
+                                                   const int64_t *toff, const int64_t *pbase,
+                                                   const int *tb_off, const int *tb_list,
+                                                   const int *box_first, const int *box_count,
+                                                   const int *perm, zd *sums) {
+  // task = (box b, 64-row block rb, element li of T(b)), task index
+  // toff[b] + rb * |T(b)| + li
+  __shared__ QSrc src[16];
+  const int64_t task = toff[b0] + blockIdx.x;
+  const int b = upper_box(toff, b0, b1, task);
   const int nT = tb_off[b + 1] - tb_off[b];
-  const int64_t rel = q - pbase[b];
-  const int r = (int)(rel / nT), li = (int)(rel - (int64_t)r * nT);
-  const int s = perm[box_first[b] + r], t = tb_list[tb_off[b] + li];
+  const int64_t rel = task - toff[b];
+  const int rb = (int)(rel / nT), li = (int)(rel - (int64_t)rb * nT);
+  const int t = tb_list[tb_off[b] + li];
+  if (threadIdx.x < 16)
+    qsrc_point(D, T.el + (size_t)t * 24, threadIdx.x >> 2, threadIdx.x & 3, src[threadIdx.x]);
+  __syncthreads();
+  const int r = rb * 64 + threadIdx.x;
+  if (r >= box_count[b]) return;
+  const int s = perm[box_first[b] + r];
   int side[2];
-  const int cls = pair_class(T, s, t, side);
-  if (cls < 2) return;
+  if (pair_class(T, s, t, side) != 3) return;
   const int ii = T.edge_p[5 * s] - 1, pI = T.edge_p[5 * s + 1];
   zd out[3][10];
-  if (cls == 2)
-    regular_pair<6>(D, D.x6, D.w6, T.el + (size_t)ii * 24, T.el + (size_t)t * 24, pI, out);
-  else
-    regular_pair<4>(D, D.x4, D.w4, T.el + (size_t)ii * 24, T.el + (size_t)t * 24, pI, out);
-  store30(out, sums + (size_t)i * 30);
+  regular_pair_tab(D, T.el + (size_t)ii * 24, src, pI, out);
+  store30(out, sums + (size_t)(pbase[b] - pbase[b0] + (int64_t)r * nT + li) * 30);
 }
 
 __global__ void k_qn_scatter(int n, const int64_t *dst, const zd *src, zd *sums) {
@@ -1111,12 +1187,13 @@ template void launch_qdisagg<zd>(int, int, int, const int *, const int *, const 
                                  const zd *, zd *, hipStream_t);
 template void launch_qdisagg<zf>(int, int, int, const int *, const int *, const int *, const zf *,
                                  const zf *, zf *, hipStream_t);
-void launch_qn_regular(const QuadDev &d, const QuadTopo &T, int b0, int b1, int64_t nq,
-                       const int64_t *pbase, const int *tb_off, const int *tb_list,
-                       const int *box_first, const int *perm, zd *sums, hipStream_t st) {
-  if (nq <= 0) return;
-  hipLaunchKernelGGL(k_qn_regular, dim3(nb(nq, 128)), dim3(128), 0, st, d, T, b0, b1, nq, pbase,
-                     tb_off, tb_list, box_first, perm, sums);
+void launch_qn_regular(const QuadDev &d, const QuadTopo &T, int b0, int b1, int64_t ntask,
+                       const int64_t *toff, const int64_t *pbase, const int *tb_off,
+                       const int *tb_list, const int *box_first, const int *box_count,
+                       const int *perm, zd *sums, hipStream_t st) {
+  if (ntask <= 0) return;
+  hipLaunchKernelGGL(k_qn_regular, dim3((unsigned)ntask), dim3(64), 0, st, d, T, b0, b1, toff,
+                     pbase, tb_off, tb_list, box_first, box_count, perm, sums);
 }
 void launch_qn_scatter(int n, const int64_t *dst, const zd *src, zd *sums, hipStream_t st) {
   if (n <= 0) return;
@@ -1152,10 +1229,10 @@ void launch_quad_scatter(int n, int nel, const int *r, const int *t, const zd *s
 
 void launch_quad_regular_grid(const QuadDev &d, const QuadTopo &T, int s0, int rows, zd *sums,
                               hipStream_t st) {
-  const int64_t n = (int64_t)rows * T.nel;
-  if (n <= 0) return;
-  hipLaunchKernelGGL(k_quad_regular_grid, dim3(nb(n, 128)), dim3(128), 0, st, d, T, s0, rows,
-                     sums);
+  if (rows <= 0 || T.nel <= 0) return;
+  const int nrb = (rows + 63) / 64;
+  hipLaunchKernelGGL(k_quad_regular_grid, dim3((unsigned)((int64_t)T.nel * nrb)), dim3(64), 0, st,
+                     d, T, s0, rows, nrb, sums);
 }
 void launch_quad_regular_list(const QuadDev &d, const QuadTopo &T, int n, const int *s,
                               const int *t, const int *ngp, zd *out, hipStream_t st) {
@@ -1470,17 +1547,20 @@ int spk_quad_fill_device(spk_quad *q, void *d_MM) {
   if (!q->ready) return set_error(SPK_ERR_STATE, "spk_quad_setup has not completed");
   const QuadMesh &m = q->mesh;
   const QuadTopo T = q->topo();
-  // rows of edge functions in chunks whose [rows][elements][30] sums fit 256 MiB
+  // rows of edge functions in chunks whose [rows][elements][30] sums fit
+  // 256 MiB, whole 64-row blocks of the regular-pair kernel where they can
   const size_t per_row = (size_t)m.nel * 30 * sizeof(zd);
-  const int chunk = (int)std::max<size_t>(1, std::min<size_t>((size_t)m.medge, (256u << 20) / per_row));
-  QBuf<zd> sums, so, eo;
-  QBuf<int> sl, el_s, el_t, el_side, sc_r, sc_t, sc_r2;
+  size_t rows_fit = std::max<size_t>(1, (256u << 20) / per_row);
+  if (rows_fit >= 64) rows_fit -= rows_fit % 64;
+  const int chunk = (int)std::min<size_t>((size_t)m.medge, rows_fit);
+  QBuf<zd> sums, so, eo, co;
+  QBuf<int> sl, el_s, el_t, el_side, sc_r, sc_t, sc_r2, cl_s, cl_t, cl_g, sc_r3;
   QCHK(sums.alloc((size_t)chunk * m.nel * 30));
   for (int s0 = 0; s0 < m.medge; s0 += chunk) {
     const int rows = std::min(chunk, m.medge - s0);
     launch_quad_regular_grid(q->dev, T, s0, rows, sums.p, q->st);
-    // self and edge-adjacent pairs of these rows
-    std::vector<int> ss(rows), es, et, eside;
+    // self, edge- and corner-adjacent pairs of these rows
+    std::vector<int> ss(rows), es, et, eside, cs, ct, cr;
     for (int r = 0; r < rows; ++r) {
       ss[r] = s0 + r;
       const int ii = m.edge_p[5 * (size_t)(s0 + r)] - 1;
@@ -1489,6 +1569,14 @@ int spk_quad_fill_device(spk_quad *q, void *d_MM) {
         et.push_back(a.other - 1);
         eside.push_back(a.side[0]);
         eside.push_back(a.side[1]);
+      }
+      for (int c : m.cadj[ii]) {
+        bool edge = false;
+        for (const QuadEdgeAdj &a : m.eadj[ii]) edge = edge || a.other == c;
+        if (edge || c - 1 == ii) continue;
+        cs.push_back(s0 + r);
+        ct.push_back(c - 1);
+        cr.push_back(r);
       }
     }
     QCHK(sl.upload(ss));
@@ -1499,6 +1587,12 @@ int spk_quad_fill_device(spk_quad *q, void *d_MM) {
     QCHK(el_side.upload(eside));
     QCHK(eo.alloc(es.size() * 30));
     launch_quad_edge_adj(q->dev, T, (int)es.size(), el_s.p, el_t.p, el_side.p, eo.p, q->st);
+    const std::vector<int> cg(cs.size(), 6);
+    QCHK(cl_s.upload(cs));
+    QCHK(cl_t.upload(ct));
+    QCHK(cl_g.upload(cg));
+    QCHK(co.alloc(cs.size() * 30));
+    launch_quad_regular_list(q->dev, T, (int)cs.size(), cl_s.p, cl_t.p, cl_g.p, co.p, q->st);
     QCHK(hipGetLastError());
     // scatter the self and edge-adjacent pairs into the sums
     std::vector<int> sr(rows), st_(rows), er(es.size());
@@ -1512,6 +1606,8 @@ int spk_quad_fill_device(spk_quad *q, void *d_MM) {
     launch_quad_scatter(rows, m.nel, sc_r.p, sc_t.p, so.p, sums.p, q->st);
     QCHK(sc_r2.upload(er));
     launch_quad_scatter((int)es.size(), m.nel, sc_r2.p, el_t.p, eo.p, sums.p, q->st);
+    QCHK(sc_r3.upload(cr));
+    launch_quad_scatter((int)cs.size(), m.nel, sc_r3.p, cl_t.p, co.p, sums.p, q->st);
     launch_quad_assemble(T, q->p.formulation, q->eta_a, q->eta12, s0, rows, sums.p, (zd *)d_MM,
                          q->st);
     QCHK(hipGetLastError());
