@@ -372,7 +372,7 @@ int near_fill_quad(spk_handle *h) {
     pbase[b + 1] = pbase[b] + (int64_t)t.box_count[b] * (int64_t)els.size();
     toff[b + 1] = toff[b] + (int64_t)((t.box_count[b] + 63) / 64) * (int64_t)els.size();
   }
-  DBuf<int> d_tb_off, d_tb_list, d_colA, d_colB, l_s, l_es, l_et, l_side, l_cs, l_ct, l_cg;
+  DBuf<int> d_tb_off, d_tb_list, d_colA, d_colB, l_es, l_et, l_side, l_cs, l_ct, l_cg;
   DBuf<int64_t> d_pbase, d_toff, l_sdst, l_edst, l_cdst;
   DBuf<zd> sums, so, eo, co;
   HIPCHK(d_tb_off.upload(tb_off));
@@ -435,10 +435,9 @@ int near_fill_quad(spk_handle *h) {
     }
     launch_qn_regular(h->qdev, T, b0, b1, toff[b1] - toff[b0], d_toff.p, d_pbase.p, d_tb_off.p,
                       d_tb_list.p, h->d_box_first.p, h->d_box_count.p, h->d_perm.p, sums.p, h->st);
-    HIPCHK(l_s.upload(ls));
     HIPCHK(l_sdst.upload(sdst));
     HIPCHK(so.alloc(ls.size() * 30));
-    launch_quad_self(h->qdev, T, (int)ls.size(), l_s.p, so.p, h->st);
+    HIPCHK(quad_self_rows(h->qdev, T, m, ls, so.p, h->st));
     HIPCHK(l_es.upload(es));
     HIPCHK(l_et.upload(et));
     HIPCHK(l_side.upload(eside));

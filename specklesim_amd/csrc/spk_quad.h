@@ -53,9 +53,11 @@ void launch_quad_regular_grid(const QuadDev &d, const QuadTopo &T, int s0, int r
 // regular pairs from a list (s, t, ngp), 0-based s and t, out[i*30]
 void launch_quad_regular_list(const QuadDev &d, const QuadTopo &T, int n, const int *s,
                               const int *t, const int *ngp, zd *out, hipStream_t st);
-// self pairs: rows s[i] (0-based) against their own element, out[i*30]
-void launch_quad_self(const QuadDev &d, const QuadTopo &T, int n, const int *s, zd *out,
-                      hipStream_t st);
+// self pairs: rows ls[i] (0-based edge functions, host list) against their
+// own element, out[i*30] (device); each element's self terms are integrated
+// once for all its rows.  Synchronises the stream before returning.
+hipError_t quad_self_rows(const QuadDev &d, const QuadTopo &T, const QuadMesh &m,
+                          const std::vector<int> &ls, zd *out, hipStream_t st);
 // edge-adjacent pairs (s, t, side_a, side_q), out[i*30]
 void launch_quad_edge_adj(const QuadDev &d, const QuadTopo &T, int n, const int *s, const int *t,
                           const int *side, zd *out, hipStream_t st);

@@ -31,6 +31,31 @@ namespace {
 
 constexpr double QPI = 3.14159265358979323846;
 
+// device array owned by one call or one handle (freed on destruction)
+template <class T>
+struct QBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  ~QBuf() { reset(); }
+  void reset() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  hipError_t alloc(size_t count) {
+    reset();
+    n = count;
+    if (!count) return hipSuccess;
+    return hipMalloc(&p, count * sizeof(T));
+  }
+  hipError_t upload(const std::vector<T> &v) {
+    hipError_t e = alloc(v.size());
+    if (e != hipSuccess || v.empty()) return e;
+    return hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+  }
+};
+
+
 __device__ inline void qgreen(zd k, double R, zd &G, zd &DG) {
   // Diver_Green (src/lib_sie_quad_calculation_mod.f90:553-564)
   const double ph = -k.x * R, mag = exp(k.y * R);
@@ -355,12 +380,12 @@ __device__ inline zd wsum(zd v) {
   }
   return v;
 }
-__device__ void block_reduce30(zd v[3][10], zd *dst) {
+__device__ void block_reduce30(zd v[3][10], zd *dst, double scale = 1.0) {
   __shared__ zd red[4][30];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
   for (int k = 0; k < 3; ++k)
     for (int n = 0; n < 10; ++n) {
-      const zd s = wsum(v[k][n]);
+      const zd s = wsum(scale == 1.0 ? v[k][n] : v[k][n] * scale);
       if (lane == 0) red[wv][k * 10 + n] = s;
     }
   __syncthreads();
@@ -372,8 +397,11 @@ __device__ void block_reduce30(zd v[3][10], zd *dst) {
   __syncthreads();
 }
 
-// self term at one outer point: Integration_KABCD_Taylor_ST_4FEdge (:299-380)
-__device__ void taylor_point(const QuadDev &D, int pI, double mu, double nu, const double *El,
+// self term at one outer point: Integration_KABCD_Taylor_ST_4FEdge (:299-380),
+// for the test-function direction dir (0: au, the functions 1-5; 1: av,
+// 6-10) at unit value; the term is linear in the test function's value, so
+// function pI's is fedge(pI - 1, mu, nu) times the result of its direction
+__device__ void taylor_point(const QuadDev &D, int dir, double mu, double nu, const double *El,
                              zd out[3][10]) {
   double dk[8], de[8], Eu[3] = {0, 0, 0}, Ev[3] = {0, 0, 0}, Ra[3];
   shape_diff(mu, nu, dk, de);
@@ -386,8 +414,7 @@ __device__ void taylor_point(const QuadDev &D, int pI, double mu, double nu, con
   Geo gp;
   geo(mu, nu, El, gp);
   rloc(mu, nu, El, Ra);
-  double Na[3];
-  obs_fn(gp, pI, mu, nu, Na);
+  const double *Na = dir ? gp.av : gp.au;
   const double pb[5] = {atan((1 - nu) / (1 - mu)), QPI - atan((1 - nu) / (1 + mu)),
                         QPI + atan((1 + nu) / (1 + mu)), 2 * QPI - atan((1 + nu) / (1 - mu)),
                         2 * QPI + atan((1 - nu) / (1 - mu))};
@@ -449,26 +476,41 @@ __device__ void taylor_point(const QuadDev &D, int pI, double mu, double nu, con
   combine(D, S, out);
 }
 
-__global__ void __launch_bounds__(256) k_quad_self(QuadDev D, QuadTopo T, const int *ls,
-                                                   zd *out) {
-  const int s = ls[blockIdx.x];
-  const int ii = T.edge_p[5 * s] - 1, pI = T.edge_p[5 * s + 1];
-  const double *El = T.el + (size_t)ii * 24;
-  zd v[3][10];
-  for (int k = 0; k < 3; ++k)
-    for (int n = 0; n < 10; ++n) v[k][n] = zd{0, 0};
+// self terms of all ten test functions of element els[blockIdx.x] against
+// the element itself, out[blockIdx.x][pI - 1][30]: one thread per outer
+// point of the 15 x 15 rule, each direction integrated once
+__global__ void __launch_bounds__(256) k_quad_self_el(QuadDev D, QuadTopo T, const int *els,
+                                                      zd *out) {
+  const double *El = T.el + (size_t)els[blockIdx.x] * 24;
   const int p = threadIdx.x;
-  if (p < 225) {
-    const int i = p % 15, j = p / 15;
+  const bool on = p < 225;
+  const double mu = D.x15[on ? p % 15 : 0], nu = D.x15[on ? p / 15 : 0];
+  double TRF = 0;
+  if (on) {
     Geo ga;
-    geo(D.x15[i], D.x15[j], El, ga);
-    const double TRF = D.w15[i] * D.w15[j] * ga.JJ;
-    zd r[3][10];
-    taylor_point(D, pI, D.x15[i], D.x15[j], El, r);
-    for (int k = 0; k < 3; ++k)
-      for (int n = 0; n < 10; ++n) v[k][n] = r[k][n] * TRF;
+    geo(mu, nu, El, ga);
+    TRF = D.w15[p % 15] * D.w15[p / 15] * ga.JJ;
   }
-  block_reduce30(v, out + (size_t)blockIdx.x * 30);
+#pragma unroll 1
+  for (int dir = 0; dir < 2; ++dir) {
+    zd r[3][10];
+    for (int k = 0; k < 3; ++k)
+      for (int n = 0; n < 10; ++n) r[k][n] = zd{0, 0};
+    if (on) taylor_point(D, dir, mu, nu, El, r);
+#pragma unroll 1
+    for (int pI = 1 + 5 * dir; pI <= 5 + 5 * dir; ++pI) {
+      const double f = on ? fedge(pI - 1, mu, nu) * TRF : 0.0;
+      block_reduce30(r, out + ((size_t)blockIdx.x * 10 + pI - 1) * 30, f);
+    }
+  }
+}
+
+// out[i][:] = src[slot[i]][:], 30 complex each
+__global__ void k_quad_pick(int n, const int *slot, const zd *src, zd *out) {
+  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= (int64_t)n * 30) return;
+  const int i = (int)(id / 30), k = (int)(id % 30);
+  out[(size_t)i * 30 + k] = src[(size_t)slot[i] * 30 + k];
 }
 
 // edge-adjacent sub-domain limits (Lm_theta/Psi/mu/Lambd_EA, :656-923)
@@ -1240,10 +1282,33 @@ void launch_quad_regular_list(const QuadDev &d, const QuadTopo &T, int n, const 
   hipLaunchKernelGGL(k_quad_regular_list, dim3(nb(n, 128)), dim3(128), 0, st, d, T, n, s, t,
                      ngp, out);
 }
-void launch_quad_self(const QuadDev &d, const QuadTopo &T, int n, const int *s, zd *out,
-                      hipStream_t st) {
-  if (n <= 0) return;
-  hipLaunchKernelGGL(k_quad_self, dim3(n), dim3(256), 0, st, d, T, s, out);
+hipError_t quad_self_rows(const QuadDev &d, const QuadTopo &T, const QuadMesh &m,
+                          const std::vector<int> &ls, zd *out, hipStream_t st) {
+  if (ls.empty()) return hipSuccess;
+  std::vector<int> els(ls.size()), slot(ls.size());
+  for (size_t i = 0; i < ls.size(); ++i) els[i] = m.edge_p[5 * (size_t)ls[i]] - 1;
+  std::sort(els.begin(), els.end());
+  els.erase(std::unique(els.begin(), els.end()), els.end());
+  for (size_t i = 0; i < ls.size(); ++i) {
+    const int *e = &m.edge_p[5 * (size_t)ls[i]];
+    const int k = (int)(std::lower_bound(els.begin(), els.end(), e[0] - 1) - els.begin());
+    slot[i] = k * 10 + e[1] - 1;
+  }
+  QBuf<int> d_els, d_slot;
+  QBuf<zd> per_el;
+  hipError_t e = d_els.upload(els);
+  if (e == hipSuccess) e = d_slot.upload(slot);
+  if (e == hipSuccess) e = per_el.alloc(els.size() * 10 * 30);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_quad_self_el, dim3((unsigned)els.size()), dim3(256), 0, st, d, T, d_els.p,
+                     per_el.p);
+  const int n = (int)ls.size();
+  hipLaunchKernelGGL(k_quad_pick, dim3(nb((int64_t)n * 30, 256)), dim3(256), 0, st, n, d_slot.p,
+                     per_el.p, out);
+  e = hipGetLastError();
+  // the temporaries are freed on return
+  const hipError_t e2 = hipStreamSynchronize(st);
+  return e != hipSuccess ? e : e2;
 }
 void launch_quad_edge_adj(const QuadDev &d, const QuadTopo &T, int n, const int *s, const int *t,
                           const int *side, zd *out, hipStream_t st) {
@@ -1335,29 +1400,6 @@ bool quad_topology_arrays(const QuadMesh &m, std::vector<double> *el, std::vecto
 // C-ABI
 // ---------------------------------------------------------------------------
 namespace {
-
-template <class T>
-struct QBuf {
-  T *p = nullptr;
-  size_t n = 0;
-  ~QBuf() { reset(); }
-  void reset() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    n = 0;
-  }
-  hipError_t alloc(size_t count) {
-    reset();
-    n = count;
-    if (!count) return hipSuccess;
-    return hipMalloc(&p, count * sizeof(T));
-  }
-  hipError_t upload(const std::vector<T> &v) {
-    hipError_t e = alloc(v.size());
-    if (e != hipSuccess || v.empty()) return e;
-    return hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
-  }
-};
 
 #define QCHK(expr)                                                                        \
   do {                                                                                    \
@@ -1513,12 +1555,11 @@ int spk_quad_edge_element(spk_quad *q, int n, const int *s, const int *t, double
     rg.push_back(g);
     ri.push_back(i);
   }
-  QBuf<int> a0, a1, a2, b0, c0, c1, c2;
+  QBuf<int> a0, a1, a2, c0, c1, c2;
   QBuf<zd> o1, o2, o3;
   QCHK(a0.upload(rs));
   QCHK(a1.upload(rt));
   QCHK(a2.upload(rg));
-  QCHK(b0.upload(ss));
   QCHK(c0.upload(es));
   QCHK(c1.upload(et));
   QCHK(c2.upload(eside));
@@ -1527,7 +1568,7 @@ int spk_quad_edge_element(spk_quad *q, int n, const int *s, const int *t, double
   QCHK(o3.alloc(es.size() * 30));
   const QuadTopo T = q->topo();
   launch_quad_regular_list(q->dev, T, (int)rs.size(), a0.p, a1.p, a2.p, o1.p, q->st);
-  launch_quad_self(q->dev, T, (int)ss.size(), b0.p, o2.p, q->st);
+  QCHK(quad_self_rows(q->dev, T, m, ss, o2.p, q->st));
   launch_quad_edge_adj(q->dev, T, (int)es.size(), c0.p, c1.p, c2.p, o3.p, q->st);
   QCHK(hipGetLastError());
   std::vector<zd> h1(rs.size() * 30), h2(ss.size() * 30), h3(es.size() * 30);
@@ -1554,7 +1595,7 @@ int spk_quad_fill_device(spk_quad *q, void *d_MM) {
   if (rows_fit >= 64) rows_fit -= rows_fit % 64;
   const int chunk = (int)std::min<size_t>((size_t)m.medge, rows_fit);
   QBuf<zd> sums, so, eo, co;
-  QBuf<int> sl, el_s, el_t, el_side, sc_r, sc_t, sc_r2, cl_s, cl_t, cl_g, sc_r3;
+  QBuf<int> el_s, el_t, el_side, sc_r, sc_t, sc_r2, cl_s, cl_t, cl_g, sc_r3;
   QCHK(sums.alloc((size_t)chunk * m.nel * 30));
   for (int s0 = 0; s0 < m.medge; s0 += chunk) {
     const int rows = std::min(chunk, m.medge - s0);
@@ -1579,9 +1620,8 @@ int spk_quad_fill_device(spk_quad *q, void *d_MM) {
         cr.push_back(r);
       }
     }
-    QCHK(sl.upload(ss));
     QCHK(so.alloc((size_t)rows * 30));
-    launch_quad_self(q->dev, T, rows, sl.p, so.p, q->st);
+    QCHK(quad_self_rows(q->dev, T, m, ss, so.p, q->st));
     QCHK(el_s.upload(es));
     QCHK(el_t.upload(et));
     QCHK(el_side.upload(eside));
