@@ -216,7 +216,8 @@ int spk_quad_rhs_plane_wave_device(spk_quad *q, const double k_in[3], const doub
                                    void *d_b);
 /* The reference's quadrilateral solve, solver_LU (ZGESV,
  * src/lib_sie_math.f90:649-676) as lib_sie_quad_solver_normal_I calls it
- * (src/lib_sie_quad_solver_normal.f90:216-217), by rocSOLVER in FP64: d_MM is
+ * (src/lib_sie_quad_solver_normal.f90:216-217), in FP64: rocSOLVER's LU with
+ * partial pivoting, then blocked triangular solves (rocBLAS).  d_MM is
  * the row-major 2M x 2M system of spk_quad_fill_device and is overwritten by
  * its LU factors; d_b holds nrhs right-hand sides of 2M (column j at
  * d_b + 2M j) and is overwritten by the solutions.  An exactly zero pivot

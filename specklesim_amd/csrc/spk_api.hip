@@ -372,7 +372,7 @@ int near_fill_quad(spk_handle *h) {
     pbase[b + 1] = pbase[b] + (int64_t)t.box_count[b] * (int64_t)els.size();
     toff[b + 1] = toff[b] + (int64_t)((t.box_count[b] + 63) / 64) * (int64_t)els.size();
   }
-  DBuf<int> d_tb_off, d_tb_list, d_colA, d_colB, l_es, l_et, l_side, l_cs, l_ct, l_cg;
+  DBuf<int> d_tb_off, d_tb_list, d_colA, d_colB, l_es, l_et, l_side, l_cs, l_ct;
   DBuf<int64_t> d_pbase, d_toff, l_sdst, l_edst, l_cdst;
   DBuf<zd> sums, so, eo, co;
   HIPCHK(d_tb_off.upload(tb_off));
@@ -382,7 +382,7 @@ int near_fill_quad(spk_handle *h) {
   HIPCHK(d_pbase.upload(pbase));
   HIPCHK(d_toff.upload(toff));
   const QuadTopo T = h->qtopo();
-  std::vector<int> ls, es, et, eside, cs, ct, cg;
+  std::vector<int> ls, es, et, eside, cs, ct;
   std::vector<int64_t> sdst, edst, cdst;
   int64_t max_pairs = 0;
   for (int b = 0; b < nbl; ++b) max_pairs = std::max(max_pairs, pbase[b + 1] - pbase[b]);
@@ -444,13 +444,11 @@ int near_fill_quad(spk_handle *h) {
     HIPCHK(l_edst.upload(edst));
     HIPCHK(eo.alloc(std::max<size_t>(1, es.size()) * 30));
     launch_quad_edge_adj(h->qdev, T, (int)es.size(), l_es.p, l_et.p, l_side.p, eo.p, h->st);
-    cg.assign(cs.size(), 6);
     HIPCHK(l_cs.upload(cs));
     HIPCHK(l_ct.upload(ct));
-    HIPCHK(l_cg.upload(cg));
     HIPCHK(l_cdst.upload(cdst));
     HIPCHK(co.alloc(std::max<size_t>(1, cs.size()) * 30));
-    launch_quad_regular_list(h->qdev, T, (int)cs.size(), l_cs.p, l_ct.p, l_cg.p, co.p, h->st);
+    launch_quad_corner(h->qdev, T, (int)cs.size(), l_cs.p, l_ct.p, co.p, h->st);
     launch_qn_scatter((int)ls.size(), l_sdst.p, so.p, sums.p, h->st);
     launch_qn_scatter((int)es.size(), l_edst.p, eo.p, sums.p, h->st);
     launch_qn_scatter((int)cs.size(), l_cdst.p, co.p, sums.p, h->st);

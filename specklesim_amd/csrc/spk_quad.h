@@ -61,6 +61,11 @@ hipError_t quad_self_rows(const QuadDev &d, const QuadTopo &T, const QuadMesh &m
 // edge-adjacent pairs (s, t, side_a, side_q), out[i*30]
 void launch_quad_edge_adj(const QuadDev &d, const QuadTopo &T, int n, const int *s, const int *t,
                           const int *side, zd *out, hipStream_t st);
+// corner-adjacent pairs (s[i], t[i]) (0-based) by the 6 x 6 rule, out[i*30]
+void launch_quad_corner(const QuadDev &d, const QuadTopo &T, int n, const int *s, const int *t,
+                        zd *out, hipStream_t st);
+// x[k] = z[perm[k]] (the row interchanges of an LU factorisation)
+void launch_quad_permute(int64_t n, const int64_t *perm, const zd *z, zd *x, hipStream_t st);
 // sums[r[i]][t[i]][:] = src[i][:] (30 complex each)
 void launch_quad_scatter(int n, int nel, const int *r, const int *t, const zd *src, zd *sums,
                          hipStream_t st);

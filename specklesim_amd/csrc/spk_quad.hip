@@ -248,7 +248,7 @@ __device__ void regular_pair(const QuadDev &D, const double *x, const double *w,
   combine(D, S, out);
 }
 
-// Source-element data of the regular 4 x 4 rule at point (a, b), shared by
+// Source-element data of the regular NG x NG rule at point (a, b), shared by
 // every pair of a workgroup that integrates over the same element: position,
 // rule weight and, per edge function n, JJ (n x N_n) and the n x curl factor
 // fedge_rot (JJ times nrot of acc_ff).
@@ -257,12 +257,14 @@ struct QSrc {
   double aJ[10][3];
   double fr[10];
 };
+template <int NG>
 __device__ inline void qsrc_point(const QuadDev &D, const double *Eq, int a, int b, QSrc &o) {
-  const double k = D.x4[a], e = D.x4[b];
+  const double *x = NG == 6 ? D.x6 : D.x4, *wt = NG == 6 ? D.w6 : D.w4;
+  const double k = x[a], e = x[b];
   Geo g;
   geo(k, e, Eq, g);
   rloc(k, e, Eq, o.R);
-  o.w = D.w4[a] * D.w4[b];
+  o.w = wt[a] * wt[b];
 #pragma unroll
   for (int n = 0; n < 10; ++n) {
     const double f = fedge(n, k, e);
@@ -275,45 +277,52 @@ __device__ inline void qsrc_point(const QuadDev &D, const double *Eq, int a, int
   }
 }
 
-// regular_pair<4> with the source element's 16 points read from src (point
-// a, b at src[4a + b], the inner loop order of regular_pair)
+// one outer point (i, j) of the NG x NG rule on the test element against the
+// source element's NG^2 points src (point a, b at src[NG a + b], the inner
+// loop order of regular_pair), accumulated into S
+template <int NG>
+__device__ void outer_point_tab(const QuadDev &D, const double *Ea, const QSrc *src, int pI,
+                                int i, int j, zd S[5][10]) {
+  const double *x = NG == 6 ? D.x6 : D.x4, *wt = NG == 6 ? D.w6 : D.w4;
+  Geo ga;
+  geo(x[i], x[j], Ea, ga);
+  double Na[3], Ra[3];
+  obs_fn(ga, pI, x[i], x[j], Na);
+  rloc(x[i], x[j], Ea, Ra);
+  const double TRF = wt[i] * wt[j] * ga.JJ;
+  for (int p = 0; p < NG * NG; ++p) {
+    const QSrc &P = src[p];
+    const double R[3] = {Ra[0] - P.R[0], Ra[1] - P.R[1], Ra[2] - P.R[2]};
+    const double nR = sqrt(dot3(R, R));
+    const double rn[3] = {R[0] / nR, R[1] / nR, R[2] / nR};
+    zd Ga, GSa, Gm, GSm;
+    qgreen(D.k1, nR, Ga, GSa);
+    qgreen(D.k2, nR, Gm, GSm);
+    const zd gA = GSa + GSm, gBE = D.ie1 * GSa + D.ie2 * GSm, gC = Ga + Gm;
+    const zd gCH = D.e1 * Ga + D.e2 * Gm;
+    const double Nrn = dot3(Na, rn), w = P.w * TRF;
+#pragma unroll
+    for (int n = 0; n < 10; ++n) {
+      double t[3];
+      cross3(P.aJ[n], rn, t);
+      const double f1 = dot3(Na, t), f2 = Nrn * P.fr[n], f3 = dot3(Na, P.aJ[n]);
+      S[0][n] = S[0][n] + gA * (f1 * w);
+      S[1][n] = S[1][n] + gBE * (f2 * w);
+      S[2][n] = S[2][n] + gA * (f2 * w);
+      S[3][n] = S[3][n] + gC * (f3 * w);
+      S[4][n] = S[4][n] + gCH * (f3 * w);
+    }
+  }
+}
+
+// regular_pair<4> with the source element's 16 points read from src
 __device__ void regular_pair_tab(const QuadDev &D, const double *Ea, const QSrc *src, int pI,
                                  zd out[3][10]) {
   zd S[5][10];
   for (int k = 0; k < 5; ++k)
     for (int n = 0; n < 10; ++n) S[k][n] = zd{0, 0};
   for (int j = 0; j < 4; ++j)
-    for (int i = 0; i < 4; ++i) {
-      Geo ga;
-      geo(D.x4[i], D.x4[j], Ea, ga);
-      double Na[3], Ra[3];
-      obs_fn(ga, pI, D.x4[i], D.x4[j], Na);
-      rloc(D.x4[i], D.x4[j], Ea, Ra);
-      const double TRF = D.w4[i] * D.w4[j] * ga.JJ;
-      for (int p = 0; p < 16; ++p) {
-        const QSrc &P = src[p];
-        const double R[3] = {Ra[0] - P.R[0], Ra[1] - P.R[1], Ra[2] - P.R[2]};
-        const double nR = sqrt(dot3(R, R));
-        const double rn[3] = {R[0] / nR, R[1] / nR, R[2] / nR};
-        zd Ga, GSa, Gm, GSm;
-        qgreen(D.k1, nR, Ga, GSa);
-        qgreen(D.k2, nR, Gm, GSm);
-        const zd gA = GSa + GSm, gBE = D.ie1 * GSa + D.ie2 * GSm, gC = Ga + Gm;
-        const zd gCH = D.e1 * Ga + D.e2 * Gm;
-        const double Nrn = dot3(Na, rn), w = P.w * TRF;
-#pragma unroll
-        for (int n = 0; n < 10; ++n) {
-          double t[3];
-          cross3(P.aJ[n], rn, t);
-          const double f1 = dot3(Na, t), f2 = Nrn * P.fr[n], f3 = dot3(Na, P.aJ[n]);
-          S[0][n] = S[0][n] + gA * (f1 * w);
-          S[1][n] = S[1][n] + gBE * (f2 * w);
-          S[2][n] = S[2][n] + gA * (f2 * w);
-          S[3][n] = S[3][n] + gC * (f3 * w);
-          S[4][n] = S[4][n] + gCH * (f3 * w);
-        }
-      }
-    }
+    for (int i = 0; i < 4; ++i) outer_point_tab<4>(D, Ea, src, pI, i, j, S);
   combine(D, S, out);
 }
 
@@ -344,7 +353,7 @@ __global__ void __launch_bounds__(64) k_quad_regular_grid(QuadDev D, QuadTopo T,
   __shared__ QSrc src[16];
   const int t = blockIdx.x / nrb, rb = blockIdx.x - t * nrb;
   if (threadIdx.x < 16)
-    qsrc_point(D, T.el + (size_t)t * 24, threadIdx.x >> 2, threadIdx.x & 3, src[threadIdx.x]);
+    qsrc_point<4>(D, T.el + (size_t)t * 24, threadIdx.x >> 2, threadIdx.x & 3, src[threadIdx.x]);
   __syncthreads();
   const int r = rb * 64 + threadIdx.x;
   if (r >= rows) return;
@@ -395,6 +404,27 @@ __device__ void block_reduce30(zd v[3][10], zd *dst, double scale = 1.0) {
     dst[threadIdx.x] = s;
   }
   __syncthreads();
+}
+
+// corner-adjacent pairs (the 6 x 6 rule of regular_pair<6>): one workgroup
+// per listed pair (s[i], t[i]), the source points in LDS, one thread per
+// outer point, summed in a fixed order; out[i*30]
+__global__ void __launch_bounds__(64) k_quad_corner(QuadDev D, QuadTopo T, const int *ls,
+                                                    const int *lt, zd *out) {
+  __shared__ QSrc src[36];
+  const int s = ls[blockIdx.x], t = lt[blockIdx.x];
+  if (threadIdx.x < 36)
+    qsrc_point<6>(D, T.el + (size_t)t * 24, threadIdx.x / 6, threadIdx.x % 6, src[threadIdx.x]);
+  __syncthreads();
+  const int ii = T.edge_p[5 * s] - 1, pI = T.edge_p[5 * s + 1];
+  zd S[5][10];
+  for (int k = 0; k < 5; ++k)
+    for (int n = 0; n < 10; ++n) S[k][n] = zd{0, 0};
+  if (threadIdx.x < 36)
+    outer_point_tab<6>(D, T.el + (size_t)ii * 24, src, pI, threadIdx.x % 6, threadIdx.x / 6, S);
+  zd r[3][10];
+  combine(D, S, r);
+  block_reduce30(r, out + (size_t)blockIdx.x * 30);
 }
 
 // self term at one outer point: Integration_KABCD_Taylor_ST_4FEdge (:299-380),
@@ -503,6 +533,12 @@ __global__ void __launch_bounds__(256) k_quad_self_el(QuadDev D, QuadTopo T, con
       block_reduce30(r, out + ((size_t)blockIdx.x * 10 + pI - 1) * 30, f);
     }
   }
+}
+
+// x[k] = z[perm[k]]
+__global__ void k_quad_permute(int64_t n, const int64_t *perm, const zd *z, zd *x) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) x[k] = z[perm[k]];
 }
 
 // out[i][:] = src[slot[i]][:], 30 complex each
@@ -1038,7 +1074,7 @@ __global__ void __launch_bounds__(64) k_qn_regular(QuadDev D, QuadTopo T, int b0
   const int rb = (int)(rel / nT), li = (int)(rel - (int64_t)rb * nT);
   const int t = tb_list[tb_off[b] + li];
   if (threadIdx.x < 16)
-    qsrc_point(D, T.el + (size_t)t * 24, threadIdx.x >> 2, threadIdx.x & 3, src[threadIdx.x]);
+    qsrc_point<4>(D, T.el + (size_t)t * 24, threadIdx.x >> 2, threadIdx.x & 3, src[threadIdx.x]);
   __syncthreads();
   const int r = rb * 64 + threadIdx.x;
   if (r >= box_count[b]) return;
@@ -1282,6 +1318,16 @@ void launch_quad_regular_list(const QuadDev &d, const QuadTopo &T, int n, const 
   hipLaunchKernelGGL(k_quad_regular_list, dim3(nb(n, 128)), dim3(128), 0, st, d, T, n, s, t,
                      ngp, out);
 }
+void launch_quad_corner(const QuadDev &d, const QuadTopo &T, int n, const int *s, const int *t,
+                        zd *out, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_quad_corner, dim3(n), dim3(64), 0, st, d, T, s, t, out);
+}
+void launch_quad_permute(int64_t n, const int64_t *perm, const zd *z, zd *x, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_quad_permute, dim3(nb(n, 256)), dim3(256), 0, st, n, perm, z, x);
+}
+
 hipError_t quad_self_rows(const QuadDev &d, const QuadTopo &T, const QuadMesh &m,
                           const std::vector<int> &ls, zd *out, hipStream_t st) {
   if (ls.empty()) return hipSuccess;
@@ -1424,7 +1470,8 @@ struct spk_quad {
   QBuf<int> d_edge_p, d_eadj, d_eside, d_cadj;
   // dense solve: rocSOLVER handle on st, pivots and status of the last factorisation
   rocblas_handle rb = nullptr;
-  QBuf<int64_t> ipiv, info;
+  QBuf<int64_t> ipiv, info, perm;
+  QBuf<zd> lu_tmp;
   QuadTopo topo() const {
     return QuadTopo{d_el.p, d_edge_p.p, d_eadj.p, d_eside.p, d_cadj.p, mesh.nel, mesh.medge};
   }
@@ -1595,15 +1642,21 @@ int spk_quad_fill_device(spk_quad *q, void *d_MM) {
   if (rows_fit >= 64) rows_fit -= rows_fit % 64;
   const int chunk = (int)std::min<size_t>((size_t)m.medge, rows_fit);
   QBuf<zd> sums, so, eo, co;
-  QBuf<int> el_s, el_t, el_side, sc_r, sc_t, sc_r2, cl_s, cl_t, cl_g, sc_r3;
+  QBuf<int> el_s, el_t, el_side, sc_r, sc_t, sc_r2, cl_s, cl_t, sc_r3;
   QCHK(sums.alloc((size_t)chunk * m.nel * 30));
+  // the self terms of every row at once (one workgroup per element)
+  {
+    std::vector<int> all(m.medge);
+    for (int i = 0; i < m.medge; ++i) all[i] = i;
+    QCHK(so.alloc((size_t)m.medge * 30));
+    QCHK(quad_self_rows(q->dev, T, m, all, so.p, q->st));
+  }
   for (int s0 = 0; s0 < m.medge; s0 += chunk) {
     const int rows = std::min(chunk, m.medge - s0);
     launch_quad_regular_grid(q->dev, T, s0, rows, sums.p, q->st);
-    // self, edge- and corner-adjacent pairs of these rows
-    std::vector<int> ss(rows), es, et, eside, cs, ct, cr;
+    // edge- and corner-adjacent pairs of these rows
+    std::vector<int> es, et, eside, cs, ct, cr;
     for (int r = 0; r < rows; ++r) {
-      ss[r] = s0 + r;
       const int ii = m.edge_p[5 * (size_t)(s0 + r)] - 1;
       for (const QuadEdgeAdj &a : m.eadj[ii]) {
         es.push_back(s0 + r);
@@ -1620,19 +1673,15 @@ int spk_quad_fill_device(spk_quad *q, void *d_MM) {
         cr.push_back(r);
       }
     }
-    QCHK(so.alloc((size_t)rows * 30));
-    QCHK(quad_self_rows(q->dev, T, m, ss, so.p, q->st));
     QCHK(el_s.upload(es));
     QCHK(el_t.upload(et));
     QCHK(el_side.upload(eside));
     QCHK(eo.alloc(es.size() * 30));
     launch_quad_edge_adj(q->dev, T, (int)es.size(), el_s.p, el_t.p, el_side.p, eo.p, q->st);
-    const std::vector<int> cg(cs.size(), 6);
     QCHK(cl_s.upload(cs));
     QCHK(cl_t.upload(ct));
-    QCHK(cl_g.upload(cg));
     QCHK(co.alloc(cs.size() * 30));
-    launch_quad_regular_list(q->dev, T, (int)cs.size(), cl_s.p, cl_t.p, cl_g.p, co.p, q->st);
+    launch_quad_corner(q->dev, T, (int)cs.size(), cl_s.p, cl_t.p, co.p, q->st);
     QCHK(hipGetLastError());
     // scatter the self and edge-adjacent pairs into the sums
     std::vector<int> sr(rows), st_(rows), er(es.size());
@@ -1643,7 +1692,7 @@ int spk_quad_fill_device(spk_quad *q, void *d_MM) {
     for (size_t k = 0; k < es.size(); ++k) er[k] = es[k] - s0;
     QCHK(sc_r.upload(sr));
     QCHK(sc_t.upload(st_));
-    launch_quad_scatter(rows, m.nel, sc_r.p, sc_t.p, so.p, sums.p, q->st);
+    launch_quad_scatter(rows, m.nel, sc_r.p, sc_t.p, so.p + (size_t)s0 * 30, sums.p, q->st);
     QCHK(sc_r2.upload(er));
     launch_quad_scatter((int)es.size(), m.nel, sc_r2.p, el_t.p, eo.p, sums.p, q->st);
     QCHK(sc_r3.upload(cr));
@@ -1714,9 +1763,50 @@ int spk_quad_solve_lu_device(spk_quad *q, void *d_MM, int nrhs, void *d_b) {
   if (info > 0)
     return set_error(SPK_ERR_STATE, "U(" + std::to_string(info) + "," + std::to_string(info) +
                                         ") is zero: the system matrix is singular");
-  if (rocsolver_zgetrs_64(q->rb, rocblas_operation_transpose, n, nrhs, A, n, q->ipiv.p,
-                          (rocblas_double_complex *)d_b, n) != rocblas_status_success)
-    return set_error(SPK_ERR_HIP, "rocsolver_zgetrs_64 failed");
+  // MM = U^T L^T P^T: x = P (L^T)^-1 (U^T)^-1 b.  The triangular solves go
+  // by blocks of kLuBlock rows: a transposed GEMV of the solved part (its
+  // reads run down contiguous columns of the factors) and a small TRSV of
+  // the diagonal block; the interchanges become one gather by the
+  // permutation they compose to (the interchanges of ipiv in reverse order).
+  {
+    std::vector<int64_t> piv(n), idx(n);
+    QCHK(hipMemcpyAsync(piv.data(), q->ipiv.p, n * sizeof(int64_t), hipMemcpyDeviceToHost, q->st));
+    QCHK(hipStreamSynchronize(q->st));
+    for (int64_t i = 0; i < n; ++i) idx[i] = i;
+    for (int64_t i = n - 1; i >= 0; --i) std::swap(idx[i], idx[piv[i] - 1]);
+    QCHK(q->perm.upload(idx));
+    if (q->lu_tmp.n != (size_t)n) QCHK(q->lu_tmp.alloc(n));
+  }
+  constexpr int64_t kLuBlock = 512;
+  const rocblas_double_complex one = {1.0, 0.0}, mone = {-1.0, 0.0};
+  for (int j = 0; j < nrhs; ++j) {
+    auto *b = (rocblas_double_complex *)d_b + (size_t)j * n;
+    bool ok = true;
+    for (int64_t k0 = 0; k0 < n && ok; k0 += kLuBlock) {  // U^T y = b, forward
+      const int64_t m = std::min(kLuBlock, n - k0);
+      if (k0 > 0)
+        ok = rocblas_zgemv_64(q->rb, rocblas_operation_transpose, k0, m, &mone, A + k0 * n, n, b,
+                              1, &one, b + k0, 1) == rocblas_status_success;
+      ok = ok && rocblas_ztrsv_64(q->rb, rocblas_fill_upper, rocblas_operation_transpose,
+                                  rocblas_diagonal_non_unit, m, A + k0 + k0 * n, n, b + k0,
+                                  1) == rocblas_status_success;
+    }
+    const int64_t last = ((n - 1) / kLuBlock) * kLuBlock;
+    for (int64_t k0 = last; k0 >= 0 && ok; k0 -= kLuBlock) {  // L^T w = y, backward
+      const int64_t m = std::min(kLuBlock, n - k0), k1 = k0 + m;
+      if (k1 < n)
+        ok = rocblas_zgemv_64(q->rb, rocblas_operation_transpose, n - k1, m, &mone,
+                              A + k1 + k0 * n, n, b + k1, 1, &one, b + k0,
+                              1) == rocblas_status_success;
+      ok = ok && rocblas_ztrsv_64(q->rb, rocblas_fill_lower, rocblas_operation_transpose,
+                                  rocblas_diagonal_unit, m, A + k0 + k0 * n, n, b + k0,
+                                  1) == rocblas_status_success;
+    }
+    if (!ok) return set_error(SPK_ERR_HIP, "rocBLAS triangular solve failed");
+    launch_quad_permute(n, q->perm.p, (const zd *)b, q->lu_tmp.p, q->st);
+    QCHK(hipMemcpyAsync(b, q->lu_tmp.p, n * sizeof(zd), hipMemcpyDeviceToDevice, q->st));
+  }
+  QCHK(hipGetLastError());
   QCHK(hipStreamSynchronize(q->st));
   return SPK_OK;
 }
